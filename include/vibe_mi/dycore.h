@@ -1,0 +1,129 @@
+/* vibe_mi/dycore.h -- C ABI of the MI355X dynamical-core step.
+ *
+ * Replaces, for one GPU, the reference's time-step interface
+ *   Integrator::step(State&, StepContext)        include/vibe/time/integrator.hpp:62-82
+ *   RungeKutta3Integrator::step / acoustic_substep  src/time/runge_kutta.cpp:65-118
+ *   Equations::tendencies / acoustic_tendencies  include/vibe/dyn/equations.hpp:61-82
+ *   AcousticSubstepper::step (HEVI)              include/vibe/time/acoustic_substep.hpp:55-85
+ *   Damping::apply_sponge                        src/dyn/damping.cpp:76-138
+ *   energy_budget                                src/dyn/diagnostics.cpp:255-277
+ *   TimeStepController / Advection::cfl_number   src/time/timestep_control.cpp:23-56
+ * and the setup helpers the reference's driver uses to build inputs
+ * (src/grid/geometry.cpp, src/dyn/reference_state.cpp,
+ * src/driver/initial_conditions.cpp, src/dyn/damping.cpp:24-41).
+ *
+ * State layout (the build's own; dense, x fastest, periodic in x and y):
+ *   u, v, rho', theta', pi' : [nz][ny][nx]   (u: west x-face, v: south y-face)
+ *   w                       : [nz+1][ny][nx] (interface below cell k)
+ * rho', theta', pi' are perturbations from the reference state rho0,
+ * theta0, pi0 (cell centres, [nz][ny][nx]).  All host arrays are FP64.
+ *
+ * Ownership: a domain handle owns its device buffers (allocated once at
+ * creation); step/substep/tendency calls never allocate.  Calls on one
+ * handle must not overlap; different handles are independent.
+ * Return codes: 0 ok, 2 bad argument, >= 100 device/runtime errors
+ * (see c_kernels.h).
+ */
+#ifndef VIBE_MI_DYCORE_H
+#define VIBE_MI_DYCORE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VIBE_MI_FP64 0  /* plain FP64: storage, compute and reductions   */
+#define VIBE_MI_MIXED 1 /* FP64 state + RK3 accumulation, FP32 compute,  */
+                        /* FP64 compensated reductions                   */
+
+typedef struct {
+  int nx, ny, nz;
+  double dx, dy, ztop;
+  const double *zeta;   /* nz+1 normalised interfaces (0 .. 1)          */
+  const double *zs;     /* ny*nx terrain height (m)                     */
+  const double *rho0;   /* nz*ny*nx reference density                   */
+  const double *theta0; /* nz*ny*nx reference potential temperature     */
+  const double *pi0;    /* nz*ny*nx reference Exner function            */
+  const double *sponge; /* nz Rayleigh rates (1/s)                      */
+  double eps_s;         /* HEVI off-centering (0 = Crank-Nicolson)      */
+  int precision;        /* VIBE_MI_FP64 or VIBE_MI_MIXED                */
+} vibe_mi_domain_desc;
+
+typedef struct vibe_mi_domain vibe_mi_domain;
+
+int vibe_mi_domain_create(const vibe_mi_domain_desc *desc, vibe_mi_domain **out);
+int vibe_mi_domain_destroy(vibe_mi_domain *d);
+/* Upload / download the FP64 state (host arrays in the layout above). */
+int vibe_mi_domain_set_state(vibe_mi_domain *d, const double *u, const double *v,
+                             const double *w, const double *rho, const double *theta,
+                             const double *pi);
+int vibe_mi_domain_get_state(vibe_mi_domain *d, double *u, double *v, double *w,
+                             double *rho, double *theta, double *pi);
+/* nsteps WS-RK3 steps of length dt with nsub acoustic substeps per stage
+ * (runge_kutta.cpp:77-118).  Returns VIBE_MI_ERR_NONFINITE if the state
+ * became non-finite in any stage. */
+int vibe_mi_domain_step(vibe_mi_domain *d, double dt, int nsub, int nsteps);
+/* Same, timed with HIP events on the domain's stream: total elapsed ms. */
+int vibe_mi_domain_step_timed(vibe_mi_domain *d, double dt, int nsub, int nsteps,
+                              double *ms_total);
+/* Per-kernel time of the last timed call (ms summed over the call). */
+int vibe_mi_domain_kernel_times(vibe_mi_domain *d, double *ms_stage, double *ms_substep,
+                                int *n_stage, int *n_substep);
+/* Slow tendencies of the current state (equations.cpp:60-150 role), copied
+ * to host arrays (layout of the state). */
+int vibe_mi_domain_slow_tendency(vibe_mi_domain *d, double *ru, double *rv, double *rw,
+                                 double *rrho, double *rtheta, double *rpi);
+/* One acoustic substep in place (runge_kutta.cpp:65-75 role). */
+int vibe_mi_domain_substep(vibe_mi_domain *d, double dtau);
+/* Compensated FP64 domain totals {mass, perturbation mass, kinetic,
+ * internal, potential, total} (diagnostics.cpp:255-277). */
+int vibe_mi_domain_energy(vibe_mi_domain *d, double out[6]);
+/* Advective CFL of the current state (advection.cpp:325-340). */
+int vibe_mi_domain_cfl(vibe_mi_domain *d, double dt, double *cfl, double *max_wind);
+/* hipStream_t of the domain, as void*. */
+void *vibe_mi_domain_stream(vibe_mi_domain *d);
+/* Bytes of device memory the handle holds. */
+size_t vibe_mi_domain_device_bytes(vibe_mi_domain *d);
+
+/* ---- setup helpers (host only; inputs to the step) --------------------- */
+/* zeta = k/nz */
+int vibe_mi_zeta_uniform(int nz, double *zeta);
+/* tanh-stretched layers from dz_bottom to dz_top summing to ztop. */
+int vibe_mi_zeta_tanh(int nz, double ztop, double dz_bottom, double dz_top, double *zeta);
+/* Witch-of-Agnesi ridge h a^2/(a^2+(x-xc)^2), x at cell centres, xc at the
+ * domain centre, uniform in y (geometry.cpp:103-117 is the radial form). */
+int vibe_mi_terrain_agnesi_ridge(int nx, int ny, double dx, double h, double a, double *zs);
+/* Seeded Gaussian random terrain, std sigma, Gaussian correlation length L,
+ * periodic, minimum shifted to 0. */
+int vibe_mi_terrain_gaussian(int nx, int ny, double dx, double dy, double sigma, double L,
+                             uint64_t seed, double *zs);
+/* Reference state at cell centres.  kind 0: constant theta (isothermal-theta,
+ * reference_state.cpp:35-59); kind 1: constant Brunt-Vaisala frequency bv. */
+int vibe_mi_reference_state(int kind, double theta_sfc, double bv, double p_sfc, int nx,
+                            int ny, int nz, double ztop, const double *zeta, const double *zs,
+                            double *rho0, double *theta0, double *pi0);
+/* Rayleigh sponge rates per level (damping.cpp:24-41, column (0,0)). */
+int vibe_mi_sponge(int nz, double ztop, const double *zeta, double zs00, double start_fraction,
+                   double alpha_max, double *alpha);
+/* Warm/cold bubble theta' += amp cos^2(pi r/2) (initial_conditions.cpp:136-159). */
+int vibe_mi_ic_bubble(int nx, int ny, int nz, double dx, double dy, double ztop,
+                      const double *zeta, const double *zs, double xc, double yc, double zc,
+                      double rx, double rz, double amp, double *theta);
+/* Seeded uniform noise theta' += U(-amp, amp) below zmax (SplitMix64). */
+int vibe_mi_ic_noise(int nx, int ny, int nz, double ztop, const double *zeta, const double *zs,
+                     double amp, double zmax, uint64_t seed, double *theta);
+/* u = u0 + umax * tanh(z / hscale) (hscale <= 0: uniform u0). */
+int vibe_mi_ic_wind(int nx, int ny, int nz, double ztop, const double *zeta, const double *zs,
+                    double u0, double umax, double hscale, double *u);
+/* pi' from the equation of state at fixed rho (enforce_state_equation,
+ * initial_conditions.cpp:110-121), dry. */
+int vibe_mi_ic_balance_pi(size_t ncell, const double *rho0, const double *theta0,
+                          const double *rho, const double *theta, double *pi);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

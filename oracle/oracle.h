@@ -1,0 +1,80 @@
+/* oracle/oracle.h -- FP64 CPU oracle of the dynamical-core step.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the GPU library.  Nothing in the
+ * product (vibe-model_amd/) includes, links or calls this code.
+ *
+ * Field layout (all arrays dense, x fastest):
+ *   cell / u-face / v-face fields: [nz][ny][nx]
+ *   w-face fields:                 [nz+1][ny][nx]
+ * u(i,j,k) is the x-face on the west side of cell i, v(i,j,k) the y-face on
+ * the south side of cell j, w(i,j,k) the interface below cell k.  x and y are
+ * periodic.  The prognostic state is (u, v, w, rho', theta', pi'), the
+ * primed fields being perturbations from the reference state (rho0, theta0,
+ * pi0) stored at cell centres.
+ */
+#ifndef VIBE_ORACLE_H
+#define VIBE_ORACLE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- C11 kernel restatements (ckernels.c) ------------------------------ */
+int orc_tridiagonal_solve(const double *a, const double *b, const double *c,
+                          const double *d, int n, double *x, double *ws);
+void orc_weno5(double qm2, double qm1, double q0, double q1, double q2,
+               double *q_left, double *q_right);
+double orc_bilinear(const double *f, int halo, int nsx, int nsy, int nz,
+                    double xi, double yj, int k);
+double orc_cfl_scan(const double *u, const double *v, const double *w,
+                    const double *dzeta, const double *zs, int nx, int ny, int nz,
+                    double dx, double dy, double z_top, double dt,
+                    double *max_wind_out, double *min_dz_out);
+double orc_neumaier_sum(const double *x, size_t n);
+void orc_advect_scalar_flux(const double *q, const double *rho, const double *u,
+                            const double *v, const double *w, const double *dzeta,
+                            const double *zs, int nx, int ny, int nz, double dx,
+                            double dy, double z_top, double *dqdt);
+
+/* ---- dynamical core (dycore.c) ---------------------------------------- */
+typedef struct {
+  int nx, ny, nz;
+  double dx, dy, ztop;
+  const double *zeta;   /* nz+1 normalised interfaces, zeta[0]=0, zeta[nz]=1 */
+  const double *zs;     /* ny*nx terrain height (m) */
+  const double *rho0;   /* reference state at cell centres, nz*ny*nx */
+  const double *theta0;
+  const double *pi0;
+  const double *sponge; /* nz Rayleigh rates (1/s), 0 below the sponge */
+  double eps_s;         /* HEVI off-centering, weights (1+eps)/2, (1-eps)/2 */
+} orc_domain;
+
+typedef struct {
+  double *u, *v, *w, *rho, *theta, *pi;
+} orc_state;
+
+typedef struct orc_dycore orc_dycore;
+
+orc_dycore *orc_dycore_create(const orc_domain *d);
+void orc_dycore_destroy(orc_dycore *dc);
+/* One WS-RK3 step with nsub acoustic substeps per stage and the sponge.
+ * Returns 0, or 102 if the state became non-finite. */
+int orc_dycore_step(orc_dycore *dc, orc_state *s, double dt, int nsub);
+/* Slow (large-step) tendencies of state s into R (same layout). */
+void orc_dycore_slow_tendency(orc_dycore *dc, const orc_state *s, orc_state *R);
+/* One forward-backward HEVI substep in place on (u, v, w, rho', pi'). */
+void orc_dycore_substep(orc_dycore *dc, orc_state *s, double dtau);
+/* Backward-Euler Rayleigh sponge over dt, in place. */
+void orc_dycore_sponge(orc_dycore *dc, orc_state *s, double dt);
+/* Compensated domain totals: out = {mass, perturbation mass, kinetic,
+ * internal, potential, kinetic+internal+potential}. */
+void orc_dycore_energy(orc_dycore *dc, const orc_state *s, double out[6]);
+void orc_set_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,18 @@
+# Compiles the reference's own C11 hot kernels, from the sources where they
+# lie under /root/reference (never copied), into oracle/_ref/ for use as a
+# checker of the oracle restatement.  Only src/common/c_kernels.c and its
+# header are needed; nothing else of the reference is built.
+REF ?= /root/reference
+CC ?= gcc
+CFLAGS = -O2 -std=c11 -fPIC -ffp-contract=off -fno-fast-math -I$(REF)/include
+
+OUT = _ref/libref_ckernels.so
+
+all: $(OUT)
+
+$(OUT): $(REF)/src/common/c_kernels.c $(REF)/include/vibe/common/c_kernels.h
+	mkdir -p _ref
+	$(CC) $(CFLAGS) -shared -o $@ $(REF)/src/common/c_kernels.c -lm
+
+clean:
+	rm -f $(OUT)

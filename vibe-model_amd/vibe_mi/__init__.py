@@ -1,0 +1,349 @@
+"""vibe_mi -- Python host side of the MI355X dynamical-core library.
+
+Thin ctypes bindings over libvibe_mi.so (the C ABI declared in
+include/vibe_mi/c_kernels.h and include/vibe_mi/dycore.h).  The Python
+names mirror the reference's interfaces:
+
+  * ``tridiagonal_solve``, ``weno5_reconstruct``, ``bilinear``, ``cfl_scan``,
+    ``neumaier_sum``, ``advect_scalar_flux`` -- the C11 kernels of the
+    reference's include/vibe/common/c_kernels.h:54-113, same argument meaning
+    and return codes;
+  * ``Domain`` with ``step(dt, substeps)`` -- the role of
+    ``RungeKutta3Integrator::step(State&, StepContext)``
+    (include/vibe/time/integrator.hpp:62-82, src/time/runge_kutta.cpp:77-118),
+    ``slow_tendency`` (Equations::tendencies, src/dyn/equations.cpp:60-150),
+    ``acoustic_substep`` (src/time/runge_kutta.cpp:65-75) and
+    ``energy_budget`` (src/dyn/diagnostics.cpp:255-277).
+
+Every compute call runs on the GPU; if the library is missing or no GPU is
+visible the calls raise instead of computing anything on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libvibe_mi.so")
+
+VIBE_C_OK = 0
+VIBE_C_ERR_SINGULAR = 1
+VIBE_C_ERR_BAD_ARG = 2
+VIBE_MI_ERR_HIP = 100
+VIBE_MI_ERR_NO_DEVICE = 101
+VIBE_MI_ERR_NONFINITE = 102
+FP64 = 0
+MIXED = 1
+
+
+class VibeError(RuntimeError):
+    """A C-ABI call returned an error code."""
+
+    def __init__(self, code: int, what: str):
+        super().__init__(f"{what} failed with code {code}")
+        self.code = code
+
+
+class NumericalError(VibeError):
+    """The state became non-finite (runge_kutta.cpp:103-106 role)."""
+
+
+_D = ctypes.c_double
+_I = ctypes.c_int
+_SZ = ctypes.c_size_t
+_PD = ctypes.POINTER(ctypes.c_double)
+_PF = ctypes.POINTER(ctypes.c_float)
+_VP = ctypes.c_void_p
+
+
+class DomainDesc(ctypes.Structure):
+    _fields_ = [
+        ("nx", _I), ("ny", _I), ("nz", _I),
+        ("dx", _D), ("dy", _D), ("ztop", _D),
+        ("zeta", _PD), ("zs", _PD), ("rho0", _PD), ("theta0", _PD), ("pi0", _PD),
+        ("sponge", _PD), ("eps_s", _D), ("precision", _I),
+    ]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libvibe_mi.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "vibe_tridiagonal_solve_f64": (_I, [_PD, _PD, _PD, _PD, _I, _PD, _PD]),
+        "vibe_tridiagonal_solve_f32": (_I, [_PF, _PF, _PF, _PF, _I, _PF, _PF]),
+        "vibe_weno5_reconstruct_f64": (None, [_D] * 5 + [_PD, _PD]),
+        "vibe_weno5_reconstruct_f32": (None, [ctypes.c_float] * 5 + [_PF, _PF]),
+        "vibe_bilinear_f64": (_D, [_PD, _I, _I, _I, _I, _D, _D, _I]),
+        "vibe_cfl_scan_f64": (_D, [_PD, _PD, _PD, _PD, _PD, _I, _I, _I, _D, _D, _D, _D, _PD, _PD]),
+        "vibe_neumaier_sum_f64": (_D, [_PD, _SZ]),
+        "vibe_neumaier_sum_f32": (_D, [_PF, _SZ]),
+        "vibe_advect_scalar_flux_f64": (None, [_PD, _PD, _PD, _PD, _PD, _PD, _PD, _I, _I, _I, _D, _D, _D, _PD]),
+        "vibe_mi_tridiagonal_solve_batched_f64": (_I, [_VP, _VP, _VP, _VP, _I, _I, _VP, _VP, _VP]),
+        "vibe_mi_weno5_reconstruct_batched_f64": (_I, [_VP, _SZ, _VP, _VP, _VP]),
+        "vibe_mi_neumaier_sum_device_f64": (_I, [_VP, _SZ, _PD, _VP]),
+        "vibe_mi_last_error": (_I, []),
+        "vibe_mi_domain_create": (_I, [ctypes.POINTER(DomainDesc), ctypes.POINTER(_VP)]),
+        "vibe_mi_domain_destroy": (_I, [_VP]),
+        "vibe_mi_domain_set_state": (_I, [_VP] + [_PD] * 6),
+        "vibe_mi_domain_get_state": (_I, [_VP] + [_PD] * 6),
+        "vibe_mi_domain_step": (_I, [_VP, _D, _I, _I]),
+        "vibe_mi_domain_step_timed": (_I, [_VP, _D, _I, _I, _PD]),
+        "vibe_mi_domain_kernel_times": (_I, [_VP, _PD, _PD, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+        "vibe_mi_domain_slow_tendency": (_I, [_VP] + [_PD] * 6),
+        "vibe_mi_domain_substep": (_I, [_VP, _D]),
+        "vibe_mi_domain_energy": (_I, [_VP, _PD]),
+        "vibe_mi_domain_cfl": (_I, [_VP, _D, _PD, _PD]),
+        "vibe_mi_domain_stream": (_VP, [_VP]),
+        "vibe_mi_domain_device_bytes": (_SZ, [_VP]),
+        "vibe_mi_zeta_uniform": (_I, [_I, _PD]),
+        "vibe_mi_zeta_tanh": (_I, [_I, _D, _D, _D, _PD]),
+        "vibe_mi_terrain_agnesi_ridge": (_I, [_I, _I, _D, _D, _D, _PD]),
+        "vibe_mi_terrain_gaussian": (_I, [_I, _I, _D, _D, _D, _D, ctypes.c_uint64, _PD]),
+        "vibe_mi_reference_state": (_I, [_I, _D, _D, _D, _I, _I, _I, _D, _PD, _PD, _PD, _PD, _PD]),
+        "vibe_mi_sponge": (_I, [_I, _D, _PD, _D, _D, _D, _PD]),
+        "vibe_mi_ic_bubble": (_I, [_I, _I, _I, _D, _D, _D, _PD, _PD, _D, _D, _D, _D, _D, _D, _PD]),
+        "vibe_mi_ic_noise": (_I, [_I, _I, _I, _D, _PD, _PD, _D, _D, ctypes.c_uint64, _PD]),
+        "vibe_mi_ic_wind": (_I, [_I, _I, _I, _D, _PD, _PD, _D, _D, _D, _PD]),
+        "vibe_mi_ic_balance_pi": (_I, [_SZ, _PD, _PD, _PD, _PD, _PD]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _pd(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], "expected contiguous float64"
+    return a.ctypes.data_as(_PD)
+
+
+def _pf(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"], "expected contiguous float32"
+    return a.ctypes.data_as(_PF)
+
+
+def _check(rc: int, what: str):
+    if rc == VIBE_MI_ERR_NONFINITE:
+        raise NumericalError(rc, what)
+    if rc != 0:
+        raise VibeError(rc, what)
+
+
+def _check_last(what: str):
+    rc = lib().vibe_mi_last_error()
+    if rc != 0:
+        raise VibeError(rc, what)
+
+
+# --------------------------------------------------------------------------
+# C11 kernel ABI (c_kernels.h:54-113)
+# --------------------------------------------------------------------------
+
+def tridiagonal_solve(a, b, c, d, single=False):
+    """x with a[k-1]x[k-1] + b[k]x[k] + c[k]x[k+1] = d[k]; returns (rc, x)."""
+    dt = np.float32 if single else np.float64
+    b = np.ascontiguousarray(b, dtype=dt)
+    d = np.ascontiguousarray(d, dtype=dt)
+    n = b.size
+    a = None if a is None else np.ascontiguousarray(a, dtype=dt)
+    c = None if c is None else np.ascontiguousarray(c, dtype=dt)
+    x = np.zeros(max(n, 1), dtype=dt)
+    ws = np.zeros(max(n, 1), dtype=dt)
+    p = _pf if single else _pd
+    fn = lib().vibe_tridiagonal_solve_f32 if single else lib().vibe_tridiagonal_solve_f64
+    rc = fn(p(a), p(b), p(c), p(d), n, p(x), p(ws))
+    if rc >= 100:
+        raise VibeError(rc, "vibe_tridiagonal_solve")
+    return rc, x[:n]
+
+
+def weno5_reconstruct(qm2, qm1, q0, q1, q2, single=False):
+    """(q_left, q_right) of c_kernels.h:64-65."""
+    if single:
+        l, r = ctypes.c_float(), ctypes.c_float()
+        lib().vibe_weno5_reconstruct_f32(qm2, qm1, q0, q1, q2, ctypes.byref(l), ctypes.byref(r))
+    else:
+        l, r = ctypes.c_double(), ctypes.c_double()
+        lib().vibe_weno5_reconstruct_f64(qm2, qm1, q0, q1, q2, ctypes.byref(l), ctypes.byref(r))
+    _check_last("vibe_weno5_reconstruct")
+    return l.value, r.value
+
+
+def bilinear(f, halo, nsx, nsy, nz, xi, yj, k):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    r = lib().vibe_bilinear_f64(_pd(f), halo, nsx, nsy, nz, xi, yj, k)
+    _check_last("vibe_bilinear")
+    return r
+
+
+def cfl_scan(u, v, w, dzeta, zs, nx, ny, nz, dx, dy, z_top, dt):
+    """(cfl, max_wind, min_dz) of c_kernels.h:86-89."""
+    mw, md = ctypes.c_double(), ctypes.c_double()
+    arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (u, v, w, dzeta)]
+    zsa = None if zs is None else np.ascontiguousarray(zs, dtype=np.float64)
+    r = lib().vibe_cfl_scan_f64(*[_pd(x) for x in arrs], _pd(zsa), nx, ny, nz, dx, dy, z_top, dt,
+                                ctypes.byref(mw), ctypes.byref(md))
+    _check_last("vibe_cfl_scan")
+    return r, mw.value, md.value
+
+
+def neumaier_sum(x, single=False):
+    if single:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        r = lib().vibe_neumaier_sum_f32(_pf(x), x.size)
+    else:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        r = lib().vibe_neumaier_sum_f64(_pd(x), x.size)
+    _check_last("vibe_neumaier_sum")
+    return r
+
+
+def advect_scalar_flux(q, rho, u, v, w, dzeta, zs, nx, ny, nz, dx, dy, z_top):
+    arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (q, rho, u, v, w, dzeta)]
+    zsa = None if zs is None else np.ascontiguousarray(zs, dtype=np.float64)
+    out = np.zeros(nx * ny * nz)
+    lib().vibe_advect_scalar_flux_f64(*[_pd(x) for x in arrs], _pd(zsa), nx, ny, nz, dx, dy, z_top, _pd(out))
+    _check_last("vibe_advect_scalar_flux")
+    return out
+
+
+# --------------------------------------------------------------------------
+# dynamical core
+# --------------------------------------------------------------------------
+
+STATE_FIELDS = ("u", "v", "w", "rho", "theta", "pi")
+
+
+@dataclass
+class Case:
+    """Geometry, reference state and initial perturbation state of one domain
+    (build with vibe_mi.configs)."""
+
+    name: str
+    nx: int
+    ny: int
+    nz: int
+    dx: float
+    dy: float
+    ztop: float
+    zeta: np.ndarray
+    zs: np.ndarray
+    rho0: np.ndarray
+    theta0: np.ndarray
+    pi0: np.ndarray
+    sponge: np.ndarray
+    state: dict
+    dt: float
+    nsub: int
+    steps: int
+    eps_s: float = 0.1
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def ncell(self):
+        return self.nx * self.ny * self.nz
+
+    def copy_state(self):
+        return {k: v.copy() for k, v in self.state.items()}
+
+
+class Domain:
+    """One domain resident on the GPU (RungeKutta3Integrator role)."""
+
+    def __init__(self, case: Case, precision: int = FP64):
+        L = lib()
+        self.case = case
+        self.precision = precision
+        self._keep = [np.ascontiguousarray(a, dtype=np.float64) for a in
+                      (case.zeta, case.zs, case.rho0, case.theta0, case.pi0, case.sponge)]
+        z, zs, r0, t0, p0, sp = self._keep
+        self.desc = DomainDesc(case.nx, case.ny, case.nz, case.dx, case.dy, case.ztop,
+                               _pd(z), _pd(zs), _pd(r0), _pd(t0), _pd(p0), _pd(sp),
+                               case.eps_s, precision)
+        h = _VP()
+        _check(L.vibe_mi_domain_create(ctypes.byref(self.desc), ctypes.byref(h)), "vibe_mi_domain_create")
+        self.h = h
+        self.set_state(case.state)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vibe_mi_domain_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _shapes(self):
+        c = self.case
+        cell = c.nx * c.ny * c.nz
+        return {"u": cell, "v": cell, "w": c.nx * c.ny * (c.nz + 1), "rho": cell, "theta": cell, "pi": cell}
+
+    def set_state(self, st: dict):
+        arrs = [np.ascontiguousarray(st[k], dtype=np.float64).reshape(-1) for k in STATE_FIELDS]
+        for k, a in zip(STATE_FIELDS, arrs):
+            assert a.size == self._shapes()[k], f"field {k} has {a.size} values"
+        _check(lib().vibe_mi_domain_set_state(self.h, *[_pd(a) for a in arrs]), "vibe_mi_domain_set_state")
+
+    def get_state(self) -> dict:
+        out = {k: np.empty(n) for k, n in self._shapes().items()}
+        _check(lib().vibe_mi_domain_get_state(self.h, *[_pd(out[k]) for k in STATE_FIELDS]),
+               "vibe_mi_domain_get_state")
+        return out
+
+    def step(self, dt: float, substeps: int, nsteps: int = 1):
+        _check(lib().vibe_mi_domain_step(self.h, dt, substeps, nsteps), "vibe_mi_domain_step")
+
+    def step_timed(self, dt: float, substeps: int, nsteps: int = 1) -> float:
+        ms = ctypes.c_double()
+        _check(lib().vibe_mi_domain_step_timed(self.h, dt, substeps, nsteps, ctypes.byref(ms)),
+               "vibe_mi_domain_step_timed")
+        return ms.value
+
+    def kernel_times(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        na, nb = _I(), _I()
+        _check(lib().vibe_mi_domain_kernel_times(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(na),
+                                                 ctypes.byref(nb)), "vibe_mi_domain_kernel_times")
+        return {"ms_stage": a.value, "ms_substep": b.value, "n_stage": na.value, "n_substep": nb.value}
+
+    def slow_tendency(self) -> dict:
+        out = {k: np.empty(n) for k, n in self._shapes().items()}
+        _check(lib().vibe_mi_domain_slow_tendency(self.h, *[_pd(out[k]) for k in STATE_FIELDS]),
+               "vibe_mi_domain_slow_tendency")
+        return out
+
+    def acoustic_substep(self, dtau: float):
+        _check(lib().vibe_mi_domain_substep(self.h, dtau), "vibe_mi_domain_substep")
+
+    def energy_budget(self) -> np.ndarray:
+        out = np.zeros(6)
+        _check(lib().vibe_mi_domain_energy(self.h, _pd(out)), "vibe_mi_domain_energy")
+        return out
+
+    def cfl(self, dt: float):
+        c, w = ctypes.c_double(), ctypes.c_double()
+        _check(lib().vibe_mi_domain_cfl(self.h, dt, ctypes.byref(c), ctypes.byref(w)), "vibe_mi_domain_cfl")
+        return c.value, w.value
+
+    def device_bytes(self) -> int:
+        return lib().vibe_mi_domain_device_bytes(self.h)

@@ -1,0 +1,130 @@
+"""Builders of the benchmark workloads (geometry, reference state, initial
+state), using the library's host-side setup helpers.
+
+  warm_bubble     dry 3-D warm bubble, 64x64x40, 250 m, neutral theta0 = 300 K,
+                  cos^2 bubble 2 K / r = 2 km at z = 2 km, Rayleigh sponge in
+                  the top 3 km, dt = 2 s, 6 substeps, 150 steps
+                  (src/driver/initial_conditions.cpp:136-159 form)
+  mountain_wave   512x512x80, 1 km, tanh-stretched levels to 20 km
+                  (dz 50 -> 500 m), Witch-of-Agnesi ridge h = 1 km, a = 10 km,
+                  U = 10 m/s, N = 0.01 1/s, theta_sfc = 300 K, seeded
+                  +/-0.1 K noise below 1 km, dt = 6 s, 6 substeps, 600 steps
+  mountain_wave_large  the same at 1024x1024x96
+
+Every size can be overridden so that tests run the same physics on small
+grids.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import Case, _check, _pd, lib
+
+P_SFC = 100000.0
+
+
+def _levels(nz, ztop, kind, dz_bottom=None, dz_top=None):
+    zeta = np.zeros(nz + 1)
+    if kind == "uniform":
+        _check(lib().vibe_mi_zeta_uniform(nz, _pd(zeta)), "zeta_uniform")
+    else:
+        _check(lib().vibe_mi_zeta_tanh(nz, ztop, dz_bottom, dz_top, _pd(zeta)), "zeta_tanh")
+    return zeta
+
+
+def _reference(kind, theta_sfc, bv, nx, ny, nz, ztop, zeta, zs):
+    n = nx * ny * nz
+    r0, t0, p0 = np.zeros(n), np.zeros(n), np.zeros(n)
+    _check(lib().vibe_mi_reference_state(kind, theta_sfc, bv, P_SFC, nx, ny, nz, ztop, _pd(zeta), _pd(zs),
+                                         _pd(r0), _pd(t0), _pd(p0)), "reference_state")
+    return r0, t0, p0
+
+
+def _sponge(nz, ztop, zeta, zs00, start_fraction, alpha_max):
+    a = np.zeros(nz)
+    _check(lib().vibe_mi_sponge(nz, ztop, _pd(zeta), zs00, start_fraction, alpha_max, _pd(a)), "sponge")
+    return a
+
+
+def _zero_state(nx, ny, nz):
+    cell = nx * ny * nz
+    return {"u": np.zeros(cell), "v": np.zeros(cell), "w": np.zeros(nx * ny * (nz + 1)),
+            "rho": np.zeros(cell), "theta": np.zeros(cell), "pi": np.zeros(cell)}
+
+
+def _balance(st, r0, t0):
+    _check(lib().vibe_mi_ic_balance_pi(r0.size, _pd(r0), _pd(t0), _pd(st["rho"]), _pd(st["theta"]),
+                                       _pd(st["pi"])), "balance_pi")
+
+
+def warm_bubble(nx=64, ny=64, nz=40, dx=250.0, dz=250.0, amp=2.0, radius=2000.0, zc=2000.0,
+                sponge_depth=3000.0, alpha_max=0.2, dt=2.0, nsub=6, steps=150) -> Case:
+    ztop = nz * dz
+    zeta = _levels(nz, ztop, "uniform")
+    zs = np.zeros(nx * ny)
+    r0, t0, p0 = _reference(0, 300.0, 0.0, nx, ny, nz, ztop, zeta, zs)
+    sp = _sponge(nz, ztop, zeta, 0.0, max(0.0, (ztop - sponge_depth) / ztop), alpha_max)
+    st = _zero_state(nx, ny, nz)
+    _check(lib().vibe_mi_ic_bubble(nx, ny, nz, dx, dx, ztop, _pd(zeta), _pd(zs), 0.5 * nx * dx, 0.5 * ny * dx, zc,
+                                   radius, radius, amp, _pd(st["theta"])), "ic_bubble")
+    _balance(st, r0, t0)
+    return Case("warm_bubble", nx, ny, nz, dx, dx, ztop, zeta, zs, r0, t0, p0, sp, st, dt, nsub, steps)
+
+
+def mountain_wave(nx=512, ny=512, nz=80, dx=1000.0, ztop=20000.0, dz_bottom=50.0, dz_top=500.0, h=1000.0,
+                  a=10000.0, U=10.0, bv=0.01, theta_sfc=300.0, noise=0.1, noise_top=1000.0, seed=20240611,
+                  sponge_start=0.8, alpha_max=0.2, dt=6.0, nsub=6, steps=600) -> Case:
+    mean = ztop / nz
+    if not dz_bottom < mean < dz_top:  # small test grids: keep the 1:10 stretching
+        dz_bottom, dz_top = 0.2 * mean, 2.0 * mean
+    zeta = _levels(nz, ztop, "tanh", dz_bottom, dz_top)
+    zs = np.zeros(nx * ny)
+    _check(lib().vibe_mi_terrain_agnesi_ridge(nx, ny, dx, h, a, _pd(zs)), "agnesi")
+    r0, t0, p0 = _reference(1, theta_sfc, bv, nx, ny, nz, ztop, zeta, zs)
+    sp = _sponge(nz, ztop, zeta, float(zs[0]), sponge_start, alpha_max)
+    st = _zero_state(nx, ny, nz)
+    _check(lib().vibe_mi_ic_wind(nx, ny, nz, ztop, _pd(zeta), _pd(zs), U, 0.0, 0.0, _pd(st["u"])), "ic_wind")
+    if noise > 0:
+        _check(lib().vibe_mi_ic_noise(nx, ny, nz, ztop, _pd(zeta), _pd(zs), noise, noise_top, seed,
+                                      _pd(st["theta"])), "ic_noise")
+    _balance(st, r0, t0)
+    c = Case("mountain_wave", nx, ny, nz, dx, dx, ztop, zeta, zs, r0, t0, p0, sp, st, dt, nsub, steps)
+    c.meta["seed"] = seed
+    return c
+
+
+def mountain_wave_large(**kw) -> Case:
+    args = dict(nx=1024, ny=1024, nz=96)
+    args.update(kw)
+    c = mountain_wave(**args)
+    c.name = "mountain_wave_large"
+    return c
+
+
+def random_terrain_case(nx=64, ny=48, nz=24, dx=2000.0, ztop=16000.0, sigma=300.0, L=20000.0, seed=7,
+                        U=8.0, bv=0.01, dt=6.0, nsub=6, steps=10) -> Case:
+    """Small stable-stratified case over seeded 2-D random terrain (exercises
+    both metric slopes)."""
+    zeta = _levels(nz, ztop, "tanh", 0.4 * ztop / nz, 2.0 * ztop / nz)
+    zs = np.zeros(nx * ny)
+    _check(lib().vibe_mi_terrain_gaussian(nx, ny, dx, dx, sigma, L, seed, _pd(zs)), "terrain_gaussian")
+    r0, t0, p0 = _reference(1, 300.0, bv, nx, ny, nz, ztop, zeta, zs)
+    sp = _sponge(nz, ztop, zeta, float(zs[0]), 0.75, 0.1)
+    st = _zero_state(nx, ny, nz)
+    _check(lib().vibe_mi_ic_wind(nx, ny, nz, ztop, _pd(zeta), _pd(zs), U, 0.0, 0.0, _pd(st["u"])), "ic_wind")
+    # v also blows so that both horizontal metric terms act
+    st["v"][:] = 0.5 * U
+    _check(lib().vibe_mi_ic_bubble(nx, ny, nz, dx, dx, ztop, _pd(zeta), _pd(zs), 0.5 * nx * dx, 0.5 * ny * dx,
+                                   3000.0, 8000.0, 2000.0, 1.5, _pd(st["theta"])), "ic_bubble")
+    _check(lib().vibe_mi_ic_noise(nx, ny, nz, ztop, _pd(zeta), _pd(zs), 0.2, 2000.0, seed + 1,
+                                  _pd(st["theta"])), "ic_noise")
+    _balance(st, r0, t0)
+    return Case("random_terrain", nx, ny, nz, dx, dx, ztop, zeta, zs, r0, t0, p0, sp, st, dt, nsub, steps)
+
+
+CONFIGS = {
+    "warm_bubble": warm_bubble,
+    "mountain_wave": mountain_wave,
+    "mountain_wave_large": mountain_wave_large,
+    "random_terrain": random_terrain_case,
+}
