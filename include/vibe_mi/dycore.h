@@ -30,6 +30,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "vibe_mi/c_kernels.h" /* error codes */
+
 #ifdef __cplusplus
 extern "C" {
 #endif

@@ -1,0 +1,169 @@
+// vibe_mi/dycore.hpp -- C++20 host interface over the C ABI (dycore.h).
+//
+// Mirrors the roles of the reference's time-integration interface:
+//   vibe::timeint::StepContext          include/vibe/time/integrator.hpp:42-49
+//   vibe::timeint::Integrator::step     include/vibe/time/integrator.hpp:62-82
+//   vibe::timeint::RungeKutta3Integrator  include/vibe/time/runge_kutta.hpp:46-81
+//   vibe::dyn::energy_budget            src/dyn/diagnostics.cpp:255-277
+//   vibe::NumericalError                include/vibe/common/error.hpp:35
+// Header-only; link against libvibe_mi.so.  The state lives on the GPU; the
+// host-side State is used only to upload initial conditions and download
+// results.
+#pragma once
+
+#include <cstddef>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "vibe_mi/c_kernels.h"
+#include "vibe_mi/dycore.h"
+
+namespace vibe_mi {
+
+/// Non-finite state (runge_kutta.cpp:103-106) or a device failure.
+class NumericalError : public std::runtime_error {
+ public:
+  using std::runtime_error::runtime_error;
+};
+class DeviceError : public std::runtime_error {
+ public:
+  DeviceError(const std::string &what, int code)
+      : std::runtime_error(what + " failed with code " + std::to_string(code)), code_(code) {}
+  int code() const noexcept { return code_; }
+
+ private:
+  int code_;
+};
+
+inline void check(int rc, const char *what) {
+  if (rc == VIBE_MI_ERR_NONFINITE) throw NumericalError(std::string(what) + ": non-finite model state");
+  if (rc != 0) throw DeviceError(what, rc);
+}
+
+/// Same fields and factory as the reference's StepContext.
+struct StepContext {
+  double dt = 1.0;
+  double dt_acoustic = 1.0;
+  int acoustic_substeps = 1;
+  double time = 0.0;
+  static StepContext make(double dt, int substeps, double time) {
+    StepContext c;
+    c.dt = dt;
+    c.acoustic_substeps = substeps;
+    c.dt_acoustic = dt / static_cast<double>(substeps);
+    c.time = time;
+    return c;
+  }
+};
+
+/// Host copy of the prognostic perturbation state (layout of dycore.h).
+struct State {
+  std::vector<double> u, v, w, rho, theta, pi;
+  double time = 0.0;
+  int step = 0;
+  State() = default;
+  State(int nx, int ny, int nz) {
+    const std::size_t c = static_cast<std::size_t>(nx) * ny * nz;
+    u.assign(c, 0.0);
+    v.assign(c, 0.0);
+    w.assign(c + static_cast<std::size_t>(nx) * ny, 0.0);
+    rho.assign(c, 0.0);
+    theta.assign(c, 0.0);
+    pi.assign(c, 0.0);
+  }
+};
+
+/// Geometry + reference state of one domain (host arrays copied at creation).
+struct DomainSpec {
+  int nx = 0, ny = 0, nz = 0;
+  double dx = 0, dy = 0, ztop = 0;
+  std::vector<double> zeta, zs, rho0, theta0, pi0, sponge;
+  double eps_s = 0.1;
+  int precision = VIBE_MI_FP64;
+};
+
+struct EnergyBudget {
+  double mass = 0, mass_perturbation = 0, kinetic = 0, internal = 0, potential = 0, total = 0;
+};
+
+/// Abstract integrator (Integrator role).
+class Integrator {
+ public:
+  virtual ~Integrator() = default;
+  virtual void step(State &s, const StepContext &ctx) = 0;
+  virtual const char *name() const noexcept = 0;
+};
+
+/// WS-RK3 + HEVI acoustic substeps on one GPU (RungeKutta3Integrator role).
+/// The GPU copy of the state is authoritative between calls: step() on the
+/// State it was last synchronised with only downloads when asked to.
+class RungeKutta3Integrator final : public Integrator {
+ public:
+  explicit RungeKutta3Integrator(const DomainSpec &spec) : nx_(spec.nx), ny_(spec.ny), nz_(spec.nz) {
+    vibe_mi_domain_desc d{};
+    d.nx = spec.nx;
+    d.ny = spec.ny;
+    d.nz = spec.nz;
+    d.dx = spec.dx;
+    d.dy = spec.dy;
+    d.ztop = spec.ztop;
+    d.zeta = spec.zeta.data();
+    d.zs = spec.zs.data();
+    d.rho0 = spec.rho0.data();
+    d.theta0 = spec.theta0.data();
+    d.pi0 = spec.pi0.data();
+    d.sponge = spec.sponge.data();
+    d.eps_s = spec.eps_s;
+    d.precision = spec.precision;
+    check(vibe_mi_domain_create(&d, &h_), "vibe_mi_domain_create");
+  }
+  ~RungeKutta3Integrator() override {
+    if (h_ != nullptr) vibe_mi_domain_destroy(h_);
+  }
+  RungeKutta3Integrator(const RungeKutta3Integrator &) = delete;
+  RungeKutta3Integrator &operator=(const RungeKutta3Integrator &) = delete;
+
+  const char *name() const noexcept override { return "rk3_acoustic_hevi_mi355x"; }
+
+  void upload(const State &s) {
+    check(vibe_mi_domain_set_state(h_, s.u.data(), s.v.data(), s.w.data(), s.rho.data(), s.theta.data(),
+                                   s.pi.data()),
+          "vibe_mi_domain_set_state");
+  }
+  void download(State &s) const {
+    check(vibe_mi_domain_get_state(h_, s.u.data(), s.v.data(), s.w.data(), s.rho.data(), s.theta.data(),
+                                   s.pi.data()),
+          "vibe_mi_domain_get_state");
+  }
+  /// Advance the device state by n steps (no host transfer).
+  void advance(const StepContext &ctx, int nsteps = 1) {
+    check(vibe_mi_domain_step(h_, ctx.dt, ctx.acoustic_substeps, nsteps), "vibe_mi_domain_step");
+  }
+  /// Integrator::step semantics: upload s, one step, download into s.
+  void step(State &s, const StepContext &ctx) override {
+    upload(s);
+    advance(ctx, 1);
+    download(s);
+    s.time = ctx.time + ctx.dt;
+    s.step += 1;
+  }
+  EnergyBudget energy_budget() const {
+    double e[6];
+    check(vibe_mi_domain_energy(h_, e), "vibe_mi_domain_energy");
+    return {e[0], e[1], e[2], e[3], e[4], e[5]};
+  }
+  std::pair<double, double> cfl(double dt) const {
+    double c = 0, w = 0;
+    check(vibe_mi_domain_cfl(h_, dt, &c, &w), "vibe_mi_domain_cfl");
+    return {c, w};
+  }
+  State make_state() const { return State(nx_, ny_, nz_); }
+
+ private:
+  vibe_mi_domain *h_ = nullptr;
+  int nx_, ny_, nz_;
+};
+
+}  // namespace vibe_mi

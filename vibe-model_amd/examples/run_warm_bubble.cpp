@@ -1,0 +1,70 @@
+// run_warm_bubble.cpp -- C++20 caller of the MI355X step, the role of the
+// reference's examples/embed_model.cpp:61-131: build grid, reference state
+// and initial state, construct the integrator, loop step(), report the
+// energy budget before and after.
+//
+//   run_warm_bubble [nx ny nz steps]     (defaults 64 64 40 150)
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "vibe_mi/dycore.h"
+#include "vibe_mi/dycore.hpp"
+
+int main(int argc, char **argv) {
+  const int nx = argc > 1 ? std::atoi(argv[1]) : 64;
+  const int ny = argc > 2 ? std::atoi(argv[2]) : 64;
+  const int nz = argc > 3 ? std::atoi(argv[3]) : 40;
+  const int steps = argc > 4 ? std::atoi(argv[4]) : 150;
+  const double dx = 250.0, ztop = 250.0 * nz, dt = 2.0;
+  const int nsub = 6;
+
+  vibe_mi::DomainSpec spec;
+  spec.nx = nx;
+  spec.ny = ny;
+  spec.nz = nz;
+  spec.dx = spec.dy = dx;
+  spec.ztop = ztop;
+  const std::size_t ncell = static_cast<std::size_t>(nx) * ny * nz;
+  spec.zeta.resize(nz + 1);
+  spec.zs.assign(static_cast<std::size_t>(nx) * ny, 0.0);
+  spec.rho0.resize(ncell);
+  spec.theta0.resize(ncell);
+  spec.pi0.resize(ncell);
+  spec.sponge.resize(nz);
+  vibe_mi::check(vibe_mi_zeta_uniform(nz, spec.zeta.data()), "zeta");
+  vibe_mi::check(vibe_mi_reference_state(0, 300.0, 0.0, 1e5, nx, ny, nz, ztop, spec.zeta.data(), spec.zs.data(),
+                                         spec.rho0.data(), spec.theta0.data(), spec.pi0.data()),
+                 "reference state");
+  vibe_mi::check(vibe_mi_sponge(nz, ztop, spec.zeta.data(), 0.0, (ztop - 3000.0) / ztop, 0.2, spec.sponge.data()),
+                 "sponge");
+
+  vibe_mi::RungeKutta3Integrator integ(spec);
+  vibe_mi::State s = integ.make_state();
+  vibe_mi::check(vibe_mi_ic_bubble(nx, ny, nz, dx, dx, ztop, spec.zeta.data(), spec.zs.data(), 0.5 * nx * dx,
+                                   0.5 * ny * dx, 2000.0, 2000.0, 2000.0, 2.0, s.theta.data()),
+                 "bubble");
+  vibe_mi::check(vibe_mi_ic_balance_pi(ncell, spec.rho0.data(), spec.theta0.data(), s.rho.data(), s.theta.data(),
+                                       s.pi.data()),
+                 "balance");
+  integ.upload(s);
+  const auto e0 = integ.energy_budget();
+  std::printf("initial: mass %.12e kg  total energy %.12e J\n", e0.mass, e0.total);
+  double t = 0.0;
+  for (int n = 1; n <= steps; ++n) {
+    integ.advance(vibe_mi::StepContext::make(dt, nsub, t));
+    t += dt;
+    if (n % 50 == 0 || n == steps) {
+      integ.download(s);
+      double wmax = 0.0;
+      for (double v : s.w) wmax = std::fmax(wmax, v);
+      std::printf("  step %4d  t = %7.1f s  w_max = %8.3f m/s\n", n, t, wmax);
+    }
+  }
+  const auto e1 = integ.energy_budget();
+  std::printf("final:   mass %.12e kg  total energy %.12e J\n", e1.mass, e1.total);
+  std::printf("relative mass change %.3e, energy change %.3e\n", (e1.mass - e0.mass) / e0.mass,
+              (e1.total - e0.total) / e0.total);
+  return 0;
+}
