@@ -52,6 +52,10 @@ struct orc_dycore {
   double *Dv;     /* nxy  */
   double *sx;     /* nxy  : d zs/dx at u-faces */
   double *sy;     /* nxy  : d zs/dy at v-faces */
+  double *rDc, *rDu, *rDv; /* nxy: reciprocal column depths */
+  double *metx, *mety;     /* nxy: sx / Du, sy / Dv */
+  double *rdzeta, *rdzw;   /* 1/dzeta (nz), 1/dzw (nz+1, 1..nz-1) */
+  double rdx, rdy;
   double *kap;    /* ncell: (Rd/cv) pi0 / rho0 */
   double *lam;    /* ncell: (Rd/cv) pi0 / theta0 */
   /* scratch */
@@ -91,11 +95,22 @@ orc_dycore *orc_dycore_create(const orc_domain *d) {
     dc->zc[k] = 0.5 * (d->zeta[k] + d->zeta[k + 1]);
   }
   for (int k = 1; k < nz; ++k) dc->dzw[k] = dc->zc[k] - dc->zc[k - 1];
+  dc->rdzeta = (double *)calloc(nz, sizeof(double));
+  dc->rdzw = (double *)calloc(nz + 1, sizeof(double));
+  for (int k = 0; k < nz; ++k) dc->rdzeta[k] = 1.0 / dc->dzeta[k];
+  for (int k = 1; k < nz; ++k) dc->rdzw[k] = 1.0 / dc->dzw[k];
+  dc->rdx = 1.0 / d->dx;
+  dc->rdy = 1.0 / d->dy;
   dc->Dc = (double *)calloc(dc->nxy, sizeof(double));
   dc->Du = (double *)calloc(dc->nxy, sizeof(double));
   dc->Dv = (double *)calloc(dc->nxy, sizeof(double));
   dc->sx = (double *)calloc(dc->nxy, sizeof(double));
   dc->sy = (double *)calloc(dc->nxy, sizeof(double));
+  dc->rDc = (double *)calloc(dc->nxy, sizeof(double));
+  dc->rDu = (double *)calloc(dc->nxy, sizeof(double));
+  dc->rDv = (double *)calloc(dc->nxy, sizeof(double));
+  dc->metx = (double *)calloc(dc->nxy, sizeof(double));
+  dc->mety = (double *)calloc(dc->nxy, sizeof(double));
   for (int j = 0; j < ny; ++j)
     for (int i = 0; i < nx; ++i) {
       const double z = d->zs[HX(i, j)], zw = d->zs[HX(i - 1, j)], zsth = d->zs[HX(i, j - 1)];
@@ -104,6 +119,11 @@ orc_dycore *orc_dycore_create(const orc_domain *d) {
       dc->Dv[HX(i, j)] = d->ztop - 0.5 * (zsth + z);
       dc->sx[HX(i, j)] = (z - zw) / d->dx;
       dc->sy[HX(i, j)] = (z - zsth) / d->dy;
+      dc->rDc[HX(i, j)] = 1.0 / dc->Dc[HX(i, j)];
+      dc->rDu[HX(i, j)] = 1.0 / dc->Du[HX(i, j)];
+      dc->rDv[HX(i, j)] = 1.0 / dc->Dv[HX(i, j)];
+      dc->metx[HX(i, j)] = dc->sx[HX(i, j)] * dc->rDu[HX(i, j)];
+      dc->mety[HX(i, j)] = dc->sy[HX(i, j)] * dc->rDv[HX(i, j)];
     }
   dc->kap = (double *)calloc(dc->ncell, sizeof(double));
   dc->lam = (double *)calloc(dc->ncell, sizeof(double));
@@ -133,6 +153,8 @@ void orc_dycore_destroy(orc_dycore *dc) {
   if (!dc) return;
   free(dc->dzeta); free(dc->zc); free(dc->dzw);
   free(dc->Dc); free(dc->Du); free(dc->Dv); free(dc->sx); free(dc->sy);
+  free(dc->rDc); free(dc->rDu); free(dc->rDv); free(dc->metx); free(dc->mety);
+  free(dc->rdzeta); free(dc->rdzw);
   free(dc->kap); free(dc->lam); free(dc->om); free(dc->mc); free(dc->buf);
   free(dc);
 }
@@ -142,32 +164,42 @@ void orc_dycore_destroy(orc_dycore *dc) {
 /* 5th-order upwind face value (Wicker & Skamarock 2002), face between q0 and
  * q1: flux = U*c6 - |U|*d5. */
 static inline double c6(double qm2, double qm1, double q0, double q1, double q2, double q3) {
-  return (37.0 * (q0 + q1) - 8.0 * (qm1 + q2) + (qm2 + q3)) / 60.0;
+  return (37.0 * (q0 + q1) - 8.0 * (qm1 + q2) + (qm2 + q3)) * (1.0 / 60.0);
 }
 static inline double d5(double qm2, double qm1, double q0, double q1, double q2, double q3) {
-  return (10.0 * (q1 - q0) - 5.0 * (q2 - qm1) + (q3 - qm2)) / 60.0;
+  return (10.0 * (q1 - q0) - 5.0 * (q2 - qm1) + (q3 - qm2)) * (1.0 / 60.0);
 }
 static inline double up5(double U, double qm2, double qm1, double q0, double q1, double q2,
                          double q3) {
   return U * c6(qm2, qm1, q0, q1, q2, q3) - fabs(U) * d5(qm2, qm1, q0, q1, q2, q3);
 }
 
-/* WENO5 (c_kernels.c:64-104) split into its two halves. */
-static inline double weno_left(double qm2, double qm1, double q0, double q1, double q2) {
-  double l, r;
-  orc_weno5(qm2, qm1, q0, q1, q2, &l, &r);
-  return l;
+/* WENO5 value at the face right of c from the stencil (a, b, c, d, e) with
+ * the weights of c_kernels.c:64-104 (d = 0.1, 0.6, 0.3; eps = 1e-6), written
+ * with a single division: multiplying the weights alpha_k = d_k/(eps+beta_k)^2
+ * by prod_j (eps+beta_j)^2 leaves w_k = d_k prod_{j!=k} (eps+beta_j)^2. */
+static inline double weno_face(double a, double b, double c, double d, double e) {
+  const double eps = 1e-6;
+  const double p0 = 2.0 * a - 7.0 * b + 11.0 * c;
+  const double p1 = -b + 5.0 * c + 2.0 * d;
+  const double p2 = 2.0 * c + 5.0 * d - e;
+  const double s0a = a - 2.0 * b + c, s0b = a - 4.0 * b + 3.0 * c;
+  const double s1a = b - 2.0 * c + d, s1b = b - d;
+  const double s2a = c - 2.0 * d + e, s2b = 3.0 * c - 4.0 * d + e;
+  const double e0 = eps + ((13.0 / 12.0) * s0a * s0a + 0.25 * s0b * s0b);
+  const double e1 = eps + ((13.0 / 12.0) * s1a * s1a + 0.25 * s1b * s1b);
+  const double e2 = eps + ((13.0 / 12.0) * s2a * s2a + 0.25 * s2b * s2b);
+  const double q0 = e0 * e0, q1 = e1 * e1, q2 = e2 * e2;
+  const double w0 = 0.1 * (q1 * q2), w1 = 0.6 * (q0 * q2), w2 = 0.3 * (q0 * q1);
+  return (w0 * p0 + w1 * p1 + w2 * p2) / (6.0 * (w0 + w1 + w2));
 }
-static inline double weno_right(double qm2, double qm1, double q0, double q1, double q2) {
-  double l, r;
-  orc_weno5(qm2, qm1, q0, q1, q2, &l, &r);
-  return r;
-}
-/* Local Lax-Friedrichs face flux with alpha = max(|U|, 1e-3), as
- * advection.cpp:151-156; qm/qp are the left/right states of this face. */
-static inline double llf(double U, double qm, double qp) {
-  const double a = fmax(fabs(U), 1e-3);
-  return 0.5 * (U * (qm + qp) - a * (qp - qm));
+/* Upwind WENO5 flux through the face between cells f-1 and f, given the six
+ * cells q[f-3..f+2]: U >= 0 takes the left-biased value (stencil centred on
+ * f-1), U < 0 the right-biased one (stencil centred on f, mirrored). */
+static inline double upwind_weno(double U, double qm3, double qm2, double qm1, double q0, double q1,
+                                 double q2) {
+  const double qf = (U >= 0.0) ? weno_face(qm3, qm2, qm1, q0, q1) : weno_face(q2, q1, q0, qm1, qm2);
+  return U * qf;
 }
 
 /* ---- kinematics -------------------------------------------------------- */
@@ -200,7 +232,7 @@ static void compute_omega(orc_dycore *dc, const double *u, const double *v, cons
 static inline double gz(const orc_dycore *dc, const double *pi, int i, int j, int k) {
   const int nx = dc->nx, ny = dc->ny, nz = dc->nz;
   if (k <= 0 || k >= nz) return 0.0;
-  return (pi[IX(i, j, k)] - pi[IX(i, j, k - 1)]) / dc->dzw[k];
+  return (pi[IX(i, j, k)] - pi[IX(i, j, k - 1)]) * dc->rdzw[k];
 }
 static inline double gzc(const orc_dycore *dc, const double *pi, int i, int j, int k) {
   return 0.5 * (gz(dc, pi, i, j, k) + gz(dc, pi, i, j, k + 1));
@@ -209,14 +241,14 @@ static inline double gzc(const orc_dycore *dc, const double *pi, int i, int j, i
  *   PX = d_x pi' |zeta - (1-zeta_c) * sx / Du * (d pi'/d zeta)  at u-faces */
 static inline double px_op(const orc_dycore *dc, const double *pi, int i, int j, int k) {
   const int nx = dc->nx, ny = dc->ny;
-  const double dpx = (pi[IX(i, j, k)] - pi[IX(i - 1, j, k)]) / dc->d.dx;
-  const double met = (1.0 - dc->zc[k]) * dc->sx[HX(i, j)] / dc->Du[HX(i, j)];
+  const double dpx = (pi[IX(i, j, k)] - pi[IX(i - 1, j, k)]) * dc->rdx;
+  const double met = (1.0 - dc->zc[k]) * dc->metx[HX(i, j)];
   return dpx - met * (0.5 * (gzc(dc, pi, i - 1, j, k) + gzc(dc, pi, i, j, k)));
 }
 static inline double py_op(const orc_dycore *dc, const double *pi, int i, int j, int k) {
   const int nx = dc->nx, ny = dc->ny;
-  const double dpy = (pi[IX(i, j, k)] - pi[IX(i, j - 1, k)]) / dc->d.dy;
-  const double met = (1.0 - dc->zc[k]) * dc->sy[HX(i, j)] / dc->Dv[HX(i, j)];
+  const double dpy = (pi[IX(i, j, k)] - pi[IX(i, j - 1, k)]) * dc->rdy;
+  const double met = (1.0 - dc->zc[k]) * dc->mety[HX(i, j)];
   return dpy - met * (0.5 * (gzc(dc, pi, i, j - 1, k) + gzc(dc, pi, i, j, k)));
 }
 
@@ -225,16 +257,17 @@ static inline double gam(const orc_dycore *dc, int i, int j, int k) {
   const int nx = dc->nx, ny = dc->ny, nz = dc->nz;
   int kk = clampi(k, 1, nz - 1);
   const double *t0 = dc->d.theta0;
-  return (t0[IX(i, j, kk)] - t0[IX(i, j, kk - 1)]) / (dc->Dc[HX(i, j)] * dc->dzw[kk]);
+  return (t0[IX(i, j, kk)] - t0[IX(i, j, kk - 1)]) * dc->rDc[HX(i, j)] * dc->rdzw[kk];
 }
 
 /* ---- slow tendencies --------------------------------------------------- */
 
 void orc_dycore_slow_tendency(orc_dycore *dc, const orc_state *s, orc_state *R) {
   const int nx = dc->nx, ny = dc->ny, nz = dc->nz;
-  const double dx = dc->d.dx, dy = dc->d.dy;
+  const double rdx = dc->rdx, rdy = dc->rdy;
+  const double *rdzeta = dc->rdzeta, *rdzw = dc->rdzw, *rDc = dc->rDc, *rDu = dc->rDu, *rDv = dc->rDv;
   const double *u = s->u, *v = s->v, *w = s->w, *rho = s->rho, *th = s->theta, *pi = s->pi;
-  const double *Du = dc->Du, *Dv = dc->Dv, *Dc = dc->Dc, *dzeta = dc->dzeta, *dzw = dc->dzw;
+  const double *Du = dc->Du, *Dv = dc->Dv;
   const double *om = dc->om;
   compute_omega(dc, u, v, w, dc->om, dc->mc);
 
@@ -267,12 +300,12 @@ void orc_dycore_slow_tendency(orc_dycore *dc, const orc_state *s, orc_state *R) 
           fzc[q] = F;
         }
         const double uc = u[IX(i, j, k)];
-        const double adv = -(fxc[1] - fxc[0]) / dx - (fyc[1] - fyc[0]) / dy -
-                           (fzc[1] - fzc[0]) / dzeta[k] +
-                           uc * ((uxc[1] - uxc[0]) / dx + (vyc[1] - vyc[0]) / dy +
-                                 (ozc[1] - ozc[0]) / dzeta[k]);
+        const double adv = -(fxc[1] - fxc[0]) * rdx - (fyc[1] - fyc[0]) * rdy -
+                           (fzc[1] - fzc[0]) * rdzeta[k] +
+                           uc * ((uxc[1] - uxc[0]) * rdx + (vyc[1] - vyc[0]) * rdy +
+                                 (ozc[1] - ozc[0]) * rdzeta[k]);
         const double thu = 0.5 * (th[IX(i - 1, j, k)] + th[IX(i, j, k)]);
-        R->u[IX(i, j, k)] = adv / Du[HX(i, j)] - CP_ * thu * px_op(dc, pi, i, j, k);
+        R->u[IX(i, j, k)] = adv * rDu[HX(i, j)] - CP_ * thu * px_op(dc, pi, i, j, k);
 #undef U_
       }
 
@@ -305,12 +338,12 @@ void orc_dycore_slow_tendency(orc_dycore *dc, const orc_state *s, orc_state *R) 
           fzc[q] = F;
         }
         const double vc = v[IX(i, j, k)];
-        const double adv = -(fxc[1] - fxc[0]) / dx - (fyc[1] - fyc[0]) / dy -
-                           (fzc[1] - fzc[0]) / dzeta[k] +
-                           vc * ((uxc[1] - uxc[0]) / dx + (vyc[1] - vyc[0]) / dy +
-                                 (ozc[1] - ozc[0]) / dzeta[k]);
+        const double adv = -(fxc[1] - fxc[0]) * rdx - (fyc[1] - fyc[0]) * rdy -
+                           (fzc[1] - fzc[0]) * rdzeta[k] +
+                           vc * ((uxc[1] - uxc[0]) * rdx + (vyc[1] - vyc[0]) * rdy +
+                                 (ozc[1] - ozc[0]) * rdzeta[k]);
         const double thv = 0.5 * (th[IX(i, j - 1, k)] + th[IX(i, j, k)]);
-        R->v[IX(i, j, k)] = adv / Dv[HX(i, j)] - CP_ * thv * py_op(dc, pi, i, j, k);
+        R->v[IX(i, j, k)] = adv * rDv[HX(i, j)] - CP_ * thv * py_op(dc, pi, i, j, k);
 #undef V_
       }
 
@@ -343,15 +376,15 @@ void orc_dycore_slow_tendency(orc_dycore *dc, const orc_state *s, orc_state *R) 
                        W_(i, j, c + 2), W_(i, j, c + 3));
         }
         const double wc = w[IX(i, j, k)];
-        const double adv = -(fxc[1] - fxc[0]) / dx - (fyc[1] - fyc[0]) / dy -
-                           (fzc[1] - fzc[0]) / dzw[k] +
-                           wc * ((uxc[1] - uxc[0]) / dx + (vyc[1] - vyc[0]) / dy +
-                                 (ozc[1] - ozc[0]) / dzw[k]);
-        const double D = Dc[HX(i, j)];
+        const double adv = -(fxc[1] - fxc[0]) * rdx - (fyc[1] - fyc[0]) * rdy -
+                           (fzc[1] - fzc[0]) * rdzw[k] +
+                           wc * ((uxc[1] - uxc[0]) * rdx + (vyc[1] - vyc[0]) * rdy +
+                                 (ozc[1] - ozc[0]) * rdzw[k]);
+        const double rD = rDc[HX(i, j)];
         const double thw = 0.5 * (th[IX(i, j, k - 1)] + th[IX(i, j, k)]);
         const double th0w = 0.5 * (dc->d.theta0[IX(i, j, k - 1)] + dc->d.theta0[IX(i, j, k)]);
-        const double pz = ((pi[IX(i, j, k)] - pi[IX(i, j, k - 1)]) / dzw[k]) / D;
-        R->w[IX(i, j, k)] = adv / D - CP_ * thw * pz + G_ * thw / th0w;
+        const double pz = (pi[IX(i, j, k)] - pi[IX(i, j, k - 1)]) * rdzw[k] * rD;
+        R->w[IX(i, j, k)] = adv * rD - CP_ * thw * pz + G_ * thw / th0w;
 #undef W_
       }
 
@@ -361,7 +394,7 @@ void orc_dycore_slow_tendency(orc_dycore *dc, const orc_state *s, orc_state *R) 
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < nx; ++i) {
 #define T_(ii, jj, kk) th[IX(ii, jj, clampi(kk, 0, nz - 1))]
-        const double D = Dc[HX(i, j)];
+        const double rD = rDc[HX(i, j)];
         /* mass: -(1/D) div(rho' V), second-order face averages */
         double frx[2], fry[2], frz[2], ftx[2], fty[2], ftz[2], mfx[2], mfy[2], mfz[2];
         for (int q = 0; q < 2; ++q) {
@@ -369,31 +402,31 @@ void orc_dycore_slow_tendency(orc_dycore *dc, const orc_state *s, orc_state *R) 
           const double U = u[IX(f, j, k)];
           mfx[q] = Du[HX(f, j)] * U;
           frx[q] = Du[HX(f, j)] * (0.5 * (rho[IX(f - 1, j, k)] + rho[IX(f, j, k)])) * U;
-          ftx[q] = Du[HX(f, j)] * llf(U, weno_left(T_(f - 3, j, k), T_(f - 2, j, k), T_(f - 1, j, k), T_(f, j, k), T_(f + 1, j, k)),
-                                      weno_right(T_(f - 2, j, k), T_(f - 1, j, k), T_(f, j, k), T_(f + 1, j, k), T_(f + 2, j, k)));
+          ftx[q] = Du[HX(f, j)] * upwind_weno(U, T_(f - 3, j, k), T_(f - 2, j, k), T_(f - 1, j, k), T_(f, j, k),
+                                              T_(f + 1, j, k), T_(f + 2, j, k));
           const int g = j + q;
           const double V = v[IX(i, g, k)];
           mfy[q] = Dv[HX(i, g)] * V;
           fry[q] = Dv[HX(i, g)] * (0.5 * (rho[IX(i, g - 1, k)] + rho[IX(i, g, k)])) * V;
-          fty[q] = Dv[HX(i, g)] * llf(V, weno_left(T_(i, g - 3, k), T_(i, g - 2, k), T_(i, g - 1, k), T_(i, g, k), T_(i, g + 1, k)),
-                                      weno_right(T_(i, g - 2, k), T_(i, g - 1, k), T_(i, g, k), T_(i, g + 1, k), T_(i, g + 2, k)));
+          fty[q] = Dv[HX(i, g)] * upwind_weno(V, T_(i, g - 3, k), T_(i, g - 2, k), T_(i, g - 1, k), T_(i, g, k),
+                                              T_(i, g + 1, k), T_(i, g + 2, k));
           const int h = k + q;
           double O = 0.0, FR = 0.0, FT = 0.0;
           if (h > 0 && h < nz) {
             O = om[IX(i, j, h)];
             FR = (0.5 * (rho[IX(i, j, h - 1)] + rho[IX(i, j, h)])) * O;
-            FT = llf(O, weno_left(T_(i, j, h - 3), T_(i, j, h - 2), T_(i, j, h - 1), T_(i, j, h), T_(i, j, h + 1)),
-                     weno_right(T_(i, j, h - 2), T_(i, j, h - 1), T_(i, j, h), T_(i, j, h + 1), T_(i, j, h + 2)));
+            FT = upwind_weno(O, T_(i, j, h - 3), T_(i, j, h - 2), T_(i, j, h - 1), T_(i, j, h), T_(i, j, h + 1),
+                             T_(i, j, h + 2));
           }
           mfz[q] = O;
           frz[q] = FR;
           ftz[q] = FT;
         }
         const size_t c = IX(i, j, k);
-        const double rr = -((frx[1] - frx[0]) / dx + (fry[1] - fry[0]) / dy + (frz[1] - frz[0]) / dzeta[k]) / D;
-        const double div = (mfx[1] - mfx[0]) / dx + (mfy[1] - mfy[0]) / dy + (mfz[1] - mfz[0]) / dzeta[k];
-        const double fd = (ftx[1] - ftx[0]) / dx + (fty[1] - fty[0]) / dy + (ftz[1] - ftz[0]) / dzeta[k];
-        const double rt = -(fd - th[c] * div) / D -
+        const double rr = -((frx[1] - frx[0]) * rdx + (fry[1] - fry[0]) * rdy + (frz[1] - frz[0]) * rdzeta[k]) * rD;
+        const double div = (mfx[1] - mfx[0]) * rdx + (mfy[1] - mfy[0]) * rdy + (mfz[1] - mfz[0]) * rdzeta[k];
+        const double fd = (ftx[1] - ftx[0]) * rdx + (fty[1] - fty[0]) * rdy + (ftz[1] - ftz[0]) * rdzeta[k];
+        const double rt = -(fd - th[c] * div) * rD -
                           0.5 * (w[IX(i, j, k)] * gam(dc, i, j, k) + w[IX(i, j, k + 1)] * gam(dc, i, j, k + 1));
         R->rho[c] = rr;
         R->theta[c] = rt;
@@ -406,7 +439,6 @@ void orc_dycore_slow_tendency(orc_dycore *dc, const orc_state *s, orc_state *R) 
 
 void orc_dycore_substep(orc_dycore *dc, orc_state *s, double dtau) {
   const int nx = dc->nx, ny = dc->ny, nz = dc->nz;
-  const double dx = dc->d.dx, dy = dc->d.dy;
   const double *r0 = dc->d.rho0, *t0 = dc->d.theta0, *zeta = dc->d.zeta;
   const double an = 0.5 * (1.0 + dc->d.eps_s), ao = 0.5 * (1.0 - dc->d.eps_s);
   const double an2 = an * an;
@@ -427,18 +459,21 @@ void orc_dycore_substep(orc_dycore *dc, orc_state *s, double dtau) {
   memcpy(v, vn, dc->ncell * sizeof(double));
 
   /* 2) per column: w / pi' / rho' with the vertical terms off-centred
-   *    implicit, solved by Thomas on the interior interfaces 1..nz-1. */
+   *    implicit.  Explicit parts E (rho'), Q (pi'), Fh (vertical flux with
+   *    the old w), then a tridiagonal system for the new w on interfaces
+   *    1..nz-1 solved by a factored Thomas sweep (reciprocal pivots), then
+   *    the vertical flux divergence closes rho' and pi'. */
 #pragma omp parallel
   {
-    double *mcol = (double *)malloc(sizeof(double) * (size_t)(6 * (nz + 1) + 4 * nz));
-    double *Mw = mcol, *r0w = Mw + (nz + 1), *gk = r0w + (nz + 1), *wn = gk + (nz + 1);
-    double *Fh = wn + (nz + 1), *F = Fh + (nz + 1);
-    double *HD = F + (nz + 1), *ck = HD + nz, *Ph = ck + nz, *mcc = Ph + nz;
-    double ta[512], tb[512], tc[512], td[512], tx[512], tw[512];
+    double *col = (double *)malloc(sizeof(double) * (size_t)(12 * (nz + 1)));
+    double *mcc = col, *HD = mcc + (nz + 1), *E = HD + (nz + 1), *Q = E + (nz + 1);
+    double *ck = Q + (nz + 1), *Mw = ck + (nz + 1), *r0w = Mw + (nz + 1), *Fh = r0w + (nz + 1);
+    double *Ph = Fh + (nz + 1), *gk = Ph + (nz + 1), *wn = gk + (nz + 1), *F = wn + (nz + 1);
+    double ta[512], tb[512], tc[512], td[512], tinv[512], tcp[512];
 #pragma omp for collapse(2) schedule(static)
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < nx; ++i) {
-        const double D = dc->Dc[HX(i, j)];
+        const double dtD = dtau * dc->rDc[HX(i, j)];
         for (int k = 0; k < nz; ++k) {
           mcc[k] = 0.5 * (u[IX(i, j, k)] * dc->sx[HX(i, j)] + u[IX(i + 1, j, k)] * dc->sx[HX(i + 1, j)]) +
                    0.5 * (v[IX(i, j, k)] * dc->sy[HX(i, j)] + v[IX(i, j + 1, k)] * dc->sy[HX(i, j + 1)]);
@@ -446,9 +481,12 @@ void orc_dycore_substep(orc_dycore *dc, orc_state *s, double dtau) {
           const double r0wst = 0.5 * (r0[IX(i - 1, j, k)] + r0[IX(i, j, k)]);
           const double r0n = 0.5 * (r0[IX(i, j, k)] + r0[IX(i, j + 1, k)]);
           const double r0s = 0.5 * (r0[IX(i, j - 1, k)] + r0[IX(i, j, k)]);
-          HD[k] = (dc->Du[HX(i + 1, j)] * r0e * u[IX(i + 1, j, k)] - dc->Du[HX(i, j)] * r0wst * u[IX(i, j, k)]) / dx +
-                  (dc->Dv[HX(i, j + 1)] * r0n * v[IX(i, j + 1, k)] - dc->Dv[HX(i, j)] * r0s * v[IX(i, j, k)]) / dy;
-          ck[k] = dtau * dc->kap[IX(i, j, k)] / (D * dc->dzeta[k]);
+          HD[k] = (dc->Du[HX(i + 1, j)] * r0e * u[IX(i + 1, j, k)] - dc->Du[HX(i, j)] * r0wst * u[IX(i, j, k)]) * dc->rdx +
+                  (dc->Dv[HX(i, j + 1)] * r0n * v[IX(i, j + 1, k)] - dc->Dv[HX(i, j)] * r0s * v[IX(i, j, k)]) * dc->rdy;
+          const double cq = dc->kap[IX(i, j, k)] * dtD;
+          ck[k] = cq * dc->rdzeta[k];
+          E[k] = rho[IX(i, j, k)] - dtD * HD[k];
+          Q[k] = pi[IX(i, j, k)] - cq * HD[k];
         }
         for (int k = 0; k <= nz; ++k) {
           Mw[k] = r0w[k] = gk[k] = Fh[k] = 0.0;
@@ -456,38 +494,44 @@ void orc_dycore_substep(orc_dycore *dc, orc_state *s, double dtau) {
             Mw[k] = (1.0 - zeta[k]) * (0.5 * (mcc[k - 1] + mcc[k]));
             r0w[k] = 0.5 * (r0[IX(i, j, k - 1)] + r0[IX(i, j, k)]);
             const double th0w = 0.5 * (t0[IX(i, j, k - 1)] + t0[IX(i, j, k)]);
-            gk[k] = dtau * CP_ * th0w / (D * dc->dzw[k]);
+            gk[k] = dtD * CP_ * th0w * dc->rdzw[k];
             Fh[k] = r0w[k] * (ao * w[IX(i, j, k)] - Mw[k]);
           }
         }
-        for (int k = 0; k < nz; ++k)
-          Ph[k] = pi[IX(i, j, k)] - dc->kap[IX(i, j, k)] * (dtau / D) * HD[k] - ck[k] * (Fh[k + 1] - Fh[k]);
+        for (int k = 0; k < nz; ++k) Ph[k] = Q[k] - ck[k] * (Fh[k + 1] - Fh[k]);
         const int n = nz - 1;
         for (int m = 0; m < n; ++m) {
           const int k = m + 1;
           const double ga = gk[k] * an2;
-          if (m > 0) ta[m - 1] = -ga * (ck[k - 1] * r0w[k - 1]);
+          ta[m] = (m > 0) ? -ga * (ck[k - 1] * r0w[k - 1]) : 0.0;
           tb[m] = 1.0 + ga * ((ck[k] + ck[k - 1]) * r0w[k]);
-          if (m < n - 1) tc[m] = -ga * (ck[k] * r0w[k + 1]);
+          tc[m] = (m < n - 1) ? -ga * (ck[k] * r0w[k + 1]) : 0.0;
           td[m] = w[IX(i, j, k)] - gk[k] * ao * (pi[IX(i, j, k)] - pi[IX(i, j, k - 1)]) -
                   gk[k] * an * (Ph[k] - Ph[k - 1]);
         }
-        if (n > 0) orc_tridiagonal_solve(ta, tb, tc, td, n, tx, tw);
+        /* factored Thomas (helmholtz.cpp:88-125 form) with reciprocal
+         * pivots inv_m, scaled sub-diagonal beta_m = a_m inv_m and c'_m:
+         *   d'_m = td_m inv_m - beta_m d'_{m-1},  x_m = d'_m - c'_m x_{m+1} */
+        for (int m = 0; m < n; ++m) {
+          const double den = (m == 0) ? tb[0] : tb[m] - ta[m] * tcp[m - 1];
+          tinv[m] = 1.0 / den;
+          tcp[m] = tc[m] * tinv[m];
+          ta[m] = ta[m] * tinv[m];
+        }
+        for (int m = 0; m < n; ++m) td[m] = (m == 0) ? td[0] * tinv[0] : td[m] * tinv[m] - ta[m] * td[m - 1];
+        for (int m = n - 2; m >= 0; --m) td[m] = td[m] - tcp[m] * td[m + 1];
         wn[0] = mcc[0];
         wn[nz] = w[IX(i, j, nz)];
-        for (int m = 0; m < n; ++m) wn[m + 1] = tx[m];
-        for (int k = 0; k <= nz; ++k) {
-          F[k] = 0.0;
-          if (k > 0 && k < nz) F[k] = r0w[k] * (an * wn[k] + ao * w[IX(i, j, k)] - Mw[k]);
-        }
+        for (int m = 0; m < n; ++m) wn[m + 1] = td[m];
+        for (int k = 0; k <= nz; ++k) F[k] = (k > 0 && k < nz) ? Fh[k] + (an * r0w[k]) * wn[k] : 0.0;
         for (int k = 0; k < nz; ++k) {
-          const double drho = -(dtau / D) * (HD[k] + (F[k + 1] - F[k]) / dc->dzeta[k]);
-          rho[IX(i, j, k)] += drho;
-          pi[IX(i, j, k)] += dc->kap[IX(i, j, k)] * drho;
+          const double vd = -dtD * ((F[k + 1] - F[k]) * dc->rdzeta[k]);
+          rho[IX(i, j, k)] = E[k] + vd;
+          pi[IX(i, j, k)] = Q[k] + dc->kap[IX(i, j, k)] * vd;
         }
         for (int k = 0; k <= nz; ++k) w[IX(i, j, k)] = wn[k];
       }
-    free(mcol);
+    free(col);
   }
 }
 

@@ -37,47 +37,27 @@
 // Compiled with -ffp-contract=off so the FP64 path rounds like the oracle.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "common.hpp"
+#include "geo.hpp"
 #include "stencils.hpp"
 #include "vibe_mi/dycore.h"
 
 namespace vibe_mi {
 
-template <typename T>
-struct Geo {
-  int nx, ny, nz;
-  T dx, dy;
-  const T *zeta;   // nz+1
-  const T *dzeta;  // nz
-  const T *zc;     // nz
-  const T *dzw;    // nz+1 (1..nz-1 valid)
-  const T *Dc, *Du, *Dv, *sx, *sy;  // nx*ny
-  const T *rho0, *theta0, *kap, *lam;  // nz*ny*nx
-};
-
-__device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
-__device__ __forceinline__ int clampk(int k, int hi) { return k < 0 ? 0 : (k > hi ? hi : k); }
-
-template <typename T>
-struct Acc {
-  int nx, ny;
-  size_t nxy;
-  __device__ size_t h(int i, int j) const {
-    return static_cast<size_t>(wrapi(j, ny)) * nx + wrapi(i, nx);
-  }
-  __device__ size_t at(int i, int j, int k) const { return static_cast<size_t>(k) * nxy + h(i, j); }
-};
-
 // ---- kinematics -------------------------------------------------------------
 
 template <typename T, typename S>
-__device__ __forceinline__ T mc_at(const Geo<T> &g, const Acc<T> &A, const S *__restrict__ u,
-                                   const S *__restrict__ v, int i, int j, int k) {
-  return T(0.5) * (T(u[A.at(i, j, k)]) * g.sx[A.h(i, j)] + T(u[A.at(i + 1, j, k)]) * g.sx[A.h(i + 1, j)]) +
-         T(0.5) * (T(v[A.at(i, j, k)]) * g.sy[A.h(i, j)] + T(v[A.at(i, j + 1, k)]) * g.sy[A.h(i, j + 1)]);
+__device__ __forceinline__ T mc_at(const Geo<T> &g, const Acc &A, const S *__restrict__ u, const S *__restrict__ v,
+                                   int i, int j, int k) {
+  return T(0.5) * (T(u[A.at(i, j, k)]) * g.sx[gcol(g, i, j)] + T(u[A.at(i + 1, j, k)]) * g.sx[gcol(g, i + 1, j)]) +
+         T(0.5) * (T(v[A.at(i, j, k)]) * g.sy[gcol(g, i, j)] + T(v[A.at(i, j + 1, k)]) * g.sy[gcol(g, i, j + 1)]);
 }
 
 template <typename T, typename S>
@@ -85,7 +65,7 @@ __global__ void __launch_bounds__(256) k_omega(Geo<T> g, const S *__restrict__ u
                                                const S *__restrict__ w, T *__restrict__ om) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
   if (i >= g.nx) return;
-  const Acc<T> A{g.nx, g.ny, static_cast<size_t>(g.nx) * g.ny};
+  const Acc A{g.nx, g.ny, static_cast<size_t>(g.nx) * g.ny};
   T o = T(0);
   if (k > 0 && k < g.nz)
     o = T(w[A.at(i, j, k)]) -
@@ -93,355 +73,615 @@ __global__ void __launch_bounds__(256) k_omega(Geo<T> g, const S *__restrict__ u
   om[A.at(i, j, k)] = o;
 }
 
-// ---- pressure-gradient operators ------------------------------------------
+// ---- pressure-gradient operators (Gal-Chen, along constant z) --------------
 
 template <typename T, typename S>
-__device__ __forceinline__ T gz(const Geo<T> &g, const Acc<T> &A, const S *__restrict__ pi, int i, int j, int k) {
+__device__ __forceinline__ T gz(const Geo<T> &g, const Acc &A, const S *__restrict__ pi, int i, int j, int k) {
   if (k <= 0 || k >= g.nz) return T(0);
-  return (T(pi[A.at(i, j, k)]) - T(pi[A.at(i, j, k - 1)])) / g.dzw[k];
+  return (T(pi[A.at(i, j, k)]) - T(pi[A.at(i, j, k - 1)])) * g.rdzw[k];
 }
 template <typename T, typename S>
-__device__ __forceinline__ T gzc(const Geo<T> &g, const Acc<T> &A, const S *__restrict__ pi, int i, int j, int k) {
+__device__ __forceinline__ T gzc(const Geo<T> &g, const Acc &A, const S *__restrict__ pi, int i, int j, int k) {
   return T(0.5) * (gz<T, S>(g, A, pi, i, j, k) + gz<T, S>(g, A, pi, i, j, k + 1));
 }
 template <typename T, typename S>
-__device__ __forceinline__ T px_op(const Geo<T> &g, const Acc<T> &A, const S *__restrict__ pi, int i, int j, int k) {
-  const T dpx = (T(pi[A.at(i, j, k)]) - T(pi[A.at(i - 1, j, k)])) / g.dx;
-  const T met = (T(1) - g.zc[k]) * g.sx[A.h(i, j)] / g.Du[A.h(i, j)];
+__device__ __forceinline__ T px_op(const Geo<T> &g, const Acc &A, const S *__restrict__ pi, int i, int j, int k) {
+  const T dpx = (T(pi[A.at(i, j, k)]) - T(pi[A.at(i - 1, j, k)])) * g.rdx;
+  const T met = (T(1) - g.zc[k]) * g.metx[gcol(g, i, j)];
   return dpx - met * (T(0.5) * (gzc<T, S>(g, A, pi, i - 1, j, k) + gzc<T, S>(g, A, pi, i, j, k)));
 }
 template <typename T, typename S>
-__device__ __forceinline__ T py_op(const Geo<T> &g, const Acc<T> &A, const S *__restrict__ pi, int i, int j, int k) {
-  const T dpy = (T(pi[A.at(i, j, k)]) - T(pi[A.at(i, j - 1, k)])) / g.dy;
-  const T met = (T(1) - g.zc[k]) * g.sy[A.h(i, j)] / g.Dv[A.h(i, j)];
+__device__ __forceinline__ T py_op(const Geo<T> &g, const Acc &A, const S *__restrict__ pi, int i, int j, int k) {
+  const T dpy = (T(pi[A.at(i, j, k)]) - T(pi[A.at(i, j - 1, k)])) * g.rdy;
+  const T met = (T(1) - g.zc[k]) * g.mety[gcol(g, i, j)];
   return dpy - met * (T(0.5) * (gzc<T, S>(g, A, pi, i, j - 1, k) + gzc<T, S>(g, A, pi, i, j, k)));
 }
 
-// ---- stage kernels: R and s = s0 + h R --------------------------------------
-// S: state storage (double), O: output type of the acoustic fields.
+// ---- fused z-marching stage kernel -------------------------------------------
+// One thread per column (i, j), marching up the levels.  Every vertical stencil
+// (5th-order momentum, WENO5 theta', pressure gradient, Omega averages) reads
+// a register window of the thread's own column; horizontal neighbours are read
+// at the current level only, so a workgroup's 64 x 4 tile of columns shares
+// them through L1/L2 while it marches.  Writes u, v, w, rho', pi' (type O) and
+// theta' (FP64) of s = s0 + h R in one pass.  Each interface flux is evaluated
+// once: the flux through the top of level k is carried to level k+1 as its
+// bottom flux (the same expression on the same operands).
+struct StageIn {
+  const double *u, *v, *w, *rho, *th, *pi;
+};
+template <typename O>
+struct StageOut {
+  O *u, *v, *w, *rho, *pi;
+  double *th;
+};
 
 template <typename T, typename O>
-__global__ void __launch_bounds__(256) k_stage_u(Geo<T> g, const double *__restrict__ u, const double *__restrict__ v,
-                                                 const double *__restrict__ th, const double *__restrict__ pi,
-                                                 const T *__restrict__ om, const double *__restrict__ u0, double h,
-                                                 O *__restrict__ uo) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
-  if (i >= g.nx) return;
-  const int nz = g.nz;
-  const Acc<T> A{g.nx, g.ny, static_cast<size_t>(g.nx) * g.ny};
-  auto U_ = [&](int ii, int jj, int kk) { return T(u[A.at(ii, jj, clampk(kk, nz - 1))]); };
-  T fxc[2], uxc[2], fyc[2], vyc[2], fzc[2], ozc[2];
+__global__ void __launch_bounds__(256) k_stage_col(Geo<T> g, StageIn cur, StageIn s0, const T *__restrict__ om,
+                                                   double h, StageOut<O> out, int *__restrict__ flag) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  if (i >= g.nx || j >= g.ny) return;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, rk = g.rk;
+  const size_t nxy = static_cast<size_t>(nx) * ny;
+  const double *__restrict__ U = cur.u;
+  const double *__restrict__ V = cur.v;
+  const double *__restrict__ W = cur.w;
+  const double *__restrict__ RH = cur.rho;
+  const double *__restrict__ TH = cur.th;
+  const double *__restrict__ PI = cur.pi;
+  int xi[7];
+  size_t yj[7];
 #pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const int c = i - 1 + q;
-    const T Ux = T(0.5) * (g.Du[A.h(c, j)] * T(u[A.at(c, j, k)]) + g.Du[A.h(c + 1, j)] * T(u[A.at(c + 1, j, k)]));
-    uxc[q] = Ux;
-    fxc[q] = up5<T>(Ux, U_(c - 2, j, k), U_(c - 1, j, k), U_(c, j, k), U_(c + 1, j, k), U_(c + 2, j, k), U_(c + 3, j, k));
-    const int jf = j + q;
-    const T Vy = T(0.5) * (g.Dv[A.h(i - 1, jf)] * T(v[A.at(i - 1, jf, k)]) + g.Dv[A.h(i, jf)] * T(v[A.at(i, jf, k)]));
-    vyc[q] = Vy;
-    fyc[q] = up5<T>(Vy, U_(i, jf - 3, k), U_(i, jf - 2, k), U_(i, jf - 1, k), U_(i, jf, k), U_(i, jf + 1, k),
-                    U_(i, jf + 2, k));
-    const int kf = k + q;
-    T O_ = T(0), F = T(0);
-    if (kf > 0 && kf < nz) {
-      O_ = T(0.5) * (om[A.at(i - 1, j, kf)] + om[A.at(i, j, kf)]);
-      F = up5<T>(O_, U_(i, j, kf - 3), U_(i, j, kf - 2), U_(i, j, kf - 1), U_(i, j, kf), U_(i, j, kf + 1),
-                 U_(i, j, kf + 2));
-    }
-    ozc[q] = O_;
-    fzc[q] = F;
+  for (int d = 0; d < 7; ++d) {
+    xi[d] = wrapi(i + d - 3, nx);
+    yj[d] = static_cast<size_t>(wrapi(j + d - 3, ny)) * nx;
   }
-  const size_t c0 = A.at(i, j, k);
-  const T uc = T(u[c0]);
-  const T adv = -(fxc[1] - fxc[0]) / g.dx - (fyc[1] - fyc[0]) / g.dy - (fzc[1] - fzc[0]) / g.dzeta[k] +
-                uc * ((uxc[1] - uxc[0]) / g.dx + (vyc[1] - vyc[0]) / g.dy + (ozc[1] - ozc[0]) / g.dzeta[k]);
-  const T thu = T(0.5) * (T(th[A.at(i - 1, j, k)]) + T(th[c0]));
-  const T R = adv / g.Du[A.h(i, j)] - T(kCp) * thu * px_op<T, double>(g, A, pi, i, j, k);
-  uo[c0] = static_cast<O>(u0[c0] + h * static_cast<double>(R));
-}
+  const size_t cC = yj[3] + xi[3], cW = yj[3] + xi[2], cS = yj[2] + xi[3];
+  const size_t cE = yj[3] + xi[4], cN = yj[4] + xi[3];
+  const size_t cWN = yj[4] + xi[2], cES = yj[2] + xi[4];
+  // column geometry (compacted)
+  const int gC = gcol(g, i, j), gW = gcol(g, i - 1, j), gE = gcol(g, i + 1, j), gS = gcol(g, i, j - 1),
+            gN = gcol(g, i, j + 1), gWN = gcol(g, i - 1, j + 1), gES = gcol(g, i + 1, j - 1);
+  const T DuW = g.Du[gW], DuC = g.Du[gC], DuE = g.Du[gE], DuES = g.Du[gES];
+  const T DvS = g.Dv[gS], DvC = g.Dv[gC], DvN = g.Dv[gN], DvWN = g.Dv[gWN], DvW = g.Dv[gW];
+  const T rDuC = g.rDu[gC], rDvC = g.rDv[gC], rD = g.rDc[gC], mxC = g.metx[gC], myC = g.mety[gC];
 
-template <typename T, typename O>
-__global__ void __launch_bounds__(256) k_stage_v(Geo<T> g, const double *__restrict__ u, const double *__restrict__ v,
-                                                 const double *__restrict__ th, const double *__restrict__ pi,
-                                                 const T *__restrict__ om, const double *__restrict__ v0, double h,
-                                                 O *__restrict__ vo) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
-  if (i >= g.nx) return;
-  const int nz = g.nz;
-  const Acc<T> A{g.nx, g.ny, static_cast<size_t>(g.nx) * g.ny};
-  auto V_ = [&](int ii, int jj, int kk) { return T(v[A.at(ii, jj, clampk(kk, nz - 1))]); };
-  T fxc[2], uxc[2], fyc[2], vyc[2], fzc[2], ozc[2];
+  auto ldd = [&](const double *f, int k, size_t c) { return T(f[static_cast<size_t>(k) * nxy + c]); };
+  T uw[7], vw[7], ww[7], tw[7], rw[3], pw[3], pWw[3], pSw[3], ow[3], oW[2], oS[2];
 #pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const int ifc = i + q;
-    const T Ux = T(0.5) * (g.Du[A.h(ifc, j - 1)] * T(u[A.at(ifc, j - 1, k)]) + g.Du[A.h(ifc, j)] * T(u[A.at(ifc, j, k)]));
-    uxc[q] = Ux;
-    fxc[q] = up5<T>(Ux, V_(ifc - 3, j, k), V_(ifc - 2, j, k), V_(ifc - 1, j, k), V_(ifc, j, k), V_(ifc + 1, j, k),
-                    V_(ifc + 2, j, k));
-    const int c = j - 1 + q;
-    const T Vy = T(0.5) * (g.Dv[A.h(i, c)] * T(v[A.at(i, c, k)]) + g.Dv[A.h(i, c + 1)] * T(v[A.at(i, c + 1, k)]));
-    vyc[q] = Vy;
-    fyc[q] = up5<T>(Vy, V_(i, c - 2, k), V_(i, c - 1, k), V_(i, c, k), V_(i, c + 1, k), V_(i, c + 2, k), V_(i, c + 3, k));
-    const int kf = k + q;
-    T O_ = T(0), F = T(0);
-    if (kf > 0 && kf < nz) {
-      O_ = T(0.5) * (om[A.at(i, j - 1, kf)] + om[A.at(i, j, kf)]);
-      F = up5<T>(O_, V_(i, j, kf - 3), V_(i, j, kf - 2), V_(i, j, kf - 1), V_(i, j, kf), V_(i, j, kf + 1),
-                 V_(i, j, kf + 2));
-    }
-    ozc[q] = O_;
-    fzc[q] = F;
+  for (int q = 0; q < 7; ++q) {
+    const int kc = clampk(q - 3, nz - 1);
+    uw[q] = ldd(U, kc, cC);
+    vw[q] = ldd(V, kc, cC);
+    tw[q] = ldd(TH, kc, cC);
+    ww[q] = ldd(W, clampk(q - 3, nz), cC);
   }
-  const size_t c0 = A.at(i, j, k);
-  const T vc = T(v[c0]);
-  const T adv = -(fxc[1] - fxc[0]) / g.dx - (fyc[1] - fyc[0]) / g.dy - (fzc[1] - fzc[0]) / g.dzeta[k] +
-                vc * ((uxc[1] - uxc[0]) / g.dx + (vyc[1] - vyc[0]) / g.dy + (ozc[1] - ozc[0]) / g.dzeta[k]);
-  const T thv = T(0.5) * (T(th[A.at(i, j - 1, k)]) + T(th[c0]));
-  const T R = adv / g.Dv[A.h(i, j)] - T(kCp) * thv * py_op<T, double>(g, A, pi, i, j, k);
-  vo[c0] = static_cast<O>(v0[c0] + h * static_cast<double>(R));
-}
-
-template <typename T, typename O>
-__global__ void __launch_bounds__(256) k_stage_w(Geo<T> g, const double *__restrict__ u, const double *__restrict__ v,
-                                                 const double *__restrict__ w, const double *__restrict__ th,
-                                                 const double *__restrict__ pi, const T *__restrict__ om,
-                                                 const double *__restrict__ w0, double h, O *__restrict__ wo) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;  // k = 0..nz
-  if (i >= g.nx) return;
-  const int nz = g.nz;
-  const Acc<T> A{g.nx, g.ny, static_cast<size_t>(g.nx) * g.ny};
-  const size_t c0 = A.at(i, j, k);
-  if (k == 0 || k == nz) {
-    wo[c0] = static_cast<O>(w0[c0]);
-    return;
-  }
-  auto W_ = [&](int ii, int jj, int kk) { return T(w[A.at(ii, jj, clampk(kk, nz))]); };
-  T fxc[2], uxc[2], fyc[2], vyc[2], fzc[2], ozc[2];
 #pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const int ifc = i + q;
-    const T Ux = T(0.5) * (g.Du[A.h(ifc, j)] * T(u[A.at(ifc, j, k - 1)]) + g.Du[A.h(ifc, j)] * T(u[A.at(ifc, j, k)]));
-    uxc[q] = Ux;
-    fxc[q] = up5<T>(Ux, W_(ifc - 3, j, k), W_(ifc - 2, j, k), W_(ifc - 1, j, k), W_(ifc, j, k), W_(ifc + 1, j, k),
-                    W_(ifc + 2, j, k));
-    const int jf = j + q;
-    const T Vy = T(0.5) * (g.Dv[A.h(i, jf)] * T(v[A.at(i, jf, k - 1)]) + g.Dv[A.h(i, jf)] * T(v[A.at(i, jf, k)]));
-    vyc[q] = Vy;
-    fyc[q] = up5<T>(Vy, W_(i, jf - 3, k), W_(i, jf - 2, k), W_(i, jf - 1, k), W_(i, jf, k), W_(i, jf + 1, k),
-                    W_(i, jf + 2, k));
-    const int c = k - 1 + q;
-    const T Oc = T(0.5) * (om[A.at(i, j, c)] + om[A.at(i, j, c + 1)]);
-    ozc[q] = Oc;
-    fzc[q] = up5<T>(Oc, W_(i, j, c - 2), W_(i, j, c - 1), W_(i, j, c), W_(i, j, c + 1), W_(i, j, c + 2), W_(i, j, c + 3));
+  for (int q = 0; q < 3; ++q) {
+    const int kc = clampk(q - 1, nz - 1);
+    rw[q] = ldd(RH, kc, cC);
+    pw[q] = ldd(PI, kc, cC);
+    pWw[q] = ldd(PI, kc, cW);
+    pSw[q] = ldd(PI, kc, cS);
+    ow[q] = om[static_cast<size_t>(clampk(q - 1, nz)) * nxy + cC];
   }
-  const T wc = T(w[c0]);
-  const T adv = -(fxc[1] - fxc[0]) / g.dx - (fyc[1] - fyc[0]) / g.dy - (fzc[1] - fzc[0]) / g.dzw[k] +
-                wc * ((uxc[1] - uxc[0]) / g.dx + (vyc[1] - vyc[0]) / g.dy + (ozc[1] - ozc[0]) / g.dzw[k]);
-  const T D = g.Dc[A.h(i, j)];
-  const size_t cb = A.at(i, j, k - 1);
-  const T thw = T(0.5) * (T(th[cb]) + T(th[c0]));
-  const T th0w = T(0.5) * (g.theta0[cb] + g.theta0[c0]);
-  const T pz = ((T(pi[c0]) - T(pi[cb])) / g.dzw[k]) / D;
-  const T R = adv / D - T(kCp) * thw * pz + T(kG) * thw / th0w;
-  wo[c0] = static_cast<O>(w0[c0] + h * static_cast<double>(R));
-}
+  oW[0] = om[cW];
+  oW[1] = om[nxy + cW];
+  oS[0] = om[cS];
+  oS[1] = om[nxy + cS];
+  T uE_prev = T(0), vN_prev = T(0);
+  // z-face fluxes of the level above, reused as the next level's lower face
+  T cu_O = T(0), cu_f = T(0), cv_O = T(0), cv_f = T(0), cw_O = T(0), cw_f = T(0);
+  T cm_z = T(0), cr_z = T(0), ct_z = T(0);
+  int bad = 0;
+  out.w[cC] = static_cast<O>(s0.w[cC]);  // interface 0 keeps its value (R_w = 0)
 
-template <typename T>
-__device__ __forceinline__ T gam_at(const Geo<T> &g, const Acc<T> &A, int i, int j, int k) {
-  const int kk = k < 1 ? 1 : (k > g.nz - 1 ? g.nz - 1 : k);
-  return (g.theta0[A.at(i, j, kk)] - g.theta0[A.at(i, j, kk - 1)]) / (g.Dc[A.h(i, j)] * g.dzw[kk]);
-}
-
-template <typename T, typename O>
-__global__ void __launch_bounds__(256) k_stage_cell(Geo<T> g, const double *__restrict__ u, const double *__restrict__ v,
-                                                    const double *__restrict__ w, const double *__restrict__ rho,
-                                                    const double *__restrict__ th, const double *__restrict__ pi,
-                                                    const T *__restrict__ om, const double *__restrict__ rho0s,
-                                                    const double *__restrict__ th0s, const double *__restrict__ pi0s,
-                                                    double h, O *__restrict__ rho_o, double *__restrict__ th_o,
-                                                    O *__restrict__ pi_o, int *__restrict__ flag) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
-  if (i >= g.nx) return;
-  const int nz = g.nz;
-  const Acc<T> A{g.nx, g.ny, static_cast<size_t>(g.nx) * g.ny};
-  auto T_ = [&](int ii, int jj, int kk) { return T(th[A.at(ii, jj, clampk(kk, nz - 1))]); };
-  const T D = g.Dc[A.h(i, j)];
-  T frx[2], fry[2], frz[2], ftx[2], fty[2], ftz[2], mfx[2], mfy[2], mfz[2];
-#pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const int f = i + q;
-    const T Ux = T(u[A.at(f, j, k)]);
-    mfx[q] = g.Du[A.h(f, j)] * Ux;
-    frx[q] = g.Du[A.h(f, j)] * (T(0.5) * (T(rho[A.at(f - 1, j, k)]) + T(rho[A.at(f, j, k)]))) * Ux;
-    ftx[q] = g.Du[A.h(f, j)] * llf<T>(Ux, weno_left<T>(T_(f - 3, j, k), T_(f - 2, j, k), T_(f - 1, j, k), T_(f, j, k), T_(f + 1, j, k)),
-                                      weno_right<T>(T_(f - 2, j, k), T_(f - 1, j, k), T_(f, j, k), T_(f + 1, j, k), T_(f + 2, j, k)));
-    const int gg = j + q;
-    const T Vy = T(v[A.at(i, gg, k)]);
-    mfy[q] = g.Dv[A.h(i, gg)] * Vy;
-    fry[q] = g.Dv[A.h(i, gg)] * (T(0.5) * (T(rho[A.at(i, gg - 1, k)]) + T(rho[A.at(i, gg, k)]))) * Vy;
-    fty[q] = g.Dv[A.h(i, gg)] * llf<T>(Vy, weno_left<T>(T_(i, gg - 3, k), T_(i, gg - 2, k), T_(i, gg - 1, k), T_(i, gg, k), T_(i, gg + 1, k)),
-                                       weno_right<T>(T_(i, gg - 2, k), T_(i, gg - 1, k), T_(i, gg, k), T_(i, gg + 1, k), T_(i, gg + 2, k)));
-    const int hh = k + q;
-    T O_ = T(0), FR = T(0), FT = T(0);
-    if (hh > 0 && hh < nz) {
-      O_ = om[A.at(i, j, hh)];
-      FR = (T(0.5) * (T(rho[A.at(i, j, hh - 1)]) + T(rho[A.at(i, j, hh)]))) * O_;
-      FT = llf<T>(O_, weno_left<T>(T_(i, j, hh - 3), T_(i, j, hh - 2), T_(i, j, hh - 1), T_(i, j, hh), T_(i, j, hh + 1)),
-                  weno_right<T>(T_(i, j, hh - 2), T_(i, j, hh - 1), T_(i, j, hh), T_(i, j, hh + 1), T_(i, j, hh + 2)));
-    }
-    mfz[q] = O_;
-    frz[q] = FR;
-    ftz[q] = FT;
-  }
-  const size_t c = A.at(i, j, k);
-  const T rr = -((frx[1] - frx[0]) / g.dx + (fry[1] - fry[0]) / g.dy + (frz[1] - frz[0]) / g.dzeta[k]) / D;
-  const T div = (mfx[1] - mfx[0]) / g.dx + (mfy[1] - mfy[0]) / g.dy + (mfz[1] - mfz[0]) / g.dzeta[k];
-  const T fd = (ftx[1] - ftx[0]) / g.dx + (fty[1] - fty[0]) / g.dy + (ftz[1] - ftz[0]) / g.dzeta[k];
-  const T rt = -(fd - T(th[c]) * div) / D -
-               T(0.5) * (T(w[c]) * gam_at<T>(g, A, i, j, k) + T(w[A.at(i, j, k + 1)]) * gam_at<T>(g, A, i, j, k + 1));
-  const T rp = g.kap[c] * rr + g.lam[c] * rt;
-  const double rn = rho0s[c] + h * static_cast<double>(rr);
-  const double tn = th0s[c] + h * static_cast<double>(rt);
-  const double pn = pi0s[c] + h * static_cast<double>(rp);
-  rho_o[c] = static_cast<O>(rn);
-  th_o[c] = tn;
-  pi_o[c] = static_cast<O>(pn);
-  if (!isfinite(tn)) atomicOr(flag, 1);
-}
-
-// ---- acoustic substep -------------------------------------------------------
-
-template <typename T, typename I, typename O>
-__global__ void __launch_bounds__(256) k_sub_h(Geo<T> g, const I *__restrict__ pi, const I *__restrict__ u,
-                                               const I *__restrict__ v, T dtau, O *__restrict__ uo,
-                                               O *__restrict__ vo, int check, int *__restrict__ flag) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
-  if (i >= g.nx) return;
-  const Acc<T> A{g.nx, g.ny, static_cast<size_t>(g.nx) * g.ny};
-  const size_t c = A.at(i, j, k);
-  const T th0u = T(0.5) * (g.theta0[A.at(i - 1, j, k)] + g.theta0[c]);
-  const T th0v = T(0.5) * (g.theta0[A.at(i, j - 1, k)] + g.theta0[c]);
-  const T un = T(u[c]) - dtau * (T(kCp) * th0u * px_op<T, I>(g, A, pi, i, j, k));
-  const T vn = T(v[c]) - dtau * (T(kCp) * th0v * py_op<T, I>(g, A, pi, i, j, k));
-  uo[c] = static_cast<O>(un);
-  vo[c] = static_cast<O>(vn);
-  if (check && !(isfinite(un) && isfinite(vn))) atomicOr(flag, 1);
-}
-
-// Per-column HEVI solve.  scr holds 5 work arrays of nz*nx*ny values laid
-// out [array][k][column] so that each k-step of a wavefront is coalesced.
-template <typename T, typename I, typename O>
-__global__ void __launch_bounds__(256) k_sub_v(Geo<T> g, const O *__restrict__ un, const O *__restrict__ vn,
-                                               const I *__restrict__ w, const I *__restrict__ rho,
-                                               const I *__restrict__ pi, T dtau, T an, T ao, O *__restrict__ wo,
-                                               O *__restrict__ rho_o, O *__restrict__ pi_o, T *__restrict__ scr,
-                                               int check, int *__restrict__ flag) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-  if (i >= g.nx) return;
-  const int nz = g.nz;
-  const Acc<T> A{g.nx, g.ny, static_cast<size_t>(g.nx) * g.ny};
-  const size_t nxy = A.nxy, ncell = nxy * nz;
-  const size_t col = A.h(i, j);
-  T *MCC = scr, *HD = scr + ncell, *PH = scr + 2 * ncell, *CP = scr + 3 * ncell, *DP = scr + 4 * ncell;
-  const T D = g.Dc[col];
-  const T an2 = an * an;
-  const T dtD = dtau / D;
-  auto ck_at = [&](int k) { return dtau * g.kap[A.at(i, j, k)] / (D * g.dzeta[k]); };
-  auto r0w_at = [&](int k) { return T(0.5) * (g.rho0[A.at(i, j, k - 1)] + g.rho0[A.at(i, j, k)]); };
-  auto mw_at = [&](int k) {
-    return (T(1) - g.zeta[k]) * (T(0.5) * (MCC[static_cast<size_t>(k - 1) * nxy + col] + MCC[static_cast<size_t>(k) * nxy + col]));
-  };
-  auto gk_at = [&](int k) {
-    const T th0w = T(0.5) * (g.theta0[A.at(i, j, k - 1)] + g.theta0[A.at(i, j, k)]);
-    return dtau * T(kCp) * th0w / (D * g.dzw[k]);
-  };
-  auto fh_at = [&](int k) {
-    if (k <= 0 || k >= nz) return T(0);
-    return r0w_at(k) * (ao * T(w[A.at(i, j, k)]) - mw_at(k));
-  };
-  // pass 1: terrain slope flux and horizontal rho0 mass-flux divergence
   for (int k = 0; k < nz; ++k) {
-    const size_t cc = A.at(i, j, k);
-    const T mc = T(0.5) * (T(un[cc]) * g.sx[col] + T(un[A.at(i + 1, j, k)]) * g.sx[A.h(i + 1, j)]) +
-                 T(0.5) * (T(vn[cc]) * g.sy[col] + T(vn[A.at(i, j + 1, k)]) * g.sy[A.h(i, j + 1)]);
-    const T r0e = T(0.5) * (g.rho0[cc] + g.rho0[A.at(i + 1, j, k)]);
-    const T r0wst = T(0.5) * (g.rho0[A.at(i - 1, j, k)] + g.rho0[cc]);
-    const T r0n = T(0.5) * (g.rho0[cc] + g.rho0[A.at(i, j + 1, k)]);
-    const T r0s = T(0.5) * (g.rho0[A.at(i, j - 1, k)] + g.rho0[cc]);
-    const T hd = (g.Du[A.h(i + 1, j)] * r0e * T(un[A.at(i + 1, j, k)]) - g.Du[col] * r0wst * T(un[cc])) / g.dx +
-                 (g.Dv[A.h(i, j + 1)] * r0n * T(vn[A.at(i, j + 1, k)]) - g.Dv[col] * r0s * T(vn[cc])) / g.dy;
-    MCC[static_cast<size_t>(k) * nxy + col] = mc;
-    HD[static_cast<size_t>(k) * nxy + col] = hd;
-  }
-  // pass 2: explicit pi' estimate P^ (vertical fluxes with the old w)
-  {
-    T fh_lo = T(0);
-    for (int k = 0; k < nz; ++k) {
-      const T fh_hi = fh_at(k + 1);
-      const size_t cc = A.at(i, j, k);
-      PH[static_cast<size_t>(k) * nxy + col] =
-          T(pi[cc]) - g.kap[cc] * dtD * HD[static_cast<size_t>(k) * nxy + col] - ck_at(k) * (fh_hi - fh_lo);
-      fh_lo = fh_hi;
+    const size_t kb = static_cast<size_t>(k) * nxy;
+    const int rb = k * rk;
+    // horizontal neighbours at level k
+    T ux[7], uy[7], vx[7], vy[7], tx[7], ty[7], wx[7], wy[7];
+#pragma unroll
+    for (int d = 0; d < 7; ++d) {
+      if (d == 3) continue;
+      ux[d] = T(U[kb + yj[3] + xi[d]]);
+      uy[d] = T(U[kb + yj[d] + xi[3]]);
+      vx[d] = T(V[kb + yj[3] + xi[d]]);
+      vy[d] = T(V[kb + yj[d] + xi[3]]);
+      tx[d] = T(TH[kb + yj[3] + xi[d]]);
+      ty[d] = T(TH[kb + yj[d] + xi[3]]);
+      wx[d] = T(W[kb + yj[3] + xi[d]]);
+      wy[d] = T(W[kb + yj[d] + xi[3]]);
     }
+    ux[3] = uy[3] = uw[3];
+    vx[3] = vy[3] = vw[3];
+    tx[3] = ty[3] = tw[3];
+    wx[3] = wy[3] = ww[3];
+    const T vWN = T(V[kb + cWN]), uES = T(U[kb + cES]);
+    const T rWst = T(RH[kb + cW]), rE = T(RH[kb + cE]), rSth = T(RH[kb + cS]), rN = T(RH[kb + cN]);
+    const T omz = T(1) - g.zc[k];
+    const T rdzk = g.rdzeta[k];
+    // vertical pi' gradients (0 at ground and lid)
+    auto gpair = [&](const T *p) {
+      const T glo = (k >= 1) ? (p[1] - p[0]) * g.rdzw[k] : T(0);
+      const T ghi = (k + 1 <= nz - 1) ? (p[2] - p[1]) * g.rdzw[k + 1] : T(0);
+      return T(0.5) * (glo + ghi);
+    };
+    const T GcC = gpair(pw), GcW = gpair(pWw), GcS = gpair(pSw);
+
+    // ---------------- u at the west face ----------------
+    {
+      const T U0 = T(0.5) * (DuW * ux[2] + DuC * ux[3]);
+      const T U1 = T(0.5) * (DuC * ux[3] + DuE * ux[4]);
+      const T fx0 = up5<T>(U0, ux[0], ux[1], ux[2], ux[3], ux[4], ux[5]);
+      const T fx1 = up5<T>(U1, ux[1], ux[2], ux[3], ux[4], ux[5], ux[6]);
+      const T V0 = T(0.5) * (DvW * vx[2] + DvC * vw[3]);
+      const T V1 = T(0.5) * (DvWN * vWN + DvN * vy[4]);
+      const T fy0 = up5<T>(V0, uy[0], uy[1], uy[2], uy[3], uy[4], uy[5]);
+      const T fy1 = up5<T>(V1, uy[1], uy[2], uy[3], uy[4], uy[5], uy[6]);
+      const T O0 = cu_O, fz0 = cu_f;  // interface k (0 at the ground)
+      T O1 = T(0), fz1 = T(0);
+      if (k + 1 < nz) {
+        O1 = T(0.5) * (oW[1] + ow[2]);
+        fz1 = up5<T>(O1, uw[1], uw[2], uw[3], uw[4], uw[5], uw[6]);
+      }
+      cu_O = O1;
+      cu_f = fz1;
+      const T adv = -(fx1 - fx0) * g.rdx - (fy1 - fy0) * g.rdy - (fz1 - fz0) * rdzk +
+                    uw[3] * ((U1 - U0) * g.rdx + (V1 - V0) * g.rdy + (O1 - O0) * rdzk);
+      const T thu = T(0.5) * (tx[2] + tw[3]);
+      const T px = (pw[1] - pWw[1]) * g.rdx - (omz * mxC) * (T(0.5) * (GcW + GcC));
+      const T R = adv * rDuC - T(kCp) * thu * px;
+      out.u[kb + cC] = static_cast<O>(s0.u[kb + cC] + h * static_cast<double>(R));
+    }
+    // ---------------- v at the south face ----------------
+    {
+      const T U0 = T(0.5) * (g.Du[gcol(g, i, j - 1)] * uy[2] + DuC * uw[3]);
+      const T U1 = T(0.5) * (DuES * uES + DuE * ux[4]);
+      const T fx0 = up5<T>(U0, vx[0], vx[1], vx[2], vx[3], vx[4], vx[5]);
+      const T fx1 = up5<T>(U1, vx[1], vx[2], vx[3], vx[4], vx[5], vx[6]);
+      const T V0 = T(0.5) * (DvS * vy[2] + DvC * vy[3]);
+      const T V1 = T(0.5) * (DvC * vy[3] + DvN * vy[4]);
+      const T fy0 = up5<T>(V0, vy[0], vy[1], vy[2], vy[3], vy[4], vy[5]);
+      const T fy1 = up5<T>(V1, vy[1], vy[2], vy[3], vy[4], vy[5], vy[6]);
+      const T O0 = cv_O, fz0 = cv_f;
+      T O1 = T(0), fz1 = T(0);
+      if (k + 1 < nz) {
+        O1 = T(0.5) * (oS[1] + ow[2]);
+        fz1 = up5<T>(O1, vw[1], vw[2], vw[3], vw[4], vw[5], vw[6]);
+      }
+      cv_O = O1;
+      cv_f = fz1;
+      const T adv = -(fx1 - fx0) * g.rdx - (fy1 - fy0) * g.rdy - (fz1 - fz0) * rdzk +
+                    vw[3] * ((U1 - U0) * g.rdx + (V1 - V0) * g.rdy + (O1 - O0) * rdzk);
+      const T thv = T(0.5) * (ty[2] + tw[3]);
+      const T py = (pw[1] - pSw[1]) * g.rdy - (omz * myC) * (T(0.5) * (GcS + GcC));
+      const T R = adv * rDvC - T(kCp) * thv * py;
+      out.v[kb + cC] = static_cast<O>(s0.v[kb + cC] + h * static_cast<double>(R));
+    }
+    // ---------------- w at interface k (1..nz-1) ----------------
+    // flux of w through the cell centre below / above interface k
+    const T Oc1 = T(0.5) * (ow[1] + ow[2]);
+    const T fzc1 = up5<T>(Oc1, ww[1], ww[2], ww[3], ww[4], ww[5], ww[6]);
+    if (k >= 1) {
+      const T U0 = T(0.5) * (DuC * uw[2] + DuC * uw[3]);
+      const T U1 = T(0.5) * (DuE * uE_prev + DuE * ux[4]);
+      const T fx0 = up5<T>(U0, wx[0], wx[1], wx[2], wx[3], wx[4], wx[5]);
+      const T fx1 = up5<T>(U1, wx[1], wx[2], wx[3], wx[4], wx[5], wx[6]);
+      const T V0 = T(0.5) * (DvC * vw[2] + DvC * vw[3]);
+      const T V1 = T(0.5) * (DvN * vN_prev + DvN * vy[4]);
+      const T fy0 = up5<T>(V0, wy[0], wy[1], wy[2], wy[3], wy[4], wy[5]);
+      const T fy1 = up5<T>(V1, wy[1], wy[2], wy[3], wy[4], wy[5], wy[6]);
+      const T Oc0 = cw_O, fz0 = cw_f;
+      const T fz1 = fzc1;
+      const T rz = g.rdzw[k];
+      const T adv = -(fx1 - fx0) * g.rdx - (fy1 - fy0) * g.rdy - (fz1 - fz0) * rz +
+                    ww[3] * ((U1 - U0) * g.rdx + (V1 - V0) * g.rdy + (Oc1 - Oc0) * rz);
+      const T thw = T(0.5) * (tw[2] + tw[3]);
+      const T th0w = T(0.5) * (g.theta0[rb - rk + gC] + g.theta0[rb + gC]);
+      const T pz = (pw[1] - pw[0]) * rz * rD;
+      const T R = adv * rD - T(kCp) * thw * pz + T(kG) * thw / th0w;
+      out.w[kb + cC] = static_cast<O>(s0.w[kb + cC] + h * static_cast<double>(R));
+    }
+    cw_O = Oc1;
+    cw_f = fzc1;
+    // ---------------- cell: rho', theta', pi' ----------------
+    {
+      const T mfx0 = DuC * uw[3], mfx1 = DuE * ux[4];
+      const T frx0 = DuC * (T(0.5) * (rWst + rw[1])) * uw[3];
+      const T frx1 = DuE * (T(0.5) * (rw[1] + rE)) * ux[4];
+      const T ftx0 = DuC * upwind_weno<T>(uw[3], tx[0], tx[1], tx[2], tx[3], tx[4], tx[5]);
+      const T ftx1 = DuE * upwind_weno<T>(ux[4], tx[1], tx[2], tx[3], tx[4], tx[5], tx[6]);
+      const T mfy0 = DvC * vw[3], mfy1 = DvN * vy[4];
+      const T fry0 = DvC * (T(0.5) * (rSth + rw[1])) * vw[3];
+      const T fry1 = DvN * (T(0.5) * (rw[1] + rN)) * vy[4];
+      const T fty0 = DvC * upwind_weno<T>(vw[3], ty[0], ty[1], ty[2], ty[3], ty[4], ty[5]);
+      const T fty1 = DvN * upwind_weno<T>(vy[4], ty[1], ty[2], ty[3], ty[4], ty[5], ty[6]);
+      const T mfz0 = cm_z, frz0 = cr_z, ftz0 = ct_z;
+      T mfz1 = T(0), frz1 = T(0), ftz1 = T(0);
+      if (k + 1 < nz) {
+        mfz1 = ow[2];
+        frz1 = (T(0.5) * (rw[1] + rw[2])) * ow[2];
+        ftz1 = upwind_weno<T>(ow[2], tw[1], tw[2], tw[3], tw[4], tw[5], tw[6]);
+      }
+      cm_z = mfz1;
+      cr_z = frz1;
+      ct_z = ftz1;
+      const T rr = -((frx1 - frx0) * g.rdx + (fry1 - fry0) * g.rdy + (frz1 - frz0) * rdzk) * rD;
+      const T div = (mfx1 - mfx0) * g.rdx + (mfy1 - mfy0) * g.rdy + (mfz1 - mfz0) * rdzk;
+      const T fd = (ftx1 - ftx0) * g.rdx + (fty1 - fty0) * g.rdy + (ftz1 - ftz0) * rdzk;
+      auto gam = [&](int kq) {
+        const int kk = kq < 1 ? 1 : (kq > nz - 1 ? nz - 1 : kq);
+        return (g.theta0[kk * rk + gC] - g.theta0[(kk - 1) * rk + gC]) * rD * g.rdzw[kk];
+      };
+      const T rt = -(fd - tw[3] * div) * rD - T(0.5) * (ww[3] * gam(k) + ww[4] * gam(k + 1));
+      const T rp = g.kap[rb + gC] * rr + g.lam[rb + gC] * rt;
+      const double rn = s0.rho[kb + cC] + h * static_cast<double>(rr);
+      const double tn = s0.th[kb + cC] + h * static_cast<double>(rt);
+      const double pn = s0.pi[kb + cC] + h * static_cast<double>(rp);
+      out.rho[kb + cC] = static_cast<O>(rn);
+      out.th[kb + cC] = tn;
+      out.pi[kb + cC] = static_cast<O>(pn);
+      bad |= !isfinite(tn);
+    }
+    // ---------------- advance the vertical windows ----------------
+    uE_prev = ux[4];
+    vN_prev = vy[4];
+    const int kn = clampk(k + 4, nz - 1), kw = clampk(k + 4, nz);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      uw[q] = uw[q + 1];
+      vw[q] = vw[q + 1];
+      tw[q] = tw[q + 1];
+      ww[q] = ww[q + 1];
+    }
+    uw[6] = ldd(U, kn, cC);
+    vw[6] = ldd(V, kn, cC);
+    tw[6] = ldd(TH, kn, cC);
+    ww[6] = ldd(W, kw, cC);
+    const int k2 = clampk(k + 2, nz - 1);
+    rw[0] = rw[1];
+    rw[1] = rw[2];
+    rw[2] = ldd(RH, k2, cC);
+    pw[0] = pw[1];
+    pw[1] = pw[2];
+    pw[2] = ldd(PI, k2, cC);
+    pWw[0] = pWw[1];
+    pWw[1] = pWw[2];
+    pWw[2] = ldd(PI, k2, cW);
+    pSw[0] = pSw[1];
+    pSw[1] = pSw[2];
+    pSw[2] = ldd(PI, k2, cS);
+    const int ko = clampk(k + 2, nz);
+    ow[0] = ow[1];
+    ow[1] = ow[2];
+    ow[2] = om[static_cast<size_t>(ko) * nxy + cC];
+    oW[0] = oW[1];
+    oW[1] = om[static_cast<size_t>(ko) * nxy + cW];
+    oS[0] = oS[1];
+    oS[1] = om[static_cast<size_t>(ko) * nxy + cS];
   }
-  // pass 3: Thomas forward sweep over interfaces 1..nz-1 (row m = k-1)
-  const int n = nz - 1;
-  T cprev = T(0), dprev = T(0);
+  out.w[static_cast<size_t>(nz) * nxy + cC] = static_cast<O>(s0.w[static_cast<size_t>(nz) * nxy + cC]);
+  if (bad) atomicOr(flag, 1);
+}
+
+// ---- HEVI factorisation table ----------------------------------------------
+// The tridiagonal matrix of the vertically implicit w system depends only on
+// the reference state, the column depth and dtau, so it is factorised once
+// per (stage, column): beta_m = a_m/pivot_m, 1/pivot_m and c'_m, stored
+// [m][compact column].  One thread per compact column.
+template <typename T>
+__global__ void __launch_bounds__(64) k_hevi_factor(Geo<T> g, T dtau, T an, T *__restrict__ tA, T *__restrict__ tINV,
+                                                    T *__restrict__ tCP) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.rk) return;
+  const int nz = g.nz, n = nz - 1, rk = g.rk;
+  const T dtD = dtau * g.rDc[c];
+  const T an2 = an * an;
+  auto ck = [&](int k) { return g.kap[k * rk + c] * dtD * g.rdzeta[k]; };
+  auto r0w = [&](int k) { return T(0.5) * (g.rho0[(k - 1) * rk + c] + g.rho0[k * rk + c]); };
+  T cp_prev = T(0);
   for (int m = 0; m < n; ++m) {
     const int k = m + 1;
-    const T ga = gk_at(k) * an2;
-    const T ckk = ck_at(k), ckm = ck_at(k - 1);
-    const T am = (m > 0) ? -ga * (ckm * r0w_at(k - 1)) : T(0);
-    const T bm = T(1) + ga * ((ckk + ckm) * r0w_at(k));
-    const T cm = (m < n - 1) ? -ga * (ckk * r0w_at(k + 1)) : T(0);
-    const T gkk = gk_at(k);
-    const T td = T(w[A.at(i, j, k)]) - gkk * ao * (T(pi[A.at(i, j, k)]) - T(pi[A.at(i, j, k - 1)])) -
-                 gkk * an * (PH[static_cast<size_t>(k) * nxy + col] - PH[static_cast<size_t>(k - 1) * nxy + col]);
-    T denom, cp, dp;
-    if (m == 0) {
-      denom = bm;
-      cp = cm / denom;
-      dp = td / denom;
-    } else {
-      denom = bm - am * cprev;
-      cp = cm / denom;
-      dp = (td - am * dprev) / denom;
-    }
-    CP[static_cast<size_t>(m) * nxy + col] = cp;
-    DP[static_cast<size_t>(m) * nxy + col] = dp;
-    cprev = cp;
-    dprev = dp;
+    const T th0w = T(0.5) * (g.theta0[(k - 1) * rk + c] + g.theta0[k * rk + c]);
+    const T gk = dtD * T(kCp) * th0w * g.rdzw[k];
+    const T ga = gk * an2;
+    const T ckk = ck(k), ckm = ck(k - 1);
+    const T a = (m > 0) ? -ga * (ckm * r0w(k - 1)) : T(0);
+    const T b = T(1) + ga * ((ckk + ckm) * r0w(k));
+    const T cc = (m < n - 1) ? -ga * (ckk * r0w(k + 1)) : T(0);
+    const T den = (m == 0) ? b : b - a * cp_prev;
+    const T inv = T(1) / den;
+    const T cp = cc * inv;
+    tA[m * rk + c] = a * inv;
+    tINV[m * rk + c] = inv;
+    tCP[m * rk + c] = cp;
+    cp_prev = cp;
   }
-  // pass 4: back substitution fused with the rho' / pi' update, top down
-  int bad = 0;
-  T wup = T(0);                 // w_new at the interface above (k+1)
-  T fup = T(0);                 // vertical flux F at interface k+1
-  const size_t ctop = A.at(i, j, nz);
-  wo[ctop] = static_cast<O>(T(w[ctop]));
-  for (int k = nz - 1; k >= 0; --k) {
-    T fk = T(0);
-    if (k >= 1) {
-      const int m = k - 1;
-      T wk = DP[static_cast<size_t>(m) * nxy + col];
-      if (m < n - 1) wk = wk - CP[static_cast<size_t>(m) * nxy + col] * wup;
-      fk = r0w_at(k) * (an * wk + ao * T(w[A.at(i, j, k)]) - mw_at(k));
-      wo[A.at(i, j, k)] = static_cast<O>(wk);
-      bad |= !isfinite(wk);
-      wup = wk;
-    } else {
-      wo[A.at(i, j, 0)] = static_cast<O>(MCC[col]);
+}
+
+// ---- fused acoustic substep -------------------------------------------------
+// One workgroup = 64 consecutive columns (one per lane) x NW waves; wave wv
+// owns the levels [wv*ch, wv*ch+ch).  Phase A (all waves, their levels):
+// horizontal momentum forward step with the old pi', slope flux Mc, rho0 mass
+// divergence, explicit rho'/pi' parts E, Q and the old-w part of the w RHS.
+// Phases B: vertical fluxes with the old w and the explicit pi' estimate,
+// exchanging the chunk-boundary values through LDS.  Phase C (wave 0): the
+// serial Thomas sweeps of the 64 columns over the RHS held in LDS, with the
+// stage's precomputed factorisation.  Phase D (all waves): vertical flux
+// divergence closes rho' and pi', and w is written.  State traffic per cell:
+// read u, v, w, rho', pi' once, write them once.
+template <typename T, typename I, typename O, int NW, int MAXCH>
+__global__ void __launch_bounds__(64 * NW) k_substep(Geo<T> g, const I *__restrict__ u, const I *__restrict__ v,
+                                                     const I *__restrict__ w, const I *__restrict__ rho,
+                                                     const I *__restrict__ pi, const T *__restrict__ tA,
+                                                     const T *__restrict__ tINV, const T *__restrict__ tCP, T dtau,
+                                                     T an, T ao, O *__restrict__ uo, O *__restrict__ vo,
+                                                     O *__restrict__ wo, O *__restrict__ rho_o, O *__restrict__ pi_o,
+                                                     int check, int *__restrict__ flag) {
+  extern __shared__ unsigned char smem_raw[];
+  const int nlds = (g.nz - 1) * 64;
+  T *s_td = reinterpret_cast<T *>(smem_raw);  // [nz-1][64] RHS -> d' -> w
+  T *s_be = s_td + nlds;                      // [nz-1][64] beta
+  T *s_cp = s_be + nlds;                      // [nz-1][64] c'
+  __shared__ T s_mc[NW][64], s_fh[NW][64], s_ph[NW][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, rk = g.rk;
+  const size_t nxy = static_cast<size_t>(nx) * ny;
+  const int i_raw = blockIdx.x * 64 + lane;
+  const bool active = i_raw < nx;
+  const int i = active ? i_raw : nx - 1;
+  const int j = blockIdx.y;
+  const int ch = (nz + NW - 1) / NW;
+  const int k0 = wv * ch;
+  const int nlev = max(0, min(nz, k0 + ch) - k0);
+  const int iW = wrapi(i - 1, nx), iE = wrapi(i + 1, nx), jS = wrapi(j - 1, ny), jN = wrapi(j + 1, ny);
+  const size_t cC = static_cast<size_t>(j) * nx + i, cW = static_cast<size_t>(j) * nx + iW,
+               cE = static_cast<size_t>(j) * nx + iE, cS = static_cast<size_t>(jS) * nx + i,
+               cN = static_cast<size_t>(jN) * nx + i;
+  const int gC = gcol(g, i, j), gW = gcol(g, iW, j), gE = gcol(g, iE, j), gS = gcol(g, i, jS), gN = gcol(g, i, jN);
+  const T rD = g.rDc[gC];
+  const T dtD = dtau * rD;
+  const T Du_w = g.Du[gC], Du_e = g.Du[gE], Dv_s = g.Dv[gC], Dv_n = g.Dv[gN];
+  const T sx_w = g.sx[gC], sx_e = g.sx[gE], sy_s = g.sy[gC], sy_n = g.sy[gN];
+  const T mx_w = g.metx[gC], mx_e = g.metx[gE], my_s = g.mety[gC], my_n = g.mety[gN];
+  const T cpdt = dtau;
+
+  T E[MAXCH], Q[MAXCH], Wt[MAXCH], Fh[MAXCH], Ph[MAXCH];
+  T mc_first = T(0), mc_last = T(0), mc0 = T(0), aw_first = T(0), r0w_first = T(0);
+
+  // ---------------- phase A ----------------
+  if (nlev > 0) {
+    auto ld5 = [&](int k, T *p) {
+      const size_t b = static_cast<size_t>(k) * nxy;
+      p[0] = T(pi[b + cC]);
+      p[1] = T(pi[b + cW]);
+      p[2] = T(pi[b + cE]);
+      p[3] = T(pi[b + cS]);
+      p[4] = T(pi[b + cN]);
+    };
+    T pm[5], p0[5], pp[5];
+    ld5(k0, p0);
+    if (k0 >= 1) ld5(k0 - 1, pm); else for (int q = 0; q < 5; ++q) pm[q] = p0[q];
+    T t0C_prev = (k0 >= 1) ? g.theta0[(k0 - 1) * rk + gC] : T(0);
+    T r0C_prev = (k0 >= 1) ? g.rho0[(k0 - 1) * rk + gC] : T(0);
+    T mc_prev = T(0);
+#pragma unroll
+    for (int c = 0; c < MAXCH; ++c) {
+      if (c < nlev) {
+        const int k = k0 + c;
+        const size_t b = static_cast<size_t>(k) * nxy;
+        if (k + 1 < nz) ld5(k + 1, pp); else for (int q = 0; q < 5; ++q) pp[q] = p0[q];
+        const T rz_lo = g.rdzw[k], rz_hi = g.rdzw[k + 1];
+        T Gc[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          const T glo = (k >= 1) ? (p0[q] - pm[q]) * rz_lo : T(0);
+          const T ghi = (k + 1 <= nz - 1) ? (pp[q] - p0[q]) * rz_hi : T(0);
+          Gc[q] = T(0.5) * (glo + ghi);
+        }
+        const T omz = T(1) - g.zc[k];
+        const T px_w = (p0[0] - p0[1]) * g.rdx - (omz * mx_w) * (T(0.5) * (Gc[1] + Gc[0]));
+        const T px_e = (p0[2] - p0[0]) * g.rdx - (omz * mx_e) * (T(0.5) * (Gc[0] + Gc[2]));
+        const T py_s = (p0[0] - p0[3]) * g.rdy - (omz * my_s) * (T(0.5) * (Gc[3] + Gc[0]));
+        const T py_n = (p0[4] - p0[0]) * g.rdy - (omz * my_n) * (T(0.5) * (Gc[0] + Gc[4]));
+        const int rb = k * rk;
+        const T t0C = g.theta0[rb + gC], t0W = g.theta0[rb + gW], t0E = g.theta0[rb + gE], t0S = g.theta0[rb + gS],
+                t0N = g.theta0[rb + gN];
+        const T un_w = T(u[b + cC]) - cpdt * (T(kCp) * (T(0.5) * (t0W + t0C)) * px_w);
+        const T un_e = T(u[b + cE]) - cpdt * (T(kCp) * (T(0.5) * (t0C + t0E)) * px_e);
+        const T vn_s = T(v[b + cC]) - cpdt * (T(kCp) * (T(0.5) * (t0S + t0C)) * py_s);
+        const T vn_n = T(v[b + cN]) - cpdt * (T(kCp) * (T(0.5) * (t0C + t0N)) * py_n);
+        if (active) {
+          uo[b + cC] = static_cast<O>(un_w);
+          vo[b + cC] = static_cast<O>(vn_s);
+        }
+        const T mc = T(0.5) * (un_w * sx_w + un_e * sx_e) + T(0.5) * (vn_s * sy_s + vn_n * sy_n);
+        const T r0C = g.rho0[rb + gC], r0W = g.rho0[rb + gW], r0E = g.rho0[rb + gE], r0S = g.rho0[rb + gS],
+                r0N = g.rho0[rb + gN];
+        const T hd = (Du_e * (T(0.5) * (r0C + r0E)) * un_e - Du_w * (T(0.5) * (r0W + r0C)) * un_w) * g.rdx +
+                     (Dv_n * (T(0.5) * (r0C + r0N)) * vn_n - Dv_s * (T(0.5) * (r0S + r0C)) * vn_s) * g.rdy;
+        const T cq = g.kap[rb + gC] * dtD;
+        E[c] = T(rho[b + cC]) - dtD * hd;
+        Q[c] = p0[0] - cq * hd;
+        const T wk = T(w[b + cC]);
+        Fh[c] = T(0);
+        Wt[c] = T(0);
+        if (k >= 1) {
+          const T th0w = T(0.5) * (t0C_prev + t0C);
+          const T gk = dtD * T(kCp) * th0w * g.rdzw[k];
+          Wt[c] = wk - gk * ao * (p0[0] - pm[0]);
+          const T r0w = T(0.5) * (r0C_prev + r0C);
+          if (c > 0) {
+            const T Mw = (T(1) - g.zeta[k]) * (T(0.5) * (mc_prev + mc));
+            Fh[c] = r0w * (ao * wk - Mw);
+          } else {
+            aw_first = ao * wk;
+            r0w_first = r0w;
+          }
+        }
+        if (c == 0) mc_first = mc;
+        if (k == 0) mc0 = mc;
+        mc_prev = mc;
+        t0C_prev = t0C;
+        r0C_prev = r0C;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          pm[q] = p0[q];
+          p0[q] = pp[q];
+        }
+      }
     }
-    const size_t cc = A.at(i, j, k);
-    const T drho = -dtD * (HD[static_cast<size_t>(k) * nxy + col] + (fup - fk) / g.dzeta[k]);
-    const T rn = T(rho[cc]) + drho;
-    const T pn = T(pi[cc]) + g.kap[cc] * drho;
-    rho_o[cc] = static_cast<O>(rn);
-    pi_o[cc] = static_cast<O>(pn);
-    bad |= !(isfinite(rn) && isfinite(pn));
-    fup = fk;
+    mc_last = mc_prev;
+    s_mc[wv][lane] = mc_last;
   }
-  if (check && bad) atomicOr(flag, 1);
+  __syncthreads();
+  // ---------------- phase B: boundary vertical flux, P^, RHS ----------------
+  if (nlev > 0) {
+    if (k0 >= 1) {
+      const T Mw = (T(1) - g.zeta[k0]) * (T(0.5) * (s_mc[wv - 1][lane] + mc_first));
+      Fh[0] = r0w_first * (aw_first - Mw);
+    }
+    s_fh[wv][lane] = Fh[0];
+  }
+  __syncthreads();
+  if (nlev > 0) {
+    const T fh_next_chunk = (k0 + nlev < nz) ? s_fh[wv + 1][lane] : T(0);
+#pragma unroll
+    for (int c = 0; c < MAXCH; ++c) {
+      if (c < nlev) {
+        const int k = k0 + c;
+        const T fhn = (c + 1 < nlev) ? Fh[(c + 1) < MAXCH ? c + 1 : c] : fh_next_chunk;
+        const T ck = g.kap[k * rk + gC] * dtD * g.rdzeta[k];
+        Ph[c] = Q[c] - ck * (fhn - Fh[c]);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < MAXCH; ++c)
+      if (c == nlev - 1) s_ph[wv][lane] = Ph[c];
+  }
+  __syncthreads();
+  if (nlev > 0) {
+    const T ph_prev_chunk = (k0 >= 1) ? s_ph[wv - 1][lane] : T(0);
+#pragma unroll
+    for (int c = 0; c < MAXCH; ++c) {
+      if (c < nlev) {
+        const int k = k0 + c;
+        if (k >= 1) {
+          const T php = (c > 0) ? Ph[c > 0 ? c - 1 : 0] : ph_prev_chunk;
+          const T th0w = T(0.5) * (g.theta0[(k - 1) * rk + gC] + g.theta0[k * rk + gC]);
+          const T gk = dtD * T(kCp) * th0w * g.rdzw[k];
+          const int m = k - 1;
+          const T td = Wt[c] - gk * an * (Ph[c] - php);
+          s_td[m * 64 + lane] = td * tINV[m * rk + gC];
+          s_be[m * 64 + lane] = tA[m * rk + gC];
+          s_cp[m * 64 + lane] = tCP[m * rk + gC];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // ---------------- phase C: Thomas sweeps (wave 0) ----------------
+  // RHS, beta and c' are in LDS (staged by all waves in phase B); operands
+  // are fetched one block of eight levels ahead so only the dependent
+  // multiply-subtract per level stays on the critical path.
+  if (wv == 0) {
+    constexpr int B = 8;
+    const int n = nz - 1;
+    const int nb = (n + B - 1) / B;
+    T ba[B], ta[B], bb[B], tb2[B];
+    auto fetch_fwd = [&](int blk, T *be, T *td) {
+#pragma unroll
+      for (int q = 0; q < B; ++q) {
+        const int m = blk * B + q;
+        const int mm = (m < n ? m : n - 1) * 64 + lane;
+        be[q] = s_be[mm];
+        td[q] = s_td[mm];
+      }
+    };
+    T dp = T(0);
+    fetch_fwd(0, ba, ta);
+    for (int blk = 0; blk < nb; ++blk) {
+      if (blk + 1 < nb) fetch_fwd(blk + 1, bb, tb2);
+#pragma unroll
+      for (int q = 0; q < B; ++q) {
+        const int m = blk * B + q;
+        if (m < n) {
+          dp = (m == 0) ? ta[q] : ta[q] - ba[q] * dp;
+          s_td[m * 64 + lane] = dp;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < B; ++q) {
+        ba[q] = bb[q];
+        ta[q] = tb2[q];
+      }
+    }
+    auto fetch_bwd = [&](int blk, T *cp, T *d) {
+#pragma unroll
+      for (int q = 0; q < B; ++q) {
+        const int m = n - 2 - (blk * B + q);
+        const int mm = (m >= 0 ? m : 0) * 64 + lane;
+        cp[q] = s_cp[mm];
+        d[q] = s_td[mm];
+      }
+    };
+    T x = dp;
+    const int nbb = (n - 1 + B - 1) / B;
+    if (nbb > 0) fetch_bwd(0, ba, ta);
+    for (int blk = 0; blk < nbb; ++blk) {
+      if (blk + 1 < nbb) fetch_bwd(blk + 1, bb, tb2);
+#pragma unroll
+      for (int q = 0; q < B; ++q) {
+        const int m = n - 2 - (blk * B + q);
+        if (m >= 0) {
+          x = ta[q] - ba[q] * x;
+          s_td[m * 64 + lane] = x;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < B; ++q) {
+        ba[q] = bb[q];
+        ta[q] = tb2[q];
+      }
+    }
+  }
+  __syncthreads();
+  // ---------------- phase D: close rho', pi', write w ----------------
+  if (nlev > 0) {
+    int bad = 0;
+    auto fluxF = [&](int k, T fh) {
+      if (k <= 0 || k >= nz) return T(0);
+      const T r0w = T(0.5) * (g.rho0[(k - 1) * rk + gC] + g.rho0[k * rk + gC]);
+      return fh + (an * r0w) * s_td[(k - 1) * 64 + lane];
+    };
+    const T fh_next_chunk = (k0 + nlev < nz) ? s_fh[wv + 1][lane] : T(0);
+    T flo = fluxF(k0, Fh[0]);
+#pragma unroll
+    for (int c = 0; c < MAXCH; ++c) {
+      if (c < nlev) {
+        const int k = k0 + c;
+        const size_t b = static_cast<size_t>(k) * nxy;
+        const T fhn = (c + 1 < nlev) ? Fh[(c + 1) < MAXCH ? c + 1 : c] : fh_next_chunk;
+        const T fhi = fluxF(k + 1, fhn);
+        const T vd = -dtD * ((fhi - flo) * g.rdzeta[k]);
+        const T rn = E[c] + vd;
+        const T pn = Q[c] + g.kap[k * rk + gC] * vd;
+        const T wn = (k >= 1) ? s_td[(k - 1) * 64 + lane] : mc0;
+        if (active) {
+          rho_o[b + cC] = static_cast<O>(rn);
+          pi_o[b + cC] = static_cast<O>(pn);
+          wo[b + cC] = static_cast<O>(wn);
+        }
+        bad |= !(isfinite(rn) && isfinite(pn) && isfinite(wn));
+        flo = fhi;
+      }
+    }
+    if (k0 + nlev == nz && active) {
+      const size_t b = static_cast<size_t>(nz) * nxy;
+      wo[b + cC] = static_cast<O>(T(w[b + cC]));
+    }
+    if (check && bad && active) atomicOr(flag, 1);
+  }
 }
 
 // ---- sponge (damping.cpp:76-138 with perturbation targets 0) -------------
@@ -598,6 +838,7 @@ struct vibe_mi_domain {
   Geo<double> g64{};
   Geo<float> g32{};
   double *zs = nullptr, *sponge = nullptr, *rho0 = nullptr, *theta0 = nullptr, *pi0 = nullptr;
+  double *dzeta64 = nullptr, *zc64 = nullptr;  // diagnostics (FP64, per level)
   std::vector<double> h_sponge;
   int kmin_sponge = 0;
   FieldSet S[3];  // [0] = s0 of the step; [1], [2] stage outputs
@@ -605,7 +846,11 @@ struct vibe_mi_domain {
   AcousticSet<double> A64[2];
   AcousticSet<float> A32[2];
   void *om = nullptr;   // Omega (T)
-  void *scr = nullptr;  // 5 column work arrays (T)
+  void *tab = nullptr;  // HEVI factorisation tables: 4 slots x {a, 1/pivot, c'} x (nz-1) x rk (T)
+  double tab_dtau[4] = {-1.0, -1.0, -1.0, -1.0};
+  int nxr = 1, nyr = 1, rk = 1;
+  int nw_fp32 = 4, nw_fp64 = 8;  // waves per substep workgroup (autotuned)
+  bool tuned[2] = {false, false};
   int *flag = nullptr;
   NSum *part = nullptr;
   double *dres = nullptr;
@@ -636,77 +881,110 @@ int upload(vibe_mi_domain *d, P **out, const std::vector<P> &h) {
   return 0;
 }
 
+// Host-side derived geometry in FP64 (the same expressions as the oracle),
+// compacted over directions in which the domain is uniform, then converted to
+// the compute precision T.
 template <typename T>
 int build_geo(vibe_mi_domain *d, const vibe_mi_domain_desc *ds, Geo<T> &g) {
   const int nx = ds->nx, ny = ds->ny, nz = ds->nz;
-  const size_t nxy = d->nxy, nc = d->ncell;
-  std::vector<double> zeta(ds->zeta, ds->zeta + nz + 1), dzeta(nz), zc(nz), dzw(nz + 1, 0.0);
+  const int nxr = d->nxr, nyr = d->nyr, rk = d->rk;
+  std::vector<double> zeta(ds->zeta, ds->zeta + nz + 1), zc(nz), rdzeta(nz), rdzw(nz + 1, 0.0);
   for (int k = 0; k < nz; ++k) {
-    dzeta[k] = zeta[k + 1] - zeta[k];
     zc[k] = 0.5 * (zeta[k] + zeta[k + 1]);
+    rdzeta[k] = 1.0 / (zeta[k + 1] - zeta[k]);
   }
-  for (int k = 1; k < nz; ++k) dzw[k] = zc[k] - zc[k - 1];
-  std::vector<double> Dc(nxy), Du(nxy), Dv(nxy), sx(nxy), sy(nxy);
+  for (int k = 1; k < nz; ++k) rdzw[k] = 1.0 / (zc[k] - zc[k - 1]);
   auto H = [&](int i, int j) { return static_cast<size_t>((j + ny) % ny) * nx + (i + nx) % nx; };
-  for (int j = 0; j < ny; ++j)
-    for (int i = 0; i < nx; ++i) {
-      const double z = ds->zs[H(i, j)], zw = ds->zs[H(i - 1, j)], zsth = ds->zs[H(i, j - 1)];
-      Dc[H(i, j)] = ds->ztop - z;
-      Du[H(i, j)] = ds->ztop - 0.5 * (zw + z);
-      Dv[H(i, j)] = ds->ztop - 0.5 * (zsth + z);
-      sx[H(i, j)] = (z - zw) / ds->dx;
-      sy[H(i, j)] = (z - zsth) / ds->dy;
+  std::vector<double> Du(rk), Dv(rk), sx(rk), sy(rk), rDc(rk), rDu(rk), rDv(rk), metx(rk), mety(rk);
+  for (int jr = 0; jr < nyr; ++jr)
+    for (int ir = 0; ir < nxr; ++ir) {
+      const size_t q = static_cast<size_t>(jr) * nxr + ir;
+      const double z = ds->zs[H(ir, jr)], zw = ds->zs[H(ir - 1, jr)], zsth = ds->zs[H(ir, jr - 1)];
+      const double dc = ds->ztop - z;
+      Du[q] = ds->ztop - 0.5 * (zw + z);
+      Dv[q] = ds->ztop - 0.5 * (zsth + z);
+      sx[q] = (z - zw) / ds->dx;
+      sy[q] = (z - zsth) / ds->dy;
+      rDc[q] = 1.0 / dc;
+      rDu[q] = 1.0 / Du[q];
+      rDv[q] = 1.0 / Dv[q];
+      metx[q] = sx[q] * rDu[q];
+      mety[q] = sy[q] * rDv[q];
     }
-  std::vector<double> kap(nc), lam(nc);
-  for (size_t c = 0; c < nc; ++c) {
-    kap[c] = (kRd / kCv) * ds->pi0[c] / ds->rho0[c];
-    lam[c] = (kRd / kCv) * ds->pi0[c] / ds->theta0[c];
-  }
+  const size_t nr = static_cast<size_t>(rk) * nz;
+  std::vector<double> r0(nr), t0(nr), kap(nr), lam(nr);
+  for (int k = 0; k < nz; ++k)
+    for (int jr = 0; jr < nyr; ++jr)
+      for (int ir = 0; ir < nxr; ++ir) {
+        const size_t q = static_cast<size_t>(k) * rk + static_cast<size_t>(jr) * nxr + ir;
+        const size_t c = static_cast<size_t>(k) * d->nxy + H(ir, jr);
+        r0[q] = ds->rho0[c];
+        t0[q] = ds->theta0[c];
+        kap[q] = (kRd / kCv) * ds->pi0[c] / ds->rho0[c];
+        lam[q] = (kRd / kCv) * ds->pi0[c] / ds->theta0[c];
+      }
   auto cvt = [](const std::vector<double> &a) { return std::vector<T>(a.begin(), a.end()); };
   g.nx = nx;
   g.ny = ny;
   g.nz = nz;
-  g.dx = static_cast<T>(ds->dx);
-  g.dy = static_cast<T>(ds->dy);
+  g.hi = nxr > 1 ? 1 : 0;
+  g.hj = nyr > 1 ? nxr : 0;
+  g.rk = rk;
+  g.rdx = static_cast<T>(1.0 / ds->dx);
+  g.rdy = static_cast<T>(1.0 / ds->dy);
   T *p;
-  if (int rc = upload(d, &p, cvt(zeta))) return rc;
-  g.zeta = p;
-  if (int rc = upload(d, &p, cvt(dzeta))) return rc;
-  g.dzeta = p;
-  if (int rc = upload(d, &p, cvt(zc))) return rc;
-  g.zc = p;
-  if (int rc = upload(d, &p, cvt(dzw))) return rc;
-  g.dzw = p;
-  if (int rc = upload(d, &p, cvt(Dc))) return rc;
-  g.Dc = p;
-  if (int rc = upload(d, &p, cvt(Du))) return rc;
-  g.Du = p;
-  if (int rc = upload(d, &p, cvt(Dv))) return rc;
-  g.Dv = p;
-  if (int rc = upload(d, &p, cvt(sx))) return rc;
-  g.sx = p;
-  if (int rc = upload(d, &p, cvt(sy))) return rc;
-  g.sy = p;
-  if (int rc = upload(d, &p, std::vector<T>(ds->rho0, ds->rho0 + nc))) return rc;
-  g.rho0 = p;
-  if (int rc = upload(d, &p, std::vector<T>(ds->theta0, ds->theta0 + nc))) return rc;
-  g.theta0 = p;
-  if (int rc = upload(d, &p, cvt(kap))) return rc;
-  g.kap = p;
-  if (int rc = upload(d, &p, cvt(lam))) return rc;
-  g.lam = p;
+#define VMI_UP(field, src)                            \
+  if (int rc = upload(d, &p, cvt(src))) return rc;    \
+  g.field = p;
+  VMI_UP(zeta, zeta)
+  VMI_UP(zc, zc)
+  VMI_UP(rdzeta, rdzeta)
+  VMI_UP(rdzw, rdzw)
+  VMI_UP(Du, Du)
+  VMI_UP(Dv, Dv)
+  VMI_UP(sx, sx)
+  VMI_UP(sy, sy)
+  VMI_UP(rDc, rDc)
+  VMI_UP(rDu, rDu)
+  VMI_UP(rDv, rDv)
+  VMI_UP(metx, metx)
+  VMI_UP(mety, mety)
+  VMI_UP(rho0, r0)
+  VMI_UP(theta0, t0)
+  VMI_UP(kap, kap)
+  VMI_UP(lam, lam)
+#undef VMI_UP
   return 0;
+}
+
+// Directions in which terrain and reference state are uniform.
+void detect_uniform(const vibe_mi_domain_desc *ds, bool &xuni, bool &yuni) {
+  const int nx = ds->nx, ny = ds->ny, nz = ds->nz;
+  const size_t nxy = static_cast<size_t>(nx) * ny;
+  xuni = yuni = true;
+  const double *f3[3] = {ds->rho0, ds->theta0, ds->pi0};
+  for (int j = 0; j < ny && (xuni || yuni); ++j)
+    for (int i = 0; i < nx; ++i) {
+      const size_t c = static_cast<size_t>(j) * nx + i;
+      if (ds->zs[c] != ds->zs[static_cast<size_t>(j) * nx]) xuni = false;
+      if (ds->zs[c] != ds->zs[i]) yuni = false;
+    }
+  for (int k = 0; k < nz && (xuni || yuni); ++k)
+    for (const double *f : f3) {
+      const double *pl = f + static_cast<size_t>(k) * nxy;
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+          const double x = pl[static_cast<size_t>(j) * nx + i];
+          if (xuni && x != pl[static_cast<size_t>(j) * nx]) xuni = false;
+          if (yuni && x != pl[i]) yuni = false;
+        }
+    }
 }
 
 inline dim3 grid_of(const vibe_mi_domain *d, int nk) {
   return dim3(ceil_div(d->nx, 64), d->ny, nk);
 }
 constexpr unsigned kBlk = 64;
-
-int timed_begin(vibe_mi_domain *d, bool timed, int slot) {
-  if (timed) VMI_HIP(hipEventRecord(d->ev[slot], d->stream));
-  return 0;
-}
 
 template <typename T>
 int run_omega(vibe_mi_domain *d, const Geo<T> &g, const FieldSet &s) {
@@ -720,29 +998,176 @@ template <typename T>
 int run_stage(vibe_mi_domain *d, const Geo<T> &g, const FieldSet &cur, const FieldSet &s0, double h,
               AcousticSet<T> &a, FieldSet &nxt) {
   if (int rc = run_omega<T>(d, g, cur)) return rc;
-  const T *om = static_cast<const T *>(d->om);
-  hipLaunchKernelGGL((k_stage_u<T, T>), grid_of(d, d->nz), dim3(kBlk), 0, d->stream, g, cur.u, cur.v, cur.th, cur.pi,
-                     om, s0.u, h, a.u);
-  hipLaunchKernelGGL((k_stage_v<T, T>), grid_of(d, d->nz), dim3(kBlk), 0, d->stream, g, cur.u, cur.v, cur.th, cur.pi,
-                     om, s0.v, h, a.v);
-  hipLaunchKernelGGL((k_stage_w<T, T>), grid_of(d, d->nz + 1), dim3(kBlk), 0, d->stream, g, cur.u, cur.v, cur.w,
-                     cur.th, cur.pi, om, s0.w, h, a.w);
-  hipLaunchKernelGGL((k_stage_cell<T, T>), grid_of(d, d->nz), dim3(kBlk), 0, d->stream, g, cur.u, cur.v, cur.w,
-                     cur.rho, cur.th, cur.pi, om, s0.rho, s0.th, s0.pi, h, a.rho, nxt.th, a.pi, d->flag);
+  const StageIn ci{cur.u, cur.v, cur.w, cur.rho, cur.th, cur.pi};
+  const StageIn si{s0.u, s0.v, s0.w, s0.rho, s0.th, s0.pi};
+  const StageOut<T> so{a.u, a.v, a.w, a.rho, a.pi, nxt.th};
+  hipLaunchKernelGGL((k_stage_col<T, T>), dim3(ceil_div(d->nx, 64), ceil_div(d->ny, 4)), dim3(64, 4), 0, d->stream, g,
+                     ci, si, static_cast<const T *>(d->om), h, so, d->flag);
+  VMI_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+T *table_slot(vibe_mi_domain *d, int slot) {
+  const size_t per = static_cast<size_t>(d->nz > 1 ? d->nz - 1 : 1) * d->rk;
+  return static_cast<T *>(d->tab) + static_cast<size_t>(slot) * 3 * per;
+}
+
+// Factorise the HEVI matrix for dtau into table slot `slot` (cached).
+template <typename T>
+int ensure_table(vibe_mi_domain *d, const Geo<T> &g, int slot, double dtau) {
+  if (d->tab_dtau[slot] == dtau) return 0;
+  const size_t per = static_cast<size_t>(d->nz - 1) * d->rk;
+  T *t = table_slot<T>(d, slot);
+  const T an = static_cast<T>(0.5 * (1.0 + d->eps_s));
+  hipLaunchKernelGGL((k_hevi_factor<T>), dim3(ceil_div(d->rk, 64)), dim3(64), 0, d->stream, g, static_cast<T>(dtau),
+                     an, t, t + per, t + 2 * per);
+  VMI_HIP(hipGetLastError());
+  d->tab_dtau[slot] = dtau;
+  return 0;
+}
+
+constexpr size_t kMaxLds = 160 * 1024;  // LDS per CU (and per workgroup) on gfx950
+
+template <typename T, typename I, typename O, int NW, int MAXCH>
+int launch_substep(vibe_mi_domain *d, const Geo<T> &g, const T *tab, const I *u, const I *v, const I *w,
+                   const I *rho, const I *pi, O *uo, O *vo, O *wo, O *rho_o, O *pi_o, T dtau, T an, T ao,
+                   int check) {
+  const size_t per = static_cast<size_t>(d->nz - 1) * d->rk;
+  const size_t lds = 3 * static_cast<size_t>(d->nz - 1) * 64 * sizeof(T);
+  const size_t lds_static = 3 * NW * 64 * sizeof(T);
+  if (lds + lds_static > kMaxLds) return VIBE_C_ERR_BAD_ARG;  // column too tall for one workgroup
+  static bool attr_set = false;  // once per instantiation
+  if (!attr_set) {
+    VMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_substep<T, I, O, NW, MAXCH>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kMaxLds - lds_static)));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k_substep<T, I, O, NW, MAXCH>), dim3(ceil_div(d->nx, 64), d->ny), dim3(64 * NW), lds, d->stream,
+                     g, u, v, w, rho, pi, tab, tab + per, tab + 2 * per, dtau, an, ao, uo, vo, wo, rho_o, pi_o, check,
+                     d->flag);
+  return 0;
+}
+
+template <typename T, typename I, typename O, int NW>
+int dispatch_substep(vibe_mi_domain *d, const Geo<T> &g, const T *tab, const I *u, const I *v, const I *w,
+                     const I *rho, const I *pi, O *uo, O *vo, O *wo, O *rho_o, O *pi_o, T dtau, T an, T ao, int check) {
+  const int ch = (d->nz + NW - 1) / NW;
+#define VMI_SUB(M) launch_substep<T, I, O, NW, M>(d, g, tab, u, v, w, rho, pi, uo, vo, wo, rho_o, pi_o, dtau, an, ao, check)
+  int rc;
+  if constexpr (NW == 16) {
+    if (ch <= 4) rc = VMI_SUB(4);
+    else if (ch <= 8) rc = VMI_SUB(8);
+    else return VIBE_C_ERR_BAD_ARG;
+  } else if constexpr (NW == 8) {
+    if (ch <= 4) rc = VMI_SUB(4);
+    else if (ch <= 8) rc = VMI_SUB(8);
+    else if (ch <= 12) rc = VMI_SUB(12);
+    else if (ch <= 16) rc = VMI_SUB(16);
+    else return VIBE_C_ERR_BAD_ARG;
+  } else {
+    if (ch <= 8) rc = VMI_SUB(8);
+    else if (ch <= 12) rc = VMI_SUB(12);
+    else if (ch <= 16) rc = VMI_SUB(16);
+    else if (ch <= 24) rc = VMI_SUB(24);
+    else if (ch <= 32) rc = VMI_SUB(32);
+    else return VIBE_C_ERR_BAD_ARG;
+  }
+#undef VMI_SUB
+  if (rc) return rc;
   VMI_HIP(hipGetLastError());
   return 0;
 }
 
 template <typename T, typename I, typename O>
-int run_substep(vibe_mi_domain *d, const Geo<T> &g, const I *u, const I *v, const I *w, const I *rho, const I *pi,
-                O *uo, O *vo, O *wo, O *rho_o, O *pi_o, double dtau, int check) {
+int substep_nw(vibe_mi_domain *d, const Geo<T> &g, int nw, const T *tab, const I *u, const I *v, const I *w,
+               const I *rho, const I *pi, O *uo, O *vo, O *wo, O *rho_o, O *pi_o, T dtau, T an, T ao, int check) {
+  if (nw == 16)
+    return dispatch_substep<T, I, O, 16>(d, g, tab, u, v, w, rho, pi, uo, vo, wo, rho_o, pi_o, dtau, an, ao, check);
+  if (nw == 8)
+    return dispatch_substep<T, I, O, 8>(d, g, tab, u, v, w, rho, pi, uo, vo, wo, rho_o, pi_o, dtau, an, ao, check);
+  return dispatch_substep<T, I, O, 4>(d, g, tab, u, v, w, rho, pi, uo, vo, wo, rho_o, pi_o, dtau, an, ao, check);
+}
+
+// ---- substep workgroup autotuner --------------------------------------------
+// Waves per substep workgroup (4, 8 or 16: levels per wave = nz/waves) trade
+// registers per wave, LDS per workgroup and the length of each wave's serial
+// march against the idle time of the other waves during the Thomas sweep.  The
+// fastest is measured once per (nx, ny, nz, compaction, precision) with HIP
+// events on scratch buffers (the state is not touched) and cached for the
+// process.  VIBE_MI_SUBSTEP_WAVES=4|8|16 overrides the measurement.
+struct TuneKey {
+  int nx, ny, nz, rk, prec;
+  bool operator<(const TuneKey &o) const {
+    return std::tie(nx, ny, nz, rk, prec) < std::tie(o.nx, o.ny, o.nz, o.rk, o.prec);
+  }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, int> g_tune;
+
+template <typename T>
+int autotune_substep(vibe_mi_domain *d, const Geo<T> &g, AcousticSet<T> *A, double dtau) {
+  int &slot_nw = sizeof(T) == 8 ? d->nw_fp64 : d->nw_fp32;
+  if (d->tuned[sizeof(T) == 8]) return 0;
+  if (const char *env = std::getenv("VIBE_MI_SUBSTEP_WAVES")) {
+    const int v = std::atoi(env);
+    if (v == 4 || v == 8 || v == 16) {
+      slot_nw = v;
+      d->tuned[sizeof(T) == 8] = true;
+      return 0;
+    }
+  }
+  const TuneKey key{d->nx, d->ny, d->nz, d->rk, static_cast<int>(sizeof(T))};
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      slot_nw = it->second;
+      d->tuned[sizeof(T) == 8] = true;
+      return 0;
+    }
+  }
+  if (int rc = ensure_table<T>(d, g, 3, dtau)) return rc;
   const T an = static_cast<T>(0.5 * (1.0 + d->eps_s)), ao = static_cast<T>(0.5 * (1.0 - d->eps_s));
-  hipLaunchKernelGGL((k_sub_h<T, I, O>), grid_of(d, d->nz), dim3(kBlk), 0, d->stream, g, pi, u, v,
-                     static_cast<T>(dtau), uo, vo, check, d->flag);
-  hipLaunchKernelGGL((k_sub_v<T, I, O>), dim3(ceil_div(d->nx, 64), d->ny, 1), dim3(kBlk), 0, d->stream, g, uo, vo, w,
-                     rho, pi, static_cast<T>(dtau), an, ao, wo, rho_o, pi_o, static_cast<T *>(d->scr), check, d->flag);
-  VMI_HIP(hipGetLastError());
+  const T *tab = table_slot<T>(d, 3);
+  AcousticSet<T> &in = A[0], &out = A[1];
+  float best = 1e30f;
+  int best_nw = slot_nw;
+  for (int nw : {4, 8, 16}) {
+    float total = 0.f;
+    bool ok = true;
+    for (int rep = 0; rep < 4 && ok; ++rep) {
+      VMI_HIP(hipEventRecord(d->ev[0], d->stream));
+      ok = substep_nw<T, T, T>(d, g, nw, tab, in.u, in.v, in.w, in.rho, in.pi, out.u, out.v, out.w, out.rho, out.pi,
+                               static_cast<T>(dtau), an, ao, 0) == 0;
+      VMI_HIP(hipEventRecord(d->ev[1], d->stream));
+      VMI_HIP(hipEventSynchronize(d->ev[1]));
+      float ms = 0.f;
+      VMI_HIP(hipEventElapsedTime(&ms, d->ev[0], d->ev[1]));
+      if (rep > 0) total += ms;  // first launch warms the instruction cache
+    }
+    if (ok && total < best) {
+      best = total;
+      best_nw = nw;
+    }
+  }
+  (void)hipGetLastError();  // a shape that does not fit is simply skipped
+  slot_nw = best_nw;
+  d->tuned[sizeof(T) == 8] = true;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune[key] = best_nw;
   return 0;
+}
+
+template <typename T, typename I, typename O>
+int run_substep(vibe_mi_domain *d, const Geo<T> &g, int slot, const I *u, const I *v, const I *w, const I *rho,
+                const I *pi, O *uo, O *vo, O *wo, O *rho_o, O *pi_o, double dtau, int check) {
+  if (int rc = ensure_table<T>(d, g, slot, dtau)) return rc;
+  const T an = static_cast<T>(0.5 * (1.0 + d->eps_s)), ao = static_cast<T>(0.5 * (1.0 - d->eps_s));
+  const T *tab = table_slot<T>(d, slot);
+  const int nw = sizeof(T) == 8 ? d->nw_fp64 : d->nw_fp32;
+  return substep_nw<T, I, O>(d, g, nw, tab, u, v, w, rho, pi, uo, vo, wo, rho_o, pi_o, static_cast<T>(dtau), an, ao,
+                             check);
 }
 
 // One full RK3 step.  Stage outputs go to S[(cur0+1)%3] / S[(cur0+2)%3].
@@ -750,6 +1175,7 @@ template <typename T>
 int step_impl(vibe_mi_domain *d, const Geo<T> &g, AcousticSet<T> *A, double dt, int nsub, bool timed) {
   const int ns = nsub < 1 ? 1 : nsub;
   const double hk[3] = {dt / 3.0, dt / 2.0, dt};
+  if (int rc = autotune_substep<T>(d, g, A, dt / static_cast<double>(ns))) return rc;
   const int i0 = d->cur0, i1 = (d->cur0 + 1) % 3, i2 = (d->cur0 + 2) % 3;
   const FieldSet &s0 = d->S[i0];
   const int outs[3] = {i1, i2, i1};
@@ -768,13 +1194,13 @@ int step_impl(vibe_mi_domain *d, const Geo<T> &g, AcousticSet<T> *A, double dt, 
       const AcousticSet<T> &in = A[p];
       if (!last) {
         AcousticSet<T> &out = A[1 - p];
-        if (int rc = run_substep<T, T, T>(d, g, in.u, in.v, in.w, in.rho, in.pi, out.u, out.v, out.w, out.rho, out.pi,
-                                          dtau, 0))
+        if (int rc = run_substep<T, T, T>(d, g, st, in.u, in.v, in.w, in.rho, in.pi, out.u, out.v, out.w, out.rho,
+                                          out.pi, dtau, 0))
           return rc;
         p = 1 - p;
       } else {
-        if (int rc = run_substep<T, T, double>(d, g, in.u, in.v, in.w, in.rho, in.pi, nxt.u, nxt.v, nxt.w, nxt.rho,
-                                               nxt.pi, dtau, 1))
+        if (int rc = run_substep<T, T, double>(d, g, st, in.u, in.v, in.w, in.rho, in.pi, nxt.u, nxt.v, nxt.w,
+                                               nxt.rho, nxt.pi, dtau, 1))
           return rc;
       }
     }
@@ -850,11 +1276,25 @@ int create_impl(const vibe_mi_domain_desc *ds, vibe_mi_domain *d) {
   VMI_HIP(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
   d->ev.resize(4);
   for (auto &e : d->ev) VMI_HIP(hipEventCreate(&e));
+  bool xuni = false, yuni = false;
+  detect_uniform(ds, xuni, yuni);
+  d->nxr = xuni ? 1 : d->nx;
+  d->nyr = yuni ? 1 : d->ny;
+  d->rk = d->nxr * d->nyr;
   if (int rc = build_geo<double>(d, ds, d->g64)) return rc;
   if (d->precision == VIBE_MI_MIXED)
     if (int rc = build_geo<float>(d, ds, d->g32)) return rc;
   const size_t nc = d->ncell;
   if (int rc = upload(d, &d->zs, std::vector<double>(ds->zs, ds->zs + d->nxy))) return rc;
+  {
+    std::vector<double> dz(ds->nz), zc(ds->nz);
+    for (int k = 0; k < ds->nz; ++k) {
+      dz[k] = ds->zeta[k + 1] - ds->zeta[k];
+      zc[k] = 0.5 * (ds->zeta[k] + ds->zeta[k + 1]);
+    }
+    if (int rc = upload(d, &d->dzeta64, dz)) return rc;
+    if (int rc = upload(d, &d->zc64, zc)) return rc;
+  }
   d->h_sponge.assign(ds->sponge, ds->sponge + ds->nz);
   if (int rc = upload(d, &d->sponge, d->h_sponge)) return rc;
   d->kmin_sponge = d->nz + 1;
@@ -879,8 +1319,8 @@ int create_impl(const vibe_mi_domain_desc *ds, vibe_mi_domain *d) {
   char *p;
   if (int rc = dalloc(d, &p, d->nwf * tsz)) return rc;
   d->om = p;
-  if (int rc = dalloc(d, &p, 5 * nc * tsz)) return rc;
-  d->scr = p;
+  if (int rc = dalloc(d, &p, 4 * 3 * static_cast<size_t>(d->nz - 1) * d->rk * tsz)) return rc;
+  d->tab = p;
   if (int rc = dalloc(d, &d->flag, 4)) return rc;
   if (int rc = dalloc(d, &d->part, static_cast<size_t>(kRedBlocks) * kEnergyComps)) return rc;
   if (int rc = dalloc(d, &d->dres, 8)) return rc;
@@ -1063,8 +1503,8 @@ int vibe_mi_domain_substep(vibe_mi_domain *d, double dtau) {
   if (int rc = set_device(d)) return rc;
   FieldSet &s = d->S[d->cur0];
   FieldSet &o = d->S[(d->cur0 + 1) % 3];
-  if (int rc = run_substep<double, double, double>(d, d->g64, s.u, s.v, s.w, s.rho, s.pi, o.u, o.v, o.w, o.rho, o.pi,
-                                                   dtau, 1))
+  if (int rc = run_substep<double, double, double>(d, d->g64, 3, s.u, s.v, s.w, s.rho, s.pi, o.u, o.v, o.w, o.rho,
+                                                   o.pi, dtau, 1))
     return rc;
   VMI_HIP(hipMemcpyAsync(o.th, s.th, d->ncell * 8, hipMemcpyDeviceToDevice, d->stream));
   d->cur0 = (d->cur0 + 1) % 3;
@@ -1078,7 +1518,7 @@ int vibe_mi_domain_energy(vibe_mi_domain *d, double out[6]) {
   const size_t chunk = (d->ncell + kRedBlocks - 1) / kRedBlocks;
   const int blocks = static_cast<int>((d->ncell + chunk - 1) / chunk);
   hipLaunchKernelGGL(k_energy_partial, dim3(blocks), dim3(kRedThreads), 0, d->stream, d->nx, d->ny, d->nz, d->dx,
-                     d->dy, d->ztop, d->zs, d->g64.dzeta, d->g64.zc, d->rho0, d->theta0, d->pi0, f.u, f.v, f.w, f.rho,
+                     d->dy, d->ztop, d->zs, d->dzeta64, d->zc64, d->rho0, d->theta0, d->pi0, f.u, f.v, f.w, f.rho,
                      f.th, f.pi, chunk, d->part);
   hipLaunchKernelGGL(k_red_final, dim3(1), dim3(kRedThreads), 0, d->stream, d->part, blocks, kEnergyComps, d->dres);
   VMI_HIP(hipGetLastError());
@@ -1097,7 +1537,7 @@ int vibe_mi_domain_cfl(vibe_mi_domain *d, double dt, double *cfl, double *max_wi
   VMI_HIP(hipMemsetAsync(d->dmax, 0, 2 * sizeof(unsigned long long), d->stream));
   const unsigned blocks = std::min<size_t>(ceil_div(d->ncell, 256), 2048);
   hipLaunchKernelGGL(k_cfl_state, dim3(blocks), dim3(256), 0, d->stream, d->nx, d->ny, d->nz, d->dx, d->dy, d->ztop,
-                     d->zs, d->g64.dzeta, f.u, f.v, f.w, dt, d->dmax);
+                     d->zs, d->dzeta64, f.u, f.v, f.w, dt, d->dmax);
   VMI_HIP(hipGetLastError());
   unsigned long long r[2];
   VMI_HIP(hipMemcpyAsync(r, d->dmax, sizeof(r), hipMemcpyDeviceToHost, d->stream));

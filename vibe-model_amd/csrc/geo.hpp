@@ -1,0 +1,51 @@
+// geo.hpp -- device view of one domain's geometry and reference state.
+//
+// Column fields (terrain depth, slopes, ...) and the 3-D reference state are
+// stored compacted: a direction in which neither the terrain nor the
+// reference state varies is stored once.  Flat terrain gives 1-D (per-level)
+// reference arrays, a y-uniform ridge (x, z) arrays, general terrain full
+// 3-D arrays.  Compacted arrays stay resident in L2 across a launch, so the
+// reference state costs no HBM traffic on flat and ridge grids.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace vibe_mi {
+
+template <typename T>
+struct Geo {
+  int nx, ny, nz;
+  int hi, hj;  // column index = j*hj + i*hi (0 where the domain is uniform)
+  int rk;      // 3-D compact index = k*rk + j*hj + i*hi
+  T rdx, rdy;
+  const T *zeta;    // nz+1 interfaces
+  const T *zc;      // nz   cell-centre zeta
+  const T *rdzeta;  // nz   1/(zeta[k+1]-zeta[k])
+  const T *rdzw;    // nz+1 1/(zc[k]-zc[k-1]) for 1..nz-1, 0 elsewhere
+  const T *Du, *Dv, *sx, *sy, *rDc, *rDu, *rDv, *metx, *mety;  // columns
+  const T *rho0, *theta0, *kap, *lam;                           // 3-D reference
+};
+
+__device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+__device__ __forceinline__ int clampk(int k, int hi) { return k < 0 ? 0 : (k > hi ? hi : k); }
+
+// Index helpers for the dense state arrays ([k][j][i], periodic i and j).
+struct Acc {
+  int nx, ny;
+  size_t nxy;
+  __device__ __forceinline__ size_t h(int i, int j) const {
+    return static_cast<size_t>(wrapi(j, ny)) * nx + wrapi(i, nx);
+  }
+  __device__ __forceinline__ size_t at(int i, int j, int k) const { return static_cast<size_t>(k) * nxy + h(i, j); }
+};
+
+template <typename T>
+__device__ __forceinline__ int gcol(const Geo<T> &g, int i, int j) {
+  return wrapi(j, g.ny) * g.hj + wrapi(i, g.nx) * g.hi;
+}
+template <typename T>
+__device__ __forceinline__ int gref(const Geo<T> &g, int i, int j, int k) {
+  return k * g.rk + gcol(g, i, j);
+}
+
+}  // namespace vibe_mi
