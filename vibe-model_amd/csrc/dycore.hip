@@ -128,16 +128,15 @@ __global__ void __launch_bounds__(256) k_stage_col(Geo<T> g, StageIn cur, StageI
   const double *__restrict__ RH = cur.rho;
   const double *__restrict__ TH = cur.th;
   const double *__restrict__ PI = cur.pi;
-  int xi[7];
-  size_t yj[7];
+  uint32_t xi[7], yj[7];
 #pragma unroll
   for (int d = 0; d < 7; ++d) {
-    xi[d] = wrapi(i + d - 3, nx);
-    yj[d] = static_cast<size_t>(wrapi(j + d - 3, ny)) * nx;
+    xi[d] = static_cast<uint32_t>(wrapi(i + d - 3, nx));
+    yj[d] = static_cast<uint32_t>(wrapi(j + d - 3, ny)) * static_cast<uint32_t>(nx);
   }
-  const size_t cC = yj[3] + xi[3], cW = yj[3] + xi[2], cS = yj[2] + xi[3];
-  const size_t cE = yj[3] + xi[4], cN = yj[4] + xi[3];
-  const size_t cWN = yj[4] + xi[2], cES = yj[2] + xi[4];
+  const uint32_t cC = yj[3] + xi[3], cW = yj[3] + xi[2], cS = yj[2] + xi[3];
+  const uint32_t cE = yj[3] + xi[4], cN = yj[4] + xi[3];
+  const uint32_t cWN = yj[4] + xi[2], cES = yj[2] + xi[4];
   // column geometry (compacted)
   const int gC = gcol(g, i, j), gW = gcol(g, i - 1, j), gE = gcol(g, i + 1, j), gS = gcol(g, i, j - 1),
             gN = gcol(g, i, j + 1), gWN = gcol(g, i - 1, j + 1), gES = gcol(g, i + 1, j - 1);
@@ -145,7 +144,8 @@ __global__ void __launch_bounds__(256) k_stage_col(Geo<T> g, StageIn cur, StageI
   const T DvS = g.Dv[gS], DvC = g.Dv[gC], DvN = g.Dv[gN], DvWN = g.Dv[gWN], DvW = g.Dv[gW];
   const T rDuC = g.rDu[gC], rDvC = g.rDv[gC], rD = g.rDc[gC], mxC = g.metx[gC], myC = g.mety[gC];
 
-  auto ldd = [&](const double *f, int k, size_t c) { return T(f[static_cast<size_t>(k) * nxy + c]); };
+  auto ldd = [&](const double *f, int k, uint32_t c) { return T(ld32(f, static_cast<size_t>(k) * nxy, c)); };
+  auto ldo = [&](int k, uint32_t c) { return ld32(om, static_cast<size_t>(k) * nxy, c); };
   T uw[7], vw[7], ww[7], tw[7], rw[3], pw[3], pWw[3], pSw[3], ow[3], oW[2], oS[2];
 #pragma unroll
   for (int q = 0; q < 7; ++q) {
@@ -162,18 +162,18 @@ __global__ void __launch_bounds__(256) k_stage_col(Geo<T> g, StageIn cur, StageI
     pw[q] = ldd(PI, kc, cC);
     pWw[q] = ldd(PI, kc, cW);
     pSw[q] = ldd(PI, kc, cS);
-    ow[q] = om[static_cast<size_t>(clampk(q - 1, nz)) * nxy + cC];
+    ow[q] = ldo(clampk(q - 1, nz), cC);
   }
-  oW[0] = om[cW];
-  oW[1] = om[nxy + cW];
-  oS[0] = om[cS];
-  oS[1] = om[nxy + cS];
+  oW[0] = ldo(0, cW);
+  oW[1] = ldo(1, cW);
+  oS[0] = ldo(0, cS);
+  oS[1] = ldo(1, cS);
   T uE_prev = T(0), vN_prev = T(0);
   // z-face fluxes of the level above, reused as the next level's lower face
   T cu_O = T(0), cu_f = T(0), cv_O = T(0), cv_f = T(0), cw_O = T(0), cw_f = T(0);
   T cm_z = T(0), cr_z = T(0), ct_z = T(0);
   int bad = 0;
-  out.w[cC] = static_cast<O>(s0.w[cC]);  // interface 0 keeps its value (R_w = 0)
+  st32(out.w, 0, cC, static_cast<O>(ld32(s0.w, 0, cC)));  // interface 0 keeps its value (R_w = 0)
 
   for (int k = 0; k < nz; ++k) {
     const size_t kb = static_cast<size_t>(k) * nxy;
@@ -183,21 +183,21 @@ __global__ void __launch_bounds__(256) k_stage_col(Geo<T> g, StageIn cur, StageI
 #pragma unroll
     for (int d = 0; d < 7; ++d) {
       if (d == 3) continue;
-      ux[d] = T(U[kb + yj[3] + xi[d]]);
-      uy[d] = T(U[kb + yj[d] + xi[3]]);
-      vx[d] = T(V[kb + yj[3] + xi[d]]);
-      vy[d] = T(V[kb + yj[d] + xi[3]]);
-      tx[d] = T(TH[kb + yj[3] + xi[d]]);
-      ty[d] = T(TH[kb + yj[d] + xi[3]]);
-      wx[d] = T(W[kb + yj[3] + xi[d]]);
-      wy[d] = T(W[kb + yj[d] + xi[3]]);
+      ux[d] = T(ld32(U, kb, yj[3] + xi[d]));
+      uy[d] = T(ld32(U, kb, yj[d] + xi[3]));
+      vx[d] = T(ld32(V, kb, yj[3] + xi[d]));
+      vy[d] = T(ld32(V, kb, yj[d] + xi[3]));
+      tx[d] = T(ld32(TH, kb, yj[3] + xi[d]));
+      ty[d] = T(ld32(TH, kb, yj[d] + xi[3]));
+      wx[d] = T(ld32(W, kb, yj[3] + xi[d]));
+      wy[d] = T(ld32(W, kb, yj[d] + xi[3]));
     }
     ux[3] = uy[3] = uw[3];
     vx[3] = vy[3] = vw[3];
     tx[3] = ty[3] = tw[3];
     wx[3] = wy[3] = ww[3];
-    const T vWN = T(V[kb + cWN]), uES = T(U[kb + cES]);
-    const T rWst = T(RH[kb + cW]), rE = T(RH[kb + cE]), rSth = T(RH[kb + cS]), rN = T(RH[kb + cN]);
+    const T vWN = T(ld32(V, kb, cWN)), uES = T(ld32(U, kb, cES));
+    const T rWst = T(ld32(RH, kb, cW)), rE = T(ld32(RH, kb, cE)), rSth = T(ld32(RH, kb, cS)), rN = T(ld32(RH, kb, cN));
     const T omz = T(1) - g.zc[k];
     const T rdzk = g.rdzeta[k];
     // vertical pi' gradients (0 at ground and lid)
@@ -231,7 +231,7 @@ __global__ void __launch_bounds__(256) k_stage_col(Geo<T> g, StageIn cur, StageI
       const T thu = T(0.5) * (tx[2] + tw[3]);
       const T px = (pw[1] - pWw[1]) * g.rdx - (omz * mxC) * (T(0.5) * (GcW + GcC));
       const T R = adv * rDuC - T(kCp) * thu * px;
-      out.u[kb + cC] = static_cast<O>(s0.u[kb + cC] + h * static_cast<double>(R));
+      st32(out.u, kb, cC, static_cast<O>(ld32(s0.u, kb, cC) + h * static_cast<double>(R)));
     }
     // ---------------- v at the south face ----------------
     {
@@ -256,7 +256,7 @@ __global__ void __launch_bounds__(256) k_stage_col(Geo<T> g, StageIn cur, StageI
       const T thv = T(0.5) * (ty[2] + tw[3]);
       const T py = (pw[1] - pSw[1]) * g.rdy - (omz * myC) * (T(0.5) * (GcS + GcC));
       const T R = adv * rDvC - T(kCp) * thv * py;
-      out.v[kb + cC] = static_cast<O>(s0.v[kb + cC] + h * static_cast<double>(R));
+      st32(out.v, kb, cC, static_cast<O>(ld32(s0.v, kb, cC) + h * static_cast<double>(R)));
     }
     // ---------------- w at interface k (1..nz-1) ----------------
     // flux of w through the cell centre below / above interface k
@@ -280,7 +280,7 @@ __global__ void __launch_bounds__(256) k_stage_col(Geo<T> g, StageIn cur, StageI
       const T th0w = T(0.5) * (g.theta0[rb - rk + gC] + g.theta0[rb + gC]);
       const T pz = (pw[1] - pw[0]) * rz * rD;
       const T R = adv * rD - T(kCp) * thw * pz + T(kG) * thw / th0w;
-      out.w[kb + cC] = static_cast<O>(s0.w[kb + cC] + h * static_cast<double>(R));
+      st32(out.w, kb, cC, static_cast<O>(ld32(s0.w, kb, cC) + h * static_cast<double>(R)));
     }
     cw_O = Oc1;
     cw_f = fzc1;
@@ -315,12 +315,12 @@ __global__ void __launch_bounds__(256) k_stage_col(Geo<T> g, StageIn cur, StageI
       };
       const T rt = -(fd - tw[3] * div) * rD - T(0.5) * (ww[3] * gam(k) + ww[4] * gam(k + 1));
       const T rp = g.kap[rb + gC] * rr + g.lam[rb + gC] * rt;
-      const double rn = s0.rho[kb + cC] + h * static_cast<double>(rr);
-      const double tn = s0.th[kb + cC] + h * static_cast<double>(rt);
-      const double pn = s0.pi[kb + cC] + h * static_cast<double>(rp);
-      out.rho[kb + cC] = static_cast<O>(rn);
-      out.th[kb + cC] = tn;
-      out.pi[kb + cC] = static_cast<O>(pn);
+      const double rn = ld32(s0.rho, kb, cC) + h * static_cast<double>(rr);
+      const double tn = ld32(s0.th, kb, cC) + h * static_cast<double>(rt);
+      const double pn = ld32(s0.pi, kb, cC) + h * static_cast<double>(rp);
+      st32(out.rho, kb, cC, static_cast<O>(rn));
+      st32(out.th, kb, cC, tn);
+      st32(out.pi, kb, cC, static_cast<O>(pn));
       bad |= !isfinite(tn);
     }
     // ---------------- advance the vertical windows ----------------
@@ -354,13 +354,13 @@ __global__ void __launch_bounds__(256) k_stage_col(Geo<T> g, StageIn cur, StageI
     const int ko = clampk(k + 2, nz);
     ow[0] = ow[1];
     ow[1] = ow[2];
-    ow[2] = om[static_cast<size_t>(ko) * nxy + cC];
+    ow[2] = ldo(ko, cC);
     oW[0] = oW[1];
-    oW[1] = om[static_cast<size_t>(ko) * nxy + cW];
+    oW[1] = ldo(ko, cW);
     oS[0] = oS[1];
-    oS[1] = om[static_cast<size_t>(ko) * nxy + cS];
+    oS[1] = ldo(ko, cS);
   }
-  out.w[static_cast<size_t>(nz) * nxy + cC] = static_cast<O>(s0.w[static_cast<size_t>(nz) * nxy + cC]);
+  st32(out.w, static_cast<size_t>(nz) * nxy, cC, static_cast<O>(ld32(s0.w, static_cast<size_t>(nz) * nxy, cC)));
   if (bad) atomicOr(flag, 1);
 }
 
@@ -424,7 +424,10 @@ __global__ void __launch_bounds__(64 * NW) k_substep(Geo<T> g, const I *__restri
   T *s_be = s_td + nlds;                      // [nz-1][64] beta
   T *s_cp = s_be + nlds;                      // [nz-1][64] c'
   __shared__ T s_mc[NW][64], s_fh[NW][64], s_ph[NW][64];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  // wave index: uniform within the wavefront, so the level bases below live in
+  // scalar registers and loads use the scalar-base + 32-bit offset form
+  const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
   const int nx = g.nx, ny = g.ny, nz = g.nz, rk = g.rk;
   const size_t nxy = static_cast<size_t>(nx) * ny;
   const int i_raw = blockIdx.x * 64 + lane;
@@ -435,10 +438,11 @@ __global__ void __launch_bounds__(64 * NW) k_substep(Geo<T> g, const I *__restri
   const int k0 = wv * ch;
   const int nlev = max(0, min(nz, k0 + ch) - k0);
   const int iW = wrapi(i - 1, nx), iE = wrapi(i + 1, nx), jS = wrapi(j - 1, ny), jN = wrapi(j + 1, ny);
-  const size_t cC = static_cast<size_t>(j) * nx + i, cW = static_cast<size_t>(j) * nx + iW,
-               cE = static_cast<size_t>(j) * nx + iE, cS = static_cast<size_t>(jS) * nx + i,
-               cN = static_cast<size_t>(jN) * nx + i;
-  const int gC = gcol(g, i, j), gW = gcol(g, iW, j), gE = gcol(g, iE, j), gS = gcol(g, i, jS), gN = gcol(g, i, jN);
+  const uint32_t unx = static_cast<uint32_t>(nx);
+  const uint32_t cC = static_cast<uint32_t>(j) * unx + i, cW = static_cast<uint32_t>(j) * unx + iW,
+                 cE = static_cast<uint32_t>(j) * unx + iE, cS = static_cast<uint32_t>(jS) * unx + i,
+                 cN = static_cast<uint32_t>(jN) * unx + i;
+  const uint32_t gC = gcol(g, i, j), gW = gcol(g, iW, j), gE = gcol(g, iE, j), gS = gcol(g, i, jS), gN = gcol(g, i, jN);
   const T rD = g.rDc[gC];
   const T dtD = dtau * rD;
   const T Du_w = g.Du[gC], Du_e = g.Du[gE], Dv_s = g.Dv[gC], Dv_n = g.Dv[gN];
@@ -453,17 +457,29 @@ __global__ void __launch_bounds__(64 * NW) k_substep(Geo<T> g, const I *__restri
   if (nlev > 0) {
     auto ld5 = [&](int k, T *p) {
       const size_t b = static_cast<size_t>(k) * nxy;
-      p[0] = T(pi[b + cC]);
-      p[1] = T(pi[b + cW]);
-      p[2] = T(pi[b + cE]);
-      p[3] = T(pi[b + cS]);
-      p[4] = T(pi[b + cN]);
+      p[0] = T(ld32(pi, b, cC));
+      p[1] = T(ld32(pi, b, cW));
+      p[2] = T(ld32(pi, b, cE));
+      p[3] = T(ld32(pi, b, cS));
+      p[4] = T(ld32(pi, b, cN));
     };
-    T pm[5], p0[5], pp[5];
+    // pi' of the five columns at the current and next level; the vertical
+    // gradient of the interface below is carried from the previous level.
+    T p0[5], pp[5], gl[5];
     ld5(k0, p0);
-    if (k0 >= 1) ld5(k0 - 1, pm); else for (int q = 0; q < 5; ++q) pm[q] = p0[q];
-    T t0C_prev = (k0 >= 1) ? g.theta0[(k0 - 1) * rk + gC] : T(0);
-    T r0C_prev = (k0 >= 1) ? g.rho0[(k0 - 1) * rk + gC] : T(0);
+    T pmC = p0[0];
+    if (k0 >= 1) {
+      T pm[5];
+      ld5(k0 - 1, pm);
+      pmC = pm[0];
+#pragma unroll
+      for (int q = 0; q < 5; ++q) gl[q] = (p0[q] - pm[q]) * g.rdzw[k0];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) gl[q] = T(0);
+    }
+    T t0C_prev = (k0 >= 1) ? ld32(g.theta0, static_cast<size_t>(k0 - 1) * rk, gC) : T(0);
+    T r0C_prev = (k0 >= 1) ? ld32(g.rho0, static_cast<size_t>(k0 - 1) * rk, gC) : T(0);
     T mc_prev = T(0);
 #pragma unroll
     for (int c = 0; c < MAXCH; ++c) {
@@ -471,13 +487,13 @@ __global__ void __launch_bounds__(64 * NW) k_substep(Geo<T> g, const I *__restri
         const int k = k0 + c;
         const size_t b = static_cast<size_t>(k) * nxy;
         if (k + 1 < nz) ld5(k + 1, pp); else for (int q = 0; q < 5; ++q) pp[q] = p0[q];
-        const T rz_lo = g.rdzw[k], rz_hi = g.rdzw[k + 1];
+        const T rz_hi = g.rdzw[k + 1];
         T Gc[5];
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
-          const T glo = (k >= 1) ? (p0[q] - pm[q]) * rz_lo : T(0);
           const T ghi = (k + 1 <= nz - 1) ? (pp[q] - p0[q]) * rz_hi : T(0);
-          Gc[q] = T(0.5) * (glo + ghi);
+          Gc[q] = T(0.5) * (gl[q] + ghi);
+          gl[q] = ghi;
         }
         const T omz = T(1) - g.zc[k];
         const T px_w = (p0[0] - p0[1]) * g.rdx - (omz * mx_w) * (T(0.5) * (Gc[1] + Gc[0]));
@@ -485,31 +501,31 @@ __global__ void __launch_bounds__(64 * NW) k_substep(Geo<T> g, const I *__restri
         const T py_s = (p0[0] - p0[3]) * g.rdy - (omz * my_s) * (T(0.5) * (Gc[3] + Gc[0]));
         const T py_n = (p0[4] - p0[0]) * g.rdy - (omz * my_n) * (T(0.5) * (Gc[0] + Gc[4]));
         const int rb = k * rk;
-        const T t0C = g.theta0[rb + gC], t0W = g.theta0[rb + gW], t0E = g.theta0[rb + gE], t0S = g.theta0[rb + gS],
-                t0N = g.theta0[rb + gN];
-        const T un_w = T(u[b + cC]) - cpdt * (T(kCp) * (T(0.5) * (t0W + t0C)) * px_w);
-        const T un_e = T(u[b + cE]) - cpdt * (T(kCp) * (T(0.5) * (t0C + t0E)) * px_e);
-        const T vn_s = T(v[b + cC]) - cpdt * (T(kCp) * (T(0.5) * (t0S + t0C)) * py_s);
-        const T vn_n = T(v[b + cN]) - cpdt * (T(kCp) * (T(0.5) * (t0C + t0N)) * py_n);
+        const T t0C = ld32(g.theta0, static_cast<size_t>(rb), gC), t0W = ld32(g.theta0, static_cast<size_t>(rb), gW), t0E = ld32(g.theta0, static_cast<size_t>(rb), gE), t0S = ld32(g.theta0, static_cast<size_t>(rb), gS),
+                t0N = ld32(g.theta0, static_cast<size_t>(rb), gN);
+        const T un_w = T(ld32(u, b, cC)) - cpdt * (T(kCp) * (T(0.5) * (t0W + t0C)) * px_w);
+        const T un_e = T(ld32(u, b, cE)) - cpdt * (T(kCp) * (T(0.5) * (t0C + t0E)) * px_e);
+        const T vn_s = T(ld32(v, b, cC)) - cpdt * (T(kCp) * (T(0.5) * (t0S + t0C)) * py_s);
+        const T vn_n = T(ld32(v, b, cN)) - cpdt * (T(kCp) * (T(0.5) * (t0C + t0N)) * py_n);
         if (active) {
-          uo[b + cC] = static_cast<O>(un_w);
-          vo[b + cC] = static_cast<O>(vn_s);
+          st32(uo, b, cC, static_cast<O>(un_w));
+          st32(vo, b, cC, static_cast<O>(vn_s));
         }
         const T mc = T(0.5) * (un_w * sx_w + un_e * sx_e) + T(0.5) * (vn_s * sy_s + vn_n * sy_n);
-        const T r0C = g.rho0[rb + gC], r0W = g.rho0[rb + gW], r0E = g.rho0[rb + gE], r0S = g.rho0[rb + gS],
-                r0N = g.rho0[rb + gN];
+        const T r0C = ld32(g.rho0, static_cast<size_t>(rb), gC), r0W = ld32(g.rho0, static_cast<size_t>(rb), gW), r0E = ld32(g.rho0, static_cast<size_t>(rb), gE), r0S = ld32(g.rho0, static_cast<size_t>(rb), gS),
+                r0N = ld32(g.rho0, static_cast<size_t>(rb), gN);
         const T hd = (Du_e * (T(0.5) * (r0C + r0E)) * un_e - Du_w * (T(0.5) * (r0W + r0C)) * un_w) * g.rdx +
                      (Dv_n * (T(0.5) * (r0C + r0N)) * vn_n - Dv_s * (T(0.5) * (r0S + r0C)) * vn_s) * g.rdy;
-        const T cq = g.kap[rb + gC] * dtD;
-        E[c] = T(rho[b + cC]) - dtD * hd;
+        const T cq = ld32(g.kap, static_cast<size_t>(rb), gC) * dtD;
+        E[c] = T(ld32(rho, b, cC)) - dtD * hd;
         Q[c] = p0[0] - cq * hd;
-        const T wk = T(w[b + cC]);
+        const T wk = T(ld32(w, b, cC));
         Fh[c] = T(0);
         Wt[c] = T(0);
         if (k >= 1) {
           const T th0w = T(0.5) * (t0C_prev + t0C);
           const T gk = dtD * T(kCp) * th0w * g.rdzw[k];
-          Wt[c] = wk - gk * ao * (p0[0] - pm[0]);
+          Wt[c] = wk - gk * ao * (p0[0] - pmC);
           const T r0w = T(0.5) * (r0C_prev + r0C);
           if (c > 0) {
             const T Mw = (T(1) - g.zeta[k]) * (T(0.5) * (mc_prev + mc));
@@ -524,11 +540,9 @@ __global__ void __launch_bounds__(64 * NW) k_substep(Geo<T> g, const I *__restri
         mc_prev = mc;
         t0C_prev = t0C;
         r0C_prev = r0C;
+        pmC = p0[0];
 #pragma unroll
-        for (int q = 0; q < 5; ++q) {
-          pm[q] = p0[q];
-          p0[q] = pp[q];
-        }
+        for (int q = 0; q < 5; ++q) p0[q] = pp[q];
       }
     }
     mc_last = mc_prev;
@@ -551,7 +565,7 @@ __global__ void __launch_bounds__(64 * NW) k_substep(Geo<T> g, const I *__restri
       if (c < nlev) {
         const int k = k0 + c;
         const T fhn = (c + 1 < nlev) ? Fh[(c + 1) < MAXCH ? c + 1 : c] : fh_next_chunk;
-        const T ck = g.kap[k * rk + gC] * dtD * g.rdzeta[k];
+        const T ck = ld32(g.kap, static_cast<size_t>(k) * rk, gC) * dtD * g.rdzeta[k];
         Ph[c] = Q[c] - ck * (fhn - Fh[c]);
       }
     }
@@ -568,13 +582,13 @@ __global__ void __launch_bounds__(64 * NW) k_substep(Geo<T> g, const I *__restri
         const int k = k0 + c;
         if (k >= 1) {
           const T php = (c > 0) ? Ph[c > 0 ? c - 1 : 0] : ph_prev_chunk;
-          const T th0w = T(0.5) * (g.theta0[(k - 1) * rk + gC] + g.theta0[k * rk + gC]);
+          const T th0w = T(0.5) * (ld32(g.theta0, static_cast<size_t>(k - 1) * rk, gC) + ld32(g.theta0, static_cast<size_t>(k) * rk, gC));
           const T gk = dtD * T(kCp) * th0w * g.rdzw[k];
           const int m = k - 1;
           const T td = Wt[c] - gk * an * (Ph[c] - php);
-          s_td[m * 64 + lane] = td * tINV[m * rk + gC];
-          s_be[m * 64 + lane] = tA[m * rk + gC];
-          s_cp[m * 64 + lane] = tCP[m * rk + gC];
+          s_td[m * 64 + lane] = td * ld32(tINV, static_cast<size_t>(m) * rk, gC);
+          s_be[m * 64 + lane] = ld32(tA, static_cast<size_t>(m) * rk, gC);
+          s_cp[m * 64 + lane] = ld32(tCP, static_cast<size_t>(m) * rk, gC);
         }
       }
     }
@@ -651,7 +665,7 @@ __global__ void __launch_bounds__(64 * NW) k_substep(Geo<T> g, const I *__restri
     int bad = 0;
     auto fluxF = [&](int k, T fh) {
       if (k <= 0 || k >= nz) return T(0);
-      const T r0w = T(0.5) * (g.rho0[(k - 1) * rk + gC] + g.rho0[k * rk + gC]);
+      const T r0w = T(0.5) * (ld32(g.rho0, static_cast<size_t>(k - 1) * rk, gC) + ld32(g.rho0, static_cast<size_t>(k) * rk, gC));
       return fh + (an * r0w) * s_td[(k - 1) * 64 + lane];
     };
     const T fh_next_chunk = (k0 + nlev < nz) ? s_fh[wv + 1][lane] : T(0);
@@ -665,12 +679,12 @@ __global__ void __launch_bounds__(64 * NW) k_substep(Geo<T> g, const I *__restri
         const T fhi = fluxF(k + 1, fhn);
         const T vd = -dtD * ((fhi - flo) * g.rdzeta[k]);
         const T rn = E[c] + vd;
-        const T pn = Q[c] + g.kap[k * rk + gC] * vd;
+        const T pn = Q[c] + ld32(g.kap, static_cast<size_t>(k) * rk, gC) * vd;
         const T wn = (k >= 1) ? s_td[(k - 1) * 64 + lane] : mc0;
         if (active) {
-          rho_o[b + cC] = static_cast<O>(rn);
-          pi_o[b + cC] = static_cast<O>(pn);
-          wo[b + cC] = static_cast<O>(wn);
+          st32(rho_o, b, cC, static_cast<O>(rn));
+          st32(pi_o, b, cC, static_cast<O>(pn));
+          st32(wo, b, cC, static_cast<O>(wn));
         }
         bad |= !(isfinite(rn) && isfinite(pn) && isfinite(wn));
         flo = fhi;
@@ -678,7 +692,7 @@ __global__ void __launch_bounds__(64 * NW) k_substep(Geo<T> g, const I *__restri
     }
     if (k0 + nlev == nz && active) {
       const size_t b = static_cast<size_t>(nz) * nxy;
-      wo[b + cC] = static_cast<O>(T(w[b + cC]));
+      st32(wo, b, cC, static_cast<O>(T(ld32(w, b, cC))));
     }
     if (check && bad && active) atomicOr(flag, 1);
   }

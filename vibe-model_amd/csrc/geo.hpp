@@ -39,6 +39,20 @@ struct Acc {
   __device__ __forceinline__ size_t at(int i, int j, int k) const { return static_cast<size_t>(k) * nxy + h(i, j); }
 };
 
+// Loads/stores addressed as (wave-uniform base of a level, 32-bit per-lane
+// element index): the base lives in scalar registers and the byte offset fits
+// 32 bits, so the compiler emits the scalar-base + 32-bit vector-offset form
+// instead of 64-bit per-lane address arithmetic.  A level of one field must
+// stay below 4 GiB (nx*ny*8 bytes), far above any single-GPU domain here.
+template <typename P>
+__device__ __forceinline__ P ld32(const P *__restrict__ f, size_t base, uint32_t idx) {
+  return *reinterpret_cast<const P *>(reinterpret_cast<const char *>(f + base) + idx * static_cast<uint32_t>(sizeof(P)));
+}
+template <typename P>
+__device__ __forceinline__ void st32(P *__restrict__ f, size_t base, uint32_t idx, P v) {
+  *reinterpret_cast<P *>(reinterpret_cast<char *>(f + base) + idx * static_cast<uint32_t>(sizeof(P))) = v;
+}
+
 template <typename T>
 __device__ __forceinline__ int gcol(const Geo<T> &g, int i, int j) {
   return wrapi(j, g.ny) * g.hj + wrapi(i, g.nx) * g.hi;
