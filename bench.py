@@ -15,8 +15,17 @@ each stage kernel group and each substep group.
 Roofline: algorithmic bytes per cell-update (compulsory prognostic traffic)
   FP64   8 * (48 + 30 * n_s) = 1824 B     (n_s = 6)
   mixed  8 * 48 + 4 * 30 * n_s = 1104 B
-against 8.0 TB/s.  The dominant kernel group (acoustic substeps) is reported
-with its own algorithmic bytes: 10 field accesses per cell per substep.
+against 8.0 TB/s.  The dominant kernel is the fused acoustic substep
+k_substep (3 n_s launches per step, about 60 % of the step).  Its algorithmic
+bytes per launch: read u, v, w, rho', pi' and write them once, 4 B each in
+the FP32-mixed path (40 B per cell), except the last substep of a stage,
+which writes its FP64 stage output (20 + 40 = 60 B per cell); averaged over
+the n_s launches of a stage: (40 (n_s - 1) + 60) / n_s B per cell.  The
+launch duration is measured with HIP events on the domain's stream around
+each stage's substep group (only k_substep launches are in the group).
+"traffic" (HBM bytes per launch) comes from the rocprofv3 FETCH_SIZE /
+WRITE_SIZE per-dispatch summary committed for the same workload, each
+scaled by what the counter reads for a plain 1 GiB copy (--copy-calibration).
 
 cpu_baseline: the FP64 CPU oracle (a restatement of the reference's step)
 timed on this host on a bounded sample of the same workload.
@@ -40,6 +49,45 @@ def log(*a):
 
 def bytes_per_cell_update(nsub: int, mixed: bool) -> int:
     return 8 * 48 + (4 if mixed else 8) * 30 * nsub
+
+
+def substep_bytes_per_launch(cells: int, nsub: int) -> float:
+    """Algorithmic bytes of one k_substep launch, averaged over a stage."""
+    return cells * (40.0 * (nsub - 1) + 60.0) / nsub
+
+
+def measured_traffic(case):
+    """HBM bytes per k_substep launch from the committed rocprofv3 summary of
+    this workload (FETCH_SIZE + WRITE_SIZE per dispatch, copy-calibrated)."""
+    name = f"k_substep_{case.name}_{case.nx}x{case.ny}x{case.nz}_hbm_traffic.txt"
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None, None
+    vals = {}
+    with open(path) as fh:
+        for line in fh:
+            if "=" in line and not line.lstrip().startswith("#"):
+                k, v = line.split("=", 1)
+                try:
+                    vals[k.strip()] = float(v.split()[0])
+                except (ValueError, IndexError):
+                    pass
+    if "traffic_bytes_per_launch" not in vals:
+        return None, None
+    return vals["traffic_bytes_per_launch"], "profiles/" + name
+
+
+def copy_calibration(nbytes: int = 1 << 30, reps: int = 3):
+    """A plain device copy of nbytes (torch's copy kernel), for scaling the
+    FETCH_SIZE / WRITE_SIZE counters: run under rocprofv3 --pmc."""
+    import torch
+
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").uniform_()
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps, "ok": bool(torch.equal(a, b))}))
 
 
 def run_gpu(case, precision, steps, warmup):
@@ -88,7 +136,12 @@ def main():
     ap.add_argument("--nx", type=int, default=0)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
@@ -120,14 +173,16 @@ def main():
                  "ms_stage_per_launch_group": kt_d["ms_stage"] / max(kt_d["n_stage"], 1),
                  "ms_substep_per_launch_group": kt_d["ms_substep"] / max(kt_d["n_substep"], 1)}
 
-    # dominant kernel group: acoustic substep (10 field accesses per cell)
+    # dominant kernel: the fused acoustic substep (k_substep), FP32-mixed
     sub_ms = kt_m["ms_substep"] / max(kt_m["n_substep"], 1)
-    sub_bytes = 10 * 4 * cells
+    sub_bytes = substep_bytes_per_launch(cells, case.nsub)
     achieved = sub_bytes / (sub_ms * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic(case)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved * 1e9 / HBM_PEAK, "traffic": None,
-                "kernel": "acoustic substep (k_sub_h + k_sub_v), FP32", "algorithmic_bytes_per_launch": sub_bytes,
-                "avg_launch_ms": sub_ms}
+                "frac": achieved * 1e9 / HBM_PEAK, "traffic": traffic,
+                "kernel": "k_substep (fused HEVI acoustic substep, FP32)",
+                "algorithmic_bytes_per_launch": sub_bytes, "avg_launch_ms": sub_ms,
+                "traffic_source": traffic_src}
 
     cpu = None
     if not args.no_cpu:
@@ -155,6 +210,8 @@ def main():
         "bytes_per_cell_update": bytes_per_cell_update(case.nsub, True),
         "fp64": res64,
         "speedup_mixed_vs_fp64": (cups_m / res64["value"]) if res64 else None,
+        "kernel_ms": {"stage_group_per_stage": kt_m["ms_stage"] / max(kt_m["n_stage"], 1),
+                      "substep_per_launch": sub_ms},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "device_bytes": mem_m,

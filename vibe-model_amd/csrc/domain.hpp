@@ -1,0 +1,76 @@
+// domain.hpp -- host-side state of one domain (shared by the single-domain
+// and the batched paths).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "common.hpp"
+#include "geo.hpp"
+#include "vibe_mi/dycore.h"
+
+namespace vibe_mi {
+
+// FP64 prognostic state of one domain (device pointers).
+struct FieldSet {
+  double *u = nullptr, *v = nullptr, *w = nullptr, *rho = nullptr, *th = nullptr, *pi = nullptr;
+};
+template <typename T>
+struct AcousticSet {
+  T *u = nullptr, *v = nullptr, *w = nullptr, *rho = nullptr, *pi = nullptr;
+};
+
+
+}  // namespace vibe_mi
+
+struct vibe_mi_domain {
+  using FieldSet = vibe_mi::FieldSet;
+  template <typename T>
+  using AcousticSet = vibe_mi::AcousticSet<T>;
+  int nx = 0, ny = 0, nz = 0;
+  size_t nxy = 0, ncell = 0, nwf = 0;
+  double dx = 0, dy = 0, ztop = 0, eps_s = 0;
+  int precision = VIBE_MI_FP64;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<void *> allocs;
+  size_t bytes = 0;
+  // FP64 geometry / reference for diagnostics and the FP64 path
+  vibe_mi::Geo<double> g64{};
+  vibe_mi::Geo<float> g32{};
+  double *zs = nullptr, *sponge = nullptr, *rho0 = nullptr, *theta0 = nullptr, *pi0 = nullptr;
+  double *dzeta64 = nullptr, *zc64 = nullptr;  // diagnostics (FP64, per level)
+  std::vector<double> h_sponge;
+  int kmin_sponge = 0;
+  FieldSet S[3];  // [0] = s0 of the step; [1], [2] stage outputs
+  int cur0 = 0;   // index of s^n
+  AcousticSet<double> A64[2];
+  AcousticSet<float> A32[2];
+  void *om = nullptr;   // Omega (T)
+  void *tab = nullptr;  // HEVI factorisation tables: 4 slots x {a, 1/pivot, c'} x (nz-1) x rk (T)
+  double tab_dtau[4] = {-1.0, -1.0, -1.0, -1.0};
+  int nxr = 1, nyr = 1, rk = 1;
+  int nw_fp32 = 4, nw_fp64 = 8;  // waves per substep workgroup (autotuned)
+  bool tuned[2] = {false, false};
+  int *flag = nullptr;
+  vibe_mi::NSum *part = nullptr;
+  double *dres = nullptr;
+  unsigned long long *dmax = nullptr;
+  // timing
+  std::vector<hipEvent_t> ev;
+  double ms_stage = 0, ms_sub = 0;
+  int n_stage = 0, n_sub = 0;
+};
+
+
+namespace vibe_mi {
+
+// Helpers of the single-domain path used by the batched path (dycore.hip).
+int domain_create(const vibe_mi_domain_desc *ds, vibe_mi_domain **out);
+// Factorise the HEVI matrix of table slot `slot` for dtau (cached per slot).
+int domain_ensure_table(vibe_mi_domain *d, int slot, double dtau);
+// Base of table slot `slot` ({beta, 1/pivot, c'} x (nz-1) x rk of type T).
+void *domain_table(vibe_mi_domain *d, int slot);
+
+}  // namespace vibe_mi

@@ -1,0 +1,826 @@
+// kernels.hpp -- device code of the dynamical-core step on gfx950 (shared by
+// the single-domain and the batched host paths).
+//
+// One WS-RK3 step (src/time/runge_kutta.cpp:77-118 of the reference) is:
+//   for each stage h in {dt/3, dt/2, dt}:
+//     k_omega      contravariant vertical velocity of the current state
+//     k_stage_col  slow tendencies R (equations.cpp:60-150 role, balanced form)
+//                  fused with the RK3 update s = s0 + h R (FP64 accumulation),
+//                  one thread per column marching up the levels
+//     nsub x k_substep
+//                  forward-backward acoustic substep: horizontal momentum
+//                  explicit, w / pi' / rho' vertically implicit (HEVI), the
+//                  Thomas sweep of each column on one lane
+//   k_sponge       backward-Euler Rayleigh sponge (damping.cpp:76-138)
+// Each kernel is a body function over one tile (a column, a row of 64
+// columns) plus a thin __global__ wrapper: the single-domain wrappers map
+// blockIdx to a tile, the batched wrappers (batch.hip) take tiles of many
+// domains from a device work list.
+//
+// Discrete equations (identical to the FP64 CPU oracle term by term):
+//   geometry  D = H - zs, Du/Dv at faces, slopes sx/sy at faces,
+//             normalised zeta levels, dzeta = layer thickness, dzw = centre
+//             spacing; Gal-Chen metric: z_x|zeta = (1 - zeta) * sx.
+//   Omega     w - (1 - zeta_k) (Mc(k-1) + Mc(k))/2 at interior interfaces,
+//             Mc = cell-centred u.grad(zs); Omega = 0 at ground and lid.
+//   u, v, w   flux-form advective 5th-order upwind transport (D-weighted
+//             mass fluxes), - cp theta' grad(pi'), + g theta'/theta0 on w.
+//   rho'      - (1/D) div(rho' V) (second-order centred faces).
+//   theta'    upwind WENO5 flux form (one reconstruction per face, picked by
+//             the sign of the face mass flux) minus theta' div V,
+//             - w dtheta0/dz.
+//   pi'       kappa R_rho + lambda R_theta (EOS linearised about the
+//             reference state: kappa = (Rd/cv) pi0/rho0, lambda = (Rd/cv)
+//             pi0/theta0).
+//   substep   u -= dtau cp theta0_u PX(pi'), v likewise; then per column
+//             rho' += -(dtau/D)[div_h(D rho0 V) + d_zeta(rho0 Omega~)],
+//             pi' += kappa * delta rho', w with off-centred implicit
+//             vertical pressure gradient; Omega~ = an w_new + ao w_old - M.
+//
+// Precision: VIBE_MI_FP64 runs every kernel in double.  VIBE_MI_MIXED keeps
+// the FP64 state (s0, stage outputs) and does the RK3 accumulation
+// s0 + h*R in FP64, computes tendencies and substeps in FP32 on the vector
+// ALUs, and sums mass/energy with FP64 compensated reductions.
+//
+// Compiled with -ffp-contract=off so the FP64 path rounds like the oracle.
+#pragma once
+
+#include "common.hpp"
+#include "geo.hpp"
+#include "stencils.hpp"
+
+namespace vibe_mi {
+
+// ---- kinematics -------------------------------------------------------------
+
+template <typename T, typename S>
+__device__ __forceinline__ T mc_at(const Geo<T> &g, const Acc &A, const S *__restrict__ u, const S *__restrict__ v,
+                                   int i, int j, int k) {
+  return T(0.5) * (T(u[A.at(i, j, k)]) * g.sx[gcol(g, i, j)] + T(u[A.at(i + 1, j, k)]) * g.sx[gcol(g, i + 1, j)]) +
+         T(0.5) * (T(v[A.at(i, j, k)]) * g.sy[gcol(g, i, j)] + T(v[A.at(i, j + 1, k)]) * g.sy[gcol(g, i, j + 1)]);
+}
+
+template <typename T, typename S>
+__global__ void __launch_bounds__(256) k_omega(Geo<T> g, const S *__restrict__ u, const S *__restrict__ v,
+                                               const S *__restrict__ w, T *__restrict__ om) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+  if (i >= g.nx) return;
+  const Acc A{g.nx, g.ny, static_cast<size_t>(g.nx) * g.ny};
+  T o = T(0);
+  if (k > 0 && k < g.nz)
+    o = T(w[A.at(i, j, k)]) -
+        (T(1) - g.zeta[k]) * (T(0.5) * (mc_at<T, S>(g, A, u, v, i, j, k - 1) + mc_at<T, S>(g, A, u, v, i, j, k)));
+  om[A.at(i, j, k)] = o;
+}
+
+// ---- pressure-gradient operators (Gal-Chen, along constant z) --------------
+
+template <typename T, typename S>
+__device__ __forceinline__ T gz(const Geo<T> &g, const Acc &A, const S *__restrict__ pi, int i, int j, int k) {
+  if (k <= 0 || k >= g.nz) return T(0);
+  return (T(pi[A.at(i, j, k)]) - T(pi[A.at(i, j, k - 1)])) * g.rdzw[k];
+}
+template <typename T, typename S>
+__device__ __forceinline__ T gzc(const Geo<T> &g, const Acc &A, const S *__restrict__ pi, int i, int j, int k) {
+  return T(0.5) * (gz<T, S>(g, A, pi, i, j, k) + gz<T, S>(g, A, pi, i, j, k + 1));
+}
+template <typename T, typename S>
+__device__ __forceinline__ T px_op(const Geo<T> &g, const Acc &A, const S *__restrict__ pi, int i, int j, int k) {
+  const T dpx = (T(pi[A.at(i, j, k)]) - T(pi[A.at(i - 1, j, k)])) * g.rdx;
+  const T met = (T(1) - g.zc[k]) * g.metx[gcol(g, i, j)];
+  return dpx - met * (T(0.5) * (gzc<T, S>(g, A, pi, i - 1, j, k) + gzc<T, S>(g, A, pi, i, j, k)));
+}
+template <typename T, typename S>
+__device__ __forceinline__ T py_op(const Geo<T> &g, const Acc &A, const S *__restrict__ pi, int i, int j, int k) {
+  const T dpy = (T(pi[A.at(i, j, k)]) - T(pi[A.at(i, j - 1, k)])) * g.rdy;
+  const T met = (T(1) - g.zc[k]) * g.mety[gcol(g, i, j)];
+  return dpy - met * (T(0.5) * (gzc<T, S>(g, A, pi, i, j - 1, k) + gzc<T, S>(g, A, pi, i, j, k)));
+}
+
+// ---- fused z-marching stage kernel -------------------------------------------
+// One thread per column (i, j), marching up the levels.  Every vertical stencil
+// (5th-order momentum, WENO5 theta', pressure gradient, Omega averages) reads
+// a register window of the thread's own column; horizontal neighbours are read
+// at the current level only, so a workgroup's 64 x 4 tile of columns shares
+// them through L1/L2 while it marches.  Writes u, v, w, rho', pi' (type O) and
+// theta' (FP64) of s = s0 + h R in one pass.  Each interface flux is evaluated
+// once: the flux through the top of level k is carried to level k+1 as its
+// bottom flux (the same expression on the same operands).
+struct StageIn {
+  const double *u, *v, *w, *rho, *th, *pi;
+};
+template <typename O>
+struct StageOut {
+  O *u, *v, *w, *rho, *pi;
+  double *th;
+};
+
+template <typename T, typename O>
+__global__ void __launch_bounds__(256) k_stage_col(Geo<T> g, StageIn cur, StageIn s0, const T *__restrict__ om,
+                                                   double h, StageOut<O> out, int *__restrict__ flag) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  if (i >= g.nx || j >= g.ny) return;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, rk = g.rk;
+  const size_t nxy = static_cast<size_t>(nx) * ny;
+  const double *__restrict__ U = cur.u;
+  const double *__restrict__ V = cur.v;
+  const double *__restrict__ W = cur.w;
+  const double *__restrict__ RH = cur.rho;
+  const double *__restrict__ TH = cur.th;
+  const double *__restrict__ PI = cur.pi;
+  uint32_t xi[7], yj[7];
+#pragma unroll
+  for (int d = 0; d < 7; ++d) {
+    xi[d] = static_cast<uint32_t>(wrapi(i + d - 3, nx));
+    yj[d] = static_cast<uint32_t>(wrapi(j + d - 3, ny)) * static_cast<uint32_t>(nx);
+  }
+  const uint32_t cC = yj[3] + xi[3], cW = yj[3] + xi[2], cS = yj[2] + xi[3];
+  const uint32_t cE = yj[3] + xi[4], cN = yj[4] + xi[3];
+  const uint32_t cWN = yj[4] + xi[2], cES = yj[2] + xi[4];
+  // column geometry (compacted)
+  const int gC = gcol(g, i, j), gW = gcol(g, i - 1, j), gE = gcol(g, i + 1, j), gS = gcol(g, i, j - 1),
+            gN = gcol(g, i, j + 1), gWN = gcol(g, i - 1, j + 1), gES = gcol(g, i + 1, j - 1);
+  const T DuW = g.Du[gW], DuC = g.Du[gC], DuE = g.Du[gE], DuES = g.Du[gES];
+  const T DvS = g.Dv[gS], DvC = g.Dv[gC], DvN = g.Dv[gN], DvWN = g.Dv[gWN], DvW = g.Dv[gW];
+  const T rDuC = g.rDu[gC], rDvC = g.rDv[gC], rD = g.rDc[gC], mxC = g.metx[gC], myC = g.mety[gC];
+
+  auto ldd = [&](const double *f, int k, uint32_t c) { return T(ld32(f, static_cast<size_t>(k) * nxy, c)); };
+  auto ldo = [&](int k, uint32_t c) { return ld32(om, static_cast<size_t>(k) * nxy, c); };
+  T uw[7], vw[7], ww[7], tw[7], rw[3], pw[3], pWw[3], pSw[3], ow[3], oW[2], oS[2];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    const int kc = clampk(q - 3, nz - 1);
+    uw[q] = ldd(U, kc, cC);
+    vw[q] = ldd(V, kc, cC);
+    tw[q] = ldd(TH, kc, cC);
+    ww[q] = ldd(W, clampk(q - 3, nz), cC);
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int kc = clampk(q - 1, nz - 1);
+    rw[q] = ldd(RH, kc, cC);
+    pw[q] = ldd(PI, kc, cC);
+    pWw[q] = ldd(PI, kc, cW);
+    pSw[q] = ldd(PI, kc, cS);
+    ow[q] = ldo(clampk(q - 1, nz), cC);
+  }
+  oW[0] = ldo(0, cW);
+  oW[1] = ldo(1, cW);
+  oS[0] = ldo(0, cS);
+  oS[1] = ldo(1, cS);
+  T uE_prev = T(0), vN_prev = T(0);
+  // z-face fluxes of the level above, reused as the next level's lower face
+  T cu_O = T(0), cu_f = T(0), cv_O = T(0), cv_f = T(0), cw_O = T(0), cw_f = T(0);
+  T cm_z = T(0), cr_z = T(0), ct_z = T(0);
+  int bad = 0;
+  st32(out.w, 0, cC, static_cast<O>(ld32(s0.w, 0, cC)));  // interface 0 keeps its value (R_w = 0)
+
+  for (int k = 0; k < nz; ++k) {
+    const size_t kb = static_cast<size_t>(k) * nxy;
+    const int rb = k * rk;
+    // horizontal neighbours at level k
+    T ux[7], uy[7], vx[7], vy[7], tx[7], ty[7], wx[7], wy[7];
+#pragma unroll
+    for (int d = 0; d < 7; ++d) {
+      if (d == 3) continue;
+      ux[d] = T(ld32(U, kb, yj[3] + xi[d]));
+      uy[d] = T(ld32(U, kb, yj[d] + xi[3]));
+      vx[d] = T(ld32(V, kb, yj[3] + xi[d]));
+      vy[d] = T(ld32(V, kb, yj[d] + xi[3]));
+      tx[d] = T(ld32(TH, kb, yj[3] + xi[d]));
+      ty[d] = T(ld32(TH, kb, yj[d] + xi[3]));
+      wx[d] = T(ld32(W, kb, yj[3] + xi[d]));
+      wy[d] = T(ld32(W, kb, yj[d] + xi[3]));
+    }
+    ux[3] = uy[3] = uw[3];
+    vx[3] = vy[3] = vw[3];
+    tx[3] = ty[3] = tw[3];
+    wx[3] = wy[3] = ww[3];
+    const T vWN = T(ld32(V, kb, cWN)), uES = T(ld32(U, kb, cES));
+    const T rWst = T(ld32(RH, kb, cW)), rE = T(ld32(RH, kb, cE)), rSth = T(ld32(RH, kb, cS)), rN = T(ld32(RH, kb, cN));
+    const T omz = T(1) - g.zc[k];
+    const T rdzk = g.rdzeta[k];
+    // vertical pi' gradients (0 at ground and lid)
+    auto gpair = [&](const T *p) {
+      const T glo = (k >= 1) ? (p[1] - p[0]) * g.rdzw[k] : T(0);
+      const T ghi = (k + 1 <= nz - 1) ? (p[2] - p[1]) * g.rdzw[k + 1] : T(0);
+      return T(0.5) * (glo + ghi);
+    };
+    const T GcC = gpair(pw), GcW = gpair(pWw), GcS = gpair(pSw);
+
+    // ---------------- u at the west face ----------------
+    {
+      const T U0 = T(0.5) * (DuW * ux[2] + DuC * ux[3]);
+      const T U1 = T(0.5) * (DuC * ux[3] + DuE * ux[4]);
+      const T fx0 = up5<T>(U0, ux[0], ux[1], ux[2], ux[3], ux[4], ux[5]);
+      const T fx1 = up5<T>(U1, ux[1], ux[2], ux[3], ux[4], ux[5], ux[6]);
+      const T V0 = T(0.5) * (DvW * vx[2] + DvC * vw[3]);
+      const T V1 = T(0.5) * (DvWN * vWN + DvN * vy[4]);
+      const T fy0 = up5<T>(V0, uy[0], uy[1], uy[2], uy[3], uy[4], uy[5]);
+      const T fy1 = up5<T>(V1, uy[1], uy[2], uy[3], uy[4], uy[5], uy[6]);
+      const T O0 = cu_O, fz0 = cu_f;  // interface k (0 at the ground)
+      T O1 = T(0), fz1 = T(0);
+      if (k + 1 < nz) {
+        O1 = T(0.5) * (oW[1] + ow[2]);
+        fz1 = up5<T>(O1, uw[1], uw[2], uw[3], uw[4], uw[5], uw[6]);
+      }
+      cu_O = O1;
+      cu_f = fz1;
+      const T adv = -(fx1 - fx0) * g.rdx - (fy1 - fy0) * g.rdy - (fz1 - fz0) * rdzk +
+                    uw[3] * ((U1 - U0) * g.rdx + (V1 - V0) * g.rdy + (O1 - O0) * rdzk);
+      const T thu = T(0.5) * (tx[2] + tw[3]);
+      const T px = (pw[1] - pWw[1]) * g.rdx - (omz * mxC) * (T(0.5) * (GcW + GcC));
+      const T R = adv * rDuC - T(kCp) * thu * px;
+      st32(out.u, kb, cC, static_cast<O>(ld32(s0.u, kb, cC) + h * static_cast<double>(R)));
+    }
+    // ---------------- v at the south face ----------------
+    {
+      const T U0 = T(0.5) * (g.Du[gcol(g, i, j - 1)] * uy[2] + DuC * uw[3]);
+      const T U1 = T(0.5) * (DuES * uES + DuE * ux[4]);
+      const T fx0 = up5<T>(U0, vx[0], vx[1], vx[2], vx[3], vx[4], vx[5]);
+      const T fx1 = up5<T>(U1, vx[1], vx[2], vx[3], vx[4], vx[5], vx[6]);
+      const T V0 = T(0.5) * (DvS * vy[2] + DvC * vy[3]);
+      const T V1 = T(0.5) * (DvC * vy[3] + DvN * vy[4]);
+      const T fy0 = up5<T>(V0, vy[0], vy[1], vy[2], vy[3], vy[4], vy[5]);
+      const T fy1 = up5<T>(V1, vy[1], vy[2], vy[3], vy[4], vy[5], vy[6]);
+      const T O0 = cv_O, fz0 = cv_f;
+      T O1 = T(0), fz1 = T(0);
+      if (k + 1 < nz) {
+        O1 = T(0.5) * (oS[1] + ow[2]);
+        fz1 = up5<T>(O1, vw[1], vw[2], vw[3], vw[4], vw[5], vw[6]);
+      }
+      cv_O = O1;
+      cv_f = fz1;
+      const T adv = -(fx1 - fx0) * g.rdx - (fy1 - fy0) * g.rdy - (fz1 - fz0) * rdzk +
+                    vw[3] * ((U1 - U0) * g.rdx + (V1 - V0) * g.rdy + (O1 - O0) * rdzk);
+      const T thv = T(0.5) * (ty[2] + tw[3]);
+      const T py = (pw[1] - pSw[1]) * g.rdy - (omz * myC) * (T(0.5) * (GcS + GcC));
+      const T R = adv * rDvC - T(kCp) * thv * py;
+      st32(out.v, kb, cC, static_cast<O>(ld32(s0.v, kb, cC) + h * static_cast<double>(R)));
+    }
+    // ---------------- w at interface k (1..nz-1) ----------------
+    // flux of w through the cell centre below / above interface k
+    const T Oc1 = T(0.5) * (ow[1] + ow[2]);
+    const T fzc1 = up5<T>(Oc1, ww[1], ww[2], ww[3], ww[4], ww[5], ww[6]);
+    if (k >= 1) {
+      const T U0 = T(0.5) * (DuC * uw[2] + DuC * uw[3]);
+      const T U1 = T(0.5) * (DuE * uE_prev + DuE * ux[4]);
+      const T fx0 = up5<T>(U0, wx[0], wx[1], wx[2], wx[3], wx[4], wx[5]);
+      const T fx1 = up5<T>(U1, wx[1], wx[2], wx[3], wx[4], wx[5], wx[6]);
+      const T V0 = T(0.5) * (DvC * vw[2] + DvC * vw[3]);
+      const T V1 = T(0.5) * (DvN * vN_prev + DvN * vy[4]);
+      const T fy0 = up5<T>(V0, wy[0], wy[1], wy[2], wy[3], wy[4], wy[5]);
+      const T fy1 = up5<T>(V1, wy[1], wy[2], wy[3], wy[4], wy[5], wy[6]);
+      const T Oc0 = cw_O, fz0 = cw_f;
+      const T fz1 = fzc1;
+      const T rz = g.rdzw[k];
+      const T adv = -(fx1 - fx0) * g.rdx - (fy1 - fy0) * g.rdy - (fz1 - fz0) * rz +
+                    ww[3] * ((U1 - U0) * g.rdx + (V1 - V0) * g.rdy + (Oc1 - Oc0) * rz);
+      const T thw = T(0.5) * (tw[2] + tw[3]);
+      const T th0w = T(0.5) * (g.theta0[rb - rk + gC] + g.theta0[rb + gC]);
+      const T pz = (pw[1] - pw[0]) * rz * rD;
+      const T R = adv * rD - T(kCp) * thw * pz + T(kG) * thw / th0w;
+      st32(out.w, kb, cC, static_cast<O>(ld32(s0.w, kb, cC) + h * static_cast<double>(R)));
+    }
+    cw_O = Oc1;
+    cw_f = fzc1;
+    // ---------------- cell: rho', theta', pi' ----------------
+    {
+      const T mfx0 = DuC * uw[3], mfx1 = DuE * ux[4];
+      const T frx0 = DuC * (T(0.5) * (rWst + rw[1])) * uw[3];
+      const T frx1 = DuE * (T(0.5) * (rw[1] + rE)) * ux[4];
+      const T ftx0 = DuC * upwind_weno<T>(uw[3], tx[0], tx[1], tx[2], tx[3], tx[4], tx[5]);
+      const T ftx1 = DuE * upwind_weno<T>(ux[4], tx[1], tx[2], tx[3], tx[4], tx[5], tx[6]);
+      const T mfy0 = DvC * vw[3], mfy1 = DvN * vy[4];
+      const T fry0 = DvC * (T(0.5) * (rSth + rw[1])) * vw[3];
+      const T fry1 = DvN * (T(0.5) * (rw[1] + rN)) * vy[4];
+      const T fty0 = DvC * upwind_weno<T>(vw[3], ty[0], ty[1], ty[2], ty[3], ty[4], ty[5]);
+      const T fty1 = DvN * upwind_weno<T>(vy[4], ty[1], ty[2], ty[3], ty[4], ty[5], ty[6]);
+      const T mfz0 = cm_z, frz0 = cr_z, ftz0 = ct_z;
+      T mfz1 = T(0), frz1 = T(0), ftz1 = T(0);
+      if (k + 1 < nz) {
+        mfz1 = ow[2];
+        frz1 = (T(0.5) * (rw[1] + rw[2])) * ow[2];
+        ftz1 = upwind_weno<T>(ow[2], tw[1], tw[2], tw[3], tw[4], tw[5], tw[6]);
+      }
+      cm_z = mfz1;
+      cr_z = frz1;
+      ct_z = ftz1;
+      const T rr = -((frx1 - frx0) * g.rdx + (fry1 - fry0) * g.rdy + (frz1 - frz0) * rdzk) * rD;
+      const T div = (mfx1 - mfx0) * g.rdx + (mfy1 - mfy0) * g.rdy + (mfz1 - mfz0) * rdzk;
+      const T fd = (ftx1 - ftx0) * g.rdx + (fty1 - fty0) * g.rdy + (ftz1 - ftz0) * rdzk;
+      auto gam = [&](int kq) {
+        const int kk = kq < 1 ? 1 : (kq > nz - 1 ? nz - 1 : kq);
+        return (g.theta0[kk * rk + gC] - g.theta0[(kk - 1) * rk + gC]) * rD * g.rdzw[kk];
+      };
+      const T rt = -(fd - tw[3] * div) * rD - T(0.5) * (ww[3] * gam(k) + ww[4] * gam(k + 1));
+      const T rp = g.kap[rb + gC] * rr + g.lam[rb + gC] * rt;
+      const double rn = ld32(s0.rho, kb, cC) + h * static_cast<double>(rr);
+      const double tn = ld32(s0.th, kb, cC) + h * static_cast<double>(rt);
+      const double pn = ld32(s0.pi, kb, cC) + h * static_cast<double>(rp);
+      st32(out.rho, kb, cC, static_cast<O>(rn));
+      st32(out.th, kb, cC, tn);
+      st32(out.pi, kb, cC, static_cast<O>(pn));
+      bad |= !isfinite(tn);
+    }
+    // ---------------- advance the vertical windows ----------------
+    uE_prev = ux[4];
+    vN_prev = vy[4];
+    const int kn = clampk(k + 4, nz - 1), kw = clampk(k + 4, nz);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      uw[q] = uw[q + 1];
+      vw[q] = vw[q + 1];
+      tw[q] = tw[q + 1];
+      ww[q] = ww[q + 1];
+    }
+    uw[6] = ldd(U, kn, cC);
+    vw[6] = ldd(V, kn, cC);
+    tw[6] = ldd(TH, kn, cC);
+    ww[6] = ldd(W, kw, cC);
+    const int k2 = clampk(k + 2, nz - 1);
+    rw[0] = rw[1];
+    rw[1] = rw[2];
+    rw[2] = ldd(RH, k2, cC);
+    pw[0] = pw[1];
+    pw[1] = pw[2];
+    pw[2] = ldd(PI, k2, cC);
+    pWw[0] = pWw[1];
+    pWw[1] = pWw[2];
+    pWw[2] = ldd(PI, k2, cW);
+    pSw[0] = pSw[1];
+    pSw[1] = pSw[2];
+    pSw[2] = ldd(PI, k2, cS);
+    const int ko = clampk(k + 2, nz);
+    ow[0] = ow[1];
+    ow[1] = ow[2];
+    ow[2] = ldo(ko, cC);
+    oW[0] = oW[1];
+    oW[1] = ldo(ko, cW);
+    oS[0] = oS[1];
+    oS[1] = ldo(ko, cS);
+  }
+  st32(out.w, static_cast<size_t>(nz) * nxy, cC, static_cast<O>(ld32(s0.w, static_cast<size_t>(nz) * nxy, cC)));
+  if (bad) atomicOr(flag, 1);
+}
+
+// ---- HEVI factorisation table ----------------------------------------------
+// The tridiagonal matrix of the vertically implicit w system depends only on
+// the reference state, the column depth and dtau, so it is factorised once
+// per (stage, column): beta_m = a_m/pivot_m, 1/pivot_m and c'_m, stored
+// [m][compact column].  One thread per compact column.
+template <typename T>
+__global__ void __launch_bounds__(64) k_hevi_factor(Geo<T> g, T dtau, T an, T *__restrict__ tA, T *__restrict__ tINV,
+                                                    T *__restrict__ tCP) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.rk) return;
+  const int nz = g.nz, n = nz - 1, rk = g.rk;
+  const T dtD = dtau * g.rDc[c];
+  const T an2 = an * an;
+  auto ck = [&](int k) { return g.kap[k * rk + c] * dtD * g.rdzeta[k]; };
+  auto r0w = [&](int k) { return T(0.5) * (g.rho0[(k - 1) * rk + c] + g.rho0[k * rk + c]); };
+  T cp_prev = T(0);
+  for (int m = 0; m < n; ++m) {
+    const int k = m + 1;
+    const T th0w = T(0.5) * (g.theta0[(k - 1) * rk + c] + g.theta0[k * rk + c]);
+    const T gk = dtD * T(kCp) * th0w * g.rdzw[k];
+    const T ga = gk * an2;
+    const T ckk = ck(k), ckm = ck(k - 1);
+    const T a = (m > 0) ? -ga * (ckm * r0w(k - 1)) : T(0);
+    const T b = T(1) + ga * ((ckk + ckm) * r0w(k));
+    const T cc = (m < n - 1) ? -ga * (ckk * r0w(k + 1)) : T(0);
+    const T den = (m == 0) ? b : b - a * cp_prev;
+    const T inv = T(1) / den;
+    const T cp = cc * inv;
+    tA[m * rk + c] = a * inv;
+    tINV[m * rk + c] = inv;
+    tCP[m * rk + c] = cp;
+    cp_prev = cp;
+  }
+}
+
+// ---- fused acoustic substep -------------------------------------------------
+// One workgroup = 64 consecutive columns (one per lane) x NW waves; wave wv
+// owns the levels [wv*ch, wv*ch+ch).  Phase A (all waves, their levels):
+// horizontal momentum forward step with the old pi', slope flux Mc, rho0 mass
+// divergence, explicit rho'/pi' parts E, Q and the old-w part of the w RHS.
+// Phases B: vertical fluxes with the old w and the explicit pi' estimate,
+// exchanging the chunk-boundary values through LDS.  Phase C (wave 0): the
+// serial Thomas sweeps of the 64 columns over the RHS held in LDS, with the
+// stage's precomputed factorisation.  Phase D (all waves): vertical flux
+// divergence closes rho' and pi', and w is written.  State traffic per cell:
+// read u, v, w, rho', pi' once, write them once.
+template <typename T, typename I, typename O, int NW, int MAXCH>
+__global__ void __launch_bounds__(64 * NW) k_substep(Geo<T> g, const I *__restrict__ u, const I *__restrict__ v,
+                                                     const I *__restrict__ w, const I *__restrict__ rho,
+                                                     const I *__restrict__ pi, const T *__restrict__ tA,
+                                                     const T *__restrict__ tINV, const T *__restrict__ tCP, T dtau,
+                                                     T an, T ao, O *__restrict__ uo, O *__restrict__ vo,
+                                                     O *__restrict__ wo, O *__restrict__ rho_o, O *__restrict__ pi_o,
+                                                     int check, int *__restrict__ flag) {
+  extern __shared__ unsigned char smem_raw[];
+  const int nlds = (g.nz - 1) * 64;
+  T *s_td = reinterpret_cast<T *>(smem_raw);  // [nz-1][64] RHS -> d' -> w
+  T *s_be = s_td + nlds;                      // [nz-1][64] beta
+  T *s_cp = s_be + nlds;                      // [nz-1][64] c'
+  __shared__ T s_mc[NW][64], s_fh[NW][64], s_ph[NW][64];
+  const int lane = threadIdx.x & 63;
+  // wave index: uniform within the wavefront, so the level bases below live in
+  // scalar registers and loads use the scalar-base + 32-bit offset form
+  const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const int nx = g.nx, ny = g.ny, nz = g.nz, rk = g.rk;
+  const size_t nxy = static_cast<size_t>(nx) * ny;
+  const int i_raw = blockIdx.x * 64 + lane;
+  const bool active = i_raw < nx;
+  const int i = active ? i_raw : nx - 1;
+  const int j = blockIdx.y;
+  const int ch = (nz + NW - 1) / NW;
+  const int k0 = wv * ch;
+  const int nlev = max(0, min(nz, k0 + ch) - k0);
+  const int iW = wrapi(i - 1, nx), iE = wrapi(i + 1, nx), jS = wrapi(j - 1, ny), jN = wrapi(j + 1, ny);
+  const uint32_t unx = static_cast<uint32_t>(nx);
+  const uint32_t cC = static_cast<uint32_t>(j) * unx + i, cW = static_cast<uint32_t>(j) * unx + iW,
+                 cE = static_cast<uint32_t>(j) * unx + iE, cS = static_cast<uint32_t>(jS) * unx + i,
+                 cN = static_cast<uint32_t>(jN) * unx + i;
+  const uint32_t gC = gcol(g, i, j), gW = gcol(g, iW, j), gE = gcol(g, iE, j), gS = gcol(g, i, jS), gN = gcol(g, i, jN);
+  const T rD = g.rDc[gC];
+  const T dtD = dtau * rD;
+  const T Du_w = g.Du[gC], Du_e = g.Du[gE], Dv_s = g.Dv[gC], Dv_n = g.Dv[gN];
+  const T sx_w = g.sx[gC], sx_e = g.sx[gE], sy_s = g.sy[gC], sy_n = g.sy[gN];
+  const T mx_w = g.metx[gC], mx_e = g.metx[gE], my_s = g.mety[gC], my_n = g.mety[gN];
+  const T cpdt = dtau;
+
+  T E[MAXCH], Q[MAXCH], Wt[MAXCH], Fh[MAXCH], Ph[MAXCH];
+  T mc_first = T(0), mc_last = T(0), mc0 = T(0), aw_first = T(0), r0w_first = T(0);
+
+  // ---------------- phase A ----------------
+  if (nlev > 0) {
+    auto ld5 = [&](int k, T *p) {
+      const size_t b = static_cast<size_t>(k) * nxy;
+      p[0] = T(ld32(pi, b, cC));
+      p[1] = T(ld32(pi, b, cW));
+      p[2] = T(ld32(pi, b, cE));
+      p[3] = T(ld32(pi, b, cS));
+      p[4] = T(ld32(pi, b, cN));
+    };
+    // pi' of the five columns at the current and next level; the vertical
+    // gradient of the interface below is carried from the previous level.
+    T p0[5], pp[5], gl[5];
+    ld5(k0, p0);
+    T pmC = p0[0];
+    if (k0 >= 1) {
+      T pm[5];
+      ld5(k0 - 1, pm);
+      pmC = pm[0];
+#pragma unroll
+      for (int q = 0; q < 5; ++q) gl[q] = (p0[q] - pm[q]) * g.rdzw[k0];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) gl[q] = T(0);
+    }
+    T t0C_prev = (k0 >= 1) ? ld32(g.theta0, static_cast<size_t>(k0 - 1) * rk, gC) : T(0);
+    T r0C_prev = (k0 >= 1) ? ld32(g.rho0, static_cast<size_t>(k0 - 1) * rk, gC) : T(0);
+    T mc_prev = T(0);
+#pragma unroll
+    for (int c = 0; c < MAXCH; ++c) {
+      if (c < nlev) {
+        const int k = k0 + c;
+        const size_t b = static_cast<size_t>(k) * nxy;
+        if (k + 1 < nz) ld5(k + 1, pp); else for (int q = 0; q < 5; ++q) pp[q] = p0[q];
+        const T rz_hi = g.rdzw[k + 1];
+        T Gc[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          const T ghi = (k + 1 <= nz - 1) ? (pp[q] - p0[q]) * rz_hi : T(0);
+          Gc[q] = T(0.5) * (gl[q] + ghi);
+          gl[q] = ghi;
+        }
+        const T omz = T(1) - g.zc[k];
+        const T px_w = (p0[0] - p0[1]) * g.rdx - (omz * mx_w) * (T(0.5) * (Gc[1] + Gc[0]));
+        const T px_e = (p0[2] - p0[0]) * g.rdx - (omz * mx_e) * (T(0.5) * (Gc[0] + Gc[2]));
+        const T py_s = (p0[0] - p0[3]) * g.rdy - (omz * my_s) * (T(0.5) * (Gc[3] + Gc[0]));
+        const T py_n = (p0[4] - p0[0]) * g.rdy - (omz * my_n) * (T(0.5) * (Gc[0] + Gc[4]));
+        const int rb = k * rk;
+        const T t0C = ld32(g.theta0, static_cast<size_t>(rb), gC), t0W = ld32(g.theta0, static_cast<size_t>(rb), gW), t0E = ld32(g.theta0, static_cast<size_t>(rb), gE), t0S = ld32(g.theta0, static_cast<size_t>(rb), gS),
+                t0N = ld32(g.theta0, static_cast<size_t>(rb), gN);
+        const T un_w = T(ld32(u, b, cC)) - cpdt * (T(kCp) * (T(0.5) * (t0W + t0C)) * px_w);
+        const T un_e = T(ld32(u, b, cE)) - cpdt * (T(kCp) * (T(0.5) * (t0C + t0E)) * px_e);
+        const T vn_s = T(ld32(v, b, cC)) - cpdt * (T(kCp) * (T(0.5) * (t0S + t0C)) * py_s);
+        const T vn_n = T(ld32(v, b, cN)) - cpdt * (T(kCp) * (T(0.5) * (t0C + t0N)) * py_n);
+        if (active) {
+          st32(uo, b, cC, static_cast<O>(un_w));
+          st32(vo, b, cC, static_cast<O>(vn_s));
+        }
+        const T mc = T(0.5) * (un_w * sx_w + un_e * sx_e) + T(0.5) * (vn_s * sy_s + vn_n * sy_n);
+        const T r0C = ld32(g.rho0, static_cast<size_t>(rb), gC), r0W = ld32(g.rho0, static_cast<size_t>(rb), gW), r0E = ld32(g.rho0, static_cast<size_t>(rb), gE), r0S = ld32(g.rho0, static_cast<size_t>(rb), gS),
+                r0N = ld32(g.rho0, static_cast<size_t>(rb), gN);
+        const T hd = (Du_e * (T(0.5) * (r0C + r0E)) * un_e - Du_w * (T(0.5) * (r0W + r0C)) * un_w) * g.rdx +
+                     (Dv_n * (T(0.5) * (r0C + r0N)) * vn_n - Dv_s * (T(0.5) * (r0S + r0C)) * vn_s) * g.rdy;
+        const T cq = ld32(g.kap, static_cast<size_t>(rb), gC) * dtD;
+        E[c] = T(ld32(rho, b, cC)) - dtD * hd;
+        Q[c] = p0[0] - cq * hd;
+        const T wk = T(ld32(w, b, cC));
+        Fh[c] = T(0);
+        Wt[c] = T(0);
+        if (k >= 1) {
+          const T th0w = T(0.5) * (t0C_prev + t0C);
+          const T gk = dtD * T(kCp) * th0w * g.rdzw[k];
+          Wt[c] = wk - gk * ao * (p0[0] - pmC);
+          const T r0w = T(0.5) * (r0C_prev + r0C);
+          if (c > 0) {
+            const T Mw = (T(1) - g.zeta[k]) * (T(0.5) * (mc_prev + mc));
+            Fh[c] = r0w * (ao * wk - Mw);
+          } else {
+            aw_first = ao * wk;
+            r0w_first = r0w;
+          }
+        }
+        if (c == 0) mc_first = mc;
+        if (k == 0) mc0 = mc;
+        mc_prev = mc;
+        t0C_prev = t0C;
+        r0C_prev = r0C;
+        pmC = p0[0];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) p0[q] = pp[q];
+      }
+    }
+    mc_last = mc_prev;
+    s_mc[wv][lane] = mc_last;
+  }
+  __syncthreads();
+  // ---------------- phase B: boundary vertical flux, P^, RHS ----------------
+  if (nlev > 0) {
+    if (k0 >= 1) {
+      const T Mw = (T(1) - g.zeta[k0]) * (T(0.5) * (s_mc[wv - 1][lane] + mc_first));
+      Fh[0] = r0w_first * (aw_first - Mw);
+    }
+    s_fh[wv][lane] = Fh[0];
+  }
+  __syncthreads();
+  if (nlev > 0) {
+    const T fh_next_chunk = (k0 + nlev < nz) ? s_fh[wv + 1][lane] : T(0);
+#pragma unroll
+    for (int c = 0; c < MAXCH; ++c) {
+      if (c < nlev) {
+        const int k = k0 + c;
+        const T fhn = (c + 1 < nlev) ? Fh[(c + 1) < MAXCH ? c + 1 : c] : fh_next_chunk;
+        const T ck = ld32(g.kap, static_cast<size_t>(k) * rk, gC) * dtD * g.rdzeta[k];
+        Ph[c] = Q[c] - ck * (fhn - Fh[c]);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < MAXCH; ++c)
+      if (c == nlev - 1) s_ph[wv][lane] = Ph[c];
+  }
+  __syncthreads();
+  if (nlev > 0) {
+    const T ph_prev_chunk = (k0 >= 1) ? s_ph[wv - 1][lane] : T(0);
+#pragma unroll
+    for (int c = 0; c < MAXCH; ++c) {
+      if (c < nlev) {
+        const int k = k0 + c;
+        if (k >= 1) {
+          const T php = (c > 0) ? Ph[c > 0 ? c - 1 : 0] : ph_prev_chunk;
+          const T th0w = T(0.5) * (ld32(g.theta0, static_cast<size_t>(k - 1) * rk, gC) + ld32(g.theta0, static_cast<size_t>(k) * rk, gC));
+          const T gk = dtD * T(kCp) * th0w * g.rdzw[k];
+          const int m = k - 1;
+          const T td = Wt[c] - gk * an * (Ph[c] - php);
+          s_td[m * 64 + lane] = td * ld32(tINV, static_cast<size_t>(m) * rk, gC);
+          s_be[m * 64 + lane] = ld32(tA, static_cast<size_t>(m) * rk, gC);
+          s_cp[m * 64 + lane] = ld32(tCP, static_cast<size_t>(m) * rk, gC);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // ---------------- phase C: Thomas sweeps (wave 0) ----------------
+  // RHS, beta and c' are in LDS (staged by all waves in phase B); operands
+  // are fetched one block of eight levels ahead so only the dependent
+  // multiply-subtract per level stays on the critical path.
+  if (wv == 0) {
+    constexpr int B = 8;
+    const int n = nz - 1;
+    const int nb = (n + B - 1) / B;
+    T ba[B], ta[B], bb[B], tb2[B];
+    auto fetch_fwd = [&](int blk, T *be, T *td) {
+#pragma unroll
+      for (int q = 0; q < B; ++q) {
+        const int m = blk * B + q;
+        const int mm = (m < n ? m : n - 1) * 64 + lane;
+        be[q] = s_be[mm];
+        td[q] = s_td[mm];
+      }
+    };
+    T dp = T(0);
+    fetch_fwd(0, ba, ta);
+    for (int blk = 0; blk < nb; ++blk) {
+      if (blk + 1 < nb) fetch_fwd(blk + 1, bb, tb2);
+#pragma unroll
+      for (int q = 0; q < B; ++q) {
+        const int m = blk * B + q;
+        if (m < n) {
+          dp = (m == 0) ? ta[q] : ta[q] - ba[q] * dp;
+          s_td[m * 64 + lane] = dp;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < B; ++q) {
+        ba[q] = bb[q];
+        ta[q] = tb2[q];
+      }
+    }
+    auto fetch_bwd = [&](int blk, T *cp, T *d) {
+#pragma unroll
+      for (int q = 0; q < B; ++q) {
+        const int m = n - 2 - (blk * B + q);
+        const int mm = (m >= 0 ? m : 0) * 64 + lane;
+        cp[q] = s_cp[mm];
+        d[q] = s_td[mm];
+      }
+    };
+    T x = dp;
+    const int nbb = (n - 1 + B - 1) / B;
+    if (nbb > 0) fetch_bwd(0, ba, ta);
+    for (int blk = 0; blk < nbb; ++blk) {
+      if (blk + 1 < nbb) fetch_bwd(blk + 1, bb, tb2);
+#pragma unroll
+      for (int q = 0; q < B; ++q) {
+        const int m = n - 2 - (blk * B + q);
+        if (m >= 0) {
+          x = ta[q] - ba[q] * x;
+          s_td[m * 64 + lane] = x;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < B; ++q) {
+        ba[q] = bb[q];
+        ta[q] = tb2[q];
+      }
+    }
+  }
+  __syncthreads();
+  // ---------------- phase D: close rho', pi', write w ----------------
+  if (nlev > 0) {
+    int bad = 0;
+    auto fluxF = [&](int k, T fh) {
+      if (k <= 0 || k >= nz) return T(0);
+      const T r0w = T(0.5) * (ld32(g.rho0, static_cast<size_t>(k - 1) * rk, gC) + ld32(g.rho0, static_cast<size_t>(k) * rk, gC));
+      return fh + (an * r0w) * s_td[(k - 1) * 64 + lane];
+    };
+    const T fh_next_chunk = (k0 + nlev < nz) ? s_fh[wv + 1][lane] : T(0);
+    T flo = fluxF(k0, Fh[0]);
+#pragma unroll
+    for (int c = 0; c < MAXCH; ++c) {
+      if (c < nlev) {
+        const int k = k0 + c;
+        const size_t b = static_cast<size_t>(k) * nxy;
+        const T fhn = (c + 1 < nlev) ? Fh[(c + 1) < MAXCH ? c + 1 : c] : fh_next_chunk;
+        const T fhi = fluxF(k + 1, fhn);
+        const T vd = -dtD * ((fhi - flo) * g.rdzeta[k]);
+        const T rn = E[c] + vd;
+        const T pn = Q[c] + ld32(g.kap, static_cast<size_t>(k) * rk, gC) * vd;
+        const T wn = (k >= 1) ? s_td[(k - 1) * 64 + lane] : mc0;
+        if (active) {
+          st32(rho_o, b, cC, static_cast<O>(rn));
+          st32(pi_o, b, cC, static_cast<O>(pn));
+          st32(wo, b, cC, static_cast<O>(wn));
+        }
+        bad |= !(isfinite(rn) && isfinite(pn) && isfinite(wn));
+        flo = fhi;
+      }
+    }
+    if (k0 + nlev == nz && active) {
+      const size_t b = static_cast<size_t>(nz) * nxy;
+      st32(wo, b, cC, static_cast<O>(T(ld32(w, b, cC))));
+    }
+    if (check && bad && active) atomicOr(flag, 1);
+  }
+}
+
+// ---- sponge (damping.cpp:76-138 with perturbation targets 0) -------------
+
+__global__ void __launch_bounds__(256) k_sponge(int nx, int ny, int nz, int kmin, const double *__restrict__ alpha,
+                                                double dt, double *__restrict__ u, double *__restrict__ v,
+                                                double *__restrict__ w, double *__restrict__ rho,
+                                                double *__restrict__ th, double *__restrict__ pi) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = kmin + static_cast<int>(blockIdx.z);
+  if (i >= nx || k > nz) return;
+  const double a = alpha[k < nz - 1 ? k : nz - 1];
+  if (a <= 0.0) return;
+  const double f = 1.0 / (1.0 + dt * a);
+  const size_t c = (static_cast<size_t>(k) * ny + j) * nx + i;
+  w[c] = f * w[c];
+  if (k < nz) {
+    u[c] = f * u[c];
+    v[c] = f * v[c];
+    rho[c] = f * rho[c];
+    th[c] = f * th[c];
+    pi[c] = f * pi[c];
+  }
+}
+
+// ---- diagnostics ------------------------------------------------------------
+
+constexpr int kRedThreads = 256;
+constexpr int kRedBlocks = 1024;
+constexpr int kEnergyComps = 5;
+
+// Per-cell terms of energy_budget (diagnostics.cpp:255-277), compensated.
+__global__ void __launch_bounds__(kRedThreads) k_energy_partial(
+    int nx, int ny, int nz, double dx, double dy, double H, const double *__restrict__ zs,
+    const double *__restrict__ dzeta, const double *__restrict__ zcc, const double *__restrict__ rho0,
+    const double *__restrict__ th0, const double *__restrict__ pi0, const double *__restrict__ u,
+    const double *__restrict__ v, const double *__restrict__ w, const double *__restrict__ rho,
+    const double *__restrict__ th, const double *__restrict__ pi, size_t chunk, NSum *__restrict__ part) {
+  const size_t nxy = static_cast<size_t>(nx) * ny, n = nxy * nz;
+  const size_t beg = static_cast<size_t>(blockIdx.x) * chunk;
+  const size_t end = beg + chunk < n ? beg + chunk : n;
+  NSum acc[kEnergyComps];
+  for (int q = 0; q < kEnergyComps; ++q) acc[q] = NSum{0.0, 0.0};
+  for (size_t c = beg + threadIdx.x; c < end; c += kRedThreads) {
+    const size_t k = c / nxy, col = c - k * nxy;
+    const int j = static_cast<int>(col / nx), i = static_cast<int>(col - static_cast<size_t>(j) * nx);
+    const int ip = i + 1 == nx ? 0 : i + 1, jp = j + 1 == ny ? 0 : j + 1;
+    const double z0 = zs[col];
+    const double dv = dx * dy * ((H - z0) / H) * dzeta[k] * H;
+    const double r = rho0[c] + rho[c];
+    const double uc = 0.5 * (u[c] + u[k * nxy + static_cast<size_t>(j) * nx + ip]);
+    const double vc = 0.5 * (v[c] + v[k * nxy + static_cast<size_t>(jp) * nx + i]);
+    const double wc = 0.5 * (w[c] + w[c + nxy]);
+    const double T = (th0[c] + th[c]) * (pi0[c] + pi[c]);
+    const double z = z0 + zcc[k] * (H - z0);
+    acc[0].add(r * dv);
+    acc[1].add(rho[c] * dv);
+    acc[2].add(0.5 * r * (uc * uc + vc * vc + wc * wc) * dv);
+    acc[3].add(r * kCv * T * dv);
+    acc[4].add(r * kG * z * dv);
+  }
+  __shared__ NSum sh[kRedThreads];
+  for (int q = 0; q < kEnergyComps; ++q) {
+    sh[threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int hh = kRedThreads / 2; hh > 0; hh >>= 1) {
+      if (threadIdx.x < static_cast<unsigned>(hh)) sh[threadIdx.x].merge(sh[threadIdx.x + hh]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[static_cast<size_t>(q) * gridDim.x + blockIdx.x] = sh[0];
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kRedThreads) k_red_final(const NSum *__restrict__ part, int nparts, int ncomp,
+                                                           double *__restrict__ out) {
+  __shared__ NSum sh[kRedThreads];
+  for (int comp = 0; comp < ncomp; ++comp) {
+    NSum acc{0.0, 0.0};
+    for (int p = threadIdx.x; p < nparts; p += kRedThreads) acc.merge(part[comp * nparts + p]);
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int hh = kRedThreads / 2; hh > 0; hh >>= 1) {
+      if (threadIdx.x < static_cast<unsigned>(hh)) sh[threadIdx.x].merge(sh[threadIdx.x + hh]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[comp] = sh[0].s + sh[0].c;
+    __syncthreads();
+  }
+}
+
+// Advective CFL of the state (advection.cpp:325-340) and max wind speed.
+__global__ void __launch_bounds__(256) k_cfl_state(int nx, int ny, int nz, double dx, double dy, double H,
+                                                   const double *__restrict__ zs, const double *__restrict__ dzeta,
+                                                   const double *__restrict__ u, const double *__restrict__ v,
+                                                   const double *__restrict__ w, double dt,
+                                                   unsigned long long *__restrict__ res) {
+  const size_t nxy = static_cast<size_t>(nx) * ny, n = nxy * nz;
+  double worst = 0.0, wind = 0.0;
+  for (size_t c = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
+       c += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const size_t k = c / nxy, col = c - k * nxy;
+    const int j = static_cast<int>(col / nx), i = static_cast<int>(col - static_cast<size_t>(j) * nx);
+    const int ip = i + 1 == nx ? 0 : i + 1, jp = j + 1 == ny ? 0 : j + 1;
+    const double dz = fmax(dzeta[k] * (H - zs[col]), 1.0);
+    const double uu = 0.5 * (u[c] + u[k * nxy + static_cast<size_t>(j) * nx + ip]);
+    const double vv = 0.5 * (v[c] + v[k * nxy + static_cast<size_t>(jp) * nx + i]);
+    const double ww = 0.5 * (w[c] + w[c + nxy]);
+    worst = fmax(worst, dt * (fabs(uu) / dx + fabs(vv) / dy + fabs(ww) / dz));
+    wind = fmax(wind, sqrt(uu * uu + vv * vv + ww * ww));
+  }
+  __shared__ double s0[256], s1[256];
+  s0[threadIdx.x] = worst;
+  s1[threadIdx.x] = wind;
+  __syncthreads();
+  for (int hh = 128; hh > 0; hh >>= 1) {
+    if (threadIdx.x < static_cast<unsigned>(hh)) {
+      s0[threadIdx.x] = fmax(s0[threadIdx.x], s0[threadIdx.x + hh]);
+      s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + hh]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicMax(&res[0], static_cast<unsigned long long>(__double_as_longlong(s0[0])));
+    atomicMax(&res[1], static_cast<unsigned long long>(__double_as_longlong(s1[0])));
+  }
+}
+
+}  // namespace vibe_mi
