@@ -7,7 +7,8 @@ import subprocess
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HDRS = [os.path.join(ROOT, "include", "vibe_mi", h) for h in ("c_kernels.h", "dycore.h")]
+HDRS = sorted(os.path.join(ROOT, "include", "vibe_mi", h) for h in os.listdir(os.path.join(ROOT, "include", "vibe_mi"))
+              if h.endswith(".h"))
 
 
 def declared_functions():

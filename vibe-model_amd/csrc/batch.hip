@@ -1,0 +1,728 @@
+// batch.hip -- many domains of uneven size advanced by one persistent launch
+// per kernel phase, from device-side tile work lists.
+//
+// Per RK3 step of the active members:
+//   kb_reset                   zero this step's work-queue counters
+//   for each stage st:
+//     kb_omega    rows         Omega of every active member
+//     kb_stage    64 x TY      fused slow tendencies + RK3 update
+//     nsub x kb_substep rows   fused HEVI acoustic substep
+//   kb_sponge   rows           Rayleigh sponge
+//   kb_advance                 rotate every active member's state buffers
+// Each launch is persistent: a grid of about one wave of resident
+// workgroups, each claiming tiles from an atomic counter until the list is
+// drained (tiles are ordered largest column first so the tail is short).
+// Tiles are {member, x-block, y-block}; every member's operands (geometry,
+// state buffers, HEVI tables, stage lengths) come from a device descriptor,
+// read once per tile into scalar registers.  The tile bodies are the same
+// functions as the single-domain kernels', so each member's result is
+// identical to stepping it alone.
+//
+// hipGraph: the whole launch sequence of one step reads everything that
+// changes between steps (buffer rotation, dt) from device memory, so it is
+// captured once per (active set, nsub) and replayed; counters are cleared by
+// a kernel, not a memset node.
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "domain.hpp"
+#include "kernels.hpp"
+#include "vibe_mi/batch.h"
+
+namespace vibe_mi {
+
+// Device descriptor of one member.
+template <typename T>
+struct BDom {
+  Geo<T> g;
+  FieldSet S[3];        // FP64 states; S[cur0] = s^n
+  AcousticSet<T> A[2];  // acoustic ping-pong buffers
+  T *om;                // Omega
+  const T *tab[3];      // HEVI factorisation of stage st: {beta, 1/pivot, c'} x tab_per
+  size_t tab_per;
+  double h[3];  // stage lengths dt/3, dt/2, dt
+  T dtau[3];    // substep lengths h/nsub
+  T an, ao;     // HEVI off-centring weights
+  double dt;
+  const double *sponge;
+  int kmin_sponge;
+  int cur0;  // rotation index, advanced on the device by kb_advance
+  int *flag;
+};
+
+// Persistent tile loop: one lane claims the next tile while the workgroup
+// works on the current one; the claimed index is broadcast through LDS.
+// Every thread of the workgroup runs body(t) for the same t.
+template <typename F>
+__device__ __forceinline__ void tile_loop(int *__restrict__ ctr, int ntiles, F &&body) {
+  __shared__ int s_next[2];
+  const bool leader = threadIdx.x == 0 && threadIdx.y == 0;
+  if (leader) s_next[0] = atomicAdd(ctr, 1);
+  __syncthreads();
+  int par = 0;
+  for (;;) {
+    const int t = __builtin_amdgcn_readfirstlane(s_next[par]);
+    if (t >= ntiles) break;
+    if (leader) s_next[par ^ 1] = atomicAdd(ctr, 1);
+    body(t);
+    __syncthreads();
+    par ^= 1;
+  }
+}
+
+__device__ __forceinline__ int stage_out_index(int cur0, int st) { return (cur0 + (st == 1 ? 2 : 1)) % 3; }
+
+__global__ void __launch_bounds__(64) kb_reset(int *__restrict__ ctr, int n) {
+  for (int q = threadIdx.x; q < n; q += 64) ctr[q] = 0;
+}
+
+// Rotates the state buffers of the members in ids[0..n): s^{n+1} = S[cur0+1].
+template <typename T>
+__global__ void __launch_bounds__(64) kb_advance(BDom<T> *__restrict__ D, const int *__restrict__ ids, int n) {
+  for (int q = threadIdx.x; q < n; q += 64) {
+    BDom<T> &dd = D[ids[q]];
+    dd.cur0 = (dd.cur0 + 1) % 3;
+  }
+}
+
+// Omega over row tiles {member, x-block, j, -}; 64 x 4 threads, 4 levels at a time.
+template <typename T>
+__global__ void __launch_bounds__(256) kb_omega(const BDom<T> *__restrict__ D, const int4 *__restrict__ tiles,
+                                                int ntiles, int st, int *__restrict__ ctr) {
+  tile_loop(ctr, ntiles, [&](int t) {
+    const int4 tl = tiles[t];
+    const BDom<T> &dd = D[tl.x];
+    const Geo<T> g = dd.g;
+    const FieldSet cur = dd.S[(dd.cur0 + st) % 3];
+    const int i = tl.y * 64 + static_cast<int>(threadIdx.x);
+    if (i < g.nx)
+      for (int k = threadIdx.y; k <= g.nz; k += 4) omega_at<T, double>(g, cur.u, cur.v, cur.w, dd.om, i, tl.z, k);
+  });
+}
+
+// Fused stage kernel over tiles {member, x-block, y-block, -} of 64 x TY columns.
+template <typename T, int TY>
+__global__ void __launch_bounds__(64 * TY) kb_stage(const BDom<T> *__restrict__ D, const int4 *__restrict__ tiles,
+                                                    int ntiles, int st, int *__restrict__ ctr) {
+  tile_loop(ctr, ntiles, [&](int t) {
+    const int4 tl = tiles[t];
+    const BDom<T> &dd = D[tl.x];
+    const Geo<T> g = dd.g;
+    const int c0 = dd.cur0;
+    const FieldSet cur = dd.S[(c0 + st) % 3], s0 = dd.S[c0], nxt = dd.S[stage_out_index(c0, st)];
+    const AcousticSet<T> a = dd.A[0];
+    const StageIn ci{cur.u, cur.v, cur.w, cur.rho, cur.th, cur.pi};
+    const StageIn si{s0.u, s0.v, s0.w, s0.rho, s0.th, s0.pi};
+    const StageOut<T> so{a.u, a.v, a.w, a.rho, a.pi, nxt.th};
+    const int i = tl.y * 64 + static_cast<int>(threadIdx.x), j = tl.z * TY + static_cast<int>(threadIdx.y);
+    if (i < g.nx && j < g.ny) stage_col<T, T>(g, ci, si, dd.om, dd.h[st], so, dd.flag, i, j);
+  });
+}
+
+// Acoustic substep q of stage st over row tiles.  The last substep of a
+// stage (last = 1, O = double) writes the stage output S[out] and checks
+// finiteness; the others write the acoustic buffer A[(q+1)&1] (O = T).
+template <typename T, typename O, int NW, int MAXCH>
+__global__ void __launch_bounds__(64 * NW) kb_substep(const BDom<T> *__restrict__ D, const int4 *__restrict__ tiles,
+                                                      int ntiles, int st, int q, int last, int *__restrict__ ctr) {
+  tile_loop(ctr, ntiles, [&](int t) {
+    const int4 tl = tiles[t];
+    const BDom<T> &dd = D[tl.x];
+    const Geo<T> g = dd.g;
+    const AcousticSet<T> in = dd.A[q & 1];
+    const T *tab = dd.tab[st];
+    const size_t per = dd.tab_per;
+    SubArgs<T, T, O> sa{};
+    sa.u = in.u;
+    sa.v = in.v;
+    sa.w = in.w;
+    sa.rho = in.rho;
+    sa.pi = in.pi;
+    sa.tA = tab;
+    sa.tINV = tab + per;
+    sa.tCP = tab + 2 * per;
+    sa.dtau = dd.dtau[st];
+    sa.an = dd.an;
+    sa.ao = dd.ao;
+    sa.flag = dd.flag;
+    if constexpr (sizeof(O) == 8) {
+      if (last) {
+        const FieldSet o = dd.S[stage_out_index(dd.cur0, st)];
+        sa.uo = o.u;
+        sa.vo = o.v;
+        sa.wo = o.w;
+        sa.rho_o = o.rho;
+        sa.pi_o = o.pi;
+        sa.check = 1;
+      } else if constexpr (sizeof(T) == 8) {
+        const AcousticSet<T> o = dd.A[(q + 1) & 1];
+        sa.uo = o.u;
+        sa.vo = o.v;
+        sa.wo = o.w;
+        sa.rho_o = o.rho;
+        sa.pi_o = o.pi;
+      }
+    } else {
+      const AcousticSet<T> o = dd.A[(q + 1) & 1];
+      sa.uo = o.u;
+      sa.vo = o.v;
+      sa.wo = o.w;
+      sa.rho_o = o.rho;
+      sa.pi_o = o.pi;
+    }
+    substep_row<T, T, O, NW, MAXCH>(g, sa, tl.y, tl.z);
+  });
+}
+
+// Sponge over row tiles of the members' final stage output.
+template <typename T>
+__global__ void __launch_bounds__(256) kb_sponge(const BDom<T> *__restrict__ D, const int4 *__restrict__ tiles,
+                                                 int ntiles, int *__restrict__ ctr) {
+  tile_loop(ctr, ntiles, [&](int t) {
+    const int4 tl = tiles[t];
+    const BDom<T> &dd = D[tl.x];
+    const int nx = dd.g.nx, ny = dd.g.ny, nz = dd.g.nz;
+    const FieldSet fin = dd.S[(dd.cur0 + 1) % 3];
+    const int i = tl.y * 64 + static_cast<int>(threadIdx.x);
+    if (i < nx)
+      for (int k = dd.kmin_sponge + static_cast<int>(threadIdx.y); k <= nz; k += 4)
+        sponge_at(nx, ny, nz, dd.sponge, dd.dt, fin, i, tl.z, k);
+  });
+}
+
+}  // namespace vibe_mi
+
+using namespace vibe_mi;
+
+// ---------------------------------------------------------------------------
+// batch handle
+// ---------------------------------------------------------------------------
+
+namespace {
+
+constexpr int kMaxCounters = 256;  // work-queue counters per step (3 stages x (2 + nsub) + 1)
+constexpr int kMaxNsub = 64;
+
+struct WorkList {
+  int4 *rows = nullptr;  // {member, x-block, j}
+  int nrows = 0;
+  int4 *stage[2] = {nullptr, nullptr};  // tiles for TY = 2, 4
+  int nstage[2] = {0, 0};
+  int *ids = nullptr;  // active members
+  int nids = 0;
+  int4 *sponge_rows = nullptr;  // rows of members with a sponge
+  int nsponge = 0;
+};
+
+}  // namespace
+
+struct vibe_mi_batch {
+  int precision = VIBE_MI_FP64;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<vibe_mi_domain *> doms;
+  void *desc = nullptr;  // BDom<T>[ndom] on the device
+  std::vector<unsigned char> hdesc;
+  int *ctr = nullptr;
+  int *flag = nullptr;
+  std::vector<void *> allocs;
+  size_t bytes = 0;
+  int nzmax = 0;
+  std::map<std::string, WorkList> lists;
+  std::map<std::string, hipGraphExec_t> graphs;
+  std::vector<hipGraph_t> graph_objs;
+  bool use_graph = true;
+  bool tuned = false;
+  int nw = 8;  // waves per substep workgroup
+  int ty = 4;  // rows per stage workgroup
+  int cus = 256;
+  std::vector<double> dt_cur;
+  int nsub_cur = -1;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+namespace {
+
+template <typename P>
+int balloc(vibe_mi_batch *b, P **out, size_t count) {
+  void *p = nullptr;
+  const size_t n = count * sizeof(P);
+  VMI_HIP(hipMalloc(&p, n > 0 ? n : 1));
+  b->allocs.push_back(p);
+  b->bytes += n;
+  *out = static_cast<P *>(p);
+  return 0;
+}
+
+template <typename P>
+int bupload(vibe_mi_batch *b, P **out, const std::vector<P> &h) {
+  if (int rc = balloc(b, out, h.size())) return rc;
+  if (!h.empty()) VMI_HIP(hipMemcpy(*out, h.data(), h.size() * sizeof(P), hipMemcpyHostToDevice));
+  return 0;
+}
+
+std::string key_of(const std::vector<unsigned char> &act) { return std::string(act.begin(), act.end()); }
+
+// Builds (or returns the cached) work lists of an active set.  Members are
+// visited deepest column first so the longest tiles are claimed first.
+int work_list(vibe_mi_batch *b, const std::vector<unsigned char> &act, WorkList **out) {
+  const std::string key = key_of(act);
+  auto it = b->lists.find(key);
+  if (it != b->lists.end()) {
+    *out = &it->second;
+    return 0;
+  }
+  std::vector<int> order;
+  for (int m = 0; m < static_cast<int>(b->doms.size()); ++m)
+    if (act[m]) order.push_back(m);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->doms[x]->nz > b->doms[y]->nz; });
+  std::vector<int4> rows, sp, st2, st4;
+  for (int m : order) {
+    const vibe_mi_domain *d = b->doms[m];
+    const int nbx = static_cast<int>(ceil_div(d->nx, 64));
+    for (int j = 0; j < d->ny; ++j)
+      for (int bx = 0; bx < nbx; ++bx) {
+        rows.push_back(make_int4(m, bx, j, 0));
+        if (d->kmin_sponge <= d->nz) sp.push_back(make_int4(m, bx, j, 0));
+      }
+    for (int by = 0; by < static_cast<int>(ceil_div(d->ny, 2)); ++by)
+      for (int bx = 0; bx < nbx; ++bx) st2.push_back(make_int4(m, bx, by, 0));
+    for (int by = 0; by < static_cast<int>(ceil_div(d->ny, 4)); ++by)
+      for (int bx = 0; bx < nbx; ++bx) st4.push_back(make_int4(m, bx, by, 0));
+  }
+  WorkList w;
+  if (int rc = bupload(b, &w.rows, rows)) return rc;
+  if (int rc = bupload(b, &w.sponge_rows, sp)) return rc;
+  if (int rc = bupload(b, &w.stage[0], st2)) return rc;
+  if (int rc = bupload(b, &w.stage[1], st4)) return rc;
+  if (int rc = bupload(b, &w.ids, order)) return rc;
+  w.nrows = static_cast<int>(rows.size());
+  w.nsponge = static_cast<int>(sp.size());
+  w.nstage[0] = static_cast<int>(st2.size());
+  w.nstage[1] = static_cast<int>(st4.size());
+  w.nids = static_cast<int>(order.size());
+  *out = &(b->lists[key] = w);
+  return 0;
+}
+
+// Resident workgroups of a kernel on the whole device (persistent grid size).
+template <typename K>
+int resident(vibe_mi_batch *b, K kernel, int threads, size_t lds, int ntiles, unsigned *grid) {
+  int per_cu = 0;
+  VMI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds));
+  const long cap = static_cast<long>(std::max(per_cu, 1)) * b->cus;
+  *grid = static_cast<unsigned>(std::max(1L, std::min<long>(cap, ntiles)));
+  return 0;
+}
+
+// Writes the member descriptors for step lengths dt[] (cur0 from the host
+// mirror in each member handle) and factorises the HEVI tables.
+template <typename T>
+int upload_desc(vibe_mi_batch *b, const double *dt, int nsub) {
+  const int n = static_cast<int>(b->doms.size());
+  std::vector<BDom<T>> h(n);
+  for (int m = 0; m < n; ++m) {
+    vibe_mi_domain *d = b->doms[m];
+    BDom<T> &x = h[m];
+    std::memset(static_cast<void *>(&x), 0, sizeof(x));
+    if constexpr (sizeof(T) == 4) {
+      x.g = d->g32;
+      x.A[0] = d->A32[0];
+      x.A[1] = d->A32[1];
+    } else {
+      x.g = d->g64;
+      x.A[0] = d->A64[0];
+      x.A[1] = d->A64[1];
+    }
+    for (int q = 0; q < 3; ++q) x.S[q] = d->S[q];
+    x.om = static_cast<T *>(d->om);
+    const double hk[3] = {dt[m] / 3.0, dt[m] / 2.0, dt[m]};
+    for (int st = 0; st < 3; ++st) {
+      const double dtau = hk[st] / static_cast<double>(nsub);
+      if (int rc = domain_ensure_table(d, st, dtau)) return rc;
+      x.tab[st] = static_cast<const T *>(domain_table(d, st));
+      x.h[st] = hk[st];
+      x.dtau[st] = static_cast<T>(dtau);
+    }
+    x.tab_per = static_cast<size_t>(d->nz - 1) * d->rk;
+    x.an = static_cast<T>(0.5 * (1.0 + d->eps_s));
+    x.ao = static_cast<T>(0.5 * (1.0 - d->eps_s));
+    x.dt = dt[m];
+    x.sponge = d->sponge;
+    x.kmin_sponge = d->kmin_sponge;
+    x.cur0 = d->cur0;
+    x.flag = b->flag;
+    VMI_HIP(hipStreamSynchronize(d->stream));
+  }
+  VMI_HIP(hipMemcpyAsync(b->desc, h.data(), sizeof(BDom<T>) * n, hipMemcpyHostToDevice, b->stream));
+  VMI_HIP(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+// dry: only set the kernel's LDS attribute (before a graph capture).
+template <typename T, typename O, int NW>
+int launch_substep_nw(vibe_mi_batch *b, const WorkList &w, int st, int q, int last, int *ctr, bool dry) {
+  const auto *D = static_cast<const BDom<T> *>(b->desc);
+  const int ch = (b->nzmax + NW - 1) / NW;
+  const size_t lds = 3 * static_cast<size_t>(b->nzmax - 1) * 64 * sizeof(T);
+  const size_t lds_static = 3 * NW * 64 * sizeof(T) + 64;
+  if (lds + lds_static > 160 * 1024) return VIBE_C_ERR_BAD_ARG;
+#define VMI_BSUB(M)                                                                                           \
+  do {                                                                                                        \
+    auto *kern = &kb_substep<T, O, NW, M>;                                                                    \
+    static bool attr = false;                                                                                 \
+    if (!attr) {                                                                                              \
+      VMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),                                       \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,                                 \
+                                  static_cast<int>(160 * 1024 - lds_static)));                                \
+      attr = true;                                                                                            \
+    }                                                                                                         \
+    if (dry) break;                                                                                           \
+    unsigned grid = 0;                                                                                        \
+    if (int rc = resident(b, kern, 64 * NW, lds, w.nrows, &grid)) return rc;                                  \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * NW), lds, b->stream, D, w.rows, w.nrows, st, q, last, ctr); \
+  } while (0)
+  if constexpr (NW == 16) {
+    if (ch <= 4) VMI_BSUB(4);
+    else if (ch <= 8) VMI_BSUB(8);
+    else return VIBE_C_ERR_BAD_ARG;
+  } else if constexpr (NW == 8) {
+    if (ch <= 4) VMI_BSUB(4);
+    else if (ch <= 8) VMI_BSUB(8);
+    else if (ch <= 12) VMI_BSUB(12);
+    else if (ch <= 16) VMI_BSUB(16);
+    else return VIBE_C_ERR_BAD_ARG;
+  } else {
+    if (ch <= 8) VMI_BSUB(8);
+    else if (ch <= 12) VMI_BSUB(12);
+    else if (ch <= 16) VMI_BSUB(16);
+    else if (ch <= 24) VMI_BSUB(24);
+    else if (ch <= 32) VMI_BSUB(32);
+    else return VIBE_C_ERR_BAD_ARG;
+  }
+#undef VMI_BSUB
+  VMI_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename T, typename O>
+int launch_substep(vibe_mi_batch *b, int nw, const WorkList &w, int st, int q, int last, int *ctr,
+                   bool dry = false) {
+  if (nw == 16) return launch_substep_nw<T, O, 16>(b, w, st, q, last, ctr, dry);
+  if (nw == 8) return launch_substep_nw<T, O, 8>(b, w, st, q, last, ctr, dry);
+  return launch_substep_nw<T, O, 4>(b, w, st, q, last, ctr, dry);
+}
+
+template <typename T, int TY>
+int launch_stage_ty(vibe_mi_batch *b, const WorkList &w, int st, int *ctr) {
+  const auto *D = static_cast<const BDom<T> *>(b->desc);
+  const int li = TY == 2 ? 0 : 1;
+  auto *kern = &kb_stage<T, TY>;
+  unsigned grid = 0;
+  if (int rc = resident(b, kern, 64 * TY, 0, w.nstage[li], &grid)) return rc;
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(64, TY), 0, b->stream, D, w.stage[li], w.nstage[li], st, ctr);
+  VMI_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int launch_stage(vibe_mi_batch *b, int ty, const WorkList &w, int st, int *ctr) {
+  return ty == 2 ? launch_stage_ty<T, 2>(b, w, st, ctr) : launch_stage_ty<T, 4>(b, w, st, ctr);
+}
+
+// Enqueues one RK3 step of the active set on the batch stream (capturable:
+// kernels only).
+template <typename T>
+int enqueue_step(vibe_mi_batch *b, const WorkList &w, int nsub) {
+  const auto *D = static_cast<const BDom<T> *>(b->desc);
+  int *ctr = b->ctr;
+  int c = 0;
+  hipLaunchKernelGGL(kb_reset, dim3(1), dim3(64), 0, b->stream, ctr, kMaxCounters);
+  for (int st = 0; st < 3; ++st) {
+    {
+      auto *kern = &kb_omega<T>;
+      unsigned grid = 0;
+      if (int rc = resident(b, kern, 256, 0, w.nrows, &grid)) return rc;
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(64, 4), 0, b->stream, D, w.rows, w.nrows, st, ctr + c++);
+    }
+    if (int rc = launch_stage<T>(b, b->ty, w, st, ctr + c++)) return rc;
+    for (int q = 0; q < nsub; ++q) {
+      const bool last = q == nsub - 1;
+      int rc;
+      if constexpr (sizeof(T) == 4) {
+        rc = last ? launch_substep<T, double>(b, b->nw, w, st, q, 1, ctr + c++)
+                  : launch_substep<T, T>(b, b->nw, w, st, q, 0, ctr + c++);
+      } else {
+        rc = launch_substep<T, T>(b, b->nw, w, st, q, last ? 1 : 0, ctr + c++);
+      }
+      if (rc) return rc;
+    }
+  }
+  if (w.nsponge > 0) {
+    auto *kern = &kb_sponge<T>;
+    unsigned grid = 0;
+    if (int rc = resident(b, kern, 256, 0, w.nsponge, &grid)) return rc;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64, 4), 0, b->stream, D, w.sponge_rows, w.nsponge, ctr + c++);
+  }
+  hipLaunchKernelGGL((kb_advance<T>), dim3(1), dim3(64), 0, b->stream, static_cast<BDom<T> *>(b->desc), w.ids, w.nids);
+  VMI_HIP(hipGetLastError());
+  return 0;
+}
+
+// Autotuner: waves per substep workgroup and rows per stage workgroup,
+// measured on this batch's full work list with HIP events (the substep runs
+// on the acoustic scratch buffers, the stage writes the stage-1 output
+// buffers, neither touches s^n), cached per batch shape for the process.
+std::map<std::string, std::pair<int, int>> g_btune;
+
+template <typename T>
+int autotune(vibe_mi_batch *b, const WorkList &w) {
+  if (b->tuned) return 0;
+  std::string shape = std::to_string(sizeof(T));
+  for (auto *d : b->doms) shape += "," + std::to_string(d->nx) + "x" + std::to_string(d->ny) + "x" + std::to_string(d->nz) + "r" + std::to_string(d->rk);
+  auto it = g_btune.find(shape);
+  if (it != g_btune.end()) {
+    b->nw = it->second.first;
+    b->ty = it->second.second;
+    b->tuned = true;
+    return 0;
+  }
+  if (const char *env = std::getenv("VIBE_MI_SUBSTEP_WAVES")) {
+    const int v = std::atoi(env);
+    if (v == 4 || v == 8 || v == 16) b->nw = v;
+  }
+  auto time_it = [&](auto &&launch, float *ms) -> int {
+    float total = 0.f;
+    for (int rep = 0; rep < 3; ++rep) {
+      hipLaunchKernelGGL(kb_reset, dim3(1), dim3(64), 0, b->stream, b->ctr, kMaxCounters);
+      VMI_HIP(hipEventRecord(b->ev[0], b->stream));
+      if (int rc = launch()) return rc;
+      VMI_HIP(hipEventRecord(b->ev[1], b->stream));
+      VMI_HIP(hipEventSynchronize(b->ev[1]));
+      float x = 0.f;
+      VMI_HIP(hipEventElapsedTime(&x, b->ev[0], b->ev[1]));
+      if (rep > 0) total += x;
+    }
+    *ms = total;
+    return 0;
+  };
+  if (!std::getenv("VIBE_MI_SUBSTEP_WAVES")) {
+    float best = 1e30f;
+    for (int nw : {4, 8, 16}) {
+      float ms = 0.f;
+      const int rc = time_it([&]() { return launch_substep<T, T>(b, nw, w, 0, 0, 0, b->ctr); }, &ms);
+      if (rc == 0 && ms < best) {
+        best = ms;
+        b->nw = nw;
+      }
+    }
+    (void)hipGetLastError();
+  }
+  {
+    float best = 1e30f;
+    {  // a valid Omega for the stage kernel's operands
+      hipLaunchKernelGGL(kb_reset, dim3(1), dim3(64), 0, b->stream, b->ctr, kMaxCounters);
+      auto *kern = &kb_omega<T>;
+      unsigned grid = 0;
+      if (int rc = resident(b, kern, 256, 0, w.nrows, &grid)) return rc;
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(64, 4), 0, b->stream, static_cast<const BDom<T> *>(b->desc), w.rows,
+                         w.nrows, 0, b->ctr);
+      VMI_HIP(hipGetLastError());
+    }
+    for (int ty : {2, 4}) {
+      float ms = 0.f;
+      const int rc = time_it([&]() { return launch_stage<T>(b, ty, w, 0, b->ctr); }, &ms);
+      if (rc == 0 && ms < best) {
+        best = ms;
+        b->ty = ty;
+      }
+    }
+    (void)hipGetLastError();
+  }
+  b->tuned = true;
+  g_btune[shape] = {b->nw, b->ty};
+  return 0;
+}
+
+void free_batch(vibe_mi_batch *b) {
+  for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto g : b->graph_objs) (void)hipGraphDestroy(g);
+  b->graphs.clear();
+  b->graph_objs.clear();
+  for (void *p : b->allocs) (void)hipFree(p);
+  b->allocs.clear();
+  for (auto *d : b->doms) vibe_mi_domain_destroy(d);
+  b->doms.clear();
+  for (auto &e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  b->stream = nullptr;
+}
+
+template <typename T>
+int step_batch(vibe_mi_batch *b, const double *dt, int nsub, int nsteps, const unsigned char *active, bool timed,
+               double *ms_total) {
+  const int n = static_cast<int>(b->doms.size());
+  std::vector<unsigned char> act(n, 1);
+  if (active)
+    for (int m = 0; m < n; ++m) act[m] = active[m] ? 1 : 0;
+  for (int m = 0; m < n; ++m)
+    if (act[m] && !(dt[m] > 0.0)) return VIBE_C_ERR_BAD_ARG;
+  WorkList *w = nullptr;
+  if (int rc = work_list(b, act, &w)) return rc;
+  if (w->nids == 0 || nsteps == 0) {
+    if (ms_total) *ms_total = 0.0;
+    return 0;
+  }
+  if (int rc = upload_desc<T>(b, dt, nsub)) return rc;
+  if (int rc = autotune<T>(b, *w)) return rc;
+  if constexpr (sizeof(T) == 4)
+    if (int rc = launch_substep<T, double>(b, b->nw, *w, 0, 0, 1, b->ctr, true)) return rc;
+  if (int rc = launch_substep<T, T>(b, b->nw, *w, 0, 0, 0, b->ctr, true)) return rc;
+  VMI_HIP(hipMemsetAsync(b->flag, 0, sizeof(int), b->stream));
+  hipGraphExec_t exec = nullptr;
+  if (b->use_graph) {
+    const std::string gk = key_of(act) + "|" + std::to_string(nsub);
+    auto it = b->graphs.find(gk);
+    if (it == b->graphs.end()) {
+      hipGraph_t graph = nullptr;
+      VMI_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      const int rc = enqueue_step<T>(b, *w, nsub);
+      hipGraph_t captured = nullptr;
+      const hipError_t ec = hipStreamEndCapture(b->stream, &captured);
+      if (rc) return rc;
+      VMI_HIP(ec);
+      graph = captured;
+      b->graph_objs.push_back(graph);
+      VMI_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      b->graphs[gk] = exec;
+    } else {
+      exec = it->second;
+    }
+  }
+  if (timed) VMI_HIP(hipEventRecord(b->ev[0], b->stream));
+  for (int s = 0; s < nsteps; ++s) {
+    if (exec) {
+      VMI_HIP(hipGraphLaunch(exec, b->stream));
+    } else if (int rc = enqueue_step<T>(b, *w, nsub)) {
+      return rc;
+    }
+  }
+  if (timed) VMI_HIP(hipEventRecord(b->ev[1], b->stream));
+  int f = 0;
+  VMI_HIP(hipMemcpyAsync(&f, b->flag, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  VMI_HIP(hipStreamSynchronize(b->stream));
+  if (timed && ms_total) {
+    float ms = 0.f;
+    VMI_HIP(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+    *ms_total = ms;
+  }
+  for (int m = 0; m < n; ++m)
+    if (act[m]) b->doms[m]->cur0 = (b->doms[m]->cur0 + nsteps) % 3;
+  return f ? VIBE_MI_ERR_NONFINITE : 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vibe_mi_batch_create(const vibe_mi_domain_desc *descs, int ndom, vibe_mi_batch **out) {
+  if (descs == nullptr || out == nullptr || ndom <= 0) return VIBE_C_ERR_BAD_ARG;
+  *out = nullptr;
+  for (int m = 1; m < ndom; ++m)
+    if (descs[m].precision != descs[0].precision) return VIBE_C_ERR_BAD_ARG;
+  auto *b = new vibe_mi_batch();
+  b->precision = descs[0].precision;
+  auto fail = [&](int rc) {
+    free_batch(b);
+    delete b;
+    set_last_error(rc);
+    return rc;
+  };
+  for (int m = 0; m < ndom; ++m) {
+    vibe_mi_domain *d = nullptr;
+    if (int rc = domain_create(&descs[m], &d)) return fail(rc);
+    b->doms.push_back(d);
+    b->nzmax = std::max(b->nzmax, d->nz);
+  }
+  if (hipGetDevice(&b->device) != hipSuccess) return fail(VIBE_MI_ERR_HIP);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, b->device) != hipSuccess) return fail(VIBE_MI_ERR_HIP);
+  b->cus = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail(VIBE_MI_ERR_HIP);
+  for (auto &e : b->ev)
+    if (hipEventCreate(&e) != hipSuccess) return fail(VIBE_MI_ERR_HIP);
+  const size_t dsz = b->precision == VIBE_MI_MIXED ? sizeof(BDom<float>) : sizeof(BDom<double>);
+  unsigned char *p = nullptr;
+  if (int rc = balloc(b, &p, dsz * ndom)) return fail(rc);
+  b->desc = p;
+  if (int rc = balloc(b, &b->ctr, kMaxCounters)) return fail(rc);
+  if (int rc = balloc(b, &b->flag, 1)) return fail(rc);
+  if (hipMemset(b->flag, 0, sizeof(int)) != hipSuccess) return fail(VIBE_MI_ERR_HIP);
+  *out = b;
+  return 0;
+}
+
+int vibe_mi_batch_destroy(vibe_mi_batch *b) {
+  if (b == nullptr) return VIBE_C_ERR_BAD_ARG;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  free_batch(b);
+  delete b;
+  return 0;
+}
+
+int vibe_mi_batch_member(vibe_mi_batch *b, int idx, vibe_mi_domain **out) {
+  if (b == nullptr || out == nullptr || idx < 0 || idx >= static_cast<int>(b->doms.size())) return VIBE_C_ERR_BAD_ARG;
+  *out = b->doms[idx];
+  return 0;
+}
+
+static int batch_step_common(vibe_mi_batch *b, const double *dt, int nsub, int nsteps, const unsigned char *active,
+                             bool timed, double *ms_total) {
+  if (b == nullptr || dt == nullptr || nsteps < 0 || nsub < 1 || 3 * (nsub + 2) + 1 > kMaxCounters ||
+      nsub > kMaxNsub)
+    return VIBE_C_ERR_BAD_ARG;
+  if (hipSetDevice(b->device) != hipSuccess) return VIBE_MI_ERR_HIP;
+  for (auto *d : b->doms)
+    if (hipStreamSynchronize(d->stream) != hipSuccess) return VIBE_MI_ERR_HIP;
+  const int rc = b->precision == VIBE_MI_MIXED ? step_batch<float>(b, dt, nsub, nsteps, active, timed, ms_total)
+                                               : step_batch<double>(b, dt, nsub, nsteps, active, timed, ms_total);
+  if (rc) set_last_error(rc);
+  return rc;
+}
+
+int vibe_mi_batch_step(vibe_mi_batch *b, const double *dt, int nsub, int nsteps, const unsigned char *active) {
+  return batch_step_common(b, dt, nsub, nsteps, active, false, nullptr);
+}
+
+int vibe_mi_batch_step_timed(vibe_mi_batch *b, const double *dt, int nsub, int nsteps, const unsigned char *active,
+                             double *ms_total) {
+  return batch_step_common(b, dt, nsub, nsteps, active, true, ms_total);
+}
+
+int vibe_mi_batch_set_graph(vibe_mi_batch *b, int use_graph) {
+  if (b == nullptr) return VIBE_C_ERR_BAD_ARG;
+  b->use_graph = use_graph != 0;
+  return 0;
+}
+
+int vibe_mi_batch_tuning(vibe_mi_batch *b, int *substep_waves, int *stage_rows) {
+  if (b == nullptr) return VIBE_C_ERR_BAD_ARG;
+  if (substep_waves) *substep_waves = b->nw;
+  if (stage_rows) *stage_rows = b->ty;
+  return 0;
+}
+
+size_t vibe_mi_batch_device_bytes(vibe_mi_batch *b) {
+  if (b == nullptr) return 0;
+  size_t n = b->bytes;
+  for (auto *d : b->doms) n += vibe_mi_domain_device_bytes(d);
+  return n;
+}
+
+}  // extern "C"

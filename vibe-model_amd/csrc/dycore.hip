@@ -204,9 +204,10 @@ int launch_substep(vibe_mi_domain *d, const Geo<T> &g, const T *tab, const I *u,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kMaxLds - lds_static)));
     attr_set = true;
   }
+  const SubArgs<T, I, O> sa{u, v, w, rho, pi, tab, tab + per, tab + 2 * per, dtau, an, ao,
+                            uo, vo, wo, rho_o, pi_o, check, d->flag};
   hipLaunchKernelGGL((k_substep<T, I, O, NW, MAXCH>), dim3(ceil_div(d->nx, 64), d->ny), dim3(64 * NW), lds, d->stream,
-                     g, u, v, w, rho, pi, tab, tab + per, tab + 2 * per, dtau, an, ao, uo, vo, wo, rho_o, pi_o, check,
-                     d->flag);
+                     g, sa);
   return 0;
 }
 
@@ -383,7 +384,7 @@ int step_impl(vibe_mi_domain *d, const Geo<T> &g, AcousticSet<T> *A, double dt, 
   const int nk = d->nz + 1 - d->kmin_sponge;
   if (d->kmin_sponge <= d->nz && nk > 0) {
     hipLaunchKernelGGL(k_sponge, grid_of(d, nk), dim3(kBlk), 0, d->stream, d->nx, d->ny, d->nz, d->kmin_sponge,
-                       d->sponge, dt, fin.u, fin.v, fin.w, fin.rho, fin.th, fin.pi);
+                       d->sponge, dt, fin);
     VMI_HIP(hipGetLastError());
   }
   d->cur0 = i1;
@@ -512,6 +513,22 @@ int set_device(vibe_mi_domain *d) {
 }
 
 }  // namespace
+
+namespace vibe_mi {
+
+int domain_create(const vibe_mi_domain_desc *ds, vibe_mi_domain **out) { return vibe_mi_domain_create(ds, out); }
+
+int domain_ensure_table(vibe_mi_domain *d, int slot, double dtau) {
+  if (d->precision == VIBE_MI_MIXED) return ensure_table<float>(d, d->g32, slot, dtau);
+  return ensure_table<double>(d, d->g64, slot, dtau);
+}
+
+void *domain_table(vibe_mi_domain *d, int slot) {
+  if (d->precision == VIBE_MI_MIXED) return table_slot<float>(d, slot);
+  return table_slot<double>(d, slot);
+}
+
+}  // namespace vibe_mi
 
 extern "C" {
 

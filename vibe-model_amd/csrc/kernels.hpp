@@ -48,6 +48,7 @@
 #include "common.hpp"
 #include "geo.hpp"
 #include "stencils.hpp"
+#include "domain.hpp"
 
 namespace vibe_mi {
 
@@ -60,17 +61,24 @@ __device__ __forceinline__ T mc_at(const Geo<T> &g, const Acc &A, const S *__res
          T(0.5) * (T(v[A.at(i, j, k)]) * g.sy[gcol(g, i, j)] + T(v[A.at(i, j + 1, k)]) * g.sy[gcol(g, i, j + 1)]);
 }
 
+// Omega at interface k of column (i, j).
 template <typename T, typename S>
-__global__ void __launch_bounds__(256) k_omega(Geo<T> g, const S *__restrict__ u, const S *__restrict__ v,
-                                               const S *__restrict__ w, T *__restrict__ om) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
-  if (i >= g.nx) return;
+__device__ __forceinline__ void omega_at(const Geo<T> &g, const S *__restrict__ u, const S *__restrict__ v,
+                                         const S *__restrict__ w, T *__restrict__ om, int i, int j, int k) {
   const Acc A{g.nx, g.ny, static_cast<size_t>(g.nx) * g.ny};
   T o = T(0);
   if (k > 0 && k < g.nz)
     o = T(w[A.at(i, j, k)]) -
         (T(1) - g.zeta[k]) * (T(0.5) * (mc_at<T, S>(g, A, u, v, i, j, k - 1) + mc_at<T, S>(g, A, u, v, i, j, k)));
   om[A.at(i, j, k)] = o;
+}
+
+template <typename T, typename S>
+__global__ void __launch_bounds__(256) k_omega(Geo<T> g, const S *__restrict__ u, const S *__restrict__ v,
+                                               const S *__restrict__ w, T *__restrict__ om) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+  if (i >= g.nx) return;
+  omega_at<T, S>(g, u, v, w, om, i, j, k);
 }
 
 // ---- pressure-gradient operators (Gal-Chen, along constant z) --------------
@@ -116,10 +124,9 @@ struct StageOut {
 };
 
 template <typename T, typename O>
-__global__ void __launch_bounds__(256) k_stage_col(Geo<T> g, StageIn cur, StageIn s0, const T *__restrict__ om,
-                                                   double h, StageOut<O> out, int *__restrict__ flag) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
-  if (i >= g.nx || j >= g.ny) return;
+__device__ __forceinline__ void stage_col(const Geo<T> &g, const StageIn &cur, const StageIn &s0,
+                                          const T *__restrict__ om, double h, const StageOut<O> &out,
+                                          int *__restrict__ flag, int i, int j) {
   const int nx = g.nx, ny = g.ny, nz = g.nz, rk = g.rk;
   const size_t nxy = static_cast<size_t>(nx) * ny;
   const double *__restrict__ U = cur.u;
@@ -364,6 +371,14 @@ __global__ void __launch_bounds__(256) k_stage_col(Geo<T> g, StageIn cur, StageI
   if (bad) atomicOr(flag, 1);
 }
 
+template <typename T, typename O>
+__global__ void __launch_bounds__(256) k_stage_col(Geo<T> g, StageIn cur, StageIn s0, const T *__restrict__ om,
+                                                   double h, StageOut<O> out, int *__restrict__ flag) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  if (i >= g.nx || j >= g.ny) return;
+  stage_col<T, O>(g, cur, s0, om, h, out, flag, i, j);
+}
+
 // ---- HEVI factorisation table ----------------------------------------------
 // The tridiagonal matrix of the vertically implicit w system depends only on
 // the reference state, the column depth and dtau, so it is factorised once
@@ -410,14 +425,37 @@ __global__ void __launch_bounds__(64) k_hevi_factor(Geo<T> g, T dtau, T an, T *_
 // stage's precomputed factorisation.  Phase D (all waves): vertical flux
 // divergence closes rho' and pi', and w is written.  State traffic per cell:
 // read u, v, w, rho', pi' once, write them once.
+// Substep operands of one domain (device pointers and scalars).
+template <typename T, typename I, typename O>
+struct SubArgs {
+  const I *u, *v, *w, *rho, *pi;  // acoustic state in
+  const T *tA, *tINV, *tCP;       // HEVI factorisation of this stage
+  T dtau, an, ao;
+  O *uo, *vo, *wo, *rho_o, *pi_o;  // acoustic state out
+  int check;                       // 1: flag non-finite outputs
+  int *flag;
+};
+
+// Row of 64 columns starting at x-block bx, row j; all 64*NW threads of the
+// workgroup call it together (it synchronises the workgroup).
 template <typename T, typename I, typename O, int NW, int MAXCH>
-__global__ void __launch_bounds__(64 * NW) k_substep(Geo<T> g, const I *__restrict__ u, const I *__restrict__ v,
-                                                     const I *__restrict__ w, const I *__restrict__ rho,
-                                                     const I *__restrict__ pi, const T *__restrict__ tA,
-                                                     const T *__restrict__ tINV, const T *__restrict__ tCP, T dtau,
-                                                     T an, T ao, O *__restrict__ uo, O *__restrict__ vo,
-                                                     O *__restrict__ wo, O *__restrict__ rho_o, O *__restrict__ pi_o,
-                                                     int check, int *__restrict__ flag) {
+__device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I, O> &sa, int bx, int j) {
+  const I *__restrict__ u = sa.u;
+  const I *__restrict__ v = sa.v;
+  const I *__restrict__ w = sa.w;
+  const I *__restrict__ rho = sa.rho;
+  const I *__restrict__ pi = sa.pi;
+  const T *__restrict__ tA = sa.tA;
+  const T *__restrict__ tINV = sa.tINV;
+  const T *__restrict__ tCP = sa.tCP;
+  const T dtau = sa.dtau, an = sa.an, ao = sa.ao;
+  O *__restrict__ uo = sa.uo;
+  O *__restrict__ vo = sa.vo;
+  O *__restrict__ wo = sa.wo;
+  O *__restrict__ rho_o = sa.rho_o;
+  O *__restrict__ pi_o = sa.pi_o;
+  const int check = sa.check;
+  int *__restrict__ flag = sa.flag;
   extern __shared__ unsigned char smem_raw[];
   const int nlds = (g.nz - 1) * 64;
   T *s_td = reinterpret_cast<T *>(smem_raw);  // [nz-1][64] RHS -> d' -> w
@@ -430,10 +468,9 @@ __global__ void __launch_bounds__(64 * NW) k_substep(Geo<T> g, const I *__restri
   const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
   const int nx = g.nx, ny = g.ny, nz = g.nz, rk = g.rk;
   const size_t nxy = static_cast<size_t>(nx) * ny;
-  const int i_raw = blockIdx.x * 64 + lane;
+  const int i_raw = bx * 64 + lane;
   const bool active = i_raw < nx;
   const int i = active ? i_raw : nx - 1;
-  const int j = blockIdx.y;
   const int ch = (nz + NW - 1) / NW;
   const int k0 = wv * ch;
   const int nlev = max(0, min(nz, k0 + ch) - k0);
@@ -698,14 +735,21 @@ __global__ void __launch_bounds__(64 * NW) k_substep(Geo<T> g, const I *__restri
   }
 }
 
+template <typename T, typename I, typename O, int NW, int MAXCH>
+__global__ void __launch_bounds__(64 * NW) k_substep(Geo<T> g, SubArgs<T, I, O> sa) {
+  substep_row<T, I, O, NW, MAXCH>(g, sa, blockIdx.x, blockIdx.y);
+}
+
 // ---- sponge (damping.cpp:76-138 with perturbation targets 0) -------------
 
-__global__ void __launch_bounds__(256) k_sponge(int nx, int ny, int nz, int kmin, const double *__restrict__ alpha,
-                                                double dt, double *__restrict__ u, double *__restrict__ v,
-                                                double *__restrict__ w, double *__restrict__ rho,
-                                                double *__restrict__ th, double *__restrict__ pi) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = kmin + static_cast<int>(blockIdx.z);
-  if (i >= nx || k > nz) return;
+__device__ __forceinline__ void sponge_at(int nx, int ny, int nz, const double *__restrict__ alpha, double dt,
+                                          const FieldSet &s, int i, int j, int k) {
+  double *__restrict__ u = s.u;
+  double *__restrict__ v = s.v;
+  double *__restrict__ w = s.w;
+  double *__restrict__ rho = s.rho;
+  double *__restrict__ th = s.th;
+  double *__restrict__ pi = s.pi;
   const double a = alpha[k < nz - 1 ? k : nz - 1];
   if (a <= 0.0) return;
   const double f = 1.0 / (1.0 + dt * a);
@@ -720,6 +764,13 @@ __global__ void __launch_bounds__(256) k_sponge(int nx, int ny, int nz, int kmin
   }
 }
 
+static __global__ void __launch_bounds__(256) k_sponge(int nx, int ny, int nz, int kmin, const double *__restrict__ alpha,
+                                                double dt, FieldSet s) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = kmin + static_cast<int>(blockIdx.z);
+  if (i >= nx || k > nz) return;
+  sponge_at(nx, ny, nz, alpha, dt, s, i, j, k);
+}
+
 // ---- diagnostics ------------------------------------------------------------
 
 constexpr int kRedThreads = 256;
@@ -727,7 +778,7 @@ constexpr int kRedBlocks = 1024;
 constexpr int kEnergyComps = 5;
 
 // Per-cell terms of energy_budget (diagnostics.cpp:255-277), compensated.
-__global__ void __launch_bounds__(kRedThreads) k_energy_partial(
+static __global__ void __launch_bounds__(kRedThreads) k_energy_partial(
     int nx, int ny, int nz, double dx, double dy, double H, const double *__restrict__ zs,
     const double *__restrict__ dzeta, const double *__restrict__ zcc, const double *__restrict__ rho0,
     const double *__restrict__ th0, const double *__restrict__ pi0, const double *__restrict__ u,
@@ -769,7 +820,7 @@ __global__ void __launch_bounds__(kRedThreads) k_energy_partial(
   }
 }
 
-__global__ void __launch_bounds__(kRedThreads) k_red_final(const NSum *__restrict__ part, int nparts, int ncomp,
+static __global__ void __launch_bounds__(kRedThreads) k_red_final(const NSum *__restrict__ part, int nparts, int ncomp,
                                                            double *__restrict__ out) {
   __shared__ NSum sh[kRedThreads];
   for (int comp = 0; comp < ncomp; ++comp) {
@@ -787,7 +838,7 @@ __global__ void __launch_bounds__(kRedThreads) k_red_final(const NSum *__restric
 }
 
 // Advective CFL of the state (advection.cpp:325-340) and max wind speed.
-__global__ void __launch_bounds__(256) k_cfl_state(int nx, int ny, int nz, double dx, double dy, double H,
+static __global__ void __launch_bounds__(256) k_cfl_state(int nx, int ny, int nz, double dx, double dy, double H,
                                                    const double *__restrict__ zs, const double *__restrict__ dzeta,
                                                    const double *__restrict__ u, const double *__restrict__ v,
                                                    const double *__restrict__ w, double dt,

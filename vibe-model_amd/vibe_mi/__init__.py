@@ -106,6 +106,14 @@ def lib() -> ctypes.CDLL:
         "vibe_mi_domain_cfl": (_I, [_VP, _D, _PD, _PD]),
         "vibe_mi_domain_stream": (_VP, [_VP]),
         "vibe_mi_domain_device_bytes": (_SZ, [_VP]),
+        "vibe_mi_batch_create": (_I, [ctypes.POINTER(DomainDesc), _I, ctypes.POINTER(_VP)]),
+        "vibe_mi_batch_destroy": (_I, [_VP]),
+        "vibe_mi_batch_member": (_I, [_VP, _I, ctypes.POINTER(_VP)]),
+        "vibe_mi_batch_step": (_I, [_VP, _PD, _I, _I, _VP]),
+        "vibe_mi_batch_step_timed": (_I, [_VP, _PD, _I, _I, _VP, _PD]),
+        "vibe_mi_batch_set_graph": (_I, [_VP, _I]),
+        "vibe_mi_batch_tuning": (_I, [_VP, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+        "vibe_mi_batch_device_bytes": (_SZ, [_VP]),
         "vibe_mi_zeta_uniform": (_I, [_I, _PD]),
         "vibe_mi_zeta_tanh": (_I, [_I, _D, _D, _D, _PD]),
         "vibe_mi_terrain_agnesi_ridge": (_I, [_I, _I, _D, _D, _D, _PD]),
@@ -264,6 +272,16 @@ class Case:
         return {k: v.copy() for k, v in self.state.items()}
 
 
+def _desc_of(case: Case, precision: int):
+    """DomainDesc of a case plus the arrays it points to (keep them alive)."""
+    keep = [np.ascontiguousarray(a, dtype=np.float64) for a in
+            (case.zeta, case.zs, case.rho0, case.theta0, case.pi0, case.sponge)]
+    z, zs, r0, t0, p0, sp = keep
+    desc = DomainDesc(case.nx, case.ny, case.nz, case.dx, case.dy, case.ztop,
+                      _pd(z), _pd(zs), _pd(r0), _pd(t0), _pd(p0), _pd(sp), case.eps_s, precision)
+    return desc, keep
+
+
 class Domain:
     """One domain resident on the GPU (RungeKutta3Integrator role)."""
 
@@ -271,20 +289,27 @@ class Domain:
         L = lib()
         self.case = case
         self.precision = precision
-        self._keep = [np.ascontiguousarray(a, dtype=np.float64) for a in
-                      (case.zeta, case.zs, case.rho0, case.theta0, case.pi0, case.sponge)]
-        z, zs, r0, t0, p0, sp = self._keep
-        self.desc = DomainDesc(case.nx, case.ny, case.nz, case.dx, case.dy, case.ztop,
-                               _pd(z), _pd(zs), _pd(r0), _pd(t0), _pd(p0), _pd(sp),
-                               case.eps_s, precision)
+        self.owner = None
+        self.desc, self._keep = _desc_of(case, precision)
         h = _VP()
         _check(L.vibe_mi_domain_create(ctypes.byref(self.desc), ctypes.byref(h)), "vibe_mi_domain_create")
         self.h = h
         self.set_state(case.state)
 
+    @classmethod
+    def _member(cls, case: Case, precision: int, handle, owner):
+        """Non-owning view of a batch member."""
+        self = cls.__new__(cls)
+        self.case = case
+        self.precision = precision
+        self.owner = owner
+        self.h = handle
+        return self
+
     def close(self):
         if getattr(self, "h", None):
-            lib().vibe_mi_domain_destroy(self.h)
+            if self.owner is None:
+                lib().vibe_mi_domain_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -347,3 +372,87 @@ class Domain:
 
     def device_bytes(self) -> int:
         return lib().vibe_mi_domain_device_bytes(self.h)
+
+
+class Batch:
+    """Many domains of uneven size advanced together: one persistent launch
+    per kernel phase over a device tile work list, the step's launch
+    sequence replayed from a hipGraph (vibe_mi/batch.h)."""
+
+    def __init__(self, cases, precision: int = FP64, use_graph: bool = True):
+        L = lib()
+        self.cases = list(cases)
+        self.precision = precision
+        n = len(self.cases)
+        descs = (DomainDesc * n)()
+        self._keep = []
+        for m, c in enumerate(self.cases):
+            d, keep = _desc_of(c, precision)
+            descs[m] = d
+            self._keep.append(keep)
+        h = _VP()
+        _check(L.vibe_mi_batch_create(descs, n, ctypes.byref(h)), "vibe_mi_batch_create")
+        self.h = h
+        self._keep = None  # the members hold their own copies of the inputs
+        self.members = []
+        for m, c in enumerate(self.cases):
+            mh = _VP()
+            _check(L.vibe_mi_batch_member(h, m, ctypes.byref(mh)), "vibe_mi_batch_member")
+            dom = Domain._member(c, precision, mh, self)
+            dom.set_state(c.state)
+            self.members.append(dom)
+        self.set_graph(use_graph)
+
+    def __len__(self):
+        return len(self.members)
+
+    def close(self):
+        if getattr(self, "h", None):
+            for m in self.members:
+                m.h = None
+            lib().vibe_mi_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _args(self, dt, active):
+        n = len(self.members)
+        if dt is None:
+            dt = [c.dt for c in self.cases]
+        dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, dtype=np.float64), (n,)))
+        act = None
+        if active is not None:
+            act = np.ascontiguousarray(np.asarray(active, dtype=np.uint8).reshape(n))
+        return dt, act
+
+    def step(self, dt=None, nsub: int = 6, nsteps: int = 1, active=None):
+        dt, act = self._args(dt, active)
+        _check(lib().vibe_mi_batch_step(self.h, _pd(dt), nsub, nsteps,
+                                        None if act is None else act.ctypes.data_as(_VP)), "vibe_mi_batch_step")
+
+    def step_timed(self, dt=None, nsub: int = 6, nsteps: int = 1, active=None) -> float:
+        dt, act = self._args(dt, active)
+        ms = ctypes.c_double()
+        _check(lib().vibe_mi_batch_step_timed(self.h, _pd(dt), nsub, nsteps,
+                                              None if act is None else act.ctypes.data_as(_VP), ctypes.byref(ms)),
+               "vibe_mi_batch_step_timed")
+        return ms.value
+
+    def set_graph(self, use_graph: bool):
+        _check(lib().vibe_mi_batch_set_graph(self.h, 1 if use_graph else 0), "vibe_mi_batch_set_graph")
+
+    def tuning(self):
+        a, b = _I(), _I()
+        _check(lib().vibe_mi_batch_tuning(self.h, ctypes.byref(a), ctypes.byref(b)), "vibe_mi_batch_tuning")
+        return {"substep_waves": a.value, "stage_rows": b.value}
+
+    def device_bytes(self) -> int:
+        return lib().vibe_mi_batch_device_bytes(self.h)
+
+    @property
+    def ncell(self):
+        return sum(c.ncell for c in self.cases)

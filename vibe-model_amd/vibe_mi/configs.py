@@ -10,6 +10,10 @@ state), using the library's host-side setup helpers.
                   U = 10 m/s, N = 0.01 1/s, theta_sfc = 300 K, seeded
                   +/-0.1 K noise below 1 km, dt = 6 s, 6 substeps, 600 steps
   mountain_wave_large  the same at 1024x1024x96
+  ensemble        48 members of uneven size (nx, ny in 96..320 step 32,
+                  nz in {40, 60, 80}), dx = 2 km, N = 0.01 1/s,
+                  U(z) = 15 tanh(z / 3 km), 1-5 seeded warm bubbles each
+                  (returns a list of cases for vibe_mi.Batch)
 
 Every size can be overridden so that tests run the same physics on small
 grids.
@@ -120,6 +124,48 @@ def random_terrain_case(nx=64, ny=48, nz=24, dx=2000.0, ztop=16000.0, sigma=300.
                                   _pd(st["theta"])), "ic_noise")
     _balance(st, r0, t0)
     return Case("random_terrain", nx, ny, nz, dx, dx, ztop, zeta, zs, r0, t0, p0, sp, st, dt, nsub, steps)
+
+
+def ensemble_member(nx, ny, nz, seed, dx=2000.0, ztop=20000.0, bv=0.01, theta_sfc=300.0, umax=15.0,
+                    hshear=3000.0, dt=8.0, nsub=6, steps=300, name="ensemble_member") -> Case:
+    """One ensemble member: flat terrain, uniform levels, constant-N base
+    state, U(z) = umax tanh(z / hshear), 1-5 cos^2 warm bubbles of 1-3 K and
+    2-6 km radius at seeded positions (numpy PCG64 seeded with `seed`)."""
+    rng = np.random.default_rng(seed)
+    zeta = _levels(nz, ztop, "uniform")
+    zs = np.zeros(nx * ny)
+    r0, t0, p0 = _reference(1, theta_sfc, bv, nx, ny, nz, ztop, zeta, zs)
+    sp = _sponge(nz, ztop, zeta, 0.0, 0.75, 0.1)
+    st = _zero_state(nx, ny, nz)
+    _check(lib().vibe_mi_ic_wind(nx, ny, nz, ztop, _pd(zeta), _pd(zs), 0.0, umax, hshear, _pd(st["u"])), "ic_wind")
+    nb = int(rng.integers(1, 6))
+    bubbles = []
+    for _ in range(nb):
+        amp = float(rng.uniform(1.0, 3.0))
+        r = float(rng.uniform(2000.0, 6000.0))
+        xc = float(rng.uniform(0.0, nx * dx))
+        yc = float(rng.uniform(0.0, ny * dx))
+        zc = float(rng.uniform(2000.0, 6000.0))
+        bubbles.append((xc, yc, zc, r, amp))
+        _check(lib().vibe_mi_ic_bubble(nx, ny, nz, dx, dx, ztop, _pd(zeta), _pd(zs), xc, yc, zc, r, r, amp,
+                                       _pd(st["theta"])), "ic_bubble")
+    _balance(st, r0, t0)
+    c = Case(name, nx, ny, nz, dx, dx, ztop, zeta, zs, r0, t0, p0, sp, st, dt, nsub, steps)
+    c.meta.update(seed=seed, bubbles=bubbles)
+    return c
+
+
+def ensemble(n=48, seed=20240611, sizes=tuple(range(96, 321, 32)), levels=(40, 60, 80), **kw):
+    """n members of uneven size: nx, ny drawn per axis from `sizes`, nz from
+    `levels` (PCG64 seeded with `seed`; member m uses seed + 1 + m)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for m in range(n):
+        nx = int(rng.choice(sizes))
+        ny = int(rng.choice(sizes))
+        nz = int(rng.choice(levels))
+        out.append(ensemble_member(nx, ny, nz, seed + 1 + m, name=f"ensemble{m:02d}", **kw))
+    return out
 
 
 CONFIGS = {
