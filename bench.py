@@ -2,27 +2,32 @@
 
 Workload (default): the 1024x1024x96 mountain-wave configuration (Witch-of-
 Agnesi ridge, N = 0.01 1/s, U = 10 m/s, dt = 6 s, 6 acoustic substeps per RK3
-stage).  The headline value is the FP32-mixed step (FP64 state and RK3
-accumulation, FP32 tendencies/substeps, FP64 compensated reductions) in
-cell-updates/s; the plain FP64 step is timed beside it on the same inputs.
+stage).  --config ensemble runs the 48-member uneven ensemble (nx, ny in
+96..320, nz in 40/60/80, dx = 2 km, dt = 8 s) as one batch.  The headline
+value is the FP32-mixed batched step (FP64 state and RK3 accumulation, FP32
+tendencies/substeps, FP64 compensated reductions; one persistent launch per
+kernel phase over the tile work list of all members, each step replayed from
+a hipGraph) in cell-updates/s.  Beside it: the plain FP64 batched step on the
+same inputs, and every member stepped alone through the single-domain path.
 
-One "step" = one WS-RK3 step of the whole domain (3 stages, 3 x 6 substeps,
+One "step" = one WS-RK3 step of every member (3 stages, 3 x 6 substeps,
 sponge).  Inputs are resident in HBM before timing starts.  Timing: W untimed
 warm-up steps, then K steps bracketed by device synchronisation, measured
-with HIP events on the domain's stream; per-kernel times from events around
-each stage kernel group and each substep group.
+with HIP events on the batch stream.  Per-group kernel times come from a
+second, eager (no graph) run with HIP events around each stage's Omega +
+stage kernel and around its substep launches.
 
 Roofline: algorithmic bytes per cell-update (compulsory prognostic traffic)
   FP64   8 * (48 + 30 * n_s) = 1824 B     (n_s = 6)
   mixed  8 * 48 + 4 * 30 * n_s = 1104 B
 against 8.0 TB/s.  The dominant kernel is the fused acoustic substep
-k_substep (3 n_s launches per step, about 60 % of the step).  Its algorithmic
+kb_substep (3 n_s launches per step, about 60 % of the step).  Its algorithmic
 bytes per launch: read u, v, w, rho', pi' and write them once, 4 B each in
 the FP32-mixed path (40 B per cell), except the last substep of a stage,
 which writes its FP64 stage output (20 + 40 = 60 B per cell); averaged over
 the n_s launches of a stage: (40 (n_s - 1) + 60) / n_s B per cell.  The
-launch duration is measured with HIP events on the domain's stream around
-each stage's substep group (only k_substep launches are in the group).
+launch duration is measured with HIP events on the batch stream around each
+stage's substep group (only kb_substep launches are in the group).
 "traffic" (HBM bytes per launch) comes from the rocprofv3 FETCH_SIZE /
 WRITE_SIZE per-dispatch summary committed for the same workload, each
 scaled by what the counter reads for a plain 1 GiB copy (--copy-calibration).
@@ -56,10 +61,14 @@ def substep_bytes_per_launch(cells: int, nsub: int) -> float:
     return cells * (40.0 * (nsub - 1) + 60.0) / nsub
 
 
-def measured_traffic(case):
-    """HBM bytes per k_substep launch from the committed rocprofv3 summary of
+def measured_traffic(cases):
+    """HBM bytes per kb_substep launch from the committed rocprofv3 summary of
     this workload (FETCH_SIZE + WRITE_SIZE per dispatch, copy-calibrated)."""
-    name = f"k_substep_{case.name}_{case.nx}x{case.ny}x{case.nz}_hbm_traffic.txt"
+    if len(cases) == 1:
+        c = cases[0]
+        name = f"kb_substep_{c.name}_{c.nx}x{c.ny}x{c.nz}_hbm_traffic.txt"
+    else:
+        name = f"kb_substep_ensemble{len(cases)}_hbm_traffic.txt"
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None, None
@@ -90,18 +99,33 @@ def copy_calibration(nbytes: int = 1 << 30, reps: int = 3):
     print(json.dumps({"copy_bytes": nbytes, "reps": reps, "ok": bool(torch.equal(a, b))}))
 
 
-def run_gpu(case, precision, steps, warmup):
+def run_batch(cases, precision, steps, warmup, graph=True):
+    """Batched path (persistent tile-list kernels, hipGraph replay)."""
     import vibe_mi
 
-    dom = vibe_mi.Domain(case, precision)
+    b = vibe_mi.Batch(cases, precision, use_graph=graph)
+    nsub = cases[0].nsub
     if warmup:
-        dom.step(case.dt, case.nsub, warmup)
-    ms = dom.step_timed(case.dt, case.nsub, steps)
-    kt = dom.kernel_times()
-    e = dom.energy_budget()
-    nbytes = dom.device_bytes()
-    dom.close()
-    return ms, kt, e, nbytes
+        b.step(None, nsub, warmup)
+    ms = b.step_timed(None, nsub, steps)
+    out = {"ms": ms, "kt": b.kernel_times(), "tuning": b.tuning(), "device_bytes": b.device_bytes()}
+    b.close()
+    return out
+
+
+def run_alone(cases, precision, steps, warmup):
+    """Each member stepped alone through the single-domain path, one after
+    the other; returns the summed timed milliseconds."""
+    import vibe_mi
+
+    total = 0.0
+    for c in cases:
+        d = vibe_mi.Domain(c, precision)
+        if warmup:
+            d.step(c.dt, c.nsub, warmup)
+        total += d.step_timed(c.dt, c.nsub, steps)
+        d.close()
+    return total
 
 
 def cpu_baseline(name, nsub, threads):
@@ -111,7 +135,12 @@ def cpu_baseline(name, nsub, threads):
     from vibe_mi import configs
 
     nxs = 96
-    case = configs.CONFIGS[name](nx=nxs, ny=nxs)
+    if name == "ensemble":
+        case = configs.ensemble_member(nxs, nxs, 60, seed=20240612)
+        what = f"one ensemble member at {nxs}x{nxs}x60"
+    else:
+        case = configs.CONFIGS[name](nx=nxs, ny=nxs)
+        what = f"{nxs}x{nxs}x{case.nz} columns of the same configuration"
     orc = OracleDycore(case, threads=threads)
     orc.step(case.dt, nsub, 1)  # warm
     t0 = time.perf_counter()
@@ -124,7 +153,7 @@ def cpu_baseline(name, nsub, threads):
             break
     cups = case.ncell * n / el
     orc.close()
-    return cups, f"{nxs}x{nxs}x{case.nz} columns of the same configuration, {n} RK3 steps, FP64 oracle"
+    return cups, f"{what}, {n} RK3 steps, FP64 oracle"
 
 
 def main():
@@ -132,10 +161,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="mountain_wave_large")
+    ap.add_argument("--config", default="mountain_wave_large",
+                    help="mountain_wave_large (default), mountain_wave, warm_bubble, ensemble")
     ap.add_argument("--nx", type=int, default=0)
+    ap.add_argument("--members", type=int, default=48, help="ensemble size")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-alone", action="store_true", help="skip the unbatched comparison")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -150,37 +182,51 @@ def main():
     import vibe_mi
     from vibe_mi import configs
 
-    kw = {}
-    if args.nx:
-        kw = {"nx": args.nx, "ny": args.nx}
     t0 = time.perf_counter()
-    case = configs.CONFIGS[args.config](**kw)
-    log(f"[bench] built {case.name} {case.nx}x{case.ny}x{case.nz} in {time.perf_counter() - t0:.1f}s")
-    cells = case.ncell
+    if args.config == "ensemble":
+        cases = configs.ensemble(n=args.members)
+        wl = (f"ensemble of {len(cases)} members (nx, ny in 96..320, nz in 40/60/80), dx=2 km, "
+              f"dt={cases[0].dt}s, {cases[0].nsub} substeps/stage")
+    else:
+        kw = {"nx": args.nx, "ny": args.nx} if args.nx else {}
+        c = configs.CONFIGS[args.config](**kw)
+        cases = [c]
+        wl = f"{c.name} {c.nx}x{c.ny}x{c.nz}, dt={c.dt}s, {c.nsub} substeps/stage"
+    nsub = cases[0].nsub
+    cells = sum(c.ncell for c in cases)
+    log(f"[bench] built {wl}: {cells} cells in {time.perf_counter() - t0:.1f}s")
 
-    log("[bench] FP32-mixed timed run")
-    ms_m, kt_m, e_m, mem_m = run_gpu(case, vibe_mi.MIXED, args.steps, args.warmup)
-    cups_m = cells * args.steps / (ms_m * 1e-3)
-    log(f"[bench] mixed: {ms_m / args.steps:.2f} ms/step, {cups_m:.3e} cell-updates/s")
+    log("[bench] FP32-mixed batched (hipGraph) timed run")
+    rm = run_batch(cases, vibe_mi.MIXED, args.steps, args.warmup, graph=True)
+    cups_m = cells * args.steps / (rm["ms"] * 1e-3)
+    log(f"[bench] mixed: {rm['ms'] / args.steps:.2f} ms/step, {cups_m:.3e} cell-updates/s, tiles {rm['tuning']}")
+    log("[bench] FP32-mixed batched eager run with per-group HIP events")
+    re_ = run_batch(cases, vibe_mi.MIXED, args.steps, 1, graph=False)
+    kt_m = re_["kt"]
     res64 = None
     if not args.no_fp64:
-        log("[bench] FP64 timed run")
-        ms_d, kt_d, e_d, mem_d = run_gpu(case, vibe_mi.FP64, args.steps, args.warmup)
-        cups_d = cells * args.steps / (ms_d * 1e-3)
-        log(f"[bench] fp64: {ms_d / args.steps:.2f} ms/step, {cups_d:.3e} cell-updates/s")
-        res64 = {"value": cups_d, "ms_per_step": ms_d / args.steps,
-                 "hbm_frac": cups_d * bytes_per_cell_update(case.nsub, False) / HBM_PEAK,
-                 "ms_stage_per_launch_group": kt_d["ms_stage"] / max(kt_d["n_stage"], 1),
-                 "ms_substep_per_launch_group": kt_d["ms_substep"] / max(kt_d["n_substep"], 1)}
+        log("[bench] FP64 batched (hipGraph) timed run")
+        rd = run_batch(cases, vibe_mi.FP64, args.steps, args.warmup, graph=True)
+        cups_d = cells * args.steps / (rd["ms"] * 1e-3)
+        log(f"[bench] fp64: {rd['ms'] / args.steps:.2f} ms/step, {cups_d:.3e} cell-updates/s")
+        res64 = {"value": cups_d, "ms_per_step": rd["ms"] / args.steps,
+                 "hbm_frac": cups_d * bytes_per_cell_update(nsub, False) / HBM_PEAK, "tiles": rd["tuning"]}
+    alone = None
+    if not args.no_alone:
+        log("[bench] each member alone (single-domain path), FP32-mixed")
+        ms_a = run_alone(cases, vibe_mi.MIXED, args.steps, args.warmup)
+        alone = {"value": cells * args.steps / (ms_a * 1e-3), "ms_per_step": ms_a / args.steps,
+                 "batched_speedup": ms_a / rm["ms"]}
+        log(f"[bench] alone: {ms_a / args.steps:.2f} ms/step; batched speedup {ms_a / rm['ms']:.3f}x")
 
-    # dominant kernel: the fused acoustic substep (k_substep), FP32-mixed
+    # dominant kernel: the fused acoustic substep (kb_substep), FP32-mixed
     sub_ms = kt_m["ms_substep"] / max(kt_m["n_substep"], 1)
-    sub_bytes = substep_bytes_per_launch(cells, case.nsub)
+    sub_bytes = sum(substep_bytes_per_launch(c.ncell, nsub) for c in cases)
     achieved = sub_bytes / (sub_ms * 1e-3) / 1e9
-    traffic, traffic_src = measured_traffic(case)
+    traffic, traffic_src = measured_traffic(cases)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved * 1e9 / HBM_PEAK, "traffic": traffic,
-                "kernel": "k_substep (fused HEVI acoustic substep, FP32)",
+                "kernel": "kb_substep (fused HEVI acoustic substep, FP32, batched persistent)",
                 "algorithmic_bytes_per_launch": sub_bytes, "avg_launch_ms": sub_ms,
                 "traffic_source": traffic_src}
 
@@ -188,7 +234,7 @@ def main():
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log(f"[bench] CPU oracle baseline on {threads} threads")
-        cpu_cups, sample = cpu_baseline(args.config, case.nsub, threads)
+        cpu_cups, sample = cpu_baseline(args.config, nsub, threads)
         cpu = {"value": cpu_cups, "unit": "cell-updates/s", "cores": threads, "kind": "port", "sample": sample}
 
     out = {
@@ -198,23 +244,24 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_m / args.steps,
+        "ms_per_step": rm["ms"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 compute, f64 state",
-        "data": "synthetic (seeded terrain/noise initial state)",
-        "config": {"workload": f"{case.name} {case.nx}x{case.ny}x{case.nz}, dt={case.dt}s, "
-                               f"{case.nsub} substeps/stage", "cells": cells, "parallelism": "single GPU"},
-        "hbm_frac_step": cups_m * bytes_per_cell_update(case.nsub, True) / HBM_PEAK,
-        "bytes_per_cell_update": bytes_per_cell_update(case.nsub, True),
+        "data": "synthetic (seeded terrain/noise/bubble initial states)",
+        "config": {"workload": wl, "cells": cells, "members": len(cases), "parallelism": "single GPU"},
+        "hbm_frac_step": cups_m * bytes_per_cell_update(nsub, True) / HBM_PEAK,
+        "bytes_per_cell_update": bytes_per_cell_update(nsub, True),
+        "tiles": rm["tuning"],
         "fp64": res64,
         "speedup_mixed_vs_fp64": (cups_m / res64["value"]) if res64 else None,
+        "unbatched_mixed": alone,
         "kernel_ms": {"stage_group_per_stage": kt_m["ms_stage"] / max(kt_m["n_stage"], 1),
                       "substep_per_launch": sub_ms},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "device_bytes": mem_m,
+        "device_bytes": rm["device_bytes"],
     }
     print(json.dumps(out))
 

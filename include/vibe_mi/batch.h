@@ -45,6 +45,11 @@ int vibe_mi_batch_step(vibe_mi_batch *b, const double *dt, int nsub, int nsteps,
 /* Same, timed with HIP events on the batch stream (total ms). */
 int vibe_mi_batch_step_timed(vibe_mi_batch *b, const double *dt, int nsub, int nsteps,
                              const unsigned char *active, double *ms_total);
+/* Per-group times of the last timed call made without a graph (ms summed
+ * over the call): Omega + stage kernel per stage, and the stage's substep
+ * launches, measured with HIP events on the batch stream. */
+int vibe_mi_batch_kernel_times(vibe_mi_batch *b, double *ms_stage, double *ms_substep, int *n_stage,
+                               int *n_substep);
 /* use_graph = 1 (default): a step's launch sequence is captured once per
  * (active set, nsub) in a hipGraph and replayed; 0: launched eagerly. */
 int vibe_mi_batch_set_graph(vibe_mi_batch *b, int use_graph);

@@ -54,20 +54,44 @@ struct BDom {
   int *flag;
 };
 
-// Persistent tile loop: one lane claims the next tile while the workgroup
-// works on the current one; the claimed index is broadcast through LDS.
-// Every thread of the workgroup runs body(t) for the same t.
+// A tile work list split into one contiguous segment per XCD: seg[x] ..
+// seg[x+1] are the tiles workgroups on XCD x take first, so that tiles that
+// share halo rows are processed under the same L2.
+constexpr int kXcds = 8;
+struct TileList {
+  const int4 *tiles;
+  const int *seg;  // kXcds + 1 segment starts, seg[kXcds] = ntiles
+  int ntiles;
+};
+
+// Persistent tile loop.  Workgroups are dispatched to the XCDs round-robin
+// by workgroup id, so workgroup b serves segment b % 8 first, then steals
+// from the other segments in turn until every counter is past its segment.
+// One lane claims the next tile while the workgroup works on the current
+// one; the claimed index is broadcast through LDS.  Every thread of the
+// workgroup runs body(t) for the same t.  ctr: kXcds counters, zero at launch.
 template <typename F>
-__device__ __forceinline__ void tile_loop(int *__restrict__ ctr, int ntiles, F &&body) {
+__device__ __forceinline__ void tile_loop(int *__restrict__ ctr, const TileList &L, F &&body) {
   __shared__ int s_next[2];
   const bool leader = threadIdx.x == 0 && threadIdx.y == 0;
-  if (leader) s_next[0] = atomicAdd(ctr, 1);
+  const int home = static_cast<int>(blockIdx.x % kXcds);
+  int qk = 0;  // segments exhausted so far (leader only)
+  auto claim = [&]() {
+    while (qk < kXcds) {
+      const int x = (home + qk) % kXcds;
+      const int t = L.seg[x] + atomicAdd(ctr + x, 1);
+      if (t < L.seg[x + 1]) return t;
+      ++qk;
+    }
+    return L.ntiles;
+  };
+  if (leader) s_next[0] = claim();
   __syncthreads();
   int par = 0;
   for (;;) {
     const int t = __builtin_amdgcn_readfirstlane(s_next[par]);
-    if (t >= ntiles) break;
-    if (leader) s_next[par ^ 1] = atomicAdd(ctr, 1);
+    if (t >= L.ntiles) break;
+    if (leader) s_next[par ^ 1] = claim();
     body(t);
     __syncthreads();
     par ^= 1;
@@ -91,10 +115,10 @@ __global__ void __launch_bounds__(64) kb_advance(BDom<T> *__restrict__ D, const 
 
 // Omega over row tiles {member, x-block, j, -}; 64 x 4 threads, 4 levels at a time.
 template <typename T>
-__global__ void __launch_bounds__(256) kb_omega(const BDom<T> *__restrict__ D, const int4 *__restrict__ tiles,
-                                                int ntiles, int st, int *__restrict__ ctr) {
-  tile_loop(ctr, ntiles, [&](int t) {
-    const int4 tl = tiles[t];
+__global__ void __launch_bounds__(256) kb_omega(const BDom<T> *__restrict__ D, TileList L, int st,
+                                                int *__restrict__ ctr) {
+  tile_loop(ctr, L, [&](int t) {
+    const int4 tl = L.tiles[t];
     const BDom<T> &dd = D[tl.x];
     const Geo<T> g = dd.g;
     const FieldSet cur = dd.S[(dd.cur0 + st) % 3];
@@ -106,10 +130,10 @@ __global__ void __launch_bounds__(256) kb_omega(const BDom<T> *__restrict__ D, c
 
 // Fused stage kernel over tiles {member, x-block, y-block, -} of 64 x TY columns.
 template <typename T, int TY>
-__global__ void __launch_bounds__(64 * TY) kb_stage(const BDom<T> *__restrict__ D, const int4 *__restrict__ tiles,
-                                                    int ntiles, int st, int *__restrict__ ctr) {
-  tile_loop(ctr, ntiles, [&](int t) {
-    const int4 tl = tiles[t];
+__global__ void __launch_bounds__(64 * TY) kb_stage(const BDom<T> *__restrict__ D, TileList L, int st,
+                                                    int *__restrict__ ctr) {
+  tile_loop(ctr, L, [&](int t) {
+    const int4 tl = L.tiles[t];
     const BDom<T> &dd = D[tl.x];
     const Geo<T> g = dd.g;
     const int c0 = dd.cur0;
@@ -127,10 +151,10 @@ __global__ void __launch_bounds__(64 * TY) kb_stage(const BDom<T> *__restrict__ 
 // stage (last = 1, O = double) writes the stage output S[out] and checks
 // finiteness; the others write the acoustic buffer A[(q+1)&1] (O = T).
 template <typename T, typename O, int NW, int MAXCH>
-__global__ void __launch_bounds__(64 * NW) kb_substep(const BDom<T> *__restrict__ D, const int4 *__restrict__ tiles,
-                                                      int ntiles, int st, int q, int last, int *__restrict__ ctr) {
-  tile_loop(ctr, ntiles, [&](int t) {
-    const int4 tl = tiles[t];
+__global__ void __launch_bounds__(64 * NW) kb_substep(const BDom<T> *__restrict__ D, TileList L, int st, int q,
+                                                      int last, int *__restrict__ ctr) {
+  tile_loop(ctr, L, [&](int t) {
+    const int4 tl = L.tiles[t];
     const BDom<T> &dd = D[tl.x];
     const Geo<T> g = dd.g;
     const AcousticSet<T> in = dd.A[q & 1];
@@ -180,10 +204,10 @@ __global__ void __launch_bounds__(64 * NW) kb_substep(const BDom<T> *__restrict_
 
 // Sponge over row tiles of the members' final stage output.
 template <typename T>
-__global__ void __launch_bounds__(256) kb_sponge(const BDom<T> *__restrict__ D, const int4 *__restrict__ tiles,
-                                                 int ntiles, int *__restrict__ ctr) {
-  tile_loop(ctr, ntiles, [&](int t) {
-    const int4 tl = tiles[t];
+__global__ void __launch_bounds__(256) kb_sponge(const BDom<T> *__restrict__ D, TileList L,
+                                                 int *__restrict__ ctr) {
+  tile_loop(ctr, L, [&](int t) {
+    const int4 tl = L.tiles[t];
     const BDom<T> &dd = D[tl.x];
     const int nx = dd.g.nx, ny = dd.g.ny, nz = dd.g.nz;
     const FieldSet fin = dd.S[(dd.cur0 + 1) % 3];
@@ -204,18 +228,16 @@ using namespace vibe_mi;
 
 namespace {
 
-constexpr int kMaxCounters = 256;  // work-queue counters per step (3 stages x (2 + nsub) + 1)
+constexpr int kMaxLaunches = 128;                 // persistent launches per step: 3 (2 + nsub) + 1
+constexpr int kMaxCounters = kMaxLaunches * kXcds;  // one counter per XCD segment and launch
 constexpr int kMaxNsub = 64;
 
 struct WorkList {
-  int4 *rows = nullptr;  // {member, x-block, j}
-  int nrows = 0;
-  int4 *stage[2] = {nullptr, nullptr};  // tiles for TY = 2, 4
-  int nstage[2] = {0, 0};
-  int *ids = nullptr;  // active members
+  TileList rows{};       // {member, x-block, j}
+  TileList stage[2]{};   // {member, x-block, y-block} for TY = 2, 4
+  TileList sponge{};     // rows of members with a sponge
+  int *ids = nullptr;    // active members
   int nids = 0;
-  int4 *sponge_rows = nullptr;  // rows of members with a sponge
-  int nsponge = 0;
 };
 
 }  // namespace
@@ -240,9 +262,11 @@ struct vibe_mi_batch {
   int nw = 8;  // waves per substep workgroup
   int ty = 4;  // rows per stage workgroup
   int cus = 256;
-  std::vector<double> dt_cur;
-  int nsub_cur = -1;
-  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  // per-group kernel times of the last timed eager call (ms, summed)
+  bool group_timing = false;
+  double ms_stage = 0, ms_sub = 0;
+  int n_stage = 0, n_sub = 0;
 };
 
 namespace {
@@ -266,6 +290,26 @@ int bupload(vibe_mi_batch *b, P **out, const std::vector<P> &h) {
 }
 
 std::string key_of(const std::vector<unsigned char> &act) { return std::string(act.begin(), act.end()); }
+
+// Uploads a tile list with its XCD segments: contiguous runs of near-equal
+// cost (cost of a tile = levels of its member).
+int upload_list(vibe_mi_batch *b, const std::vector<int4> &t, TileList *out) {
+  std::vector<double> cum(t.size() + 1, 0.0);
+  for (size_t q = 0; q < t.size(); ++q) cum[q + 1] = cum[q] + b->doms[t[q].x]->nz;
+  std::vector<int> seg(kXcds + 1, 0);
+  for (int x = 1; x < kXcds; ++x) {
+    const double target = cum.back() * x / kXcds;
+    seg[x] = static_cast<int>(std::lower_bound(cum.begin(), cum.end(), target) - cum.begin());
+    seg[x] = std::max(seg[x], seg[x - 1]);
+  }
+  seg[kXcds] = static_cast<int>(t.size());
+  int4 *dt = nullptr;
+  int *ds = nullptr;
+  if (int rc = bupload(b, &dt, t)) return rc;
+  if (int rc = bupload(b, &ds, seg)) return rc;
+  *out = TileList{dt, ds, static_cast<int>(t.size())};
+  return 0;
+}
 
 // Builds (or returns the cached) work lists of an active set.  Members are
 // visited deepest column first so the longest tiles are claimed first.
@@ -295,15 +339,11 @@ int work_list(vibe_mi_batch *b, const std::vector<unsigned char> &act, WorkList 
       for (int bx = 0; bx < nbx; ++bx) st4.push_back(make_int4(m, bx, by, 0));
   }
   WorkList w;
-  if (int rc = bupload(b, &w.rows, rows)) return rc;
-  if (int rc = bupload(b, &w.sponge_rows, sp)) return rc;
-  if (int rc = bupload(b, &w.stage[0], st2)) return rc;
-  if (int rc = bupload(b, &w.stage[1], st4)) return rc;
+  if (int rc = upload_list(b, rows, &w.rows)) return rc;
+  if (int rc = upload_list(b, sp, &w.sponge)) return rc;
+  if (int rc = upload_list(b, st2, &w.stage[0])) return rc;
+  if (int rc = upload_list(b, st4, &w.stage[1])) return rc;
   if (int rc = bupload(b, &w.ids, order)) return rc;
-  w.nrows = static_cast<int>(rows.size());
-  w.nsponge = static_cast<int>(sp.size());
-  w.nstage[0] = static_cast<int>(st2.size());
-  w.nstage[1] = static_cast<int>(st4.size());
   w.nids = static_cast<int>(order.size());
   *out = &(b->lists[key] = w);
   return 0;
@@ -383,8 +423,8 @@ int launch_substep_nw(vibe_mi_batch *b, const WorkList &w, int st, int q, int la
     }                                                                                                         \
     if (dry) break;                                                                                           \
     unsigned grid = 0;                                                                                        \
-    if (int rc = resident(b, kern, 64 * NW, lds, w.nrows, &grid)) return rc;                                  \
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * NW), lds, b->stream, D, w.rows, w.nrows, st, q, last, ctr); \
+    if (int rc = resident(b, kern, 64 * NW, lds, w.rows.ntiles, &grid)) return rc;                            \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * NW), lds, b->stream, D, w.rows, st, q, last, ctr);         \
   } while (0)
   if constexpr (NW == 16) {
     if (ch <= 4) VMI_BSUB(4);
@@ -423,8 +463,8 @@ int launch_stage_ty(vibe_mi_batch *b, const WorkList &w, int st, int *ctr) {
   const int li = TY == 2 ? 0 : 1;
   auto *kern = &kb_stage<T, TY>;
   unsigned grid = 0;
-  if (int rc = resident(b, kern, 64 * TY, 0, w.nstage[li], &grid)) return rc;
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(64, TY), 0, b->stream, D, w.stage[li], w.nstage[li], st, ctr);
+  if (int rc = resident(b, kern, 64 * TY, 0, w.stage[li].ntiles, &grid)) return rc;
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(64, TY), 0, b->stream, D, w.stage[li], st, ctr);
   VMI_HIP(hipGetLastError());
   return 0;
 }
@@ -442,31 +482,45 @@ int enqueue_step(vibe_mi_batch *b, const WorkList &w, int nsub) {
   int *ctr = b->ctr;
   int c = 0;
   hipLaunchKernelGGL(kb_reset, dim3(1), dim3(64), 0, b->stream, ctr, kMaxCounters);
+  const bool gt = b->group_timing;
   for (int st = 0; st < 3; ++st) {
+    if (gt) VMI_HIP(hipEventRecord(b->ev[2], b->stream));
     {
       auto *kern = &kb_omega<T>;
       unsigned grid = 0;
-      if (int rc = resident(b, kern, 256, 0, w.nrows, &grid)) return rc;
-      hipLaunchKernelGGL(kern, dim3(grid), dim3(64, 4), 0, b->stream, D, w.rows, w.nrows, st, ctr + c++);
+      if (int rc = resident(b, kern, 256, 0, w.rows.ntiles, &grid)) return rc;
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(64, 4), 0, b->stream, D, w.rows, st, ctr + kXcds * c++);
     }
-    if (int rc = launch_stage<T>(b, b->ty, w, st, ctr + c++)) return rc;
+    if (int rc = launch_stage<T>(b, b->ty, w, st, ctr + kXcds * c++)) return rc;
+    if (gt) VMI_HIP(hipEventRecord(b->ev[3], b->stream));
     for (int q = 0; q < nsub; ++q) {
       const bool last = q == nsub - 1;
       int rc;
       if constexpr (sizeof(T) == 4) {
-        rc = last ? launch_substep<T, double>(b, b->nw, w, st, q, 1, ctr + c++)
-                  : launch_substep<T, T>(b, b->nw, w, st, q, 0, ctr + c++);
+        rc = last ? launch_substep<T, double>(b, b->nw, w, st, q, 1, ctr + kXcds * c++)
+                  : launch_substep<T, T>(b, b->nw, w, st, q, 0, ctr + kXcds * c++);
       } else {
-        rc = launch_substep<T, T>(b, b->nw, w, st, q, last ? 1 : 0, ctr + c++);
+        rc = launch_substep<T, T>(b, b->nw, w, st, q, last ? 1 : 0, ctr + kXcds * c++);
       }
       if (rc) return rc;
     }
+    if (gt) {
+      VMI_HIP(hipEventRecord(b->ev[4], b->stream));
+      VMI_HIP(hipEventSynchronize(b->ev[4]));
+      float a = 0.f, x = 0.f;
+      VMI_HIP(hipEventElapsedTime(&a, b->ev[2], b->ev[3]));
+      VMI_HIP(hipEventElapsedTime(&x, b->ev[3], b->ev[4]));
+      b->ms_stage += a;
+      b->ms_sub += x;
+      b->n_stage += 1;
+      b->n_sub += nsub;
+    }
   }
-  if (w.nsponge > 0) {
+  if (w.sponge.ntiles > 0) {
     auto *kern = &kb_sponge<T>;
     unsigned grid = 0;
-    if (int rc = resident(b, kern, 256, 0, w.nsponge, &grid)) return rc;
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(64, 4), 0, b->stream, D, w.sponge_rows, w.nsponge, ctr + c++);
+    if (int rc = resident(b, kern, 256, 0, w.sponge.ntiles, &grid)) return rc;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64, 4), 0, b->stream, D, w.sponge, ctr + kXcds * c++);
   }
   hipLaunchKernelGGL((kb_advance<T>), dim3(1), dim3(64), 0, b->stream, static_cast<BDom<T> *>(b->desc), w.ids, w.nids);
   VMI_HIP(hipGetLastError());
@@ -528,9 +582,9 @@ int autotune(vibe_mi_batch *b, const WorkList &w) {
       hipLaunchKernelGGL(kb_reset, dim3(1), dim3(64), 0, b->stream, b->ctr, kMaxCounters);
       auto *kern = &kb_omega<T>;
       unsigned grid = 0;
-      if (int rc = resident(b, kern, 256, 0, w.nrows, &grid)) return rc;
+      if (int rc = resident(b, kern, 256, 0, w.rows.ntiles, &grid)) return rc;
       hipLaunchKernelGGL(kern, dim3(grid), dim3(64, 4), 0, b->stream, static_cast<const BDom<T> *>(b->desc), w.rows,
-                         w.nrows, 0, b->ctr);
+                         0, b->ctr);
       VMI_HIP(hipGetLastError());
     }
     for (int ty : {2, 4}) {
@@ -604,6 +658,11 @@ int step_batch(vibe_mi_batch *b, const double *dt, int nsub, int nsteps, const u
       exec = it->second;
     }
   }
+  b->group_timing = timed && exec == nullptr;
+  if (timed) {
+    b->ms_stage = b->ms_sub = 0;
+    b->n_stage = b->n_sub = 0;
+  }
   if (timed) VMI_HIP(hipEventRecord(b->ev[0], b->stream));
   for (int s = 0; s < nsteps; ++s) {
     if (exec) {
@@ -613,6 +672,7 @@ int step_batch(vibe_mi_batch *b, const double *dt, int nsub, int nsteps, const u
     }
   }
   if (timed) VMI_HIP(hipEventRecord(b->ev[1], b->stream));
+  b->group_timing = false;
   int f = 0;
   VMI_HIP(hipMemcpyAsync(&f, b->flag, sizeof(int), hipMemcpyDeviceToHost, b->stream));
   VMI_HIP(hipStreamSynchronize(b->stream));
@@ -684,7 +744,7 @@ int vibe_mi_batch_member(vibe_mi_batch *b, int idx, vibe_mi_domain **out) {
 
 static int batch_step_common(vibe_mi_batch *b, const double *dt, int nsub, int nsteps, const unsigned char *active,
                              bool timed, double *ms_total) {
-  if (b == nullptr || dt == nullptr || nsteps < 0 || nsub < 1 || 3 * (nsub + 2) + 1 > kMaxCounters ||
+  if (b == nullptr || dt == nullptr || nsteps < 0 || nsub < 1 || 3 * (nsub + 2) + 1 > kMaxLaunches ||
       nsub > kMaxNsub)
     return VIBE_C_ERR_BAD_ARG;
   if (hipSetDevice(b->device) != hipSuccess) return VIBE_MI_ERR_HIP;
@@ -708,6 +768,16 @@ int vibe_mi_batch_step_timed(vibe_mi_batch *b, const double *dt, int nsub, int n
 int vibe_mi_batch_set_graph(vibe_mi_batch *b, int use_graph) {
   if (b == nullptr) return VIBE_C_ERR_BAD_ARG;
   b->use_graph = use_graph != 0;
+  return 0;
+}
+
+int vibe_mi_batch_kernel_times(vibe_mi_batch *b, double *ms_stage, double *ms_substep, int *n_stage,
+                               int *n_substep) {
+  if (b == nullptr) return VIBE_C_ERR_BAD_ARG;
+  if (ms_stage) *ms_stage = b->ms_stage;
+  if (ms_substep) *ms_substep = b->ms_sub;
+  if (n_stage) *n_stage = b->n_stage;
+  if (n_substep) *n_substep = b->n_sub;
   return 0;
 }
 

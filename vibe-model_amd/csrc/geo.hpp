@@ -39,6 +39,29 @@ struct Acc {
   __device__ __forceinline__ size_t at(int i, int j, int k) const { return static_cast<size_t>(k) * nxy + h(i, j); }
 };
 
+// Global-address-space views of a pointer.  Pointers read from device
+// memory (the batched kernels' member descriptors) carry no address space,
+// so accesses through them would be generic (flat) instructions; the views
+// make every access a global load/store with scalar base and vector offset.
+#define VMI_GLOBAL __attribute__((address_space(1)))
+template <typename P>
+__device__ __forceinline__ const VMI_GLOBAL P *G(const P *p) {
+  return (const VMI_GLOBAL P *)p;
+}
+template <typename P>
+__device__ __forceinline__ VMI_GLOBAL P *G(P *p) {
+  return (VMI_GLOBAL P *)p;
+}
+
+// Constant-address-space view for arrays no kernel writes while it reads
+// them (geometry, reference state, HEVI factorisation): wave-uniform
+// accesses through it become scalar loads.
+#define VMI_CONST __attribute__((address_space(4)))
+template <typename P>
+__device__ __forceinline__ const VMI_CONST P *C(const P *p) {
+  return (const VMI_CONST P *)p;
+}
+
 // Loads/stores addressed as (wave-uniform base of a level, 32-bit per-lane
 // element index): the base lives in scalar registers and the byte offset fits
 // 32 bits, so the compiler emits the scalar-base + 32-bit vector-offset form
@@ -46,11 +69,18 @@ struct Acc {
 // stay below 4 GiB (nx*ny*8 bytes), far above any single-GPU domain here.
 template <typename P>
 __device__ __forceinline__ P ld32(const P *__restrict__ f, size_t base, uint32_t idx) {
-  return *reinterpret_cast<const P *>(reinterpret_cast<const char *>(f + base) + idx * static_cast<uint32_t>(sizeof(P)));
+  const VMI_GLOBAL char *b = (const VMI_GLOBAL char *)(f + base);
+  return *(const VMI_GLOBAL P *)(b + idx * static_cast<uint32_t>(sizeof(P)));
+}
+template <typename P>
+__device__ __forceinline__ P ldc32(const P *__restrict__ f, size_t base, uint32_t idx) {
+  const VMI_CONST char *b = (const VMI_CONST char *)(f + base);
+  return *(const VMI_CONST P *)(b + idx * static_cast<uint32_t>(sizeof(P)));
 }
 template <typename P>
 __device__ __forceinline__ void st32(P *__restrict__ f, size_t base, uint32_t idx, P v) {
-  *reinterpret_cast<P *>(reinterpret_cast<char *>(f + base) + idx * static_cast<uint32_t>(sizeof(P))) = v;
+  VMI_GLOBAL char *b = (VMI_GLOBAL char *)(f + base);
+  *(VMI_GLOBAL P *)(b + idx * static_cast<uint32_t>(sizeof(P))) = v;
 }
 
 template <typename T>

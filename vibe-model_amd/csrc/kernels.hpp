@@ -57,8 +57,8 @@ namespace vibe_mi {
 template <typename T, typename S>
 __device__ __forceinline__ T mc_at(const Geo<T> &g, const Acc &A, const S *__restrict__ u, const S *__restrict__ v,
                                    int i, int j, int k) {
-  return T(0.5) * (T(u[A.at(i, j, k)]) * g.sx[gcol(g, i, j)] + T(u[A.at(i + 1, j, k)]) * g.sx[gcol(g, i + 1, j)]) +
-         T(0.5) * (T(v[A.at(i, j, k)]) * g.sy[gcol(g, i, j)] + T(v[A.at(i, j + 1, k)]) * g.sy[gcol(g, i, j + 1)]);
+  return T(0.5) * (T(G(u)[A.at(i, j, k)]) * C(g.sx)[gcol(g, i, j)] + T(G(u)[A.at(i + 1, j, k)]) * C(g.sx)[gcol(g, i + 1, j)]) +
+         T(0.5) * (T(G(v)[A.at(i, j, k)]) * C(g.sy)[gcol(g, i, j)] + T(G(v)[A.at(i, j + 1, k)]) * C(g.sy)[gcol(g, i, j + 1)]);
 }
 
 // Omega at interface k of column (i, j).
@@ -68,9 +68,9 @@ __device__ __forceinline__ void omega_at(const Geo<T> &g, const S *__restrict__ 
   const Acc A{g.nx, g.ny, static_cast<size_t>(g.nx) * g.ny};
   T o = T(0);
   if (k > 0 && k < g.nz)
-    o = T(w[A.at(i, j, k)]) -
-        (T(1) - g.zeta[k]) * (T(0.5) * (mc_at<T, S>(g, A, u, v, i, j, k - 1) + mc_at<T, S>(g, A, u, v, i, j, k)));
-  om[A.at(i, j, k)] = o;
+    o = T(G(w)[A.at(i, j, k)]) -
+        (T(1) - C(g.zeta)[k]) * (T(0.5) * (mc_at<T, S>(g, A, u, v, i, j, k - 1) + mc_at<T, S>(g, A, u, v, i, j, k)));
+  G(om)[A.at(i, j, k)] = o;
 }
 
 template <typename T, typename S>
@@ -86,7 +86,7 @@ __global__ void __launch_bounds__(256) k_omega(Geo<T> g, const S *__restrict__ u
 template <typename T, typename S>
 __device__ __forceinline__ T gz(const Geo<T> &g, const Acc &A, const S *__restrict__ pi, int i, int j, int k) {
   if (k <= 0 || k >= g.nz) return T(0);
-  return (T(pi[A.at(i, j, k)]) - T(pi[A.at(i, j, k - 1)])) * g.rdzw[k];
+  return (T(pi[A.at(i, j, k)]) - T(pi[A.at(i, j, k - 1)])) * C(g.rdzw)[k];
 }
 template <typename T, typename S>
 __device__ __forceinline__ T gzc(const Geo<T> &g, const Acc &A, const S *__restrict__ pi, int i, int j, int k) {
@@ -95,13 +95,13 @@ __device__ __forceinline__ T gzc(const Geo<T> &g, const Acc &A, const S *__restr
 template <typename T, typename S>
 __device__ __forceinline__ T px_op(const Geo<T> &g, const Acc &A, const S *__restrict__ pi, int i, int j, int k) {
   const T dpx = (T(pi[A.at(i, j, k)]) - T(pi[A.at(i - 1, j, k)])) * g.rdx;
-  const T met = (T(1) - g.zc[k]) * g.metx[gcol(g, i, j)];
+  const T met = (T(1) - C(g.zc)[k]) * C(g.metx)[gcol(g, i, j)];
   return dpx - met * (T(0.5) * (gzc<T, S>(g, A, pi, i - 1, j, k) + gzc<T, S>(g, A, pi, i, j, k)));
 }
 template <typename T, typename S>
 __device__ __forceinline__ T py_op(const Geo<T> &g, const Acc &A, const S *__restrict__ pi, int i, int j, int k) {
   const T dpy = (T(pi[A.at(i, j, k)]) - T(pi[A.at(i, j - 1, k)])) * g.rdy;
-  const T met = (T(1) - g.zc[k]) * g.mety[gcol(g, i, j)];
+  const T met = (T(1) - C(g.zc)[k]) * C(g.mety)[gcol(g, i, j)];
   return dpy - met * (T(0.5) * (gzc<T, S>(g, A, pi, i, j - 1, k) + gzc<T, S>(g, A, pi, i, j, k)));
 }
 
@@ -147,9 +147,9 @@ __device__ __forceinline__ void stage_col(const Geo<T> &g, const StageIn &cur, c
   // column geometry (compacted)
   const int gC = gcol(g, i, j), gW = gcol(g, i - 1, j), gE = gcol(g, i + 1, j), gS = gcol(g, i, j - 1),
             gN = gcol(g, i, j + 1), gWN = gcol(g, i - 1, j + 1), gES = gcol(g, i + 1, j - 1);
-  const T DuW = g.Du[gW], DuC = g.Du[gC], DuE = g.Du[gE], DuES = g.Du[gES];
-  const T DvS = g.Dv[gS], DvC = g.Dv[gC], DvN = g.Dv[gN], DvWN = g.Dv[gWN], DvW = g.Dv[gW];
-  const T rDuC = g.rDu[gC], rDvC = g.rDv[gC], rD = g.rDc[gC], mxC = g.metx[gC], myC = g.mety[gC];
+  const T DuW = C(g.Du)[gW], DuC = C(g.Du)[gC], DuE = C(g.Du)[gE], DuES = C(g.Du)[gES];
+  const T DvS = C(g.Dv)[gS], DvC = C(g.Dv)[gC], DvN = C(g.Dv)[gN], DvWN = C(g.Dv)[gWN], DvW = C(g.Dv)[gW];
+  const T rDuC = C(g.rDu)[gC], rDvC = C(g.rDv)[gC], rD = C(g.rDc)[gC], mxC = C(g.metx)[gC], myC = C(g.mety)[gC];
 
   auto ldd = [&](const double *f, int k, uint32_t c) { return T(ld32(f, static_cast<size_t>(k) * nxy, c)); };
   auto ldo = [&](int k, uint32_t c) { return ld32(om, static_cast<size_t>(k) * nxy, c); };
@@ -205,12 +205,12 @@ __device__ __forceinline__ void stage_col(const Geo<T> &g, const StageIn &cur, c
     wx[3] = wy[3] = ww[3];
     const T vWN = T(ld32(V, kb, cWN)), uES = T(ld32(U, kb, cES));
     const T rWst = T(ld32(RH, kb, cW)), rE = T(ld32(RH, kb, cE)), rSth = T(ld32(RH, kb, cS)), rN = T(ld32(RH, kb, cN));
-    const T omz = T(1) - g.zc[k];
-    const T rdzk = g.rdzeta[k];
+    const T omz = T(1) - C(g.zc)[k];
+    const T rdzk = C(g.rdzeta)[k];
     // vertical pi' gradients (0 at ground and lid)
     auto gpair = [&](const T *p) {
-      const T glo = (k >= 1) ? (p[1] - p[0]) * g.rdzw[k] : T(0);
-      const T ghi = (k + 1 <= nz - 1) ? (p[2] - p[1]) * g.rdzw[k + 1] : T(0);
+      const T glo = (k >= 1) ? (p[1] - p[0]) * C(g.rdzw)[k] : T(0);
+      const T ghi = (k + 1 <= nz - 1) ? (p[2] - p[1]) * C(g.rdzw)[k + 1] : T(0);
       return T(0.5) * (glo + ghi);
     };
     const T GcC = gpair(pw), GcW = gpair(pWw), GcS = gpair(pSw);
@@ -242,7 +242,7 @@ __device__ __forceinline__ void stage_col(const Geo<T> &g, const StageIn &cur, c
     }
     // ---------------- v at the south face ----------------
     {
-      const T U0 = T(0.5) * (g.Du[gcol(g, i, j - 1)] * uy[2] + DuC * uw[3]);
+      const T U0 = T(0.5) * (C(g.Du)[gcol(g, i, j - 1)] * uy[2] + DuC * uw[3]);
       const T U1 = T(0.5) * (DuES * uES + DuE * ux[4]);
       const T fx0 = up5<T>(U0, vx[0], vx[1], vx[2], vx[3], vx[4], vx[5]);
       const T fx1 = up5<T>(U1, vx[1], vx[2], vx[3], vx[4], vx[5], vx[6]);
@@ -280,11 +280,11 @@ __device__ __forceinline__ void stage_col(const Geo<T> &g, const StageIn &cur, c
       const T fy1 = up5<T>(V1, wy[1], wy[2], wy[3], wy[4], wy[5], wy[6]);
       const T Oc0 = cw_O, fz0 = cw_f;
       const T fz1 = fzc1;
-      const T rz = g.rdzw[k];
+      const T rz = C(g.rdzw)[k];
       const T adv = -(fx1 - fx0) * g.rdx - (fy1 - fy0) * g.rdy - (fz1 - fz0) * rz +
                     ww[3] * ((U1 - U0) * g.rdx + (V1 - V0) * g.rdy + (Oc1 - Oc0) * rz);
       const T thw = T(0.5) * (tw[2] + tw[3]);
-      const T th0w = T(0.5) * (g.theta0[rb - rk + gC] + g.theta0[rb + gC]);
+      const T th0w = T(0.5) * (C(g.theta0)[rb - rk + gC] + C(g.theta0)[rb + gC]);
       const T pz = (pw[1] - pw[0]) * rz * rD;
       const T R = adv * rD - T(kCp) * thw * pz + T(kG) * thw / th0w;
       st32(out.w, kb, cC, static_cast<O>(ld32(s0.w, kb, cC) + h * static_cast<double>(R)));
@@ -318,10 +318,10 @@ __device__ __forceinline__ void stage_col(const Geo<T> &g, const StageIn &cur, c
       const T fd = (ftx1 - ftx0) * g.rdx + (fty1 - fty0) * g.rdy + (ftz1 - ftz0) * rdzk;
       auto gam = [&](int kq) {
         const int kk = kq < 1 ? 1 : (kq > nz - 1 ? nz - 1 : kq);
-        return (g.theta0[kk * rk + gC] - g.theta0[(kk - 1) * rk + gC]) * rD * g.rdzw[kk];
+        return (C(g.theta0)[kk * rk + gC] - C(g.theta0)[(kk - 1) * rk + gC]) * rD * C(g.rdzw)[kk];
       };
       const T rt = -(fd - tw[3] * div) * rD - T(0.5) * (ww[3] * gam(k) + ww[4] * gam(k + 1));
-      const T rp = g.kap[rb + gC] * rr + g.lam[rb + gC] * rt;
+      const T rp = C(g.kap)[rb + gC] * rr + C(g.lam)[rb + gC] * rt;
       const double rn = ld32(s0.rho, kb, cC) + h * static_cast<double>(rr);
       const double tn = ld32(s0.th, kb, cC) + h * static_cast<double>(rt);
       const double pn = ld32(s0.pi, kb, cC) + h * static_cast<double>(rp);
@@ -390,15 +390,15 @@ __global__ void __launch_bounds__(64) k_hevi_factor(Geo<T> g, T dtau, T an, T *_
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.rk) return;
   const int nz = g.nz, n = nz - 1, rk = g.rk;
-  const T dtD = dtau * g.rDc[c];
+  const T dtD = dtau * C(g.rDc)[c];
   const T an2 = an * an;
-  auto ck = [&](int k) { return g.kap[k * rk + c] * dtD * g.rdzeta[k]; };
-  auto r0w = [&](int k) { return T(0.5) * (g.rho0[(k - 1) * rk + c] + g.rho0[k * rk + c]); };
+  auto ck = [&](int k) { return C(g.kap)[k * rk + c] * dtD * C(g.rdzeta)[k]; };
+  auto r0w = [&](int k) { return T(0.5) * (C(g.rho0)[(k - 1) * rk + c] + C(g.rho0)[k * rk + c]); };
   T cp_prev = T(0);
   for (int m = 0; m < n; ++m) {
     const int k = m + 1;
-    const T th0w = T(0.5) * (g.theta0[(k - 1) * rk + c] + g.theta0[k * rk + c]);
-    const T gk = dtD * T(kCp) * th0w * g.rdzw[k];
+    const T th0w = T(0.5) * (C(g.theta0)[(k - 1) * rk + c] + C(g.theta0)[k * rk + c]);
+    const T gk = dtD * T(kCp) * th0w * C(g.rdzw)[k];
     const T ga = gk * an2;
     const T ckk = ck(k), ckm = ck(k - 1);
     const T a = (m > 0) ? -ga * (ckm * r0w(k - 1)) : T(0);
@@ -480,11 +480,11 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
                  cE = static_cast<uint32_t>(j) * unx + iE, cS = static_cast<uint32_t>(jS) * unx + i,
                  cN = static_cast<uint32_t>(jN) * unx + i;
   const uint32_t gC = gcol(g, i, j), gW = gcol(g, iW, j), gE = gcol(g, iE, j), gS = gcol(g, i, jS), gN = gcol(g, i, jN);
-  const T rD = g.rDc[gC];
+  const T rD = C(g.rDc)[gC];
   const T dtD = dtau * rD;
-  const T Du_w = g.Du[gC], Du_e = g.Du[gE], Dv_s = g.Dv[gC], Dv_n = g.Dv[gN];
-  const T sx_w = g.sx[gC], sx_e = g.sx[gE], sy_s = g.sy[gC], sy_n = g.sy[gN];
-  const T mx_w = g.metx[gC], mx_e = g.metx[gE], my_s = g.mety[gC], my_n = g.mety[gN];
+  const T Du_w = C(g.Du)[gC], Du_e = C(g.Du)[gE], Dv_s = C(g.Dv)[gC], Dv_n = C(g.Dv)[gN];
+  const T sx_w = C(g.sx)[gC], sx_e = C(g.sx)[gE], sy_s = C(g.sy)[gC], sy_n = C(g.sy)[gN];
+  const T mx_w = C(g.metx)[gC], mx_e = C(g.metx)[gE], my_s = C(g.mety)[gC], my_n = C(g.mety)[gN];
   const T cpdt = dtau;
 
   T E[MAXCH], Q[MAXCH], Wt[MAXCH], Fh[MAXCH], Ph[MAXCH];
@@ -510,13 +510,13 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
       ld5(k0 - 1, pm);
       pmC = pm[0];
 #pragma unroll
-      for (int q = 0; q < 5; ++q) gl[q] = (p0[q] - pm[q]) * g.rdzw[k0];
+      for (int q = 0; q < 5; ++q) gl[q] = (p0[q] - pm[q]) * C(g.rdzw)[k0];
     } else {
 #pragma unroll
       for (int q = 0; q < 5; ++q) gl[q] = T(0);
     }
-    T t0C_prev = (k0 >= 1) ? ld32(g.theta0, static_cast<size_t>(k0 - 1) * rk, gC) : T(0);
-    T r0C_prev = (k0 >= 1) ? ld32(g.rho0, static_cast<size_t>(k0 - 1) * rk, gC) : T(0);
+    T t0C_prev = (k0 >= 1) ? ldc32(g.theta0, static_cast<size_t>(k0 - 1) * rk, gC) : T(0);
+    T r0C_prev = (k0 >= 1) ? ldc32(g.rho0, static_cast<size_t>(k0 - 1) * rk, gC) : T(0);
     T mc_prev = T(0);
 #pragma unroll
     for (int c = 0; c < MAXCH; ++c) {
@@ -524,7 +524,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         const int k = k0 + c;
         const size_t b = static_cast<size_t>(k) * nxy;
         if (k + 1 < nz) ld5(k + 1, pp); else for (int q = 0; q < 5; ++q) pp[q] = p0[q];
-        const T rz_hi = g.rdzw[k + 1];
+        const T rz_hi = C(g.rdzw)[k + 1];
         T Gc[5];
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
@@ -532,14 +532,14 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
           Gc[q] = T(0.5) * (gl[q] + ghi);
           gl[q] = ghi;
         }
-        const T omz = T(1) - g.zc[k];
+        const T omz = T(1) - C(g.zc)[k];
         const T px_w = (p0[0] - p0[1]) * g.rdx - (omz * mx_w) * (T(0.5) * (Gc[1] + Gc[0]));
         const T px_e = (p0[2] - p0[0]) * g.rdx - (omz * mx_e) * (T(0.5) * (Gc[0] + Gc[2]));
         const T py_s = (p0[0] - p0[3]) * g.rdy - (omz * my_s) * (T(0.5) * (Gc[3] + Gc[0]));
         const T py_n = (p0[4] - p0[0]) * g.rdy - (omz * my_n) * (T(0.5) * (Gc[0] + Gc[4]));
         const int rb = k * rk;
-        const T t0C = ld32(g.theta0, static_cast<size_t>(rb), gC), t0W = ld32(g.theta0, static_cast<size_t>(rb), gW), t0E = ld32(g.theta0, static_cast<size_t>(rb), gE), t0S = ld32(g.theta0, static_cast<size_t>(rb), gS),
-                t0N = ld32(g.theta0, static_cast<size_t>(rb), gN);
+        const T t0C = ldc32(g.theta0, static_cast<size_t>(rb), gC), t0W = ldc32(g.theta0, static_cast<size_t>(rb), gW), t0E = ldc32(g.theta0, static_cast<size_t>(rb), gE), t0S = ldc32(g.theta0, static_cast<size_t>(rb), gS),
+                t0N = ldc32(g.theta0, static_cast<size_t>(rb), gN);
         const T un_w = T(ld32(u, b, cC)) - cpdt * (T(kCp) * (T(0.5) * (t0W + t0C)) * px_w);
         const T un_e = T(ld32(u, b, cE)) - cpdt * (T(kCp) * (T(0.5) * (t0C + t0E)) * px_e);
         const T vn_s = T(ld32(v, b, cC)) - cpdt * (T(kCp) * (T(0.5) * (t0S + t0C)) * py_s);
@@ -549,11 +549,11 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
           st32(vo, b, cC, static_cast<O>(vn_s));
         }
         const T mc = T(0.5) * (un_w * sx_w + un_e * sx_e) + T(0.5) * (vn_s * sy_s + vn_n * sy_n);
-        const T r0C = ld32(g.rho0, static_cast<size_t>(rb), gC), r0W = ld32(g.rho0, static_cast<size_t>(rb), gW), r0E = ld32(g.rho0, static_cast<size_t>(rb), gE), r0S = ld32(g.rho0, static_cast<size_t>(rb), gS),
-                r0N = ld32(g.rho0, static_cast<size_t>(rb), gN);
+        const T r0C = ldc32(g.rho0, static_cast<size_t>(rb), gC), r0W = ldc32(g.rho0, static_cast<size_t>(rb), gW), r0E = ldc32(g.rho0, static_cast<size_t>(rb), gE), r0S = ldc32(g.rho0, static_cast<size_t>(rb), gS),
+                r0N = ldc32(g.rho0, static_cast<size_t>(rb), gN);
         const T hd = (Du_e * (T(0.5) * (r0C + r0E)) * un_e - Du_w * (T(0.5) * (r0W + r0C)) * un_w) * g.rdx +
                      (Dv_n * (T(0.5) * (r0C + r0N)) * vn_n - Dv_s * (T(0.5) * (r0S + r0C)) * vn_s) * g.rdy;
-        const T cq = ld32(g.kap, static_cast<size_t>(rb), gC) * dtD;
+        const T cq = ldc32(g.kap, static_cast<size_t>(rb), gC) * dtD;
         E[c] = T(ld32(rho, b, cC)) - dtD * hd;
         Q[c] = p0[0] - cq * hd;
         const T wk = T(ld32(w, b, cC));
@@ -561,11 +561,11 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         Wt[c] = T(0);
         if (k >= 1) {
           const T th0w = T(0.5) * (t0C_prev + t0C);
-          const T gk = dtD * T(kCp) * th0w * g.rdzw[k];
+          const T gk = dtD * T(kCp) * th0w * C(g.rdzw)[k];
           Wt[c] = wk - gk * ao * (p0[0] - pmC);
           const T r0w = T(0.5) * (r0C_prev + r0C);
           if (c > 0) {
-            const T Mw = (T(1) - g.zeta[k]) * (T(0.5) * (mc_prev + mc));
+            const T Mw = (T(1) - C(g.zeta)[k]) * (T(0.5) * (mc_prev + mc));
             Fh[c] = r0w * (ao * wk - Mw);
           } else {
             aw_first = ao * wk;
@@ -589,7 +589,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   // ---------------- phase B: boundary vertical flux, P^, RHS ----------------
   if (nlev > 0) {
     if (k0 >= 1) {
-      const T Mw = (T(1) - g.zeta[k0]) * (T(0.5) * (s_mc[wv - 1][lane] + mc_first));
+      const T Mw = (T(1) - C(g.zeta)[k0]) * (T(0.5) * (s_mc[wv - 1][lane] + mc_first));
       Fh[0] = r0w_first * (aw_first - Mw);
     }
     s_fh[wv][lane] = Fh[0];
@@ -602,7 +602,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
       if (c < nlev) {
         const int k = k0 + c;
         const T fhn = (c + 1 < nlev) ? Fh[(c + 1) < MAXCH ? c + 1 : c] : fh_next_chunk;
-        const T ck = ld32(g.kap, static_cast<size_t>(k) * rk, gC) * dtD * g.rdzeta[k];
+        const T ck = ldc32(g.kap, static_cast<size_t>(k) * rk, gC) * dtD * C(g.rdzeta)[k];
         Ph[c] = Q[c] - ck * (fhn - Fh[c]);
       }
     }
@@ -619,13 +619,13 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         const int k = k0 + c;
         if (k >= 1) {
           const T php = (c > 0) ? Ph[c > 0 ? c - 1 : 0] : ph_prev_chunk;
-          const T th0w = T(0.5) * (ld32(g.theta0, static_cast<size_t>(k - 1) * rk, gC) + ld32(g.theta0, static_cast<size_t>(k) * rk, gC));
-          const T gk = dtD * T(kCp) * th0w * g.rdzw[k];
+          const T th0w = T(0.5) * (ldc32(g.theta0, static_cast<size_t>(k - 1) * rk, gC) + ldc32(g.theta0, static_cast<size_t>(k) * rk, gC));
+          const T gk = dtD * T(kCp) * th0w * C(g.rdzw)[k];
           const int m = k - 1;
           const T td = Wt[c] - gk * an * (Ph[c] - php);
-          s_td[m * 64 + lane] = td * ld32(tINV, static_cast<size_t>(m) * rk, gC);
-          s_be[m * 64 + lane] = ld32(tA, static_cast<size_t>(m) * rk, gC);
-          s_cp[m * 64 + lane] = ld32(tCP, static_cast<size_t>(m) * rk, gC);
+          s_td[m * 64 + lane] = td * ldc32(tINV, static_cast<size_t>(m) * rk, gC);
+          s_be[m * 64 + lane] = ldc32(tA, static_cast<size_t>(m) * rk, gC);
+          s_cp[m * 64 + lane] = ldc32(tCP, static_cast<size_t>(m) * rk, gC);
         }
       }
     }
@@ -702,7 +702,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
     int bad = 0;
     auto fluxF = [&](int k, T fh) {
       if (k <= 0 || k >= nz) return T(0);
-      const T r0w = T(0.5) * (ld32(g.rho0, static_cast<size_t>(k - 1) * rk, gC) + ld32(g.rho0, static_cast<size_t>(k) * rk, gC));
+      const T r0w = T(0.5) * (ldc32(g.rho0, static_cast<size_t>(k - 1) * rk, gC) + ldc32(g.rho0, static_cast<size_t>(k) * rk, gC));
       return fh + (an * r0w) * s_td[(k - 1) * 64 + lane];
     };
     const T fh_next_chunk = (k0 + nlev < nz) ? s_fh[wv + 1][lane] : T(0);
@@ -714,9 +714,9 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         const size_t b = static_cast<size_t>(k) * nxy;
         const T fhn = (c + 1 < nlev) ? Fh[(c + 1) < MAXCH ? c + 1 : c] : fh_next_chunk;
         const T fhi = fluxF(k + 1, fhn);
-        const T vd = -dtD * ((fhi - flo) * g.rdzeta[k]);
+        const T vd = -dtD * ((fhi - flo) * C(g.rdzeta)[k]);
         const T rn = E[c] + vd;
-        const T pn = Q[c] + ld32(g.kap, static_cast<size_t>(k) * rk, gC) * vd;
+        const T pn = Q[c] + ldc32(g.kap, static_cast<size_t>(k) * rk, gC) * vd;
         const T wn = (k >= 1) ? s_td[(k - 1) * 64 + lane] : mc0;
         if (active) {
           st32(rho_o, b, cC, static_cast<O>(rn));
@@ -744,13 +744,13 @@ __global__ void __launch_bounds__(64 * NW) k_substep(Geo<T> g, SubArgs<T, I, O> 
 
 __device__ __forceinline__ void sponge_at(int nx, int ny, int nz, const double *__restrict__ alpha, double dt,
                                           const FieldSet &s, int i, int j, int k) {
-  double *__restrict__ u = s.u;
-  double *__restrict__ v = s.v;
-  double *__restrict__ w = s.w;
-  double *__restrict__ rho = s.rho;
-  double *__restrict__ th = s.th;
-  double *__restrict__ pi = s.pi;
-  const double a = alpha[k < nz - 1 ? k : nz - 1];
+  VMI_GLOBAL double *__restrict__ u = G(s.u);
+  VMI_GLOBAL double *__restrict__ v = G(s.v);
+  VMI_GLOBAL double *__restrict__ w = G(s.w);
+  VMI_GLOBAL double *__restrict__ rho = G(s.rho);
+  VMI_GLOBAL double *__restrict__ th = G(s.th);
+  VMI_GLOBAL double *__restrict__ pi = G(s.pi);
+  const double a = G(alpha)[k < nz - 1 ? k : nz - 1];
   if (a <= 0.0) return;
   const double f = 1.0 / (1.0 + dt * a);
   const size_t c = (static_cast<size_t>(k) * ny + j) * nx + i;

@@ -113,6 +113,7 @@ def lib() -> ctypes.CDLL:
         "vibe_mi_batch_step_timed": (_I, [_VP, _PD, _I, _I, _VP, _PD]),
         "vibe_mi_batch_set_graph": (_I, [_VP, _I]),
         "vibe_mi_batch_tuning": (_I, [_VP, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+        "vibe_mi_batch_kernel_times": (_I, [_VP, _PD, _PD, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
         "vibe_mi_batch_device_bytes": (_SZ, [_VP]),
         "vibe_mi_zeta_uniform": (_I, [_I, _PD]),
         "vibe_mi_zeta_tanh": (_I, [_I, _D, _D, _D, _PD]),
@@ -444,6 +445,14 @@ class Batch:
 
     def set_graph(self, use_graph: bool):
         _check(lib().vibe_mi_batch_set_graph(self.h, 1 if use_graph else 0), "vibe_mi_batch_set_graph")
+
+    def kernel_times(self):
+        """Per-group times of the last timed call made without a graph."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        na, nb = _I(), _I()
+        _check(lib().vibe_mi_batch_kernel_times(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(na),
+                                                ctypes.byref(nb)), "vibe_mi_batch_kernel_times")
+        return {"ms_stage": a.value, "ms_substep": b.value, "n_stage": na.value, "n_substep": nb.value}
 
     def tuning(self):
         a, b = _I(), _I()
