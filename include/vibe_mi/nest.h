@@ -1,0 +1,60 @@
+/* vibe_mi/nest.h -- C ABI of the nest lateral boundary (one-way nesting).
+ *
+ * Replaces, on the device, the reference's nest operators
+ *   Nest::interpolate_from_parent    src/grid/nest.cpp:52-83
+ *   Nest::apply_lateral_boundary     src/grid/nest.cpp:110-128
+ *   davies_profile                   src/grid/nest.cpp:34-43
+ *   vibe_bilinear_f64 (arithmetic)   src/common/c_kernels.c:108-144
+ * for child and parent domains that live on the same GPU (vibe_mi/dycore.h
+ * handles, batch members included).  The parent is periodic (the build's
+ * domains are), so parent indices wrap instead of clamping.
+ *
+ * Placement: child cell (i, j) has its centre at parent index coordinates
+ *   xp = (i0 + i + 0.5) / r - 0.5,  yp = (j0 + j + 0.5) / r - 0.5
+ * (nest.cpp:67-68: i0, j0 are offsets counted in child cells).  Cell fields
+ * (rho', theta', pi', and w, which sits at cell centres horizontally) use that
+ * point; u (west x-face) uses xp = (i0 + i) / r and v (south y-face)
+ * yp = (j0 + j) / r, each field at its own staggered position.  Levels map
+ * one to one (shared vertical coordinate).  Perturbation fields are
+ * interpolated, each domain keeping its own reference state.
+ *
+ * All arithmetic is FP64 on the FP64 states of both domains.
+ */
+#ifndef VIBE_MI_NEST_H
+#define VIBE_MI_NEST_H
+
+#include "vibe_mi/dycore.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int ratio;        /* refinement ratio r >= 2 (NestPlacement::ratio)           */
+  int i0, j0;       /* child origin offset from the parent origin, child cells */
+  int zone;         /* Davies zone width in child points (boundary_zone), 1..16 */
+  double alpha_max; /* Davies weight at the outer boundary (davies_profile)   */
+} vibe_mi_nest_desc;
+
+/* Davies weights alpha(n) = alpha_max (1 - cos(pi (zone - n) / zone)) / 2,
+ * n = 0 .. zone-1 (nest.cpp:34-43), written to alpha[zone]. */
+int vibe_mi_davies_profile(int zone, double alpha_max, double *alpha);
+
+/* Child state <- bilinear interpolation of the parent's current state over
+ * the whole child (initialisation; nest.cpp:52-83). */
+int vibe_mi_nest_interpolate(vibe_mi_domain *child, vibe_mi_domain *parent, const vibe_mi_nest_desc *nd);
+
+/* Davies relaxation of the child's lateral zone (points at distance
+ * d = min(i, nx-1-i, j, ny-1-j) < zone from the child's edge) toward the
+ * parent state interpolated bilinearly in space and linearly in time:
+ *   p = p_old + frac (p_new - p_old),  c <- (1 - alpha(d)) c + alpha(d) p
+ * where p_new is the parent's current state and p_old its state before its
+ * last step (nest.cpp:110-128).  frac in [0, 1]. */
+int vibe_mi_nest_relax(vibe_mi_domain *child, vibe_mi_domain *parent, const vibe_mi_nest_desc *nd,
+                       double frac);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

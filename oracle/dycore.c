@@ -202,6 +202,15 @@ static inline double upwind_weno(double U, double qm3, double qm2, double qm1, d
   return U * qf;
 }
 
+/* Exported views of the two face primitives for the oracle's own tests. */
+double orc_weno_face(double a, double b, double c, double d, double e) { return weno_face(a, b, c, d, e); }
+double orc_upwind_weno(double U, double qm3, double qm2, double qm1, double q0, double q1, double q2) {
+  return upwind_weno(U, qm3, qm2, qm1, q0, q1, q2);
+}
+double orc_up5(double U, double qm2, double qm1, double q0, double q1, double q2, double q3) {
+  return up5(U, qm2, qm1, q0, q1, q2, q3);
+}
+
 /* ---- kinematics -------------------------------------------------------- */
 
 /* Mc = horizontal velocity . grad(zs) at cell centres;

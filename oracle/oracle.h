@@ -72,6 +72,20 @@ void orc_dycore_sponge(orc_dycore *dc, orc_state *s, double dt);
  * internal, potential, kinetic+internal+potential}. */
 void orc_dycore_energy(orc_dycore *dc, const orc_state *s, double out[6]);
 void orc_set_threads(int n);
+/* Face primitives of the dycore (dycore.c), exported for tests:
+ * orc_weno_face   WENO5 value at the face right of c, one-division form
+ * orc_upwind_weno U times the upwind WENO5 value at the face between q-1, q0
+ * orc_up5         5th-order upwind flux U c6 - |U| d5 at the face q0|q1 */
+double orc_weno_face(double a, double b, double c, double d, double e);
+double orc_upwind_weno(double U, double qm3, double qm2, double qm1, double q0, double q1, double q2);
+double orc_up5(double U, double qm2, double qm1, double q0, double q1, double q2, double q3);
+
+/* ---- nest lateral boundary (nest.c) ------------------------------------- */
+void orc_davies_profile(int zone, double alpha_max, double *alpha);
+/* full = 1: child <- parent interpolation everywhere; full = 0: Davies
+ * relaxation of the zone toward pold + frac (pnew - pold). */
+void orc_nest_apply(const orc_state *pnew, const orc_state *pold, int pnx, int pny, orc_state *child, int cnx,
+                    int cny, int nz, int r, int i0, int j0, int zone, double alpha_max, double frac, int full);
 
 #ifdef __cplusplus
 }

@@ -76,6 +76,17 @@ def oracle_lib():
         L.orc_dycore_energy.argtypes = [_VP, ctypes.POINTER(OrcState), _PD]
         L.orc_set_threads.restype = None
         L.orc_set_threads.argtypes = [_I]
+        L.orc_weno_face.restype = _D
+        L.orc_weno_face.argtypes = [_D] * 5
+        L.orc_upwind_weno.restype = _D
+        L.orc_upwind_weno.argtypes = [_D] * 7
+        L.orc_up5.restype = _D
+        L.orc_up5.argtypes = [_D] * 7
+        L.orc_davies_profile.restype = None
+        L.orc_davies_profile.argtypes = [_I, _D, _PD]
+        L.orc_nest_apply.restype = None
+        L.orc_nest_apply.argtypes = [ctypes.POINTER(OrcState), ctypes.POINTER(OrcState), _I, _I,
+                                     ctypes.POINTER(OrcState), _I, _I, _I, _I, _I, _I, _I, _D, _D, _I]
         _olib = L
     return _olib
 
@@ -155,3 +166,22 @@ class OracleDycore:
         out = np.zeros(6)
         oracle_lib().orc_dycore_energy(self.h, ctypes.byref(self._st(self.state)), _pd(out))
         return out
+
+
+FIELDS = ("u", "v", "w", "rho", "theta", "pi")
+
+
+def _ostate(st):
+    arrs = {k: np.ascontiguousarray(st[k], dtype=np.float64) for k in FIELDS}
+    return OrcState(*[_pd(arrs[k]) for k in FIELDS]), arrs
+
+
+def oracle_nest_apply(pnew, pold, pnx, pny, child, cnx, cny, nz, ratio, i0, j0, zone, alpha_max, frac, full):
+    """Oracle nest operator on dict states; returns the new child state."""
+    out = {k: np.array(child[k], dtype=np.float64, copy=True) for k in FIELDS}
+    sn, _kn = _ostate(pnew)
+    so, _ko = _ostate(pold if pold is not None else pnew)
+    sc = OrcState(*[_pd(out[k]) for k in FIELDS])
+    oracle_lib().orc_nest_apply(ctypes.byref(sn), ctypes.byref(so), pnx, pny, ctypes.byref(sc), cnx, cny, nz,
+                                ratio, i0, j0, zone, alpha_max, frac, full)
+    return out

@@ -68,6 +68,11 @@ class DomainDesc(ctypes.Structure):
     ]
 
 
+class NestDesc(ctypes.Structure):
+    """vibe_mi_nest_desc (vibe_mi/nest.h)."""
+    _fields_ = [("ratio", _I), ("i0", _I), ("j0", _I), ("zone", _I), ("alpha_max", _D)]
+
+
 _lib = None
 
 
@@ -115,6 +120,9 @@ def lib() -> ctypes.CDLL:
         "vibe_mi_batch_tuning": (_I, [_VP, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
         "vibe_mi_batch_kernel_times": (_I, [_VP, _PD, _PD, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
         "vibe_mi_batch_device_bytes": (_SZ, [_VP]),
+        "vibe_mi_davies_profile": (_I, [_I, _D, _PD]),
+        "vibe_mi_nest_interpolate": (_I, [_VP, _VP, ctypes.POINTER(NestDesc)]),
+        "vibe_mi_nest_relax": (_I, [_VP, _VP, ctypes.POINTER(NestDesc), _D]),
         "vibe_mi_zeta_uniform": (_I, [_I, _PD]),
         "vibe_mi_zeta_tanh": (_I, [_I, _D, _D, _D, _PD]),
         "vibe_mi_terrain_agnesi_ridge": (_I, [_I, _I, _D, _D, _D, _PD]),
@@ -465,3 +473,25 @@ class Batch:
     @property
     def ncell(self):
         return sum(c.ncell for c in self.cases)
+
+
+# --------------------------------------------------------------------------
+# nest lateral boundary (vibe_mi/nest.h)
+# --------------------------------------------------------------------------
+
+def davies_profile(zone: int, alpha_max: float = 1.0) -> np.ndarray:
+    out = np.zeros(zone)
+    _check(lib().vibe_mi_davies_profile(zone, alpha_max, _pd(out)), "vibe_mi_davies_profile")
+    return out
+
+
+def nest_interpolate(child: "Domain", parent: "Domain", desc: NestDesc):
+    """Child state <- bilinear interpolation of the parent's current state."""
+    _check(lib().vibe_mi_nest_interpolate(child.h, parent.h, ctypes.byref(desc)), "vibe_mi_nest_interpolate")
+
+
+def nest_relax(child: "Domain", parent: "Domain", desc: NestDesc, frac: float = 1.0):
+    """Davies relaxation of the child's lateral zone toward the parent state
+    interpolated in space and (with frac) in time between the parent's last two
+    states."""
+    _check(lib().vibe_mi_nest_relax(child.h, parent.h, ctypes.byref(desc), frac), "vibe_mi_nest_relax")
