@@ -325,20 +325,18 @@ int work_list(vibe_mi_batch *b, const std::vector<unsigned char> &act, WorkList 
     if (act[m]) order.push_back(m);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->doms[x]->nz > b->doms[y]->nz; });
   std::vector<int4> rows, sp, st2, st4;
-  // Within a member, tiles run down a 64-column strip (y fastest): the
-  // workgroups of one XCD then hold consecutive rows of one strip, whose
-  // halo rows (j +- 1 in the substep, j +- 3 in the stage) they share in L2.
   for (int m : order) {
     const vibe_mi_domain *d = b->doms[m];
     const int nbx = static_cast<int>(ceil_div(d->nx, 64));
-    for (int bx = 0; bx < nbx; ++bx) {
-      for (int j = 0; j < d->ny; ++j) {
+    for (int j = 0; j < d->ny; ++j)
+      for (int bx = 0; bx < nbx; ++bx) {
         rows.push_back(make_int4(m, bx, j, 0));
         if (d->kmin_sponge <= d->nz) sp.push_back(make_int4(m, bx, j, 0));
       }
-      for (int by = 0; by < static_cast<int>(ceil_div(d->ny, 2)); ++by) st2.push_back(make_int4(m, bx, by, 0));
-      for (int by = 0; by < static_cast<int>(ceil_div(d->ny, 4)); ++by) st4.push_back(make_int4(m, bx, by, 0));
-    }
+    for (int by = 0; by < static_cast<int>(ceil_div(d->ny, 2)); ++by)
+      for (int bx = 0; bx < nbx; ++bx) st2.push_back(make_int4(m, bx, by, 0));
+    for (int by = 0; by < static_cast<int>(ceil_div(d->ny, 4)); ++by)
+      for (int bx = 0; bx < nbx; ++bx) st4.push_back(make_int4(m, bx, by, 0));
   }
   WorkList w;
   if (int rc = upload_list(b, rows, &w.rows)) return rc;

@@ -14,6 +14,9 @@ state), using the library's host-side setup helpers.
                   nz in {40, 60, 80}), dx = 2 km, N = 0.01 1/s,
                   U(z) = 15 tanh(z / 3 km), 1-5 seeded warm bubbles each
                   (returns a list of cases for vibe_mi.Batch)
+  nest_hierarchy  384x384x60 parent at 3 km, three ratio-3 nests at 1 km and
+                  one at 333 m, random terrain, dt refined per level
+                  (returns cases and nest links for vibe_mi.nesting.NestedRun)
 
 Every size can be overridden so that tests run the same physics on small
 grids.
@@ -166,6 +169,91 @@ def ensemble(n=48, seed=20240611, sizes=tuple(range(96, 321, 32)), levels=(40, 6
         nz = int(rng.choice(levels))
         out.append(ensemble_member(nx, ny, nz, seed + 1 + m, name=f"ensemble{m:02d}", **kw))
     return out
+
+
+def _bilinear_terrain(zs_p, pnx, pny, cnx, cny, r, i0, j0):
+    """Child terrain at cell centres, bilinear in the (periodic) parent
+    terrain, with the nest placement of vibe_mi/nest.h."""
+    Z = zs_p.reshape(pny, pnx)
+    i = np.arange(cnx)
+    j = np.arange(cny)
+    xp = (i0 + i + 0.5) / r - 0.5
+    yp = (j0 + j + 0.5) / r - 0.5
+    ia, ja = np.floor(xp).astype(int), np.floor(yp).astype(int)
+    wx, wy = xp - ia, yp - ja
+    i0w, i1w = ia % pnx, (ia + 1) % pnx
+    j0w, j1w = ja % pny, (ja + 1) % pny
+    q00 = Z[np.ix_(j0w, i0w)]
+    q10 = Z[np.ix_(j0w, i1w)]
+    q01 = Z[np.ix_(j1w, i0w)]
+    q11 = Z[np.ix_(j1w, i1w)]
+    WX = wx[None, :]
+    WY = wy[:, None]
+    lo = q00 + WX * (q10 - q00)
+    hi = q01 + WX * (q11 - q01)
+    return np.ascontiguousarray((lo + WY * (hi - lo)).ravel())
+
+
+def _nest_domain(name, nx, ny, nz, dx, ztop, zeta, zs, theta_sfc, bv, U, dt, nsub, steps, alpha):
+    r0, t0, p0 = _reference(1, theta_sfc, bv, nx, ny, nz, ztop, zeta, zs)
+    sp = _sponge(nz, ztop, zeta, float(zs[0]), 0.8, alpha)
+    st = _zero_state(nx, ny, nz)
+    _check(lib().vibe_mi_ic_wind(nx, ny, nz, ztop, _pd(zeta), _pd(zs), U, 0.0, 0.0, _pd(st["u"])), "ic_wind")
+    return Case(name, nx, ny, nz, dx, dx, ztop, zeta, zs, r0, t0, p0, sp, st, dt, nsub, steps)
+
+
+def nest_hierarchy(pnx=384, pny=384, nz=60, dx=3000.0, ztop=20000.0, sigma=300.0, L=20000.0, seed=20240611,
+                   theta_amp=0.5, U=10.0, bv=0.01, theta_sfc=300.0, dt=18.0, nsub=6, steps=300, nests=None,
+                   sponge_alpha=0.1):
+    """Parent 384x384x60 at 3 km with three ratio-3 nests (271x271, 211x331,
+    151x151 at 1 km) and one second-level nest (181x181 at 333 m inside the
+    first), sharing one 60-level stretched vertical coordinate.  Seeded
+    Gaussian random terrain (sigma 300 m, correlation length 20 km) on the
+    parent; nest terrain is the bilinear interpolation of their parent's.
+    Stable base state N = 0.01 1/s, uniform U, random-phase theta' of
+    amplitude theta_amp below 4 km on the parent.  Nest states start as the
+    parent's interpolation (vibe_mi.nesting.NestedRun does that on the
+    device).  Time steps refine with the grid: dt, dt/3, dt/9.
+
+    Returns (cases, links) with links = [(child, parent, ratio, i0, j0)],
+    i0 / j0 counted in child cells."""
+    if nests is None:
+        # (nx, ny, parent, offset in parent cells)
+        nests = [(271, 271, 0, (40, 40)), (211, 331, 0, (200, 30)), (151, 151, 0, (150, 250)),
+                 (181, 181, 1, (45, 45))]
+    r = 3
+    zeta = _levels(nz, ztop, "tanh", 0.3 * ztop / nz, 1.8 * ztop / nz)
+    zs = np.zeros(pnx * pny)
+    _check(lib().vibe_mi_terrain_gaussian(pnx, pny, dx, dx, sigma, L, seed, _pd(zs)), "terrain_gaussian")
+    parent = _nest_domain("nest_parent", pnx, pny, nz, dx, ztop, zeta, zs, theta_sfc, bv, U, dt, nsub, steps,
+                          sponge_alpha)
+    # random-phase theta' on the parent: 16 modes, integer wavenumbers 1..8
+    rng = np.random.default_rng(seed)
+    x = (np.arange(pnx) + 0.5) / pnx
+    y = (np.arange(pny) + 0.5) / pny
+    X, Y = np.meshgrid(x, y)
+    plane = np.zeros_like(X)
+    for _ in range(16):
+        kx, ky = rng.integers(1, 9, size=2)
+        plane += np.cos(2 * np.pi * (kx * X + ky * Y) + rng.uniform(0, 2 * np.pi))
+    plane *= theta_amp / np.max(np.abs(plane))
+    zc = 0.5 * (zeta[:-1] + zeta[1:])
+    Z = zs.reshape(pny, pnx)[None, :, :] + zc[:, None, None] * (ztop - zs.reshape(pny, pnx))[None, :, :]
+    env = np.where(Z < 2000.0, 1.0, np.where(Z < 4000.0, np.cos(0.5 * np.pi * (Z - 2000.0) / 2000.0) ** 2, 0.0))
+    parent.state["theta"] = np.ascontiguousarray((env * plane[None, :, :]).ravel())
+    _balance(parent.state, parent.rho0, parent.theta0)
+    parent.meta.update(seed=seed, level=0)
+    cases, links = [parent], []
+    for q, (nx, ny, par, (pi0, pj0)) in enumerate(nests):
+        P = cases[par]
+        czs = _bilinear_terrain(P.zs, P.nx, P.ny, nx, ny, r, r * pi0, r * pj0)
+        level = P.meta["level"] + 1
+        c = _nest_domain(f"nest{q + 1}", nx, ny, nz, P.dx / r, ztop, zeta, czs, theta_sfc, bv, U, P.dt / r, nsub,
+                         steps * r ** level, sponge_alpha)
+        c.meta.update(level=level)
+        cases.append(c)
+        links.append((q + 1, par, r, r * pi0, r * pj0))
+    return cases, links
 
 
 CONFIGS = {
