@@ -42,8 +42,7 @@ struct BDom {
   FieldSet S[3];        // FP64 states; S[cur0] = s^n
   AcousticSet<T> A[2];  // acoustic ping-pong buffers
   T *om;                // Omega
-  const T *tab[3];      // HEVI factorisation of stage st: {beta, 1/pivot, c'} x tab_per
-  size_t tab_per;
+  HeviTab<T> tab[3];    // HEVI factorisation and coefficients of stage st
   double h[3];  // stage lengths dt/3, dt/2, dt
   T dtau[3];    // substep lengths h/nsub
   T an, ao;     // HEVI off-centring weights
@@ -158,17 +157,13 @@ __global__ void __launch_bounds__(64 * NW) kb_substep(const BDom<T> *__restrict_
     const BDom<T> &dd = D[tl.x];
     const Geo<T> g = dd.g;
     const AcousticSet<T> in = dd.A[q & 1];
-    const T *tab = dd.tab[st];
-    const size_t per = dd.tab_per;
     SubArgs<T, T, O> sa{};
     sa.u = in.u;
     sa.v = in.v;
     sa.w = in.w;
     sa.rho = in.rho;
     sa.pi = in.pi;
-    sa.tA = tab;
-    sa.tINV = tab + per;
-    sa.tCP = tab + 2 * per;
+    sa.tab = dd.tab[st];
     sa.dtau = dd.dtau[st];
     sa.an = dd.an;
     sa.ao = dd.ao;
@@ -384,11 +379,10 @@ int upload_desc(vibe_mi_batch *b, const double *dt, int nsub) {
     for (int st = 0; st < 3; ++st) {
       const double dtau = hk[st] / static_cast<double>(nsub);
       if (int rc = domain_ensure_table(d, st, dtau)) return rc;
-      x.tab[st] = static_cast<const T *>(domain_table(d, st));
+      x.tab[st] = HeviTab<T>::at(static_cast<const T *>(domain_table(d, st)), d->nz, d->rk);
       x.h[st] = hk[st];
       x.dtau[st] = static_cast<T>(dtau);
     }
-    x.tab_per = static_cast<size_t>(d->nz - 1) * d->rk;
     x.an = static_cast<T>(0.5 * (1.0 + d->eps_s));
     x.ao = static_cast<T>(0.5 * (1.0 - d->eps_s));
     x.dt = dt[m];

@@ -48,7 +48,7 @@ struct vibe_mi_domain {
   AcousticSet<double> A64[2];
   AcousticSet<float> A32[2];
   void *om = nullptr;   // Omega (T)
-  void *tab = nullptr;  // HEVI factorisation tables: 4 slots x {a, 1/pivot, c'} x (nz-1) x rk (T)
+  void *tab = nullptr;  // HEVI tables: 4 slots of hevi_slot_elems(nz, rk) entries (T)
   double tab_dtau[4] = {-1.0, -1.0, -1.0, -1.0};
   int nxr = 1, nyr = 1, rk = 1;
   int nw_fp32 = 4, nw_fp64 = 8;  // waves per substep workgroup (autotuned)
@@ -70,7 +70,7 @@ namespace vibe_mi {
 int domain_create(const vibe_mi_domain_desc *ds, vibe_mi_domain **out);
 // Factorise the HEVI matrix of table slot `slot` for dtau (cached per slot).
 int domain_ensure_table(vibe_mi_domain *d, int slot, double dtau);
-// Base of table slot `slot` ({beta, 1/pivot, c'} x (nz-1) x rk of type T).
+// Base of table slot `slot` (layout of HeviTab, entries of type T).
 void *domain_table(vibe_mi_domain *d, int slot);
 
 }  // namespace vibe_mi

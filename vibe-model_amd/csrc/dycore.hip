@@ -84,6 +84,25 @@ int build_geo(vibe_mi_domain *d, const vibe_mi_domain_desc *ds, Geo<T> &g) {
         kap[q] = (kRd / kCv) * ds->pi0[c] / ds->rho0[c];
         lam[q] = (kRd / kCv) * ds->pi0[c] / ds->theta0[c];
       }
+  // face / interface products of the reference state (same association as
+  // the oracle's expressions: cp * ((t_l + t_r) / 2) and D * ((r_l + r_r) / 2))
+  std::vector<double> cthu(nr), cthv(nr), rhu(nr), rhv(nr), r0wv(static_cast<size_t>(rk) * (nz + 1), 0.0);
+  {
+    auto cidx = [&](int ir, int jr) { return static_cast<size_t>((jr + nyr) % nyr) * nxr + (ir + nxr) % nxr; };
+    for (int k = 0; k < nz; ++k)
+      for (int jr = 0; jr < nyr; ++jr)
+        for (int ir = 0; ir < nxr; ++ir) {
+          const size_t q = static_cast<size_t>(k) * rk + cidx(ir, jr);
+          const size_t qw = static_cast<size_t>(k) * rk + cidx(ir - 1, jr);
+          const size_t qs = static_cast<size_t>(k) * rk + cidx(ir, jr - 1);
+          const size_t col = cidx(ir, jr);
+          cthu[q] = kCp * (0.5 * (t0[qw] + t0[q]));
+          cthv[q] = kCp * (0.5 * (t0[qs] + t0[q]));
+          rhu[q] = Du[col] * (0.5 * (r0[qw] + r0[q]));
+          rhv[q] = Dv[col] * (0.5 * (r0[qs] + r0[q]));
+          if (k >= 1) r0wv[q] = 0.5 * (r0[q - rk] + r0[q]);
+        }
+  }
   auto cvt = [](const std::vector<double> &a) { return std::vector<T>(a.begin(), a.end()); };
   g.nx = nx;
   g.ny = ny;
@@ -114,6 +133,11 @@ int build_geo(vibe_mi_domain *d, const vibe_mi_domain_desc *ds, Geo<T> &g) {
   VMI_UP(theta0, t0)
   VMI_UP(kap, kap)
   VMI_UP(lam, lam)
+  VMI_UP(cthu, cthu)
+  VMI_UP(cthv, cthv)
+  VMI_UP(rhu, rhu)
+  VMI_UP(rhv, rhv)
+  VMI_UP(r0w, r0wv)
 #undef VMI_UP
   return 0;
 }
@@ -170,19 +194,17 @@ int run_stage(vibe_mi_domain *d, const Geo<T> &g, const FieldSet &cur, const Fie
 
 template <typename T>
 T *table_slot(vibe_mi_domain *d, int slot) {
-  const size_t per = static_cast<size_t>(d->nz > 1 ? d->nz - 1 : 1) * d->rk;
-  return static_cast<T *>(d->tab) + static_cast<size_t>(slot) * 3 * per;
+  return static_cast<T *>(d->tab) + static_cast<size_t>(slot) * hevi_slot_elems(d->nz, d->rk);
 }
 
 // Factorise the HEVI matrix for dtau into table slot `slot` (cached).
 template <typename T>
 int ensure_table(vibe_mi_domain *d, const Geo<T> &g, int slot, double dtau) {
   if (d->tab_dtau[slot] == dtau) return 0;
-  const size_t per = static_cast<size_t>(d->nz - 1) * d->rk;
   T *t = table_slot<T>(d, slot);
   const T an = static_cast<T>(0.5 * (1.0 + d->eps_s));
   hipLaunchKernelGGL((k_hevi_factor<T>), dim3(ceil_div(d->rk, 64)), dim3(64), 0, d->stream, g, static_cast<T>(dtau),
-                     an, t, t + per, t + 2 * per);
+                     an, t);
   VMI_HIP(hipGetLastError());
   d->tab_dtau[slot] = dtau;
   return 0;
@@ -194,7 +216,6 @@ template <typename T, typename I, typename O, int NW, int MAXCH>
 int launch_substep(vibe_mi_domain *d, const Geo<T> &g, const T *tab, const I *u, const I *v, const I *w,
                    const I *rho, const I *pi, O *uo, O *vo, O *wo, O *rho_o, O *pi_o, T dtau, T an, T ao,
                    int check) {
-  const size_t per = static_cast<size_t>(d->nz - 1) * d->rk;
   const size_t lds = 3 * static_cast<size_t>(d->nz - 1) * 64 * sizeof(T);
   const size_t lds_static = 3 * NW * 64 * sizeof(T);
   if (lds + lds_static > kMaxLds) return VIBE_C_ERR_BAD_ARG;  // column too tall for one workgroup
@@ -204,7 +225,7 @@ int launch_substep(vibe_mi_domain *d, const Geo<T> &g, const T *tab, const I *u,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kMaxLds - lds_static)));
     attr_set = true;
   }
-  const SubArgs<T, I, O> sa{u, v, w, rho, pi, tab, tab + per, tab + 2 * per, dtau, an, ao,
+  const SubArgs<T, I, O> sa{u, v, w, rho, pi, HeviTab<T>::at(tab, d->nz, d->rk), dtau, an, ao,
                             uo, vo, wo, rho_o, pi_o, check, d->flag};
   hipLaunchKernelGGL((k_substep<T, I, O, NW, MAXCH>), dim3(ceil_div(d->nx, 64), d->ny), dim3(64 * NW), lds, d->stream,
                      g, sa);
@@ -481,7 +502,7 @@ int create_impl(const vibe_mi_domain_desc *ds, vibe_mi_domain *d) {
   char *p;
   if (int rc = dalloc(d, &p, d->nwf * tsz)) return rc;
   d->om = p;
-  if (int rc = dalloc(d, &p, 4 * 3 * static_cast<size_t>(d->nz - 1) * d->rk * tsz)) return rc;
+  if (int rc = dalloc(d, &p, 4 * hevi_slot_elems(d->nz, d->rk) * tsz)) return rc;
   d->tab = p;
   if (int rc = dalloc(d, &d->flag, 4)) return rc;
   if (int rc = dalloc(d, &d->part, static_cast<size_t>(kRedBlocks) * kEnergyComps)) return rc;

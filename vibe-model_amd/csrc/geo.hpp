@@ -24,6 +24,29 @@ struct Geo {
   const T *rdzw;    // nz+1 1/(zc[k]-zc[k-1]) for 1..nz-1, 0 elsewhere
   const T *Du, *Dv, *sx, *sy, *rDc, *rDu, *rDv, *metx, *mety;  // columns
   const T *rho0, *theta0, *kap, *lam;                           // 3-D reference
+  // face and interface products of the reference state (3-D compact, like
+  // rho0): cthu = cp (theta0_{i-1} + theta0_i)/2 and rhu = Du (rho0_{i-1} +
+  // rho0_i)/2 at u faces, cthv / rhv at v faces, r0w = (rho0_{k-1} + rho0_k)/2
+  // at interfaces (index k*rk + column, 0 at k = 0 and nz)
+  const T *cthu, *cthv, *rhu, *rhv, *r0w;
+};
+
+// Layout of one HEVI table slot (a stage's factorisation for one dtau):
+// per = (nz-1)*rk entries each of beta, 1/pivot, c' and gk (interfaces
+// 1..nz-1), then nz*rk entries of ck (levels), all [level][compact column].
+//   gk = dtau/D cp theta0_w / dzw   (implicit vertical PGF coefficient)
+//   ck = kappa dtau/D / dzeta       (vertical flux-divergence coefficient)
+__host__ __device__ inline size_t hevi_per(int nz, int rk) { return static_cast<size_t>(nz - 1) * rk; }
+__host__ __device__ inline size_t hevi_slot_elems(int nz, int rk) {
+  return 4 * hevi_per(nz, rk) + static_cast<size_t>(nz) * rk;
+}
+template <typename T>
+struct HeviTab {
+  const T *A, *INV, *CP, *GK, *CK;
+  __host__ __device__ static HeviTab at(const T *base, int nz, int rk) {
+    const size_t per = hevi_per(nz, rk);
+    return HeviTab{base, base + per, base + 2 * per, base + 3 * per, base + 4 * per};
+  }
 };
 
 __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }

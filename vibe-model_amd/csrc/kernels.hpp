@@ -385,15 +385,21 @@ __global__ void __launch_bounds__(256) k_stage_col(Geo<T> g, StageIn cur, StageI
 // per (stage, column): beta_m = a_m/pivot_m, 1/pivot_m and c'_m, stored
 // [m][compact column].  One thread per compact column.
 template <typename T>
-__global__ void __launch_bounds__(64) k_hevi_factor(Geo<T> g, T dtau, T an, T *__restrict__ tA, T *__restrict__ tINV,
-                                                    T *__restrict__ tCP) {
+__global__ void __launch_bounds__(64) k_hevi_factor(Geo<T> g, T dtau, T an, T *__restrict__ tab) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.rk) return;
   const int nz = g.nz, n = nz - 1, rk = g.rk;
+  const size_t per = hevi_per(nz, rk);
+  T *__restrict__ tA = tab;
+  T *__restrict__ tINV = tab + per;
+  T *__restrict__ tCP = tab + 2 * per;
+  T *__restrict__ tGK = tab + 3 * per;
+  T *__restrict__ tCK = tab + 4 * per;
   const T dtD = dtau * C(g.rDc)[c];
   const T an2 = an * an;
   auto ck = [&](int k) { return C(g.kap)[k * rk + c] * dtD * C(g.rdzeta)[k]; };
-  auto r0w = [&](int k) { return T(0.5) * (C(g.rho0)[(k - 1) * rk + c] + C(g.rho0)[k * rk + c]); };
+  auto r0w = [&](int k) { return C(g.r0w)[k * rk + c]; };
+  for (int k = 0; k < nz; ++k) tCK[k * rk + c] = ck(k);
   T cp_prev = T(0);
   for (int m = 0; m < n; ++m) {
     const int k = m + 1;
@@ -410,6 +416,7 @@ __global__ void __launch_bounds__(64) k_hevi_factor(Geo<T> g, T dtau, T an, T *_
     tA[m * rk + c] = a * inv;
     tINV[m * rk + c] = inv;
     tCP[m * rk + c] = cp;
+    tGK[m * rk + c] = gk;
     cp_prev = cp;
   }
 }
@@ -429,7 +436,7 @@ __global__ void __launch_bounds__(64) k_hevi_factor(Geo<T> g, T dtau, T an, T *_
 template <typename T, typename I, typename O>
 struct SubArgs {
   const I *u, *v, *w, *rho, *pi;  // acoustic state in
-  const T *tA, *tINV, *tCP;       // HEVI factorisation of this stage
+  HeviTab<T> tab;                 // HEVI factorisation and coefficients of this stage
   T dtau, an, ao;
   O *uo, *vo, *wo, *rho_o, *pi_o;  // acoustic state out
   int check;                       // 1: flag non-finite outputs
@@ -445,9 +452,11 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   const I *__restrict__ w = sa.w;
   const I *__restrict__ rho = sa.rho;
   const I *__restrict__ pi = sa.pi;
-  const T *__restrict__ tA = sa.tA;
-  const T *__restrict__ tINV = sa.tINV;
-  const T *__restrict__ tCP = sa.tCP;
+  const T *__restrict__ tA = sa.tab.A;
+  const T *__restrict__ tINV = sa.tab.INV;
+  const T *__restrict__ tCP = sa.tab.CP;
+  const T *__restrict__ tGK = sa.tab.GK;
+  const T *__restrict__ tCK = sa.tab.CK;
   const T dtau = sa.dtau, an = sa.an, ao = sa.ao;
   O *__restrict__ uo = sa.uo;
   O *__restrict__ vo = sa.vo;
@@ -458,7 +467,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   int *__restrict__ flag = sa.flag;
   extern __shared__ unsigned char smem_raw[];
   const int nlds = (g.nz - 1) * 64;
-  T *s_td = reinterpret_cast<T *>(smem_raw);  // [nz-1][64] RHS -> d' -> w
+  T *s_td = reinterpret_cast<T *>(smem_raw);  // [nz-1][64] old-w RHS part -> RHS -> d' -> w
   T *s_be = s_td + nlds;                      // [nz-1][64] beta
   T *s_cp = s_be + nlds;                      // [nz-1][64] c'
   __shared__ T s_mc[NW][64], s_fh[NW][64], s_ph[NW][64];
@@ -479,15 +488,16 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   const uint32_t cC = static_cast<uint32_t>(j) * unx + i, cW = static_cast<uint32_t>(j) * unx + iW,
                  cE = static_cast<uint32_t>(j) * unx + iE, cS = static_cast<uint32_t>(jS) * unx + i,
                  cN = static_cast<uint32_t>(jN) * unx + i;
-  const uint32_t gC = gcol(g, i, j), gW = gcol(g, iW, j), gE = gcol(g, iE, j), gS = gcol(g, i, jS), gN = gcol(g, i, jN);
+  const uint32_t gC = gcol(g, i, j), gE = gcol(g, iE, j), gN = gcol(g, i, jN);
   const T rD = C(g.rDc)[gC];
   const T dtD = dtau * rD;
-  const T Du_w = C(g.Du)[gC], Du_e = C(g.Du)[gE], Dv_s = C(g.Dv)[gC], Dv_n = C(g.Dv)[gN];
   const T sx_w = C(g.sx)[gC], sx_e = C(g.sx)[gE], sy_s = C(g.sy)[gC], sy_n = C(g.sy)[gN];
   const T mx_w = C(g.metx)[gC], mx_e = C(g.metx)[gE], my_s = C(g.mety)[gC], my_n = C(g.mety)[gN];
   const T cpdt = dtau;
 
-  T E[MAXCH], Q[MAXCH], Wt[MAXCH], Fh[MAXCH], Ph[MAXCH];
+  // per level of this wave: explicit rho' and pi' parts and the old-w part
+  // of the vertical mass flux (the old-w part of the w RHS goes to s_td)
+  T E[MAXCH], Q[MAXCH], Fh[MAXCH];
   T mc_first = T(0), mc_last = T(0), mc0 = T(0), aw_first = T(0), r0w_first = T(0);
 
   // ---------------- phase A ----------------
@@ -515,8 +525,6 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
 #pragma unroll
       for (int q = 0; q < 5; ++q) gl[q] = T(0);
     }
-    T t0C_prev = (k0 >= 1) ? ldc32(g.theta0, static_cast<size_t>(k0 - 1) * rk, gC) : T(0);
-    T r0C_prev = (k0 >= 1) ? ldc32(g.rho0, static_cast<size_t>(k0 - 1) * rk, gC) : T(0);
     T mc_prev = T(0);
 #pragma unroll
     for (int c = 0; c < MAXCH; ++c) {
@@ -537,33 +545,28 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         const T px_e = (p0[2] - p0[0]) * g.rdx - (omz * mx_e) * (T(0.5) * (Gc[0] + Gc[2]));
         const T py_s = (p0[0] - p0[3]) * g.rdy - (omz * my_s) * (T(0.5) * (Gc[3] + Gc[0]));
         const T py_n = (p0[4] - p0[0]) * g.rdy - (omz * my_n) * (T(0.5) * (Gc[0] + Gc[4]));
-        const int rb = k * rk;
-        const T t0C = ldc32(g.theta0, static_cast<size_t>(rb), gC), t0W = ldc32(g.theta0, static_cast<size_t>(rb), gW), t0E = ldc32(g.theta0, static_cast<size_t>(rb), gE), t0S = ldc32(g.theta0, static_cast<size_t>(rb), gS),
-                t0N = ldc32(g.theta0, static_cast<size_t>(rb), gN);
-        const T un_w = T(ld32(u, b, cC)) - cpdt * (T(kCp) * (T(0.5) * (t0W + t0C)) * px_w);
-        const T un_e = T(ld32(u, b, cE)) - cpdt * (T(kCp) * (T(0.5) * (t0C + t0E)) * px_e);
-        const T vn_s = T(ld32(v, b, cC)) - cpdt * (T(kCp) * (T(0.5) * (t0S + t0C)) * py_s);
-        const T vn_n = T(ld32(v, b, cN)) - cpdt * (T(kCp) * (T(0.5) * (t0C + t0N)) * py_n);
+        const size_t rb = static_cast<size_t>(k) * rk;
+        const T un_w = T(ld32(u, b, cC)) - cpdt * (ldc32(g.cthu, rb, gC) * px_w);
+        const T un_e = T(ld32(u, b, cE)) - cpdt * (ldc32(g.cthu, rb, gE) * px_e);
+        const T vn_s = T(ld32(v, b, cC)) - cpdt * (ldc32(g.cthv, rb, gC) * py_s);
+        const T vn_n = T(ld32(v, b, cN)) - cpdt * (ldc32(g.cthv, rb, gN) * py_n);
         if (active) {
           st32(uo, b, cC, static_cast<O>(un_w));
           st32(vo, b, cC, static_cast<O>(vn_s));
         }
         const T mc = T(0.5) * (un_w * sx_w + un_e * sx_e) + T(0.5) * (vn_s * sy_s + vn_n * sy_n);
-        const T r0C = ldc32(g.rho0, static_cast<size_t>(rb), gC), r0W = ldc32(g.rho0, static_cast<size_t>(rb), gW), r0E = ldc32(g.rho0, static_cast<size_t>(rb), gE), r0S = ldc32(g.rho0, static_cast<size_t>(rb), gS),
-                r0N = ldc32(g.rho0, static_cast<size_t>(rb), gN);
-        const T hd = (Du_e * (T(0.5) * (r0C + r0E)) * un_e - Du_w * (T(0.5) * (r0W + r0C)) * un_w) * g.rdx +
-                     (Dv_n * (T(0.5) * (r0C + r0N)) * vn_n - Dv_s * (T(0.5) * (r0S + r0C)) * vn_s) * g.rdy;
-        const T cq = ldc32(g.kap, static_cast<size_t>(rb), gC) * dtD;
+        const T hd = (ldc32(g.rhu, rb, gE) * un_e - ldc32(g.rhu, rb, gC) * un_w) * g.rdx +
+                     (ldc32(g.rhv, rb, gN) * vn_n - ldc32(g.rhv, rb, gC) * vn_s) * g.rdy;
+        const T cq = ldc32(g.kap, rb, gC) * dtD;
         E[c] = T(ld32(rho, b, cC)) - dtD * hd;
         Q[c] = p0[0] - cq * hd;
         const T wk = T(ld32(w, b, cC));
         Fh[c] = T(0);
-        Wt[c] = T(0);
         if (k >= 1) {
-          const T th0w = T(0.5) * (t0C_prev + t0C);
-          const T gk = dtD * T(kCp) * th0w * C(g.rdzw)[k];
-          Wt[c] = wk - gk * ao * (p0[0] - pmC);
-          const T r0w = T(0.5) * (r0C_prev + r0C);
+          const size_t mb = static_cast<size_t>(k - 1) * rk;
+          const T gk = ldc32(tGK, mb, gC);
+          s_td[(k - 1) * 64 + lane] = wk - gk * ao * (p0[0] - pmC);
+          const T r0w = ldc32(g.r0w, rb, gC);
           if (c > 0) {
             const T Mw = (T(1) - C(g.zeta)[k]) * (T(0.5) * (mc_prev + mc));
             Fh[c] = r0w * (ao * wk - Mw);
@@ -575,8 +578,6 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         if (c == 0) mc_first = mc;
         if (k == 0) mc0 = mc;
         mc_prev = mc;
-        t0C_prev = t0C;
-        r0C_prev = r0C;
         pmC = p0[0];
 #pragma unroll
         for (int q = 0; q < 5; ++q) p0[q] = pp[q];
@@ -587,6 +588,9 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   }
   __syncthreads();
   // ---------------- phase B: boundary vertical flux, P^, RHS ----------------
+  // P^ = Q - ck (Fh_{k+1} - Fh_k) is the pi' estimate with the old-w fluxes;
+  // the RHS of interface k is the old-w part minus gk an (P^_k - P^_{k-1}),
+  // scaled by 1/pivot for the factored sweep.
   if (nlev > 0) {
     if (k0 >= 1) {
       const T Mw = (T(1) - C(g.zeta)[k0]) * (T(0.5) * (s_mc[wv - 1][lane] + mc_first));
@@ -599,34 +603,33 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
     const T fh_next_chunk = (k0 + nlev < nz) ? s_fh[wv + 1][lane] : T(0);
 #pragma unroll
     for (int c = 0; c < MAXCH; ++c) {
-      if (c < nlev) {
+      if (c == nlev - 1) {
         const int k = k0 + c;
-        const T fhn = (c + 1 < nlev) ? Fh[(c + 1) < MAXCH ? c + 1 : c] : fh_next_chunk;
-        const T ck = ldc32(g.kap, static_cast<size_t>(k) * rk, gC) * dtD * C(g.rdzeta)[k];
-        Ph[c] = Q[c] - ck * (fhn - Fh[c]);
+        const T ck = ldc32(tCK, static_cast<size_t>(k) * rk, gC);
+        s_ph[wv][lane] = Q[c] - ck * (fh_next_chunk - Fh[c]);
       }
     }
-#pragma unroll
-    for (int c = 0; c < MAXCH; ++c)
-      if (c == nlev - 1) s_ph[wv][lane] = Ph[c];
   }
   __syncthreads();
   if (nlev > 0) {
-    const T ph_prev_chunk = (k0 >= 1) ? s_ph[wv - 1][lane] : T(0);
+    const T fh_next_chunk = (k0 + nlev < nz) ? s_fh[wv + 1][lane] : T(0);
+    T php = (k0 >= 1) ? s_ph[wv - 1][lane] : T(0);
 #pragma unroll
     for (int c = 0; c < MAXCH; ++c) {
       if (c < nlev) {
         const int k = k0 + c;
+        const T fhn = (c + 1 < nlev) ? Fh[(c + 1) < MAXCH ? c + 1 : c] : fh_next_chunk;
+        const T ph = Q[c] - ldc32(tCK, static_cast<size_t>(k) * rk, gC) * (fhn - Fh[c]);
         if (k >= 1) {
-          const T php = (c > 0) ? Ph[c > 0 ? c - 1 : 0] : ph_prev_chunk;
-          const T th0w = T(0.5) * (ldc32(g.theta0, static_cast<size_t>(k - 1) * rk, gC) + ldc32(g.theta0, static_cast<size_t>(k) * rk, gC));
-          const T gk = dtD * T(kCp) * th0w * C(g.rdzw)[k];
           const int m = k - 1;
-          const T td = Wt[c] - gk * an * (Ph[c] - php);
-          s_td[m * 64 + lane] = td * ldc32(tINV, static_cast<size_t>(m) * rk, gC);
-          s_be[m * 64 + lane] = ldc32(tA, static_cast<size_t>(m) * rk, gC);
-          s_cp[m * 64 + lane] = ldc32(tCP, static_cast<size_t>(m) * rk, gC);
+          const size_t mb = static_cast<size_t>(m) * rk;
+          const T gk = ldc32(tGK, mb, gC);
+          const T td = s_td[m * 64 + lane] - gk * an * (ph - php);
+          s_td[m * 64 + lane] = td * ldc32(tINV, mb, gC);
+          s_be[m * 64 + lane] = ldc32(tA, mb, gC);
+          s_cp[m * 64 + lane] = ldc32(tCP, mb, gC);
         }
+        php = ph;
       }
     }
   }
@@ -702,7 +705,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
     int bad = 0;
     auto fluxF = [&](int k, T fh) {
       if (k <= 0 || k >= nz) return T(0);
-      const T r0w = T(0.5) * (ldc32(g.rho0, static_cast<size_t>(k - 1) * rk, gC) + ldc32(g.rho0, static_cast<size_t>(k) * rk, gC));
+      const T r0w = ldc32(g.r0w, static_cast<size_t>(k) * rk, gC);
       return fh + (an * r0w) * s_td[(k - 1) * 64 + lane];
     };
     const T fh_next_chunk = (k0 + nlev < nz) ? s_fh[wv + 1][lane] : T(0);
