@@ -150,7 +150,7 @@ __global__ void __launch_bounds__(64 * TY) kb_stage(const BDom<T> *__restrict__ 
 // stage (last = 1, O = double) writes the stage output S[out] and checks
 // finiteness; the others write the acoustic buffer A[(q+1)&1] (O = T).
 template <typename T, typename O, int NW, int MAXCH>
-__global__ void __launch_bounds__(64 * NW) kb_substep(const BDom<T> *__restrict__ D, TileList L, int st, int q,
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kSubstepWavesPerEu<T, NW, MAXCH>))) kb_substep(const BDom<T> *__restrict__ D, TileList L, int st, int q,
                                                       int last, int *__restrict__ ctr) {
   tile_loop(ctr, L, [&](int t) {
     const int4 tl = L.tiles[t];

@@ -443,6 +443,12 @@ struct SubArgs {
   int *flag;
 };
 
+// Waves per SIMD the substep kernels are compiled for: FP32 shapes with up
+// to 12 levels per wave are held to 128 VGPRs (four waves per SIMD); the
+// other shapes are register-limited by their own size.
+template <typename T, int NW, int MAXCH>
+inline constexpr int kSubstepWavesPerEu = (sizeof(T) == 4 && MAXCH <= 12) ? 4 : 1;
+
 // Row of 64 columns starting at x-block bx, row j; all 64*NW threads of the
 // workgroup call it together (it synchronises the workgroup).
 template <typename T, typename I, typename O, int NW, int MAXCH>
@@ -525,12 +531,26 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
 #pragma unroll
       for (int q = 0; q < 5; ++q) gl[q] = T(0);
     }
+    // the state operands of a level are loaded one level ahead, so their
+    // latency overlaps the previous level's arithmetic
+    auto ld_state = [&](int k, T *x) {
+      const size_t b = static_cast<size_t>(k) * nxy;
+      x[0] = T(ld32(u, b, cC));
+      x[1] = T(ld32(u, b, cE));
+      x[2] = T(ld32(v, b, cC));
+      x[3] = T(ld32(v, b, cN));
+      x[4] = T(ld32(w, b, cC));
+      x[5] = T(ld32(rho, b, cC));
+    };
+    T xs[6], xn[6];
+    ld_state(k0, xs);
     T mc_prev = T(0);
 #pragma unroll
     for (int c = 0; c < MAXCH; ++c) {
       if (c < nlev) {
         const int k = k0 + c;
         const size_t b = static_cast<size_t>(k) * nxy;
+        if (c + 1 < nlev) ld_state(k + 1, xn);
         if (k + 1 < nz) ld5(k + 1, pp); else for (int q = 0; q < 5; ++q) pp[q] = p0[q];
         const T rz_hi = C(g.rdzw)[k + 1];
         T Gc[5];
@@ -546,10 +566,10 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         const T py_s = (p0[0] - p0[3]) * g.rdy - (omz * my_s) * (T(0.5) * (Gc[3] + Gc[0]));
         const T py_n = (p0[4] - p0[0]) * g.rdy - (omz * my_n) * (T(0.5) * (Gc[0] + Gc[4]));
         const size_t rb = static_cast<size_t>(k) * rk;
-        const T un_w = T(ld32(u, b, cC)) - cpdt * (ldc32(g.cthu, rb, gC) * px_w);
-        const T un_e = T(ld32(u, b, cE)) - cpdt * (ldc32(g.cthu, rb, gE) * px_e);
-        const T vn_s = T(ld32(v, b, cC)) - cpdt * (ldc32(g.cthv, rb, gC) * py_s);
-        const T vn_n = T(ld32(v, b, cN)) - cpdt * (ldc32(g.cthv, rb, gN) * py_n);
+        const T un_w = xs[0] - cpdt * (ldc32(g.cthu, rb, gC) * px_w);
+        const T un_e = xs[1] - cpdt * (ldc32(g.cthu, rb, gE) * px_e);
+        const T vn_s = xs[2] - cpdt * (ldc32(g.cthv, rb, gC) * py_s);
+        const T vn_n = xs[3] - cpdt * (ldc32(g.cthv, rb, gN) * py_n);
         if (active) {
           st32(uo, b, cC, static_cast<O>(un_w));
           st32(vo, b, cC, static_cast<O>(vn_s));
@@ -558,9 +578,9 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         const T hd = (ldc32(g.rhu, rb, gE) * un_e - ldc32(g.rhu, rb, gC) * un_w) * g.rdx +
                      (ldc32(g.rhv, rb, gN) * vn_n - ldc32(g.rhv, rb, gC) * vn_s) * g.rdy;
         const T cq = ldc32(g.kap, rb, gC) * dtD;
-        E[c] = T(ld32(rho, b, cC)) - dtD * hd;
+        E[c] = xs[5] - dtD * hd;
         Q[c] = p0[0] - cq * hd;
-        const T wk = T(ld32(w, b, cC));
+        const T wk = xs[4];
         Fh[c] = T(0);
         if (k >= 1) {
           const size_t mb = static_cast<size_t>(k - 1) * rk;
@@ -581,6 +601,8 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         pmC = p0[0];
 #pragma unroll
         for (int q = 0; q < 5; ++q) p0[q] = pp[q];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) xs[q] = xn[q];
       }
     }
     mc_last = mc_prev;
@@ -739,7 +761,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
 }
 
 template <typename T, typename I, typename O, int NW, int MAXCH>
-__global__ void __launch_bounds__(64 * NW) k_substep(Geo<T> g, SubArgs<T, I, O> sa) {
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kSubstepWavesPerEu<T, NW, MAXCH>))) k_substep(Geo<T> g, SubArgs<T, I, O> sa) {
   substep_row<T, I, O, NW, MAXCH>(g, sa, blockIdx.x, blockIdx.y);
 }
 
