@@ -141,8 +141,7 @@ __global__ void __launch_bounds__(64 * TY) kb_stage(const BDom<T> *__restrict__ 
     const StageIn ci{cur.u, cur.v, cur.w, cur.rho, cur.th, cur.pi};
     const StageIn si{s0.u, s0.v, s0.w, s0.rho, s0.th, s0.pi};
     const StageOut<T> so{a.u, a.v, a.w, a.rho, a.pi, nxt.th};
-    const int i = tl.y * 64 + static_cast<int>(threadIdx.x), j = tl.z * TY + static_cast<int>(threadIdx.y);
-    if (i < g.nx && j < g.ny) stage_col<T, T>(g, ci, si, dd.om, dd.h[st], so, dd.flag, i, j);
+    stage_tile<T, T, TY>(g, ci, si, dd.om, dd.h[st], so, dd.flag, tl.y, tl.z);
   });
 }
 

@@ -186,8 +186,8 @@ int run_stage(vibe_mi_domain *d, const Geo<T> &g, const FieldSet &cur, const Fie
   const StageIn ci{cur.u, cur.v, cur.w, cur.rho, cur.th, cur.pi};
   const StageIn si{s0.u, s0.v, s0.w, s0.rho, s0.th, s0.pi};
   const StageOut<T> so{a.u, a.v, a.w, a.rho, a.pi, nxt.th};
-  hipLaunchKernelGGL((k_stage_col<T, T>), dim3(ceil_div(d->nx, 64), ceil_div(d->ny, 4)), dim3(64, 4), 0, d->stream, g,
-                     ci, si, static_cast<const T *>(d->om), h, so, d->flag);
+  hipLaunchKernelGGL((k_stage_col<T, T, 4>), dim3(ceil_div(d->nx, 64), ceil_div(d->ny, 4)), dim3(64, 4), 0, d->stream,
+                     g, ci, si, static_cast<const T *>(d->om), h, so, d->flag);
   VMI_HIP(hipGetLastError());
   return 0;
 }

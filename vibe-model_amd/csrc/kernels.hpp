@@ -123,33 +123,67 @@ struct StageOut {
   double *th;
 };
 
-template <typename T, typename O>
-__device__ __forceinline__ void stage_col(const Geo<T> &g, const StageIn &cur, const StageIn &s0,
-                                          const T *__restrict__ om, double h, const StageOut<O> &out,
-                                          int *__restrict__ flag, int i, int j) {
+template <typename T, typename O, int TY>
+__device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, const StageIn &s0,
+                                           const T *__restrict__ om, double h, const StageOut<O> &out,
+                                           int *__restrict__ flag, int bx, int by) {
+  // level k of u, v, w, theta', rho' over the tile and a 3-cell halo,
+  // double-buffered across levels
+  constexpr int HX = 64 + 6, HY = TY + 6, AREA = HX * HY, NT = 64 * TY;
+  constexpr int NSLOT = (AREA + NT - 1) / NT;
+  constexpr int FU = 0, FV = 1, FW = 2, FT = 3, FR = 4;
+  __shared__ T s_h[2][5][AREA];
+  const int lx = static_cast<int>(threadIdx.x), ly = static_cast<int>(threadIdx.y), tid = ly * 64 + lx;
   const int nx = g.nx, ny = g.ny, nz = g.nz, rk = g.rk;
   const size_t nxy = static_cast<size_t>(nx) * ny;
+  const int i_raw = bx * 64 + lx, j_raw = by * TY + ly;
+  const bool act = i_raw < nx && j_raw < ny;  // threads outside the domain only stage the halo
+  const int i = i_raw < nx ? i_raw : nx - 1, j = j_raw < ny ? j_raw : ny - 1;
   const double *__restrict__ U = cur.u;
   const double *__restrict__ V = cur.v;
   const double *__restrict__ W = cur.w;
   const double *__restrict__ RH = cur.rho;
   const double *__restrict__ TH = cur.th;
   const double *__restrict__ PI = cur.pi;
-  uint32_t xi[7], yj[7];
+  // element offsets (within a level) of this thread's halo slots
+  uint32_t hoff[NSLOT];
 #pragma unroll
-  for (int d = 0; d < 7; ++d) {
-    xi[d] = static_cast<uint32_t>(wrapi(i + d - 3, nx));
-    yj[d] = static_cast<uint32_t>(wrapi(j + d - 3, ny)) * static_cast<uint32_t>(nx);
+  for (int q = 0; q < NSLOT; ++q) {
+    const int e = tid + q * NT < AREA ? tid + q * NT : AREA - 1;
+    const int hx = e % HX, hy = e / HX;
+    const int gi = ((bx * 64 - 3 + hx) % nx + nx) % nx, gj = ((by * TY - 3 + hy) % ny + ny) % ny;
+    hoff[q] = static_cast<uint32_t>(gj) * static_cast<uint32_t>(nx) + static_cast<uint32_t>(gi);
   }
-  const uint32_t cC = yj[3] + xi[3], cW = yj[3] + xi[2], cS = yj[2] + xi[3];
-  const uint32_t cE = yj[3] + xi[4], cN = yj[4] + xi[3];
-  const uint32_t cWN = yj[4] + xi[2], cES = yj[2] + xi[4];
+  // the halo values of the next level are loaded into registers before the
+  // current level's arithmetic and written to the other buffer after it
+  T pre[5][NSLOT];
+  auto load_level = [&](int k) {
+    const size_t b = static_cast<size_t>(k) * nxy;
+    const double *src[5] = {U, V, W, TH, RH};
+#pragma unroll
+    for (int f = 0; f < 5; ++f)
+#pragma unroll
+      for (int q = 0; q < NSLOT; ++q) pre[f][q] = T(ld32(src[f], b, hoff[q]));
+  };
+  auto store_level = [&](int buf) {
+#pragma unroll
+    for (int f = 0; f < 5; ++f)
+#pragma unroll
+      for (int q = 0; q < NSLOT; ++q)
+        if (q + 1 < NSLOT || tid + q * NT < AREA) s_h[buf][f][tid + q * NT] = pre[f][q];
+  };
+  const int lc = (ly + 3) * HX + lx + 3;  // own column in the halo tile
+  const uint32_t unx = static_cast<uint32_t>(nx);
+  const uint32_t cC = static_cast<uint32_t>(j) * unx + static_cast<uint32_t>(i);
+  const uint32_t cW = static_cast<uint32_t>(j) * unx + static_cast<uint32_t>(wrapi(i - 1, nx));
+  const uint32_t cS = static_cast<uint32_t>(wrapi(j - 1, ny)) * unx + static_cast<uint32_t>(i);
   // column geometry (compacted)
   const int gC = gcol(g, i, j), gW = gcol(g, i - 1, j), gE = gcol(g, i + 1, j), gS = gcol(g, i, j - 1),
             gN = gcol(g, i, j + 1), gWN = gcol(g, i - 1, j + 1), gES = gcol(g, i + 1, j - 1);
   const T DuW = C(g.Du)[gW], DuC = C(g.Du)[gC], DuE = C(g.Du)[gE], DuES = C(g.Du)[gES];
   const T DvS = C(g.Dv)[gS], DvC = C(g.Dv)[gC], DvN = C(g.Dv)[gN], DvWN = C(g.Dv)[gWN], DvW = C(g.Dv)[gW];
   const T rDuC = C(g.rDu)[gC], rDvC = C(g.rDv)[gC], rD = C(g.rDc)[gC], mxC = C(g.metx)[gC], myC = C(g.mety)[gC];
+  const T DuS = C(g.Du)[gS];
 
   auto ldd = [&](const double *f, int k, uint32_t c) { return T(ld32(f, static_cast<size_t>(k) * nxy, c)); };
   auto ldo = [&](int k, uint32_t c) { return ld32(om, static_cast<size_t>(k) * nxy, c); };
@@ -180,31 +214,36 @@ __device__ __forceinline__ void stage_col(const Geo<T> &g, const StageIn &cur, c
   T cu_O = T(0), cu_f = T(0), cv_O = T(0), cv_f = T(0), cw_O = T(0), cw_f = T(0);
   T cm_z = T(0), cr_z = T(0), ct_z = T(0);
   int bad = 0;
-  st32(out.w, 0, cC, static_cast<O>(ld32(s0.w, 0, cC)));  // interface 0 keeps its value (R_w = 0)
+  if (act) st32(out.w, 0, cC, static_cast<O>(ld32(s0.w, 0, cC)));  // interface 0 keeps its value (R_w = 0)
 
+  load_level(0);
+  store_level(0);
   for (int k = 0; k < nz; ++k) {
     const size_t kb = static_cast<size_t>(k) * nxy;
     const int rb = k * rk;
-    // horizontal neighbours at level k
+    const int buf = k & 1;
+    __syncthreads();  // level k staged; level k-1's reads of the other buffer are done
+    if (k + 1 < nz) load_level(k + 1);
+    // horizontal neighbours at level k, from the staged tile
+    const T *__restrict__ Hu = s_h[buf][FU] + lc;
+    const T *__restrict__ Hv = s_h[buf][FV] + lc;
+    const T *__restrict__ Hw = s_h[buf][FW] + lc;
+    const T *__restrict__ Ht = s_h[buf][FT] + lc;
+    const T *__restrict__ Hr = s_h[buf][FR] + lc;
     T ux[7], uy[7], vx[7], vy[7], tx[7], ty[7], wx[7], wy[7];
 #pragma unroll
     for (int d = 0; d < 7; ++d) {
-      if (d == 3) continue;
-      ux[d] = T(ld32(U, kb, yj[3] + xi[d]));
-      uy[d] = T(ld32(U, kb, yj[d] + xi[3]));
-      vx[d] = T(ld32(V, kb, yj[3] + xi[d]));
-      vy[d] = T(ld32(V, kb, yj[d] + xi[3]));
-      tx[d] = T(ld32(TH, kb, yj[3] + xi[d]));
-      ty[d] = T(ld32(TH, kb, yj[d] + xi[3]));
-      wx[d] = T(ld32(W, kb, yj[3] + xi[d]));
-      wy[d] = T(ld32(W, kb, yj[d] + xi[3]));
+      ux[d] = Hu[d - 3];
+      uy[d] = Hu[(d - 3) * HX];
+      vx[d] = Hv[d - 3];
+      vy[d] = Hv[(d - 3) * HX];
+      tx[d] = Ht[d - 3];
+      ty[d] = Ht[(d - 3) * HX];
+      wx[d] = Hw[d - 3];
+      wy[d] = Hw[(d - 3) * HX];
     }
-    ux[3] = uy[3] = uw[3];
-    vx[3] = vy[3] = vw[3];
-    tx[3] = ty[3] = tw[3];
-    wx[3] = wy[3] = ww[3];
-    const T vWN = T(ld32(V, kb, cWN)), uES = T(ld32(U, kb, cES));
-    const T rWst = T(ld32(RH, kb, cW)), rE = T(ld32(RH, kb, cE)), rSth = T(ld32(RH, kb, cS)), rN = T(ld32(RH, kb, cN));
+    const T vWN = Hv[HX - 1], uES = Hu[1 - HX];
+    const T rWst = Hr[-1], rE = Hr[1], rSth = Hr[-HX], rN = Hr[HX];
     const T omz = T(1) - C(g.zc)[k];
     const T rdzk = C(g.rdzeta)[k];
     // vertical pi' gradients (0 at ground and lid)
@@ -238,11 +277,11 @@ __device__ __forceinline__ void stage_col(const Geo<T> &g, const StageIn &cur, c
       const T thu = T(0.5) * (tx[2] + tw[3]);
       const T px = (pw[1] - pWw[1]) * g.rdx - (omz * mxC) * (T(0.5) * (GcW + GcC));
       const T R = adv * rDuC - T(kCp) * thu * px;
-      st32(out.u, kb, cC, static_cast<O>(ld32(s0.u, kb, cC) + h * static_cast<double>(R)));
+      if (act) st32(out.u, kb, cC, static_cast<O>(ld32(s0.u, kb, cC) + h * static_cast<double>(R)));
     }
     // ---------------- v at the south face ----------------
     {
-      const T U0 = T(0.5) * (C(g.Du)[gcol(g, i, j - 1)] * uy[2] + DuC * uw[3]);
+      const T U0 = T(0.5) * (DuS * uy[2] + DuC * uw[3]);
       const T U1 = T(0.5) * (DuES * uES + DuE * ux[4]);
       const T fx0 = up5<T>(U0, vx[0], vx[1], vx[2], vx[3], vx[4], vx[5]);
       const T fx1 = up5<T>(U1, vx[1], vx[2], vx[3], vx[4], vx[5], vx[6]);
@@ -263,7 +302,7 @@ __device__ __forceinline__ void stage_col(const Geo<T> &g, const StageIn &cur, c
       const T thv = T(0.5) * (ty[2] + tw[3]);
       const T py = (pw[1] - pSw[1]) * g.rdy - (omz * myC) * (T(0.5) * (GcS + GcC));
       const T R = adv * rDvC - T(kCp) * thv * py;
-      st32(out.v, kb, cC, static_cast<O>(ld32(s0.v, kb, cC) + h * static_cast<double>(R)));
+      if (act) st32(out.v, kb, cC, static_cast<O>(ld32(s0.v, kb, cC) + h * static_cast<double>(R)));
     }
     // ---------------- w at interface k (1..nz-1) ----------------
     // flux of w through the cell centre below / above interface k
@@ -287,7 +326,7 @@ __device__ __forceinline__ void stage_col(const Geo<T> &g, const StageIn &cur, c
       const T th0w = T(0.5) * (C(g.theta0)[rb - rk + gC] + C(g.theta0)[rb + gC]);
       const T pz = (pw[1] - pw[0]) * rz * rD;
       const T R = adv * rD - T(kCp) * thw * pz + T(kG) * thw / th0w;
-      st32(out.w, kb, cC, static_cast<O>(ld32(s0.w, kb, cC) + h * static_cast<double>(R)));
+      if (act) st32(out.w, kb, cC, static_cast<O>(ld32(s0.w, kb, cC) + h * static_cast<double>(R)));
     }
     cw_O = Oc1;
     cw_f = fzc1;
@@ -325,10 +364,12 @@ __device__ __forceinline__ void stage_col(const Geo<T> &g, const StageIn &cur, c
       const double rn = ld32(s0.rho, kb, cC) + h * static_cast<double>(rr);
       const double tn = ld32(s0.th, kb, cC) + h * static_cast<double>(rt);
       const double pn = ld32(s0.pi, kb, cC) + h * static_cast<double>(rp);
-      st32(out.rho, kb, cC, static_cast<O>(rn));
-      st32(out.th, kb, cC, tn);
-      st32(out.pi, kb, cC, static_cast<O>(pn));
-      bad |= !isfinite(tn);
+      if (act) {
+        st32(out.rho, kb, cC, static_cast<O>(rn));
+        st32(out.th, kb, cC, tn);
+        st32(out.pi, kb, cC, static_cast<O>(pn));
+        bad |= !isfinite(tn);
+      }
     }
     // ---------------- advance the vertical windows ----------------
     uE_prev = ux[4];
@@ -366,17 +407,16 @@ __device__ __forceinline__ void stage_col(const Geo<T> &g, const StageIn &cur, c
     oW[1] = ldo(ko, cW);
     oS[0] = oS[1];
     oS[1] = ldo(ko, cS);
+    if (k + 1 < nz) store_level(buf ^ 1);
   }
-  st32(out.w, static_cast<size_t>(nz) * nxy, cC, static_cast<O>(ld32(s0.w, static_cast<size_t>(nz) * nxy, cC)));
+  if (act) st32(out.w, static_cast<size_t>(nz) * nxy, cC, static_cast<O>(ld32(s0.w, static_cast<size_t>(nz) * nxy, cC)));
   if (bad) atomicOr(flag, 1);
 }
 
-template <typename T, typename O>
-__global__ void __launch_bounds__(256) k_stage_col(Geo<T> g, StageIn cur, StageIn s0, const T *__restrict__ om,
-                                                   double h, StageOut<O> out, int *__restrict__ flag) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
-  if (i >= g.nx || j >= g.ny) return;
-  stage_col<T, O>(g, cur, s0, om, h, out, flag, i, j);
+template <typename T, typename O, int TY>
+__global__ void __launch_bounds__(64 * TY) k_stage_col(Geo<T> g, StageIn cur, StageIn s0, const T *__restrict__ om,
+                                                        double h, StageOut<O> out, int *__restrict__ flag) {
+  stage_tile<T, O, TY>(g, cur, s0, om, h, out, flag, blockIdx.x, blockIdx.y);
 }
 
 // ---- HEVI factorisation table ----------------------------------------------
