@@ -4,8 +4,7 @@
 // Per RK3 step of the active members:
 //   kb_reset                   zero this step's work-queue counters
 //   for each stage st:
-//     kb_omega    rows         Omega of every active member
-//     kb_stage    64 x TY      fused slow tendencies + RK3 update
+//     kb_stage    64 x TY      fused Omega, slow tendencies and RK3 update
 //     nsub x kb_substep rows   fused HEVI acoustic substep
 //   kb_sponge   rows           Rayleigh sponge
 //   kb_advance                 rotate every active member's state buffers
@@ -41,7 +40,6 @@ struct BDom {
   Geo<T> g;
   FieldSet S[3];        // FP64 states; S[cur0] = s^n
   AcousticSet<T> A[2];  // acoustic ping-pong buffers
-  T *om;                // Omega
   HeviTab<T> tab[3];    // HEVI factorisation and coefficients of stage st
   double h[3];  // stage lengths dt/3, dt/2, dt
   T dtau[3];    // substep lengths h/nsub
@@ -112,21 +110,6 @@ __global__ void __launch_bounds__(64) kb_advance(BDom<T> *__restrict__ D, const 
   }
 }
 
-// Omega over row tiles {member, x-block, j, -}; 64 x 4 threads, 4 levels at a time.
-template <typename T>
-__global__ void __launch_bounds__(256) kb_omega(const BDom<T> *__restrict__ D, TileList L, int st,
-                                                int *__restrict__ ctr) {
-  tile_loop(ctr, L, [&](int t) {
-    const int4 tl = L.tiles[t];
-    const BDom<T> &dd = D[tl.x];
-    const Geo<T> g = dd.g;
-    const FieldSet cur = dd.S[(dd.cur0 + st) % 3];
-    const int i = tl.y * 64 + static_cast<int>(threadIdx.x);
-    if (i < g.nx)
-      for (int k = threadIdx.y; k <= g.nz; k += 4) omega_at<T, double>(g, cur.u, cur.v, cur.w, dd.om, i, tl.z, k);
-  });
-}
-
 // Fused stage kernel over tiles {member, x-block, y-block, -} of 64 x TY columns.
 template <typename T, int TY>
 __global__ void __launch_bounds__(64 * TY) kb_stage(const BDom<T> *__restrict__ D, TileList L, int st,
@@ -141,7 +124,7 @@ __global__ void __launch_bounds__(64 * TY) kb_stage(const BDom<T> *__restrict__ 
     const StageIn ci{cur.u, cur.v, cur.w, cur.rho, cur.th, cur.pi};
     const StageIn si{s0.u, s0.v, s0.w, s0.rho, s0.th, s0.pi};
     const StageOut<T> so{a.u, a.v, a.w, a.rho, a.pi, nxt.th};
-    stage_tile<T, T, TY>(g, ci, si, dd.om, dd.h[st], so, dd.flag, tl.y, tl.z);
+    stage_tile<T, T, TY>(g, ci, si, dd.h[st], so, dd.flag, tl.y, tl.z);
   });
 }
 
@@ -222,7 +205,7 @@ using namespace vibe_mi;
 
 namespace {
 
-constexpr int kMaxLaunches = 128;                 // persistent launches per step: 3 (2 + nsub) + 1
+constexpr int kMaxLaunches = 128;                 // persistent launches per step: 3 (1 + nsub) + 1
 constexpr int kMaxCounters = kMaxLaunches * kXcds;  // one counter per XCD segment and launch
 constexpr int kMaxNsub = 64;
 
@@ -373,7 +356,6 @@ int upload_desc(vibe_mi_batch *b, const double *dt, int nsub) {
       x.A[1] = d->A64[1];
     }
     for (int q = 0; q < 3; ++q) x.S[q] = d->S[q];
-    x.om = static_cast<T *>(d->om);
     const double hk[3] = {dt[m] / 3.0, dt[m] / 2.0, dt[m]};
     for (int st = 0; st < 3; ++st) {
       const double dtau = hk[st] / static_cast<double>(nsub);
@@ -478,12 +460,6 @@ int enqueue_step(vibe_mi_batch *b, const WorkList &w, int nsub) {
   const bool gt = b->group_timing;
   for (int st = 0; st < 3; ++st) {
     if (gt) VMI_HIP(hipEventRecord(b->ev[2], b->stream));
-    {
-      auto *kern = &kb_omega<T>;
-      unsigned grid = 0;
-      if (int rc = resident(b, kern, 256, 0, w.rows.ntiles, &grid)) return rc;
-      hipLaunchKernelGGL(kern, dim3(grid), dim3(64, 4), 0, b->stream, D, w.rows, st, ctr + kXcds * c++);
-    }
     if (int rc = launch_stage<T>(b, b->ty, w, st, ctr + kXcds * c++)) return rc;
     if (gt) VMI_HIP(hipEventRecord(b->ev[3], b->stream));
     for (int q = 0; q < nsub; ++q) {
@@ -571,15 +547,6 @@ int autotune(vibe_mi_batch *b, const WorkList &w) {
   }
   {
     float best = 1e30f;
-    {  // a valid Omega for the stage kernel's operands
-      hipLaunchKernelGGL(kb_reset, dim3(1), dim3(64), 0, b->stream, b->ctr, kMaxCounters);
-      auto *kern = &kb_omega<T>;
-      unsigned grid = 0;
-      if (int rc = resident(b, kern, 256, 0, w.rows.ntiles, &grid)) return rc;
-      hipLaunchKernelGGL(kern, dim3(grid), dim3(64, 4), 0, b->stream, static_cast<const BDom<T> *>(b->desc), w.rows,
-                         0, b->ctr);
-      VMI_HIP(hipGetLastError());
-    }
     for (int ty : {2, 4}) {
       float ms = 0.f;
       const int rc = time_it([&]() { return launch_stage<T>(b, ty, w, 0, b->ctr); }, &ms);
@@ -737,7 +704,7 @@ int vibe_mi_batch_member(vibe_mi_batch *b, int idx, vibe_mi_domain **out) {
 
 static int batch_step_common(vibe_mi_batch *b, const double *dt, int nsub, int nsteps, const unsigned char *active,
                              bool timed, double *ms_total) {
-  if (b == nullptr || dt == nullptr || nsteps < 0 || nsub < 1 || 3 * (nsub + 2) + 1 > kMaxLaunches ||
+  if (b == nullptr || dt == nullptr || nsteps < 0 || nsub < 1 || 3 * (nsub + 1) + 1 > kMaxLaunches ||
       nsub > kMaxNsub)
     return VIBE_C_ERR_BAD_ARG;
   if (hipSetDevice(b->device) != hipSuccess) return VIBE_MI_ERR_HIP;

@@ -47,7 +47,6 @@ struct vibe_mi_domain {
   int cur0 = 0;   // index of s^n
   AcousticSet<double> A64[2];
   AcousticSet<float> A32[2];
-  void *om = nullptr;   // Omega (T)
   void *tab = nullptr;  // HEVI tables: 4 slots of hevi_slot_elems(nz, rk) entries (T)
   double tab_dtau[4] = {-1.0, -1.0, -1.0, -1.0};
   int nxr = 1, nyr = 1, rk = 1;

@@ -172,22 +172,13 @@ inline dim3 grid_of(const vibe_mi_domain *d, int nk) {
 constexpr unsigned kBlk = 64;
 
 template <typename T>
-int run_omega(vibe_mi_domain *d, const Geo<T> &g, const FieldSet &s) {
-  hipLaunchKernelGGL((k_omega<T, double>), grid_of(d, d->nz + 1), dim3(kBlk), 0, d->stream, g, s.u, s.v, s.w,
-                     static_cast<T *>(d->om));
-  VMI_HIP(hipGetLastError());
-  return 0;
-}
-
-template <typename T>
 int run_stage(vibe_mi_domain *d, const Geo<T> &g, const FieldSet &cur, const FieldSet &s0, double h,
               AcousticSet<T> &a, FieldSet &nxt) {
-  if (int rc = run_omega<T>(d, g, cur)) return rc;
   const StageIn ci{cur.u, cur.v, cur.w, cur.rho, cur.th, cur.pi};
   const StageIn si{s0.u, s0.v, s0.w, s0.rho, s0.th, s0.pi};
   const StageOut<T> so{a.u, a.v, a.w, a.rho, a.pi, nxt.th};
   hipLaunchKernelGGL((k_stage_col<T, T, 4>), dim3(ceil_div(d->nx, 64), ceil_div(d->ny, 4)), dim3(64, 4), 0, d->stream,
-                     g, ci, si, static_cast<const T *>(d->om), h, so, d->flag);
+                     g, ci, si, h, so, d->flag);
   VMI_HIP(hipGetLastError());
   return 0;
 }
@@ -500,8 +491,6 @@ int create_impl(const vibe_mi_domain_desc *ds, vibe_mi_domain *d) {
       if (int rc = alloc_acoustic<double>(d, a)) return rc;
   }
   char *p;
-  if (int rc = dalloc(d, &p, d->nwf * tsz)) return rc;
-  d->om = p;
   if (int rc = dalloc(d, &p, 4 * hevi_slot_elems(d->nz, d->rk) * tsz)) return rc;
   d->tab = p;
   if (int rc = dalloc(d, &d->flag, 4)) return rc;

@@ -3,10 +3,11 @@
 //
 // One WS-RK3 step (src/time/runge_kutta.cpp:77-118 of the reference) is:
 //   for each stage h in {dt/3, dt/2, dt}:
-//     k_omega      contravariant vertical velocity of the current state
-//     k_stage_col  slow tendencies R (equations.cpp:60-150 role, balanced form)
+//     k_stage_col  contravariant vertical velocity Omega and the slow
+//                  tendencies R (equations.cpp:60-150 role, balanced form)
 //                  fused with the RK3 update s = s0 + h R (FP64 accumulation),
-//                  one thread per column marching up the levels
+//                  one thread per column of a 64 x TY tile marching up the
+//                  levels with the tile's horizontal neighbourhood in LDS
 //     nsub x k_substep
 //                  forward-backward acoustic substep: horizontal momentum
 //                  explicit, w / pi' / rho' vertically implicit (HEVI), the
@@ -52,68 +53,18 @@
 
 namespace vibe_mi {
 
-// ---- kinematics -------------------------------------------------------------
-
-template <typename T, typename S>
-__device__ __forceinline__ T mc_at(const Geo<T> &g, const Acc &A, const S *__restrict__ u, const S *__restrict__ v,
-                                   int i, int j, int k) {
-  return T(0.5) * (T(G(u)[A.at(i, j, k)]) * C(g.sx)[gcol(g, i, j)] + T(G(u)[A.at(i + 1, j, k)]) * C(g.sx)[gcol(g, i + 1, j)]) +
-         T(0.5) * (T(G(v)[A.at(i, j, k)]) * C(g.sy)[gcol(g, i, j)] + T(G(v)[A.at(i, j + 1, k)]) * C(g.sy)[gcol(g, i, j + 1)]);
-}
-
-// Omega at interface k of column (i, j).
-template <typename T, typename S>
-__device__ __forceinline__ void omega_at(const Geo<T> &g, const S *__restrict__ u, const S *__restrict__ v,
-                                         const S *__restrict__ w, T *__restrict__ om, int i, int j, int k) {
-  const Acc A{g.nx, g.ny, static_cast<size_t>(g.nx) * g.ny};
-  T o = T(0);
-  if (k > 0 && k < g.nz)
-    o = T(G(w)[A.at(i, j, k)]) -
-        (T(1) - C(g.zeta)[k]) * (T(0.5) * (mc_at<T, S>(g, A, u, v, i, j, k - 1) + mc_at<T, S>(g, A, u, v, i, j, k)));
-  G(om)[A.at(i, j, k)] = o;
-}
-
-template <typename T, typename S>
-__global__ void __launch_bounds__(256) k_omega(Geo<T> g, const S *__restrict__ u, const S *__restrict__ v,
-                                               const S *__restrict__ w, T *__restrict__ om) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
-  if (i >= g.nx) return;
-  omega_at<T, S>(g, u, v, w, om, i, j, k);
-}
-
-// ---- pressure-gradient operators (Gal-Chen, along constant z) --------------
-
-template <typename T, typename S>
-__device__ __forceinline__ T gz(const Geo<T> &g, const Acc &A, const S *__restrict__ pi, int i, int j, int k) {
-  if (k <= 0 || k >= g.nz) return T(0);
-  return (T(pi[A.at(i, j, k)]) - T(pi[A.at(i, j, k - 1)])) * C(g.rdzw)[k];
-}
-template <typename T, typename S>
-__device__ __forceinline__ T gzc(const Geo<T> &g, const Acc &A, const S *__restrict__ pi, int i, int j, int k) {
-  return T(0.5) * (gz<T, S>(g, A, pi, i, j, k) + gz<T, S>(g, A, pi, i, j, k + 1));
-}
-template <typename T, typename S>
-__device__ __forceinline__ T px_op(const Geo<T> &g, const Acc &A, const S *__restrict__ pi, int i, int j, int k) {
-  const T dpx = (T(pi[A.at(i, j, k)]) - T(pi[A.at(i - 1, j, k)])) * g.rdx;
-  const T met = (T(1) - C(g.zc)[k]) * C(g.metx)[gcol(g, i, j)];
-  return dpx - met * (T(0.5) * (gzc<T, S>(g, A, pi, i - 1, j, k) + gzc<T, S>(g, A, pi, i, j, k)));
-}
-template <typename T, typename S>
-__device__ __forceinline__ T py_op(const Geo<T> &g, const Acc &A, const S *__restrict__ pi, int i, int j, int k) {
-  const T dpy = (T(pi[A.at(i, j, k)]) - T(pi[A.at(i, j - 1, k)])) * g.rdy;
-  const T met = (T(1) - C(g.zc)[k]) * C(g.mety)[gcol(g, i, j)];
-  return dpy - met * (T(0.5) * (gzc<T, S>(g, A, pi, i, j - 1, k) + gzc<T, S>(g, A, pi, i, j, k)));
-}
-
 // ---- fused z-marching stage kernel -------------------------------------------
-// One thread per column (i, j), marching up the levels.  Every vertical stencil
-// (5th-order momentum, WENO5 theta', pressure gradient, Omega averages) reads
-// a register window of the thread's own column; horizontal neighbours are read
-// at the current level only, so a workgroup's 64 x 4 tile of columns shares
-// them through L1/L2 while it marches.  Writes u, v, w, rho', pi' (type O) and
-// theta' (FP64) of s = s0 + h R in one pass.  Each interface flux is evaluated
-// once: the flux through the top of level k is carried to level k+1 as its
-// bottom flux (the same expression on the same operands).
+// A workgroup owns a 64 x TY tile of columns, one thread per column, marching
+// up the levels.  Every vertical stencil (5th-order momentum, WENO5 theta',
+// pressure gradient, Omega averages) reads a register window of the thread's
+// own column; horizontal neighbours come from the level's u, v, w, theta',
+// rho' staged in LDS with a 3-cell halo.  Omega is computed in place from the
+// staged next level (the slope flux Mc of the own, west and south columns is
+// carried up), so no separate Omega pass or array exists.  Writes u, v, w,
+// rho', pi' (type O) and theta' (FP64) of s = s0 + h R in one pass.  Each
+// interface flux is evaluated once: the flux through the top of level k is
+// carried to level k+1 as its bottom flux (the same expression on the same
+// operands).
 struct StageIn {
   const double *u, *v, *w, *rho, *th, *pi;
 };
@@ -124,15 +75,15 @@ struct StageOut {
 };
 
 template <typename T, typename O, int TY>
-__device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, const StageIn &s0,
-                                           const T *__restrict__ om, double h, const StageOut<O> &out,
-                                           int *__restrict__ flag, int bx, int by) {
-  // level k of u, v, w, theta', rho' over the tile and a 3-cell halo,
-  // double-buffered across levels
+__device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, const StageIn &s0, double h,
+                                           const StageOut<O> &out, int *__restrict__ flag, int bx, int by) {
+  // levels k and k+1 of u, v, w, theta', rho' over the tile and a 3-cell
+  // halo, in a ring of three LDS buffers (level k+2 is filled meanwhile);
+  // level k+1 gives Omega at the top face of level k
   constexpr int HX = 64 + 6, HY = TY + 6, AREA = HX * HY, NT = 64 * TY;
   constexpr int NSLOT = (AREA + NT - 1) / NT;
   constexpr int FU = 0, FV = 1, FW = 2, FT = 3, FR = 4;
-  __shared__ T s_h[2][5][AREA];
+  __shared__ T s_h[3][5][AREA];
   const int lx = static_cast<int>(threadIdx.x), ly = static_cast<int>(threadIdx.y), tid = ly * 64 + lx;
   const int nx = g.nx, ny = g.ny, nz = g.nz, rk = g.rk;
   const size_t nxy = static_cast<size_t>(nx) * ny;
@@ -184,10 +135,12 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, 
   const T DvS = C(g.Dv)[gS], DvC = C(g.Dv)[gC], DvN = C(g.Dv)[gN], DvWN = C(g.Dv)[gWN], DvW = C(g.Dv)[gW];
   const T rDuC = C(g.rDu)[gC], rDvC = C(g.rDv)[gC], rD = C(g.rDc)[gC], mxC = C(g.metx)[gC], myC = C(g.mety)[gC];
   const T DuS = C(g.Du)[gS];
+  // slopes of the columns whose Omega the tile needs (own, west, south)
+  const T sxC = C(g.sx)[gC], sxE = C(g.sx)[gE], sxW = C(g.sx)[gW], sxS = C(g.sx)[gS], sxES = C(g.sx)[gES];
+  const T syC = C(g.sy)[gC], syN = C(g.sy)[gN], syW = C(g.sy)[gW], syWN = C(g.sy)[gWN], syS = C(g.sy)[gS];
 
   auto ldd = [&](const double *f, int k, uint32_t c) { return T(ld32(f, static_cast<size_t>(k) * nxy, c)); };
-  auto ldo = [&](int k, uint32_t c) { return ld32(om, static_cast<size_t>(k) * nxy, c); };
-  T uw[7], vw[7], ww[7], tw[7], rw[3], pw[3], pWw[3], pSw[3], ow[3], oW[2], oS[2];
+  T uw[7], vw[7], ww[7], tw[7], rw[3], pw[3], pWw[3], pSw[3];
 #pragma unroll
   for (int q = 0; q < 7; ++q) {
     const int kc = clampk(q - 3, nz - 1);
@@ -203,12 +156,18 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, 
     pw[q] = ldd(PI, kc, cC);
     pWw[q] = ldd(PI, kc, cW);
     pSw[q] = ldd(PI, kc, cS);
-    ow[q] = ldo(clampk(q - 1, nz), cC);
   }
-  oW[0] = ldo(0, cW);
-  oW[1] = ldo(1, cW);
-  oS[0] = ldo(0, cS);
-  oS[1] = ldo(1, cS);
+  // terrain slope flux Mc (cell centres) of the own, west and south columns
+  // at the staged level in buffer `buf` (the Omega kernel's expression)
+  T mcC = T(0), mcW = T(0), mcS = T(0);
+  auto slope_flux = [&](int buf, T &mc_c, T &mc_w, T &mc_s) {
+    const T *__restrict__ Hu = s_h[buf][FU] + lc;
+    const T *__restrict__ Hv = s_h[buf][FV] + lc;
+    mc_c = T(0.5) * (Hu[0] * sxC + Hu[1] * sxE) + T(0.5) * (Hv[0] * syC + Hv[HX] * syN);
+    mc_w = T(0.5) * (Hu[-1] * sxW + Hu[0] * sxC) + T(0.5) * (Hv[-1] * syW + Hv[HX - 1] * syWN);
+    mc_s = T(0.5) * (Hu[-HX] * sxS + Hu[1 - HX] * sxES) + T(0.5) * (Hv[-HX] * syS + Hv[0] * syC);
+  };
+  T omC0 = T(0);  // Omega at the bottom face of the current level (0 at the ground)
   T uE_prev = T(0), vN_prev = T(0);
   // z-face fluxes of the level above, reused as the next level's lower face
   T cu_O = T(0), cu_f = T(0), cv_O = T(0), cv_f = T(0), cw_O = T(0), cw_f = T(0);
@@ -218,12 +177,37 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, 
 
   load_level(0);
   store_level(0);
+  if (nz > 1) {
+    load_level(1);
+    store_level(1);
+  }
+  __syncthreads();
+  slope_flux(0, mcC, mcW, mcS);
   for (int k = 0; k < nz; ++k) {
     const size_t kb = static_cast<size_t>(k) * nxy;
     const int rb = k * rk;
-    const int buf = k & 1;
-    __syncthreads();  // level k staged; level k-1's reads of the other buffer are done
-    if (k + 1 < nz) load_level(k + 1);
+    const int buf = k % 3;
+    // levels k and k+1 staged; level k-1's reads of buffer (k+2)%3 are done
+    __syncthreads();
+    if (k + 2 < nz) load_level(k + 2);
+    // Omega at interface k+1 of the own, west and south columns (0 at the lid)
+    T ow[3], oW[2], oS[2];
+    ow[1] = omC0;
+    ow[2] = oW[1] = oS[1] = T(0);
+    if (k + 1 < nz) {
+      const int b1 = (k + 1) % 3;
+      T m1C, m1W, m1S;
+      slope_flux(b1, m1C, m1W, m1S);
+      const T zf = T(1) - C(g.zeta)[k + 1];
+      const T *__restrict__ Hw1 = s_h[b1][FW] + lc;
+      ow[2] = Hw1[0] - zf * (T(0.5) * (mcC + m1C));
+      oW[1] = Hw1[-1] - zf * (T(0.5) * (mcW + m1W));
+      oS[1] = Hw1[-HX] - zf * (T(0.5) * (mcS + m1S));
+      mcC = m1C;
+      mcW = m1W;
+      mcS = m1S;
+    }
+    omC0 = ow[2];
     // horizontal neighbours at level k, from the staged tile
     const T *__restrict__ Hu = s_h[buf][FU] + lc;
     const T *__restrict__ Hv = s_h[buf][FV] + lc;
@@ -399,24 +383,16 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, 
     pSw[0] = pSw[1];
     pSw[1] = pSw[2];
     pSw[2] = ldd(PI, k2, cS);
-    const int ko = clampk(k + 2, nz);
-    ow[0] = ow[1];
-    ow[1] = ow[2];
-    ow[2] = ldo(ko, cC);
-    oW[0] = oW[1];
-    oW[1] = ldo(ko, cW);
-    oS[0] = oS[1];
-    oS[1] = ldo(ko, cS);
-    if (k + 1 < nz) store_level(buf ^ 1);
+    if (k + 2 < nz) store_level((k + 2) % 3);
   }
   if (act) st32(out.w, static_cast<size_t>(nz) * nxy, cC, static_cast<O>(ld32(s0.w, static_cast<size_t>(nz) * nxy, cC)));
   if (bad) atomicOr(flag, 1);
 }
 
 template <typename T, typename O, int TY>
-__global__ void __launch_bounds__(64 * TY) k_stage_col(Geo<T> g, StageIn cur, StageIn s0, const T *__restrict__ om,
-                                                        double h, StageOut<O> out, int *__restrict__ flag) {
-  stage_tile<T, O, TY>(g, cur, s0, om, h, out, flag, blockIdx.x, blockIdx.y);
+__global__ void __launch_bounds__(64 * TY) k_stage_col(Geo<T> g, StageIn cur, StageIn s0, double h, StageOut<O> out,
+                                                        int *__restrict__ flag) {
+  stage_tile<T, O, TY>(g, cur, s0, h, out, flag, blockIdx.x, blockIdx.y);
 }
 
 // ---- HEVI factorisation table ----------------------------------------------
