@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(64) kb_advance(BDom<T> *__restrict__ D, const 
 }
 
 // Fused stage kernel over tiles {member, x-block, y-block, -} of 64 x TY columns.
-template <typename T, int TY>
+template <typename T, int TY, int GM>
 __global__ void __launch_bounds__(64 * TY) kb_stage(const BDom<T> *__restrict__ D, TileList L, int st,
                                                     int *__restrict__ ctr) {
   tile_loop(ctr, L, [&](int t) {
@@ -124,14 +124,14 @@ __global__ void __launch_bounds__(64 * TY) kb_stage(const BDom<T> *__restrict__ 
     const StageIn ci{cur.u, cur.v, cur.w, cur.rho, cur.th, cur.pi};
     const StageIn si{s0.u, s0.v, s0.w, s0.rho, s0.th, s0.pi};
     const StageOut<T> so{a.u, a.v, a.w, a.rho, a.pi, nxt.th};
-    stage_tile<T, T, TY>(g, ci, si, dd.h[st], so, dd.flag, tl.y, tl.z);
+    stage_tile<T, T, TY, GM>(g, ci, si, dd.h[st], so, dd.flag, tl.y, tl.z);
   });
 }
 
 // Acoustic substep q of stage st over row tiles.  The last substep of a
 // stage (last = 1, O = double) writes the stage output S[out] and checks
 // finiteness; the others write the acoustic buffer A[(q+1)&1] (O = T).
-template <typename T, typename O, int NW, int MAXCH>
+template <typename T, typename O, int NW, int MAXCH, int GM>
 __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kSubstepWavesPerEu<T, NW, MAXCH>))) kb_substep(const BDom<T> *__restrict__ D, TileList L, int st, int q,
                                                       int last, int *__restrict__ ctr) {
   tile_loop(ctr, L, [&](int t) {
@@ -175,7 +175,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kS
       sa.rho_o = o.rho;
       sa.pi_o = o.pi;
     }
-    substep_row<T, T, O, NW, MAXCH>(g, sa, tl.y, tl.z);
+    substep_row<T, T, O, NW, MAXCH, GM>(g, sa, tl.y, tl.z);
   });
 }
 
@@ -239,6 +239,7 @@ struct vibe_mi_batch {
   int nw = 8;  // waves per substep workgroup
   int ty = 4;  // rows per stage workgroup
   int cus = 256;
+  int gm = kGeoFlat;  // geometry mode of the batch: the most general of its members
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   // per-group kernel times of the last timed eager call (ms, summed)
   bool group_timing = false;
@@ -379,7 +380,7 @@ int upload_desc(vibe_mi_batch *b, const double *dt, int nsub) {
 }
 
 // dry: only set the kernel's LDS attribute (before a graph capture).
-template <typename T, typename O, int NW>
+template <typename T, typename O, int NW, int GM>
 int launch_substep_nw(vibe_mi_batch *b, const WorkList &w, int st, int q, int last, int *ctr, bool dry) {
   const auto *D = static_cast<const BDom<T> *>(b->desc);
   const int ch = (b->nzmax + NW - 1) / NW;
@@ -388,7 +389,7 @@ int launch_substep_nw(vibe_mi_batch *b, const WorkList &w, int st, int q, int la
   if (lds + lds_static > 160 * 1024) return VIBE_C_ERR_BAD_ARG;
 #define VMI_BSUB(M)                                                                                           \
   do {                                                                                                        \
-    auto *kern = &kb_substep<T, O, NW, M>;                                                                    \
+    auto *kern = &kb_substep<T, O, NW, M, GM>;                                                                    \
     static bool attr = false;                                                                                 \
     if (!attr) {                                                                                              \
       VMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),                                       \
@@ -424,19 +425,26 @@ int launch_substep_nw(vibe_mi_batch *b, const WorkList &w, int st, int q, int la
   return 0;
 }
 
+template <typename T, typename O, int GM>
+int launch_substep_gm(vibe_mi_batch *b, int nw, const WorkList &w, int st, int q, int last, int *ctr, bool dry) {
+  if (nw == 16) return launch_substep_nw<T, O, 16, GM>(b, w, st, q, last, ctr, dry);
+  if (nw == 8) return launch_substep_nw<T, O, 8, GM>(b, w, st, q, last, ctr, dry);
+  return launch_substep_nw<T, O, 4, GM>(b, w, st, q, last, ctr, dry);
+}
+
 template <typename T, typename O>
 int launch_substep(vibe_mi_batch *b, int nw, const WorkList &w, int st, int q, int last, int *ctr,
                    bool dry = false) {
-  if (nw == 16) return launch_substep_nw<T, O, 16>(b, w, st, q, last, ctr, dry);
-  if (nw == 8) return launch_substep_nw<T, O, 8>(b, w, st, q, last, ctr, dry);
-  return launch_substep_nw<T, O, 4>(b, w, st, q, last, ctr, dry);
+  if (b->gm == kGeoFlat) return launch_substep_gm<T, O, kGeoFlat>(b, nw, w, st, q, last, ctr, dry);
+  if (b->gm == kGeoX) return launch_substep_gm<T, O, kGeoX>(b, nw, w, st, q, last, ctr, dry);
+  return launch_substep_gm<T, O, kGeoAny>(b, nw, w, st, q, last, ctr, dry);
 }
 
-template <typename T, int TY>
+template <typename T, int TY, int GM>
 int launch_stage_ty(vibe_mi_batch *b, const WorkList &w, int st, int *ctr) {
   const auto *D = static_cast<const BDom<T> *>(b->desc);
   const int li = TY == 2 ? 0 : 1;
-  auto *kern = &kb_stage<T, TY>;
+  auto *kern = &kb_stage<T, TY, GM>;
   unsigned grid = 0;
   if (int rc = resident(b, kern, 64 * TY, 0, w.stage[li].ntiles, &grid)) return rc;
   hipLaunchKernelGGL(kern, dim3(grid), dim3(64, TY), 0, b->stream, D, w.stage[li], st, ctr);
@@ -444,9 +452,16 @@ int launch_stage_ty(vibe_mi_batch *b, const WorkList &w, int st, int *ctr) {
   return 0;
 }
 
+template <typename T, int GM>
+int launch_stage_gm(vibe_mi_batch *b, int ty, const WorkList &w, int st, int *ctr) {
+  return ty == 2 ? launch_stage_ty<T, 2, GM>(b, w, st, ctr) : launch_stage_ty<T, 4, GM>(b, w, st, ctr);
+}
+
 template <typename T>
 int launch_stage(vibe_mi_batch *b, int ty, const WorkList &w, int st, int *ctr) {
-  return ty == 2 ? launch_stage_ty<T, 2>(b, w, st, ctr) : launch_stage_ty<T, 4>(b, w, st, ctr);
+  if (b->gm == kGeoFlat) return launch_stage_gm<T, kGeoFlat>(b, ty, w, st, ctr);
+  if (b->gm == kGeoX) return launch_stage_gm<T, kGeoX>(b, ty, w, st, ctr);
+  return launch_stage_gm<T, kGeoAny>(b, ty, w, st, ctr);
 }
 
 // Enqueues one RK3 step of the active set on the batch stream (capturable:
@@ -668,6 +683,8 @@ int vibe_mi_batch_create(const vibe_mi_domain_desc *descs, int ndom, vibe_mi_bat
     if (int rc = domain_create(&descs[m], &d)) return fail(rc);
     b->doms.push_back(d);
     b->nzmax = std::max(b->nzmax, d->nz);
+    const int gm = d->rk == 1 ? kGeoFlat : ((d->nyr == 1 && d->nxr == d->nx) ? kGeoX : kGeoAny);
+    b->gm = std::max(b->gm, gm);
   }
   if (hipGetDevice(&b->device) != hipSuccess) return fail(VIBE_MI_ERR_HIP);
   hipDeviceProp_t prop;

@@ -110,6 +110,24 @@ template <typename T>
 __device__ __forceinline__ int gcol(const Geo<T> &g, int i, int j) {
   return wrapi(j, g.ny) * g.hj + wrapi(i, g.nx) * g.hi;
 }
+// Geometry compaction fixed at compile time (kernels specialised on it):
+//   kGeoFlat  terrain and reference state uniform in x and y (rk = 1): every
+//             slope and metric term is zero, every geometry load is uniform;
+//   kGeoX     uniform in y (hj = 0; rk = nx, or 1 for flat members): the
+//             y slope and metric terms are zero;
+//   kGeoAny   general (runtime hi, hj).
+constexpr int kGeoFlat = 0, kGeoX = 1, kGeoAny = 2;
+template <int GM, typename T>
+__device__ __forceinline__ int gcolm(const Geo<T> &g, int i, int j) {
+  if constexpr (GM == kGeoFlat) {
+    return 0;
+  } else if constexpr (GM == kGeoX) {
+    return wrapi(i, g.nx) * g.hi;
+  } else {
+    return gcol(g, i, j);
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ int gref(const Geo<T> &g, int i, int j, int k) {
   return k * g.rk + gcol(g, i, j);

@@ -74,7 +74,7 @@ struct StageOut {
   double *th;
 };
 
-template <typename T, typename O, int TY>
+template <typename T, typename O, int TY, int GM>
 __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, const StageIn &s0, double h,
                                            const StageOut<O> &out, int *__restrict__ flag, int bx, int by) {
   // levels k and k+1 of u, v, w, theta', rho' over the tile and a 3-cell
@@ -129,8 +129,10 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, 
   const uint32_t cW = static_cast<uint32_t>(j) * unx + static_cast<uint32_t>(wrapi(i - 1, nx));
   const uint32_t cS = static_cast<uint32_t>(wrapi(j - 1, ny)) * unx + static_cast<uint32_t>(i);
   // column geometry (compacted)
-  const int gC = gcol(g, i, j), gW = gcol(g, i - 1, j), gE = gcol(g, i + 1, j), gS = gcol(g, i, j - 1),
-            gN = gcol(g, i, j + 1), gWN = gcol(g, i - 1, j + 1), gES = gcol(g, i + 1, j - 1);
+  const int gC = gcolm<GM>(g, i, j), gW = gcolm<GM>(g, i - 1, j), gE = gcolm<GM>(g, i + 1, j),
+            gS = gcolm<GM>(g, i, j - 1), gN = gcolm<GM>(g, i, j + 1), gWN = gcolm<GM>(g, i - 1, j + 1),
+            gES = gcolm<GM>(g, i + 1, j - 1);
+  constexpr bool xslope = GM != kGeoFlat, yslope = GM == kGeoAny;  // nonzero slope / metric terms
   const T DuW = C(g.Du)[gW], DuC = C(g.Du)[gC], DuE = C(g.Du)[gE], DuES = C(g.Du)[gES];
   const T DvS = C(g.Dv)[gS], DvC = C(g.Dv)[gC], DvN = C(g.Dv)[gN], DvWN = C(g.Dv)[gWN], DvW = C(g.Dv)[gW];
   const T rDuC = C(g.rDu)[gC], rDvC = C(g.rDv)[gC], rD = C(g.rDc)[gC], mxC = C(g.metx)[gC], myC = C(g.mety)[gC];
@@ -163,9 +165,17 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, 
   auto slope_flux = [&](int buf, T &mc_c, T &mc_w, T &mc_s) {
     const T *__restrict__ Hu = s_h[buf][FU] + lc;
     const T *__restrict__ Hv = s_h[buf][FV] + lc;
-    mc_c = T(0.5) * (Hu[0] * sxC + Hu[1] * sxE) + T(0.5) * (Hv[0] * syC + Hv[HX] * syN);
-    mc_w = T(0.5) * (Hu[-1] * sxW + Hu[0] * sxC) + T(0.5) * (Hv[-1] * syW + Hv[HX - 1] * syWN);
-    mc_s = T(0.5) * (Hu[-HX] * sxS + Hu[1 - HX] * sxES) + T(0.5) * (Hv[-HX] * syS + Hv[0] * syC);
+    if constexpr (!xslope) {
+      mc_c = mc_w = mc_s = T(0);
+    } else if constexpr (!yslope) {
+      mc_c = T(0.5) * (Hu[0] * sxC + Hu[1] * sxE);
+      mc_w = T(0.5) * (Hu[-1] * sxW + Hu[0] * sxC);
+      mc_s = T(0.5) * (Hu[-HX] * sxS + Hu[1 - HX] * sxES);
+    } else {
+      mc_c = T(0.5) * (Hu[0] * sxC + Hu[1] * sxE) + T(0.5) * (Hv[0] * syC + Hv[HX] * syN);
+      mc_w = T(0.5) * (Hu[-1] * sxW + Hu[0] * sxC) + T(0.5) * (Hv[-1] * syW + Hv[HX - 1] * syWN);
+      mc_s = T(0.5) * (Hu[-HX] * sxS + Hu[1 - HX] * sxES) + T(0.5) * (Hv[-HX] * syS + Hv[0] * syC);
+    }
   };
   T omC0 = T(0);  // Omega at the bottom face of the current level (0 at the ground)
   T uE_prev = T(0), vN_prev = T(0);
@@ -236,7 +246,7 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, 
       const T ghi = (k + 1 <= nz - 1) ? (p[2] - p[1]) * C(g.rdzw)[k + 1] : T(0);
       return T(0.5) * (glo + ghi);
     };
-    const T GcC = gpair(pw), GcW = gpair(pWw), GcS = gpair(pSw);
+    const T GcC = xslope ? gpair(pw) : T(0), GcW = xslope ? gpair(pWw) : T(0), GcS = yslope ? gpair(pSw) : T(0);
 
     // ---------------- u at the west face ----------------
     {
@@ -259,7 +269,8 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, 
       const T adv = -(fx1 - fx0) * g.rdx - (fy1 - fy0) * g.rdy - (fz1 - fz0) * rdzk +
                     uw[3] * ((U1 - U0) * g.rdx + (V1 - V0) * g.rdy + (O1 - O0) * rdzk);
       const T thu = T(0.5) * (tx[2] + tw[3]);
-      const T px = (pw[1] - pWw[1]) * g.rdx - (omz * mxC) * (T(0.5) * (GcW + GcC));
+      T px = (pw[1] - pWw[1]) * g.rdx;
+      if constexpr (xslope) px = px - (omz * mxC) * (T(0.5) * (GcW + GcC));
       const T R = adv * rDuC - T(kCp) * thu * px;
       if (act) st32(out.u, kb, cC, static_cast<O>(ld32(s0.u, kb, cC) + h * static_cast<double>(R)));
     }
@@ -284,7 +295,8 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, 
       const T adv = -(fx1 - fx0) * g.rdx - (fy1 - fy0) * g.rdy - (fz1 - fz0) * rdzk +
                     vw[3] * ((U1 - U0) * g.rdx + (V1 - V0) * g.rdy + (O1 - O0) * rdzk);
       const T thv = T(0.5) * (ty[2] + tw[3]);
-      const T py = (pw[1] - pSw[1]) * g.rdy - (omz * myC) * (T(0.5) * (GcS + GcC));
+      T py = (pw[1] - pSw[1]) * g.rdy;
+      if constexpr (yslope) py = py - (omz * myC) * (T(0.5) * (GcS + GcC));
       const T R = adv * rDvC - T(kCp) * thv * py;
       if (act) st32(out.v, kb, cC, static_cast<O>(ld32(s0.v, kb, cC) + h * static_cast<double>(R)));
     }
@@ -392,7 +404,7 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, 
 template <typename T, typename O, int TY>
 __global__ void __launch_bounds__(64 * TY) k_stage_col(Geo<T> g, StageIn cur, StageIn s0, double h, StageOut<O> out,
                                                         int *__restrict__ flag) {
-  stage_tile<T, O, TY>(g, cur, s0, h, out, flag, blockIdx.x, blockIdx.y);
+  stage_tile<T, O, TY, kGeoAny>(g, cur, s0, h, out, flag, blockIdx.x, blockIdx.y);
 }
 
 // ---- HEVI factorisation table ----------------------------------------------
@@ -467,7 +479,7 @@ inline constexpr int kSubstepWavesPerEu = (sizeof(T) == 4 && MAXCH <= 12) ? 4 : 
 
 // Row of 64 columns starting at x-block bx, row j; all 64*NW threads of the
 // workgroup call it together (it synchronises the workgroup).
-template <typename T, typename I, typename O, int NW, int MAXCH>
+template <typename T, typename I, typename O, int NW, int MAXCH, int GM>
 __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I, O> &sa, int bx, int j) {
   const I *__restrict__ u = sa.u;
   const I *__restrict__ v = sa.v;
@@ -510,7 +522,8 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   const uint32_t cC = static_cast<uint32_t>(j) * unx + i, cW = static_cast<uint32_t>(j) * unx + iW,
                  cE = static_cast<uint32_t>(j) * unx + iE, cS = static_cast<uint32_t>(jS) * unx + i,
                  cN = static_cast<uint32_t>(jN) * unx + i;
-  const uint32_t gC = gcol(g, i, j), gE = gcol(g, iE, j), gN = gcol(g, i, jN);
+  const uint32_t gC = gcolm<GM>(g, i, j), gE = gcolm<GM>(g, iE, j), gN = gcolm<GM>(g, i, jN);
+  constexpr bool xslope = GM != kGeoFlat, yslope = GM == kGeoAny;  // nonzero slope / metric terms
   const T rD = C(g.rDc)[gC];
   const T dtD = dtau * rD;
   const T sx_w = C(g.sx)[gC], sx_e = C(g.sx)[gE], sy_s = C(g.sy)[gC], sy_n = C(g.sy)[gN];
@@ -572,15 +585,25 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         T Gc[5];
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
-          const T ghi = (k + 1 <= nz - 1) ? (pp[q] - p0[q]) * rz_hi : T(0);
-          Gc[q] = T(0.5) * (gl[q] + ghi);
-          gl[q] = ghi;
+          if ((q <= 2 && xslope) || yslope) {  // vertical gradients only where a metric term uses them
+            const T ghi = (k + 1 <= nz - 1) ? (pp[q] - p0[q]) * rz_hi : T(0);
+            Gc[q] = T(0.5) * (gl[q] + ghi);
+            gl[q] = ghi;
+          } else {
+            Gc[q] = T(0);
+          }
         }
         const T omz = T(1) - C(g.zc)[k];
-        const T px_w = (p0[0] - p0[1]) * g.rdx - (omz * mx_w) * (T(0.5) * (Gc[1] + Gc[0]));
-        const T px_e = (p0[2] - p0[0]) * g.rdx - (omz * mx_e) * (T(0.5) * (Gc[0] + Gc[2]));
-        const T py_s = (p0[0] - p0[3]) * g.rdy - (omz * my_s) * (T(0.5) * (Gc[3] + Gc[0]));
-        const T py_n = (p0[4] - p0[0]) * g.rdy - (omz * my_n) * (T(0.5) * (Gc[0] + Gc[4]));
+        T px_w = (p0[0] - p0[1]) * g.rdx, px_e = (p0[2] - p0[0]) * g.rdx;
+        T py_s = (p0[0] - p0[3]) * g.rdy, py_n = (p0[4] - p0[0]) * g.rdy;
+        if constexpr (xslope) {
+          px_w = px_w - (omz * mx_w) * (T(0.5) * (Gc[1] + Gc[0]));
+          px_e = px_e - (omz * mx_e) * (T(0.5) * (Gc[0] + Gc[2]));
+        }
+        if constexpr (yslope) {
+          py_s = py_s - (omz * my_s) * (T(0.5) * (Gc[3] + Gc[0]));
+          py_n = py_n - (omz * my_n) * (T(0.5) * (Gc[0] + Gc[4]));
+        }
         const size_t rb = static_cast<size_t>(k) * rk;
         const T un_w = xs[0] - cpdt * (ldc32(g.cthu, rb, gC) * px_w);
         const T un_e = xs[1] - cpdt * (ldc32(g.cthu, rb, gE) * px_e);
@@ -590,7 +613,11 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
           st32(uo, b, cC, static_cast<O>(un_w));
           st32(vo, b, cC, static_cast<O>(vn_s));
         }
-        const T mc = T(0.5) * (un_w * sx_w + un_e * sx_e) + T(0.5) * (vn_s * sy_s + vn_n * sy_n);
+        T mc = T(0);
+        if constexpr (yslope)
+          mc = T(0.5) * (un_w * sx_w + un_e * sx_e) + T(0.5) * (vn_s * sy_s + vn_n * sy_n);
+        else if constexpr (xslope)
+          mc = T(0.5) * (un_w * sx_w + un_e * sx_e);
         const T hd = (ldc32(g.rhu, rb, gE) * un_e - ldc32(g.rhu, rb, gC) * un_w) * g.rdx +
                      (ldc32(g.rhv, rb, gN) * vn_n - ldc32(g.rhv, rb, gC) * vn_s) * g.rdy;
         const T cq = ldc32(g.kap, rb, gC) * dtD;
@@ -778,7 +805,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
 
 template <typename T, typename I, typename O, int NW, int MAXCH>
 __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kSubstepWavesPerEu<T, NW, MAXCH>))) k_substep(Geo<T> g, SubArgs<T, I, O> sa) {
-  substep_row<T, I, O, NW, MAXCH>(g, sa, blockIdx.x, blockIdx.y);
+  substep_row<T, I, O, NW, MAXCH, kGeoAny>(g, sa, blockIdx.x, blockIdx.y);
 }
 
 // ---- sponge (damping.cpp:76-138 with perturbation targets 0) -------------
