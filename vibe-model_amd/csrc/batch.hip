@@ -28,174 +28,11 @@
 #include <string>
 #include <vector>
 
-#include "domain.hpp"
-#include "kernels.hpp"
+#include "batch_kernels.hpp"
 #include "vibe_mi/batch.h"
 
-namespace vibe_mi {
-
-// Device descriptor of one member.
-template <typename T>
-struct BDom {
-  Geo<T> g;
-  FieldSet S[3];        // FP64 states; S[cur0] = s^n
-  AcousticSet<T> A[2];  // acoustic ping-pong buffers
-  HeviTab<T> tab[3];    // HEVI factorisation and coefficients of stage st
-  double h[3];  // stage lengths dt/3, dt/2, dt
-  T dtau[3];    // substep lengths h/nsub
-  T an, ao;     // HEVI off-centring weights
-  double dt;
-  const double *sponge;
-  int kmin_sponge;
-  int cur0;  // rotation index, advanced on the device by kb_advance
-  int *flag;
-};
-
-// A tile work list split into one contiguous segment per XCD: seg[x] ..
-// seg[x+1] are the tiles workgroups on XCD x take first, so that tiles that
-// share halo rows are processed under the same L2.
-constexpr int kXcds = 8;
-struct TileList {
-  const int4 *tiles;
-  const int *seg;  // kXcds + 1 segment starts, seg[kXcds] = ntiles
-  int ntiles;
-};
-
-// Persistent tile loop.  Workgroups are dispatched to the XCDs round-robin
-// by workgroup id, so workgroup b serves segment b % 8 first, then steals
-// from the other segments in turn until every counter is past its segment.
-// One lane claims the next tile while the workgroup works on the current
-// one; the claimed index is broadcast through LDS.  Every thread of the
-// workgroup runs body(t) for the same t.  ctr: kXcds counters, zero at launch.
-template <typename F>
-__device__ __forceinline__ void tile_loop(int *__restrict__ ctr, const TileList &L, F &&body) {
-  __shared__ int s_next[2];
-  const bool leader = threadIdx.x == 0 && threadIdx.y == 0;
-  const int home = static_cast<int>(blockIdx.x % kXcds);
-  int qk = 0;  // segments exhausted so far (leader only)
-  auto claim = [&]() {
-    while (qk < kXcds) {
-      const int x = (home + qk) % kXcds;
-      const int t = L.seg[x] + atomicAdd(ctr + x, 1);
-      if (t < L.seg[x + 1]) return t;
-      ++qk;
-    }
-    return L.ntiles;
-  };
-  if (leader) s_next[0] = claim();
-  __syncthreads();
-  int par = 0;
-  for (;;) {
-    const int t = __builtin_amdgcn_readfirstlane(s_next[par]);
-    if (t >= L.ntiles) break;
-    if (leader) s_next[par ^ 1] = claim();
-    body(t);
-    __syncthreads();
-    par ^= 1;
-  }
-}
-
-__device__ __forceinline__ int stage_out_index(int cur0, int st) { return (cur0 + (st == 1 ? 2 : 1)) % 3; }
-
-__global__ void __launch_bounds__(64) kb_reset(int *__restrict__ ctr, int n) {
-  for (int q = threadIdx.x; q < n; q += 64) ctr[q] = 0;
-}
-
-// Rotates the state buffers of the members in ids[0..n): s^{n+1} = S[cur0+1].
-template <typename T>
-__global__ void __launch_bounds__(64) kb_advance(BDom<T> *__restrict__ D, const int *__restrict__ ids, int n) {
-  for (int q = threadIdx.x; q < n; q += 64) {
-    BDom<T> &dd = D[ids[q]];
-    dd.cur0 = (dd.cur0 + 1) % 3;
-  }
-}
-
-// Fused stage kernel over tiles {member, x-block, y-block, -} of 64 x TY columns.
-template <typename T, int TY, int GM>
-__global__ void __launch_bounds__(64 * TY) kb_stage(const BDom<T> *__restrict__ D, TileList L, int st,
-                                                    int *__restrict__ ctr) {
-  tile_loop(ctr, L, [&](int t) {
-    const int4 tl = L.tiles[t];
-    const BDom<T> &dd = D[tl.x];
-    const Geo<T> g = dd.g;
-    const int c0 = dd.cur0;
-    const FieldSet cur = dd.S[(c0 + st) % 3], s0 = dd.S[c0], nxt = dd.S[stage_out_index(c0, st)];
-    const AcousticSet<T> a = dd.A[0];
-    const StageIn ci{cur.u, cur.v, cur.w, cur.rho, cur.th, cur.pi};
-    const StageIn si{s0.u, s0.v, s0.w, s0.rho, s0.th, s0.pi};
-    const StageOut<T> so{a.u, a.v, a.w, a.rho, a.pi, nxt.th};
-    stage_tile<T, T, TY, GM>(g, ci, si, dd.h[st], so, dd.flag, tl.y, tl.z);
-  });
-}
-
-// Acoustic substep q of stage st over row tiles.  The last substep of a
-// stage (last = 1, O = double) writes the stage output S[out] and checks
-// finiteness; the others write the acoustic buffer A[(q+1)&1] (O = T).
-template <typename T, typename O, int NW, int MAXCH, int GM>
-__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kSubstepWavesPerEu<T, NW, MAXCH>))) kb_substep(const BDom<T> *__restrict__ D, TileList L, int st, int q,
-                                                      int last, int *__restrict__ ctr) {
-  tile_loop(ctr, L, [&](int t) {
-    const int4 tl = L.tiles[t];
-    const BDom<T> &dd = D[tl.x];
-    const Geo<T> g = dd.g;
-    const AcousticSet<T> in = dd.A[q & 1];
-    SubArgs<T, T, O> sa{};
-    sa.u = in.u;
-    sa.v = in.v;
-    sa.w = in.w;
-    sa.rho = in.rho;
-    sa.pi = in.pi;
-    sa.tab = dd.tab[st];
-    sa.dtau = dd.dtau[st];
-    sa.an = dd.an;
-    sa.ao = dd.ao;
-    sa.flag = dd.flag;
-    if constexpr (sizeof(O) == 8) {
-      if (last) {
-        const FieldSet o = dd.S[stage_out_index(dd.cur0, st)];
-        sa.uo = o.u;
-        sa.vo = o.v;
-        sa.wo = o.w;
-        sa.rho_o = o.rho;
-        sa.pi_o = o.pi;
-        sa.check = 1;
-      } else if constexpr (sizeof(T) == 8) {
-        const AcousticSet<T> o = dd.A[(q + 1) & 1];
-        sa.uo = o.u;
-        sa.vo = o.v;
-        sa.wo = o.w;
-        sa.rho_o = o.rho;
-        sa.pi_o = o.pi;
-      }
-    } else {
-      const AcousticSet<T> o = dd.A[(q + 1) & 1];
-      sa.uo = o.u;
-      sa.vo = o.v;
-      sa.wo = o.w;
-      sa.rho_o = o.rho;
-      sa.pi_o = o.pi;
-    }
-    substep_row<T, T, O, NW, MAXCH, GM>(g, sa, tl.y, tl.z);
-  });
-}
-
-// Sponge over row tiles of the members' final stage output.
-template <typename T>
-__global__ void __launch_bounds__(256) kb_sponge(const BDom<T> *__restrict__ D, TileList L,
-                                                 int *__restrict__ ctr) {
-  tile_loop(ctr, L, [&](int t) {
-    const int4 tl = L.tiles[t];
-    const BDom<T> &dd = D[tl.x];
-    const int nx = dd.g.nx, ny = dd.g.ny, nz = dd.g.nz;
-    const FieldSet fin = dd.S[(dd.cur0 + 1) % 3];
-    const int i = tl.y * 64 + static_cast<int>(threadIdx.x);
-    if (i < nx)
-      for (int k = dd.kmin_sponge + static_cast<int>(threadIdx.y); k <= nz; k += 4)
-        sponge_at(nx, ny, nz, dd.sponge, dd.dt, fin, i, tl.z, k);
-  });
-}
-
-}  // namespace vibe_mi
+// FP32 kernels live in batch_f32.hip (compiled with multiply-add contraction)
+VMI_BATCH_F32_ALL(extern)
 
 using namespace vibe_mi;
 
