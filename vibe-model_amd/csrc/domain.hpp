@@ -41,6 +41,8 @@ struct vibe_mi_domain {
   vibe_mi::Geo<float> g32{};
   double *zs = nullptr, *sponge = nullptr, *rho0 = nullptr, *theta0 = nullptr, *pi0 = nullptr;
   double *dzeta64 = nullptr, *zc64 = nullptr;  // diagnostics (FP64, per level)
+  double *ubg64 = nullptr;                     // FP32-mixed background x-wind per level
+  float *ubg32 = nullptr;
   std::vector<double> h_sponge;
   int kmin_sponge = 0;
   FieldSet S[3];  // [0] = s0 of the step; [1], [2] stage outputs

@@ -458,6 +458,12 @@ int create_impl(const vibe_mi_domain_desc *ds, vibe_mi_domain *d) {
   if (int rc = build_geo<double>(d, ds, d->g64)) return rc;
   if (d->precision == VIBE_MI_MIXED)
     if (int rc = build_geo<float>(d, ds, d->g32)) return rc;
+  if (int rc = upload(d, &d->ubg64, std::vector<double>(ds->nz, 0.0))) return rc;
+  if (int rc = upload(d, &d->ubg32, std::vector<float>(ds->nz, 0.0f))) return rc;
+  d->g64.ubg = nullptr;
+  d->g64.ubg64 = d->ubg64;
+  d->g32.ubg = d->ubg32;
+  d->g32.ubg64 = d->ubg64;
   const size_t nc = d->ncell;
   if (int rc = upload(d, &d->zs, std::vector<double>(ds->zs, ds->zs + d->nxy))) return rc;
   {
@@ -577,6 +583,20 @@ int vibe_mi_domain_set_state(vibe_mi_domain *d, const double *u, const double *v
   if (int rc = set_device(d)) return rc;
   const FieldSet &f = d->S[d->cur0];
   const size_t nc = d->ncell * 8;
+  if (d->precision == VIBE_MI_MIXED) {  // background x-wind: level means of u
+    std::vector<double> ub(d->nz);
+    std::vector<float> ub32(d->nz);
+    for (int k = 0; k < d->nz; ++k) {
+      double acc = 0.0;
+      const double *pl = u + static_cast<size_t>(k) * d->nxy;
+      for (size_t c = 0; c < d->nxy; ++c) acc += pl[c];
+      ub[k] = acc / static_cast<double>(d->nxy);
+      ub32[k] = static_cast<float>(ub[k]);
+      ub[k] = static_cast<double>(ub32[k]);  // the same value in both precisions
+    }
+    VMI_HIP(hipMemcpy(d->ubg64, ub.data(), ub.size() * sizeof(double), hipMemcpyHostToDevice));
+    VMI_HIP(hipMemcpy(d->ubg32, ub32.data(), ub32.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
   VMI_HIP(hipMemcpyAsync(f.u, u, nc, hipMemcpyHostToDevice, d->stream));
   VMI_HIP(hipMemcpyAsync(f.v, v, nc, hipMemcpyHostToDevice, d->stream));
   VMI_HIP(hipMemcpyAsync(f.w, w, d->nwf * 8, hipMemcpyHostToDevice, d->stream));

@@ -29,6 +29,11 @@ struct Geo {
   // rho0_i)/2 at u faces, cthv / rhv at v faces, r0w = (rho0_{k-1} + rho0_k)/2
   // at interfaces (index k*rk + column, 0 at k = 0 and nz)
   const T *cthu, *cthv, *rhu, *rhv, *r0w;
+  // FP32-mixed only: per-level background x-wind (the level mean of the
+  // state set on the domain); the FP32 acoustic buffers hold u - ubg, whose
+  // rounding error scales with the deviation instead of the full wind
+  const T *ubg;
+  const double *ubg64;
 };
 
 // Layout of one HEVI table slot (a stage's factorisation for one dtau):

@@ -272,7 +272,9 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, 
       T px = (pw[1] - pWw[1]) * g.rdx;
       if constexpr (xslope) px = px - (omz * mxC) * (T(0.5) * (GcW + GcC));
       const T R = adv * rDuC - T(kCp) * thu * px;
-      if (act) st32(out.u, kb, cC, static_cast<O>(ld32(s0.u, kb, cC) + h * static_cast<double>(R)));
+      double un = ld32(s0.u, kb, cC) + h * static_cast<double>(R);
+      if constexpr (sizeof(T) == 4) un = un - C(g.ubg64)[k];  // FP32 acoustic buffers hold u - ubg
+      if (act) st32(out.u, kb, cC, static_cast<O>(un));
     }
     // ---------------- v at the south face ----------------
     {
@@ -605,12 +607,22 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
           py_n = py_n - (omz * my_n) * (T(0.5) * (Gc[0] + Gc[4]));
         }
         const size_t rb = static_cast<size_t>(k) * rk;
-        const T un_w = xs[0] - cpdt * (ldc32(g.cthu, rb, gC) * px_w);
-        const T un_e = xs[1] - cpdt * (ldc32(g.cthu, rb, gE) * px_e);
+        // FP32 buffers hold u - ubg (see Geo::ubg); un_* are the full winds
+        const T du_w = xs[0] - cpdt * (ldc32(g.cthu, rb, gC) * px_w);
+        const T du_e = xs[1] - cpdt * (ldc32(g.cthu, rb, gE) * px_e);
+        T un_w = du_w, un_e = du_e;
+        if constexpr (sizeof(T) == 4) {
+          const T ubk = C(g.ubg)[k];
+          un_w = ubk + du_w;
+          un_e = ubk + du_e;
+        }
         const T vn_s = xs[2] - cpdt * (ldc32(g.cthv, rb, gC) * py_s);
         const T vn_n = xs[3] - cpdt * (ldc32(g.cthv, rb, gN) * py_n);
         if (active) {
-          st32(uo, b, cC, static_cast<O>(un_w));
+          if constexpr (sizeof(T) == 4 && sizeof(O) == 8)
+            st32(uo, b, cC, C(g.ubg64)[k] + static_cast<double>(du_w));
+          else
+            st32(uo, b, cC, static_cast<O>(du_w));
           st32(vo, b, cC, static_cast<O>(vn_s));
         }
         T mc = T(0);

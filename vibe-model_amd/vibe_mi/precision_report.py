@@ -28,26 +28,35 @@ import time
 
 import numpy as np
 
-from . import FP64, MIXED, Domain
+from . import FP64, MIXED, Batch, Domain
 from . import configs
 
 FIELDS = ("u", "v", "w", "rho", "theta", "pi")
 
 
-def _run(case, precision, nsteps, block, label):
-    dom = Domain(case, precision)
+def _run(case, precision, nsteps, block, label, batched=True):
+    """Steps one domain; batched (default) uses the persistent batched kernels
+    with hipGraph replay, the path bench.py measures."""
+    if batched:
+        b = Batch([case], precision)
+        dom = b.members[0]
+        advance = lambda n: b.step([case.dt], case.nsub, n)
+    else:
+        b = None
+        dom = Domain(case, precision)
+        advance = lambda n: dom.step(case.dt, case.nsub, n)
     e0 = dom.energy_budget()
     t0 = time.perf_counter()
     done = 0
     while done < nsteps:
         n = min(block, nsteps - done)
-        dom.step(case.dt, case.nsub, n)
+        advance(n)
         done += n
         print(f"[precision] {label}: {done}/{nsteps} steps, {time.perf_counter() - t0:.1f} s",
               file=sys.stderr, flush=True)
     e1 = dom.energy_budget()
     st = dom.get_state()
-    dom.close()
+    (b or dom).close()
     return st, e0, e1
 
 
@@ -79,17 +88,19 @@ def main(argv=None):
     ap.add_argument("--nx", type=int, default=0)
     ap.add_argument("--block", type=int, default=50)
     ap.add_argument("--out", default="")
+    ap.add_argument("--unbatched", action="store_true", help="single-domain kernels instead of the batched ones")
     args = ap.parse_args(argv)
     kw = {"nx": args.nx, "ny": args.nx} if args.nx else {}
     case = configs.CONFIGS[args.config](**kw)
     nsteps = args.steps or int(math.ceil(args.hours * 3600.0 / case.dt))
     print(f"[precision] {case.name} {case.nx}x{case.ny}x{case.nz}, dt={case.dt} s, {nsteps} steps "
           f"({nsteps * case.dt / 3600.0:.3f} h simulated)", file=sys.stderr, flush=True)
-    ref, r0, r1 = _run(case, FP64, nsteps, args.block, "fp64")
-    mix, m0, m1 = _run(case, MIXED, nsteps, args.block, "mixed")
+    ref, r0, r1 = _run(case, FP64, nsteps, args.block, "fp64", not args.unbatched)
+    mix, m0, m1 = _run(case, MIXED, nsteps, args.block, "mixed", not args.unbatched)
     err = compare(mix, ref)
     rep = {
         "workload": f"{case.name} {case.nx}x{case.ny}x{case.nz}",
+        "path": "single-domain kernels" if args.unbatched else "batched kernels (hipGraph)",
         "dt": case.dt, "substeps": case.nsub, "steps": nsteps,
         "simulated_hours": nsteps * case.dt / 3600.0,
         "error_mixed_vs_fp64": err,
