@@ -55,6 +55,9 @@ typedef struct {
 
 typedef struct vibe_mi_domain vibe_mi_domain;
 
+/* VIBE_C_ERR_BAD_ARG also when nx*ny*(nz+1)*8 bytes reach 4 GiB (the kernels
+ * address a field with 32-bit byte offsets); larger grids run as several
+ * members of a batch. */
 int vibe_mi_domain_create(const vibe_mi_domain_desc *desc, vibe_mi_domain **out);
 int vibe_mi_domain_destroy(vibe_mi_domain *d);
 /* Upload / download the FP64 state (host arrays in the layout above). */

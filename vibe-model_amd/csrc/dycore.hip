@@ -556,6 +556,11 @@ int vibe_mi_domain_create(const vibe_mi_domain_desc *ds, vibe_mi_domain **out) {
       ds->pi0 == nullptr || ds->sponge == nullptr ||
       (ds->precision != VIBE_MI_FP64 && ds->precision != VIBE_MI_MIXED))
     return VIBE_C_ERR_BAD_ARG;
+  // kernels address a field array with 32-bit byte offsets: the largest one
+  // (FP64 w, nz+1 levels) must stay below 4 GiB
+  if (static_cast<uint64_t>(ds->nx) * static_cast<uint64_t>(ds->ny) * static_cast<uint64_t>(ds->nz + 1) * 8u >=
+      (uint64_t{1} << 32))
+    return VIBE_C_ERR_BAD_ARG;
   auto *d = new vibe_mi_domain();
   const int rc = create_impl(ds, d);
   if (rc != 0) {

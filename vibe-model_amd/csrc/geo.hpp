@@ -111,6 +111,27 @@ __device__ __forceinline__ void st32(P *__restrict__ f, size_t base, uint32_t id
   *(VMI_GLOBAL P *)(b + idx * static_cast<uint32_t>(sizeof(P))) = v;
 }
 
+// Loads/stores at (array base, 32-bit byte offset).  The callers form the
+// offset as (per-lane column offset + wave-uniform level offset) in 32-bit
+// arithmetic next to the access, so the compiler emits the scalar-base +
+// 32-bit vector-offset form (one 32-bit add per distinct offset, no 64-bit
+// per-lane address).  An array addressed this way stays below 4 GiB
+// (domain creation checks it).
+template <typename P>
+__device__ __forceinline__ P gld(const P *__restrict__ f, uint32_t off) {
+  return *(const VMI_GLOBAL P *)((const VMI_GLOBAL char *)f + off);
+}
+// the same through the constant-address-space view (read-only arrays: a
+// wave-uniform offset becomes a scalar load)
+template <typename P>
+__device__ __forceinline__ P gldc(const P *__restrict__ f, uint32_t off) {
+  return *(const VMI_CONST P *)((const VMI_CONST char *)f + off);
+}
+template <typename P>
+__device__ __forceinline__ void gst(P *__restrict__ f, uint32_t off, P v) {
+  *(VMI_GLOBAL P *)((VMI_GLOBAL char *)f + off) = v;
+}
+
 template <typename T>
 __device__ __forceinline__ int gcol(const Geo<T> &g, int i, int j) {
   return wrapi(j, g.ny) * g.hj + wrapi(i, g.nx) * g.hi;

@@ -481,6 +481,19 @@ inline constexpr int kSubstepWavesPerEu = (sizeof(T) == 4 && MAXCH <= 12) ? 4 : 
 
 // Row of 64 columns starting at x-block bx, row j; all 64*NW threads of the
 // workgroup call it together (it synchronises the workgroup).
+//
+// Memory access is software-pipelined: every global operand of a level (the
+// state, pi' two levels ahead, the face and interface products, the HEVI
+// table) is loaded one level before the level's arithmetic, and a level's
+// loads are issued before the arithmetic of the level before it.  The
+// vector-memory counter retires in issue order, so the wait for level k's
+// operands never waits for level k+1's loads: the memory latency of one level
+// overlaps the arithmetic of the previous one.  The level loops are fully
+// unrolled and the operand buffers are indexed by (level mod 2) or (level
+// mod 3), which the unrolling turns into fixed registers (no copies).
+// Addresses are (array base, per-lane 32-bit column offset + uniform level
+// offset); the 32-bit sum is formed next to each access, which keeps the
+// scalar-base + vector-offset addressing form.
 template <typename T, typename I, typename O, int NW, int MAXCH, int GM>
 __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I, O> &sa, int bx, int j) {
   const I *__restrict__ u = sa.u;
@@ -488,11 +501,6 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   const I *__restrict__ w = sa.w;
   const I *__restrict__ rho = sa.rho;
   const I *__restrict__ pi = sa.pi;
-  const T *__restrict__ tA = sa.tab.A;
-  const T *__restrict__ tINV = sa.tab.INV;
-  const T *__restrict__ tCP = sa.tab.CP;
-  const T *__restrict__ tGK = sa.tab.GK;
-  const T *__restrict__ tCK = sa.tab.CK;
   const T dtau = sa.dtau, an = sa.an, ao = sa.ao;
   O *__restrict__ uo = sa.uo;
   O *__restrict__ vo = sa.vo;
@@ -508,11 +516,10 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   T *s_cp = s_be + nlds;                      // [nz-1][64] c'
   __shared__ T s_mc[NW][64], s_fh[NW][64], s_ph[NW][64];
   const int lane = threadIdx.x & 63;
-  // wave index: uniform within the wavefront, so the level bases below live in
-  // scalar registers and loads use the scalar-base + 32-bit offset form
+  // wave index: uniform within the wavefront, so level offsets are scalars
   const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
   const int nx = g.nx, ny = g.ny, nz = g.nz, rk = g.rk;
-  const size_t nxy = static_cast<size_t>(nx) * ny;
+  const uint32_t nxy = static_cast<uint32_t>(nx) * static_cast<uint32_t>(ny);
   const int i_raw = bx * 64 + lane;
   const bool active = i_raw < nx;
   const int i = active ? i_raw : nx - 1;
@@ -521,11 +528,18 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   const int nlev = max(0, min(nz, k0 + ch) - k0);
   const int iW = wrapi(i - 1, nx), iE = wrapi(i + 1, nx), jS = wrapi(j - 1, ny), jN = wrapi(j + 1, ny);
   const uint32_t unx = static_cast<uint32_t>(nx);
-  const uint32_t cC = static_cast<uint32_t>(j) * unx + i, cW = static_cast<uint32_t>(j) * unx + iW,
-                 cE = static_cast<uint32_t>(j) * unx + iE, cS = static_cast<uint32_t>(jS) * unx + i,
-                 cN = static_cast<uint32_t>(jN) * unx + i;
+  constexpr uint32_t SI = sizeof(I), SO = sizeof(O), ST = sizeof(T);
+  // per-lane byte offsets of the five state columns within a level
+  const uint32_t cC = static_cast<uint32_t>(j) * unx + i;
+  const uint32_t vC = cC * SI, vW = (static_cast<uint32_t>(j) * unx + iW) * SI,
+                 vE = (static_cast<uint32_t>(j) * unx + iE) * SI, vS = (static_cast<uint32_t>(jS) * unx + i) * SI,
+                 vN = (static_cast<uint32_t>(jN) * unx + i) * SI, oC = cC * SO;
   const uint32_t gC = gcolm<GM>(g, i, j), gE = gcolm<GM>(g, iE, j), gN = gcolm<GM>(g, i, jN);
+  const uint32_t tC = gC * ST, tE = gE * ST, tN = gN * ST;
+  // level strides in bytes (uniform)
+  const uint32_t lsI = nxy * SI, lsO = nxy * SO, lsT = static_cast<uint32_t>(rk) * ST;
   constexpr bool xslope = GM != kGeoFlat, yslope = GM == kGeoAny;  // nonzero slope / metric terms
+  constexpr bool colE = GM != kGeoFlat, colN = GM == kGeoAny;     // east / north geometry column differs
   const T rD = C(g.rDc)[gC];
   const T dtD = dtau * rD;
   const T sx_w = C(g.sx)[gC], sx_e = C(g.sx)[gE], sy_s = C(g.sy)[gC], sy_n = C(g.sy)[gN];
@@ -540,55 +554,89 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   // ---------------- phase A ----------------
   if (nlev > 0) {
     auto ld5 = [&](int k, T *p) {
-      const size_t b = static_cast<size_t>(k) * nxy;
-      p[0] = T(ld32(pi, b, cC));
-      p[1] = T(ld32(pi, b, cW));
-      p[2] = T(ld32(pi, b, cE));
-      p[3] = T(ld32(pi, b, cS));
-      p[4] = T(ld32(pi, b, cN));
+      const uint32_t s = static_cast<uint32_t>(k) * lsI;
+      p[0] = T(gld(pi, vC + s));
+      p[1] = T(gld(pi, vW + s));
+      p[2] = T(gld(pi, vE + s));
+      p[3] = T(gld(pi, vS + s));
+      p[4] = T(gld(pi, vN + s));
     };
-    // pi' of the five columns at the current and next level; the vertical
-    // gradient of the interface below is carried from the previous level.
-    T p0[5], pp[5], gl[5];
-    ld5(k0, p0);
-    T pmC = p0[0];
+    auto ld_state = [&](int k, T *x) {
+      const uint32_t s = static_cast<uint32_t>(k) * lsI;
+      x[0] = T(gld(u, vC + s));
+      x[1] = T(gld(u, vE + s));
+      x[2] = T(gld(v, vC + s));
+      x[3] = T(gld(v, vN + s));
+      x[4] = T(gld(w, vC + s));
+      x[5] = T(gld(rho, vC + s));
+    };
+    // face and interface products of a level: cp theta0 and D rho0 at the
+    // west/east u faces and south/north v faces, kappa, gk and r0w of the
+    // interface below the level
+    enum { GUW, GUE, GVS, GVN, HUW, HUE, HVS, HVN, KAP, GKB, R0B, NG };
+    auto ld_geo = [&](int k, T *q) {
+      const uint32_t s = static_cast<uint32_t>(k) * lsT;
+      q[GUW] = gldc(g.cthu, tC + s);
+      q[GUE] = colE ? gldc(g.cthu, tE + s) : q[GUW];
+      q[GVS] = gldc(g.cthv, tC + s);
+      q[GVN] = colN ? gldc(g.cthv, tN + s) : q[GVS];
+      q[HUW] = gldc(g.rhu, tC + s);
+      q[HUE] = colE ? gldc(g.rhu, tE + s) : q[HUW];
+      q[HVS] = gldc(g.rhv, tC + s);
+      q[HVN] = colN ? gldc(g.rhv, tN + s) : q[HVS];
+      q[KAP] = gldc(g.kap, tC + s);
+      // unconditional (level 0 reads entries it does not use): a load under a
+      // branch would be waited for where the branches join
+      q[GKB] = gldc(sa.tab.GK, tC + (k >= 1 ? s - lsT : 0u));
+      q[R0B] = gldc(g.r0w, tC + s);
+    };
+    // pi' of the five columns at levels k (slot c%3) and k+1 (slot (c+1)%3);
+    // the vertical gradient of the interface below k is carried in gl.
+    T P[3][5], gl[5];
+    ld5(k0, P[0]);
+    T pmC = P[0][0];
     if (k0 >= 1) {
       T pm[5];
       ld5(k0 - 1, pm);
       pmC = pm[0];
 #pragma unroll
-      for (int q = 0; q < 5; ++q) gl[q] = (p0[q] - pm[q]) * C(g.rdzw)[k0];
+      for (int q = 0; q < 5; ++q) gl[q] = (P[0][q] - pm[q]) * C(g.rdzw)[k0];
     } else {
 #pragma unroll
       for (int q = 0; q < 5; ++q) gl[q] = T(0);
     }
-    // the state operands of a level are loaded one level ahead, so their
-    // latency overlaps the previous level's arithmetic
-    auto ld_state = [&](int k, T *x) {
-      const size_t b = static_cast<size_t>(k) * nxy;
-      x[0] = T(ld32(u, b, cC));
-      x[1] = T(ld32(u, b, cE));
-      x[2] = T(ld32(v, b, cC));
-      x[3] = T(ld32(v, b, cN));
-      x[4] = T(ld32(w, b, cC));
-      x[5] = T(ld32(rho, b, cC));
-    };
-    T xs[6], xn[6];
-    ld_state(k0, xs);
+    ld5(min(k0 + 1, nz - 1), P[1]);
+    T X[2][6], Gq[2][NG];
+    ld_state(k0, X[0]);
+    ld_geo(k0, Gq[0]);
     T mc_prev = T(0);
 #pragma unroll
     for (int c = 0; c < MAXCH; ++c) {
+      const int cs = c & 1;
+      // operands of level k+1 (and pi' of k+2), issued before level k's
+      // arithmetic.  Unconditional (the level is clamped to the wave's last
+      // one): a load under the level guard would be waited for where the
+      // guard's branches join.
+      {
+        const int kn = k0 + min(c + 1, nlev - 1);
+        ld_state(kn, X[cs ^ 1]);
+        ld_geo(kn, Gq[cs ^ 1]);
+        ld5(min(kn + 1, nz - 1), P[(c + 2) % 3]);
+      }
       if (c < nlev) {
         const int k = k0 + c;
-        const size_t b = static_cast<size_t>(k) * nxy;
-        if (c + 1 < nlev) ld_state(k + 1, xn);
-        if (k + 1 < nz) ld5(k + 1, pp); else for (int q = 0; q < 5; ++q) pp[q] = p0[q];
+        const int p0 = c % 3, p1 = (c + 1) % 3;
+        const T *xs = X[cs];
+        const T *gs = Gq[cs];
+        const T *pc = P[p0];
+        const T *pn = P[p1];
+        const uint32_t so = static_cast<uint32_t>(k) * lsO;
         const T rz_hi = C(g.rdzw)[k + 1];
         T Gc[5];
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
           if ((q <= 2 && xslope) || yslope) {  // vertical gradients only where a metric term uses them
-            const T ghi = (k + 1 <= nz - 1) ? (pp[q] - p0[q]) * rz_hi : T(0);
+            const T ghi = (k + 1 <= nz - 1) ? (pn[q] - pc[q]) * rz_hi : T(0);
             Gc[q] = T(0.5) * (gl[q] + ghi);
             gl[q] = ghi;
           } else {
@@ -596,8 +644,8 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
           }
         }
         const T omz = T(1) - C(g.zc)[k];
-        T px_w = (p0[0] - p0[1]) * g.rdx, px_e = (p0[2] - p0[0]) * g.rdx;
-        T py_s = (p0[0] - p0[3]) * g.rdy, py_n = (p0[4] - p0[0]) * g.rdy;
+        T px_w = (pc[0] - pc[1]) * g.rdx, px_e = (pc[2] - pc[0]) * g.rdx;
+        T py_s = (pc[0] - pc[3]) * g.rdy, py_n = (pc[4] - pc[0]) * g.rdy;
         if constexpr (xslope) {
           px_w = px_w - (omz * mx_w) * (T(0.5) * (Gc[1] + Gc[0]));
           px_e = px_e - (omz * mx_e) * (T(0.5) * (Gc[0] + Gc[2]));
@@ -606,58 +654,49 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
           py_s = py_s - (omz * my_s) * (T(0.5) * (Gc[3] + Gc[0]));
           py_n = py_n - (omz * my_n) * (T(0.5) * (Gc[0] + Gc[4]));
         }
-        const size_t rb = static_cast<size_t>(k) * rk;
         // FP32 buffers hold u - ubg (see Geo::ubg); un_* are the full winds
-        const T du_w = xs[0] - cpdt * (ldc32(g.cthu, rb, gC) * px_w);
-        const T du_e = xs[1] - cpdt * (ldc32(g.cthu, rb, gE) * px_e);
+        const T du_w = xs[0] - cpdt * (gs[GUW] * px_w);
+        const T du_e = xs[1] - cpdt * (gs[GUE] * px_e);
         T un_w = du_w, un_e = du_e;
         if constexpr (sizeof(T) == 4) {
           const T ubk = C(g.ubg)[k];
           un_w = ubk + du_w;
           un_e = ubk + du_e;
         }
-        const T vn_s = xs[2] - cpdt * (ldc32(g.cthv, rb, gC) * py_s);
-        const T vn_n = xs[3] - cpdt * (ldc32(g.cthv, rb, gN) * py_n);
+        const T vn_s = xs[2] - cpdt * (gs[GVS] * py_s);
+        const T vn_n = xs[3] - cpdt * (gs[GVN] * py_n);
         if (active) {
           if constexpr (sizeof(T) == 4 && sizeof(O) == 8)
-            st32(uo, b, cC, C(g.ubg64)[k] + static_cast<double>(du_w));
+            gst(uo, oC + so, C(g.ubg64)[k] + static_cast<double>(du_w));
           else
-            st32(uo, b, cC, static_cast<O>(du_w));
-          st32(vo, b, cC, static_cast<O>(vn_s));
+            gst(uo, oC + so, static_cast<O>(du_w));
+          gst(vo, oC + so, static_cast<O>(vn_s));
         }
         T mc = T(0);
         if constexpr (yslope)
           mc = T(0.5) * (un_w * sx_w + un_e * sx_e) + T(0.5) * (vn_s * sy_s + vn_n * sy_n);
         else if constexpr (xslope)
           mc = T(0.5) * (un_w * sx_w + un_e * sx_e);
-        const T hd = (ldc32(g.rhu, rb, gE) * un_e - ldc32(g.rhu, rb, gC) * un_w) * g.rdx +
-                     (ldc32(g.rhv, rb, gN) * vn_n - ldc32(g.rhv, rb, gC) * vn_s) * g.rdy;
-        const T cq = ldc32(g.kap, rb, gC) * dtD;
+        const T hd = (gs[HUE] * un_e - gs[HUW] * un_w) * g.rdx + (gs[HVN] * vn_n - gs[HVS] * vn_s) * g.rdy;
+        const T cq = gs[KAP] * dtD;
         E[c] = xs[5] - dtD * hd;
-        Q[c] = p0[0] - cq * hd;
+        Q[c] = pc[0] - cq * hd;
         const T wk = xs[4];
         Fh[c] = T(0);
         if (k >= 1) {
-          const size_t mb = static_cast<size_t>(k - 1) * rk;
-          const T gk = ldc32(tGK, mb, gC);
-          s_td[(k - 1) * 64 + lane] = wk - gk * ao * (p0[0] - pmC);
-          const T r0w = ldc32(g.r0w, rb, gC);
+          s_td[(k - 1) * 64 + lane] = wk - gs[GKB] * ao * (pc[0] - pmC);
           if (c > 0) {
             const T Mw = (T(1) - C(g.zeta)[k]) * (T(0.5) * (mc_prev + mc));
-            Fh[c] = r0w * (ao * wk - Mw);
+            Fh[c] = gs[R0B] * (ao * wk - Mw);
           } else {
             aw_first = ao * wk;
-            r0w_first = r0w;
+            r0w_first = gs[R0B];
           }
         }
         if (c == 0) mc_first = mc;
         if (k == 0) mc0 = mc;
         mc_prev = mc;
-        pmC = p0[0];
-#pragma unroll
-        for (int q = 0; q < 5; ++q) p0[q] = pp[q];
-#pragma unroll
-        for (int q = 0; q < 6; ++q) xs[q] = xn[q];
+        pmC = pc[0];
       }
     }
     mc_last = mc_prev;
@@ -682,29 +721,41 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
     for (int c = 0; c < MAXCH; ++c) {
       if (c == nlev - 1) {
         const int k = k0 + c;
-        const T ck = ldc32(tCK, static_cast<size_t>(k) * rk, gC);
+        const T ck = gldc(sa.tab.CK, tC + static_cast<uint32_t>(k) * lsT);
         s_ph[wv][lane] = Q[c] - ck * (fh_next_chunk - Fh[c]);
       }
     }
   }
   __syncthreads();
   if (nlev > 0) {
+    // ck of a level and gk, 1/pivot, beta, c' of the interface below it
+    auto ld_tab = [&](int k, T *t) {
+      const uint32_t s = static_cast<uint32_t>(k) * lsT;
+      t[0] = gldc(sa.tab.CK, tC + s);
+      const uint32_t sm = tC + (k >= 1 ? s - lsT : 0u);  // level 0: read, unused
+      t[1] = gldc(sa.tab.GK, sm);
+      t[2] = gldc(sa.tab.INV, sm);
+      t[3] = gldc(sa.tab.A, sm);
+      t[4] = gldc(sa.tab.CP, sm);
+    };
     const T fh_next_chunk = (k0 + nlev < nz) ? s_fh[wv + 1][lane] : T(0);
     T php = (k0 >= 1) ? s_ph[wv - 1][lane] : T(0);
+    T TB[2][5];
+    ld_tab(k0, TB[0]);
 #pragma unroll
     for (int c = 0; c < MAXCH; ++c) {
+      ld_tab(k0 + min(c + 1, nlev - 1), TB[(c & 1) ^ 1]);  // next level, unconditional
       if (c < nlev) {
         const int k = k0 + c;
+        const T *tb = TB[c & 1];
         const T fhn = (c + 1 < nlev) ? Fh[(c + 1) < MAXCH ? c + 1 : c] : fh_next_chunk;
-        const T ph = Q[c] - ldc32(tCK, static_cast<size_t>(k) * rk, gC) * (fhn - Fh[c]);
+        const T ph = Q[c] - tb[0] * (fhn - Fh[c]);
         if (k >= 1) {
           const int m = k - 1;
-          const size_t mb = static_cast<size_t>(m) * rk;
-          const T gk = ldc32(tGK, mb, gC);
-          const T td = s_td[m * 64 + lane] - gk * an * (ph - php);
-          s_td[m * 64 + lane] = td * ldc32(tINV, mb, gC);
-          s_be[m * 64 + lane] = ldc32(tA, mb, gC);
-          s_cp[m * 64 + lane] = ldc32(tCP, mb, gC);
+          const T td = s_td[m * 64 + lane] - tb[1] * an * (ph - php);
+          s_td[m * 64 + lane] = td * tb[2];
+          s_be[m * 64 + lane] = tb[3];
+          s_cp[m * 64 + lane] = tb[4];
         }
         php = ph;
       }
@@ -780,36 +831,43 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   // ---------------- phase D: close rho', pi', write w ----------------
   if (nlev > 0) {
     int bad = 0;
-    auto fluxF = [&](int k, T fh) {
-      if (k <= 0 || k >= nz) return T(0);
-      const T r0w = ldc32(g.r0w, static_cast<size_t>(k) * rk, gC);
-      return fh + (an * r0w) * s_td[(k - 1) * 64 + lane];
+    // r0w of the interface above a level and kappa of the level
+    auto ld_d = [&](int k, T *t) {
+      const uint32_t s = static_cast<uint32_t>(k) * lsT;
+      t[0] = gldc(g.r0w, tC + (s + lsT));  // r0w has nz+1 levels
+      t[1] = gldc(g.kap, tC + s);
     };
     const T fh_next_chunk = (k0 + nlev < nz) ? s_fh[wv + 1][lane] : T(0);
-    T flo = fluxF(k0, Fh[0]);
+    T DD[2][2];
+    ld_d(k0, DD[0]);
+    // flux through the chunk's lower interface
+    T flo = T(0);
+    if (k0 >= 1) flo = Fh[0] + (an * gldc(g.r0w, tC + static_cast<uint32_t>(k0) * lsT)) * s_td[(k0 - 1) * 64 + lane];
 #pragma unroll
     for (int c = 0; c < MAXCH; ++c) {
+      ld_d(k0 + min(c + 1, nlev - 1), DD[(c & 1) ^ 1]);  // next level, unconditional
       if (c < nlev) {
         const int k = k0 + c;
-        const size_t b = static_cast<size_t>(k) * nxy;
+        const T *dd = DD[c & 1];
+        const uint32_t so = static_cast<uint32_t>(k) * lsO;
         const T fhn = (c + 1 < nlev) ? Fh[(c + 1) < MAXCH ? c + 1 : c] : fh_next_chunk;
-        const T fhi = fluxF(k + 1, fhn);
+        const T fhi = (k + 1 < nz) ? fhn + (an * dd[0]) * s_td[k * 64 + lane] : T(0);
         const T vd = -dtD * ((fhi - flo) * C(g.rdzeta)[k]);
         const T rn = E[c] + vd;
-        const T pn = Q[c] + ldc32(g.kap, static_cast<size_t>(k) * rk, gC) * vd;
+        const T pn = Q[c] + dd[1] * vd;
         const T wn = (k >= 1) ? s_td[(k - 1) * 64 + lane] : mc0;
         if (active) {
-          st32(rho_o, b, cC, static_cast<O>(rn));
-          st32(pi_o, b, cC, static_cast<O>(pn));
-          st32(wo, b, cC, static_cast<O>(wn));
+          gst(rho_o, oC + so, static_cast<O>(rn));
+          gst(pi_o, oC + so, static_cast<O>(pn));
+          gst(wo, oC + so, static_cast<O>(wn));
         }
         bad |= !(isfinite(rn) && isfinite(pn) && isfinite(wn));
         flo = fhi;
       }
     }
     if (k0 + nlev == nz && active) {
-      const size_t b = static_cast<size_t>(nz) * nxy;
-      st32(wo, b, cC, static_cast<O>(T(ld32(w, b, cC))));
+      const uint32_t s = static_cast<uint32_t>(nz);
+      gst(wo, oC + s * lsO, static_cast<O>(T(gld(w, vC + s * lsI))));
     }
     if (check && bad && active) atomicOr(flag, 1);
   }
