@@ -740,14 +740,16 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
     };
     const T fh_next_chunk = (k0 + nlev < nz) ? s_fh[wv + 1][lane] : T(0);
     T php = (k0 >= 1) ? s_ph[wv - 1][lane] : T(0);
-    T TB[2][5];
+    // operands two levels ahead (the level's own work is short), unconditional
+    T TB[3][5];
     ld_tab(k0, TB[0]);
+    ld_tab(k0 + min(1, nlev - 1), TB[1]);
 #pragma unroll
     for (int c = 0; c < MAXCH; ++c) {
-      ld_tab(k0 + min(c + 1, nlev - 1), TB[(c & 1) ^ 1]);  // next level, unconditional
+      ld_tab(k0 + min(c + 2, nlev - 1), TB[(c + 2) % 3]);
       if (c < nlev) {
         const int k = k0 + c;
-        const T *tb = TB[c & 1];
+        const T *tb = TB[c % 3];
         const T fhn = (c + 1 < nlev) ? Fh[(c + 1) < MAXCH ? c + 1 : c] : fh_next_chunk;
         const T ph = Q[c] - tb[0] * (fhn - Fh[c]);
         if (k >= 1) {
@@ -763,67 +765,84 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   }
   __syncthreads();
   // ---------------- phase C: Thomas sweeps (wave 0) ----------------
-  // RHS, beta and c' are in LDS (staged by all waves in phase B); operands
-  // are fetched one block of eight levels ahead so only the dependent
-  // multiply-subtract per level stays on the critical path.
+  // RHS, beta and c' are in LDS (staged by all waves in phase B).  Rows
+  // (interfaces 1..nz-1) after the first run as a short remainder followed by
+  // whole blocks of B rows; each block's operands are fetched while the
+  // previous block runs, into two register sets that alternate, so only the
+  // dependent multiply-subtract per row stays on the critical path and the
+  // rows of a block carry no guards.
   if (wv == 0) {
     constexpr int B = 8;
     const int n = nz - 1;
-    const int nb = (n + B - 1) / B;
-    T ba[B], ta[B], bb[B], tb2[B];
-    auto fetch_fwd = [&](int blk, T *be, T *td) {
+    const int r = (n - 1) % B, nbf = (n - 1) / B;
+    T *__restrict__ td = s_td + lane;
+    const T *__restrict__ be = s_be + lane;
+    const T *__restrict__ cpv = s_cp + lane;
+    T o0[B], v0[B], o1[B], v1[B];
+    // forward elimination: d'_0 = RHS_0, d'_m = RHS_m - beta_m d'_{m-1}
+    T dp = td[0];
+    for (int m = 1; m <= r; ++m) {
+      dp = td[m * 64] - be[m * 64] * dp;
+      td[m * 64] = dp;
+    }
+    auto fetch_f = [&](int blk, T *o, T *v) {
+      const int m0 = r + 1 + blk * B;
 #pragma unroll
       for (int q = 0; q < B; ++q) {
-        const int m = blk * B + q;
-        const int mm = (m < n ? m : n - 1) * 64 + lane;
-        be[q] = s_be[mm];
-        td[q] = s_td[mm];
+        o[q] = be[(m0 + q) * 64];
+        v[q] = td[(m0 + q) * 64];
       }
     };
-    T dp = T(0);
-    fetch_fwd(0, ba, ta);
-    for (int blk = 0; blk < nb; ++blk) {
-      if (blk + 1 < nb) fetch_fwd(blk + 1, bb, tb2);
+    auto run_f = [&](int blk, const T *o, const T *v) {
+      const int m0 = r + 1 + blk * B;
 #pragma unroll
       for (int q = 0; q < B; ++q) {
-        const int m = blk * B + q;
-        if (m < n) {
-          dp = (m == 0) ? ta[q] : ta[q] - ba[q] * dp;
-          s_td[m * 64 + lane] = dp;
-        }
+        dp = v[q] - o[q] * dp;
+        td[(m0 + q) * 64] = dp;
       }
-#pragma unroll
-      for (int q = 0; q < B; ++q) {
-        ba[q] = bb[q];
-        ta[q] = tb2[q];
+    };
+    if (nbf > 0) {
+      fetch_f(0, o0, v0);
+      for (int blk = 0;; blk += 2) {
+        fetch_f(min(blk + 1, nbf - 1), o1, v1);
+        run_f(blk, o0, v0);
+        if (blk + 1 >= nbf) break;
+        fetch_f(min(blk + 2, nbf - 1), o0, v0);
+        run_f(blk + 1, o1, v1);
+        if (blk + 2 >= nbf) break;
       }
     }
-    auto fetch_bwd = [&](int blk, T *cp, T *d) {
+    // back substitution: x_{n-1} = d'_{n-1}, x_m = d'_m - c'_m x_{m+1}
+    T x = dp;
+    for (int m = n - 2; m > n - 2 - r; --m) {
+      x = td[m * 64] - cpv[m * 64] * x;
+      td[m * 64] = x;
+    }
+    auto fetch_b = [&](int blk, T *o, T *v) {
+      const int m0 = n - 2 - r - blk * B;
 #pragma unroll
       for (int q = 0; q < B; ++q) {
-        const int m = n - 2 - (blk * B + q);
-        const int mm = (m >= 0 ? m : 0) * 64 + lane;
-        cp[q] = s_cp[mm];
-        d[q] = s_td[mm];
+        o[q] = cpv[(m0 - q) * 64];
+        v[q] = td[(m0 - q) * 64];
       }
     };
-    T x = dp;
-    const int nbb = (n - 1 + B - 1) / B;
-    if (nbb > 0) fetch_bwd(0, ba, ta);
-    for (int blk = 0; blk < nbb; ++blk) {
-      if (blk + 1 < nbb) fetch_bwd(blk + 1, bb, tb2);
+    auto run_b = [&](int blk, const T *o, const T *v) {
+      const int m0 = n - 2 - r - blk * B;
 #pragma unroll
       for (int q = 0; q < B; ++q) {
-        const int m = n - 2 - (blk * B + q);
-        if (m >= 0) {
-          x = ta[q] - ba[q] * x;
-          s_td[m * 64 + lane] = x;
-        }
+        x = v[q] - o[q] * x;
+        td[(m0 - q) * 64] = x;
       }
-#pragma unroll
-      for (int q = 0; q < B; ++q) {
-        ba[q] = bb[q];
-        ta[q] = tb2[q];
+    };
+    if (nbf > 0) {
+      fetch_b(0, o0, v0);
+      for (int blk = 0;; blk += 2) {
+        fetch_b(min(blk + 1, nbf - 1), o1, v1);
+        run_b(blk, o0, v0);
+        if (blk + 1 >= nbf) break;
+        fetch_b(min(blk + 2, nbf - 1), o0, v0);
+        run_b(blk + 1, o1, v1);
+        if (blk + 2 >= nbf) break;
       }
     }
   }
@@ -838,17 +857,18 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
       t[1] = gldc(g.kap, tC + s);
     };
     const T fh_next_chunk = (k0 + nlev < nz) ? s_fh[wv + 1][lane] : T(0);
-    T DD[2][2];
+    T DD[3][2];
     ld_d(k0, DD[0]);
+    ld_d(k0 + min(1, nlev - 1), DD[1]);
     // flux through the chunk's lower interface
     T flo = T(0);
     if (k0 >= 1) flo = Fh[0] + (an * gldc(g.r0w, tC + static_cast<uint32_t>(k0) * lsT)) * s_td[(k0 - 1) * 64 + lane];
 #pragma unroll
     for (int c = 0; c < MAXCH; ++c) {
-      ld_d(k0 + min(c + 1, nlev - 1), DD[(c & 1) ^ 1]);  // next level, unconditional
+      ld_d(k0 + min(c + 2, nlev - 1), DD[(c + 2) % 3]);  // two levels ahead, unconditional
       if (c < nlev) {
         const int k = k0 + c;
-        const T *dd = DD[c & 1];
+        const T *dd = DD[c % 3];
         const uint32_t so = static_cast<uint32_t>(k) * lsO;
         const T fhn = (c + 1 < nlev) ? Fh[(c + 1) < MAXCH ? c + 1 : c] : fh_next_chunk;
         const T fhi = (k + 1 < nz) ? fhn + (an * dd[0]) * s_td[k * 64 + lane] : T(0);
