@@ -96,32 +96,36 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, 
   const double *__restrict__ RH = cur.rho;
   const double *__restrict__ TH = cur.th;
   const double *__restrict__ PI = cur.pi;
-  // element offsets (within a level) of this thread's halo slots
+  // byte offsets (within a level) of this thread's halo slots
   uint32_t hoff[NSLOT];
 #pragma unroll
   for (int q = 0; q < NSLOT; ++q) {
     const int e = tid + q * NT < AREA ? tid + q * NT : AREA - 1;
     const int hx = e % HX, hy = e / HX;
     const int gi = ((bx * 64 - 3 + hx) % nx + nx) % nx, gj = ((by * TY - 3 + hy) % ny + ny) % ny;
-    hoff[q] = static_cast<uint32_t>(gj) * static_cast<uint32_t>(nx) + static_cast<uint32_t>(gi);
+    hoff[q] = (static_cast<uint32_t>(gj) * static_cast<uint32_t>(nx) + static_cast<uint32_t>(gi)) * 8u;
   }
+  // level stride of the FP64 fields in bytes (fields stay below 4 GiB)
+  const uint32_t lsD = static_cast<uint32_t>(nxy) * 8u;
   // the halo values of the next level are loaded into registers before the
-  // current level's arithmetic and written to the other buffer after it
-  T pre[5][NSLOT];
+  // current level's arithmetic and written to the other buffer after it;
+  // they stay FP64 until the write (a conversion right after the load would
+  // wait for the load at once)
+  double pre[5][NSLOT];
   auto load_level = [&](int k) {
-    const size_t b = static_cast<size_t>(k) * nxy;
+    const uint32_t b = static_cast<uint32_t>(k) * lsD;
     const double *src[5] = {U, V, W, TH, RH};
 #pragma unroll
     for (int f = 0; f < 5; ++f)
 #pragma unroll
-      for (int q = 0; q < NSLOT; ++q) pre[f][q] = T(ld32(src[f], b, hoff[q]));
+      for (int q = 0; q < NSLOT; ++q) pre[f][q] = gld(src[f], hoff[q] + b);
   };
   auto store_level = [&](int buf) {
 #pragma unroll
     for (int f = 0; f < 5; ++f)
 #pragma unroll
       for (int q = 0; q < NSLOT; ++q)
-        if (q + 1 < NSLOT || tid + q * NT < AREA) s_h[buf][f][tid + q * NT] = pre[f][q];
+        if (q + 1 < NSLOT || tid + q * NT < AREA) s_h[buf][f][tid + q * NT] = T(pre[f][q]);
   };
   const int lc = (ly + 3) * HX + lx + 3;  // own column in the halo tile
   const uint32_t unx = static_cast<uint32_t>(nx);
@@ -141,23 +145,45 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, 
   const T sxC = C(g.sx)[gC], sxE = C(g.sx)[gE], sxW = C(g.sx)[gW], sxS = C(g.sx)[gS], sxES = C(g.sx)[gES];
   const T syC = C(g.sy)[gC], syN = C(g.sy)[gN], syW = C(g.sy)[gW], syWN = C(g.sy)[gWN], syS = C(g.sy)[gS];
 
-  auto ldd = [&](const double *f, int k, uint32_t c) { return T(ld32(f, static_cast<size_t>(k) * nxy, c)); };
+  const uint32_t cCb = cC * 8u, cWb = cW * 8u, cSb = cS * 8u, oCb = cC * static_cast<uint32_t>(sizeof(O));
+  const uint32_t lsO = static_cast<uint32_t>(nxy) * static_cast<uint32_t>(sizeof(O));
+  const uint32_t tCb = static_cast<uint32_t>(gC) * static_cast<uint32_t>(sizeof(T)),
+                 lsT = static_cast<uint32_t>(rk) * static_cast<uint32_t>(sizeof(T));
+  auto ldd = [&](const double *f, int k, uint32_t cb) { return T(gld(f, cb + static_cast<uint32_t>(k) * lsD)); };
+  // s0 of the level being updated, loaded one level ahead
+  double S0[6];
+  auto ld_s0 = [&](int k) {
+    const uint32_t b = cCb + static_cast<uint32_t>(k) * lsD;
+    S0[0] = gld(s0.u, b);
+    S0[1] = gld(s0.v, b);
+    S0[2] = gld(s0.w, b);
+    S0[3] = gld(s0.rho, b);
+    S0[4] = gld(s0.th, b);
+    S0[5] = gld(s0.pi, b);
+  };
+  // reference theta0 of the own column at levels k-1, k, k+1 (clamped), the
+  // static-stability factor gamma of interface k, kappa and lambda of level
+  // k; the next level's values are loaded one level ahead
+  auto th0 = [&](int kq) { return gldc(g.theta0, tCb + static_cast<uint32_t>(clampk(kq, nz - 1)) * lsT); };
+  T thm = T(0), th00 = th0(0), thp = th0(1);
+  T gamk = (th0(1) - th0(0)) * rD * C(g.rdzw)[1];  // gamma(0) = gamma(1): interface index clamped to 1..nz-1
+  T kapk = gldc(g.kap, tCb), lamk = gldc(g.lam, tCb);
   T uw[7], vw[7], ww[7], tw[7], rw[3], pw[3], pWw[3], pSw[3];
 #pragma unroll
   for (int q = 0; q < 7; ++q) {
     const int kc = clampk(q - 3, nz - 1);
-    uw[q] = ldd(U, kc, cC);
-    vw[q] = ldd(V, kc, cC);
-    tw[q] = ldd(TH, kc, cC);
-    ww[q] = ldd(W, clampk(q - 3, nz), cC);
+    uw[q] = ldd(U, kc, cCb);
+    vw[q] = ldd(V, kc, cCb);
+    tw[q] = ldd(TH, kc, cCb);
+    ww[q] = ldd(W, clampk(q - 3, nz), cCb);
   }
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     const int kc = clampk(q - 1, nz - 1);
-    rw[q] = ldd(RH, kc, cC);
-    pw[q] = ldd(PI, kc, cC);
-    pWw[q] = ldd(PI, kc, cW);
-    pSw[q] = ldd(PI, kc, cS);
+    rw[q] = ldd(RH, kc, cCb);
+    pw[q] = ldd(PI, kc, cCb);
+    pWw[q] = ldd(PI, kc, cWb);
+    pSw[q] = ldd(PI, kc, cSb);
   }
   // terrain slope flux Mc (cell centres) of the own, west and south columns
   // at the staged level in buffer `buf` (the Omega kernel's expression)
@@ -183,7 +209,8 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, 
   T cu_O = T(0), cu_f = T(0), cv_O = T(0), cv_f = T(0), cw_O = T(0), cw_f = T(0);
   T cm_z = T(0), cr_z = T(0), ct_z = T(0);
   int bad = 0;
-  if (act) st32(out.w, 0, cC, static_cast<O>(ld32(s0.w, 0, cC)));  // interface 0 keeps its value (R_w = 0)
+  if (act) gst(out.w, oCb, static_cast<O>(gld(s0.w, cCb)));  // interface 0 keeps its value (R_w = 0)
+  ld_s0(0);
 
   load_level(0);
   store_level(0);
@@ -194,8 +221,7 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, 
   __syncthreads();
   slope_flux(0, mcC, mcW, mcS);
   for (int k = 0; k < nz; ++k) {
-    const size_t kb = static_cast<size_t>(k) * nxy;
-    const int rb = k * rk;
+    const uint32_t ko = oCb + static_cast<uint32_t>(k) * lsO;
     const int buf = k % 3;
     // levels k and k+1 staged; level k-1's reads of buffer (k+2)%3 are done
     __syncthreads();
@@ -272,9 +298,9 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, 
       T px = (pw[1] - pWw[1]) * g.rdx;
       if constexpr (xslope) px = px - (omz * mxC) * (T(0.5) * (GcW + GcC));
       const T R = adv * rDuC - T(kCp) * thu * px;
-      double un = ld32(s0.u, kb, cC) + h * static_cast<double>(R);
+      double un = S0[0] + h * static_cast<double>(R);
       if constexpr (sizeof(T) == 4) un = un - C(g.ubg64)[k];  // FP32 acoustic buffers hold u - ubg
-      if (act) st32(out.u, kb, cC, static_cast<O>(un));
+      if (act) gst(out.u, ko, static_cast<O>(un));
     }
     // ---------------- v at the south face ----------------
     {
@@ -300,7 +326,7 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, 
       T py = (pw[1] - pSw[1]) * g.rdy;
       if constexpr (yslope) py = py - (omz * myC) * (T(0.5) * (GcS + GcC));
       const T R = adv * rDvC - T(kCp) * thv * py;
-      if (act) st32(out.v, kb, cC, static_cast<O>(ld32(s0.v, kb, cC) + h * static_cast<double>(R)));
+      if (act) gst(out.v, ko, static_cast<O>(S0[1] + h * static_cast<double>(R)));
     }
     // ---------------- w at interface k (1..nz-1) ----------------
     // flux of w through the cell centre below / above interface k
@@ -321,10 +347,10 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, 
       const T adv = -(fx1 - fx0) * g.rdx - (fy1 - fy0) * g.rdy - (fz1 - fz0) * rz +
                     ww[3] * ((U1 - U0) * g.rdx + (V1 - V0) * g.rdy + (Oc1 - Oc0) * rz);
       const T thw = T(0.5) * (tw[2] + tw[3]);
-      const T th0w = T(0.5) * (C(g.theta0)[rb - rk + gC] + C(g.theta0)[rb + gC]);
+      const T th0w = T(0.5) * (thm + th00);
       const T pz = (pw[1] - pw[0]) * rz * rD;
       const T R = adv * rD - T(kCp) * thw * pz + T(kG) * thw / th0w;
-      if (act) st32(out.w, kb, cC, static_cast<O>(ld32(s0.w, kb, cC) + h * static_cast<double>(R)));
+      if (act) gst(out.w, ko, static_cast<O>(S0[2] + h * static_cast<double>(R)));
     }
     cw_O = Oc1;
     cw_f = fzc1;
@@ -353,21 +379,20 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, 
       const T rr = -((frx1 - frx0) * g.rdx + (fry1 - fry0) * g.rdy + (frz1 - frz0) * rdzk) * rD;
       const T div = (mfx1 - mfx0) * g.rdx + (mfy1 - mfy0) * g.rdy + (mfz1 - mfz0) * rdzk;
       const T fd = (ftx1 - ftx0) * g.rdx + (fty1 - fty0) * g.rdy + (ftz1 - ftz0) * rdzk;
-      auto gam = [&](int kq) {
-        const int kk = kq < 1 ? 1 : (kq > nz - 1 ? nz - 1 : kq);
-        return (C(g.theta0)[kk * rk + gC] - C(g.theta0)[(kk - 1) * rk + gC]) * rD * C(g.rdzw)[kk];
-      };
-      const T rt = -(fd - tw[3] * div) * rD - T(0.5) * (ww[3] * gam(k) + ww[4] * gam(k + 1));
-      const T rp = C(g.kap)[rb + gC] * rr + C(g.lam)[rb + gC] * rt;
-      const double rn = ld32(s0.rho, kb, cC) + h * static_cast<double>(rr);
-      const double tn = ld32(s0.th, kb, cC) + h * static_cast<double>(rt);
-      const double pn = ld32(s0.pi, kb, cC) + h * static_cast<double>(rp);
+      // gamma(k+1) with the interface index clamped to 1..nz-1
+      const T gam1 = (k + 1 <= nz - 1) ? (thp - th00) * rD * C(g.rdzw)[k + 1] : gamk;
+      const T rt = -(fd - tw[3] * div) * rD - T(0.5) * (ww[3] * gamk + ww[4] * gam1);
+      const T rp = kapk * rr + lamk * rt;
+      const double rn = S0[3] + h * static_cast<double>(rr);
+      const double tn = S0[4] + h * static_cast<double>(rt);
+      const double pn = S0[5] + h * static_cast<double>(rp);
       if (act) {
-        st32(out.rho, kb, cC, static_cast<O>(rn));
-        st32(out.th, kb, cC, tn);
-        st32(out.pi, kb, cC, static_cast<O>(pn));
+        gst(out.rho, ko, static_cast<O>(rn));
+        gst(out.th, cCb + static_cast<uint32_t>(k) * lsD, tn);
+        gst(out.pi, ko, static_cast<O>(pn));
         bad |= !isfinite(tn);
       }
+      gamk = gam1;
     }
     // ---------------- advance the vertical windows ----------------
     uE_prev = ux[4];
@@ -380,26 +405,37 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, 
       tw[q] = tw[q + 1];
       ww[q] = ww[q + 1];
     }
-    uw[6] = ldd(U, kn, cC);
-    vw[6] = ldd(V, kn, cC);
-    tw[6] = ldd(TH, kn, cC);
-    ww[6] = ldd(W, kw, cC);
+    uw[6] = ldd(U, kn, cCb);
+    vw[6] = ldd(V, kn, cCb);
+    tw[6] = ldd(TH, kn, cCb);
+    ww[6] = ldd(W, kw, cCb);
     const int k2 = clampk(k + 2, nz - 1);
     rw[0] = rw[1];
     rw[1] = rw[2];
-    rw[2] = ldd(RH, k2, cC);
+    rw[2] = ldd(RH, k2, cCb);
     pw[0] = pw[1];
     pw[1] = pw[2];
-    pw[2] = ldd(PI, k2, cC);
+    pw[2] = ldd(PI, k2, cCb);
     pWw[0] = pWw[1];
     pWw[1] = pWw[2];
-    pWw[2] = ldd(PI, k2, cW);
+    pWw[2] = ldd(PI, k2, cWb);
     pSw[0] = pSw[1];
     pSw[1] = pSw[2];
-    pSw[2] = ldd(PI, k2, cS);
+    pSw[2] = ldd(PI, k2, cSb);
+    // operands of level k+1 from s0 and the reference state (unconditional,
+    // clamped: a load under a branch would be waited for where it joins)
+    {
+      const int k1 = min(k + 1, nz - 1);
+      ld_s0(k1);
+      thm = th00;
+      th00 = thp;
+      thp = th0(k + 2);
+      kapk = gldc(g.kap, tCb + static_cast<uint32_t>(k1) * lsT);
+      lamk = gldc(g.lam, tCb + static_cast<uint32_t>(k1) * lsT);
+    }
     if (k + 2 < nz) store_level((k + 2) % 3);
   }
-  if (act) st32(out.w, static_cast<size_t>(nz) * nxy, cC, static_cast<O>(ld32(s0.w, static_cast<size_t>(nz) * nxy, cC)));
+  if (act) gst(out.w, oCb + static_cast<uint32_t>(nz) * lsO, static_cast<O>(gld(s0.w, cCb + static_cast<uint32_t>(nz) * lsD)));
   if (bad) atomicOr(flag, 1);
 }
 
