@@ -15,8 +15,8 @@ One "step" = one WS-RK3 step of every member (3 stages, 3 x 6 substeps,
 sponge).  Inputs are resident in HBM before timing starts.  Timing: W untimed
 warm-up steps, then K steps bracketed by device synchronisation, measured
 with HIP events on the batch stream.  Per-group kernel times come from a
-second, eager (no graph) run with HIP events around each stage's Omega +
-stage kernel and around its substep launches.
+second, eager (no graph) run with HIP events around each stage kernel and
+around each stage's substep launches.
 
 Roofline: algorithmic bytes per cell-update (compulsory prognostic traffic)
   FP64   8 * (48 + 30 * n_s) = 1824 B     (n_s = 6)
@@ -31,7 +31,8 @@ launch duration is measured with HIP events on the batch stream around each
 stage's substep group (only kb_substep launches are in the group).
 "traffic" (HBM bytes per launch) comes from the rocprofv3 FETCH_SIZE /
 WRITE_SIZE per-dispatch summary committed for the same workload, each
-scaled by what the counter reads for a plain 1 GiB copy (--copy-calibration).
+scaled by what the counter reads for a plain 1 GiB elementwise copy
+(--copy-calibration).
 
 cpu_baseline: the FP64 CPU oracle (a restatement of the reference's step)
 timed on this host on a bounded sample of the same workload.
@@ -88,8 +89,11 @@ def measured_traffic(cases):
 
 
 def copy_calibration(nbytes: int = 1 << 30, reps: int = 3):
-    """A plain device copy of nbytes (torch's copy kernel), for scaling the
-    FETCH_SIZE / WRITE_SIZE counters: run under rocprofv3 --pmc."""
+    """Plain copies of nbytes for scaling the FETCH_SIZE / WRITE_SIZE
+    counters (run under rocprofv3 --pmc): the runtime's device-to-device copy
+    (dispatched as a blit kernel) and torch's elementwise kernel (b = a * 1,
+    ordinary vector loads and stores like the dycore kernels); the
+    elementwise kernel sets the scale factors."""
     import torch
 
     a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").uniform_()
@@ -97,7 +101,12 @@ def copy_calibration(nbytes: int = 1 << 30, reps: int = 3):
     for _ in range(reps):
         b.copy_(a)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes": nbytes, "reps": reps, "ok": bool(torch.equal(a, b))}))
+    ok = bool(torch.equal(a, b))
+    for _ in range(reps):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    ok = ok and bool(torch.equal(a, b))
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps, "ok": ok}))
 
 
 def run_batch(cases, precision, steps, warmup, graph=True):
