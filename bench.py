@@ -151,19 +151,34 @@ def cpu_baseline(name, nsub, threads):
     else:
         case = configs.CONFIGS[name](nx=nxs, ny=nxs)
         what = f"{nxs}x{nxs}x{case.nz} columns of the same configuration"
-    orc = OracleDycore(case, threads=threads)
-    orc.step(case.dt, nsub, 1)  # warm
-    t0 = time.perf_counter()
-    n = 0
-    while True:
-        orc.step(case.dt, nsub, 1)
-        n += 1
-        el = time.perf_counter() - t0
-        if el > 8.0 or n >= 20:
-            break
-    cups = case.ncell * n / el
-    orc.close()
-    return cups, f"{what}, {n} RK3 steps, FP64 oracle"
+    def timed(nthreads, budget_s, max_steps):
+        orc = OracleDycore(case, threads=nthreads)
+        orc.step(case.dt, nsub, 1)  # warm
+        t0 = time.perf_counter()
+        n = 0
+        while True:
+            orc.step(case.dt, nsub, 1)
+            n += 1
+            el = time.perf_counter() - t0
+            if el > budget_s or n >= max_steps:
+                break
+        orc.close()
+        return case.ncell * n / el, n
+
+    cups, n = timed(threads, 8.0, 20)
+    cups1, n1 = timed(1, 8.0, 5)
+    return cups, f"{what}, {n} RK3 steps, FP64 oracle ({n1} steps on 1 thread)", cups1
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def bench_nests(args, t0):
@@ -294,8 +309,9 @@ def main():
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log(f"[bench] CPU oracle baseline on {threads} threads")
-        cpu_cups, sample = cpu_baseline(args.config, nsub, threads)
-        cpu = {"value": cpu_cups, "unit": "cell-updates/s", "cores": threads, "kind": "port", "sample": sample}
+        cpu_cups, sample, cpu_cups1 = cpu_baseline(args.config, nsub, threads)
+        cpu = {"value": cpu_cups, "unit": "cell-updates/s", "cores": threads, "kind": "port", "sample": sample,
+               "value_1_thread": cpu_cups1, "cpu_model": cpu_model()}
 
     out = {
         "metric": "cell-updates/s per RK3 step + % HBM roofline, FP32-mixed/batched vs FP64",
