@@ -42,3 +42,22 @@ def test_library_is_gfx950_code_object():
     import vibe_mi
     blob = open(vibe_mi.LIB_PATH, "rb").read()
     assert b"gfx950" in blob
+
+
+def test_domain_create_rejects_fields_of_4_gib():
+    """Kernels address a field with 32-bit byte offsets: a grid whose FP64 w
+    array (nx*ny*(nz+1)*8 B) reaches 4 GiB is refused with VIBE_C_ERR_BAD_ARG
+    before anything is allocated (no GPU is touched)."""
+    import ctypes
+    import numpy as np
+    import vibe_mi
+    L = vibe_mi.lib()
+    nz = 96
+    one = np.zeros(nz + 1)
+    nx = ny = 2400  # 2400*2400*97*8 B = 4.47e9 B
+    desc = vibe_mi.DomainDesc(nx, ny, nz, 1000.0, 1000.0, 20000.0, vibe_mi._pd(one), vibe_mi._pd(one),
+                              vibe_mi._pd(one), vibe_mi._pd(one), vibe_mi._pd(one), vibe_mi._pd(one), 0.1,
+                              vibe_mi.FP64)
+    h = ctypes.c_void_p()
+    assert L.vibe_mi_domain_create(ctypes.byref(desc), ctypes.byref(h)) == 2
+    assert not h.value
