@@ -349,7 +349,7 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, 
       const T thw = T(0.5) * (tw[2] + tw[3]);
       const T th0w = T(0.5) * (thm + th00);
       const T pz = (pw[1] - pw[0]) * rz * rD;
-      const T R = adv * rD - T(kCp) * thw * pz + T(kG) * thw / th0w;
+      const T R = adv * rD - T(kCp) * thw * pz + quot<T>(T(kG) * thw, th0w);
       if (act) gst(out.w, ko, static_cast<O>(S0[2] + h * static_cast<double>(R)));
     }
     cw_O = Oc1;

@@ -58,11 +58,22 @@ __host__ __device__ inline T up5(T U, T qm2, T qm1, T q0, T q1, T q2, T q3) {
   return U * c6<T>(qm2, qm1, q0, q1, q2, q3) - fabs(U) * d5<T>(qm2, qm1, q0, q1, q2, q3);
 }
 
+// Quotient num / den: correctly rounded in FP64 (the oracle's arithmetic);
+// in FP32 the hardware reciprocal (1 ulp) times num, which replaces the
+// ten-instruction correctly rounded division sequence.
+template <typename T>
+__device__ __forceinline__ T quot(T num, T den) {
+  if constexpr (sizeof(T) == 4)
+    return num * __builtin_amdgcn_rcpf(den);
+  else
+    return num / den;
+}
+
 // WENO5 value at the face right of c from the stencil (a, b, c, d, e), same
 // weights as weno5() (d = 0.1, 0.6, 0.3; eps = 1e-6) but with one division:
 // alpha_k = d_k/(eps+beta_k)^2 scaled by prod_j (eps+beta_j)^2.
 template <typename T>
-__host__ __device__ inline T weno_face(T a, T b, T c, T d, T e) {
+__device__ inline T weno_face(T a, T b, T c, T d, T e) {
   const T eps = T(1e-6);
   const T p0 = T(2) * a - T(7) * b + T(11) * c;
   const T p1 = -b + T(5) * c + T(2) * d;
@@ -75,7 +86,7 @@ __host__ __device__ inline T weno_face(T a, T b, T c, T d, T e) {
   const T e2 = eps + ((T(13.0 / 12.0)) * s2a * s2a + T(0.25) * s2b * s2b);
   const T q0 = e0 * e0, q1 = e1 * e1, q2 = e2 * e2;
   const T w0 = T(0.1) * (q1 * q2), w1 = T(0.6) * (q0 * q2), w2 = T(0.3) * (q0 * q1);
-  return (w0 * p0 + w1 * p1 + w2 * p2) / (T(6) * (w0 + w1 + w2));
+  return quot<T>(w0 * p0 + w1 * p1 + w2 * p2, T(6) * (w0 + w1 + w2));
 }
 
 // Upwind WENO5 flux U * q through the face between cells f-1 and f from the
@@ -83,7 +94,7 @@ __host__ __device__ inline T weno_face(T a, T b, T c, T d, T e) {
 // right-biased one.  The stencil is selected per lane, so a wavefront always
 // evaluates exactly one reconstruction per face.
 template <typename T>
-__host__ __device__ inline T upwind_weno(T U, T qm3, T qm2, T qm1, T q0, T q1, T q2) {
+__device__ inline T upwind_weno(T U, T qm3, T qm2, T qm1, T q0, T q1, T q2) {
   const bool pos = U >= T(0);
   const T a = pos ? qm3 : q2, b = pos ? qm2 : q1, c = pos ? qm1 : q0, d = pos ? q0 : qm1, e = pos ? q1 : qm2;
   return U * weno_face<T>(a, b, c, d, e);
