@@ -55,7 +55,7 @@ int vibe_mi_batch_kernel_times(vibe_mi_batch *b, double *ms_stage, double *ms_su
 int vibe_mi_batch_set_graph(vibe_mi_batch *b, int use_graph);
 /* Tile shapes picked by the autotuner for this batch (after the first
  * step): waves per substep workgroup (4, 8, 16) and rows per stage
- * workgroup (64 x rows columns: 2, 4). */
+ * workgroup (64 x rows columns: 2, 4, 8). */
 int vibe_mi_batch_tuning(vibe_mi_batch *b, int *substep_waves, int *stage_rows);
 /* Total device bytes held by the batch and its members. */
 size_t vibe_mi_batch_device_bytes(vibe_mi_batch *b);

@@ -72,6 +72,24 @@ def test_batch_matches_single(vmi, prec_name, graph):
         _assert_match(prec_name, g, ref, (c.name, graph))
 
 
+@pytest.mark.parametrize("waves,rows", [(4, 2), (8, 4), (16, 8), (8, 8)])
+def test_batch_tile_shapes_match_single(vmi, monkeypatch, waves, rows):
+    """Every substep wave count and stage tile height the autotuner can pick
+    gives the single-domain result bit for bit in FP64 (the shapes are pinned
+    through the environment, which bypasses the tuner and its cache)."""
+    monkeypatch.setenv("VIBE_MI_SUBSTEP_WAVES", str(waves))
+    monkeypatch.setenv("VIBE_MI_STAGE_ROWS", str(rows))
+    cases = _cases(vmi)
+    b = vmi.Batch(cases, vmi.FP64, use_graph=False)
+    b.step(None, nsub=6, nsteps=2)
+    assert b.tuning() == {"substep_waves": waves, "stage_rows": rows}
+    got = [m.get_state() for m in b.members]
+    b.close()
+    for c, g in zip(cases, got):
+        ref = _single(vmi, c, vmi.FP64, [c.dt] * 2, 6)
+        _assert_match("FP64", g, ref, (c.name, waves, rows))
+
+
 def test_batch_graph_replays_match_eager(vmi):
     """Several graph replays, with host work and synchronisation between
     them, each equal to the eager launch sequence."""

@@ -48,7 +48,7 @@ constexpr int kMaxNsub = 64;
 
 struct WorkList {
   TileList rows{};       // {member, x-block, j}
-  TileList stage[2]{};   // {member, x-block, y-block} for TY = 2, 4
+  TileList stage[3]{};   // {member, x-block, y-block} for TY = 2, 4, 8
   TileList sponge{};     // rows of members with a sponge
   int *ids = nullptr;    // active members
   int nids = 0;
@@ -139,7 +139,7 @@ int work_list(vibe_mi_batch *b, const std::vector<unsigned char> &act, WorkList 
   for (int m = 0; m < static_cast<int>(b->doms.size()); ++m)
     if (act[m]) order.push_back(m);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->doms[x]->nz > b->doms[y]->nz; });
-  std::vector<int4> rows, sp, st2, st4;
+  std::vector<int4> rows, sp, st2, st4, st8;
   for (int m : order) {
     const vibe_mi_domain *d = b->doms[m];
     const int nbx = static_cast<int>(ceil_div(d->nx, 64));
@@ -152,12 +152,15 @@ int work_list(vibe_mi_batch *b, const std::vector<unsigned char> &act, WorkList 
       for (int bx = 0; bx < nbx; ++bx) st2.push_back(make_int4(m, bx, by, 0));
     for (int by = 0; by < static_cast<int>(ceil_div(d->ny, 4)); ++by)
       for (int bx = 0; bx < nbx; ++bx) st4.push_back(make_int4(m, bx, by, 0));
+    for (int by = 0; by < static_cast<int>(ceil_div(d->ny, 8)); ++by)
+      for (int bx = 0; bx < nbx; ++bx) st8.push_back(make_int4(m, bx, by, 0));
   }
   WorkList w;
   if (int rc = upload_list(b, rows, &w.rows)) return rc;
   if (int rc = upload_list(b, sp, &w.sponge)) return rc;
   if (int rc = upload_list(b, st2, &w.stage[0])) return rc;
   if (int rc = upload_list(b, st4, &w.stage[1])) return rc;
+  if (int rc = upload_list(b, st8, &w.stage[2])) return rc;
   if (int rc = bupload(b, &w.ids, order)) return rc;
   w.nids = static_cast<int>(order.size());
   *out = &(b->lists[key] = w);
@@ -280,7 +283,7 @@ int launch_substep(vibe_mi_batch *b, int nw, const WorkList &w, int st, int q, i
 template <typename T, int TY, int GM>
 int launch_stage_ty(vibe_mi_batch *b, const WorkList &w, int st, int *ctr) {
   const auto *D = static_cast<const BDom<T> *>(b->desc);
-  const int li = TY == 2 ? 0 : 1;
+  const int li = TY == 2 ? 0 : (TY == 4 ? 1 : 2);
   auto *kern = &kb_stage<T, TY, GM>;
   unsigned grid = 0;
   if (int rc = resident(b, kern, 64 * TY, 0, w.stage[li].ntiles, &grid)) return rc;
@@ -291,7 +294,9 @@ int launch_stage_ty(vibe_mi_batch *b, const WorkList &w, int st, int *ctr) {
 
 template <typename T, int GM>
 int launch_stage_gm(vibe_mi_batch *b, int ty, const WorkList &w, int st, int *ctr) {
-  return ty == 2 ? launch_stage_ty<T, 2, GM>(b, w, st, ctr) : launch_stage_ty<T, 4, GM>(b, w, st, ctr);
+  if (ty == 2) return launch_stage_ty<T, 2, GM>(b, w, st, ctr);
+  if (ty == 8) return launch_stage_ty<T, 8, GM>(b, w, st, ctr);
+  return launch_stage_ty<T, 4, GM>(b, w, st, ctr);
 }
 
 template <typename T>
@@ -359,16 +364,32 @@ int autotune(vibe_mi_batch *b, const WorkList &w) {
   if (b->tuned) return 0;
   std::string shape = std::to_string(sizeof(T));
   for (auto *d : b->doms) shape += "," + std::to_string(d->nx) + "x" + std::to_string(d->ny) + "x" + std::to_string(d->nz) + "r" + std::to_string(d->rk);
-  auto it = g_btune.find(shape);
+  // VIBE_MI_SUBSTEP_WAVES (4, 8, 16) and VIBE_MI_STAGE_ROWS (2, 4, 8) pin a
+  // shape instead of measuring it (tests, profiling); pinned shapes are not
+  // cached
+  const char *env_nw = std::getenv("VIBE_MI_SUBSTEP_WAVES");
+  const char *env_ty = std::getenv("VIBE_MI_STAGE_ROWS");
+  auto it = (env_nw || env_ty) ? g_btune.end() : g_btune.find(shape);
   if (it != g_btune.end()) {
     b->nw = it->second.first;
     b->ty = it->second.second;
     b->tuned = true;
     return 0;
   }
-  if (const char *env = std::getenv("VIBE_MI_SUBSTEP_WAVES")) {
-    const int v = std::atoi(env);
-    if (v == 4 || v == 8 || v == 16) b->nw = v;
+  bool pin_nw = false, pin_ty = false;
+  if (env_nw) {
+    const int v = std::atoi(env_nw);
+    if (v == 4 || v == 8 || v == 16) {
+      b->nw = v;
+      pin_nw = true;
+    }
+  }
+  if (env_ty) {
+    const int v = std::atoi(env_ty);
+    if (v == 2 || v == 4 || v == 8) {
+      b->ty = v;
+      pin_ty = true;
+    }
   }
   auto time_it = [&](auto &&launch, float *ms) -> int {
     float total = 0.f;
@@ -385,7 +406,7 @@ int autotune(vibe_mi_batch *b, const WorkList &w) {
     *ms = total;
     return 0;
   };
-  if (!std::getenv("VIBE_MI_SUBSTEP_WAVES")) {
+  if (!pin_nw) {
     float best = 1e30f;
     for (int nw : {4, 8, 16}) {
       float ms = 0.f;
@@ -397,9 +418,9 @@ int autotune(vibe_mi_batch *b, const WorkList &w) {
     }
     (void)hipGetLastError();
   }
-  {
+  if (!pin_ty) {
     float best = 1e30f;
-    for (int ty : {2, 4}) {
+    for (int ty : {2, 4, 8}) {
       float ms = 0.f;
       const int rc = time_it([&]() { return launch_stage<T>(b, ty, w, 0, b->ctr); }, &ms);
       if (rc == 0 && ms < best) {
@@ -410,7 +431,7 @@ int autotune(vibe_mi_batch *b, const WorkList &w) {
     (void)hipGetLastError();
   }
   b->tuned = true;
-  g_btune[shape] = {b->nw, b->ty};
+  if (!env_nw && !env_ty) g_btune[shape] = {b->nw, b->ty};
   return 0;
 }
 

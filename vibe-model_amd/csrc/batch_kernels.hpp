@@ -191,7 +191,8 @@ __global__ void __launch_bounds__(256) kb_sponge(const BDom<T> *__restrict__ D, 
   VMI_BATCH_F32_SUBSTEP_GM(X, float, GM)                                                                        \
   VMI_BATCH_F32_SUBSTEP_GM(X, double, GM)                                                                       \
   X template __global__ void vibe_mi::kb_stage<float, 2, GM>(VMI_KB_STAGE_ARGS);                               \
-  X template __global__ void vibe_mi::kb_stage<float, 4, GM>(VMI_KB_STAGE_ARGS);
+  X template __global__ void vibe_mi::kb_stage<float, 4, GM>(VMI_KB_STAGE_ARGS);                               \
+  X template __global__ void vibe_mi::kb_stage<float, 8, GM>(VMI_KB_STAGE_ARGS);
 #define VMI_KB_SUBSTEP_ARGS const vibe_mi::BDom<float> *__restrict__, vibe_mi::TileList, int, int, int, int *__restrict__
 #define VMI_KB_STAGE_ARGS const vibe_mi::BDom<float> *__restrict__, vibe_mi::TileList, int, int *__restrict__
 #define VMI_BATCH_F32_ALL(X) \
