@@ -221,7 +221,7 @@ int upload_desc(vibe_mi_batch *b, const double *dt, int nsub) {
 
 // dry: only set the kernel's LDS attribute (before a graph capture).
 template <typename T, typename O, int NW, int GM>
-int launch_substep_nw(vibe_mi_batch *b, const WorkList &w, int st, int q, int last, int *ctr, bool dry) {
+int launch_substep_nw(vibe_mi_batch *b, const WorkList &w, int st, int q, int abase, int last, int *ctr, bool dry) {
   const auto *D = static_cast<const BDom<T> *>(b->desc);
   const int ch = (b->nzmax + NW - 1) / NW;
   const size_t lds = 3 * static_cast<size_t>(b->nzmax - 1) * 64 * sizeof(T);
@@ -240,7 +240,7 @@ int launch_substep_nw(vibe_mi_batch *b, const WorkList &w, int st, int q, int la
     if (dry) break;                                                                                           \
     unsigned grid = 0;                                                                                        \
     if (int rc = resident(b, kern, 64 * NW, lds, w.rows.ntiles, &grid)) return rc;                            \
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * NW), lds, b->stream, D, w.rows, st, q, last, ctr);         \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * NW), lds, b->stream, D, w.rows, st, q, abase, last, ctr);  \
   } while (0)
   if constexpr (NW == 16) {
     if (ch <= 4) VMI_BSUB(4);
@@ -266,44 +266,54 @@ int launch_substep_nw(vibe_mi_batch *b, const WorkList &w, int st, int q, int la
 }
 
 template <typename T, typename O, int GM>
-int launch_substep_gm(vibe_mi_batch *b, int nw, const WorkList &w, int st, int q, int last, int *ctr, bool dry) {
-  if (nw == 16) return launch_substep_nw<T, O, 16, GM>(b, w, st, q, last, ctr, dry);
-  if (nw == 8) return launch_substep_nw<T, O, 8, GM>(b, w, st, q, last, ctr, dry);
-  return launch_substep_nw<T, O, 4, GM>(b, w, st, q, last, ctr, dry);
+int launch_substep_gm(vibe_mi_batch *b, int nw, const WorkList &w, int st, int q, int abase, int last, int *ctr, bool dry) {
+  if (nw == 16) return launch_substep_nw<T, O, 16, GM>(b, w, st, q, abase, last, ctr, dry);
+  if (nw == 8) return launch_substep_nw<T, O, 8, GM>(b, w, st, q, abase, last, ctr, dry);
+  return launch_substep_nw<T, O, 4, GM>(b, w, st, q, abase, last, ctr, dry);
 }
 
 template <typename T, typename O>
-int launch_substep(vibe_mi_batch *b, int nw, const WorkList &w, int st, int q, int last, int *ctr,
+int launch_substep(vibe_mi_batch *b, int nw, const WorkList &w, int st, int q, int abase, int last, int *ctr,
                    bool dry = false) {
-  if (b->gm == kGeoFlat) return launch_substep_gm<T, O, kGeoFlat>(b, nw, w, st, q, last, ctr, dry);
-  if (b->gm == kGeoX) return launch_substep_gm<T, O, kGeoX>(b, nw, w, st, q, last, ctr, dry);
-  return launch_substep_gm<T, O, kGeoAny>(b, nw, w, st, q, last, ctr, dry);
+  if (b->gm == kGeoFlat) return launch_substep_gm<T, O, kGeoFlat>(b, nw, w, st, q, abase, last, ctr, dry);
+  if (b->gm == kGeoX) return launch_substep_gm<T, O, kGeoX>(b, nw, w, st, q, abase, last, ctr, dry);
+  return launch_substep_gm<T, O, kGeoAny>(b, nw, w, st, q, abase, last, ctr, dry);
 }
 
-template <typename T, int TY, int GM>
-int launch_stage_ty(vibe_mi_batch *b, const WorkList &w, int st, int *ctr) {
+// ain < 0: the stage reads the FP64 current state; otherwise (FP32 batches)
+// the FP32 acoustic set A[ain].  The stage writes A[aout].
+template <typename T, int TY, int GM, bool CUR32>
+int launch_stage_k(vibe_mi_batch *b, const WorkList &w, int st, int ain, int aout, int *ctr) {
   const auto *D = static_cast<const BDom<T> *>(b->desc);
   const int li = TY == 2 ? 0 : (TY == 4 ? 1 : 2);
-  auto *kern = &kb_stage<T, TY, GM>;
+  auto *kern = &kb_stage<T, TY, GM, CUR32>;
   unsigned grid = 0;
   if (int rc = resident(b, kern, 64 * TY, 0, w.stage[li].ntiles, &grid)) return rc;
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(64, TY), 0, b->stream, D, w.stage[li], st, ctr);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(64, TY), 0, b->stream, D, w.stage[li], st, ain, aout, ctr);
   VMI_HIP(hipGetLastError());
   return 0;
 }
 
+template <typename T, int TY, int GM>
+int launch_stage_ty(vibe_mi_batch *b, const WorkList &w, int st, int ain, int aout, int *ctr) {
+  if constexpr (sizeof(T) == 4) {
+    if (ain >= 0) return launch_stage_k<T, TY, GM, true>(b, w, st, ain, aout, ctr);
+  }
+  return launch_stage_k<T, TY, GM, false>(b, w, st, ain, aout, ctr);
+}
+
 template <typename T, int GM>
-int launch_stage_gm(vibe_mi_batch *b, int ty, const WorkList &w, int st, int *ctr) {
-  if (ty == 2) return launch_stage_ty<T, 2, GM>(b, w, st, ctr);
-  if (ty == 8) return launch_stage_ty<T, 8, GM>(b, w, st, ctr);
-  return launch_stage_ty<T, 4, GM>(b, w, st, ctr);
+int launch_stage_gm(vibe_mi_batch *b, int ty, const WorkList &w, int st, int ain, int aout, int *ctr) {
+  if (ty == 2) return launch_stage_ty<T, 2, GM>(b, w, st, ain, aout, ctr);
+  if (ty == 8) return launch_stage_ty<T, 8, GM>(b, w, st, ain, aout, ctr);
+  return launch_stage_ty<T, 4, GM>(b, w, st, ain, aout, ctr);
 }
 
 template <typename T>
-int launch_stage(vibe_mi_batch *b, int ty, const WorkList &w, int st, int *ctr) {
-  if (b->gm == kGeoFlat) return launch_stage_gm<T, kGeoFlat>(b, ty, w, st, ctr);
-  if (b->gm == kGeoX) return launch_stage_gm<T, kGeoX>(b, ty, w, st, ctr);
-  return launch_stage_gm<T, kGeoAny>(b, ty, w, st, ctr);
+int launch_stage(vibe_mi_batch *b, int ty, const WorkList &w, int st, int ain, int aout, int *ctr) {
+  if (b->gm == kGeoFlat) return launch_stage_gm<T, kGeoFlat>(b, ty, w, st, ain, aout, ctr);
+  if (b->gm == kGeoX) return launch_stage_gm<T, kGeoX>(b, ty, w, st, ain, aout, ctr);
+  return launch_stage_gm<T, kGeoAny>(b, ty, w, st, ain, aout, ctr);
 }
 
 // Enqueues one RK3 step of the active set on the batch stream (capturable:
@@ -315,18 +325,29 @@ int enqueue_step(vibe_mi_batch *b, const WorkList &w, int nsub) {
   int c = 0;
   hipLaunchKernelGGL(kb_reset, dim3(1), dim3(64), 0, b->stream, ctr, kMaxCounters);
   const bool gt = b->group_timing;
+  // Acoustic buffers: stage st writes A[aout], its substeps ping-pong from
+  // there and end in A[aend].  In FP32 batches the intermediate stage states
+  // s^(1), s^(2) only feed the next stage's tendencies, so stages 1 and 2
+  // end in FP32 (the next stage reads A[aend] and writes the other buffer);
+  // only the step's last substep writes the FP64 state.  FP64 batches keep
+  // every stage state in FP64.
+  int aout = 0, aend = 0;
   for (int st = 0; st < 3; ++st) {
+    const bool cur32 = sizeof(T) == 4 && st > 0;
+    const int ain = cur32 ? aend : -1;
+    aout = cur32 ? (aend ^ 1) : 0;
+    aend = (aout + nsub) & 1;
     if (gt) VMI_HIP(hipEventRecord(b->ev[2], b->stream));
-    if (int rc = launch_stage<T>(b, b->ty, w, st, ctr + kXcds * c++)) return rc;
+    if (int rc = launch_stage<T>(b, b->ty, w, st, ain, aout, ctr + kXcds * c++)) return rc;
     if (gt) VMI_HIP(hipEventRecord(b->ev[3], b->stream));
     for (int q = 0; q < nsub; ++q) {
       const bool last = q == nsub - 1;
       int rc;
       if constexpr (sizeof(T) == 4) {
-        rc = last ? launch_substep<T, double>(b, b->nw, w, st, q, 1, ctr + kXcds * c++)
-                  : launch_substep<T, T>(b, b->nw, w, st, q, 0, ctr + kXcds * c++);
+        rc = (last && st == 2) ? launch_substep<T, double>(b, b->nw, w, st, q, aout, 1, ctr + kXcds * c++)
+                               : launch_substep<T, T>(b, b->nw, w, st, q, aout, last ? 1 : 0, ctr + kXcds * c++);
       } else {
-        rc = launch_substep<T, T>(b, b->nw, w, st, q, last ? 1 : 0, ctr + kXcds * c++);
+        rc = launch_substep<T, T>(b, b->nw, w, st, q, aout, last ? 1 : 0, ctr + kXcds * c++);
       }
       if (rc) return rc;
     }
@@ -410,7 +431,7 @@ int autotune(vibe_mi_batch *b, const WorkList &w) {
     float best = 1e30f;
     for (int nw : {4, 8, 16}) {
       float ms = 0.f;
-      const int rc = time_it([&]() { return launch_substep<T, T>(b, nw, w, 0, 0, 0, b->ctr); }, &ms);
+      const int rc = time_it([&]() { return launch_substep<T, T>(b, nw, w, 0, 0, 0, 0, b->ctr); }, &ms);
       if (rc == 0 && ms < best) {
         best = ms;
         b->nw = nw;
@@ -422,7 +443,7 @@ int autotune(vibe_mi_batch *b, const WorkList &w) {
     float best = 1e30f;
     for (int ty : {2, 4, 8}) {
       float ms = 0.f;
-      const int rc = time_it([&]() { return launch_stage<T>(b, ty, w, 0, b->ctr); }, &ms);
+      const int rc = time_it([&]() { return launch_stage<T>(b, ty, w, 0, -1, 0, b->ctr); }, &ms);
       if (rc == 0 && ms < best) {
         best = ms;
         b->ty = ty;
@@ -468,8 +489,8 @@ int step_batch(vibe_mi_batch *b, const double *dt, int nsub, int nsteps, const u
   if (int rc = upload_desc<T>(b, dt, nsub)) return rc;
   if (int rc = autotune<T>(b, *w)) return rc;
   if constexpr (sizeof(T) == 4)
-    if (int rc = launch_substep<T, double>(b, b->nw, *w, 0, 0, 1, b->ctr, true)) return rc;
-  if (int rc = launch_substep<T, T>(b, b->nw, *w, 0, 0, 0, b->ctr, true)) return rc;
+    if (int rc = launch_substep<T, double>(b, b->nw, *w, 0, 0, 0, 1, b->ctr, true)) return rc;
+  if (int rc = launch_substep<T, T>(b, b->nw, *w, 0, 0, 0, 0, b->ctr, true)) return rc;
   VMI_HIP(hipMemsetAsync(b->flag, 0, sizeof(int), b->stream));
   hipGraphExec_t exec = nullptr;
   if (b->use_graph) {

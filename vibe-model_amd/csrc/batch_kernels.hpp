@@ -86,35 +86,47 @@ __global__ void __launch_bounds__(64) kb_advance(BDom<T> *__restrict__ D, const 
   }
 }
 
-// Fused stage kernel over tiles {member, x-block, y-block, -} of 64 x TY columns.
-template <typename T, int TY, int GM>
-__global__ void __launch_bounds__(64 * TY) kb_stage(const BDom<T> *__restrict__ D, TileList L, int st,
-                                                    int *__restrict__ ctr) {
+// Fused stage kernel over tiles {member, x-block, y-block, -} of 64 x TY
+// columns; writes u, v, w, rho', pi' of s0 + h R to the acoustic set
+// A[aout] and theta' to the stage output.  CUR32: the current state's
+// u, v, w, rho', pi' come from the FP32 acoustic set A[ain] the previous
+// stage's substeps ended in (FP32-mixed stages 2 and 3), theta' from the
+// FP64 stage output; otherwise all from the FP64 state.
+template <typename T, int TY, int GM, bool CUR32>
+__global__ void __launch_bounds__(64 * TY) kb_stage(const BDom<T> *__restrict__ D, TileList L, int st, int ain,
+                                                    int aout, int *__restrict__ ctr) {
   tile_loop(ctr, L, [&](int t) {
     const int4 tl = L.tiles[t];
     const BDom<T> &dd = D[tl.x];
     const Geo<T> g = dd.g;
     const int c0 = dd.cur0;
     const FieldSet cur = dd.S[(c0 + st) % 3], s0 = dd.S[c0], nxt = dd.S[stage_out_index(c0, st)];
-    const AcousticSet<T> a = dd.A[0];
-    const StageIn ci{cur.u, cur.v, cur.w, cur.rho, cur.th, cur.pi};
+    const AcousticSet<T> a = dd.A[aout];
     const StageIn si{s0.u, s0.v, s0.w, s0.rho, s0.th, s0.pi};
     const StageOut<T> so{a.u, a.v, a.w, a.rho, a.pi, nxt.th};
-    stage_tile<T, T, TY, GM>(g, ci, si, dd.h[st], so, dd.flag, tl.y, tl.z);
+    if constexpr (CUR32) {
+      const AcousticSet<T> ac = dd.A[ain];
+      const StageCur<T> ci{ac.u, ac.v, ac.w, ac.rho, ac.pi, cur.th};
+      stage_tile<T, T, TY, GM, T>(g, ci, si, dd.h[st], so, dd.flag, tl.y, tl.z);
+    } else {
+      const StageCur<double> ci{cur.u, cur.v, cur.w, cur.rho, cur.pi, cur.th};
+      stage_tile<T, T, TY, GM>(g, ci, si, dd.h[st], so, dd.flag, tl.y, tl.z);
+    }
   });
 }
 
-// Acoustic substep q of stage st over row tiles.  The last substep of a
-// stage (last = 1, O = double) writes the stage output S[out] and checks
-// finiteness; the others write the acoustic buffer A[(q+1)&1] (O = T).
+// Acoustic substep q of stage st over row tiles, from the acoustic set
+// A[(abase+q)&1] to A[(abase+q+1)&1].  The last substep of a stage
+// (last = 1) checks finiteness; with O = double it writes the FP64 stage
+// output S[out] instead.
 template <typename T, typename O, int NW, int MAXCH, int GM>
 __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kSubstepWavesPerEu<T, NW, MAXCH>))) kb_substep(const BDom<T> *__restrict__ D, TileList L, int st, int q,
-                                                      int last, int *__restrict__ ctr) {
+                                                      int abase, int last, int *__restrict__ ctr) {
   tile_loop(ctr, L, [&](int t) {
     const int4 tl = L.tiles[t];
     const BDom<T> &dd = D[tl.x];
     const Geo<T> g = dd.g;
-    const AcousticSet<T> in = dd.A[q & 1];
+    const AcousticSet<T> in = dd.A[(abase + q) & 1];
     SubArgs<T, T, O> sa{};
     sa.u = in.u;
     sa.v = in.v;
@@ -136,7 +148,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kS
         sa.pi_o = o.pi;
         sa.check = 1;
       } else if constexpr (sizeof(T) == 8) {
-        const AcousticSet<T> o = dd.A[(q + 1) & 1];
+        const AcousticSet<T> o = dd.A[(abase + q + 1) & 1];
         sa.uo = o.u;
         sa.vo = o.v;
         sa.wo = o.w;
@@ -144,12 +156,13 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kS
         sa.pi_o = o.pi;
       }
     } else {
-      const AcousticSet<T> o = dd.A[(q + 1) & 1];
+      const AcousticSet<T> o = dd.A[(abase + q + 1) & 1];
       sa.uo = o.u;
       sa.vo = o.v;
       sa.wo = o.w;
       sa.rho_o = o.rho;
       sa.pi_o = o.pi;
+      sa.check = last;
     }
     substep_row<T, T, O, NW, MAXCH, GM>(g, sa, tl.y, tl.z);
   });
@@ -190,10 +203,13 @@ __global__ void __launch_bounds__(256) kb_sponge(const BDom<T> *__restrict__ D, 
 #define VMI_BATCH_F32_GM(X, GM)                                                                                 \
   VMI_BATCH_F32_SUBSTEP_GM(X, float, GM)                                                                        \
   VMI_BATCH_F32_SUBSTEP_GM(X, double, GM)                                                                       \
-  X template __global__ void vibe_mi::kb_stage<float, 2, GM>(VMI_KB_STAGE_ARGS);                               \
-  X template __global__ void vibe_mi::kb_stage<float, 4, GM>(VMI_KB_STAGE_ARGS);                               \
-  X template __global__ void vibe_mi::kb_stage<float, 8, GM>(VMI_KB_STAGE_ARGS);
-#define VMI_KB_SUBSTEP_ARGS const vibe_mi::BDom<float> *__restrict__, vibe_mi::TileList, int, int, int, int *__restrict__
-#define VMI_KB_STAGE_ARGS const vibe_mi::BDom<float> *__restrict__, vibe_mi::TileList, int, int *__restrict__
+  X template __global__ void vibe_mi::kb_stage<float, 2, GM, false>(VMI_KB_STAGE_ARGS);                        \
+  X template __global__ void vibe_mi::kb_stage<float, 4, GM, false>(VMI_KB_STAGE_ARGS);                        \
+  X template __global__ void vibe_mi::kb_stage<float, 8, GM, false>(VMI_KB_STAGE_ARGS);                        \
+  X template __global__ void vibe_mi::kb_stage<float, 2, GM, true>(VMI_KB_STAGE_ARGS);                         \
+  X template __global__ void vibe_mi::kb_stage<float, 4, GM, true>(VMI_KB_STAGE_ARGS);                         \
+  X template __global__ void vibe_mi::kb_stage<float, 8, GM, true>(VMI_KB_STAGE_ARGS);
+#define VMI_KB_SUBSTEP_ARGS const vibe_mi::BDom<float> *__restrict__, vibe_mi::TileList, int, int, int, int, int *__restrict__
+#define VMI_KB_STAGE_ARGS const vibe_mi::BDom<float> *__restrict__, vibe_mi::TileList, int, int, int, int *__restrict__
 #define VMI_BATCH_F32_ALL(X) \
   VMI_BATCH_F32_GM(X, vibe_mi::kGeoFlat) VMI_BATCH_F32_GM(X, vibe_mi::kGeoX) VMI_BATCH_F32_GM(X, vibe_mi::kGeoAny)

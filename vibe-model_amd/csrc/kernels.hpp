@@ -68,14 +68,25 @@ namespace vibe_mi {
 struct StageIn {
   const double *u, *v, *w, *rho, *th, *pi;
 };
+// Current state s^(k) read by a stage: u, v, w, rho', pi' of type CI (FP64
+// state, or in FP32-mixed batches the FP32 acoustic set the previous stage's
+// substeps ended in, whose u holds u - ubg), theta' always FP64.
+template <typename CI>
+struct StageCur {
+  const CI *u, *v, *w, *rho, *pi;
+  const double *th;
+};
+__host__ __device__ inline StageCur<double> stage_cur(const StageIn &s) {
+  return StageCur<double>{s.u, s.v, s.w, s.rho, s.pi, s.th};
+}
 template <typename O>
 struct StageOut {
   O *u, *v, *w, *rho, *pi;
   double *th;
 };
 
-template <typename T, typename O, int TY, int GM>
-__device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, const StageIn &s0, double h,
+template <typename T, typename O, int TY, int GM, typename CI = double>
+__device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageCur<CI> &cur, const StageIn &s0, double h,
                                            const StageOut<O> &out, int *__restrict__ flag, int bx, int by) {
   // levels k and k+1 of u, v, w, theta', rho' over the tile and a 3-cell
   // halo, in a ring of three LDS buffers (level k+2 is filled meanwhile);
@@ -90,42 +101,58 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, 
   const int i_raw = bx * 64 + lx, j_raw = by * TY + ly;
   const bool act = i_raw < nx && j_raw < ny;  // threads outside the domain only stage the halo
   const int i = i_raw < nx ? i_raw : nx - 1, j = j_raw < ny ? j_raw : ny - 1;
-  const double *__restrict__ U = cur.u;
-  const double *__restrict__ V = cur.v;
-  const double *__restrict__ W = cur.w;
-  const double *__restrict__ RH = cur.rho;
+  const CI *__restrict__ U = cur.u;
+  const CI *__restrict__ V = cur.v;
+  const CI *__restrict__ W = cur.w;
+  const CI *__restrict__ RH = cur.rho;
   const double *__restrict__ TH = cur.th;
-  const double *__restrict__ PI = cur.pi;
-  // byte offsets (within a level) of this thread's halo slots
+  const CI *__restrict__ PI = cur.pi;
+  // FP32 current state: its u holds u - ubg (the background is added back)
+  constexpr bool cur32 = sizeof(CI) == 4;
+  constexpr uint32_t SC = sizeof(CI);
+  auto ubg_at = [&](int k) { return C(g.ubg)[k]; };
+  // element offsets (within a level) of this thread's halo slots
   uint32_t hoff[NSLOT];
 #pragma unroll
   for (int q = 0; q < NSLOT; ++q) {
     const int e = tid + q * NT < AREA ? tid + q * NT : AREA - 1;
     const int hx = e % HX, hy = e / HX;
     const int gi = ((bx * 64 - 3 + hx) % nx + nx) % nx, gj = ((by * TY - 3 + hy) % ny + ny) % ny;
-    hoff[q] = (static_cast<uint32_t>(gj) * static_cast<uint32_t>(nx) + static_cast<uint32_t>(gi)) * 8u;
+    hoff[q] = static_cast<uint32_t>(gj) * static_cast<uint32_t>(nx) + static_cast<uint32_t>(gi);
   }
-  // level stride of the FP64 fields in bytes (fields stay below 4 GiB)
-  const uint32_t lsD = static_cast<uint32_t>(nxy) * 8u;
+  // level strides in bytes of the FP64 fields and of the current-state
+  // fields (fields stay below 4 GiB)
+  const uint32_t lsD = static_cast<uint32_t>(nxy) * 8u, lsC = static_cast<uint32_t>(nxy) * SC;
   // the halo values of the next level are loaded into registers before the
   // current level's arithmetic and written to the other buffer after it;
-  // they stay FP64 until the write (a conversion right after the load would
-  // wait for the load at once)
-  double pre[5][NSLOT];
+  // they keep their storage type until the write (a conversion right after
+  // the load would wait for the load at once)
+  CI pre[4][NSLOT];  // u, v, w, rho'
+  double preT[NSLOT];  // theta'
+  T pre_ubg = T(0);
   auto load_level = [&](int k) {
-    const uint32_t b = static_cast<uint32_t>(k) * lsD;
-    const double *src[5] = {U, V, W, TH, RH};
+    const uint32_t b = static_cast<uint32_t>(k) * lsC, bt = static_cast<uint32_t>(k) * lsD;
+    const CI *src[4] = {U, V, W, RH};
 #pragma unroll
-    for (int f = 0; f < 5; ++f)
+    for (int f = 0; f < 4; ++f)
 #pragma unroll
-      for (int q = 0; q < NSLOT; ++q) pre[f][q] = gld(src[f], hoff[q] + b);
+      for (int q = 0; q < NSLOT; ++q) pre[f][q] = gld(src[f], hoff[q] * SC + b);
+#pragma unroll
+    for (int q = 0; q < NSLOT; ++q) preT[q] = gld(TH, hoff[q] * 8u + bt);
+    if constexpr (cur32) pre_ubg = ubg_at(k);
+    (void)pre_ubg;
   };
   auto store_level = [&](int buf) {
+    constexpr int dst[4] = {FU, FV, FW, FR};
 #pragma unroll
-    for (int f = 0; f < 5; ++f)
+    for (int f = 0; f < 4; ++f)
 #pragma unroll
       for (int q = 0; q < NSLOT; ++q)
-        if (q + 1 < NSLOT || tid + q * NT < AREA) s_h[buf][f][tid + q * NT] = T(pre[f][q]);
+        if (q + 1 < NSLOT || tid + q * NT < AREA)
+          s_h[buf][dst[f]][tid + q * NT] = (f == 0 && cur32) ? T(pre[f][q]) + pre_ubg : T(pre[f][q]);
+#pragma unroll
+    for (int q = 0; q < NSLOT; ++q)
+      if (q + 1 < NSLOT || tid + q * NT < AREA) s_h[buf][FT][tid + q * NT] = T(preT[q]);
   };
   const int lc = (ly + 3) * HX + lx + 3;  // own column in the halo tile
   const uint32_t unx = static_cast<uint32_t>(nx);
@@ -150,6 +177,17 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, 
   const uint32_t tCb = static_cast<uint32_t>(gC) * static_cast<uint32_t>(sizeof(T)),
                  lsT = static_cast<uint32_t>(rk) * static_cast<uint32_t>(sizeof(T));
   auto ldd = [&](const double *f, int k, uint32_t cb) { return T(gld(f, cb + static_cast<uint32_t>(k) * lsD)); };
+  // current-state fields at a level (cb: the column's FP64 byte offset);
+  // ldu adds the background wind back to an FP32 u
+  auto ldc = [&](const CI *f, int k, uint32_t cb) {
+    return T(gld(f, (cb / 8u) * SC + static_cast<uint32_t>(k) * lsC));
+  };
+  auto ldu = [&](int k, uint32_t cb) {
+    if constexpr (cur32)
+      return ldc(U, k, cb) + ubg_at(k);
+    else
+      return ldc(U, k, cb);
+  };
   // s0 of the level being updated, loaded one level ahead
   double S0[6];
   auto ld_s0 = [&](int k) {
@@ -172,18 +210,18 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, 
 #pragma unroll
   for (int q = 0; q < 7; ++q) {
     const int kc = clampk(q - 3, nz - 1);
-    uw[q] = ldd(U, kc, cCb);
-    vw[q] = ldd(V, kc, cCb);
+    uw[q] = ldu(kc, cCb);
+    vw[q] = ldc(V, kc, cCb);
     tw[q] = ldd(TH, kc, cCb);
-    ww[q] = ldd(W, clampk(q - 3, nz), cCb);
+    ww[q] = ldc(W, clampk(q - 3, nz), cCb);
   }
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     const int kc = clampk(q - 1, nz - 1);
-    rw[q] = ldd(RH, kc, cCb);
-    pw[q] = ldd(PI, kc, cCb);
-    pWw[q] = ldd(PI, kc, cWb);
-    pSw[q] = ldd(PI, kc, cSb);
+    rw[q] = ldc(RH, kc, cCb);
+    pw[q] = ldc(PI, kc, cCb);
+    pWw[q] = ldc(PI, kc, cWb);
+    pSw[q] = ldc(PI, kc, cSb);
   }
   // terrain slope flux Mc (cell centres) of the own, west and south columns
   // at the staged level in buffer `buf` (the Omega kernel's expression)
@@ -405,23 +443,23 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, 
       tw[q] = tw[q + 1];
       ww[q] = ww[q + 1];
     }
-    uw[6] = ldd(U, kn, cCb);
-    vw[6] = ldd(V, kn, cCb);
+    uw[6] = ldu(kn, cCb);
+    vw[6] = ldc(V, kn, cCb);
     tw[6] = ldd(TH, kn, cCb);
-    ww[6] = ldd(W, kw, cCb);
+    ww[6] = ldc(W, kw, cCb);
     const int k2 = clampk(k + 2, nz - 1);
     rw[0] = rw[1];
     rw[1] = rw[2];
-    rw[2] = ldd(RH, k2, cCb);
+    rw[2] = ldc(RH, k2, cCb);
     pw[0] = pw[1];
     pw[1] = pw[2];
-    pw[2] = ldd(PI, k2, cCb);
+    pw[2] = ldc(PI, k2, cCb);
     pWw[0] = pWw[1];
     pWw[1] = pWw[2];
-    pWw[2] = ldd(PI, k2, cWb);
+    pWw[2] = ldc(PI, k2, cWb);
     pSw[0] = pSw[1];
     pSw[1] = pSw[2];
-    pSw[2] = ldd(PI, k2, cSb);
+    pSw[2] = ldc(PI, k2, cSb);
     // operands of level k+1 from s0 and the reference state (unconditional,
     // clamped: a load under a branch would be waited for where it joins)
     {
@@ -442,7 +480,7 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageIn &cur, 
 template <typename T, typename O, int TY>
 __global__ void __launch_bounds__(64 * TY) k_stage_col(Geo<T> g, StageIn cur, StageIn s0, double h, StageOut<O> out,
                                                         int *__restrict__ flag) {
-  stage_tile<T, O, TY, kGeoAny>(g, cur, s0, h, out, flag, blockIdx.x, blockIdx.y);
+  stage_tile<T, O, TY, kGeoAny>(g, stage_cur(cur), s0, h, out, flag, blockIdx.x, blockIdx.y);
 }
 
 // ---- HEVI factorisation table ----------------------------------------------
