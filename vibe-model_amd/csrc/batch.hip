@@ -6,8 +6,10 @@
 //   for each stage st:
 //     kb_stage    64 x TY      fused Omega, slow tendencies and RK3 update
 //     nsub x kb_substep rows   fused HEVI acoustic substep
-//   kb_sponge   rows           Rayleigh sponge
 //   kb_advance                 rotate every active member's state buffers
+// The Rayleigh sponge is applied as the last stage and the last substep
+// write the step's final state; FP32 batches keep the intermediate stage
+// states in the FP32 acoustic buffers.
 // Each launch is persistent: a grid of about one wave of resident
 // workgroups, each claiming tiles from an atomic counter until the list is
 // drained (tiles are ordered largest column first so the tail is short).
@@ -49,7 +51,6 @@ constexpr int kMaxNsub = 64;
 struct WorkList {
   TileList rows{};       // {member, x-block, j}
   TileList stage[3]{};   // {member, x-block, y-block} for TY = 2, 4, 8
-  TileList sponge{};     // rows of members with a sponge
   int *ids = nullptr;    // active members
   int nids = 0;
 };
@@ -139,15 +140,12 @@ int work_list(vibe_mi_batch *b, const std::vector<unsigned char> &act, WorkList 
   for (int m = 0; m < static_cast<int>(b->doms.size()); ++m)
     if (act[m]) order.push_back(m);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->doms[x]->nz > b->doms[y]->nz; });
-  std::vector<int4> rows, sp, st2, st4, st8;
+  std::vector<int4> rows, st2, st4, st8;
   for (int m : order) {
     const vibe_mi_domain *d = b->doms[m];
     const int nbx = static_cast<int>(ceil_div(d->nx, 64));
     for (int j = 0; j < d->ny; ++j)
-      for (int bx = 0; bx < nbx; ++bx) {
-        rows.push_back(make_int4(m, bx, j, 0));
-        if (d->kmin_sponge <= d->nz) sp.push_back(make_int4(m, bx, j, 0));
-      }
+      for (int bx = 0; bx < nbx; ++bx) rows.push_back(make_int4(m, bx, j, 0));
     for (int by = 0; by < static_cast<int>(ceil_div(d->ny, 2)); ++by)
       for (int bx = 0; bx < nbx; ++bx) st2.push_back(make_int4(m, bx, by, 0));
     for (int by = 0; by < static_cast<int>(ceil_div(d->ny, 4)); ++by)
@@ -157,7 +155,6 @@ int work_list(vibe_mi_batch *b, const std::vector<unsigned char> &act, WorkList 
   }
   WorkList w;
   if (int rc = upload_list(b, rows, &w.rows)) return rc;
-  if (int rc = upload_list(b, sp, &w.sponge)) return rc;
   if (int rc = upload_list(b, st2, &w.stage[0])) return rc;
   if (int rc = upload_list(b, st4, &w.stage[1])) return rc;
   if (int rc = upload_list(b, st8, &w.stage[2])) return rc;
@@ -320,7 +317,6 @@ int launch_stage(vibe_mi_batch *b, int ty, const WorkList &w, int st, int ain, i
 // kernels only).
 template <typename T>
 int enqueue_step(vibe_mi_batch *b, const WorkList &w, int nsub) {
-  const auto *D = static_cast<const BDom<T> *>(b->desc);
   int *ctr = b->ctr;
   int c = 0;
   hipLaunchKernelGGL(kb_reset, dim3(1), dim3(64), 0, b->stream, ctr, kMaxCounters);
@@ -363,12 +359,8 @@ int enqueue_step(vibe_mi_batch *b, const WorkList &w, int nsub) {
       b->n_sub += nsub;
     }
   }
-  if (w.sponge.ntiles > 0) {
-    auto *kern = &kb_sponge<T>;
-    unsigned grid = 0;
-    if (int rc = resident(b, kern, 256, 0, w.sponge.ntiles, &grid)) return rc;
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(64, 4), 0, b->stream, D, w.sponge, ctr + kXcds * c++);
-  }
+  // the sponge is applied by the step's last stage kernel (theta') and last
+  // substep (u, v, w, rho', pi') as they write the final state
   hipLaunchKernelGGL((kb_advance<T>), dim3(1), dim3(64), 0, b->stream, static_cast<BDom<T> *>(b->desc), w.ids, w.nids);
   VMI_HIP(hipGetLastError());
   return 0;

@@ -104,13 +104,15 @@ __global__ void __launch_bounds__(64 * TY) kb_stage(const BDom<T> *__restrict__ 
     const AcousticSet<T> a = dd.A[aout];
     const StageIn si{s0.u, s0.v, s0.w, s0.rho, s0.th, s0.pi};
     const StageOut<T> so{a.u, a.v, a.w, a.rho, a.pi, nxt.th};
+    SpongeArgs sp;  // the last stage's theta' is the step's final one
+    if (st == 2) sp = SpongeArgs{dd.sponge, dd.dt, dd.kmin_sponge};
     if constexpr (CUR32) {
       const AcousticSet<T> ac = dd.A[ain];
       const StageCur<T> ci{ac.u, ac.v, ac.w, ac.rho, ac.pi, cur.th};
-      stage_tile<T, T, TY, GM, T>(g, ci, si, dd.h[st], so, dd.flag, tl.y, tl.z);
+      stage_tile<T, T, TY, GM, T>(g, ci, si, dd.h[st], so, dd.flag, tl.y, tl.z, sp);
     } else {
       const StageCur<double> ci{cur.u, cur.v, cur.w, cur.rho, cur.pi, cur.th};
-      stage_tile<T, T, TY, GM>(g, ci, si, dd.h[st], so, dd.flag, tl.y, tl.z);
+      stage_tile<T, T, TY, GM>(g, ci, si, dd.h[st], so, dd.flag, tl.y, tl.z, sp);
     }
   });
 }
@@ -147,6 +149,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kS
         sa.rho_o = o.rho;
         sa.pi_o = o.pi;
         sa.check = 1;
+        if (st == 2) sa.sp = SpongeArgs{dd.sponge, dd.dt, dd.kmin_sponge};  // the step's final state
       } else if constexpr (sizeof(T) == 8) {
         const AcousticSet<T> o = dd.A[(abase + q + 1) & 1];
         sa.uo = o.u;
@@ -165,22 +168,6 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kS
       sa.check = last;
     }
     substep_row<T, T, O, NW, MAXCH, GM>(g, sa, tl.y, tl.z);
-  });
-}
-
-// Sponge over row tiles of the members' final stage output.
-template <typename T>
-__global__ void __launch_bounds__(256) kb_sponge(const BDom<T> *__restrict__ D, TileList L,
-                                                 int *__restrict__ ctr) {
-  tile_loop(ctr, L, [&](int t) {
-    const int4 tl = L.tiles[t];
-    const BDom<T> &dd = D[tl.x];
-    const int nx = dd.g.nx, ny = dd.g.ny, nz = dd.g.nz;
-    const FieldSet fin = dd.S[(dd.cur0 + 1) % 3];
-    const int i = tl.y * 64 + static_cast<int>(threadIdx.x);
-    if (i < nx)
-      for (int k = dd.kmin_sponge + static_cast<int>(threadIdx.y); k <= nz; k += 4)
-        sponge_at(nx, ny, nz, dd.sponge, dd.dt, fin, i, tl.z, k);
   });
 }
 

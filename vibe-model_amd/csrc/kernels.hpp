@@ -85,9 +85,26 @@ struct StageOut {
   double *th;
 };
 
+// Rayleigh sponge folded into the writes of a step's final state
+// (damping.cpp:76-138 with perturbation targets 0): level k >= kmin is
+// scaled by 1 / (1 + dt alpha(min(k, nz-1))), the factor sponge_at applies
+// (alpha = 0 gives exactly 1).  kmin beyond nz: no sponge.
+struct SpongeArgs {
+  const double *alpha = nullptr;
+  double dt = 0.0;
+  int kmin = 0x7fffffff;
+};
+__device__ __forceinline__ double sponge_scale(const SpongeArgs &sp, int k, int nz, double x) {
+  if (k < sp.kmin) return x;
+  const double a = C(sp.alpha)[k < nz - 1 ? k : nz - 1];
+  const double f = 1.0 / (1.0 + sp.dt * a);
+  return f * x;
+}
+
 template <typename T, typename O, int TY, int GM, typename CI = double>
 __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageCur<CI> &cur, const StageIn &s0, double h,
-                                           const StageOut<O> &out, int *__restrict__ flag, int bx, int by) {
+                                           const StageOut<O> &out, int *__restrict__ flag, int bx, int by,
+                                           const SpongeArgs &sp = SpongeArgs{}) {
   // levels k and k+1 of u, v, w, theta', rho' over the tile and a 3-cell
   // halo, in a ring of three LDS buffers (level k+2 is filled meanwhile);
   // level k+1 gives Omega at the top face of level k
@@ -426,7 +443,7 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageCur<CI> &
       const double pn = S0[5] + h * static_cast<double>(rp);
       if (act) {
         gst(out.rho, ko, static_cast<O>(rn));
-        gst(out.th, cCb + static_cast<uint32_t>(k) * lsD, tn);
+        gst(out.th, cCb + static_cast<uint32_t>(k) * lsD, sponge_scale(sp, k, nz, tn));
         gst(out.pi, ko, static_cast<O>(pn));
         bad |= !isfinite(tn);
       }
@@ -545,6 +562,7 @@ struct SubArgs {
   O *uo, *vo, *wo, *rho_o, *pi_o;  // acoustic state out
   int check;                       // 1: flag non-finite outputs
   int *flag;
+  SpongeArgs sp = {};              // FP64 outputs only: sponge of the step's final state
 };
 
 // Waves per SIMD the substep kernels are compiled for: FP32 shapes with up
@@ -740,11 +758,14 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         const T vn_s = xs[2] - cpdt * (gs[GVS] * py_s);
         const T vn_n = xs[3] - cpdt * (gs[GVN] * py_n);
         if (active) {
-          if constexpr (sizeof(T) == 4 && sizeof(O) == 8)
-            gst(uo, oC + so, C(g.ubg64)[k] + static_cast<double>(du_w));
-          else
+          if constexpr (sizeof(O) == 8) {
+            const double uf = sizeof(T) == 4 ? C(g.ubg64)[k] + static_cast<double>(du_w) : static_cast<double>(du_w);
+            gst(uo, oC + so, sponge_scale(sa.sp, k, nz, uf));
+            gst(vo, oC + so, sponge_scale(sa.sp, k, nz, static_cast<double>(vn_s)));
+          } else {
             gst(uo, oC + so, static_cast<O>(du_w));
-          gst(vo, oC + so, static_cast<O>(vn_s));
+            gst(vo, oC + so, static_cast<O>(vn_s));
+          }
         }
         T mc = T(0);
         if constexpr (yslope)
@@ -951,9 +972,15 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         const T pn = Q[c] + dd[1] * vd;
         const T wn = (k >= 1) ? s_td[(k - 1) * 64 + lane] : mc0;
         if (active) {
-          gst(rho_o, oC + so, static_cast<O>(rn));
-          gst(pi_o, oC + so, static_cast<O>(pn));
-          gst(wo, oC + so, static_cast<O>(wn));
+          if constexpr (sizeof(O) == 8) {
+            gst(rho_o, oC + so, sponge_scale(sa.sp, k, nz, static_cast<double>(rn)));
+            gst(pi_o, oC + so, sponge_scale(sa.sp, k, nz, static_cast<double>(pn)));
+            gst(wo, oC + so, sponge_scale(sa.sp, k, nz, static_cast<double>(wn)));
+          } else {
+            gst(rho_o, oC + so, static_cast<O>(rn));
+            gst(pi_o, oC + so, static_cast<O>(pn));
+            gst(wo, oC + so, static_cast<O>(wn));
+          }
         }
         bad |= !(isfinite(rn) && isfinite(pn) && isfinite(wn));
         flo = fhi;
@@ -961,7 +988,10 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
     }
     if (k0 + nlev == nz && active) {
       const uint32_t s = static_cast<uint32_t>(nz);
-      gst(wo, oC + s * lsO, static_cast<O>(T(gld(w, vC + s * lsI))));
+      if constexpr (sizeof(O) == 8)
+        gst(wo, oC + s * lsO, sponge_scale(sa.sp, nz, nz, static_cast<double>(T(gld(w, vC + s * lsI)))));
+      else
+        gst(wo, oC + s * lsO, static_cast<O>(T(gld(w, vC + s * lsI))));
     }
     if (check && bad && active) atomicOr(flag, 1);
   }
