@@ -22,13 +22,14 @@ Roofline: algorithmic bytes per cell-update (compulsory prognostic traffic)
   FP64   8 * (48 + 30 * n_s) = 1824 B     (n_s = 6)
   mixed  8 * 48 + 4 * 30 * n_s = 1104 B
 against 8.0 TB/s.  The dominant kernel is the fused acoustic substep
-kb_substep (3 n_s launches per step, about 60 % of the step).  Its algorithmic
+kb_substep (3 n_s launches per step, about 70 % of the step).  Its algorithmic
 bytes per launch: read u, v, w, rho', pi' and write them once, 4 B each in
-the FP32-mixed path (40 B per cell), except the last substep of a stage,
-which writes its FP64 stage output (20 + 40 = 60 B per cell); averaged over
-the n_s launches of a stage: (40 (n_s - 1) + 60) / n_s B per cell.  The
-launch duration is measured with HIP events on the batch stream around each
-stage's substep group (only kb_substep launches are in the group).
+the FP32-mixed path (40 B per cell), except the step's last substep, which
+writes the FP64 state (20 + 40 = 60 B per cell; the intermediate stage
+states stay FP32); averaged over the 3 n_s launches of a step:
+(40 (3 n_s - 1) + 60) / (3 n_s) B per cell.  The launch duration is measured
+with HIP events on the batch stream around each stage's substep group (only
+kb_substep launches are in the group), summed over the step.
 "traffic" (HBM bytes per launch) comes from the rocprofv3 FETCH_SIZE /
 WRITE_SIZE per-dispatch summary committed for the same workload, each
 scaled by what the counter reads for a plain 1 GiB elementwise copy
@@ -59,8 +60,10 @@ def bytes_per_cell_update(nsub: int, mixed: bool) -> int:
 
 
 def substep_bytes_per_launch(cells: int, nsub: int) -> float:
-    """Algorithmic bytes of one k_substep launch, averaged over a stage."""
-    return cells * (40.0 * (nsub - 1) + 60.0) / nsub
+    """Algorithmic bytes of one kb_substep launch (FP32-mixed batch),
+    averaged over the 3 nsub launches of a step."""
+    n = 3 * nsub
+    return cells * (40.0 * (n - 1) + 60.0) / n
 
 
 def measured_traffic(cases):
