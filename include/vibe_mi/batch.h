@@ -1,12 +1,15 @@
 /* vibe_mi/batch.h -- C ABI of the batched dynamical-core step.
  *
  * Many domains of uneven size (ensemble members, nest levels) advanced
- * together on one GPU: each kernel phase of the RK3 step (Omega, the fused
- * stage kernel, each acoustic substep, the sponge) is ONE persistent launch
- * whose workgroups take tiles of every active domain from a device-side work
- * list.  The launch sequence of a step can be captured once in a hipGraph
- * and replayed.  Numerically every member advances exactly as it would
- * alone through vibe_mi_domain_step (the same kernels on the same tiles).
+ * together on one GPU: each kernel phase of the RK3 step (the fused stage
+ * kernel, each acoustic substep) is ONE persistent launch whose workgroups
+ * take tiles of every active domain from a device-side work list; the
+ * Rayleigh sponge is applied by the last stage kernel and the last substep
+ * as they write the step's final state.  The launch sequence of a step can
+ * be captured once in a hipGraph and replayed.  In FP64 every member
+ * advances bit for bit as it would alone through vibe_mi_domain_step;
+ * FP32-mixed members keep the intermediate RK3 stage states in FP32 and
+ * agree with it to the mixed-precision tolerance.
  *
  * Replaces, for a set of domains on one GPU, the reference's per-domain
  * driver loop (src/driver/driver.cpp:288, one Integrator::step per domain
@@ -46,7 +49,7 @@ int vibe_mi_batch_step(vibe_mi_batch *b, const double *dt, int nsub, int nsteps,
 int vibe_mi_batch_step_timed(vibe_mi_batch *b, const double *dt, int nsub, int nsteps,
                              const unsigned char *active, double *ms_total);
 /* Per-group times of the last timed call made without a graph (ms summed
- * over the call): Omega + stage kernel per stage, and the stage's substep
+ * over the call): the stage kernel per stage, and the stage's substep
  * launches, measured with HIP events on the batch stream. */
 int vibe_mi_batch_kernel_times(vibe_mi_batch *b, double *ms_stage, double *ms_substep, int *n_stage,
                                int *n_substep);
@@ -55,7 +58,9 @@ int vibe_mi_batch_kernel_times(vibe_mi_batch *b, double *ms_stage, double *ms_su
 int vibe_mi_batch_set_graph(vibe_mi_batch *b, int use_graph);
 /* Tile shapes picked by the autotuner for this batch (after the first
  * step): waves per substep workgroup (4, 8, 16) and rows per stage
- * workgroup (64 x rows columns: 2, 4, 8). */
+ * workgroup (64 x rows columns: 2, 4, 8).  The environment variables
+ * VIBE_MI_SUBSTEP_WAVES and VIBE_MI_STAGE_ROWS pin either choice (and skip
+ * the tuning cache). */
 int vibe_mi_batch_tuning(vibe_mi_batch *b, int *substep_waves, int *stage_rows);
 /* Total device bytes held by the batch and its members. */
 size_t vibe_mi_batch_device_bytes(vibe_mi_batch *b);

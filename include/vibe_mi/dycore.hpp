@@ -6,6 +6,9 @@
 //   vibe::timeint::RungeKutta3Integrator  include/vibe/time/runge_kutta.hpp:46-81
 //   vibe::dyn::energy_budget            src/dyn/diagnostics.cpp:255-277
 //   vibe::NumericalError                include/vibe/common/error.hpp:35
+//   vibe::io::RestartInfo, write_restart, read_restart
+//                                       include/vibe/io/field_io.hpp:82-94
+//   vibe::IoError                       include/vibe/common/error.hpp:41
 // Header-only; link against libvibe_mi.so.  The state lives on the GPU; the
 // host-side State is used only to upload initial conditions and download
 // results.
@@ -19,6 +22,7 @@
 
 #include "vibe_mi/c_kernels.h"
 #include "vibe_mi/dycore.h"
+#include "vibe_mi/vibebin.h"
 
 namespace vibe_mi {
 
@@ -37,8 +41,20 @@ class DeviceError : public std::runtime_error {
   int code_;
 };
 
+/// A .vibebin file could not be opened, parsed or written, or lacks a record.
+class IoError : public std::runtime_error {
+ public:
+  IoError(const std::string &what, int code)
+      : std::runtime_error(what + " failed with code " + std::to_string(code)), code_(code) {}
+  int code() const noexcept { return code_; }
+
+ private:
+  int code_;
+};
+
 inline void check(int rc, const char *what) {
   if (rc == VIBE_MI_ERR_NONFINITE) throw NumericalError(std::string(what) + ": non-finite model state");
+  if (rc == VIBE_MI_ERR_IO || rc == VIBE_MI_ERR_FORMAT || rc == VIBE_MI_ERR_NOT_FOUND) throw IoError(what, rc);
   if (rc != 0) throw DeviceError(what, rc);
 }
 
@@ -165,5 +181,37 @@ class RungeKutta3Integrator final : public Integrator {
   vibe_mi_domain *h_ = nullptr;
   int nx_, ny_, nz_;
 };
+
+/// time and step of a restart (the fields of the reference's RestartInfo that
+/// its restart file stores).
+struct RestartInfo {
+  double time = 0.0;
+  int step = 0;
+};
+
+/// Writes s as a .vibebin restart file (write_restart role): the reference's
+/// 12 species as totals against spec's reference state, plus the exact
+/// perturbations.
+inline void write_restart(const std::string &path, const DomainSpec &spec, const State &s,
+                          const RestartInfo &info) {
+  check(vibe_mi_restart_write(path.c_str(), spec.nx, spec.ny, spec.nz, info.time, info.step, s.u.data(),
+                              s.v.data(), s.w.data(), s.rho.data(), s.theta.data(), s.pi.data(),
+                              spec.rho0.data(), spec.theta0.data(), spec.pi0.data()),
+        "vibe_mi_restart_write");
+}
+
+/// Reads a restart file into s (read_restart role); s.time and s.step are set
+/// from the file as the reference does.
+inline RestartInfo read_restart(const std::string &path, const DomainSpec &spec, State &s) {
+  if (s.u.empty()) s = State(spec.nx, spec.ny, spec.nz);
+  RestartInfo info;
+  check(vibe_mi_restart_read(path.c_str(), spec.nx, spec.ny, spec.nz, &info.time, &info.step, s.u.data(),
+                             s.v.data(), s.w.data(), s.rho.data(), s.theta.data(), s.pi.data(),
+                             spec.rho0.data(), spec.theta0.data(), spec.pi0.data()),
+        "vibe_mi_restart_read");
+  s.time = info.time;
+  s.step = info.step;
+  return info;
+}
 
 }  // namespace vibe_mi

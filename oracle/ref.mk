@@ -16,3 +16,16 @@ $(OUT): $(REF)/src/common/c_kernels.c $(REF)/include/vibe/common/c_kernels.h
 
 clean:
 	rm -f $(OUT)
+
+# The reference's .vibebin writer (src/io/binary_writer.cpp, plus
+# src/grid/geometry.cpp for grid::Grid), linked with this directory's
+# ref_vibebin.cpp driver; writes the golden files of tests/golden/.
+CXX ?= g++
+CXXFLAGS = -O1 -std=c++20 -I$(REF)/include
+VBIN = _ref/ref_vibebin
+
+all: $(VBIN)
+
+$(VBIN): ref_vibebin.cpp $(REF)/src/io/binary_writer.cpp $(REF)/src/grid/geometry.cpp
+	mkdir -p _ref
+	$(CXX) $(CXXFLAGS) -o $@ ref_vibebin.cpp $(REF)/src/io/binary_writer.cpp $(REF)/src/grid/geometry.cpp

@@ -1,9 +1,10 @@
 // run_warm_bubble.cpp -- C++20 caller of the MI355X step, the role of the
 // reference's examples/embed_model.cpp:61-131: build grid, reference state
 // and initial state, construct the integrator, loop step(), report the
-// energy budget before and after.
+// energy budget before and after; optionally write the final state as a
+// .vibebin restart (write_restart role, src/io/field_io.cpp:207-227).
 //
-//   run_warm_bubble [nx ny nz steps]     (defaults 64 64 40 150)
+//   run_warm_bubble [nx ny nz steps [restart.vibebin]]   (defaults 64 64 40 150)
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -66,5 +67,13 @@ int main(int argc, char **argv) {
   std::printf("final:   mass %.12e kg  total energy %.12e J\n", e1.mass, e1.total);
   std::printf("relative mass change %.3e, energy change %.3e\n", (e1.mass - e0.mass) / e0.mass,
               (e1.total - e0.total) / e0.total);
+  if (argc > 5) {
+    integ.download(s);
+    vibe_mi::write_restart(argv[5], spec, s, {t, steps});
+    vibe_mi::State back;
+    const vibe_mi::RestartInfo info = vibe_mi::read_restart(argv[5], spec, back);
+    std::printf("restart %s: t = %.1f s, step %d, theta' round trip %s\n", argv[5], info.time, info.step,
+                back.theta == s.theta ? "exact" : "DIFFERS");
+  }
   return 0;
 }

@@ -13,7 +13,11 @@ names mirror the reference's interfaces:
     (include/vibe/time/integrator.hpp:62-82, src/time/runge_kutta.cpp:77-118),
     ``slow_tendency`` (Equations::tendencies, src/dyn/equations.cpp:60-150),
     ``acoustic_substep`` (src/time/runge_kutta.cpp:65-75) and
-    ``energy_budget`` (src/dyn/diagnostics.cpp:255-277).
+    ``energy_budget`` (src/dyn/diagnostics.cpp:255-277);
+  * ``vibebin_*`` and ``Domain.write_restart`` / ``read_restart`` -- the
+    reference's .vibebin BinaryWriter / BinaryReader and write_restart /
+    read_restart (src/io/binary_writer.cpp, src/io/field_io.cpp:207-250),
+    same bytes on disk (include/vibe_mi/vibebin.h).
 
 Every compute call runs on the GPU; if the library is missing or no GPU is
 visible the calls raise instead of computing anything on the CPU.
@@ -35,6 +39,9 @@ VIBE_C_ERR_BAD_ARG = 2
 VIBE_MI_ERR_HIP = 100
 VIBE_MI_ERR_NO_DEVICE = 101
 VIBE_MI_ERR_NONFINITE = 102
+VIBE_MI_ERR_IO = 110
+VIBE_MI_ERR_FORMAT = 111
+VIBE_MI_ERR_NOT_FOUND = 112
 FP64 = 0
 MIXED = 1
 
@@ -49,6 +56,11 @@ class VibeError(RuntimeError):
 
 class NumericalError(VibeError):
     """The state became non-finite (runge_kutta.cpp:103-106 role)."""
+
+
+class IoError(VibeError):
+    """A .vibebin file could not be opened, parsed or written, or lacks a
+    record (vibe::IoError role, include/vibe/common/error.hpp:41)."""
 
 
 _D = ctypes.c_double
@@ -133,6 +145,14 @@ def lib() -> ctypes.CDLL:
         "vibe_mi_ic_noise": (_I, [_I, _I, _I, _D, _PD, _PD, _D, _D, ctypes.c_uint64, _PD]),
         "vibe_mi_ic_wind": (_I, [_I, _I, _I, _D, _PD, _PD, _D, _D, _D, _PD]),
         "vibe_mi_ic_balance_pi": (_I, [_SZ, _PD, _PD, _PD, _PD, _PD]),
+        "vibe_mi_vibebin_header": (_I, [ctypes.c_char_p] + [ctypes.POINTER(_I)] * 4 + [_PD] +
+                                   [ctypes.POINTER(_I)] * 2),
+        "vibe_mi_vibebin_name": (_I, [ctypes.c_char_p, _I, ctypes.c_char_p]),
+        "vibe_mi_vibebin_write": (_I, [ctypes.c_char_p, _I, _I, _I, _D, _I,
+                                       ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(_PD)]),
+        "vibe_mi_vibebin_read": (_I, [ctypes.c_char_p, ctypes.c_char_p, _PD, _SZ]),
+        "vibe_mi_restart_write": (_I, [ctypes.c_char_p, _I, _I, _I, _D, _I] + [_PD] * 9),
+        "vibe_mi_restart_read": (_I, [ctypes.c_char_p, _I, _I, _I, _PD, ctypes.POINTER(_I)] + [_PD] * 9),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -159,6 +179,8 @@ def _pf(a):
 def _check(rc: int, what: str):
     if rc == VIBE_MI_ERR_NONFINITE:
         raise NumericalError(rc, what)
+    if rc in (VIBE_MI_ERR_IO, VIBE_MI_ERR_FORMAT, VIBE_MI_ERR_NOT_FOUND):
+        raise IoError(rc, what)
     if rc != 0:
         raise VibeError(rc, what)
 
@@ -239,6 +261,97 @@ def advect_scalar_flux(q, rho, u, v, w, dzeta, zs, nx, ny, nz, dx, dy, z_top):
     lib().vibe_advect_scalar_flux_f64(*[_pd(x) for x in arrs], _pd(zsa), nx, ny, nz, dx, dy, z_top, _pd(out))
     _check_last("vibe_advect_scalar_flux")
     return out
+
+
+# --------------------------------------------------------------------------
+# .vibebin files (include/vibe_mi/vibebin.h; src/io/binary_writer.cpp roles)
+# --------------------------------------------------------------------------
+
+def vibebin_shape(name: str, nx: int, ny: int, nz: int):
+    """(ni, nj, nk) of a record, from its name (binary_writer.cpp:117-138)."""
+    if name == "__restart_info__":
+        return (2, 1, 1)
+    if name in ("u", "u_facex"):
+        return (nx + 1, ny, nz)
+    if name in ("v", "v_facey"):
+        return (nx, ny + 1, nz)
+    if name in ("w", "zeta", "w_facez"):
+        return (nx, ny, nz + 1)
+    return (nx, ny, nz)
+
+
+def vibebin_header(path) -> dict:
+    """Header of a .vibebin file (read_binary_header role)."""
+    v = [_I() for _ in range(4)]
+    t = _D()
+    nv, nl = _I(), _I()
+    _check(lib().vibe_mi_vibebin_header(os.fsencode(path), *[ctypes.byref(x) for x in v], ctypes.byref(t),
+                                        ctypes.byref(nv), ctypes.byref(nl)), "vibe_mi_vibebin_header")
+    return {"nx": v[0].value, "ny": v[1].value, "nz": v[2].value, "real_kind": v[3].value,
+            "time": t.value, "nvars": nv.value, "nlevels": nl.value}
+
+
+def vibebin_variables(path) -> list:
+    """Record names in file order (BinaryReader::variables role)."""
+    n = vibebin_header(path)["nvars"]
+    out = []
+    for i in range(n):
+        buf = ctypes.create_string_buffer(32)
+        _check(lib().vibe_mi_vibebin_name(os.fsencode(path), i, buf), "vibe_mi_vibebin_name")
+        out.append(buf.value.decode("ascii", errors="replace"))
+    return out
+
+
+def vibebin_write(path, nx: int, ny: int, nz: int, records: dict, time: float = 0.0):
+    """Writes records {name: array} in insertion order (BinaryWriter role);
+    each array is flattened in C order and must hold the name's shape."""
+    names, arrs = [], []
+    for name, a in records.items():
+        shp = vibebin_shape(name[:31], nx, ny, nz)
+        arr = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        if arr.size != shp[0] * shp[1] * shp[2]:
+            raise ValueError(f"record {name!r} holds {arr.size} values, its name implies {shp}")
+        names.append(name.encode("ascii"))
+        arrs.append(arr)
+    cn = (ctypes.c_char_p * max(1, len(names)))(*names)
+    cd = (_PD * max(1, len(arrs)))(*[_pd(a) for a in arrs])
+    _check(lib().vibe_mi_vibebin_write(os.fsencode(path), nx, ny, nz, float(time), len(names), cn, cd),
+           "vibe_mi_vibebin_write")
+
+
+def vibebin_read(path, name: str) -> np.ndarray:
+    """One record as an (nk, nj, ni) array (BinaryReader::read role)."""
+    h = vibebin_header(path)
+    ni, nj, nk = vibebin_shape(name, h["nx"], h["ny"], h["nz"])
+    out = np.empty(ni * nj * nk)
+    _check(lib().vibe_mi_vibebin_read(os.fsencode(path), name.encode("ascii"), _pd(out), out.size),
+           "vibe_mi_vibebin_read")
+    return out.reshape(nk, nj, ni)
+
+
+def restart_write(path, case: "Case", state: dict, time: float = 0.0, step: int = 0):
+    """Restart file of a state in the layout of dycore.h (write_restart role,
+    field_io.cpp:207-227): totals of the reference's 12 species plus the exact
+    perturbations."""
+    st = [np.ascontiguousarray(state[k], dtype=np.float64).reshape(-1) for k in STATE_FIELDS]
+    ref = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (case.rho0, case.theta0, case.pi0)]
+    _check(lib().vibe_mi_restart_write(os.fsencode(path), case.nx, case.ny, case.nz, float(time), int(step),
+                                       *[_pd(a) for a in st + ref]), "vibe_mi_restart_write")
+
+
+def restart_read(path, case: "Case"):
+    """(state, time, step) from a restart file (read_restart role,
+    field_io.cpp:229-250); a file without the exact perturbations (one the
+    reference wrote) gives totals minus the case's reference state."""
+    cell = case.nx * case.ny * case.nz
+    st = {k: np.empty(cell) for k in STATE_FIELDS}
+    st["w"] = np.empty(case.nx * case.ny * (case.nz + 1))
+    ref = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (case.rho0, case.theta0, case.pi0)]
+    t, s = _D(), _I()
+    _check(lib().vibe_mi_restart_read(os.fsencode(path), case.nx, case.ny, case.nz, ctypes.byref(t),
+                                      ctypes.byref(s), *[_pd(st[k]) for k in STATE_FIELDS],
+                                      *[_pd(a) for a in ref]), "vibe_mi_restart_read")
+    return st, t.value, s.value
 
 
 # --------------------------------------------------------------------------
@@ -368,6 +481,20 @@ class Domain:
 
     def acoustic_substep(self, dtau: float):
         _check(lib().vibe_mi_domain_substep(self.h, dtau), "vibe_mi_domain_substep")
+
+    def write_restart(self, path, time: float = 0.0, step: int = 0):
+        """Downloads the state and writes it as a restart file
+        (write_restart role, src/io/field_io.cpp:207-227)."""
+        restart_write(path, self.case, self.get_state(), time, step)
+
+    def read_restart(self, path):
+        """Loads a restart file into the device state; returns (time, step)
+        (read_restart role, src/io/field_io.cpp:229-250).  FP64 domains
+        continue bit for bit; FP32-mixed domains recompute their background
+        wind from the loaded state, as every set_state does."""
+        st, t, s = restart_read(path, self.case)
+        self.set_state(st)
+        return t, s
 
     def energy_budget(self) -> np.ndarray:
         out = np.zeros(6)
