@@ -30,6 +30,7 @@ extern "C" {
 #define VIBE_MI_ERR_HIP 100       /* a HIP runtime call failed            */
 #define VIBE_MI_ERR_NO_DEVICE 101 /* no usable GPU is visible             */
 #define VIBE_MI_ERR_NONFINITE 102 /* the model state became non-finite    */
+#define VIBE_MI_ERR_PHYSICS 103   /* a physics callback returned non-zero */
 
 /* ---- reference entry points (same names / signatures) ------------------- */
 

@@ -82,6 +82,41 @@ int vibe_mi_domain_slow_tendency(vibe_mi_domain *d, double *ru, double *rv, doub
                                  double *rrho, double *rtheta, double *rpi);
 /* One acoustic substep in place (runge_kutta.cpp:65-75 role). */
 int vibe_mi_domain_substep(vibe_mi_domain *d, double dtau);
+
+/* Physics coupling: rates added to the slow tendency R of every RK stage,
+ * before the substeps (RungeKutta3Integrator::slow_tendency with a
+ * PhysicsDriver, src/time/runge_kutta.cpp:46-63; PhysicsTendency,
+ * include/vibe/dyn/tendency.hpp:65-80): theta' in K/s, u and v in m/s^2,
+ * FP64, in the state layout ([nz][ny][nx]; u at west faces, v at south
+ * faces).  The state is dry, so there are no moisture rates.  The heating
+ * enters R_theta before R_pi' is formed from R_rho' and R_theta (the
+ * linearised equation of state), so pi' follows it.  Steps, batched steps
+ * and vibe_mi_domain_slow_tendency all add the rates.
+ *
+ * set_physics_tendency holds the given rates (copied to the device) for
+ * every following stage; NULL for a field means zero; all three NULL
+ * switches the coupling off. */
+int vibe_mi_domain_set_physics_tendency(vibe_mi_domain *d, const double *theta_rate, const double *u_rate,
+                                        const double *v_rate);
+/* The device rate buffers (nx*ny*nz FP64 each; allocated and zeroed on the
+ * first call, which switches the coupling on): a GPU physics package writes
+ * them on vibe_mi_domain_stream(d) between steps. */
+int vibe_mi_domain_physics_buffers(vibe_mi_domain *d, double **theta_rate, double **u_rate, double **v_rate);
+/* Per-stage physics, the reference's call pattern (physics evaluated on each
+ * stage's state: slow_tendency at runge_kutta.cpp:88).  Before the
+ * slow tendency of each stage the step zeroes the rate buffers and calls
+ * fn(user, stage 0..2, dt of the step, stream, stage state, rates).
+ * host = 1: state and rates are host arrays (the step synchronises, copies
+ * the stage state to the host and uploads the rates after fn returns);
+ * host = 0: device pointers, and fn enqueues its work on `stream`.  A
+ * non-zero return from fn ends the step with VIBE_MI_ERR_PHYSICS.  fn = NULL
+ * removes the callback and switches the coupling off.  Single-domain steps
+ * only: vibe_mi_batch_step returns VIBE_C_ERR_BAD_ARG for a member with a
+ * callback (a batched step holds the rates of set_physics_tendency). */
+typedef int (*vibe_mi_physics_fn)(void *user, int stage, double dt, void *stream, const double *u, const double *v,
+                                  const double *w, const double *rho_p, const double *theta_p, const double *pi_p,
+                                  double *theta_rate, double *u_rate, double *v_rate);
+int vibe_mi_domain_set_physics_callback(vibe_mi_domain *d, vibe_mi_physics_fn fn, void *user, int host);
 /* Compensated FP64 domain totals {mass, perturbation mass, kinetic,
  * internal, potential, total} (diagnostics.cpp:255-277). */
 int vibe_mi_domain_energy(vibe_mi_domain *d, double out[6]);

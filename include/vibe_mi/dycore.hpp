@@ -6,6 +6,9 @@
 //   vibe::timeint::RungeKutta3Integrator  include/vibe/time/runge_kutta.hpp:46-81
 //   vibe::dyn::energy_budget            src/dyn/diagnostics.cpp:255-277
 //   vibe::NumericalError                include/vibe/common/error.hpp:35
+//   Integrator::set_physics + RungeKutta3Integrator::slow_tendency's physics
+//   coupling                            include/vibe/time/integrator.hpp:74,
+//                                       src/time/runge_kutta.cpp:46-63
 //   vibe::io::RestartInfo, write_restart, read_restart
 //                                       include/vibe/io/field_io.hpp:82-94
 //   vibe::IoError                       include/vibe/common/error.hpp:41
@@ -54,6 +57,7 @@ class IoError : public std::runtime_error {
 
 inline void check(int rc, const char *what) {
   if (rc == VIBE_MI_ERR_NONFINITE) throw NumericalError(std::string(what) + ": non-finite model state");
+  if (rc == VIBE_MI_ERR_PHYSICS) throw std::runtime_error(std::string(what) + ": the physics callback failed");
   if (rc == VIBE_MI_ERR_IO || rc == VIBE_MI_ERR_FORMAT || rc == VIBE_MI_ERR_NOT_FOUND) throw IoError(what, rc);
   if (rc != 0) throw DeviceError(what, rc);
 }
@@ -176,6 +180,22 @@ class RungeKutta3Integrator final : public Integrator {
     return {c, w};
   }
   State make_state() const { return State(nx_, ny_, nz_); }
+
+  /// Physics rates held for every following stage: theta' (K/s), u and v
+  /// (m/s^2) in the state layout; empty vectors mean zero, all empty
+  /// switches the coupling off.
+  void set_physics_tendency(const std::vector<double> &theta, const std::vector<double> &u,
+                            const std::vector<double> &v) {
+    auto p = [](const std::vector<double> &x) { return x.empty() ? nullptr : x.data(); };
+    check(vibe_mi_domain_set_physics_tendency(h_, p(theta), p(u), p(v)), "vibe_mi_domain_set_physics_tendency");
+  }
+  /// Per-stage physics (Integrator::set_physics role): fn is called before
+  /// each stage's slow tendency with the stage state and zeroed rate arrays
+  /// (host copies when host is true, else device pointers on the domain's
+  /// stream); nullptr removes it.
+  void set_physics(vibe_mi_physics_fn fn, void *user, bool host = true) {
+    check(vibe_mi_domain_set_physics_callback(h_, fn, user, host ? 1 : 0), "vibe_mi_domain_set_physics_callback");
+  }
 
  private:
   vibe_mi_domain *h_ = nullptr;

@@ -56,6 +56,11 @@ struct orc_dycore {
   double *metx, *mety;     /* nxy: sx / Du, sy / Dv */
   double *rdzeta, *rdzw;   /* 1/dzeta (nz), 1/dzw (nz+1, 1..nz-1) */
   double rdx, rdy;
+  /* physics coupling (runge_kutta.cpp:46-63): rates added to R each stage */
+  double *ph_th, *ph_u, *ph_v;
+  int ph_on;
+  orc_physics_fn ph_fn;
+  void *ph_user;
   double *kap;    /* ncell: (Rd/cv) pi0 / rho0 */
   double *lam;    /* ncell: (Rd/cv) pi0 / theta0 */
   /* scratch */
@@ -156,7 +161,40 @@ void orc_dycore_destroy(orc_dycore *dc) {
   free(dc->rDc); free(dc->rDu); free(dc->rDv); free(dc->metx); free(dc->mety);
   free(dc->rdzeta); free(dc->rdzw);
   free(dc->kap); free(dc->lam); free(dc->om); free(dc->mc); free(dc->buf);
+  free(dc->ph_th); free(dc->ph_u); free(dc->ph_v);
   free(dc);
+}
+
+static void ensure_physics(orc_dycore *dc) {
+  if (!dc->ph_th) {
+    dc->ph_th = (double *)calloc(dc->ncell, sizeof(double));
+    dc->ph_u = (double *)calloc(dc->ncell, sizeof(double));
+    dc->ph_v = (double *)calloc(dc->ncell, sizeof(double));
+  }
+}
+
+void orc_dycore_set_physics(orc_dycore *dc, const double *theta_rate, const double *u_rate, const double *v_rate) {
+  if (!theta_rate && !u_rate && !v_rate) {
+    dc->ph_on = 0;
+    return;
+  }
+  ensure_physics(dc);
+  const double *src[3] = {theta_rate, u_rate, v_rate};
+  double *dst[3] = {dc->ph_th, dc->ph_u, dc->ph_v};
+  for (int q = 0; q < 3; ++q) {
+    if (src[q])
+      memcpy(dst[q], src[q], dc->ncell * sizeof(double));
+    else
+      memset(dst[q], 0, dc->ncell * sizeof(double));
+  }
+  dc->ph_on = 1;
+}
+
+void orc_dycore_set_physics_callback(orc_dycore *dc, orc_physics_fn fn, void *user) {
+  dc->ph_fn = fn;
+  dc->ph_user = user;
+  dc->ph_on = fn != NULL;
+  if (fn) ensure_physics(dc);
 }
 
 /* ---- stencil primitives ------------------------------------------------ */
@@ -314,7 +352,9 @@ void orc_dycore_slow_tendency(orc_dycore *dc, const orc_state *s, orc_state *R) 
                            uc * ((uxc[1] - uxc[0]) * rdx + (vyc[1] - vyc[0]) * rdy +
                                  (ozc[1] - ozc[0]) * rdzeta[k]);
         const double thu = 0.5 * (th[IX(i - 1, j, k)] + th[IX(i, j, k)]);
-        R->u[IX(i, j, k)] = adv * rDu[HX(i, j)] - CP_ * thu * px_op(dc, pi, i, j, k);
+        double ru = adv * rDu[HX(i, j)] - CP_ * thu * px_op(dc, pi, i, j, k);
+        if (dc->ph_on) ru += dc->ph_u[IX(i, j, k)]; /* physics rate (runge_kutta.cpp:60) */
+        R->u[IX(i, j, k)] = ru;
 #undef U_
       }
 
@@ -352,7 +392,9 @@ void orc_dycore_slow_tendency(orc_dycore *dc, const orc_state *s, orc_state *R) 
                            vc * ((uxc[1] - uxc[0]) * rdx + (vyc[1] - vyc[0]) * rdy +
                                  (ozc[1] - ozc[0]) * rdzeta[k]);
         const double thv = 0.5 * (th[IX(i, j - 1, k)] + th[IX(i, j, k)]);
-        R->v[IX(i, j, k)] = adv * rDv[HX(i, j)] - CP_ * thv * py_op(dc, pi, i, j, k);
+        double rv = adv * rDv[HX(i, j)] - CP_ * thv * py_op(dc, pi, i, j, k);
+        if (dc->ph_on) rv += dc->ph_v[IX(i, j, k)]; /* runge_kutta.cpp:61 */
+        R->v[IX(i, j, k)] = rv;
 #undef V_
       }
 
@@ -435,8 +477,10 @@ void orc_dycore_slow_tendency(orc_dycore *dc, const orc_state *s, orc_state *R) 
         const double rr = -((frx[1] - frx[0]) * rdx + (fry[1] - fry[0]) * rdy + (frz[1] - frz[0]) * rdzeta[k]) * rD;
         const double div = (mfx[1] - mfx[0]) * rdx + (mfy[1] - mfy[0]) * rdy + (mfz[1] - mfz[0]) * rdzeta[k];
         const double fd = (ftx[1] - ftx[0]) * rdx + (fty[1] - fty[0]) * rdy + (ftz[1] - ftz[0]) * rdzeta[k];
-        const double rt = -(fd - th[c] * div) * rD -
-                          0.5 * (w[IX(i, j, k)] * gam(dc, i, j, k) + w[IX(i, j, k + 1)] * gam(dc, i, j, k + 1));
+        double rt = -(fd - th[c] * div) * rD -
+                    0.5 * (w[IX(i, j, k)] * gam(dc, i, j, k) + w[IX(i, j, k + 1)] * gam(dc, i, j, k + 1));
+        /* heating (runge_kutta.cpp:53) enters before R_pi' is formed */
+        if (dc->ph_on) rt += dc->ph_th[c];
         R->rho[c] = rr;
         R->theta[c] = rt;
         R->pi[c] = dc->kap[c] * rr + dc->lam[c] * rt;
@@ -600,6 +644,12 @@ int orc_dycore_step(orc_dycore *dc, orc_state *s, double dt, int nsub) {
   const double dtk[3] = {dt / 3.0, dt / 2.0, dt};
   copy_state(dc, &dc->s0, s);
   for (int st = 0; st < 3; ++st) {
+    if (dc->ph_fn) { /* physics on the stage state, fresh zero rates (runge_kutta.cpp:48-51) */
+      memset(dc->ph_th, 0, dc->ncell * sizeof(double));
+      memset(dc->ph_u, 0, dc->ncell * sizeof(double));
+      memset(dc->ph_v, 0, dc->ncell * sizeof(double));
+      if (dc->ph_fn(dc->ph_user, st, dt, s, dc->ph_th, dc->ph_u, dc->ph_v)) return 103;
+    }
     orc_dycore_slow_tendency(dc, s, &dc->R);
     const double h = dtk[st];
     const orc_state *s0 = &dc->s0, *R = &dc->R;

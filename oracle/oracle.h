@@ -71,6 +71,15 @@ void orc_dycore_sponge(orc_dycore *dc, orc_state *s, double dt);
 /* Compensated domain totals: out = {mass, perturbation mass, kinetic,
  * internal, potential, kinetic+internal+potential}. */
 void orc_dycore_energy(orc_dycore *dc, const orc_state *s, double out[6]);
+/* Physics coupling (runge_kutta.cpp:46-63): theta' (K/s), u, v (m/s^2)
+ * rates added to the slow tendency of every stage; theta's enters before
+ * R_pi' is formed.  set_physics copies fixed rates (all NULL: off); a
+ * callback is called before each stage's slow tendency with the stage state
+ * and zeroed rate arrays to fill (non-zero return: the step returns 103). */
+typedef int (*orc_physics_fn)(void *user, int stage, double dt, const orc_state *s, double *theta_rate,
+                              double *u_rate, double *v_rate);
+void orc_dycore_set_physics(orc_dycore *dc, const double *theta_rate, const double *u_rate, const double *v_rate);
+void orc_dycore_set_physics_callback(orc_dycore *dc, orc_physics_fn fn, void *user);
 void orc_set_threads(int n);
 /* Face primitives of the dycore (dycore.c), exported for tests:
  * orc_weno_face   WENO5 value at the face right of c, one-division form

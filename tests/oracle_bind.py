@@ -32,6 +32,10 @@ class OrcState(ctypes.Structure):
     _fields_ = [("u", _PD), ("v", _PD), ("w", _PD), ("rho", _PD), ("theta", _PD), ("pi", _PD)]
 
 
+ORC_PHYSICS_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                  ctypes.POINTER(OrcState), _PD, _PD, _PD)
+
+
 def _pd(a):
     if a is None:
         return None
@@ -66,6 +70,10 @@ def oracle_lib():
         L.orc_dycore_destroy.argtypes = [_VP]
         L.orc_dycore_step.restype = _I
         L.orc_dycore_step.argtypes = [_VP, ctypes.POINTER(OrcState), _D, _I]
+        L.orc_dycore_set_physics.restype = None
+        L.orc_dycore_set_physics.argtypes = [_VP, _PD, _PD, _PD]
+        L.orc_dycore_set_physics_callback.restype = None
+        L.orc_dycore_set_physics_callback.argtypes = [_VP, ORC_PHYSICS_FN, _VP]
         L.orc_dycore_slow_tendency.restype = None
         L.orc_dycore_slow_tendency.argtypes = [_VP, ctypes.POINTER(OrcState), ctypes.POINTER(OrcState)]
         L.orc_dycore_substep.restype = None
@@ -166,6 +174,40 @@ class OracleDycore:
         out = np.zeros(6)
         oracle_lib().orc_dycore_energy(self.h, ctypes.byref(self._st(self.state)), _pd(out))
         return out
+
+    def set_physics(self, theta=None, u=None, v=None):
+        """Fixed physics rates (orc_dycore_set_physics); all None: off."""
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (theta, u, v)]
+        self._phys_keep = arrs
+        oracle_lib().orc_dycore_set_physics(self.h, *[_pd(a) for a in arrs])
+
+    def set_physics_callback(self, fn):
+        """fn(stage, dt, state dict, rates dict) before each stage's slow
+        tendency (orc_dycore_set_physics_callback); None removes it."""
+        L = oracle_lib()
+        if fn is None:
+            self._phys_cb = None
+            L.orc_dycore_set_physics_callback(self.h, ORC_PHYSICS_FN(), None)
+            return
+        c = self.case
+        cell = c.nx * c.ny * c.nz
+        nwf = c.nx * c.ny * (c.nz + 1)
+
+        def arr(p, n):
+            return np.ctypeslib.as_array(ctypes.cast(p, _PD), shape=(n,))
+
+        def tramp(_user, stage, dt, sp, rt, ru, rv):
+            try:
+                s = sp.contents
+                st = {"u": arr(s.u, cell), "v": arr(s.v, cell), "w": arr(s.w, nwf), "rho": arr(s.rho, cell),
+                      "theta": arr(s.theta, cell), "pi": arr(s.pi, cell)}
+                rates = {"theta": arr(rt, cell), "u": arr(ru, cell), "v": arr(rv, cell)}
+                return int(fn(stage, dt, st, rates) or 0)
+            except Exception:
+                return 1
+
+        self._phys_cb = ORC_PHYSICS_FN(tramp)
+        L.orc_dycore_set_physics_callback(self.h, self._phys_cb, None)
 
 
 FIELDS = ("u", "v", "w", "rho", "theta", "pi")

@@ -78,6 +78,7 @@ struct vibe_mi_batch {
   int ty = 4;  // rows per stage workgroup
   int cus = 256;
   int gm = kGeoFlat;  // geometry mode of the batch: the most general of its members
+  bool phys = false;  // an active member has physics rates: the stage kernels add them
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   // per-group kernel times of the last timed eager call (ms, summed)
   bool group_timing = false;
@@ -206,6 +207,7 @@ int upload_desc(vibe_mi_batch *b, const double *dt, int nsub) {
     x.ao = static_cast<T>(0.5 * (1.0 - d->eps_s));
     x.dt = dt[m];
     x.sponge = d->sponge;
+    x.ph = d->ph_on ? PhysArgs{d->ph_th, d->ph_u, d->ph_v} : PhysArgs{};
     x.kmin_sponge = d->kmin_sponge;
     x.cur0 = d->cur0;
     x.flag = b->flag;
@@ -279,11 +281,11 @@ int launch_substep(vibe_mi_batch *b, int nw, const WorkList &w, int st, int q, i
 
 // ain < 0: the stage reads the FP64 current state; otherwise (FP32 batches)
 // the FP32 acoustic set A[ain].  The stage writes A[aout].
-template <typename T, int TY, int GM, bool CUR32>
+template <typename T, int TY, int GM, bool CUR32, bool PH>
 int launch_stage_k(vibe_mi_batch *b, const WorkList &w, int st, int ain, int aout, int *ctr) {
   const auto *D = static_cast<const BDom<T> *>(b->desc);
   const int li = TY == 2 ? 0 : (TY == 4 ? 1 : 2);
-  auto *kern = &kb_stage<T, TY, GM, CUR32>;
+  auto *kern = &kb_stage<T, TY, GM, CUR32, PH>;
   unsigned grid = 0;
   if (int rc = resident(b, kern, 64 * TY, 0, w.stage[li].ntiles, &grid)) return rc;
   hipLaunchKernelGGL(kern, dim3(grid), dim3(64, TY), 0, b->stream, D, w.stage[li], st, ain, aout, ctr);
@@ -291,12 +293,18 @@ int launch_stage_k(vibe_mi_batch *b, const WorkList &w, int st, int ain, int aou
   return 0;
 }
 
+template <typename T, int TY, int GM, bool PH>
+int launch_stage_ph(vibe_mi_batch *b, const WorkList &w, int st, int ain, int aout, int *ctr) {
+  if constexpr (sizeof(T) == 4) {
+    if (ain >= 0) return launch_stage_k<T, TY, GM, true, PH>(b, w, st, ain, aout, ctr);
+  }
+  return launch_stage_k<T, TY, GM, false, PH>(b, w, st, ain, aout, ctr);
+}
+
 template <typename T, int TY, int GM>
 int launch_stage_ty(vibe_mi_batch *b, const WorkList &w, int st, int ain, int aout, int *ctr) {
-  if constexpr (sizeof(T) == 4) {
-    if (ain >= 0) return launch_stage_k<T, TY, GM, true>(b, w, st, ain, aout, ctr);
-  }
-  return launch_stage_k<T, TY, GM, false>(b, w, st, ain, aout, ctr);
+  if (b->phys) return launch_stage_ph<T, TY, GM, true>(b, w, st, ain, aout, ctr);
+  return launch_stage_ph<T, TY, GM, false>(b, w, st, ain, aout, ctr);
 }
 
 template <typename T, int GM>
@@ -471,7 +479,7 @@ int step_batch(vibe_mi_batch *b, const double *dt, int nsub, int nsteps, const u
   if (active)
     for (int m = 0; m < n; ++m) act[m] = active[m] ? 1 : 0;
   for (int m = 0; m < n; ++m)
-    if (act[m] && !(dt[m] > 0.0)) return VIBE_C_ERR_BAD_ARG;
+    if (act[m] && (!(dt[m] > 0.0) || b->doms[m]->ph_fn != nullptr)) return VIBE_C_ERR_BAD_ARG;
   WorkList *w = nullptr;
   if (int rc = work_list(b, act, &w)) return rc;
   if (w->nids == 0 || nsteps == 0) {
@@ -479,6 +487,9 @@ int step_batch(vibe_mi_batch *b, const double *dt, int nsub, int nsteps, const u
     return 0;
   }
   if (int rc = upload_desc<T>(b, dt, nsub)) return rc;
+  b->phys = false;
+  for (int m = 0; m < n; ++m)
+    if (act[m] && b->doms[m]->ph_on) b->phys = true;
   if (int rc = autotune<T>(b, *w)) return rc;
   if constexpr (sizeof(T) == 4)
     if (int rc = launch_substep<T, double>(b, b->nw, *w, 0, 0, 0, 1, b->ctr, true)) return rc;
@@ -486,7 +497,7 @@ int step_batch(vibe_mi_batch *b, const double *dt, int nsub, int nsteps, const u
   VMI_HIP(hipMemsetAsync(b->flag, 0, sizeof(int), b->stream));
   hipGraphExec_t exec = nullptr;
   if (b->use_graph) {
-    const std::string gk = key_of(act) + "|" + std::to_string(nsub);
+    const std::string gk = key_of(act) + "|" + std::to_string(nsub) + (b->phys ? "|phys" : "");
     auto it = b->graphs.find(gk);
     if (it == b->graphs.end()) {
       hipGraph_t graph = nullptr;

@@ -22,6 +22,7 @@ struct BDom {
   T an, ao;     // HEVI off-centring weights
   double dt;
   const double *sponge;
+  PhysArgs ph;  // physics rates added to every stage's slow tendency (th null: none)
   int kmin_sponge;
   int cur0;  // rotation index, advanced on the device by kb_advance
   int *flag;
@@ -91,8 +92,9 @@ __global__ void __launch_bounds__(64) kb_advance(BDom<T> *__restrict__ D, const 
 // A[aout] and theta' to the stage output.  CUR32: the current state's
 // u, v, w, rho', pi' come from the FP32 acoustic set A[ain] the previous
 // stage's substeps ended in (FP32-mixed stages 2 and 3), theta' from the
-// FP64 stage output; otherwise all from the FP64 state.
-template <typename T, int TY, int GM, bool CUR32>
+// FP64 stage output; otherwise all from the FP64 state.  PH: the variant
+// with the physics coupling, launched when a member of the batch has rates.
+template <typename T, int TY, int GM, bool CUR32, bool PH>
 __global__ void __launch_bounds__(64 * TY) kb_stage(const BDom<T> *__restrict__ D, TileList L, int st, int ain,
                                                     int aout, int *__restrict__ ctr) {
   tile_loop(ctr, L, [&](int t) {
@@ -109,10 +111,10 @@ __global__ void __launch_bounds__(64 * TY) kb_stage(const BDom<T> *__restrict__ 
     if constexpr (CUR32) {
       const AcousticSet<T> ac = dd.A[ain];
       const StageCur<T> ci{ac.u, ac.v, ac.w, ac.rho, ac.pi, cur.th};
-      stage_tile<T, T, TY, GM, T>(g, ci, si, dd.h[st], so, dd.flag, tl.y, tl.z, sp);
+      stage_tile<T, T, TY, GM, T, PH>(g, ci, si, dd.h[st], so, dd.flag, tl.y, tl.z, sp, dd.ph);
     } else {
       const StageCur<double> ci{cur.u, cur.v, cur.w, cur.rho, cur.pi, cur.th};
-      stage_tile<T, T, TY, GM>(g, ci, si, dd.h[st], so, dd.flag, tl.y, tl.z, sp);
+      stage_tile<T, T, TY, GM, double, PH>(g, ci, si, dd.h[st], so, dd.flag, tl.y, tl.z, sp, dd.ph);
     }
   });
 }
@@ -187,15 +189,18 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kS
   X template __global__ void vibe_mi::kb_substep<float, O, 4, 16, GM>(VMI_KB_SUBSTEP_ARGS);                    \
   X template __global__ void vibe_mi::kb_substep<float, O, 4, 24, GM>(VMI_KB_SUBSTEP_ARGS);                    \
   X template __global__ void vibe_mi::kb_substep<float, O, 4, 32, GM>(VMI_KB_SUBSTEP_ARGS);
+#define VMI_BATCH_F32_STAGE_GM(X, GM, PH)                                                                     \
+  X template __global__ void vibe_mi::kb_stage<float, 2, GM, false, PH>(VMI_KB_STAGE_ARGS);                    \
+  X template __global__ void vibe_mi::kb_stage<float, 4, GM, false, PH>(VMI_KB_STAGE_ARGS);                    \
+  X template __global__ void vibe_mi::kb_stage<float, 8, GM, false, PH>(VMI_KB_STAGE_ARGS);                    \
+  X template __global__ void vibe_mi::kb_stage<float, 2, GM, true, PH>(VMI_KB_STAGE_ARGS);                     \
+  X template __global__ void vibe_mi::kb_stage<float, 4, GM, true, PH>(VMI_KB_STAGE_ARGS);                     \
+  X template __global__ void vibe_mi::kb_stage<float, 8, GM, true, PH>(VMI_KB_STAGE_ARGS);
 #define VMI_BATCH_F32_GM(X, GM)                                                                                 \
   VMI_BATCH_F32_SUBSTEP_GM(X, float, GM)                                                                        \
   VMI_BATCH_F32_SUBSTEP_GM(X, double, GM)                                                                       \
-  X template __global__ void vibe_mi::kb_stage<float, 2, GM, false>(VMI_KB_STAGE_ARGS);                        \
-  X template __global__ void vibe_mi::kb_stage<float, 4, GM, false>(VMI_KB_STAGE_ARGS);                        \
-  X template __global__ void vibe_mi::kb_stage<float, 8, GM, false>(VMI_KB_STAGE_ARGS);                        \
-  X template __global__ void vibe_mi::kb_stage<float, 2, GM, true>(VMI_KB_STAGE_ARGS);                         \
-  X template __global__ void vibe_mi::kb_stage<float, 4, GM, true>(VMI_KB_STAGE_ARGS);                         \
-  X template __global__ void vibe_mi::kb_stage<float, 8, GM, true>(VMI_KB_STAGE_ARGS);
+  VMI_BATCH_F32_STAGE_GM(X, GM, false)                                                                          \
+  VMI_BATCH_F32_STAGE_GM(X, GM, true)
 #define VMI_KB_SUBSTEP_ARGS const vibe_mi::BDom<float> *__restrict__, vibe_mi::TileList, int, int, int, int, int *__restrict__
 #define VMI_KB_STAGE_ARGS const vibe_mi::BDom<float> *__restrict__, vibe_mi::TileList, int, int, int, int *__restrict__
 #define VMI_BATCH_F32_ALL(X) \

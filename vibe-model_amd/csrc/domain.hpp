@@ -55,6 +55,13 @@ struct vibe_mi_domain {
   int nw_fp32 = 4, nw_fp64 = 8;  // waves per substep workgroup (autotuned)
   bool tuned[2] = {false, false};
   int *flag = nullptr;
+  // physics coupling: device rate buffers (theta', u, v; allocated on first
+  // use), whether the stages add them, and the optional per-stage callback
+  double *ph_th = nullptr, *ph_u = nullptr, *ph_v = nullptr;
+  bool ph_on = false;
+  vibe_mi_physics_fn ph_fn = nullptr;
+  void *ph_user = nullptr;
+  int ph_host = 0;
   vibe_mi::NSum *part = nullptr;
   double *dres = nullptr;
   unsigned long long *dmax = nullptr;

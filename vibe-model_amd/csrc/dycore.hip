@@ -177,10 +177,57 @@ int run_stage(vibe_mi_domain *d, const Geo<T> &g, const FieldSet &cur, const Fie
   const StageIn ci{cur.u, cur.v, cur.w, cur.rho, cur.th, cur.pi};
   const StageIn si{s0.u, s0.v, s0.w, s0.rho, s0.th, s0.pi};
   const StageOut<T> so{a.u, a.v, a.w, a.rho, a.pi, nxt.th};
-  hipLaunchKernelGGL((k_stage_col<T, T, 4>), dim3(ceil_div(d->nx, 64), ceil_div(d->ny, 4)), dim3(64, 4), 0, d->stream,
-                     g, ci, si, h, so, d->flag);
+  const dim3 grid(ceil_div(d->nx, 64), ceil_div(d->ny, 4)), block(64, 4);
+  if (d->ph_on)
+    hipLaunchKernelGGL((k_stage_col<T, T, 4, true>), grid, block, 0, d->stream, g, ci, si, h, so, d->flag,
+                       PhysArgs{d->ph_th, d->ph_u, d->ph_v});
+  else
+    hipLaunchKernelGGL((k_stage_col<T, T, 4, false>), grid, block, 0, d->stream, g, ci, si, h, so, d->flag,
+                       PhysArgs{});
   VMI_HIP(hipGetLastError());
   return 0;
+}
+
+// Allocates (once) and zeroes the physics rate buffers.
+int ensure_physics(vibe_mi_domain *d) {
+  if (d->ph_th == nullptr) {
+    if (int rc = dalloc(d, &d->ph_th, d->ncell)) return rc;
+    if (int rc = dalloc(d, &d->ph_u, d->ncell)) return rc;
+    if (int rc = dalloc(d, &d->ph_v, d->ncell)) return rc;
+    for (double *p : {d->ph_th, d->ph_u, d->ph_v}) VMI_HIP(hipMemsetAsync(p, 0, d->ncell * 8, d->stream));
+    VMI_HIP(hipStreamSynchronize(d->stream));
+  }
+  return 0;
+}
+
+// The per-stage physics call (runge_kutta.cpp:46-63 role): zero the rates,
+// hand fn the stage state (host copies or device pointers), keep its rates.
+int call_physics(vibe_mi_domain *d, int stage, double dt, const FieldSet &s) {
+  const size_t nc = d->ncell * 8;
+  for (double *p : {d->ph_th, d->ph_u, d->ph_v}) VMI_HIP(hipMemsetAsync(p, 0, nc, d->stream));
+  int rc = 0;
+  if (d->ph_host) {
+    std::vector<double> h(5 * d->ncell + d->nwf), r(3 * d->ncell, 0.0);
+    double *hu = h.data(), *hv = hu + d->ncell, *hr = hv + d->ncell, *ht = hr + d->ncell, *hp = ht + d->ncell,
+           *hw = hp + d->ncell;
+    VMI_HIP(hipStreamSynchronize(d->stream));
+    VMI_HIP(hipMemcpy(hu, s.u, nc, hipMemcpyDeviceToHost));
+    VMI_HIP(hipMemcpy(hv, s.v, nc, hipMemcpyDeviceToHost));
+    VMI_HIP(hipMemcpy(hw, s.w, d->nwf * 8, hipMemcpyDeviceToHost));
+    VMI_HIP(hipMemcpy(hr, s.rho, nc, hipMemcpyDeviceToHost));
+    VMI_HIP(hipMemcpy(ht, s.th, nc, hipMemcpyDeviceToHost));
+    VMI_HIP(hipMemcpy(hp, s.pi, nc, hipMemcpyDeviceToHost));
+    double *rt = r.data(), *ru = rt + d->ncell, *rv = ru + d->ncell;
+    rc = d->ph_fn(d->ph_user, stage, dt, d->stream, hu, hv, hw, hr, ht, hp, rt, ru, rv);
+    if (rc == 0) {
+      VMI_HIP(hipMemcpy(d->ph_th, rt, nc, hipMemcpyHostToDevice));
+      VMI_HIP(hipMemcpy(d->ph_u, ru, nc, hipMemcpyHostToDevice));
+      VMI_HIP(hipMemcpy(d->ph_v, rv, nc, hipMemcpyHostToDevice));
+    }
+  } else {
+    rc = d->ph_fn(d->ph_user, stage, dt, d->stream, s.u, s.v, s.w, s.rho, s.th, s.pi, d->ph_th, d->ph_u, d->ph_v);
+  }
+  return rc == 0 ? 0 : VIBE_MI_ERR_PHYSICS;
 }
 
 template <typename T>
@@ -356,6 +403,8 @@ int step_impl(vibe_mi_domain *d, const Geo<T> &g, AcousticSet<T> *A, double dt, 
   int cur = i0;
   for (int st = 0; st < 3; ++st) {
     FieldSet &nxt = d->S[outs[st]];
+    if (d->ph_fn != nullptr)
+      if (int rc = call_physics(d, st, dt, d->S[cur])) return rc;
     if (timed) VMI_HIP(hipEventRecord(d->ev[0], d->stream));
     if (int rc = run_stage<T>(d, g, d->S[cur], s0, hk[st], A[0], nxt)) return rc;
     if (timed) {
@@ -708,6 +757,57 @@ int vibe_mi_domain_slow_tendency(vibe_mi_domain *d, double *ru, double *rv, doub
   VMI_HIP(hipMemcpyAsync(rtheta, o.th, nc, hipMemcpyDeviceToHost, d->stream));
   VMI_HIP(hipMemcpyAsync(rpi, o.pi, nc, hipMemcpyDeviceToHost, d->stream));
   VMI_HIP(hipStreamSynchronize(d->stream));
+  return 0;
+}
+
+int vibe_mi_domain_set_physics_tendency(vibe_mi_domain *d, const double *theta_rate, const double *u_rate,
+                                        const double *v_rate) {
+  if (d == nullptr) return VIBE_C_ERR_BAD_ARG;
+  if (theta_rate == nullptr && u_rate == nullptr && v_rate == nullptr) {
+    d->ph_on = false;
+    return 0;
+  }
+  if (int rc = set_device(d)) return rc;
+  if (int rc = ensure_physics(d)) return rc;
+  const size_t nc = d->ncell * 8;
+  const double *src[3] = {theta_rate, u_rate, v_rate};
+  double *dst[3] = {d->ph_th, d->ph_u, d->ph_v};
+  for (int q = 0; q < 3; ++q) {
+    if (src[q] != nullptr)
+      VMI_HIP(hipMemcpyAsync(dst[q], src[q], nc, hipMemcpyHostToDevice, d->stream));
+    else
+      VMI_HIP(hipMemsetAsync(dst[q], 0, nc, d->stream));
+  }
+  VMI_HIP(hipStreamSynchronize(d->stream));
+  d->ph_on = true;
+  return 0;
+}
+
+int vibe_mi_domain_physics_buffers(vibe_mi_domain *d, double **theta_rate, double **u_rate, double **v_rate) {
+  if (d == nullptr || theta_rate == nullptr || u_rate == nullptr || v_rate == nullptr) return VIBE_C_ERR_BAD_ARG;
+  if (int rc = set_device(d)) return rc;
+  if (int rc = ensure_physics(d)) return rc;
+  *theta_rate = d->ph_th;
+  *u_rate = d->ph_u;
+  *v_rate = d->ph_v;
+  d->ph_on = true;
+  return 0;
+}
+
+int vibe_mi_domain_set_physics_callback(vibe_mi_domain *d, vibe_mi_physics_fn fn, void *user, int host) {
+  if (d == nullptr) return VIBE_C_ERR_BAD_ARG;
+  if (fn == nullptr) {
+    d->ph_fn = nullptr;
+    d->ph_user = nullptr;
+    d->ph_on = false;
+    return 0;
+  }
+  if (int rc = set_device(d)) return rc;
+  if (int rc = ensure_physics(d)) return rc;
+  d->ph_fn = fn;
+  d->ph_user = user;
+  d->ph_host = host ? 1 : 0;
+  d->ph_on = true;
   return 0;
 }
 

@@ -94,6 +94,15 @@ struct SpongeArgs {
   double dt = 0.0;
   int kmin = 0x7fffffff;
 };
+// Physics rates added to the slow tendency R of every stage before the
+// substeps (the coupling of src/time/runge_kutta.cpp:46-63): theta' in K/s,
+// u and v in m/s^2, FP64 in the state layout.  th == nullptr: no physics.
+// The heating enters R_theta before R_pi' is formed from R_rho' and R_theta
+// (the linearised equation of state), so pi' follows it.
+struct PhysArgs {
+  const double *th = nullptr, *u = nullptr, *v = nullptr;
+};
+
 __device__ __forceinline__ double sponge_scale(const SpongeArgs &sp, int k, int nz, double x) {
   if (k < sp.kmin) return x;
   const double a = C(sp.alpha)[k < nz - 1 ? k : nz - 1];
@@ -101,10 +110,12 @@ __device__ __forceinline__ double sponge_scale(const SpongeArgs &sp, int k, int 
   return f * x;
 }
 
-template <typename T, typename O, int TY, int GM, typename CI = double>
+// PH: the physics coupling is compiled in (a member without rates passes
+// th == nullptr); without it the stage kernel carries no trace of it.
+template <typename T, typename O, int TY, int GM, typename CI = double, bool PH = false>
 __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageCur<CI> &cur, const StageIn &s0, double h,
                                            const StageOut<O> &out, int *__restrict__ flag, int bx, int by,
-                                           const SpongeArgs &sp = SpongeArgs{}) {
+                                           const SpongeArgs &sp = SpongeArgs{}, const PhysArgs &ph = PhysArgs{}) {
   // levels k and k+1 of u, v, w, theta', rho' over the tile and a 3-cell
   // halo, in a ring of three LDS buffers (level k+2 is filled meanwhile);
   // level k+1 gives Omega at the top face of level k
@@ -353,7 +364,10 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageCur<CI> &
       T px = (pw[1] - pWw[1]) * g.rdx;
       if constexpr (xslope) px = px - (omz * mxC) * (T(0.5) * (GcW + GcC));
       const T R = adv * rDuC - T(kCp) * thu * px;
-      double un = S0[0] + h * static_cast<double>(R);
+      double Rd = static_cast<double>(R);
+      if constexpr (PH)
+        if (ph.th != nullptr) Rd = Rd + gld(ph.u, cCb + static_cast<uint32_t>(k) * lsD);
+      double un = S0[0] + h * Rd;
       if constexpr (sizeof(T) == 4) un = un - C(g.ubg64)[k];  // FP32 acoustic buffers hold u - ubg
       if (act) gst(out.u, ko, static_cast<O>(un));
     }
@@ -381,7 +395,10 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageCur<CI> &
       T py = (pw[1] - pSw[1]) * g.rdy;
       if constexpr (yslope) py = py - (omz * myC) * (T(0.5) * (GcS + GcC));
       const T R = adv * rDvC - T(kCp) * thv * py;
-      if (act) gst(out.v, ko, static_cast<O>(S0[1] + h * static_cast<double>(R)));
+      double Rd = static_cast<double>(R);
+      if constexpr (PH)
+        if (ph.th != nullptr) Rd = Rd + gld(ph.v, cCb + static_cast<uint32_t>(k) * lsD);
+      if (act) gst(out.v, ko, static_cast<O>(S0[1] + h * Rd));
     }
     // ---------------- w at interface k (1..nz-1) ----------------
     // flux of w through the cell centre below / above interface k
@@ -436,7 +453,9 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageCur<CI> &
       const T fd = (ftx1 - ftx0) * g.rdx + (fty1 - fty0) * g.rdy + (ftz1 - ftz0) * rdzk;
       // gamma(k+1) with the interface index clamped to 1..nz-1
       const T gam1 = (k + 1 <= nz - 1) ? (thp - th00) * rD * C(g.rdzw)[k + 1] : gamk;
-      const T rt = -(fd - tw[3] * div) * rD - T(0.5) * (ww[3] * gamk + ww[4] * gam1);
+      T rt = -(fd - tw[3] * div) * rD - T(0.5) * (ww[3] * gamk + ww[4] * gam1);
+      if constexpr (PH)
+        if (ph.th != nullptr) rt = rt + static_cast<T>(gld(ph.th, cCb + static_cast<uint32_t>(k) * lsD));
       const T rp = kapk * rr + lamk * rt;
       const double rn = S0[3] + h * static_cast<double>(rr);
       const double tn = S0[4] + h * static_cast<double>(rt);
@@ -494,10 +513,11 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageCur<CI> &
   if (bad) atomicOr(flag, 1);
 }
 
-template <typename T, typename O, int TY>
+template <typename T, typename O, int TY, bool PH>
 __global__ void __launch_bounds__(64 * TY) k_stage_col(Geo<T> g, StageIn cur, StageIn s0, double h, StageOut<O> out,
-                                                        int *__restrict__ flag) {
-  stage_tile<T, O, TY, kGeoAny>(g, stage_cur(cur), s0, h, out, flag, blockIdx.x, blockIdx.y);
+                                                        int *__restrict__ flag, PhysArgs ph) {
+  stage_tile<T, O, TY, kGeoAny, double, PH>(g, stage_cur(cur), s0, h, out, flag, blockIdx.x, blockIdx.y,
+                                            SpongeArgs{}, ph);
 }
 
 // ---- HEVI factorisation table ----------------------------------------------

@@ -5,6 +5,8 @@
 // .vibebin restart (write_restart role, src/io/field_io.cpp:207-227).
 //
 //   run_warm_bubble [nx ny nz steps [restart.vibebin]]   (defaults 64 64 40 150)
+//   RELAX=1 adds a Newtonian relaxation of theta' through the per-stage
+//   physics callback.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -50,6 +52,17 @@ int main(int argc, char **argv) {
                                        s.pi.data()),
                  "balance");
   integ.upload(s);
+  // a weak Newtonian relaxation of theta' (10-minute time scale) evaluated
+  // on every stage state: the per-stage physics coupling
+  const vibe_mi_physics_fn relax = [](void *user, int, double, void *, const double *, const double *,
+                                      const double *, const double *, const double *theta_p, const double *,
+                                      double *theta_rate, double *, double *) -> int {
+    const std::size_t n = *static_cast<const std::size_t *>(user);
+    for (std::size_t c = 0; c < n; ++c) theta_rate[c] = -theta_p[c] / 600.0;
+    return 0;
+  };
+  std::size_t ncell_user = ncell;
+  if (std::getenv("RELAX") != nullptr) integ.set_physics(relax, &ncell_user, true);
   const auto e0 = integ.energy_budget();
   std::printf("initial: mass %.12e kg  total energy %.12e J\n", e0.mass, e0.total);
   double t = 0.0;

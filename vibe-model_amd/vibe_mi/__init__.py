@@ -39,6 +39,7 @@ VIBE_C_ERR_BAD_ARG = 2
 VIBE_MI_ERR_HIP = 100
 VIBE_MI_ERR_NO_DEVICE = 101
 VIBE_MI_ERR_NONFINITE = 102
+VIBE_MI_ERR_PHYSICS = 103
 VIBE_MI_ERR_IO = 110
 VIBE_MI_ERR_FORMAT = 111
 VIBE_MI_ERR_NOT_FOUND = 112
@@ -69,6 +70,9 @@ _SZ = ctypes.c_size_t
 _PD = ctypes.POINTER(ctypes.c_double)
 _PF = ctypes.POINTER(ctypes.c_float)
 _VP = ctypes.c_void_p
+# int fn(user, stage, dt, stream, u, v, w, rho', theta', pi', theta_rate, u_rate, v_rate)
+PHYSICS_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_void_p,
+                              *([ctypes.c_void_p] * 9))
 
 
 class DomainDesc(ctypes.Structure):
@@ -119,6 +123,9 @@ def lib() -> ctypes.CDLL:
         "vibe_mi_domain_kernel_times": (_I, [_VP, _PD, _PD, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
         "vibe_mi_domain_slow_tendency": (_I, [_VP] + [_PD] * 6),
         "vibe_mi_domain_substep": (_I, [_VP, _D]),
+        "vibe_mi_domain_set_physics_tendency": (_I, [_VP, _PD, _PD, _PD]),
+        "vibe_mi_domain_physics_buffers": (_I, [_VP] + [ctypes.POINTER(_VP)] * 3),
+        "vibe_mi_domain_set_physics_callback": (_I, [_VP, PHYSICS_FN, _VP, _I]),
         "vibe_mi_domain_energy": (_I, [_VP, _PD]),
         "vibe_mi_domain_cfl": (_I, [_VP, _D, _PD, _PD]),
         "vibe_mi_domain_stream": (_VP, [_VP]),
@@ -481,6 +488,65 @@ class Domain:
 
     def acoustic_substep(self, dtau: float):
         _check(lib().vibe_mi_domain_substep(self.h, dtau), "vibe_mi_domain_substep")
+
+    # ---- physics coupling (runge_kutta.cpp:46-63; include/vibe_mi/dycore.h)
+
+    def set_physics_tendency(self, theta=None, u=None, v=None):
+        """Holds theta' (K/s), u and v (m/s^2) rates, added to every stage's
+        slow tendency; all None switches the coupling off."""
+        c = self.case
+        cell = c.nx * c.ny * c.nz
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (theta, u, v)]
+        for a in arrs:
+            if a is not None and a.size != cell:
+                raise ValueError(f"physics rates hold {a.size} values, the domain has {cell} cells")
+        self._phys_keep = arrs
+        _check(lib().vibe_mi_domain_set_physics_tendency(self.h, *[_pd(a) for a in arrs]),
+               "vibe_mi_domain_set_physics_tendency")
+
+    def physics_buffers(self):
+        """Device addresses (ints) of the theta', u, v rate buffers."""
+        p = [_VP() for _ in range(3)]
+        _check(lib().vibe_mi_domain_physics_buffers(self.h, *[ctypes.byref(x) for x in p]),
+               "vibe_mi_domain_physics_buffers")
+        return tuple(x.value for x in p)
+
+    def set_physics_callback(self, fn, host: bool = True):
+        """Per-stage physics: before each stage, fn(stage, dt, state, rates).
+        host=True: state is a dict of the stage's FP64 host arrays (u, v, w,
+        rho, theta, pi) and rates a dict of zeroed host arrays (theta, u, v)
+        that fn fills; host=False: both hold device addresses (ints) and fn
+        enqueues its work on rates["stream"].  fn returns None or 0 on
+        success; fn=None removes the callback."""
+        if fn is None:
+            self._phys_cb = None
+            _check(lib().vibe_mi_domain_set_physics_callback(self.h, PHYSICS_FN(), None, 0),
+                   "vibe_mi_domain_set_physics_callback")
+            return
+        c = self.case
+        cell = c.nx * c.ny * c.nz
+        nwf = c.nx * c.ny * (c.nz + 1)
+
+        def arr(p, n):
+            return np.ctypeslib.as_array(ctypes.cast(p, _PD), shape=(n,))
+
+        def tramp(_user, stage, dt, stream, u, v, w, rho, th, pi, rt, ru, rv):
+            try:
+                if host:
+                    st = {"u": arr(u, cell), "v": arr(v, cell), "w": arr(w, nwf), "rho": arr(rho, cell),
+                          "theta": arr(th, cell), "pi": arr(pi, cell)}
+                    rates = {"theta": arr(rt, cell), "u": arr(ru, cell), "v": arr(rv, cell)}
+                else:
+                    st = {"u": u, "v": v, "w": w, "rho": rho, "theta": th, "pi": pi}
+                    rates = {"theta": rt, "u": ru, "v": rv, "stream": stream}
+                r = fn(stage, dt, st, rates)
+                return int(r or 0)
+            except Exception:  # an exception must not cross the C frame
+                return 1
+
+        self._phys_cb = PHYSICS_FN(tramp)
+        _check(lib().vibe_mi_domain_set_physics_callback(self.h, self._phys_cb, None, 1 if host else 0),
+               "vibe_mi_domain_set_physics_callback")
 
     def write_restart(self, path, time: float = 0.0, step: int = 0):
         """Downloads the state and writes it as a restart file
