@@ -56,9 +56,10 @@ int vibe_mi_batch_kernel_times(vibe_mi_batch *b, double *ms_stage, double *ms_su
 /* use_graph = 1 (default): a step's launch sequence is captured once per
  * (active set, nsub) in a hipGraph and replayed; 0: launched eagerly. */
 int vibe_mi_batch_set_graph(vibe_mi_batch *b, int use_graph);
-/* Tile shapes picked by the autotuner for this batch (after the first
- * step): waves per substep workgroup (4, 8, 16) and rows per stage
- * workgroup (64 x rows columns: 2, 4, 8).  The environment variables
+/* Tile shapes the autotuner picked for the active set of the last step
+ * (each active set is tuned on its first step): waves per substep
+ * workgroup (4, 8, 16) and rows per stage workgroup (64 x rows columns:
+ * 2, 4, 8).  The environment variables
  * VIBE_MI_SUBSTEP_WAVES and VIBE_MI_STAGE_ROWS pin either choice (and skip
  * the tuning cache). */
 int vibe_mi_batch_tuning(vibe_mi_batch *b, int *substep_waves, int *stage_rows);

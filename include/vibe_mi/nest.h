@@ -23,6 +23,7 @@
 #ifndef VIBE_MI_NEST_H
 #define VIBE_MI_NEST_H
 
+#include "vibe_mi/batch.h"
 #include "vibe_mi/dycore.h"
 
 #ifdef __cplusplus
@@ -52,6 +53,14 @@ int vibe_mi_nest_interpolate(vibe_mi_domain *child, vibe_mi_domain *parent, cons
  * last step (nest.cpp:110-128).  frac in [0, 1]. */
 int vibe_mi_nest_relax(vibe_mi_domain *child, vibe_mi_domain *parent, const vibe_mi_nest_desc *nd,
                        double frac);
+
+/* The same relaxation between members `child` and `parent` of one batch,
+ * enqueued on the batch's stream behind its last step with no host
+ * synchronisation (vibe_mi_nest_relax synchronises both domains' streams):
+ * the batch's next step, and later work on the child's own stream, are
+ * ordered after it.  The relaxation between the sub-steps of a nest
+ * hierarchy (vibe-model_amd/vibe_mi/nesting.py). */
+int vibe_mi_batch_nest_relax(vibe_mi_batch *b, int child, int parent, const vibe_mi_nest_desc *nd, double frac);
 
 #ifdef __cplusplus
 }

@@ -70,3 +70,14 @@ def test_uniform_flow_stays_exact_through_nesting(vmi):
         for f in ("v", "w", "rho", "theta", "pi"):
             assert np.all(st[f] == 0.0), (c.name, f)
     run.close()
+
+
+def test_batch_nest_relax_rejects_bad_members(vmi):
+    cases, links = _hier()
+    b = vmi.Batch(cases, vmi.FP64)
+    d = vmi.NestDesc(3, links[0][3], links[0][4], 5, 1.0)
+    for c, p in ((1, 1), (-1, 0), (1, 7)):
+        with pytest.raises(vmi.VibeError) as e:
+            vmi.batch_nest_relax(b, c, p, d, 0.5)
+        assert e.value.code == vmi.VIBE_C_ERR_BAD_ARG
+    b.close()

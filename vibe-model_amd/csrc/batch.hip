@@ -53,6 +53,11 @@ struct WorkList {
   TileList stage[3]{};   // {member, x-block, y-block} for TY = 2, 4, 8
   int *ids = nullptr;    // active members
   int nids = 0;
+  std::vector<int> members;  // active members (host copy)
+  // tile shapes for this active set (autotuned on its first step): waves per
+  // substep workgroup and rows per stage workgroup
+  int nw = 8, ty = 4;
+  bool tuned = false;
 };
 
 }  // namespace
@@ -73,13 +78,12 @@ struct vibe_mi_batch {
   std::map<std::string, hipGraphExec_t> graphs;
   std::vector<hipGraph_t> graph_objs;
   bool use_graph = true;
-  bool tuned = false;
   int nw = 8;  // waves per substep workgroup
   int ty = 4;  // rows per stage workgroup
   int cus = 256;
   int gm = kGeoFlat;  // geometry mode of the batch: the most general of its members
   bool phys = false;  // an active member has physics rates: the stage kernels add them
-  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // [5]: nest relaxation
   // per-group kernel times of the last timed eager call (ms, summed)
   bool group_timing = false;
   double ms_stage = 0, ms_sub = 0;
@@ -161,6 +165,7 @@ int work_list(vibe_mi_batch *b, const std::vector<unsigned char> &act, WorkList 
   if (int rc = upload_list(b, st8, &w.stage[2])) return rc;
   if (int rc = bupload(b, &w.ids, order)) return rc;
   w.nids = static_cast<int>(order.size());
+  w.members = order;
   *out = &(b->lists[key] = w);
   return 0;
 }
@@ -342,16 +347,16 @@ int enqueue_step(vibe_mi_batch *b, const WorkList &w, int nsub) {
     aout = cur32 ? (aend ^ 1) : 0;
     aend = (aout + nsub) & 1;
     if (gt) VMI_HIP(hipEventRecord(b->ev[2], b->stream));
-    if (int rc = launch_stage<T>(b, b->ty, w, st, ain, aout, ctr + kXcds * c++)) return rc;
+    if (int rc = launch_stage<T>(b, w.ty, w, st, ain, aout, ctr + kXcds * c++)) return rc;
     if (gt) VMI_HIP(hipEventRecord(b->ev[3], b->stream));
     for (int q = 0; q < nsub; ++q) {
       const bool last = q == nsub - 1;
       int rc;
       if constexpr (sizeof(T) == 4) {
-        rc = (last && st == 2) ? launch_substep<T, double>(b, b->nw, w, st, q, aout, 1, ctr + kXcds * c++)
-                               : launch_substep<T, T>(b, b->nw, w, st, q, aout, last ? 1 : 0, ctr + kXcds * c++);
+        rc = (last && st == 2) ? launch_substep<T, double>(b, w.nw, w, st, q, aout, 1, ctr + kXcds * c++)
+                               : launch_substep<T, T>(b, w.nw, w, st, q, aout, last ? 1 : 0, ctr + kXcds * c++);
       } else {
-        rc = launch_substep<T, T>(b, b->nw, w, st, q, aout, last ? 1 : 0, ctr + kXcds * c++);
+        rc = launch_substep<T, T>(b, w.nw, w, st, q, aout, last ? 1 : 0, ctr + kXcds * c++);
       }
       if (rc) return rc;
     }
@@ -381,10 +386,19 @@ int enqueue_step(vibe_mi_batch *b, const WorkList &w, int nsub) {
 std::map<std::string, std::pair<int, int>> g_btune;
 
 template <typename T>
-int autotune(vibe_mi_batch *b, const WorkList &w) {
-  if (b->tuned) return 0;
-  std::string shape = std::to_string(sizeof(T));
-  for (auto *d : b->doms) shape += "," + std::to_string(d->nx) + "x" + std::to_string(d->ny) + "x" + std::to_string(d->nz) + "r" + std::to_string(d->rk);
+int autotune(vibe_mi_batch *b, WorkList &w) {
+  if (w.tuned) {
+    b->nw = w.nw;
+    b->ty = w.ty;
+    return 0;
+  }
+  // keyed by the active members' shapes (a small active set of a nest
+  // hierarchy wants smaller tiles than the whole batch)
+  std::string shape = std::to_string(sizeof(T)) + (b->phys ? "p" : "");
+  for (int m : w.members) {
+    const vibe_mi_domain *d = b->doms[m];
+    shape += "," + std::to_string(d->nx) + "x" + std::to_string(d->ny) + "x" + std::to_string(d->nz) + "r" + std::to_string(d->rk);
+  }
   // VIBE_MI_SUBSTEP_WAVES (4, 8, 16) and VIBE_MI_STAGE_ROWS (2, 4, 8) pin a
   // shape instead of measuring it (tests, profiling); pinned shapes are not
   // cached
@@ -392,23 +406,23 @@ int autotune(vibe_mi_batch *b, const WorkList &w) {
   const char *env_ty = std::getenv("VIBE_MI_STAGE_ROWS");
   auto it = (env_nw || env_ty) ? g_btune.end() : g_btune.find(shape);
   if (it != g_btune.end()) {
-    b->nw = it->second.first;
-    b->ty = it->second.second;
-    b->tuned = true;
+    w.nw = b->nw = it->second.first;
+    w.ty = b->ty = it->second.second;
+    w.tuned = true;
     return 0;
   }
   bool pin_nw = false, pin_ty = false;
   if (env_nw) {
     const int v = std::atoi(env_nw);
     if (v == 4 || v == 8 || v == 16) {
-      b->nw = v;
+      w.nw = v;
       pin_nw = true;
     }
   }
   if (env_ty) {
     const int v = std::atoi(env_ty);
     if (v == 2 || v == 4 || v == 8) {
-      b->ty = v;
+      w.ty = v;
       pin_ty = true;
     }
   }
@@ -434,7 +448,7 @@ int autotune(vibe_mi_batch *b, const WorkList &w) {
       const int rc = time_it([&]() { return launch_substep<T, T>(b, nw, w, 0, 0, 0, 0, b->ctr); }, &ms);
       if (rc == 0 && ms < best) {
         best = ms;
-        b->nw = nw;
+        w.nw = nw;
       }
     }
     (void)hipGetLastError();
@@ -446,13 +460,15 @@ int autotune(vibe_mi_batch *b, const WorkList &w) {
       const int rc = time_it([&]() { return launch_stage<T>(b, ty, w, 0, -1, 0, b->ctr); }, &ms);
       if (rc == 0 && ms < best) {
         best = ms;
-        b->ty = ty;
+        w.ty = ty;
       }
     }
     (void)hipGetLastError();
   }
-  b->tuned = true;
-  if (!env_nw && !env_ty) g_btune[shape] = {b->nw, b->ty};
+  w.tuned = true;
+  b->nw = w.nw;
+  b->ty = w.ty;
+  if (!env_nw && !env_ty) g_btune[shape] = {w.nw, w.ty};
   return 0;
 }
 
@@ -492,8 +508,8 @@ int step_batch(vibe_mi_batch *b, const double *dt, int nsub, int nsteps, const u
     if (act[m] && b->doms[m]->ph_on) b->phys = true;
   if (int rc = autotune<T>(b, *w)) return rc;
   if constexpr (sizeof(T) == 4)
-    if (int rc = launch_substep<T, double>(b, b->nw, *w, 0, 0, 0, 1, b->ctr, true)) return rc;
-  if (int rc = launch_substep<T, T>(b, b->nw, *w, 0, 0, 0, 0, b->ctr, true)) return rc;
+    if (int rc = launch_substep<T, double>(b, w->nw, *w, 0, 0, 0, 1, b->ctr, true)) return rc;
+  if (int rc = launch_substep<T, T>(b, w->nw, *w, 0, 0, 0, 0, b->ctr, true)) return rc;
   VMI_HIP(hipMemsetAsync(b->flag, 0, sizeof(int), b->stream));
   hipGraphExec_t exec = nullptr;
   if (b->use_graph) {
@@ -546,6 +562,18 @@ int step_batch(vibe_mi_batch *b, const double *dt, int nsub, int nsteps, const u
 }  // namespace
 
 extern "C" {
+
+int vibe_mi_batch_nest_relax(vibe_mi_batch *b, int child, int parent, const vibe_mi_nest_desc *nd, double frac) {
+  if (b == nullptr) return VIBE_C_ERR_BAD_ARG;
+  const int n = static_cast<int>(b->doms.size());
+  if (child < 0 || child >= n || parent < 0 || parent >= n) return VIBE_C_ERR_BAD_ARG;
+  vibe_mi_domain *c = b->doms[child];
+  if (int rc = nest_enqueue(c, b->doms[parent], nd, frac, 0, b->stream)) return rc;
+  // later work on the child's own stream (state copies) waits for the relaxation
+  VMI_HIP(hipEventRecord(b->ev[5], b->stream));
+  VMI_HIP(hipStreamWaitEvent(c->stream, b->ev[5], 0));
+  return 0;
+}
 
 int vibe_mi_batch_create(const vibe_mi_domain_desc *descs, int ndom, vibe_mi_batch **out) {
   if (descs == nullptr || out == nullptr || ndom <= 0) return VIBE_C_ERR_BAD_ARG;

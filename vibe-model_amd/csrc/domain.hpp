@@ -9,6 +9,7 @@
 #include "common.hpp"
 #include "geo.hpp"
 #include "vibe_mi/dycore.h"
+#include "vibe_mi/nest.h"
 
 namespace vibe_mi {
 
@@ -76,6 +77,10 @@ namespace vibe_mi {
 
 // Helpers of the single-domain path used by the batched path (dycore.hip).
 int domain_create(const vibe_mi_domain_desc *ds, vibe_mi_domain **out);
+// Nest interpolation (full = 1) or Davies relaxation of `child` toward
+// `parent` enqueued on `stream`, with no host synchronisation (nest.hip).
+int nest_enqueue(vibe_mi_domain *child, const vibe_mi_domain *parent, const vibe_mi_nest_desc *nd, double frac,
+                 int full, hipStream_t stream);
 // Factorise the HEVI matrix of table slot `slot` for dtau (cached per slot).
 int domain_ensure_table(vibe_mi_domain *d, int slot, double dtau);
 // Base of table slot `slot` (layout of HeviTab, entries of type T).

@@ -2,9 +2,10 @@
 // parent -> child interpolation and Davies relaxation of the child's lateral
 // zone (src/grid/nest.cpp:34-128 roles), for domains on the same GPU.
 //
-// One thread per child column (i, j) loops over the levels and the six
-// prognostics.  The relaxation touches only the zone columns (a thin frame),
-// so it is a small launch next to a step; interior threads exit at once.
+// One thread per child point (i, j, k) loops over the six prognostics (w has
+// one level more).  The relaxation touches only the zone columns (a thin
+// frame), so it is a small launch next to a step; interior threads exit at
+// once.
 // Arithmetic follows c_kernels.c:108-144 (lerp form a + w (b - a), the same
 // association) in FP64 on the FP64 states.
 #include <cmath>
@@ -45,7 +46,7 @@ __device__ __forceinline__ double bilinear(const double *__restrict__ f, int nx,
 }
 
 __global__ void __launch_bounds__(256) k_nest(NestArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
   if (i >= a.cnx) return;
   const int d = min(min(i, a.cnx - 1 - i), min(j, a.cny - 1 - j));
   double al = 1.0;
@@ -64,7 +65,7 @@ __global__ void __launch_bounds__(256) k_nest(NestArgs a) {
     const double xp = f == 0 ? xu : xc;
     const double yp = f == 1 ? yv : yc;
     const int nk = f == 2 ? a.nz + 1 : a.nz;
-    for (int k = 0; k < nk; ++k) {
+    if (k < nk) {
       const double pnew = bilinear(a.pn[f] + k * pxy, a.pnx, a.pny, xp, yp);
       double p = pnew;
       if (!a.full && a.frac != 1.0) {
@@ -77,13 +78,15 @@ __global__ void __launch_bounds__(256) k_nest(NestArgs a) {
   }
 }
 
-int nest_launch(vibe_mi_domain *child, vibe_mi_domain *parent, const vibe_mi_nest_desc *nd, double frac, int full) {
+}  // namespace
+
+int nest_enqueue(vibe_mi_domain *child, const vibe_mi_domain *parent, const vibe_mi_nest_desc *nd, double frac,
+                 int full, hipStream_t stream) {
   if (child == nullptr || parent == nullptr || nd == nullptr || child == parent) return VIBE_C_ERR_BAD_ARG;
   if (nd->ratio < 1 || nd->zone < 1 || nd->zone > kMaxZone || child->nz != parent->nz ||
       !(frac >= 0.0 && frac <= 1.0))
     return VIBE_C_ERR_BAD_ARG;
   VMI_HIP(hipSetDevice(child->device));
-  VMI_HIP(hipStreamSynchronize(parent->stream));
   const FieldSet &pn = parent->S[parent->cur0];
   const FieldSet &po = parent->S[(parent->cur0 + 2) % 3];
   const FieldSet &cs = child->S[child->cur0];
@@ -108,8 +111,18 @@ int nest_launch(vibe_mi_domain *child, vibe_mi_domain *parent, const vibe_mi_nes
   a.zone = nd->zone;
   a.frac = frac;
   a.full = full;
-  hipLaunchKernelGGL(k_nest, dim3(ceil_div(child->nx, 256), child->ny), dim3(256), 0, child->stream, a);
+  hipLaunchKernelGGL(k_nest, dim3(ceil_div(child->nx, 256), child->ny, child->nz + 1), dim3(256), 0, stream, a);
   VMI_HIP(hipGetLastError());
+  return 0;
+}
+
+namespace {
+
+int nest_launch(vibe_mi_domain *child, vibe_mi_domain *parent, const vibe_mi_nest_desc *nd, double frac, int full) {
+  if (child == nullptr || parent == nullptr) return VIBE_C_ERR_BAD_ARG;
+  VMI_HIP(hipSetDevice(child->device));
+  VMI_HIP(hipStreamSynchronize(parent->stream));
+  if (int rc = nest_enqueue(child, parent, nd, frac, full, child->stream)) return rc;
   VMI_HIP(hipStreamSynchronize(child->stream));
   return 0;
 }

@@ -142,6 +142,7 @@ def lib() -> ctypes.CDLL:
         "vibe_mi_davies_profile": (_I, [_I, _D, _PD]),
         "vibe_mi_nest_interpolate": (_I, [_VP, _VP, ctypes.POINTER(NestDesc)]),
         "vibe_mi_nest_relax": (_I, [_VP, _VP, ctypes.POINTER(NestDesc), _D]),
+        "vibe_mi_batch_nest_relax": (_I, [_VP, _I, _I, ctypes.POINTER(NestDesc), _D]),
         "vibe_mi_zeta_uniform": (_I, [_I, _PD]),
         "vibe_mi_zeta_tanh": (_I, [_I, _D, _D, _D, _PD]),
         "vibe_mi_terrain_agnesi_ridge": (_I, [_I, _I, _D, _D, _D, _PD]),
@@ -688,3 +689,10 @@ def nest_relax(child: "Domain", parent: "Domain", desc: NestDesc, frac: float = 
     interpolated in space and (with frac) in time between the parent's last two
     states."""
     _check(lib().vibe_mi_nest_relax(child.h, parent.h, ctypes.byref(desc), frac), "vibe_mi_nest_relax")
+
+
+def batch_nest_relax(batch: "Batch", child: int, parent: int, desc: NestDesc, frac: float = 1.0):
+    """nest_relax between members of one batch, enqueued on the batch stream
+    behind its last step without a host synchronisation."""
+    _check(lib().vibe_mi_batch_nest_relax(batch.h, child, parent, ctypes.byref(desc), frac),
+           "vibe_mi_batch_nest_relax")

@@ -9,12 +9,13 @@ active domains run in the same batched launches (vibe_mi.Batch).  After each
 sub-step every nest that just stepped is relaxed toward its parent,
 interpolated bilinearly in space and linearly in time across the parent's
 current step (frac = (m + 1) / r for the nest's m-th step inside it), coarse
-levels first.  This is the one-way form of the reference's nest operators
+levels first, enqueued on the batch stream with no host round trip.  This is
+the one-way form of the reference's nest operators
 (src/grid/nest.cpp:52-128; its driver never calls them, driver.cpp:212-219).
 """
 from __future__ import annotations
 
-from . import MIXED, Batch, NestDesc, nest_interpolate, nest_relax
+from . import MIXED, Batch, NestDesc, batch_nest_relax, nest_interpolate
 
 
 class NestedRun:
@@ -70,7 +71,7 @@ class NestedRun:
                 else:
                     self.batch.step(dts, self.nsub, 1, active)
                 for c, p, d, frac in relax:
-                    nest_relax(self.members[c], self.members[p], d, frac)
+                    batch_nest_relax(self.batch, c, p, d, frac)
         return ms
 
     def close(self):
