@@ -1,0 +1,83 @@
+/* vibe_mi/helmholtz.h -- 3-D Helmholtz solver of the semi-implicit path
+ * (C ABI, device-resident).
+ *
+ * Replaces, on the GPU, the reference's 3-D solvers
+ *   HelmholtzSolver (set_coefficients / solve / iterations / residual)
+ *                                   include/vibe/time/helmholtz.hpp:62-81
+ *   apply_operator                  src/time/helmholtz.cpp:167-195
+ *   helmholtz_residual_l2           src/time/helmholtz.cpp:205-218
+ *   KrylovJacobiSolver (BiCGSTAB)   src/time/helmholtz.cpp:286-384
+ *   RefinementSolver (low-precision solve + FP64 residual correction)
+ *                                   src/time/helmholtz.cpp:427-478
+ * for the operator of the reference,
+ *
+ *   y = -div(a grad x) + b x,
+ *   -div(a grad x) = -[(ae (x_E - x) - aw (x - x_W)) / dx^2
+ *                     + (an (x_N - x) - as (x - x_S)) / dy^2
+ *                     + (at (x_T - x) - ab (x - x_B)) / dz^2],
+ *
+ * with face coefficients the mean of the two cell values (ae = (a + a_E)/2,
+ * ...), neighbours clamped at every boundary (zero flux: x_E = x at the last
+ * column, likewise W, N, S, top and bottom) and dz = max(dzeta_k (ztop - zs),
+ * 1) per column and level.  a and b are cell arrays [nz][ny][nx], b > 0 for
+ * a definite operator.
+ *
+ * Solvers (the operator is not symmetric on a stretched grid, hence
+ * BiCGSTAB as in the reference), each preconditioned by exact solves of the
+ * vertical lines (per-column tridiagonal: the vertical couplings plus the
+ * horizontal diagonal), which removes the stiffness of thin layers:
+ *   VIBE_MI_HELM_BICGSTAB   FP64 BiCGSTAB to the relative residual tol;
+ *   VIBE_MI_HELM_REFINE     FP32 BiCGSTAB inner solves of the FP64 residual
+ *                           equation, x += dx, until the FP64 relative
+ *                           residual ||f - A x|| / ||f|| < tol (iterative
+ *                           refinement).
+ * Iterations run without host round trips (scalars stay on the device); the
+ * host reads the residual every few iterations.  Calls on one handle must not
+ * overlap.  Return codes as in dycore.h.
+ */
+#ifndef VIBE_MI_HELMHOLTZ_H
+#define VIBE_MI_HELMHOLTZ_H
+
+#include <stddef.h>
+
+#include "vibe_mi/c_kernels.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VIBE_MI_HELM_BICGSTAB 0
+#define VIBE_MI_HELM_REFINE 1
+
+typedef struct {
+  int nx, ny, nz;
+  double dx, dy, ztop;
+  const double *zeta; /* nz+1 normalised interfaces; dzeta_k = zeta[k+1] - zeta[k] */
+  const double *zs;   /* ny*nx terrain height (m)                                  */
+} vibe_mi_helmholtz_desc;
+
+typedef struct vibe_mi_helmholtz vibe_mi_helmholtz;
+
+int vibe_mi_helmholtz_create(const vibe_mi_helmholtz_desc *desc, vibe_mi_helmholtz **out);
+int vibe_mi_helmholtz_destroy(vibe_mi_helmholtz *h);
+/* a, b: host cell arrays [nz][ny][nx] (set_coefficients role); also builds
+ * the line preconditioner.  VIBE_C_ERR_BAD_ARG if b <= 0 or a < 0 anywhere. */
+int vibe_mi_helmholtz_set_coefficients(vibe_mi_helmholtz *h, const double *a, const double *b);
+/* y = A x in FP64 on the device (apply_operator), host arrays. */
+int vibe_mi_helmholtz_apply(vibe_mi_helmholtz *h, const double *x, double *y);
+/* ||A x - f||_2 (helmholtz_residual_l2), host arrays. */
+int vibe_mi_helmholtz_residual(vibe_mi_helmholtz *h, const double *f, const double *x, double *res_l2);
+/* Solves A x = f from x = 0 (the reference's solvers start from zero):
+ * kind VIBE_MI_HELM_BICGSTAB or VIBE_MI_HELM_REFINE, relative residual tol,
+ * at most max_iter BiCGSTAB iterations in total.  iterations and rel_res
+ * (either may be NULL) report the iterations taken and the final FP64
+ * relative residual; like the reference, not converging within max_iter is
+ * not an error (check rel_res). */
+int vibe_mi_helmholtz_solve(vibe_mi_helmholtz *h, const double *f, double *x, int kind, double tol, int max_iter,
+                            int *iterations, double *rel_res);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VIBE_MI_HELMHOLTZ_H */

@@ -14,6 +14,9 @@ names mirror the reference's interfaces:
     ``slow_tendency`` (Equations::tendencies, src/dyn/equations.cpp:60-150),
     ``acoustic_substep`` (src/time/runge_kutta.cpp:65-75) and
     ``energy_budget`` (src/dyn/diagnostics.cpp:255-277);
+  * ``Helmholtz`` -- the 3-D Helmholtz solvers of the semi-implicit path
+    (HelmholtzSolver, src/time/helmholtz.cpp:167-478): BiCGSTAB and FP32
+    solves with FP64 iterative refinement (include/vibe_mi/helmholtz.h);
   * ``vibebin_*`` and ``Domain.write_restart`` / ``read_restart`` -- the
     reference's .vibebin BinaryWriter / BinaryReader and write_restart /
     read_restart (src/io/binary_writer.cpp, src/io/field_io.cpp:207-250),
@@ -84,6 +87,12 @@ class DomainDesc(ctypes.Structure):
     ]
 
 
+class HelmholtzDesc(ctypes.Structure):
+    """vibe_mi_helmholtz_desc (vibe_mi/helmholtz.h)."""
+    _fields_ = [("nx", _I), ("ny", _I), ("nz", _I), ("dx", _D), ("dy", _D), ("ztop", _D),
+                ("zeta", _PD), ("zs", _PD)]
+
+
 class NestDesc(ctypes.Structure):
     """vibe_mi_nest_desc (vibe_mi/nest.h)."""
     _fields_ = [("ratio", _I), ("i0", _I), ("j0", _I), ("zone", _I), ("alpha_max", _D)]
@@ -143,6 +152,12 @@ def lib() -> ctypes.CDLL:
         "vibe_mi_nest_interpolate": (_I, [_VP, _VP, ctypes.POINTER(NestDesc)]),
         "vibe_mi_nest_relax": (_I, [_VP, _VP, ctypes.POINTER(NestDesc), _D]),
         "vibe_mi_batch_nest_relax": (_I, [_VP, _I, _I, ctypes.POINTER(NestDesc), _D]),
+        "vibe_mi_helmholtz_create": (_I, [ctypes.POINTER(HelmholtzDesc), ctypes.POINTER(_VP)]),
+        "vibe_mi_helmholtz_destroy": (_I, [_VP]),
+        "vibe_mi_helmholtz_set_coefficients": (_I, [_VP, _PD, _PD]),
+        "vibe_mi_helmholtz_apply": (_I, [_VP, _PD, _PD]),
+        "vibe_mi_helmholtz_residual": (_I, [_VP, _PD, _PD, _PD]),
+        "vibe_mi_helmholtz_solve": (_I, [_VP, _PD, _PD, _I, _D, _I, ctypes.POINTER(_I), _PD]),
         "vibe_mi_zeta_uniform": (_I, [_I, _PD]),
         "vibe_mi_zeta_tanh": (_I, [_I, _D, _D, _D, _PD]),
         "vibe_mi_terrain_agnesi_ridge": (_I, [_I, _I, _D, _D, _D, _PD]),
@@ -696,3 +711,66 @@ def batch_nest_relax(batch: "Batch", child: int, parent: int, desc: NestDesc, fr
     behind its last step without a host synchronisation."""
     _check(lib().vibe_mi_batch_nest_relax(batch.h, child, parent, ctypes.byref(desc), frac),
            "vibe_mi_batch_nest_relax")
+
+
+HELM_BICGSTAB = 0
+HELM_REFINE = 1
+
+
+class Helmholtz:
+    """3-D Helmholtz operator -div(a grad x) + b x and its solvers on the GPU
+    (HelmholtzSolver role, include/vibe/time/helmholtz.hpp:62-81)."""
+
+    def __init__(self, nx, ny, nz, dx, dy, ztop, zeta, zs=None):
+        self.shape = (nz, ny, nx)
+        self._zeta = np.ascontiguousarray(zeta, dtype=np.float64)
+        self._zs = np.ascontiguousarray(np.zeros(nx * ny) if zs is None else zs, dtype=np.float64).reshape(-1)
+        if self._zeta.size != nz + 1 or self._zs.size != nx * ny:
+            raise ValueError("zeta needs nz+1 values and zs nx*ny")
+        self.desc = HelmholtzDesc(nx, ny, nz, dx, dy, ztop, _pd(self._zeta), _pd(self._zs))
+        h = _VP()
+        _check(lib().vibe_mi_helmholtz_create(ctypes.byref(self.desc), ctypes.byref(h)), "vibe_mi_helmholtz_create")
+        self.h = h
+        self.iterations = 0
+        self.residual_rel = float("nan")
+
+    def _arr(self, x):
+        a = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        if a.size != self.shape[0] * self.shape[1] * self.shape[2]:
+            raise ValueError("expected an nz*ny*nx array")
+        return a
+
+    def set_coefficients(self, a, b):
+        _check(lib().vibe_mi_helmholtz_set_coefficients(self.h, _pd(self._arr(a)), _pd(self._arr(b))),
+               "vibe_mi_helmholtz_set_coefficients")
+
+    def apply(self, x):
+        y = np.empty(self.shape)
+        _check(lib().vibe_mi_helmholtz_apply(self.h, _pd(self._arr(x)), _pd(y)), "vibe_mi_helmholtz_apply")
+        return y
+
+    def residual(self, f, x) -> float:
+        r = _D()
+        _check(lib().vibe_mi_helmholtz_residual(self.h, _pd(self._arr(f)), _pd(self._arr(x)), ctypes.byref(r)),
+               "vibe_mi_helmholtz_residual")
+        return r.value
+
+    def solve(self, f, kind: int = HELM_BICGSTAB, tol: float = 1e-10, max_iter: int = 300):
+        """x with A x = f from x = 0; sets .iterations and .residual_rel."""
+        x = np.empty(self.shape)
+        its, rel = _I(), _D()
+        _check(lib().vibe_mi_helmholtz_solve(self.h, _pd(self._arr(f)), _pd(x), kind, tol, max_iter,
+                                             ctypes.byref(its), ctypes.byref(rel)), "vibe_mi_helmholtz_solve")
+        self.iterations, self.residual_rel = its.value, rel.value
+        return x
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vibe_mi_helmholtz_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
