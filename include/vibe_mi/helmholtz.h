@@ -3,12 +3,12 @@
  *
  * Replaces, on the GPU, the reference's 3-D solvers
  *   HelmholtzSolver (set_coefficients / solve / iterations / residual)
- *                                   include/vibe/time/helmholtz.hpp:62-81
- *   apply_operator                  src/time/helmholtz.cpp:167-195
- *   helmholtz_residual_l2           src/time/helmholtz.cpp:205-218
- *   KrylovJacobiSolver (BiCGSTAB)   src/time/helmholtz.cpp:286-384
+ *                                   include/vibe/time/helmholtz.hpp:63-86
+ *   apply_operator                  src/time/helmholtz.cpp:173-201
+ *   helmholtz_residual_l2           src/time/helmholtz.cpp:211-225
+ *   KrylovJacobiSolver (BiCGSTAB)   src/time/helmholtz.cpp:286-379
  *   RefinementSolver (low-precision solve + FP64 residual correction)
- *                                   src/time/helmholtz.cpp:427-478
+ *                                   src/time/helmholtz.cpp:420-476
  * for the operator of the reference,
  *
  *   y = -div(a grad x) + b x,

@@ -4,10 +4,10 @@ Test infrastructure only (tests/ import it as the checker of
 libvibe_mi.so's vibe_mi_helmholtz_* entry points).
 
 Follows:
-  apply_operator          src/time/helmholtz.cpp:167-195: y = -div(a grad x) + b x,
+  apply_operator          src/time/helmholtz.cpp:173-201: y = -div(a grad x) + b x,
                           face a = mean of the two cells, neighbours clamped at
                           every boundary, dz = max(dzeta_k (ztop - zs), 1)
-  helmholtz_residual_l2   src/time/helmholtz.cpp:205-218
+  helmholtz_residual_l2   src/time/helmholtz.cpp:211-225
 and, for a reference solution, the operator assembled as a sparse matrix
 and solved directly (scipy.sparse.linalg.spsolve, a library routine).
 

@@ -1,7 +1,7 @@
 // helmholtz.hip -- 3-D Helmholtz solver of the semi-implicit path on the GPU
 // (C ABI in include/vibe_mi/helmholtz.h).
 //
-// Operator (src/time/helmholtz.cpp:167-195 of the reference): one thread
+// Operator (src/time/helmholtz.cpp:173-201 of the reference): one thread
 // per cell, 7-point stencil with face coefficients the mean of the two cell
 // values of a, neighbours clamped at every boundary, dz from the column's
 // depth.  The FP64 instance performs the reference's operations in the

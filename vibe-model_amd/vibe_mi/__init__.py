@@ -15,7 +15,7 @@ names mirror the reference's interfaces:
     ``acoustic_substep`` (src/time/runge_kutta.cpp:65-75) and
     ``energy_budget`` (src/dyn/diagnostics.cpp:255-277);
   * ``Helmholtz`` -- the 3-D Helmholtz solvers of the semi-implicit path
-    (HelmholtzSolver, src/time/helmholtz.cpp:167-478): BiCGSTAB and FP32
+    (HelmholtzSolver, src/time/helmholtz.cpp:173-476): BiCGSTAB and FP32
     solves with FP64 iterative refinement (include/vibe_mi/helmholtz.h);
   * ``vibebin_*`` and ``Domain.write_restart`` / ``read_restart`` -- the
     reference's .vibebin BinaryWriter / BinaryReader and write_restart /
@@ -719,7 +719,7 @@ HELM_REFINE = 1
 
 class Helmholtz:
     """3-D Helmholtz operator -div(a grad x) + b x and its solvers on the GPU
-    (HelmholtzSolver role, include/vibe/time/helmholtz.hpp:62-81)."""
+    (HelmholtzSolver role, include/vibe/time/helmholtz.hpp:63-86)."""
 
     def __init__(self, nx, ny, nz, dx, dy, ztop, zeta, zs=None):
         self.shape = (nz, ny, nx)
