@@ -55,7 +55,10 @@ typedef struct {
 
 typedef struct vibe_mi_domain vibe_mi_domain;
 
-/* VIBE_C_ERR_BAD_ARG also when nx*ny*(nz+1)*8 bytes reach 4 GiB (the kernels
+/* Sizes: nx, ny >= 4 (the 3-cell periodic stencils); 3 <= nz <= 103 in
+ * FP64 and 3 <= nz <= 128 in FP32-mixed (the tallest column one substep
+ * workgroup solves with its factorisation in LDS).
+ * VIBE_C_ERR_BAD_ARG also when nx*ny*(nz+1)*8 bytes reach 4 GiB (the kernels
  * address a field with 32-bit byte offsets); larger grids run as several
  * members of a batch. */
 int vibe_mi_domain_create(const vibe_mi_domain_desc *desc, vibe_mi_domain **out);

@@ -61,3 +61,20 @@ def test_domain_create_rejects_fields_of_4_gib():
     h = ctypes.c_void_p()
     assert L.vibe_mi_domain_create(ctypes.byref(desc), ctypes.byref(h)) == 2
     assert not h.value
+
+
+@pytest.mark.parametrize("nx,ny,nz,prec", [(3, 8, 10, 0), (8, 3, 10, 0), (8, 8, 2, 0), (8, 8, 104, 0),
+                                           (8, 8, 129, 1)])
+def test_domain_create_rejects_sizes_outside_the_kernels_range(nx, ny, nz, prec):
+    """nx, ny >= 4 (3-cell periodic stencils) and 3 <= nz <= 103 (FP64) or
+    128 (FP32-mixed), the tallest column one substep workgroup takes; refused
+    before any allocation."""
+    import ctypes
+    import numpy as np
+    import vibe_mi
+    L = vibe_mi.lib()
+    big = np.zeros(max(nx * ny * (nz + 1), nz + 1))
+    desc = vibe_mi.DomainDesc(nx, ny, nz, 1000.0, 1000.0, 20000.0, *[vibe_mi._pd(big)] * 6, 0.1, prec)
+    h = ctypes.c_void_p()
+    assert L.vibe_mi_domain_create(ctypes.byref(desc), ctypes.byref(h)) == 2
+    assert not h.value

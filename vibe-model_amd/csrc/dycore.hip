@@ -600,7 +600,12 @@ extern "C" {
 int vibe_mi_domain_create(const vibe_mi_domain_desc *ds, vibe_mi_domain **out) {
   if (ds == nullptr || out == nullptr) return VIBE_C_ERR_BAD_ARG;
   *out = nullptr;
-  if (ds->nx < 4 || ds->ny < 4 || ds->nz < 3 || ds->dx <= 0 || ds->dy <= 0 || ds->ztop <= 0 ||
+  // the tallest column one substep workgroup solves: its HEVI factorisation
+  // (3 x (nz-1) x 64 entries) is staged in LDS next to 4 waves' buffers,
+  // which caps FP64 columns at 103 levels; FP32 columns are capped at 128 by
+  // the level chunks (at most 16 waves of 8 levels, 8 of 16 or 4 of 32)
+  const int nz_max = ds->precision == VIBE_MI_FP64 ? 103 : 128;
+  if (ds->nx < 4 || ds->ny < 4 || ds->nz < 3 || ds->nz > nz_max || ds->dx <= 0 || ds->dy <= 0 || ds->ztop <= 0 ||
       ds->zeta == nullptr || ds->zs == nullptr || ds->rho0 == nullptr || ds->theta0 == nullptr ||
       ds->pi0 == nullptr || ds->sponge == nullptr ||
       (ds->precision != VIBE_MI_FP64 && ds->precision != VIBE_MI_MIXED))
