@@ -2,9 +2,9 @@
 // (C ABI in include/vibe_mi/helmholtz.h).
 //
 // Operator (src/time/helmholtz.cpp:173-201 of the reference): one thread
-// per cell, 7-point stencil with face coefficients the mean of the two cell
-// values of a, neighbours clamped at every boundary, dz from the column's
-// depth.  The FP64 instance performs the reference's operations in the
+// per column marching up the levels, 7-point stencil with face coefficients
+// the mean of the two cell values of a, neighbours clamped at every
+// boundary, dz from the column's depth.  The FP64 instance performs the reference's operations in the
 // reference's order, so with -ffp-contract=off it rounds like a CPU
 // evaluation of the same expression.
 //
@@ -46,32 +46,74 @@ struct Op {
 
 __device__ __forceinline__ int clampi(int i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); }
 
-// y = A x (mode 0) or y = f - A x (mode 1)
+// y = A x (mode 0) or y = f - A x (mode 1).  One thread per column of a
+// 64 x 4 tile marching up the levels: the column's x and a at k-1, k, k+1
+// stay in registers (the next level's are loaded one level ahead), the
+// E/W/N/S neighbours of level k come from the same or adjacent rows of the
+// tile (cache hits for all but the tile's edge rows).  Same operations in the
+// same order as the reference's loop body.
+constexpr int kApplyRows = 4;
 template <typename T>
-__global__ void __launch_bounds__(256) k_apply(Op<T> o, const T *__restrict__ x, const T *__restrict__ f,
-                                               T *__restrict__ y, int mode) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
-  if (i >= o.nx) return;
+__global__ void __launch_bounds__(64 * kApplyRows) k_apply(Op<T> o, const T *__restrict__ x, const T *__restrict__ f,
+                                                           T *__restrict__ y, int mode) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * kApplyRows + threadIdx.y;
+  if (i >= o.nx || j >= o.ny) return;
   const size_t nxy = static_cast<size_t>(o.nx) * o.ny;
-  auto at = [&](const T *p, int ii, int jj, int kk) {
-    return p[static_cast<size_t>(clampi(kk, o.nz)) * nxy + static_cast<size_t>(clampi(jj, o.ny)) * o.nx +
-             clampi(ii, o.nx)];
+  const size_t col = static_cast<size_t>(j) * o.nx + i;
+  const size_t cE = static_cast<size_t>(j) * o.nx + clampi(i + 1, o.nx);
+  const size_t cW = static_cast<size_t>(j) * o.nx + clampi(i - 1, o.nx);
+  const size_t cN = static_cast<size_t>(clampi(j + 1, o.ny)) * o.nx + i;
+  const size_t cS = static_cast<size_t>(clampi(j - 1, o.ny)) * o.nx + i;
+  const T depth = o.ztop - o.zs[col];
+  const int k1 = o.nz > 1 ? 1 : 0;
+  T xm = x[col], xc = xm, xp = x[static_cast<size_t>(k1) * nxy + col];
+  T am = o.a[col], ac = am, ap = o.a[static_cast<size_t>(k1) * nxy + col];
+  // level k's horizontal neighbours and b, loaded during level k-1
+  T q[9];
+  auto ld_nb = [&](int k, T *v) {
+    const size_t lv = static_cast<size_t>(k) * nxy;
+    v[0] = x[lv + cE];
+    v[1] = x[lv + cW];
+    v[2] = x[lv + cN];
+    v[3] = x[lv + cS];
+    v[4] = o.a[lv + cE];
+    v[5] = o.a[lv + cW];
+    v[6] = o.a[lv + cN];
+    v[7] = o.a[lv + cS];
+    v[8] = o.b[lv + col];
   };
-  const size_t c = static_cast<size_t>(k) * nxy + static_cast<size_t>(j) * o.nx + i;
-  const T ac = o.a[c], xc = x[c];
-  T dz = o.dzeta[k] * (o.ztop - o.zs[static_cast<size_t>(j) * o.nx + i]);
-  dz = dz > T(1) ? dz : T(1);
-  const T ae = T(0.5) * (ac + at(o.a, i + 1, j, k));
-  const T aw = T(0.5) * (ac + at(o.a, i - 1, j, k));
-  const T an = T(0.5) * (ac + at(o.a, i, j + 1, k));
-  const T as = T(0.5) * (ac + at(o.a, i, j - 1, k));
-  const T atp = T(0.5) * (ac + at(o.a, i, j, k + 1));
-  const T ab = T(0.5) * (ac + at(o.a, i, j, k - 1));
-  const T fx = (ae * (at(x, i + 1, j, k) - xc) - aw * (xc - at(x, i - 1, j, k))) / o.dx2;
-  const T fy = (an * (at(x, i, j + 1, k) - xc) - as * (xc - at(x, i, j - 1, k))) / o.dy2;
-  const T fz = (atp * (at(x, i, j, k + 1) - xc) - ab * (xc - at(x, i, j, k - 1))) / (dz * dz);
-  const T ax = -fx - fy - fz + o.b[c] * xc;
-  y[c] = mode ? f[c] - ax : ax;
+  ld_nb(0, q);
+  for (int k = 0; k < o.nz; ++k) {
+    const size_t lv = static_cast<size_t>(k) * nxy;
+    const size_t ln = static_cast<size_t>(k + 2 < o.nz ? k + 2 : o.nz - 1) * nxy;
+    const T xn = x[ln + col], an_ = o.a[ln + col];  // level k+2, one level ahead
+    T qn[9];
+    ld_nb(k + 1 < o.nz ? k + 1 : k, qn);  // level k+1's neighbours (the last level reloads itself)
+    const T xE = q[0], xW = q[1], xN = q[2], xS = q[3];
+    const T aE = q[4], aW = q[5], aN = q[6], aS = q[7];
+    const T bc = q[8];
+    T dz = o.dzeta[k] * depth;
+    dz = dz > T(1) ? dz : T(1);
+    const T ae = T(0.5) * (ac + aE);
+    const T aw = T(0.5) * (ac + aW);
+    const T an = T(0.5) * (ac + aN);
+    const T as = T(0.5) * (ac + aS);
+    const T atp = T(0.5) * (ac + ap);
+    const T ab = T(0.5) * (ac + am);
+    const T fx = (ae * (xE - xc) - aw * (xc - xW)) / o.dx2;
+    const T fy = (an * (xN - xc) - as * (xc - xS)) / o.dy2;
+    const T fz = (atp * (xp - xc) - ab * (xc - xm)) / (dz * dz);
+    const T ax = -fx - fy - fz + bc * xc;
+    y[lv + col] = mode ? f[lv + col] - ax : ax;
+    xm = xc;
+    xc = xp;
+    xp = xn;
+    am = ac;
+    ac = ap;
+    ap = an_;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) q[e] = qn[e];
+  }
 }
 
 // Factorisation of the line preconditioner, one thread per column: rows
@@ -313,8 +355,8 @@ Op<T> op_of(const vibe_mi_helmholtz *h, const Work<T> &w) {
 
 template <typename T>
 int apply(vibe_mi_helmholtz *h, const Work<T> &w, const T *x, const T *f, T *y, int mode) {
-  hipLaunchKernelGGL((k_apply<T>), dim3(ceil_div(h->nx, 256), h->ny, h->nz), dim3(256), 0, h->stream, op_of(h, w), x,
-                     f, y, mode);
+  hipLaunchKernelGGL((k_apply<T>), dim3(ceil_div(h->nx, 64), ceil_div(h->ny, kApplyRows)), dim3(64, kApplyRows), 0,
+                     h->stream, op_of(h, w), x, f, y, mode);
   VMI_HIP(hipGetLastError());
   return 0;
 }
