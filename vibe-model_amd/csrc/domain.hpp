@@ -63,6 +63,7 @@ struct vibe_mi_domain {
   vibe_mi_physics_fn ph_fn = nullptr;
   void *ph_user = nullptr;
   int ph_host = 0;
+  std::vector<double> ph_state, ph_rates;  // host staging of a host-mode callback (allocated once)
   vibe_mi::NSum *part = nullptr;
   double *dres = nullptr;
   unsigned long long *dmax = nullptr;

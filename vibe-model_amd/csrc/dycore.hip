@@ -207,7 +207,9 @@ int call_physics(vibe_mi_domain *d, int stage, double dt, const FieldSet &s) {
   for (double *p : {d->ph_th, d->ph_u, d->ph_v}) VMI_HIP(hipMemsetAsync(p, 0, nc, d->stream));
   int rc = 0;
   if (d->ph_host) {
-    std::vector<double> h(5 * d->ncell + d->nwf), r(3 * d->ncell, 0.0);
+    std::vector<double> &h = d->ph_state, &r = d->ph_rates;
+    h.resize(5 * d->ncell + d->nwf);
+    r.assign(3 * d->ncell, 0.0);
     double *hu = h.data(), *hv = hu + d->ncell, *hr = hv + d->ncell, *ht = hr + d->ncell, *hp = ht + d->ncell,
            *hw = hp + d->ncell;
     VMI_HIP(hipStreamSynchronize(d->stream));
