@@ -138,6 +138,18 @@ int vibe_mi_zeta_tanh(int nz, double ztop, double dz_bottom, double dz_top, doub
 /* Witch-of-Agnesi ridge h a^2/(a^2+(x-xc)^2), x at cell centres, xc at the
  * domain centre, uniform in y (geometry.cpp:103-117 is the radial form). */
 int vibe_mi_terrain_agnesi_ridge(int nx, int ny, double dx, double h, double a, double *zs);
+/* Geometric vertical coordinate (make_stretched_zeta, geometry.cpp:39-60):
+ * layer thicknesses first_thickness * stretch^k, normalised; zeta[nz] = 1. */
+int vibe_mi_zeta_stretched(int nz, double first_thickness, double stretch, double *zeta);
+/* Radial Witch of Agnesi h0 a^2/(a^2 + d^2), d from (xc, yc) to the point
+ * (i dx, j dy) (witch_of_agnesi, geometry.cpp:103-117). */
+int vibe_mi_terrain_agnesi_bell(int nx, int ny, double dx, double dy, double h0, double a, double xc, double yc,
+                                double *zs);
+/* In-place terrain smoothing (smooth_terrain, geometry.cpp:70-101): per
+ * pass, a 1-2-1 filter of the interior that only lowers points, then the
+ * interior capped at its highest neighbour + max_slope (if max_slope > 0);
+ * the outer rows and columns keep their values. */
+int vibe_mi_terrain_smooth(double *zs, int nx, int ny, int passes, double max_slope);
 /* Seeded Gaussian random terrain, std sigma, Gaussian correlation length L,
  * periodic, minimum shifted to 0. */
 int vibe_mi_terrain_gaussian(int nx, int ny, double dx, double dy, double sigma, double L,

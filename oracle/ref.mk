@@ -29,3 +29,16 @@ all: $(VBIN)
 $(VBIN): ref_vibebin.cpp $(REF)/src/io/binary_writer.cpp $(REF)/src/grid/geometry.cpp
 	mkdir -p _ref
 	$(CXX) $(CXXFLAGS) -o $@ ref_vibebin.cpp $(REF)/src/io/binary_writer.cpp $(REF)/src/grid/geometry.cpp
+
+# The reference's grid generators, sponge profile and isothermal reference
+# state (src/grid/geometry.cpp, src/dyn/damping.cpp, src/dyn/reference_state.cpp)
+# with this directory's ref_setup.cpp driver; prints the golden JSON of
+# tests/golden/setup_reference.json.
+SETUP = _ref/ref_setup
+
+all: $(SETUP)
+
+$(SETUP): ref_setup.cpp $(REF)/src/grid/geometry.cpp $(REF)/src/dyn/damping.cpp $(REF)/src/dyn/reference_state.cpp
+	mkdir -p _ref
+	$(CXX) $(CXXFLAGS) -o $@ ref_setup.cpp $(REF)/src/grid/geometry.cpp $(REF)/src/dyn/damping.cpp \
+	  $(REF)/src/dyn/reference_state.cpp

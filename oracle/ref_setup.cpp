@@ -1,0 +1,80 @@
+// ref_setup.cpp -- golden vectors of the reference's grid and reference-state
+// setup, computed by the REFERENCE's own code (src/grid/geometry.cpp,
+// src/dyn/damping.cpp, src/dyn/reference_state.cpp, compiled from
+// /root/reference by oracle/ref.mk).
+//
+// Test infrastructure: tests/golden/make_setup_golden.py runs it and commits
+// the JSON it prints; tests/test_setup_golden.py checks libvibe_mi.so's
+// host setup helpers against it.  Cases:
+//   make_stretched_zeta(16, 1, 1.08) and (40, 1, 1.05)      geometry.cpp:39-60
+//   witch_of_agnesi(8, 6, 1000, 1500, 800, 3000, 3000, 4000)  geometry.cpp:103-117
+//   smooth_terrain(that, 8, 6, 2 passes, max_slope 50)       geometry.cpp:70-101
+//   Damping sponge profile, 16 stretched levels, ztop 20 km,
+//     sponge from 14 km, alpha_max 0.2                        damping.cpp:24-41
+//   ReferenceState::isothermal(theta 300 K, p_s 1000 hPa) on the
+//     8 x 6 x 16 grid with the Agnesi terrain                 reference_state.cpp:35-59
+#include <cstdio>
+#include <vector>
+
+#include "vibe/dyn/damping.hpp"
+#include "vibe/dyn/reference_state.hpp"
+#include "vibe/grid/geometry.hpp"
+
+using vibe::Int;
+using vibe::Real;
+namespace grid = vibe::grid;
+
+namespace {
+
+void put(const char *name, const std::vector<Real> &v, bool last = false) {
+  std::printf("  \"%s\": [", name);
+  for (size_t q = 0; q < v.size(); ++q) std::printf("%s%.17g", q ? ", " : "", v[q]);
+  std::printf("]%s\n", last ? "" : ",");
+}
+
+}  // namespace
+
+int main() {
+  const Int nx = 8, ny = 6, nz = 16;
+  const Real dx = 1000.0, dy = 1500.0, ztop = 20000.0;
+  std::printf("{\n");
+  const std::vector<Real> z16 = grid::make_stretched_zeta(nz, 1.0, 1.08);
+  put("zeta_16_1_1.08", z16);
+  put("zeta_40_1_1.05", grid::make_stretched_zeta(40, 1.0, 1.05));
+  std::vector<Real> zs = grid::witch_of_agnesi(nx, ny, dx, dy, 800.0, 3000.0, 3000.0, 4000.0);
+  put("agnesi_8x6", zs);
+  std::vector<Real> sm = zs;
+  grid::smooth_terrain(sm, nx, ny, 2, 50.0);
+  put("agnesi_8x6_smoothed_2_50", sm);
+
+  grid::Geometry flat;
+  flat.nx = nx;
+  flat.ny = ny;
+  flat.nz = nz;
+  flat.dx = dx;
+  flat.dy = dy;
+  flat.z_top = ztop;
+  flat.zeta = z16;
+  flat.name = "golden";
+  const grid::Grid gflat(flat, grid::Decomposition::make(nx, ny, nz, 1, 1, 2, true, true));
+  const vibe::dyn::Damping damp(gflat, 14000.0, ztop, 0.2);
+  put("sponge_16_14km_0.2", damp.sponge_profile());
+
+  grid::Geometry hill = flat;
+  hill.zs = zs;
+  const grid::Grid ghill(hill, grid::Decomposition::make(nx, ny, nz, 1, 1, 2, true, true));
+  const auto rs = vibe::dyn::ReferenceState::isothermal(ghill, 300.0, 100000.0, 0.0);
+  std::vector<Real> th, pi, rho;
+  for (Int k = 0; k < nz; ++k)
+    for (Int j = 0; j < ny; ++j)
+      for (Int i = 0; i < nx; ++i) {
+        th.push_back(rs.theta0()(i, j, k));
+        pi.push_back(rs.pi0()(i, j, k));
+        rho.push_back(rs.rho0()(i, j, k));
+      }
+  put("isothermal_theta0", th);
+  put("isothermal_pi0", pi);
+  put("isothermal_rho0", rho, true);
+  std::printf("}\n");
+  return 0;
+}

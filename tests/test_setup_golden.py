@@ -1,0 +1,65 @@
+"""The library's host setup helpers against golden vectors computed by the
+reference's own code (tests/golden/setup_reference.json, written by
+tests/golden/make_setup_golden.py from src/grid/geometry.cpp,
+src/dyn/damping.cpp and src/dyn/reference_state.cpp): stretched vertical
+coordinate, radial Witch of Agnesi, terrain smoothing, sponge profile and
+the isothermal reference state, bit for bit.  Host code only (no GPU)."""
+import ctypes
+import json
+import os
+
+import numpy as np
+import pytest
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "setup_reference.json")
+
+
+@pytest.fixture(scope="module")
+def gold():
+    with open(GOLD) as fh:
+        return {k: np.array(v) for k, v in json.load(fh).items()}
+
+
+@pytest.fixture(scope="module")
+def L():
+    import vibe_mi
+    return vibe_mi.lib(), vibe_mi._pd
+
+
+def test_stretched_zeta(L, gold):
+    lib, pd = L
+    for nz, h, s, key in ((16, 1.0, 1.08, "zeta_16_1_1.08"), (40, 1.0, 1.05, "zeta_40_1_1.05")):
+        z = np.empty(nz + 1)
+        assert lib.vibe_mi_zeta_stretched(nz, h, s, pd(z)) == 0
+        assert np.array_equal(z, gold[key]), key
+
+
+def test_agnesi_bell_and_smoothing(L, gold):
+    lib, pd = L
+    zs = np.empty(8 * 6)
+    assert lib.vibe_mi_terrain_agnesi_bell(8, 6, 1000.0, 1500.0, 800.0, 3000.0, 3000.0, 4000.0, pd(zs)) == 0
+    assert np.array_equal(zs, gold["agnesi_8x6"])
+    assert lib.vibe_mi_terrain_smooth(pd(zs), 8, 6, 2, 50.0) == 0
+    assert np.array_equal(zs, gold["agnesi_8x6_smoothed_2_50"])
+
+
+def test_sponge_profile(L, gold):
+    lib, pd = L
+    z = gold["zeta_16_1_1.08"].copy()
+    a = np.empty(16)
+    # Damping(grid, z_sponge_start = 14 km, z_top = 20 km, alpha_max = 0.2) on flat terrain
+    assert lib.vibe_mi_sponge(16, 20000.0, pd(z), 0.0, 0.7, 0.2, pd(a)) == 0
+    assert np.array_equal(a, gold["sponge_16_14km_0.2"])
+
+
+def test_isothermal_reference_state(L, gold):
+    lib, pd = L
+    nx, ny, nz = 8, 6, 16
+    z = gold["zeta_16_1_1.08"].copy()
+    zs = gold["agnesi_8x6"].copy()
+    r0, t0, p0 = (np.empty(nx * ny * nz) for _ in range(3))
+    assert lib.vibe_mi_reference_state(0, 300.0, 0.0, 100000.0, nx, ny, nz, 20000.0, pd(z), pd(zs),
+                                       pd(r0), pd(t0), pd(p0)) == 0
+    assert np.array_equal(t0, gold["isothermal_theta0"])
+    assert np.array_equal(p0, gold["isothermal_pi0"])
+    assert np.array_equal(r0, gold["isothermal_rho0"])

@@ -94,6 +94,64 @@ int vibe_mi_terrain_agnesi_ridge(int nx, int ny, double dx, double h, double a, 
   return 0;
 }
 
+int vibe_mi_zeta_stretched(int nz, double first_thickness, double stretch, double *zeta) {
+  if (nz <= 0 || zeta == nullptr || !(first_thickness > 0.0) || !(stretch > 0.0)) return 2;
+  // thicknesses h, h s, h s^2, ... (running product, as accumulated in
+  // geometry.cpp:46-51), interfaces = running sums / total
+  std::vector<double> dz(nz);
+  double total = 0.0, h = first_thickness;
+  for (int k = 0; k < nz; ++k) {
+    dz[k] = h;
+    total += h;
+    h *= stretch;
+  }
+  zeta[0] = 0.0;
+  double acc = 0.0;
+  for (int k = 0; k < nz; ++k) {
+    acc += dz[k];
+    zeta[k + 1] = acc / total;
+  }
+  zeta[nz] = 1.0;
+  return 0;
+}
+
+int vibe_mi_terrain_agnesi_bell(int nx, int ny, double dx, double dy, double h0, double a, double xc, double yc,
+                                double *zs) {
+  if (nx <= 0 || ny <= 0 || zs == nullptr) return 2;
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      const double x = static_cast<double>(i) * dx - xc, y = static_cast<double>(j) * dy - yc;
+      zs[static_cast<size_t>(j) * nx + i] = h0 * a * a / (a * a + (x * x + y * y));
+    }
+  return 0;
+}
+
+int vibe_mi_terrain_smooth(double *zs, int nx, int ny, int passes, double max_slope) {
+  if (zs == nullptr || nx <= 0 || ny <= 0 || passes < 0) return 2;
+  const size_t sx = static_cast<size_t>(nx);
+  std::vector<double> f(static_cast<size_t>(nx) * ny);
+  for (int p = 0; p < passes; ++p) {
+    for (int j = 1; j < ny - 1; ++j)
+      for (int i = 1; i < nx - 1; ++i) {
+        const size_t c = static_cast<size_t>(j) * sx + i;
+        f[c] = 0.25 * (zs[c - 1] + zs[c + 1] + zs[c - sx] + zs[c + sx]) + 0.5 * zs[c];
+      }
+    for (int j = 1; j < ny - 1; ++j)
+      for (int i = 1; i < nx - 1; ++i) {
+        const size_t c = static_cast<size_t>(j) * sx + i;
+        zs[c] = std::min(zs[c], f[c]);
+      }
+    if (max_slope > 0.0)
+      for (int j = 1; j < ny - 1; ++j)
+        for (int i = 1; i < nx - 1; ++i) {
+          const size_t c = static_cast<size_t>(j) * sx + i;
+          const double top = std::max(std::max(zs[c - 1], zs[c + 1]), std::max(zs[c - sx], zs[c + sx]));
+          zs[c] = std::min(zs[c], top + max_slope);
+        }
+  }
+  return 0;
+}
+
 int vibe_mi_terrain_gaussian(int nx, int ny, double dx, double dy, double sigma, double L,
                              uint64_t seed, double *zs) {
   if (nx <= 0 || ny <= 0 || zs == nullptr || L <= 0.0) return 2;

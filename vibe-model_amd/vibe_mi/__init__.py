@@ -161,6 +161,9 @@ def lib() -> ctypes.CDLL:
         "vibe_mi_zeta_uniform": (_I, [_I, _PD]),
         "vibe_mi_zeta_tanh": (_I, [_I, _D, _D, _D, _PD]),
         "vibe_mi_terrain_agnesi_ridge": (_I, [_I, _I, _D, _D, _D, _PD]),
+        "vibe_mi_zeta_stretched": (_I, [_I, _D, _D, _PD]),
+        "vibe_mi_terrain_agnesi_bell": (_I, [_I, _I, _D, _D, _D, _D, _D, _D, _PD]),
+        "vibe_mi_terrain_smooth": (_I, [_PD, _I, _I, _I, _D]),
         "vibe_mi_terrain_gaussian": (_I, [_I, _I, _D, _D, _D, _D, ctypes.c_uint64, _PD]),
         "vibe_mi_reference_state": (_I, [_I, _D, _D, _D, _I, _I, _I, _D, _PD, _PD, _PD, _PD, _PD]),
         "vibe_mi_sponge": (_I, [_I, _D, _PD, _D, _D, _D, _PD]),
