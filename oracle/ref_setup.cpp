@@ -13,10 +13,17 @@
 //     sponge from 14 km, alpha_max 0.2                        damping.cpp:24-41
 //   ReferenceState::isothermal(theta 300 K, p_s 1000 hPa) on the
 //     8 x 6 x 16 grid with the Agnesi terrain                 reference_state.cpp:35-59
+//   energy_budget of a dry state on that grid and reference state
+//     (diagnostics.cpp:255-277): u = 5 + i/4 + j/2 - k/8 (face nx = face 0),
+//     v = -2 + i/10 - j/5 + k/20 (face ny = face 0), w = (i - j)/100 + k/500,
+//     rho' = 1e-3 (i - j + k) / 16, theta' = (i + 2 j - k) / 8,
+//     pi' = 1e-4 (2 i - j + k) / 16; totals = reference + perturbation
 #include <cstdio>
 #include <vector>
 
 #include "vibe/dyn/damping.hpp"
+#include "vibe/dyn/diagnostics.hpp"
+#include "vibe/dyn/state.hpp"
 #include "vibe/dyn/reference_state.hpp"
 #include "vibe/grid/geometry.hpp"
 
@@ -74,7 +81,24 @@ int main() {
       }
   put("isothermal_theta0", th);
   put("isothermal_pi0", pi);
-  put("isothermal_rho0", rho, true);
+  put("isothermal_rho0", rho);
+
+  vibe::dyn::State st(ghill);
+  for (Int k = 0; k <= nz; ++k)
+    for (Int j = 0; j <= ny; ++j)
+      for (Int i = 0; i <= nx; ++i) {
+        const Int iw = i == nx ? 0 : i, js = j == ny ? 0 : j;
+        if (k < nz && j < ny) st.u()(i, j, k) = 5.0 + 0.25 * iw + 0.5 * j - 0.125 * k;
+        if (k < nz && i < nx) st.v()(i, j, k) = -2.0 + 0.1 * i - 0.2 * js + 0.05 * k;
+        if (i < nx && j < ny) st.w()(i, j, k) = 0.01 * (i - j) + 0.002 * k;
+        if (i < nx && j < ny && k < nz) {
+          st.rho()(i, j, k) = rs.rho0()(i, j, k) + 1e-3 * (i - j + k) / 16.0;
+          st.theta()(i, j, k) = rs.theta0()(i, j, k) + (i + 2 * j - k) / 8.0;
+          st.pi()(i, j, k) = rs.pi0()(i, j, k) + 1e-4 * (2 * i - j + k) / 16.0;
+        }
+      }
+  const vibe::dyn::EnergyBudget e = vibe::dyn::energy_budget(st, rs);
+  put("energy_mass_kinetic_internal_potential_total", {e.mass, e.kinetic, e.internal, e.potential, e.total}, true);
   std::printf("}\n");
   return 0;
 }

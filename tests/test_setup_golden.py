@@ -63,3 +63,29 @@ def test_isothermal_reference_state(L, gold):
     assert np.array_equal(t0, gold["isothermal_theta0"])
     assert np.array_equal(p0, gold["isothermal_pi0"])
     assert np.array_equal(r0, gold["isothermal_rho0"])
+
+
+def test_energy_budget_definitions_match_reference(gold):
+    """The oracle's compensated totals (the GPU's checker) on the state of
+    oracle/ref_setup.cpp against the reference's energy_budget
+    (diagnostics.cpp:255-277; plain sums, hence 1e-13 relative)."""
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import vibe_mi
+    from oracle_bind import OracleDycore
+    nx, ny, nz = 8, 6, 16
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    kw, jw, iw = np.meshgrid(np.arange(nz + 1), np.arange(ny), np.arange(nx), indexing="ij")
+    st = {"u": (5.0 + 0.25 * i + 0.5 * j - 0.125 * k).reshape(-1),
+          "v": (-2.0 + 0.1 * i - 0.2 * j + 0.05 * k).reshape(-1),
+          "w": (0.01 * (iw - jw) + 0.002 * kw).reshape(-1),
+          "rho": (1e-3 * (i - j + k) / 16.0).reshape(-1),
+          "theta": ((i + 2 * j - k) / 8.0).reshape(-1),
+          "pi": (1e-4 * (2 * i - j + k) / 16.0).reshape(-1)}
+    case = vibe_mi.Case("golden", nx, ny, nz, 1000.0, 1500.0, 20000.0, gold["zeta_16_1_1.08"],
+                        gold["agnesi_8x6"], gold["isothermal_rho0"], gold["isothermal_theta0"],
+                        gold["isothermal_pi0"], np.zeros(nz), st, 1.0, 1, 1)
+    e = OracleDycore(case).energy()
+    ref = gold["energy_mass_kinetic_internal_potential_total"]
+    got = np.array([e[0], e[2], e[3], e[4], e[5]])
+    assert np.all(np.abs(got - ref) <= 1e-13 * np.abs(ref)), (got, ref)
