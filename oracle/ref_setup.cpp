@@ -18,9 +18,11 @@
 //     v = -2 + i/10 - j/5 + k/20 (face ny = face 0), w = (i - j)/100 + k/500,
 //     rho' = 1e-3 (i - j + k) / 16, theta' = (i + 2 j - k) / 8,
 //     pi' = 1e-4 (2 i - j + k) / 16; totals = reference + perturbation
+//   Advection::cfl_number of that state for dt = 6 s     advection.cpp:325-340
 #include <cstdio>
 #include <vector>
 
+#include "vibe/dyn/advection.hpp"
 #include "vibe/dyn/damping.hpp"
 #include "vibe/dyn/diagnostics.hpp"
 #include "vibe/dyn/state.hpp"
@@ -98,7 +100,9 @@ int main() {
         }
       }
   const vibe::dyn::EnergyBudget e = vibe::dyn::energy_budget(st, rs);
-  put("energy_mass_kinetic_internal_potential_total", {e.mass, e.kinetic, e.internal, e.potential, e.total}, true);
+  put("energy_mass_kinetic_internal_potential_total", {e.mass, e.kinetic, e.internal, e.potential, e.total});
+  const vibe::dyn::Advection adv(ghill, vibe::dyn::AdvectionScheme::Central2);  // the CFL scan is the same for every scheme
+  put("cfl_dt6", {adv.cfl_number(st, 6.0)}, true);
   std::printf("}\n");
   return 0;
 }

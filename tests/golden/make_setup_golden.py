@@ -1,9 +1,10 @@
 """Writes tests/golden/setup_reference.json with the reference's own grid
-generators, sponge profile, isothermal reference state and energy budget.
+generators, sponge profile, isothermal reference state, energy budget and
+advective CFL number.
 
 oracle/ref.mk compiles the reference's src/grid/geometry.cpp,
-src/dyn/damping.cpp, src/dyn/reference_state.cpp, src/dyn/diagnostics.cpp
-and src/dyn/state.cpp (unmodified, from where
+src/dyn/damping.cpp, src/dyn/reference_state.cpp, src/dyn/diagnostics.cpp,
+src/dyn/state.cpp and src/dyn/advection.cpp (unmodified, from where
 they lie under /root/reference) with oracle/ref_setup.cpp into
 oracle/_ref/ref_setup, which prints the JSON (cases listed in
 oracle/ref_setup.cpp).
