@@ -125,6 +125,12 @@ int vibe_mi_domain_set_physics_callback(vibe_mi_domain *d, vibe_mi_physics_fn fn
 int vibe_mi_domain_energy(vibe_mi_domain *d, double out[6]);
 /* Advective CFL of the current state (advection.cpp:325-340). */
 int vibe_mi_domain_cfl(vibe_mi_domain *d, double dt, double *cfl, double *max_wind);
+/* Stability report of the current state for a step dt
+ * (TimeStepController::inspect, timestep_control.cpp:23-56): out =
+ * {advective CFL, acoustic CFL, vertical CFL, max wind speed, min dz,
+ * stable}, stable = 1 when the state is finite, the advective CFL < 1.5 and
+ * the vertical CFL < 2, else 0. */
+int vibe_mi_domain_stability(vibe_mi_domain *d, double dt, double out[6]);
 /* hipStream_t of the domain, as void*. */
 void *vibe_mi_domain_stream(vibe_mi_domain *d);
 /* Bytes of device memory the handle holds. */

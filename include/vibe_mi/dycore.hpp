@@ -5,6 +5,7 @@
 //   vibe::timeint::Integrator::step     include/vibe/time/integrator.hpp:62-82
 //   vibe::timeint::RungeKutta3Integrator  include/vibe/time/runge_kutta.hpp:46-81
 //   vibe::dyn::energy_budget            src/dyn/diagnostics.cpp:255-277
+//   vibe::timeint::TimeStepController::inspect  src/time/timestep_control.cpp:23-56
 //   vibe::NumericalError                include/vibe/common/error.hpp:35
 //   Integrator::set_physics + RungeKutta3Integrator::slow_tendency's physics
 //   coupling                            include/vibe/time/integrator.hpp:74,
@@ -104,6 +105,13 @@ struct DomainSpec {
   int precision = VIBE_MI_FP64;
 };
 
+/// Same fields as the reference's StabilityReport (timestep_control.hpp:32-42)
+/// that a dry state defines.
+struct StabilityReport {
+  double cfl_advection = 0, cfl_acoustic = 0, cfl_vertical = 0, max_wind = 0, min_dz = 0;
+  bool stable = true;
+};
+
 struct EnergyBudget {
   double mass = 0, mass_perturbation = 0, kinetic = 0, internal = 0, potential = 0, total = 0;
 };
@@ -173,6 +181,12 @@ class RungeKutta3Integrator final : public Integrator {
     double e[6];
     check(vibe_mi_domain_energy(h_, e), "vibe_mi_domain_energy");
     return {e[0], e[1], e[2], e[3], e[4], e[5]};
+  }
+  /// TimeStepController::inspect role (timestep_control.cpp:23-56).
+  StabilityReport inspect(double dt) const {
+    double o[6];
+    check(vibe_mi_domain_stability(h_, dt, o), "vibe_mi_domain_stability");
+    return {o[0], o[1], o[2], o[3], o[4], o[5] != 0.0};
   }
   std::pair<double, double> cfl(double dt) const {
     double c = 0, w = 0;

@@ -31,9 +31,9 @@ $(VBIN): ref_vibebin.cpp $(REF)/src/io/binary_writer.cpp $(REF)/src/grid/geometr
 	$(CXX) $(CXXFLAGS) -o $@ ref_vibebin.cpp $(REF)/src/io/binary_writer.cpp $(REF)/src/grid/geometry.cpp
 
 # The reference's grid generators, sponge profile, isothermal reference state
-# energy budget and CFL number (src/grid/geometry.cpp, src/dyn/damping.cpp,
-# src/dyn/reference_state.cpp, src/dyn/diagnostics.cpp, src/dyn/state.cpp,
-# src/dyn/advection.cpp)
+# energy budget, CFL number and stability report (src/grid/geometry.cpp,
+# src/dyn/damping.cpp, src/dyn/reference_state.cpp, src/dyn/diagnostics.cpp,
+# src/dyn/state.cpp, src/dyn/advection.cpp, src/time/timestep_control.cpp)
 # with this directory's ref_setup.cpp driver; prints the golden JSON of
 # tests/golden/setup_reference.json.
 SETUP = _ref/ref_setup
@@ -41,8 +41,9 @@ SETUP = _ref/ref_setup
 all: $(SETUP)
 
 $(SETUP): ref_setup.cpp $(REF)/src/grid/geometry.cpp $(REF)/src/dyn/damping.cpp $(REF)/src/dyn/reference_state.cpp \
-	  $(REF)/src/dyn/diagnostics.cpp $(REF)/src/dyn/state.cpp $(REF)/src/dyn/advection.cpp
+	  $(REF)/src/dyn/diagnostics.cpp $(REF)/src/dyn/state.cpp $(REF)/src/dyn/advection.cpp \
+	  $(REF)/src/time/timestep_control.cpp
 	mkdir -p _ref
 	$(CXX) $(CXXFLAGS) -o $@ ref_setup.cpp $(REF)/src/grid/geometry.cpp $(REF)/src/dyn/damping.cpp \
 	  $(REF)/src/dyn/reference_state.cpp $(REF)/src/dyn/diagnostics.cpp $(REF)/src/dyn/state.cpp \
-	  $(REF)/src/dyn/advection.cpp
+	  $(REF)/src/dyn/advection.cpp $(REF)/src/time/timestep_control.cpp

@@ -19,15 +19,20 @@
 //     rho' = 1e-3 (i - j + k) / 16, theta' = (i + 2 j - k) / 8,
 //     pi' = 1e-4 (2 i - j + k) / 16; totals = reference + perturbation
 //   Advection::cfl_number of that state for dt = 6 s     advection.cpp:325-340
+//   TimeStepController::inspect of that state, dt = 6 s  timestep_control.cpp:23-56
+// (acoustic_substep.cpp is not linked: it needs helmholtz.cpp, which does not
+// compile here.)
 #include <cstdio>
 #include <vector>
 
+#include "vibe/config/config.hpp"
 #include "vibe/dyn/advection.hpp"
 #include "vibe/dyn/damping.hpp"
 #include "vibe/dyn/diagnostics.hpp"
 #include "vibe/dyn/state.hpp"
 #include "vibe/dyn/reference_state.hpp"
 #include "vibe/grid/geometry.hpp"
+#include "vibe/time/timestep_control.hpp"
 
 using vibe::Int;
 using vibe::Real;
@@ -102,7 +107,14 @@ int main() {
   const vibe::dyn::EnergyBudget e = vibe::dyn::energy_budget(st, rs);
   put("energy_mass_kinetic_internal_potential_total", {e.mass, e.kinetic, e.internal, e.potential, e.total});
   const vibe::dyn::Advection adv(ghill, vibe::dyn::AdvectionScheme::Central2);  // the CFL scan is the same for every scheme
-  put("cfl_dt6", {adv.cfl_number(st, 6.0)}, true);
+  put("cfl_dt6", {adv.cfl_number(st, 6.0)});
+  vibe::config::ModelConfig cfg;
+  cfg.time.dt = 6.0;
+  const vibe::timeint::TimeStepController tsc(cfg, ghill);
+  const vibe::timeint::StabilityReport rep = tsc.inspect(st, 6.0);
+  put("inspect_dt6_adv_acoustic_vertical_maxwind_mindz_stable",
+      {rep.cfl_advection, rep.cfl_acoustic, rep.cfl_vertical, rep.max_wind, rep.min_dz, rep.stable ? 1.0 : 0.0},
+      true);
   std::printf("}\n");
   return 0;
 }

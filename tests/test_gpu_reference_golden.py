@@ -1,8 +1,10 @@
 """Device diagnostics against values computed by the reference's own code
 (tests/golden/setup_reference.json, see tests/golden/make_setup_golden.py):
-the advective CFL number (Advection::cfl_number, advection.cpp:325-340)
-bit for bit, and the energy budget (energy_budget, diagnostics.cpp:255-277)
-within 1e-13 (the reference sums plainly, the device with compensation)."""
+the advective CFL number (Advection::cfl_number, advection.cpp:325-340) and
+the stability report (TimeStepController::inspect, timestep_control.cpp:
+23-56) bit for bit, and the energy budget (energy_budget, diagnostics.cpp:
+255-277) within 1e-13 (the reference sums plainly, the device with
+compensation)."""
 import json
 import os
 
@@ -37,7 +39,12 @@ def test_cfl_and_energy_match_the_reference(vmi):
         dom = vmi.Domain(case, prec)
         cfl, _ = dom.cfl(6.0)
         e = dom.energy_budget()
+        rep = dom.stability(6.0)
         dom.close()
+        want = gold["inspect_dt6_adv_acoustic_vertical_maxwind_mindz_stable"]
+        got_rep = [rep["cfl_advection"], rep["cfl_acoustic"], rep["cfl_vertical"], rep["max_wind"], rep["min_dz"],
+                   1.0 if rep["stable"] else 0.0]
+        assert got_rep == list(want), (prec, got_rep, list(want))
         assert cfl == gold["cfl_dt6"][0], (prec, cfl, gold["cfl_dt6"][0])
         ref = gold["energy_mass_kinetic_internal_potential_total"]
         got = np.array([e[0], e[2], e[3], e[4], e[5]])

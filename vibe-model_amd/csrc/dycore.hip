@@ -553,7 +553,7 @@ int create_impl(const vibe_mi_domain_desc *ds, vibe_mi_domain *d) {
   if (int rc = dalloc(d, &d->flag, 4)) return rc;
   if (int rc = dalloc(d, &d->part, static_cast<size_t>(kRedBlocks) * kEnergyComps)) return rc;
   if (int rc = dalloc(d, &d->dres, 8)) return rc;
-  if (int rc = dalloc(d, &d->dmax, 4)) return rc;
+  if (int rc = dalloc(d, &d->dmax, 6)) return rc;
   VMI_HIP(hipMemset(d->flag, 0, 4 * sizeof(int)));
   for (auto &f : d->S) {
     VMI_HIP(hipMemset(f.u, 0, nc * 8));
@@ -850,20 +850,41 @@ int vibe_mi_domain_energy(vibe_mi_domain *d, double out[6]) {
   return 0;
 }
 
-int vibe_mi_domain_cfl(vibe_mi_domain *d, double dt, double *cfl, double *max_wind) {
-  if (d == nullptr) return VIBE_C_ERR_BAD_ARG;
+// One pass of k_cfl_state into r[6] (see the kernel).
+static int stability_pass(vibe_mi_domain *d, double dt, unsigned long long r[6]) {
   if (int rc = set_device(d)) return rc;
   const FieldSet &f = d->S[d->cur0];
-  VMI_HIP(hipMemsetAsync(d->dmax, 0, 2 * sizeof(unsigned long long), d->stream));
+  const unsigned long long init[6] = {0ull, 0ull, 0ull, 0ull, 0x7FF0000000000000ull, 0ull};  // min dz from +inf
+  VMI_HIP(hipMemcpyAsync(d->dmax, init, sizeof(init), hipMemcpyHostToDevice, d->stream));
   const unsigned blocks = std::min<size_t>(ceil_div(d->ncell, 256), 2048);
   hipLaunchKernelGGL(k_cfl_state, dim3(blocks), dim3(256), 0, d->stream, d->nx, d->ny, d->nz, d->dx, d->dy, d->ztop,
-                     d->zs, d->dzeta64, f.u, f.v, f.w, dt, d->dmax);
+                     d->zs, d->dzeta64, f.u, f.v, f.w, f.rho, f.th, f.pi, d->theta0, d->pi0, dt, d->dmax);
   VMI_HIP(hipGetLastError());
-  unsigned long long r[2];
-  VMI_HIP(hipMemcpyAsync(r, d->dmax, sizeof(r), hipMemcpyDeviceToHost, d->stream));
+  VMI_HIP(hipMemcpyAsync(r, d->dmax, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, d->stream));
   VMI_HIP(hipStreamSynchronize(d->stream));
+  return 0;
+}
+
+int vibe_mi_domain_cfl(vibe_mi_domain *d, double dt, double *cfl, double *max_wind) {
+  if (d == nullptr) return VIBE_C_ERR_BAD_ARG;
+  unsigned long long r[6];
+  if (int rc = stability_pass(d, dt, r)) return rc;
   if (cfl) *cfl = __builtin_bit_cast(double, r[0]);
   if (max_wind) *max_wind = __builtin_bit_cast(double, r[1]);
+  return 0;
+}
+
+int vibe_mi_domain_stability(vibe_mi_domain *d, double dt, double out[6]) {
+  if (d == nullptr || out == nullptr) return VIBE_C_ERR_BAD_ARG;
+  unsigned long long r[6];
+  if (int rc = stability_pass(d, dt, r)) return rc;
+  out[0] = __builtin_bit_cast(double, r[0]);  // advective CFL
+  out[1] = __builtin_bit_cast(double, r[2]);  // acoustic CFL
+  out[2] = __builtin_bit_cast(double, r[3]);  // vertical CFL
+  out[3] = __builtin_bit_cast(double, r[1]);  // max wind
+  out[4] = __builtin_bit_cast(double, r[4]);  // min dz
+  // stable: finite state, advective CFL < 1.5 and vertical CFL < 2 (timestep_control.cpp:48)
+  out[5] = (r[5] == 0 && out[0] < 1.5 && out[2] < 2.0) ? 1.0 : 0.0;
   return 0;
 }
 

@@ -1119,14 +1119,23 @@ static __global__ void __launch_bounds__(kRedThreads) k_red_final(const NSum *__
   }
 }
 
-// Advective CFL of the state (advection.cpp:325-340) and max wind speed.
+// Stability diagnostics of the state (TimeStepController::inspect,
+// timestep_control.cpp:23-56; its advective CFL is Advection::cfl_number,
+// advection.cpp:325-340), one pass: res = {advective CFL, max wind, acoustic
+// CFL, vertical CFL} as maxima, {min dz} as a minimum (slot 4, start +inf)
+// and a non-finite flag (slot 5), each an FP64 bit pattern combined with
+// integer atomics (non-negative doubles order like their bits).
 static __global__ void __launch_bounds__(256) k_cfl_state(int nx, int ny, int nz, double dx, double dy, double H,
                                                    const double *__restrict__ zs, const double *__restrict__ dzeta,
                                                    const double *__restrict__ u, const double *__restrict__ v,
-                                                   const double *__restrict__ w, double dt,
+                                                   const double *__restrict__ w, const double *__restrict__ rho,
+                                                   const double *__restrict__ th, const double *__restrict__ pi,
+                                                   const double *__restrict__ th0,
+                                                   const double *__restrict__ pi0, double dt,
                                                    unsigned long long *__restrict__ res) {
   const size_t nxy = static_cast<size_t>(nx) * ny, n = nxy * nz;
-  double worst = 0.0, wind = 0.0;
+  double worst = 0.0, wind = 0.0, cac = 0.0, cvert = 0.0, mdz = __builtin_huge_val();
+  int bad = 0;
   for (size_t c = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
        c += static_cast<size_t>(gridDim.x) * blockDim.x) {
     const size_t k = c / nxy, col = c - k * nxy;
@@ -1138,21 +1147,36 @@ static __global__ void __launch_bounds__(256) k_cfl_state(int nx, int ny, int nz
     const double ww = 0.5 * (w[c] + w[c + nxy]);
     worst = fmax(worst, dt * (fabs(uu) / dx + fabs(vv) / dy + fabs(ww) / dz));
     wind = fmax(wind, sqrt(uu * uu + vv * vv + ww * ww));
+    mdz = fmin(mdz, dz);
+    const double t = (th0[c] + th[c]) * (pi0[c] + pi[c]);
+    const double cs = sqrt((kCp / kCv) * kRd * fmax(t, 1.0));
+    cac = fmax(cac, dt * cs * (1.0 / dx + 1.0 / dy + 1.0 / dz));
+    cvert = fmax(cvert, dt * (fabs(ww) + cs) / dz);
+    bad |= !(isfinite(u[c]) && isfinite(v[c]) && isfinite(w[c]) && isfinite(rho[c]) && isfinite(th[c]) &&
+             isfinite(pi[c]));
+    if (k == static_cast<size_t>(nz) - 1) bad |= !isfinite(w[c + nxy]);
   }
-  __shared__ double s0[256], s1[256];
-  s0[threadIdx.x] = worst;
-  s1[threadIdx.x] = wind;
+  __shared__ double s[5][256];
+  __shared__ int sb[256];
+  s[0][threadIdx.x] = worst;
+  s[1][threadIdx.x] = wind;
+  s[2][threadIdx.x] = cac;
+  s[3][threadIdx.x] = cvert;
+  s[4][threadIdx.x] = mdz;
+  sb[threadIdx.x] = bad;
   __syncthreads();
   for (int hh = 128; hh > 0; hh >>= 1) {
     if (threadIdx.x < static_cast<unsigned>(hh)) {
-      s0[threadIdx.x] = fmax(s0[threadIdx.x], s0[threadIdx.x + hh]);
-      s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + hh]);
+      for (int q = 0; q < 4; ++q) s[q][threadIdx.x] = fmax(s[q][threadIdx.x], s[q][threadIdx.x + hh]);
+      s[4][threadIdx.x] = fmin(s[4][threadIdx.x], s[4][threadIdx.x + hh]);
+      sb[threadIdx.x] |= sb[threadIdx.x + hh];
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    atomicMax(&res[0], static_cast<unsigned long long>(__double_as_longlong(s0[0])));
-    atomicMax(&res[1], static_cast<unsigned long long>(__double_as_longlong(s1[0])));
+    for (int q = 0; q < 4; ++q) atomicMax(&res[q], static_cast<unsigned long long>(__double_as_longlong(s[q][0])));
+    atomicMin(&res[4], static_cast<unsigned long long>(__double_as_longlong(s[4][0])));
+    if (sb[0]) atomicMax(&res[5], 1ull);
   }
 }
 

@@ -137,6 +137,7 @@ def lib() -> ctypes.CDLL:
         "vibe_mi_domain_set_physics_callback": (_I, [_VP, PHYSICS_FN, _VP, _I]),
         "vibe_mi_domain_energy": (_I, [_VP, _PD]),
         "vibe_mi_domain_cfl": (_I, [_VP, _D, _PD, _PD]),
+        "vibe_mi_domain_stability": (_I, [_VP, _D, _PD]),
         "vibe_mi_domain_stream": (_VP, [_VP]),
         "vibe_mi_domain_device_bytes": (_SZ, [_VP]),
         "vibe_mi_batch_create": (_I, [ctypes.POINTER(DomainDesc), _I, ctypes.POINTER(_VP)]),
@@ -585,6 +586,13 @@ class Domain:
         out = np.zeros(6)
         _check(lib().vibe_mi_domain_energy(self.h, _pd(out)), "vibe_mi_domain_energy")
         return out
+
+    def stability(self, dt: float) -> dict:
+        """TimeStepController::inspect role (timestep_control.cpp:23-56)."""
+        out = np.zeros(6)
+        _check(lib().vibe_mi_domain_stability(self.h, dt, _pd(out)), "vibe_mi_domain_stability")
+        return {"cfl_advection": out[0], "cfl_acoustic": out[1], "cfl_vertical": out[2], "max_wind": out[3],
+                "min_dz": out[4], "stable": bool(out[5])}
 
     def cfl(self, dt: float):
         c, w = ctypes.c_double(), ctypes.c_double()
