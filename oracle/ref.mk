@@ -33,7 +33,8 @@ $(VBIN): ref_vibebin.cpp $(REF)/src/io/binary_writer.cpp $(REF)/src/grid/geometr
 # The reference's grid generators, sponge profile, isothermal reference state
 # energy budget, CFL number and stability report (src/grid/geometry.cpp,
 # src/dyn/damping.cpp, src/dyn/reference_state.cpp, src/dyn/diagnostics.cpp,
-# src/dyn/state.cpp, src/dyn/advection.cpp, src/time/timestep_control.cpp)
+# src/dyn/state.cpp, src/dyn/advection.cpp, src/time/timestep_control.cpp) and
+# nest operators (src/grid/nest.cpp)
 # with this directory's ref_setup.cpp driver; prints the golden JSON of
 # tests/golden/setup_reference.json.
 SETUP = _ref/ref_setup
@@ -42,8 +43,8 @@ all: $(SETUP)
 
 $(SETUP): ref_setup.cpp $(REF)/src/grid/geometry.cpp $(REF)/src/dyn/damping.cpp $(REF)/src/dyn/reference_state.cpp \
 	  $(REF)/src/dyn/diagnostics.cpp $(REF)/src/dyn/state.cpp $(REF)/src/dyn/advection.cpp \
-	  $(REF)/src/time/timestep_control.cpp
+	  $(REF)/src/time/timestep_control.cpp $(REF)/src/grid/nest.cpp
 	mkdir -p _ref
 	$(CXX) $(CXXFLAGS) -o $@ ref_setup.cpp $(REF)/src/grid/geometry.cpp $(REF)/src/dyn/damping.cpp \
 	  $(REF)/src/dyn/reference_state.cpp $(REF)/src/dyn/diagnostics.cpp $(REF)/src/dyn/state.cpp \
-	  $(REF)/src/dyn/advection.cpp $(REF)/src/time/timestep_control.cpp
+	  $(REF)/src/dyn/advection.cpp $(REF)/src/time/timestep_control.cpp $(REF)/src/grid/nest.cpp

@@ -22,6 +22,12 @@
 //   TimeStepController::inspect of that state, dt = 6 s  timestep_control.cpp:23-56
 // (acoustic_substep.cpp is not linked: it needs helmholtz.cpp, which does not
 // compile here.)
+//   nest operators (src/grid/nest.cpp:34-128) on a cell field: parent
+//     16 x 14 x 3 at 3 km, child 12 x 9 x 3 at 1 km, ratio 3, offsets
+//     (12, 15) child cells (inside the parent: no clamping), Davies zone 4;
+//     parent p = i/2 - j/4 + k/8 + i j / 100, child c = 10 + i/10 - j/5 + k;
+//     interpolate_from_parent(p) and apply_lateral_boundary(c, that), and
+//     davies_profile(4, 1) and (5, 0.8)
 #include <cstdio>
 #include <vector>
 
@@ -32,6 +38,7 @@
 #include "vibe/dyn/state.hpp"
 #include "vibe/dyn/reference_state.hpp"
 #include "vibe/grid/geometry.hpp"
+#include "vibe/grid/nest.hpp"
 #include "vibe/time/timestep_control.hpp"
 
 using vibe::Int;
@@ -113,8 +120,53 @@ int main() {
   const vibe::timeint::TimeStepController tsc(cfg, ghill);
   const vibe::timeint::StabilityReport rep = tsc.inspect(st, 6.0);
   put("inspect_dt6_adv_acoustic_vertical_maxwind_mindz_stable",
-      {rep.cfl_advection, rep.cfl_acoustic, rep.cfl_vertical, rep.max_wind, rep.min_dz, rep.stable ? 1.0 : 0.0},
-      true);
+      {rep.cfl_advection, rep.cfl_acoustic, rep.cfl_vertical, rep.max_wind, rep.min_dz, rep.stable ? 1.0 : 0.0});
+
+  auto flat_grid = [](Int gx, Int gy, Int gz, Real d) {
+    grid::Geometry g;
+    g.nx = gx;
+    g.ny = gy;
+    g.nz = gz;
+    g.dx = d;
+    g.dy = d;
+    g.z_top = 10000.0;
+    g.zeta.resize(gz + 1);
+    for (Int k = 0; k <= gz; ++k) g.zeta[k] = static_cast<Real>(k) / gz;
+    g.name = "nest";
+    return grid::Grid(g, grid::Decomposition::make(gx, gy, gz, 1, 1, 2, true, true));
+  };
+  const grid::Grid gpar = flat_grid(16, 14, 3, 3000.0), gchi = flat_grid(12, 9, 3, 1000.0);
+  grid::NestPlacement pl;
+  pl.i0 = 12;
+  pl.j0 = 15;
+  pl.ratio = 3;
+  pl.boundary_zone = 4;
+  pl.iext = 4;
+  pl.jext = 3;
+  const grid::Nest nest(gchi, gpar, pl);
+  grid::Field<Real> P(gpar, grid::Stagger::Cell, "p"), I(gchi, grid::Stagger::Cell, "i"),
+      Cf(gchi, grid::Stagger::Cell, "c");
+  for (Int k = 0; k < 3; ++k)
+    for (Int j = 0; j < 14; ++j)
+      for (Int i = 0; i < 16; ++i) P(i, j, k) = 0.5 * i - 0.25 * j + 0.125 * k + 0.01 * i * j;
+  for (Int k = 0; k < 3; ++k)
+    for (Int j = 0; j < 9; ++j)
+      for (Int i = 0; i < 12; ++i) Cf(i, j, k) = 10.0 + 0.1 * i - 0.2 * j + k;
+  std::vector<grid::Field<Real> *> ps{&P}, is{&I}, cs{&Cf};
+  nest.interpolate_from_parent(ps, is);
+  const std::vector<grid::Field<Real> *> ics{&I};
+  nest.apply_lateral_boundary(cs, ics);
+  std::vector<Real> vi, vc;
+  for (Int k = 0; k < 3; ++k)
+    for (Int j = 0; j < 9; ++j)
+      for (Int i = 0; i < 12; ++i) {
+        vi.push_back(I(i, j, k));
+        vc.push_back(Cf(i, j, k));
+      }
+  put("nest_interpolated_cell", vi);
+  put("nest_relaxed_cell_zone4", vc);
+  put("davies_4_1", grid::davies_profile(4, 1.0));
+  put("davies_5_0.8", grid::davies_profile(5, 0.8), true);
   std::printf("}\n");
   return 0;
 }

@@ -1,10 +1,11 @@
 """Writes tests/golden/setup_reference.json with the reference's own grid
-generators, sponge profile, isothermal reference state, energy budget and
-advective CFL number.
+generators, sponge profile, isothermal reference state, energy budget,
+advective CFL number, stability report and nest operators.
 
 oracle/ref.mk compiles the reference's src/grid/geometry.cpp,
 src/dyn/damping.cpp, src/dyn/reference_state.cpp, src/dyn/diagnostics.cpp,
-src/dyn/state.cpp and src/dyn/advection.cpp (unmodified, from where
+src/dyn/state.cpp, src/dyn/advection.cpp, src/time/timestep_control.cpp and
+src/grid/nest.cpp (unmodified, from where
 they lie under /root/reference) with oracle/ref_setup.cpp into
 oracle/_ref/ref_setup, which prints the JSON (cases listed in
 oracle/ref_setup.cpp).

@@ -49,3 +49,34 @@ def test_cfl_and_energy_match_the_reference(vmi):
         ref = gold["energy_mass_kinetic_internal_potential_total"]
         got = np.array([e[0], e[2], e[3], e[4], e[5]])
         assert np.all(np.abs(got - ref) <= 1e-13 * np.abs(ref)), (prec, got, ref)
+
+
+def _flat_case(vmi, nx, ny, nz, dx, state):
+    zeta = np.linspace(0.0, 1.0, nz + 1)
+    zs = np.zeros(nx * ny)
+    r0, t0, p0 = (np.empty(nx * ny * nz) for _ in range(3))
+    assert vmi.lib().vibe_mi_reference_state(0, 300.0, 0.0, 1e5, nx, ny, nz, 10000.0, vmi._pd(zeta), vmi._pd(zs),
+                                             vmi._pd(r0), vmi._pd(t0), vmi._pd(p0)) == 0
+    return vmi.Case("nest", nx, ny, nz, dx, dx, 10000.0, zeta, zs, r0, t0, p0, np.zeros(nz), state, 1.0, 1, 1)
+
+
+def test_device_nest_operators_match_reference(vmi):
+    """vibe_mi_nest_interpolate / vibe_mi_nest_relax (frac 1) against the
+    reference's Nest::interpolate_from_parent and apply_lateral_boundary
+    (nest.cpp:52-128) on the cell-centred fields, bit for bit."""
+    from test_setup_golden import _nest_states
+    with open(GOLD) as fh:
+        gold = {k: np.array(v) for k, v in json.load(fh).items()}
+    par, chi = _nest_states()
+    P = vmi.Domain(_flat_case(vmi, 16, 14, 3, 3000.0, par), vmi.FP64)
+    C = vmi.Domain(_flat_case(vmi, 12, 9, 3, 1000.0, chi), vmi.FP64)
+    d = vmi.NestDesc(3, 12, 15, 4, 1.0)
+    vmi.nest_relax(C, P, d, 1.0)
+    relaxed = C.get_state()
+    vmi.nest_interpolate(C, P, d)
+    interp = C.get_state()
+    P.close()
+    C.close()
+    for f in ("rho", "theta", "pi"):
+        assert np.array_equal(interp[f], gold["nest_interpolated_cell"]), f
+        assert np.array_equal(relaxed[f], gold["nest_relaxed_cell_zone4"]), f

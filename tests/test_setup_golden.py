@@ -89,3 +89,41 @@ def test_energy_budget_definitions_match_reference(gold):
     ref = gold["energy_mass_kinetic_internal_potential_total"]
     got = np.array([e[0], e[2], e[3], e[4], e[5]])
     assert np.all(np.abs(got - ref) <= 1e-13 * np.abs(ref)), (got, ref)
+
+
+def _nest_states():
+    """Parent (16x14x3) and child (12x9x3) states of oracle/ref_setup.cpp's
+    nest case: every field of the parent holds p, of the child c."""
+    k, j, i = np.meshgrid(np.arange(3), np.arange(14), np.arange(16), indexing="ij")
+    p = 0.5 * i - 0.25 * j + 0.125 * k + 0.01 * i * j
+    k2, j2, i2 = np.meshgrid(np.arange(3), np.arange(9), np.arange(12), indexing="ij")
+    c = 10.0 + 0.1 * i2 - 0.2 * j2 + k2
+    def st(a, nxy):
+        out = {f: a.reshape(-1).copy() for f in ("u", "v", "rho", "theta", "pi")}
+        out["w"] = np.concatenate([a.reshape(-1), a[-1].reshape(-1)])
+        return out
+    return st(p, 16 * 14), st(c, 12 * 9)
+
+
+def test_oracle_nest_operators_match_reference(gold):
+    """oracle/nest.c against the reference's Nest::interpolate_from_parent
+    and apply_lateral_boundary (nest.cpp:52-128) on the cell-centred fields
+    (u and v sit at their own faces here, at cell centres in the reference)."""
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle_bind import oracle_nest_apply
+    par, chi = _nest_states()
+    interp = oracle_nest_apply(par, None, 16, 14, chi, 12, 9, 3, 3, 12, 15, 4, 1.0, 1.0, 1)
+    relax = oracle_nest_apply(par, None, 16, 14, chi, 12, 9, 3, 3, 12, 15, 4, 1.0, 1.0, 0)
+    for f in ("rho", "theta", "pi"):
+        assert np.array_equal(interp[f], gold["nest_interpolated_cell"]), f
+        assert np.array_equal(relax[f], gold["nest_relaxed_cell_zone4"]), f
+    assert np.array_equal(interp["w"][:12 * 9 * 3], gold["nest_interpolated_cell"])
+
+
+def test_davies_profile_matches_reference(L, gold):
+    lib, pd = L
+    for zone, am, key in ((4, 1.0, "davies_4_1"), (5, 0.8, "davies_5_0.8")):
+        a = np.empty(zone)
+        assert lib.vibe_mi_davies_profile(zone, am, pd(a)) == 0
+        assert np.array_equal(a, gold[key]), key
