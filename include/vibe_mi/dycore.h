@@ -39,6 +39,12 @@ extern "C" {
 #define VIBE_MI_FP64 0  /* plain FP64: storage, compute and reductions   */
 #define VIBE_MI_MIXED 1 /* FP64 state + RK3 accumulation, FP32 compute,  */
                         /* FP64 compensated reductions                   */
+/* FP32-compensated: VIBE_MI_MIXED whose FP32 acoustic (substep) state also
+ * keeps a bfloat16 remainder per value (hi + lo, about 32 significant
+ * bits), and whose substeps add FP32 increments to it with an exact two-sum
+ * (w by its increment): the rounding of the stored state no longer
+ * accumulates over the substeps, at 6 instead of 4 bytes per stored value. */
+#define VIBE_MI_MIXED_COMP 2
 
 typedef struct {
   int nx, ny, nz;
@@ -50,7 +56,7 @@ typedef struct {
   const double *pi0;    /* nz*ny*nx reference Exner function            */
   const double *sponge; /* nz Rayleigh rates (1/s)                      */
   double eps_s;         /* HEVI off-centering (0 = Crank-Nicolson)      */
-  int precision;        /* VIBE_MI_FP64 or VIBE_MI_MIXED                */
+  int precision;        /* VIBE_MI_FP64, _MIXED or _MIXED_COMP          */
 } vibe_mi_domain_desc;
 
 typedef struct vibe_mi_domain vibe_mi_domain;

@@ -102,7 +102,7 @@ struct DomainSpec {
   double dx = 0, dy = 0, ztop = 0;
   std::vector<double> zeta, zs, rho0, theta0, pi0, sponge;
   double eps_s = 0.1;
-  int precision = VIBE_MI_FP64;
+  int precision = VIBE_MI_FP64;  // VIBE_MI_FP64, VIBE_MI_MIXED or VIBE_MI_MIXED_COMP
 };
 
 /// Same fields as the reference's StabilityReport (timestep_control.hpp:32-42)

@@ -64,11 +64,12 @@ def test_domain_create_rejects_fields_of_4_gib():
 
 
 @pytest.mark.parametrize("nx,ny,nz,prec", [(3, 8, 10, 0), (8, 3, 10, 0), (8, 8, 2, 0), (8, 8, 104, 0),
-                                           (8, 8, 129, 1)])
+                                           (8, 8, 129, 1), (8, 8, 129, 2), (8, 8, 10, 3), (8, 8, 10, -1)])
 def test_domain_create_rejects_sizes_outside_the_kernels_range(nx, ny, nz, prec):
     """nx, ny >= 4 (3-cell periodic stencils) and 3 <= nz <= 103 (FP64) or
-    128 (FP32-mixed), the tallest column one substep workgroup takes; refused
-    before any allocation."""
+    128 (FP32-mixed and FP32-compensated), the tallest column one substep
+    workgroup takes; precision one of VIBE_MI_FP64 (0), VIBE_MI_MIXED (1),
+    VIBE_MI_MIXED_COMP (2); refused before any allocation."""
     import ctypes
     import numpy as np
     import vibe_mi

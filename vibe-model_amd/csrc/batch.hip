@@ -580,6 +580,8 @@ int vibe_mi_batch_create(const vibe_mi_domain_desc *descs, int ndom, vibe_mi_bat
   *out = nullptr;
   for (int m = 1; m < ndom; ++m)
     if (descs[m].precision != descs[0].precision) return VIBE_C_ERR_BAD_ARG;
+  // the compensated FP32 mode runs on single domains only
+  if (descs[0].precision != VIBE_MI_FP64 && descs[0].precision != VIBE_MI_MIXED) return VIBE_C_ERR_BAD_ARG;
   auto *b = new vibe_mi_batch();
   b->precision = descs[0].precision;
   auto fail = [&](int rc) {

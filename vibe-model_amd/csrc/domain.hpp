@@ -21,6 +21,10 @@ template <typename T>
 struct AcousticSet {
   T *u = nullptr, *v = nullptr, *w = nullptr, *rho = nullptr, *pi = nullptr;
 };
+// bf16 remainders of an FP32 acoustic set (compensated FP32-mixed mode)
+struct AcousticLo {
+  uint16_t *u = nullptr, *v = nullptr, *w = nullptr, *rho = nullptr, *pi = nullptr;
+};
 
 
 }  // namespace vibe_mi
@@ -50,6 +54,7 @@ struct vibe_mi_domain {
   int cur0 = 0;   // index of s^n
   AcousticSet<double> A64[2];
   AcousticSet<float> A32[2];
+  vibe_mi::AcousticLo L32[2];  // compensated mode: remainders of A32
   void *tab = nullptr;  // HEVI tables: 4 slots of hevi_slot_elems(nz, rk) entries (T)
   double tab_dtau[4] = {-1.0, -1.0, -1.0, -1.0};
   int nxr = 1, nyr = 1, rk = 1;

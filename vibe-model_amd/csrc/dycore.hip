@@ -176,7 +176,11 @@ int run_stage(vibe_mi_domain *d, const Geo<T> &g, const FieldSet &cur, const Fie
               AcousticSet<T> &a, FieldSet &nxt) {
   const StageIn ci{cur.u, cur.v, cur.w, cur.rho, cur.th, cur.pi};
   const StageIn si{s0.u, s0.v, s0.w, s0.rho, s0.th, s0.pi};
-  const StageOut<T> so{a.u, a.v, a.w, a.rho, a.pi, nxt.th};
+  StageOut<T> so{a.u, a.v, a.w, a.rho, a.pi, nxt.th};
+  if (sizeof(T) == 4 && d->precision == VIBE_MI_MIXED_COMP) {
+    const AcousticLo &l = d->L32[0];
+    so.ul = l.u, so.vl = l.v, so.wl = l.w, so.rl = l.rho, so.pl = l.pi;
+  }
   const dim3 grid(ceil_div(d->nx, 64), ceil_div(d->ny, 4)), block(64, 4);
   if (d->ph_on)
     hipLaunchKernelGGL((k_stage_col<T, T, 4, true>), grid, block, 0, d->stream, g, ci, si, h, so, d->flag,
@@ -252,31 +256,51 @@ int ensure_table(vibe_mi_domain *d, const Geo<T> &g, int slot, double dtau) {
 
 constexpr size_t kMaxLds = 160 * 1024;  // LDS per CU (and per workgroup) on gfx950
 
-template <typename T, typename I, typename O, int NW, int MAXCH>
-int launch_substep(vibe_mi_domain *d, const Geo<T> &g, const T *tab, const I *u, const I *v, const I *w,
-                   const I *rho, const I *pi, O *uo, O *vo, O *wo, O *rho_o, O *pi_o, T dtau, T an, T ao,
-                   int check) {
+template <typename T, typename I, typename O, int NW, int MAXCH, bool CP>
+int launch_substep_k(vibe_mi_domain *d, const Geo<T> &g, const SubArgs<T, I, O> &sa) {
   const size_t lds = 3 * static_cast<size_t>(d->nz - 1) * 64 * sizeof(T);
-  const size_t lds_static = 3 * NW * 64 * sizeof(T);
+  // s_mc, s_fh (FP64 in the compensated mode) and s_ph
+  const size_t lds_static = ((CP ? 2 * sizeof(double) : 2 * sizeof(T)) + sizeof(T)) * NW * 64;
   if (lds + lds_static > kMaxLds) return VIBE_C_ERR_BAD_ARG;  // column too tall for one workgroup
   static bool attr_set = false;  // once per instantiation
   if (!attr_set) {
-    VMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_substep<T, I, O, NW, MAXCH>),
+    VMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_substep<T, I, O, NW, MAXCH, CP>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kMaxLds - lds_static)));
     attr_set = true;
   }
-  const SubArgs<T, I, O> sa{u, v, w, rho, pi, HeviTab<T>::at(tab, d->nz, d->rk), dtau, an, ao,
-                            uo, vo, wo, rho_o, pi_o, check, d->flag};
-  hipLaunchKernelGGL((k_substep<T, I, O, NW, MAXCH>), dim3(ceil_div(d->nx, 64), d->ny), dim3(64 * NW), lds, d->stream,
-                     g, sa);
+  hipLaunchKernelGGL((k_substep<T, I, O, NW, MAXCH, CP>), dim3(ceil_div(d->nx, 64), d->ny), dim3(64 * NW), lds,
+                     d->stream, g, sa);
   return 0;
+}
+
+// li / lo: remainders of the input and (FP32 output) output set in the
+// compensated mode, nullptr otherwise.
+template <typename T, typename I, typename O, int NW, int MAXCH>
+int launch_substep(vibe_mi_domain *d, const Geo<T> &g, const T *tab, const I *u, const I *v, const I *w,
+                   const I *rho, const I *pi, O *uo, O *vo, O *wo, O *rho_o, O *pi_o, T dtau, T an, T ao,
+                   int check, const AcousticLo *li, const AcousticLo *lo) {
+  SubArgs<T, I, O> sa{u, v, w, rho, pi, HeviTab<T>::at(tab, d->nz, d->rk), dtau, an, ao,
+                      uo, vo, wo, rho_o, pi_o, check, d->flag};
+  if constexpr (sizeof(T) == 4 && sizeof(I) == 4) {
+    if (li != nullptr) {
+      sa.ul = li->u, sa.vl = li->v, sa.wl = li->w, sa.rl = li->rho, sa.pl = li->pi;
+      if (lo != nullptr) sa.ulo = lo->u, sa.vlo = lo->v, sa.wlo = lo->w, sa.rlo = lo->rho, sa.plo = lo->pi;
+      const Geo<double> &g8 = d->g64;
+      sa.rhu64 = g8.rhu, sa.rhv64 = g8.rhv, sa.r0w64 = g8.r0w, sa.sx64 = g8.sx, sa.sy64 = g8.sy, sa.zeta64 = g8.zeta;
+      sa.rdx64 = g8.rdx, sa.rdy64 = g8.rdy;
+      return launch_substep_k<T, I, O, NW, MAXCH, true>(d, g, sa);
+    }
+  }
+  return launch_substep_k<T, I, O, NW, MAXCH, false>(d, g, sa);
 }
 
 template <typename T, typename I, typename O, int NW>
 int dispatch_substep(vibe_mi_domain *d, const Geo<T> &g, const T *tab, const I *u, const I *v, const I *w,
-                     const I *rho, const I *pi, O *uo, O *vo, O *wo, O *rho_o, O *pi_o, T dtau, T an, T ao, int check) {
+                     const I *rho, const I *pi, O *uo, O *vo, O *wo, O *rho_o, O *pi_o, T dtau, T an, T ao, int check,
+                     const AcousticLo *li, const AcousticLo *lo) {
   const int ch = (d->nz + NW - 1) / NW;
-#define VMI_SUB(M) launch_substep<T, I, O, NW, M>(d, g, tab, u, v, w, rho, pi, uo, vo, wo, rho_o, pi_o, dtau, an, ao, check)
+#define VMI_SUB(M) \
+  launch_substep<T, I, O, NW, M>(d, g, tab, u, v, w, rho, pi, uo, vo, wo, rho_o, pi_o, dtau, an, ao, check, li, lo)
   int rc;
   if constexpr (NW == 16) {
     if (ch <= 4) rc = VMI_SUB(4);
@@ -304,12 +328,16 @@ int dispatch_substep(vibe_mi_domain *d, const Geo<T> &g, const T *tab, const I *
 
 template <typename T, typename I, typename O>
 int substep_nw(vibe_mi_domain *d, const Geo<T> &g, int nw, const T *tab, const I *u, const I *v, const I *w,
-               const I *rho, const I *pi, O *uo, O *vo, O *wo, O *rho_o, O *pi_o, T dtau, T an, T ao, int check) {
+               const I *rho, const I *pi, O *uo, O *vo, O *wo, O *rho_o, O *pi_o, T dtau, T an, T ao, int check,
+               const AcousticLo *li = nullptr, const AcousticLo *lo = nullptr) {
   if (nw == 16)
-    return dispatch_substep<T, I, O, 16>(d, g, tab, u, v, w, rho, pi, uo, vo, wo, rho_o, pi_o, dtau, an, ao, check);
+    return dispatch_substep<T, I, O, 16>(d, g, tab, u, v, w, rho, pi, uo, vo, wo, rho_o, pi_o, dtau, an, ao, check,
+                                         li, lo);
   if (nw == 8)
-    return dispatch_substep<T, I, O, 8>(d, g, tab, u, v, w, rho, pi, uo, vo, wo, rho_o, pi_o, dtau, an, ao, check);
-  return dispatch_substep<T, I, O, 4>(d, g, tab, u, v, w, rho, pi, uo, vo, wo, rho_o, pi_o, dtau, an, ao, check);
+    return dispatch_substep<T, I, O, 8>(d, g, tab, u, v, w, rho, pi, uo, vo, wo, rho_o, pi_o, dtau, an, ao, check,
+                                        li, lo);
+  return dispatch_substep<T, I, O, 4>(d, g, tab, u, v, w, rho, pi, uo, vo, wo, rho_o, pi_o, dtau, an, ao, check, li,
+                                      lo);
 }
 
 // ---- substep workgroup autotuner --------------------------------------------
@@ -340,7 +368,8 @@ int autotune_substep(vibe_mi_domain *d, const Geo<T> &g, AcousticSet<T> *A, doub
       return 0;
     }
   }
-  const TuneKey key{d->nx, d->ny, d->nz, d->rk, static_cast<int>(sizeof(T))};
+  const bool comp = sizeof(T) == 4 && d->precision == VIBE_MI_MIXED_COMP;
+  const TuneKey key{d->nx, d->ny, d->nz, d->rk, static_cast<int>(sizeof(T)) + (comp ? 16 : 0)};
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune.find(key);
@@ -362,7 +391,8 @@ int autotune_substep(vibe_mi_domain *d, const Geo<T> &g, AcousticSet<T> *A, doub
     for (int rep = 0; rep < 4 && ok; ++rep) {
       VMI_HIP(hipEventRecord(d->ev[0], d->stream));
       ok = substep_nw<T, T, T>(d, g, nw, tab, in.u, in.v, in.w, in.rho, in.pi, out.u, out.v, out.w, out.rho, out.pi,
-                               static_cast<T>(dtau), an, ao, 0) == 0;
+                               static_cast<T>(dtau), an, ao, 0, comp ? &d->L32[0] : nullptr,
+                               comp ? &d->L32[1] : nullptr) == 0;
       VMI_HIP(hipEventRecord(d->ev[1], d->stream));
       VMI_HIP(hipEventSynchronize(d->ev[1]));
       float ms = 0.f;
@@ -384,13 +414,14 @@ int autotune_substep(vibe_mi_domain *d, const Geo<T> &g, AcousticSet<T> *A, doub
 
 template <typename T, typename I, typename O>
 int run_substep(vibe_mi_domain *d, const Geo<T> &g, int slot, const I *u, const I *v, const I *w, const I *rho,
-                const I *pi, O *uo, O *vo, O *wo, O *rho_o, O *pi_o, double dtau, int check) {
+                const I *pi, O *uo, O *vo, O *wo, O *rho_o, O *pi_o, double dtau, int check,
+                const AcousticLo *li = nullptr, const AcousticLo *lo = nullptr) {
   if (int rc = ensure_table<T>(d, g, slot, dtau)) return rc;
   const T an = static_cast<T>(0.5 * (1.0 + d->eps_s)), ao = static_cast<T>(0.5 * (1.0 - d->eps_s));
   const T *tab = table_slot<T>(d, slot);
   const int nw = sizeof(T) == 8 ? d->nw_fp64 : d->nw_fp32;
   return substep_nw<T, I, O>(d, g, nw, tab, u, v, w, rho, pi, uo, vo, wo, rho_o, pi_o, static_cast<T>(dtau), an, ao,
-                             check);
+                             check, li, lo);
 }
 
 // One full RK3 step.  Stage outputs go to S[(cur0+1)%3] / S[(cur0+2)%3].
@@ -398,6 +429,7 @@ template <typename T>
 int step_impl(vibe_mi_domain *d, const Geo<T> &g, AcousticSet<T> *A, double dt, int nsub, bool timed) {
   const int ns = nsub < 1 ? 1 : nsub;
   const double hk[3] = {dt / 3.0, dt / 2.0, dt};
+  const bool comp = sizeof(T) == 4 && d->precision == VIBE_MI_MIXED_COMP;
   if (int rc = autotune_substep<T>(d, g, A, dt / static_cast<double>(ns))) return rc;
   const int i0 = d->cur0, i1 = (d->cur0 + 1) % 3, i2 = (d->cur0 + 2) % 3;
   const FieldSet &s0 = d->S[i0];
@@ -420,12 +452,13 @@ int step_impl(vibe_mi_domain *d, const Geo<T> &g, AcousticSet<T> *A, double dt, 
       if (!last) {
         AcousticSet<T> &out = A[1 - p];
         if (int rc = run_substep<T, T, T>(d, g, st, in.u, in.v, in.w, in.rho, in.pi, out.u, out.v, out.w, out.rho,
-                                          out.pi, dtau, 0))
+                                          out.pi, dtau, 0, comp ? &d->L32[p] : nullptr,
+                                          comp ? &d->L32[1 - p] : nullptr))
           return rc;
         p = 1 - p;
       } else {
         if (int rc = run_substep<T, T, double>(d, g, st, in.u, in.v, in.w, in.rho, in.pi, nxt.u, nxt.v, nxt.w,
-                                               nxt.rho, nxt.pi, dtau, 1))
+                                               nxt.rho, nxt.pi, dtau, 1, comp ? &d->L32[p] : nullptr, nullptr))
           return rc;
       }
     }
@@ -507,7 +540,7 @@ int create_impl(const vibe_mi_domain_desc *ds, vibe_mi_domain *d) {
   d->nyr = yuni ? 1 : d->ny;
   d->rk = d->nxr * d->nyr;
   if (int rc = build_geo<double>(d, ds, d->g64)) return rc;
-  if (d->precision == VIBE_MI_MIXED)
+  if (d->precision != VIBE_MI_FP64)
     if (int rc = build_geo<float>(d, ds, d->g32)) return rc;
   if (int rc = upload(d, &d->ubg64, std::vector<double>(ds->nz, 0.0))) return rc;
   if (int rc = upload(d, &d->ubg32, std::vector<float>(ds->nz, 0.0f))) return rc;
@@ -539,10 +572,20 @@ int create_impl(const vibe_mi_domain_desc *ds, vibe_mi_domain *d) {
   if (int rc = upload(d, &d->pi0, std::vector<double>(ds->pi0, ds->pi0 + nc))) return rc;
   for (auto &f : d->S)
     if (int rc = alloc_fieldset(d, f)) return rc;
-  const size_t tsz = d->precision == VIBE_MI_MIXED ? sizeof(float) : sizeof(double);
-  if (d->precision == VIBE_MI_MIXED) {
+  const size_t tsz = d->precision != VIBE_MI_FP64 ? sizeof(float) : sizeof(double);
+  if (d->precision != VIBE_MI_FP64) {
     for (auto &a : d->A32)
       if (int rc = alloc_acoustic<float>(d, a)) return rc;
+    if (d->precision == VIBE_MI_MIXED_COMP)
+      for (auto &l : d->L32) {
+        if (int rc = dalloc(d, &l.u, d->ncell)) return rc;
+        if (int rc = dalloc(d, &l.v, d->ncell)) return rc;
+        if (int rc = dalloc(d, &l.w, d->nwf)) return rc;
+        if (int rc = dalloc(d, &l.rho, d->ncell)) return rc;
+        if (int rc = dalloc(d, &l.pi, d->ncell)) return rc;
+        for (uint16_t *q : {l.u, l.v, l.rho, l.pi}) VMI_HIP(hipMemset(q, 0, d->ncell * 2));
+        VMI_HIP(hipMemset(l.w, 0, d->nwf * 2));
+      }
   } else {
     for (auto &a : d->A64)
       if (int rc = alloc_acoustic<double>(d, a)) return rc;
@@ -586,12 +629,12 @@ namespace vibe_mi {
 int domain_create(const vibe_mi_domain_desc *ds, vibe_mi_domain **out) { return vibe_mi_domain_create(ds, out); }
 
 int domain_ensure_table(vibe_mi_domain *d, int slot, double dtau) {
-  if (d->precision == VIBE_MI_MIXED) return ensure_table<float>(d, d->g32, slot, dtau);
+  if (d->precision != VIBE_MI_FP64) return ensure_table<float>(d, d->g32, slot, dtau);
   return ensure_table<double>(d, d->g64, slot, dtau);
 }
 
 void *domain_table(vibe_mi_domain *d, int slot) {
-  if (d->precision == VIBE_MI_MIXED) return table_slot<float>(d, slot);
+  if (d->precision != VIBE_MI_FP64) return table_slot<float>(d, slot);
   return table_slot<double>(d, slot);
 }
 
@@ -610,7 +653,7 @@ int vibe_mi_domain_create(const vibe_mi_domain_desc *ds, vibe_mi_domain **out) {
   if (ds->nx < 4 || ds->ny < 4 || ds->nz < 3 || ds->nz > nz_max || ds->dx <= 0 || ds->dy <= 0 || ds->ztop <= 0 ||
       ds->zeta == nullptr || ds->zs == nullptr || ds->rho0 == nullptr || ds->theta0 == nullptr ||
       ds->pi0 == nullptr || ds->sponge == nullptr ||
-      (ds->precision != VIBE_MI_FP64 && ds->precision != VIBE_MI_MIXED))
+      (ds->precision != VIBE_MI_FP64 && ds->precision != VIBE_MI_MIXED && ds->precision != VIBE_MI_MIXED_COMP))
     return VIBE_C_ERR_BAD_ARG;
   // kernels address a field array with 32-bit byte offsets: the largest one
   // (FP64 w, nz+1 levels) must stay below 4 GiB
@@ -644,7 +687,7 @@ int vibe_mi_domain_set_state(vibe_mi_domain *d, const double *u, const double *v
   if (int rc = set_device(d)) return rc;
   const FieldSet &f = d->S[d->cur0];
   const size_t nc = d->ncell * 8;
-  if (d->precision == VIBE_MI_MIXED) {  // background x-wind: level means of u
+  if (d->precision != VIBE_MI_FP64) {  // background x-wind: level means of u
     std::vector<double> ub(d->nz);
     std::vector<float> ub32(d->nz);
     for (int k = 0; k < d->nz; ++k) {
@@ -701,7 +744,7 @@ static int step_common(vibe_mi_domain *d, double dt, int nsub, int nsteps, bool 
   }
   for (int n = 0; n < nsteps; ++n) {
     int rc;
-    if (d->precision == VIBE_MI_MIXED)
+    if (d->precision != VIBE_MI_FP64)
       rc = step_impl<float>(d, d->g32, d->A32, dt, nsub, timed);
     else
       rc = step_impl<double>(d, d->g64, d->A64, dt, nsub, timed);

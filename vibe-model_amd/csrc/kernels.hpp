@@ -46,6 +46,8 @@
 // Compiled with -ffp-contract=off so the FP64 path rounds like the oracle.
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 #include "geo.hpp"
 #include "stencils.hpp"
@@ -83,7 +85,47 @@ template <typename O>
 struct StageOut {
   O *u, *v, *w, *rho, *pi;
   double *th;
+  // compensated FP32 mode (FP32 sets only): bf16 remainders of u, v, w, rho',
+  // pi' (nullptr: plain FP32 or FP64 output)
+  uint16_t *ul = nullptr, *vl = nullptr, *wl = nullptr, *rl = nullptr, *pl = nullptr;
 };
+
+// ---- compensated FP32 storage -----------------------------------------------
+// An acoustic-set value is kept as an FP32 value hi and a remainder lo, stored
+// as the upper 16 bits (bfloat16, rounded to nearest even) of its FP32
+// pattern: hi + lo holds about 32 significant bits.  Substeps add FP32
+// increments with comp_add, which recovers the rounding error of hi + d
+// exactly (Knuth's two-sum) and carries it into the remainder, so no
+// increment is lost against a large value.
+__device__ __forceinline__ uint16_t bf16_bits(float x) {
+  const uint32_t b = __float_as_uint(x);
+  return static_cast<uint16_t>((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf16_val(uint16_t b) { return __uint_as_float(static_cast<uint32_t>(b) << 16); }
+struct F2 {
+  float hi, lo;
+};
+__device__ __forceinline__ F2 comp_add(float hi, float lo, float d) {
+#pragma clang fp contract(off)
+  const float s = hi + d;
+  const float bb = s - hi;
+  const float err = (hi - (s - bb)) + (d - bb);
+  const float e = err + lo;
+  const float h2 = s + e;
+  return F2{h2, e - (h2 - s)};
+}
+// Writes x to an acoustic-set array of type O at byte offset off; an FP32
+// array with a remainder array also gets the bf16 remainder x - hi.
+template <typename O>
+__device__ __forceinline__ void put_acoustic(O *p, uint16_t *lo, uint32_t off, double x) {
+  if constexpr (sizeof(O) == 4) {
+    const float hi = static_cast<float>(x);
+    gst(p, off, hi);
+    if (lo != nullptr) gst(lo, off >> 1, bf16_bits(static_cast<float>(x - static_cast<double>(hi))));
+  } else {
+    gst(p, off, static_cast<O>(x));
+  }
+}
 
 // Rayleigh sponge folded into the writes of a step's final state
 // (damping.cpp:76-138 with perturbation targets 0): level k >= kmin is
@@ -275,7 +317,7 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageCur<CI> &
   T cu_O = T(0), cu_f = T(0), cv_O = T(0), cv_f = T(0), cw_O = T(0), cw_f = T(0);
   T cm_z = T(0), cr_z = T(0), ct_z = T(0);
   int bad = 0;
-  if (act) gst(out.w, oCb, static_cast<O>(gld(s0.w, cCb)));  // interface 0 keeps its value (R_w = 0)
+  if (act) put_acoustic(out.w, out.wl, oCb, gld(s0.w, cCb));  // interface 0 keeps its value (R_w = 0)
   ld_s0(0);
 
   load_level(0);
@@ -369,7 +411,7 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageCur<CI> &
         if (ph.th != nullptr) Rd = Rd + gld(ph.u, cCb + static_cast<uint32_t>(k) * lsD);
       double un = S0[0] + h * Rd;
       if constexpr (sizeof(T) == 4) un = un - C(g.ubg64)[k];  // FP32 acoustic buffers hold u - ubg
-      if (act) gst(out.u, ko, static_cast<O>(un));
+      if (act) put_acoustic(out.u, out.ul, ko, un);
     }
     // ---------------- v at the south face ----------------
     {
@@ -398,7 +440,7 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageCur<CI> &
       double Rd = static_cast<double>(R);
       if constexpr (PH)
         if (ph.th != nullptr) Rd = Rd + gld(ph.v, cCb + static_cast<uint32_t>(k) * lsD);
-      if (act) gst(out.v, ko, static_cast<O>(S0[1] + h * Rd));
+      if (act) put_acoustic(out.v, out.vl, ko, S0[1] + h * Rd);
     }
     // ---------------- w at interface k (1..nz-1) ----------------
     // flux of w through the cell centre below / above interface k
@@ -422,7 +464,7 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageCur<CI> &
       const T th0w = T(0.5) * (thm + th00);
       const T pz = (pw[1] - pw[0]) * rz * rD;
       const T R = adv * rD - T(kCp) * thw * pz + quot<T>(T(kG) * thw, th0w);
-      if (act) gst(out.w, ko, static_cast<O>(S0[2] + h * static_cast<double>(R)));
+      if (act) put_acoustic(out.w, out.wl, ko, S0[2] + h * static_cast<double>(R));
     }
     cw_O = Oc1;
     cw_f = fzc1;
@@ -461,9 +503,9 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageCur<CI> &
       const double tn = S0[4] + h * static_cast<double>(rt);
       const double pn = S0[5] + h * static_cast<double>(rp);
       if (act) {
-        gst(out.rho, ko, static_cast<O>(rn));
+        put_acoustic(out.rho, out.rl, ko, rn);
         gst(out.th, cCb + static_cast<uint32_t>(k) * lsD, sponge_scale(sp, k, nz, tn));
-        gst(out.pi, ko, static_cast<O>(pn));
+        put_acoustic(out.pi, out.pl, ko, pn);
         bad |= !isfinite(tn);
       }
       gamk = gam1;
@@ -509,7 +551,7 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageCur<CI> &
     }
     if (k + 2 < nz) store_level((k + 2) % 3);
   }
-  if (act) gst(out.w, oCb + static_cast<uint32_t>(nz) * lsO, static_cast<O>(gld(s0.w, cCb + static_cast<uint32_t>(nz) * lsD)));
+  if (act) put_acoustic(out.w, out.wl, oCb + static_cast<uint32_t>(nz) * lsO, gld(s0.w, cCb + static_cast<uint32_t>(nz) * lsD));
   if (bad) atomicOr(flag, 1);
 }
 
@@ -583,13 +625,22 @@ struct SubArgs {
   int check;                       // 1: flag non-finite outputs
   int *flag;
   SpongeArgs sp = {};              // FP64 outputs only: sponge of the step's final state
+  // compensated FP32 mode: bf16 remainders of the state in and (FP32 outputs)
+  // out
+  const uint16_t *ul = nullptr, *vl = nullptr, *wl = nullptr, *rl = nullptr, *pl = nullptr;
+  uint16_t *ulo = nullptr, *vlo = nullptr, *wlo = nullptr, *rlo = nullptr, *plo = nullptr;
+  // compensated mode: FP64 geometry for the mass-flux sums
+  const double *rhu64 = nullptr, *rhv64 = nullptr, *r0w64 = nullptr, *sx64 = nullptr, *sy64 = nullptr,
+               *zeta64 = nullptr;
+  double rdx64 = 0.0, rdy64 = 0.0;
 };
 
 // Waves per SIMD the substep kernels are compiled for: FP32 shapes with up
 // to 12 levels per wave are held to 128 VGPRs (four waves per SIMD); the
-// other shapes are register-limited by their own size.
-template <typename T, int NW, int MAXCH>
-inline constexpr int kSubstepWavesPerEu = (sizeof(T) == 4 && MAXCH <= 12) ? 4 : 1;
+// other shapes, and the compensated kernels (about 175 VGPRs, two waves per
+// SIMD; three would spill), are register-limited by their own size.
+template <typename T, int NW, int MAXCH, bool CP = false>
+inline constexpr int kSubstepWavesPerEu = (sizeof(T) == 4 && MAXCH <= 12 && !CP) ? 4 : 1;
 
 // Row of 64 columns starting at x-block bx, row j; all 64*NW threads of the
 // workgroup call it together (it synchronises the workgroup).
@@ -606,8 +657,19 @@ inline constexpr int kSubstepWavesPerEu = (sizeof(T) == 4 && MAXCH <= 12) ? 4 : 
 // Addresses are (array base, per-lane 32-bit column offset + uniform level
 // offset); the 32-bit sum is formed next to each access, which keeps the
 // scalar-base + vector-offset addressing form.
-template <typename T, typename I, typename O, int NW, int MAXCH, int GM>
+//
+// CP (compensated FP32 storage, T = I = float): the state carries bf16
+// remainders (comp_add).  Every update is formed as an FP32 increment and
+// added to hi + lo exactly: u and v directly, rho' and pi' from increment
+// forms of E and Q, and w from the increment system A dw = -gk (ao dpi'_old
+// + an dP^), where P^ is the pi' estimate with the full old-w vertical flux
+// rho0_w (w - M).  With w_new = w + dw this is the same system as
+// A w_new = w - gk (ao dpi'_old + an dP^ao) of the plain form (P^ao with the
+// ao part of the old-w flux), and the closing flux rho0_w (w - M) + an rho0_w dw
+// equals rho0_w (an w_new + ao w - M).
+template <typename T, typename I, typename O, int NW, int MAXCH, int GM, bool CP = false>
 __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I, O> &sa, int bx, int j) {
+  static_assert(!CP || (sizeof(T) == 4 && sizeof(I) == 4), "compensated storage is FP32");
   const I *__restrict__ u = sa.u;
   const I *__restrict__ v = sa.v;
   const I *__restrict__ w = sa.w;
@@ -626,7 +688,10 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   T *s_td = reinterpret_cast<T *>(smem_raw);  // [nz-1][64] old-w RHS part -> RHS -> d' -> w
   T *s_be = s_td + nlds;                      // [nz-1][64] beta
   T *s_cp = s_be + nlds;                      // [nz-1][64] c'
-  __shared__ T s_mc[NW][64], s_fh[NW][64], s_ph[NW][64];
+  // CP: the slope and vertical mass fluxes are formed in FP64
+  using TF = std::conditional_t<CP, double, T>;
+  __shared__ TF s_mc[NW][64], s_fh[NW][64];
+  __shared__ T s_ph[NW][64];
   const int lane = threadIdx.x & 63;
   // wave index: uniform within the wavefront, so level offsets are scalars
   const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
@@ -650,6 +715,9 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   const uint32_t tC = gC * ST, tE = gE * ST, tN = gN * ST;
   // level strides in bytes (uniform)
   const uint32_t lsI = nxy * SI, lsO = nxy * SO, lsT = static_cast<uint32_t>(rk) * ST;
+  const uint32_t lC = cC * 2u, lsL = nxy * 2u;  // bf16 remainders (CP)
+  const uint32_t lE = (static_cast<uint32_t>(j) * unx + iE) * 2u, lN = (static_cast<uint32_t>(jN) * unx + i) * 2u;
+  const uint32_t tC8 = gC * 8u, tE8 = gE * 8u, tN8 = gN * 8u, lsT8 = static_cast<uint32_t>(rk) * 8u;
   constexpr bool xslope = GM != kGeoFlat, yslope = GM == kGeoAny;  // nonzero slope / metric terms
   constexpr bool colE = GM != kGeoFlat, colN = GM == kGeoAny;     // east / north geometry column differs
   const T rD = C(g.rDc)[gC];
@@ -660,8 +728,9 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
 
   // per level of this wave: explicit rho' and pi' parts and the old-w part
   // of the vertical mass flux (the old-w part of the w RHS goes to s_td)
-  T E[MAXCH], Q[MAXCH], Fh[MAXCH];
-  T mc_first = T(0), mc_last = T(0), mc0 = T(0), aw_first = T(0), r0w_first = T(0);
+  T E[MAXCH], Q[MAXCH];
+  TF Fh[MAXCH];
+  TF mc_first = TF(0), mc_last = TF(0), mc0 = TF(0), aw_first = TF(0), r0w_first = TF(0);
 
   // ---------------- phase A ----------------
   if (nlev > 0) {
@@ -681,26 +750,45 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
       x[3] = T(gld(v, vN + s));
       x[4] = T(gld(w, vC + s));
       x[5] = T(gld(rho, vC + s));
+      if constexpr (CP) {
+        const uint32_t sl = static_cast<uint32_t>(k) * lsL;
+        x[6] = bf16_val(gld(sa.ul, lC + sl));
+        x[7] = bf16_val(gld(sa.vl, lC + sl));
+        x[8] = bf16_val(gld(sa.ul, lE + sl));
+        x[9] = bf16_val(gld(sa.vl, lN + sl));
+        x[10] = bf16_val(gld(sa.wl, lC + sl));
+      }
     };
     // face and interface products of a level: cp theta0 and D rho0 at the
     // west/east u faces and south/north v faces, kappa, gk and r0w of the
     // interface below the level
     enum { GUW, GUE, GVS, GVN, HUW, HUE, HVS, HVN, KAP, GKB, R0B, NG };
-    auto ld_geo = [&](int k, T *q) {
+    // CP: the mass-flux products D rho0 at the faces and r0w come in FP64
+    // (qd: HUW, HUE, HVS, HVN, R0B) in place of their FP32 values
+    auto ld_geo = [&](int k, T *q, double *qd) {
       const uint32_t s = static_cast<uint32_t>(k) * lsT;
       q[GUW] = gldc(g.cthu, tC + s);
       q[GUE] = colE ? gldc(g.cthu, tE + s) : q[GUW];
       q[GVS] = gldc(g.cthv, tC + s);
       q[GVN] = colN ? gldc(g.cthv, tN + s) : q[GVS];
-      q[HUW] = gldc(g.rhu, tC + s);
-      q[HUE] = colE ? gldc(g.rhu, tE + s) : q[HUW];
-      q[HVS] = gldc(g.rhv, tC + s);
-      q[HVN] = colN ? gldc(g.rhv, tN + s) : q[HVS];
+      if constexpr (CP) {
+        const uint32_t s8 = static_cast<uint32_t>(k) * lsT8;
+        qd[0] = gldc(sa.rhu64, tC8 + s8);
+        qd[1] = colE ? gldc(sa.rhu64, tE8 + s8) : qd[0];
+        qd[2] = gldc(sa.rhv64, tC8 + s8);
+        qd[3] = colN ? gldc(sa.rhv64, tN8 + s8) : qd[2];
+        qd[4] = gldc(sa.r0w64, tC8 + s8);
+      } else {
+        q[HUW] = gldc(g.rhu, tC + s);
+        q[HUE] = colE ? gldc(g.rhu, tE + s) : q[HUW];
+        q[HVS] = gldc(g.rhv, tC + s);
+        q[HVN] = colN ? gldc(g.rhv, tN + s) : q[HVS];
+        q[R0B] = gldc(g.r0w, tC + s);
+      }
       q[KAP] = gldc(g.kap, tC + s);
       // unconditional (level 0 reads entries it does not use): a load under a
       // branch would be waited for where the branches join
       q[GKB] = gldc(sa.tab.GK, tC + (k >= 1 ? s - lsT : 0u));
-      q[R0B] = gldc(g.r0w, tC + s);
     };
     // pi' of the five columns at levels k (slot c%3) and k+1 (slot (c+1)%3);
     // the vertical gradient of the interface below k is carried in gl.
@@ -718,10 +806,11 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
       for (int q = 0; q < 5; ++q) gl[q] = T(0);
     }
     ld5(min(k0 + 1, nz - 1), P[1]);
-    T X[2][6], Gq[2][NG];
+    T X[2][CP ? 11 : 6], Gq[2][NG];
+    double Gd[2][CP ? 5 : 1];
     ld_state(k0, X[0]);
-    ld_geo(k0, Gq[0]);
-    T mc_prev = T(0);
+    ld_geo(k0, Gq[0], Gd[0]);
+    TF mc_prev = TF(0);
 #pragma unroll
     for (int c = 0; c < MAXCH; ++c) {
       const int cs = c & 1;
@@ -732,7 +821,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
       {
         const int kn = k0 + min(c + 1, nlev - 1);
         ld_state(kn, X[cs ^ 1]);
-        ld_geo(kn, Gq[cs ^ 1]);
+        ld_geo(kn, Gq[cs ^ 1], Gd[cs ^ 1]);
         ld5(min(kn + 1, nz - 1), P[(c + 2) % 3]);
       }
       if (c < nlev) {
@@ -740,6 +829,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         const int p0 = c % 3, p1 = (c + 1) % 3;
         const T *xs = X[cs];
         const T *gs = Gq[cs];
+        const double *gd = Gd[cs];
         const T *pc = P[p0];
         const T *pn = P[p1];
         const uint32_t so = static_cast<uint32_t>(k) * lsO;
@@ -767,7 +857,12 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
           py_n = py_n - (omz * my_n) * (T(0.5) * (Gc[0] + Gc[4]));
         }
         // FP32 buffers hold u - ubg (see Geo::ubg); un_* are the full winds
-        const T du_w = xs[0] - cpdt * (gs[GUW] * px_w);
+        F2 cu{}, cv{};
+        if constexpr (CP) {
+          cu = comp_add(xs[0], xs[6], -(cpdt * (gs[GUW] * px_w)));
+          cv = comp_add(xs[2], xs[7], -(cpdt * (gs[GVS] * py_s)));
+        }
+        const T du_w = CP ? cu.hi : xs[0] - cpdt * (gs[GUW] * px_w);
         const T du_e = xs[1] - cpdt * (gs[GUE] * px_e);
         T un_w = du_w, un_e = du_e;
         if constexpr (sizeof(T) == 4) {
@@ -775,37 +870,77 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
           un_w = ubk + du_w;
           un_e = ubk + du_e;
         }
-        const T vn_s = xs[2] - cpdt * (gs[GVS] * py_s);
+        const T vn_s = CP ? cv.hi : xs[2] - cpdt * (gs[GVS] * py_s);
         const T vn_n = xs[3] - cpdt * (gs[GVN] * py_n);
         if (active) {
           if constexpr (sizeof(O) == 8) {
-            const double uf = sizeof(T) == 4 ? C(g.ubg64)[k] + static_cast<double>(du_w) : static_cast<double>(du_w);
+            double uf = static_cast<double>(du_w), vf = static_cast<double>(vn_s);
+            if constexpr (CP) {
+              uf += static_cast<double>(cu.lo);
+              vf += static_cast<double>(cv.lo);
+            }
+            if constexpr (sizeof(T) == 4) uf += C(g.ubg64)[k];
             gst(uo, oC + so, sponge_scale(sa.sp, k, nz, uf));
-            gst(vo, oC + so, sponge_scale(sa.sp, k, nz, static_cast<double>(vn_s)));
+            gst(vo, oC + so, sponge_scale(sa.sp, k, nz, vf));
           } else {
             gst(uo, oC + so, static_cast<O>(du_w));
             gst(vo, oC + so, static_cast<O>(vn_s));
+            if constexpr (CP) {
+              const uint32_t sl = static_cast<uint32_t>(k) * lsL;
+              gst(sa.ulo, lC + sl, bf16_bits(cu.lo));
+              gst(sa.vlo, lC + sl, bf16_bits(cv.lo));
+            }
           }
         }
-        T mc = T(0);
-        if constexpr (yslope)
-          mc = T(0.5) * (un_w * sx_w + un_e * sx_e) + T(0.5) * (vn_s * sy_s + vn_n * sy_n);
-        else if constexpr (xslope)
-          mc = T(0.5) * (un_w * sx_w + un_e * sx_e);
-        const T hd = (gs[HUE] * un_e - gs[HUW] * un_w) * g.rdx + (gs[HVN] * vn_n - gs[HVS] * vn_s) * g.rdy;
+        TF mc = TF(0);
+        T hd;
+        if constexpr (CP) {
+          // full new winds hi + lo (+ background) in FP64; the face mass
+          // fluxes D rho0 u (about 1e5) are differenced across the cell in
+          // FP64 from FP64 geometry, as is the slope flux
+          const double ub = C(g.ubg64)[k];
+          const double uwd = ub + static_cast<double>(cu.hi) + static_cast<double>(cu.lo);
+          const double ued = ub + static_cast<double>(xs[1]) + static_cast<double>(xs[8]) +
+                             static_cast<double>(-(cpdt * (gs[GUE] * px_e)));
+          const double vsd = static_cast<double>(cv.hi) + static_cast<double>(cv.lo);
+          const double vnd = static_cast<double>(xs[3]) + static_cast<double>(xs[9]) +
+                             static_cast<double>(-(cpdt * (gs[GVN] * py_n)));
+          if constexpr (yslope)
+            mc = 0.5 * (uwd * C(sa.sx64)[gC] + ued * C(sa.sx64)[gE]) + 0.5 * (vsd * C(sa.sy64)[gC] + vnd * C(sa.sy64)[gN]);
+          else if constexpr (xslope)
+            mc = 0.5 * (uwd * C(sa.sx64)[gC] + ued * C(sa.sx64)[gE]);
+          const double hdd = (gd[1] * ued - gd[0] * uwd) * sa.rdx64 + (gd[3] * vnd - gd[2] * vsd) * sa.rdy64;
+          hd = static_cast<T>(hdd);
+        } else {
+          if constexpr (yslope)
+            mc = T(0.5) * (un_w * sx_w + un_e * sx_e) + T(0.5) * (vn_s * sy_s + vn_n * sy_n);
+          else if constexpr (xslope)
+            mc = T(0.5) * (un_w * sx_w + un_e * sx_e);
+          hd = (gs[HUE] * un_e - gs[HUW] * un_w) * g.rdx + (gs[HVN] * vn_n - gs[HVS] * vn_s) * g.rdy;
+        }
         const T cq = gs[KAP] * dtD;
-        E[c] = xs[5] - dtD * hd;
-        Q[c] = pc[0] - cq * hd;
+        // CP: E and Q hold the increments of rho' and pi'
+        E[c] = CP ? -(dtD * hd) : xs[5] - dtD * hd;
+        Q[c] = CP ? -(cq * hd) : pc[0] - cq * hd;
         const T wk = xs[4];
-        Fh[c] = T(0);
+        Fh[c] = TF(0);
         if (k >= 1) {
-          s_td[(k - 1) * 64 + lane] = wk - gs[GKB] * ao * (pc[0] - pmC);
-          if (c > 0) {
-            const T Mw = (T(1) - C(g.zeta)[k]) * (T(0.5) * (mc_prev + mc));
-            Fh[c] = gs[R0B] * (ao * wk - Mw);
+          s_td[(k - 1) * 64 + lane] = CP ? -(gs[GKB] * ao * (pc[0] - pmC)) : wk - gs[GKB] * ao * (pc[0] - pmC);
+          TF awk, r0k;
+          if constexpr (CP) {  // the full old-w flux
+            awk = static_cast<double>(wk) + static_cast<double>(xs[10]);
+            r0k = gd[4];
           } else {
-            aw_first = ao * wk;
-            r0w_first = gs[R0B];
+            awk = ao * wk;
+            r0k = gs[R0B];
+          }
+          if (c > 0) {
+            const TF zk = CP ? TF(C(sa.zeta64)[k]) : TF(C(g.zeta)[k]);
+            const TF Mw = (TF(1) - zk) * (TF(0.5) * (mc_prev + mc));
+            Fh[c] = r0k * (awk - Mw);
+          } else {
+            aw_first = awk;
+            r0w_first = r0k;
           }
         }
         if (c == 0) mc_first = mc;
@@ -824,20 +959,23 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   // scaled by 1/pivot for the factored sweep.
   if (nlev > 0) {
     if (k0 >= 1) {
-      const T Mw = (T(1) - C(g.zeta)[k0]) * (T(0.5) * (s_mc[wv - 1][lane] + mc_first));
+      const TF zk = CP ? TF(C(sa.zeta64)[k0]) : TF(C(g.zeta)[k0]);
+      const TF Mw = (TF(1) - zk) * (TF(0.5) * (s_mc[wv - 1][lane] + mc_first));
       Fh[0] = r0w_first * (aw_first - Mw);
     }
     s_fh[wv][lane] = Fh[0];
   }
   __syncthreads();
   if (nlev > 0) {
-    const T fh_next_chunk = (k0 + nlev < nz) ? s_fh[wv + 1][lane] : T(0);
+    const TF fh_next_chunk = (k0 + nlev < nz) ? s_fh[wv + 1][lane] : TF(0);
 #pragma unroll
     for (int c = 0; c < MAXCH; ++c) {
       if (c == nlev - 1) {
         const int k = k0 + c;
         const T ck = gldc(sa.tab.CK, tC + static_cast<uint32_t>(k) * lsT);
-        s_ph[wv][lane] = Q[c] - ck * (fh_next_chunk - Fh[c]);
+        T qf = Q[c];
+        if constexpr (CP) qf = T(gld(pi, vC + static_cast<uint32_t>(k) * lsI)) + qf;
+        s_ph[wv][lane] = qf - ck * static_cast<T>(fh_next_chunk - Fh[c]);
       }
     }
   }
@@ -852,11 +990,12 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
       t[2] = gldc(sa.tab.INV, sm);
       t[3] = gldc(sa.tab.A, sm);
       t[4] = gldc(sa.tab.CP, sm);
+      if constexpr (CP) t[5] = T(gld(pi, vC + static_cast<uint32_t>(k) * lsI));
     };
-    const T fh_next_chunk = (k0 + nlev < nz) ? s_fh[wv + 1][lane] : T(0);
+    const TF fh_next_chunk = (k0 + nlev < nz) ? s_fh[wv + 1][lane] : TF(0);
     T php = (k0 >= 1) ? s_ph[wv - 1][lane] : T(0);
     // operands two levels ahead (the level's own work is short), unconditional
-    T TB[3][5];
+    T TB[3][CP ? 6 : 5];
     ld_tab(k0, TB[0]);
     ld_tab(k0 + min(1, nlev - 1), TB[1]);
 #pragma unroll
@@ -865,8 +1004,8 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
       if (c < nlev) {
         const int k = k0 + c;
         const T *tb = TB[c % 3];
-        const T fhn = (c + 1 < nlev) ? Fh[(c + 1) < MAXCH ? c + 1 : c] : fh_next_chunk;
-        const T ph = Q[c] - tb[0] * (fhn - Fh[c]);
+        const TF fhn = (c + 1 < nlev) ? Fh[(c + 1) < MAXCH ? c + 1 : c] : fh_next_chunk;
+        const T ph = (CP ? tb[5] + Q[c] : Q[c]) - tb[0] * static_cast<T>(fhn - Fh[c]);
         if (k >= 1) {
           const int m = k - 1;
           const T td = s_td[m * 64 + lane] - tb[1] * an * (ph - php);
@@ -966,40 +1105,86 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   if (nlev > 0) {
     int bad = 0;
     // r0w of the interface above a level and kappa of the level
-    auto ld_d = [&](int k, T *t) {
+    auto ld_d = [&](int k, T *t, double *t8) {
       const uint32_t s = static_cast<uint32_t>(k) * lsT;
       t[0] = gldc(g.r0w, tC + (s + lsT));  // r0w has nz+1 levels
       t[1] = gldc(g.kap, tC + s);
+      if constexpr (CP) {  // FP64 r0w above, and the old state of the level: rho', pi', w (hi, lo)
+        t8[0] = gldc(sa.r0w64, tC8 + static_cast<uint32_t>(k + 1) * lsT8);
+        const uint32_t si = static_cast<uint32_t>(k) * lsI, sl = static_cast<uint32_t>(k) * lsL;
+        t[2] = T(gld(rho, vC + si));
+        t[3] = bf16_val(gld(sa.rl, lC + sl));
+        t[4] = T(gld(pi, vC + si));
+        t[5] = bf16_val(gld(sa.pl, lC + sl));
+        t[6] = T(gld(w, vC + si));
+        t[7] = bf16_val(gld(sa.wl, lC + sl));
+      }
     };
-    const T fh_next_chunk = (k0 + nlev < nz) ? s_fh[wv + 1][lane] : T(0);
-    T DD[3][2];
-    ld_d(k0, DD[0]);
-    ld_d(k0 + min(1, nlev - 1), DD[1]);
+    const TF fh_next_chunk = (k0 + nlev < nz) ? s_fh[wv + 1][lane] : TF(0);
+    T DD[3][CP ? 8 : 2];
+    double DD8[3][1];
+    ld_d(k0, DD[0], DD8[0]);
+    ld_d(k0 + min(1, nlev - 1), DD[1], DD8[1]);
     // flux through the chunk's lower interface
-    T flo = T(0);
-    if (k0 >= 1) flo = Fh[0] + (an * gldc(g.r0w, tC + static_cast<uint32_t>(k0) * lsT)) * s_td[(k0 - 1) * 64 + lane];
+    TF flo = TF(0);
+    if (k0 >= 1) {
+      if constexpr (CP)
+        flo = Fh[0] + (static_cast<double>(an) * gldc(sa.r0w64, tC8 + static_cast<uint32_t>(k0) * lsT8)) *
+                          static_cast<double>(s_td[(k0 - 1) * 64 + lane]);
+      else
+        flo = Fh[0] + (an * gldc(g.r0w, tC + static_cast<uint32_t>(k0) * lsT)) * s_td[(k0 - 1) * 64 + lane];
+    }
 #pragma unroll
     for (int c = 0; c < MAXCH; ++c) {
-      ld_d(k0 + min(c + 2, nlev - 1), DD[(c + 2) % 3]);  // two levels ahead, unconditional
+      ld_d(k0 + min(c + 2, nlev - 1), DD[(c + 2) % 3], DD8[(c + 2) % 3]);  // two levels ahead, unconditional
       if (c < nlev) {
         const int k = k0 + c;
         const T *dd = DD[c % 3];
         const uint32_t so = static_cast<uint32_t>(k) * lsO;
-        const T fhn = (c + 1 < nlev) ? Fh[(c + 1) < MAXCH ? c + 1 : c] : fh_next_chunk;
-        const T fhi = (k + 1 < nz) ? fhn + (an * dd[0]) * s_td[k * 64 + lane] : T(0);
-        const T vd = -dtD * ((fhi - flo) * C(g.rdzeta)[k]);
-        const T rn = E[c] + vd;
-        const T pn = Q[c] + dd[1] * vd;
-        const T wn = (k >= 1) ? s_td[(k - 1) * 64 + lane] : mc0;
+        const TF fhn = (c + 1 < nlev) ? Fh[(c + 1) < MAXCH ? c + 1 : c] : fh_next_chunk;
+        TF fhi = TF(0);
+        if (k + 1 < nz) {
+          if constexpr (CP)
+            fhi = fhn + (static_cast<double>(an) * DD8[c % 3][0]) * static_cast<double>(s_td[k * 64 + lane]);
+          else
+            fhi = fhn + (an * dd[0]) * s_td[k * 64 + lane];
+        }
+        const T vd = -dtD * (static_cast<T>(fhi - flo) * C(g.rdzeta)[k]);
+        F2 cr{}, cpn{}, cw{};
+        if constexpr (CP) {
+          cr = comp_add(dd[2], dd[3], E[c] + vd);
+          cpn = comp_add(dd[4], dd[5], Q[c] + dd[1] * vd);
+          if (k >= 1) {
+            cw = comp_add(dd[6], dd[7], s_td[(k - 1) * 64 + lane]);
+          } else {
+            cw.hi = static_cast<float>(mc0);
+            cw.lo = static_cast<float>(mc0 - static_cast<double>(cw.hi));
+          }
+        }
+        const T rn = CP ? cr.hi : E[c] + vd;
+        const T pn = CP ? cpn.hi : Q[c] + dd[1] * vd;
+        const T wn = CP ? cw.hi : (k >= 1) ? s_td[(k - 1) * 64 + lane] : static_cast<T>(mc0);
         if (active) {
           if constexpr (sizeof(O) == 8) {
-            gst(rho_o, oC + so, sponge_scale(sa.sp, k, nz, static_cast<double>(rn)));
-            gst(pi_o, oC + so, sponge_scale(sa.sp, k, nz, static_cast<double>(pn)));
-            gst(wo, oC + so, sponge_scale(sa.sp, k, nz, static_cast<double>(wn)));
+            double rf = static_cast<double>(rn), pf = static_cast<double>(pn), wf = static_cast<double>(wn);
+            if constexpr (CP) {
+              rf += static_cast<double>(cr.lo);
+              pf += static_cast<double>(cpn.lo);
+              wf += static_cast<double>(cw.lo);
+            }
+            gst(rho_o, oC + so, sponge_scale(sa.sp, k, nz, rf));
+            gst(pi_o, oC + so, sponge_scale(sa.sp, k, nz, pf));
+            gst(wo, oC + so, sponge_scale(sa.sp, k, nz, wf));
           } else {
             gst(rho_o, oC + so, static_cast<O>(rn));
             gst(pi_o, oC + so, static_cast<O>(pn));
             gst(wo, oC + so, static_cast<O>(wn));
+            if constexpr (CP) {
+              const uint32_t sl = static_cast<uint32_t>(k) * lsL;
+              gst(sa.rlo, lC + sl, bf16_bits(cr.lo));
+              gst(sa.plo, lC + sl, bf16_bits(cpn.lo));
+              gst(sa.wlo, lC + sl, bf16_bits(cw.lo));
+            }
           }
         }
         bad |= !(isfinite(rn) && isfinite(pn) && isfinite(wn));
@@ -1008,18 +1193,26 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
     }
     if (k0 + nlev == nz && active) {
       const uint32_t s = static_cast<uint32_t>(nz);
-      if constexpr (sizeof(O) == 8)
-        gst(wo, oC + s * lsO, sponge_scale(sa.sp, nz, nz, static_cast<double>(T(gld(w, vC + s * lsI)))));
-      else
+      double wt = static_cast<double>(T(gld(w, vC + s * lsI)));
+      uint16_t wtl = 0;
+      if constexpr (CP) {
+        wtl = gld(sa.wl, lC + s * lsL);
+        wt += static_cast<double>(bf16_val(wtl));
+      }
+      if constexpr (sizeof(O) == 8) {
+        gst(wo, oC + s * lsO, sponge_scale(sa.sp, nz, nz, wt));
+      } else {
         gst(wo, oC + s * lsO, static_cast<O>(T(gld(w, vC + s * lsI))));
+        if constexpr (CP) gst(sa.wlo, lC + s * lsL, wtl);
+      }
     }
     if (check && bad && active) atomicOr(flag, 1);
   }
 }
 
-template <typename T, typename I, typename O, int NW, int MAXCH>
-__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kSubstepWavesPerEu<T, NW, MAXCH>))) k_substep(Geo<T> g, SubArgs<T, I, O> sa) {
-  substep_row<T, I, O, NW, MAXCH, kGeoAny>(g, sa, blockIdx.x, blockIdx.y);
+template <typename T, typename I, typename O, int NW, int MAXCH, bool CP = false>
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kSubstepWavesPerEu<T, NW, MAXCH, CP>))) k_substep(Geo<T> g, SubArgs<T, I, O> sa) {
+  substep_row<T, I, O, NW, MAXCH, kGeoAny, CP>(g, sa, blockIdx.x, blockIdx.y);
 }
 
 // ---- sponge (damping.cpp:76-138 with perturbation targets 0) -------------

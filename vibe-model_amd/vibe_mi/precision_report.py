@@ -3,7 +3,9 @@
 Runs one workload twice from the same seeded initial state, once with
 VIBE_MI_FP64 (plain FP64, the GPU path that matches the CPU oracle to
 round-off) and once with VIBE_MI_MIXED (FP64 state and RK3 accumulation,
-FP32 tendencies and substeps, FP64 compensated reductions), for a stretch of
+FP32 tendencies and substeps, FP64 compensated reductions) or, with
+--precision comp, VIBE_MI_MIXED_COMP (the same with compensated FP32 substep
+state and FP64 mass-flux sums; single-domain kernels), for a stretch of
 simulated time, and reports per prognostic:
 
   rel_l2   ||x_mixed - x_fp64||_2 / ||x_fp64||_2
@@ -28,7 +30,7 @@ import time
 
 import numpy as np
 
-from . import FP64, MIXED, Batch, Domain
+from . import FP64, MIXED, MIXED_COMP, Batch, Domain
 from . import configs
 
 FIELDS = ("u", "v", "w", "rho", "theta", "pi")
@@ -89,18 +91,24 @@ def main(argv=None):
     ap.add_argument("--block", type=int, default=50)
     ap.add_argument("--out", default="")
     ap.add_argument("--unbatched", action="store_true", help="single-domain kernels instead of the batched ones")
+    ap.add_argument("--precision", choices=("mixed", "comp"), default="mixed",
+                    help="comp: VIBE_MI_MIXED_COMP (single-domain kernels)")
     args = ap.parse_args(argv)
+    if args.precision == "comp":
+        args.unbatched = True
     kw = {"nx": args.nx, "ny": args.nx} if args.nx else {}
     case = configs.CONFIGS[args.config](**kw)
     nsteps = args.steps or int(math.ceil(args.hours * 3600.0 / case.dt))
     print(f"[precision] {case.name} {case.nx}x{case.ny}x{case.nz}, dt={case.dt} s, {nsteps} steps "
           f"({nsteps * case.dt / 3600.0:.3f} h simulated)", file=sys.stderr, flush=True)
     ref, r0, r1 = _run(case, FP64, nsteps, args.block, "fp64", not args.unbatched)
-    mix, m0, m1 = _run(case, MIXED, nsteps, args.block, "mixed", not args.unbatched)
+    prec = MIXED_COMP if args.precision == "comp" else MIXED
+    mix, m0, m1 = _run(case, prec, nsteps, args.block, args.precision, not args.unbatched)
     err = compare(mix, ref)
     rep = {
         "workload": f"{case.name} {case.nx}x{case.ny}x{case.nz}",
         "path": "single-domain kernels" if args.unbatched else "batched kernels (hipGraph)",
+        "precision": args.precision,
         "dt": case.dt, "substeps": case.nsub, "steps": nsteps,
         "simulated_hours": nsteps * case.dt / 3600.0,
         "error_mixed_vs_fp64": err,
