@@ -35,8 +35,7 @@ extern "C" {
 
 typedef struct vibe_mi_batch vibe_mi_batch;
 
-/* Creates the members from ndom descriptors (all of one precision:
- * VIBE_MI_FP64 or VIBE_MI_MIXED; VIBE_MI_MIXED_COMP gives VIBE_C_ERR_BAD_ARG). */
+/* Creates the members from ndom descriptors (all of one precision). */
 int vibe_mi_batch_create(const vibe_mi_domain_desc *descs, int ndom, vibe_mi_batch **out);
 int vibe_mi_batch_destroy(vibe_mi_batch *b);
 /* Member handle idx (0 .. ndom-1), owned by the batch. */

@@ -4,8 +4,9 @@ The batched kernels run the same tile bodies as the single-domain kernels on
 the same operands, so in FP64 every member must match its single-domain run
 bit for bit, with and without hipGraph replay, for any active subset and
 per-member dt.  The FP32 batched kernels are compiled with multiply-add
-contraction (the single-domain FP32 kernels are not), so in FP32-mixed the
-members agree to FP32 rounding, checked at the mixed-precision bar of 1e-4
+contraction (the single-domain FP32 kernels are not), so in FP32-mixed and
+FP32-compensated the members agree to FP32 rounding, checked at the
+mixed-precision bar of 1e-4
 relative L2 (observed up to 1.4e-5 for pi' after three steps), the velocity
 components measured against the largest of them (v alone is a near-zero
 field in these cases); graph replay matches the eager batched launches bit
@@ -56,7 +57,7 @@ def _assert_match(prec_name, got, ref, what):
             assert rel < 1e-4, (what, f, rel)
 
 
-@pytest.mark.parametrize("prec_name", ["FP64", "MIXED"])
+@pytest.mark.parametrize("prec_name", ["FP64", "MIXED", "MIXED_COMP"])
 @pytest.mark.parametrize("graph", [True, False])
 def test_batch_matches_single(vmi, prec_name, graph):
     prec = getattr(vmi, prec_name)
@@ -150,7 +151,7 @@ def test_batch_member_state_roundtrip_and_energy(vmi):
 
 
 @pytest.mark.parametrize("kind", ["flat", "ridge"])
-@pytest.mark.parametrize("prec_name", ["FP64", "MIXED"])
+@pytest.mark.parametrize("prec_name", ["FP64", "MIXED", "MIXED_COMP"])
 def test_batch_geometry_specialisations_match_single(vmi, kind, prec_name):
     """Batches whose members are all flat (or all y-uniform ridges) run the
     kernels specialised for that geometry (zero slope and metric terms

@@ -8,7 +8,8 @@ every prognostic is within 3e-5 relative L2 of the FP64 path (the plain
 FP32-mixed path is held to 1e-4; at these lengths the FP32 stage arithmetic,
 common to both modes, is a large part of what remains) and the worst field is
 no further from it than in the plain FP32-mixed path; mass agrees with FP64
-to 1e-10 relative; a batch refuses the mode.
+to 1e-10 relative.  Batched members are checked against single domains in
+test_gpu_batch.py.
 """
 import numpy as np
 import pytest
@@ -53,10 +54,3 @@ def test_compensated_close_to_fp64(vmi, name):
     assert worst_comp <= worst_mixed, (worst_comp, worst_mixed)
     ec_mass = runs[vmi.MIXED_COMP][1][0]
     assert abs(ec_mass - e64[0]) <= 1e-10 * abs(e64[0])
-
-
-def test_batch_refuses_compensated_mode(vmi):
-    from vibe_mi import configs
-    case = configs.warm_bubble(nx=16, ny=16, nz=12, dz=500.0)
-    with pytest.raises(vmi.VibeError):
-        vmi.Batch([case], vmi.MIXED_COMP)

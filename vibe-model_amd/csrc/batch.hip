@@ -216,6 +216,13 @@ int upload_desc(vibe_mi_batch *b, const double *dt, int nsub) {
     x.kmin_sponge = d->kmin_sponge;
     x.cur0 = d->cur0;
     x.flag = b->flag;
+    if (d->precision == VIBE_MI_MIXED_COMP) {
+      x.L[0] = d->L32[0];
+      x.L[1] = d->L32[1];
+      const Geo<double> &g8 = d->g64;
+      x.rhu64 = g8.rhu, x.rhv64 = g8.rhv, x.r0w64 = g8.r0w, x.sx64 = g8.sx, x.sy64 = g8.sy, x.zeta64 = g8.zeta;
+      x.rdx64 = g8.rdx, x.rdy64 = g8.rdy;
+    }
     VMI_HIP(hipStreamSynchronize(d->stream));
   }
   VMI_HIP(hipMemcpyAsync(b->desc, h.data(), sizeof(BDom<T>) * n, hipMemcpyHostToDevice, b->stream));
@@ -224,16 +231,17 @@ int upload_desc(vibe_mi_batch *b, const double *dt, int nsub) {
 }
 
 // dry: only set the kernel's LDS attribute (before a graph capture).
-template <typename T, typename O, int NW, int GM>
+template <typename T, typename O, int NW, int GM, bool CP>
 int launch_substep_nw(vibe_mi_batch *b, const WorkList &w, int st, int q, int abase, int last, int *ctr, bool dry) {
   const auto *D = static_cast<const BDom<T> *>(b->desc);
   const int ch = (b->nzmax + NW - 1) / NW;
   const size_t lds = 3 * static_cast<size_t>(b->nzmax - 1) * 64 * sizeof(T);
-  const size_t lds_static = 3 * NW * 64 * sizeof(T) + 64;
+  // s_mc, s_fh (FP64 in the compensated mode), s_ph and the tile loop's slots
+  const size_t lds_static = ((CP ? 2 * sizeof(double) : 2 * sizeof(T)) + sizeof(T)) * NW * 64 + 64;
   if (lds + lds_static > 160 * 1024) return VIBE_C_ERR_BAD_ARG;
 #define VMI_BSUB(M)                                                                                           \
   do {                                                                                                        \
-    auto *kern = &kb_substep<T, O, NW, M, GM>;                                                                    \
+    auto *kern = &kb_substep<T, O, NW, M, GM, CP>;                                                                \
     static bool attr = false;                                                                                 \
     if (!attr) {                                                                                              \
       VMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),                                       \
@@ -269,11 +277,19 @@ int launch_substep_nw(vibe_mi_batch *b, const WorkList &w, int st, int q, int ab
   return 0;
 }
 
+template <typename T, typename O, int GM, bool CP>
+int launch_substep_cp(vibe_mi_batch *b, int nw, const WorkList &w, int st, int q, int abase, int last, int *ctr, bool dry) {
+  if (nw == 16) return launch_substep_nw<T, O, 16, GM, CP>(b, w, st, q, abase, last, ctr, dry);
+  if (nw == 8) return launch_substep_nw<T, O, 8, GM, CP>(b, w, st, q, abase, last, ctr, dry);
+  return launch_substep_nw<T, O, 4, GM, CP>(b, w, st, q, abase, last, ctr, dry);
+}
+
 template <typename T, typename O, int GM>
 int launch_substep_gm(vibe_mi_batch *b, int nw, const WorkList &w, int st, int q, int abase, int last, int *ctr, bool dry) {
-  if (nw == 16) return launch_substep_nw<T, O, 16, GM>(b, w, st, q, abase, last, ctr, dry);
-  if (nw == 8) return launch_substep_nw<T, O, 8, GM>(b, w, st, q, abase, last, ctr, dry);
-  return launch_substep_nw<T, O, 4, GM>(b, w, st, q, abase, last, ctr, dry);
+  if constexpr (sizeof(T) == 4)
+    if (b->precision == VIBE_MI_MIXED_COMP)
+      return launch_substep_cp<T, O, GM, true>(b, nw, w, st, q, abase, last, ctr, dry);
+  return launch_substep_cp<T, O, GM, false>(b, nw, w, st, q, abase, last, ctr, dry);
 }
 
 template <typename T, typename O>
@@ -394,7 +410,7 @@ int autotune(vibe_mi_batch *b, WorkList &w) {
   }
   // keyed by the active members' shapes (a small active set of a nest
   // hierarchy wants smaller tiles than the whole batch)
-  std::string shape = std::to_string(sizeof(T)) + (b->phys ? "p" : "");
+  std::string shape = std::to_string(sizeof(T)) + (b->phys ? "p" : "") + (b->precision == VIBE_MI_MIXED_COMP ? "c" : "");
   for (int m : w.members) {
     const vibe_mi_domain *d = b->doms[m];
     shape += "," + std::to_string(d->nx) + "x" + std::to_string(d->ny) + "x" + std::to_string(d->nz) + "r" + std::to_string(d->rk);
@@ -580,8 +596,6 @@ int vibe_mi_batch_create(const vibe_mi_domain_desc *descs, int ndom, vibe_mi_bat
   *out = nullptr;
   for (int m = 1; m < ndom; ++m)
     if (descs[m].precision != descs[0].precision) return VIBE_C_ERR_BAD_ARG;
-  // the compensated FP32 mode runs on single domains only
-  if (descs[0].precision != VIBE_MI_FP64 && descs[0].precision != VIBE_MI_MIXED) return VIBE_C_ERR_BAD_ARG;
   auto *b = new vibe_mi_batch();
   b->precision = descs[0].precision;
   auto fail = [&](int rc) {
@@ -605,7 +619,7 @@ int vibe_mi_batch_create(const vibe_mi_domain_desc *descs, int ndom, vibe_mi_bat
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail(VIBE_MI_ERR_HIP);
   for (auto &e : b->ev)
     if (hipEventCreate(&e) != hipSuccess) return fail(VIBE_MI_ERR_HIP);
-  const size_t dsz = b->precision == VIBE_MI_MIXED ? sizeof(BDom<float>) : sizeof(BDom<double>);
+  const size_t dsz = b->precision != VIBE_MI_FP64 ? sizeof(BDom<float>) : sizeof(BDom<double>);
   unsigned char *p = nullptr;
   if (int rc = balloc(b, &p, dsz * ndom)) return fail(rc);
   b->desc = p;
@@ -639,7 +653,7 @@ static int batch_step_common(vibe_mi_batch *b, const double *dt, int nsub, int n
   if (hipSetDevice(b->device) != hipSuccess) return VIBE_MI_ERR_HIP;
   for (auto *d : b->doms)
     if (hipStreamSynchronize(d->stream) != hipSuccess) return VIBE_MI_ERR_HIP;
-  const int rc = b->precision == VIBE_MI_MIXED ? step_batch<float>(b, dt, nsub, nsteps, active, timed, ms_total)
+  const int rc = b->precision != VIBE_MI_FP64 ? step_batch<float>(b, dt, nsub, nsteps, active, timed, ms_total)
                                                : step_batch<double>(b, dt, nsub, nsteps, active, timed, ms_total);
   if (rc) set_last_error(rc);
   return rc;

@@ -26,6 +26,11 @@ struct BDom {
   int kmin_sponge;
   int cur0;  // rotation index, advanced on the device by kb_advance
   int *flag;
+  // compensated FP32 mode (null otherwise): bf16 remainders of A[0], A[1] and
+  // the FP64 geometry of the substep's mass-flux sums
+  AcousticLo L[2];
+  const double *rhu64, *rhv64, *r0w64, *sx64, *sy64, *zeta64;
+  double rdx64, rdy64;
 };
 
 // A tile work list split into one contiguous segment per XCD: seg[x] ..
@@ -105,7 +110,11 @@ __global__ void __launch_bounds__(64 * TY) kb_stage(const BDom<T> *__restrict__ 
     const FieldSet cur = dd.S[(c0 + st) % 3], s0 = dd.S[c0], nxt = dd.S[stage_out_index(c0, st)];
     const AcousticSet<T> a = dd.A[aout];
     const StageIn si{s0.u, s0.v, s0.w, s0.rho, s0.th, s0.pi};
-    const StageOut<T> so{a.u, a.v, a.w, a.rho, a.pi, nxt.th};
+    StageOut<T> so{a.u, a.v, a.w, a.rho, a.pi, nxt.th};
+    if constexpr (sizeof(T) == 4) {  // remainders, compensated mode only (else null)
+      const AcousticLo l = dd.L[aout];
+      so.ul = l.u, so.vl = l.v, so.wl = l.w, so.rl = l.rho, so.pl = l.pi;
+    }
     SpongeArgs sp;  // the last stage's theta' is the step's final one
     if (st == 2) sp = SpongeArgs{dd.sponge, dd.dt, dd.kmin_sponge};
     if constexpr (CUR32) {
@@ -122,9 +131,9 @@ __global__ void __launch_bounds__(64 * TY) kb_stage(const BDom<T> *__restrict__ 
 // Acoustic substep q of stage st over row tiles, from the acoustic set
 // A[(abase+q)&1] to A[(abase+q+1)&1].  The last substep of a stage
 // (last = 1) checks finiteness; with O = double it writes the FP64 stage
-// output S[out] instead.
-template <typename T, typename O, int NW, int MAXCH, int GM>
-__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kSubstepWavesPerEu<T, NW, MAXCH>))) kb_substep(const BDom<T> *__restrict__ D, TileList L, int st, int q,
+// output S[out] instead.  CP: the compensated FP32 mode (substep_row).
+template <typename T, typename O, int NW, int MAXCH, int GM, bool CP = false>
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kSubstepWavesPerEu<T, NW, MAXCH, CP>))) kb_substep(const BDom<T> *__restrict__ D, TileList L, int st, int q,
                                                       int abase, int last, int *__restrict__ ctr) {
   tile_loop(ctr, L, [&](int t) {
     const int4 tl = L.tiles[t];
@@ -169,7 +178,17 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kS
       sa.pi_o = o.pi;
       sa.check = last;
     }
-    substep_row<T, T, O, NW, MAXCH, GM>(g, sa, tl.y, tl.z);
+    if constexpr (CP) {
+      const AcousticLo li = dd.L[(abase + q) & 1];
+      sa.ul = li.u, sa.vl = li.v, sa.wl = li.w, sa.rl = li.rho, sa.pl = li.pi;
+      if constexpr (sizeof(O) == 4) {
+        const AcousticLo lo = dd.L[(abase + q + 1) & 1];
+        sa.ulo = lo.u, sa.vlo = lo.v, sa.wlo = lo.w, sa.rlo = lo.rho, sa.plo = lo.pi;
+      }
+      sa.rhu64 = dd.rhu64, sa.rhv64 = dd.rhv64, sa.r0w64 = dd.r0w64, sa.sx64 = dd.sx64, sa.sy64 = dd.sy64;
+      sa.zeta64 = dd.zeta64, sa.rdx64 = dd.rdx64, sa.rdy64 = dd.rdy64;
+    }
+    substep_row<T, T, O, NW, MAXCH, GM, CP>(g, sa, tl.y, tl.z);
   });
 }
 
@@ -177,18 +196,18 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kS
 
 // Explicit instantiation of every FP32 batched kernel shape (X = extern for a
 // declaration in batch.hip, empty for the definitions in batch_f32.hip).
-#define VMI_BATCH_F32_SUBSTEP_GM(X, O, GM)                                                                     \
-  X template __global__ void vibe_mi::kb_substep<float, O, 16, 4, GM>(VMI_KB_SUBSTEP_ARGS);                    \
-  X template __global__ void vibe_mi::kb_substep<float, O, 16, 8, GM>(VMI_KB_SUBSTEP_ARGS);                    \
-  X template __global__ void vibe_mi::kb_substep<float, O, 8, 4, GM>(VMI_KB_SUBSTEP_ARGS);                     \
-  X template __global__ void vibe_mi::kb_substep<float, O, 8, 8, GM>(VMI_KB_SUBSTEP_ARGS);                     \
-  X template __global__ void vibe_mi::kb_substep<float, O, 8, 12, GM>(VMI_KB_SUBSTEP_ARGS);                    \
-  X template __global__ void vibe_mi::kb_substep<float, O, 8, 16, GM>(VMI_KB_SUBSTEP_ARGS);                    \
-  X template __global__ void vibe_mi::kb_substep<float, O, 4, 8, GM>(VMI_KB_SUBSTEP_ARGS);                     \
-  X template __global__ void vibe_mi::kb_substep<float, O, 4, 12, GM>(VMI_KB_SUBSTEP_ARGS);                    \
-  X template __global__ void vibe_mi::kb_substep<float, O, 4, 16, GM>(VMI_KB_SUBSTEP_ARGS);                    \
-  X template __global__ void vibe_mi::kb_substep<float, O, 4, 24, GM>(VMI_KB_SUBSTEP_ARGS);                    \
-  X template __global__ void vibe_mi::kb_substep<float, O, 4, 32, GM>(VMI_KB_SUBSTEP_ARGS);
+#define VMI_BATCH_F32_SUBSTEP_GM(X, O, GM, CP)                                                                     \
+  X template __global__ void vibe_mi::kb_substep<float, O, 16, 4, GM, CP>(VMI_KB_SUBSTEP_ARGS);                    \
+  X template __global__ void vibe_mi::kb_substep<float, O, 16, 8, GM, CP>(VMI_KB_SUBSTEP_ARGS);                    \
+  X template __global__ void vibe_mi::kb_substep<float, O, 8, 4, GM, CP>(VMI_KB_SUBSTEP_ARGS);                     \
+  X template __global__ void vibe_mi::kb_substep<float, O, 8, 8, GM, CP>(VMI_KB_SUBSTEP_ARGS);                     \
+  X template __global__ void vibe_mi::kb_substep<float, O, 8, 12, GM, CP>(VMI_KB_SUBSTEP_ARGS);                    \
+  X template __global__ void vibe_mi::kb_substep<float, O, 8, 16, GM, CP>(VMI_KB_SUBSTEP_ARGS);                    \
+  X template __global__ void vibe_mi::kb_substep<float, O, 4, 8, GM, CP>(VMI_KB_SUBSTEP_ARGS);                     \
+  X template __global__ void vibe_mi::kb_substep<float, O, 4, 12, GM, CP>(VMI_KB_SUBSTEP_ARGS);                    \
+  X template __global__ void vibe_mi::kb_substep<float, O, 4, 16, GM, CP>(VMI_KB_SUBSTEP_ARGS);                    \
+  X template __global__ void vibe_mi::kb_substep<float, O, 4, 24, GM, CP>(VMI_KB_SUBSTEP_ARGS);                    \
+  X template __global__ void vibe_mi::kb_substep<float, O, 4, 32, GM, CP>(VMI_KB_SUBSTEP_ARGS);
 #define VMI_BATCH_F32_STAGE_GM(X, GM, PH)                                                                     \
   X template __global__ void vibe_mi::kb_stage<float, 2, GM, false, PH>(VMI_KB_STAGE_ARGS);                    \
   X template __global__ void vibe_mi::kb_stage<float, 4, GM, false, PH>(VMI_KB_STAGE_ARGS);                    \
@@ -197,8 +216,10 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kS
   X template __global__ void vibe_mi::kb_stage<float, 4, GM, true, PH>(VMI_KB_STAGE_ARGS);                     \
   X template __global__ void vibe_mi::kb_stage<float, 8, GM, true, PH>(VMI_KB_STAGE_ARGS);
 #define VMI_BATCH_F32_GM(X, GM)                                                                                 \
-  VMI_BATCH_F32_SUBSTEP_GM(X, float, GM)                                                                        \
-  VMI_BATCH_F32_SUBSTEP_GM(X, double, GM)                                                                       \
+  VMI_BATCH_F32_SUBSTEP_GM(X, float, GM, false)                                                                 \
+  VMI_BATCH_F32_SUBSTEP_GM(X, double, GM, false)                                                                \
+  VMI_BATCH_F32_SUBSTEP_GM(X, float, GM, true)                                                                  \
+  VMI_BATCH_F32_SUBSTEP_GM(X, double, GM, true)                                                                 \
   VMI_BATCH_F32_STAGE_GM(X, GM, false)                                                                          \
   VMI_BATCH_F32_STAGE_GM(X, GM, true)
 #define VMI_KB_SUBSTEP_ARGS const vibe_mi::BDom<float> *__restrict__, vibe_mi::TileList, int, int, int, int, int *__restrict__

@@ -5,7 +5,7 @@ VIBE_MI_FP64 (plain FP64, the GPU path that matches the CPU oracle to
 round-off) and once with VIBE_MI_MIXED (FP64 state and RK3 accumulation,
 FP32 tendencies and substeps, FP64 compensated reductions) or, with
 --precision comp, VIBE_MI_MIXED_COMP (the same with compensated FP32 substep
-state and FP64 mass-flux sums; single-domain kernels), for a stretch of
+state and FP64 mass-flux sums), for a stretch of
 simulated time, and reports per prognostic:
 
   rel_l2   ||x_mixed - x_fp64||_2 / ||x_fp64||_2
@@ -92,10 +92,8 @@ def main(argv=None):
     ap.add_argument("--out", default="")
     ap.add_argument("--unbatched", action="store_true", help="single-domain kernels instead of the batched ones")
     ap.add_argument("--precision", choices=("mixed", "comp"), default="mixed",
-                    help="comp: VIBE_MI_MIXED_COMP (single-domain kernels)")
+                    help="comp: VIBE_MI_MIXED_COMP")
     args = ap.parse_args(argv)
-    if args.precision == "comp":
-        args.unbatched = True
     kw = {"nx": args.nx, "ny": args.nx} if args.nx else {}
     case = configs.CONFIGS[args.config](**kw)
     nsteps = args.steps or int(math.ceil(args.hours * 3600.0 / case.dt))
