@@ -1,8 +1,9 @@
 """Edge sizes on the GPU against the FP64 oracle: the smallest grid the
 kernels take (4x4x3), a ragged one (one column past a 64-column tile, few
 rows), and the tallest FP64 column (nz = 103); FP64 steps agree to 1e-12
-relative L2 per prognostic, single-domain and batched.  The tallest FP32
-column (nz = 128) stays within 1e-4 of the oracle."""
+relative L2 per prognostic, single-domain and batched.  In FP32-mixed and
+FP32-compensated, the tallest FP32 column (nz = 128), the smallest and the
+ragged grid stay within 1e-4 of the oracle, single-domain and batched."""
 import numpy as np
 import pytest
 
@@ -46,22 +47,35 @@ def test_edge_sizes_match_oracle(vmi, name):
         assert np.array_equal(gb[k], got[k]), (name, "batched", k)
 
 
-def test_tallest_fp32_column_matches_oracle(vmi):
-    from vibe_mi import configs
-    case = configs.warm_bubble(nx=8, ny=4, nz=128, dz=150.0, radius=1500.0)
+def _fp32_modes_match_oracle(vmi, case, prec_name):
+    prec = getattr(vmi, prec_name)
     orc = OracleDycore(case)
     assert orc.step(case.dt, case.nsub, 2) == 0
     for batched in (False, True):
+        print(prec_name, case.nx, case.ny, case.nz, "batched" if batched else "single", flush=True)
         if batched:
-            b = vmi.Batch([case], vmi.MIXED)
+            b = vmi.Batch([case], prec)
             b.step(None, case.nsub, 2)
             got = b.members[0].get_state()
             b.close()
         else:
-            dom = vmi.Domain(case, vmi.MIXED)
+            dom = vmi.Domain(case, prec)
             dom.step(case.dt, case.nsub, 2)
             got = dom.get_state()
             dom.close()
         for k in FIELDS:
             e = rel_l2(got[k], orc.state[k])
-            assert e <= 1e-4, (batched, k, e)
+            assert e <= 1e-4, (prec_name, batched, k, e)
+
+
+@pytest.mark.parametrize("prec_name", ["MIXED", "MIXED_COMP"])
+def test_tallest_fp32_column_matches_oracle(vmi, prec_name):
+    from vibe_mi import configs
+    _fp32_modes_match_oracle(vmi, configs.warm_bubble(nx=8, ny=4, nz=128, dz=150.0, radius=1500.0), prec_name)
+
+
+@pytest.mark.parametrize("name", ["smallest_4x4x3", "ragged_65x5x7"])
+@pytest.mark.parametrize("prec_name", ["MIXED", "MIXED_COMP"])
+def test_edge_sizes_fp32_modes_match_oracle(vmi, name, prec_name):
+    from vibe_mi import configs
+    _fp32_modes_match_oracle(vmi, _cases(configs)[name](), prec_name)
