@@ -114,3 +114,22 @@ def test_nonfinite_state_raises(vmi):
     with pytest.raises(NumericalError):
         dom.step(case.dt, case.nsub, 1)
     dom.close()
+
+
+@pytest.mark.parametrize("prec_name,tol", [("FP64", 1e-8), ("MIXED", 1e-6), ("MIXED_COMP", 1e-6)])
+def test_lamb_wave_frequency_on_device(vmi, prec_name, tol):
+    """The standing Lamb wave of tests/lamb_wave.py through the device step
+    (single domain): its frequency is the discrete forward-backward C-grid
+    value, to 1e-8 relative in FP64 and 1e-6 in the FP32 modes (pi' of 1e-5
+    held in FP32; observed 1.1e-10, 1.7e-8 and 1.2e-8)."""
+    from lamb_wave import cos_amplitude, discrete_frequency, lamb_case, measured_frequency
+    c = lamb_case(vmi)
+    d = vmi.Domain(c, getattr(vmi, prec_name))
+    a = []
+    for _ in range(c.steps):
+        a.append(cos_amplitude(d.get_state()["pi"], c))
+        d.step(c.dt, c.nsub, 1)
+    d.close()
+    om, om_d = measured_frequency(a, c.dt), discrete_frequency(c)
+    print(prec_name, "omega", om, "discrete", om_d, om / om_d - 1.0)
+    assert abs(om / om_d - 1.0) <= tol, (om, om_d)

@@ -318,3 +318,24 @@ def test_step_converges_at_first_order_in_dt():
     assert all(0.9 <= r <= 1.6 for r in rates), rates
     bad = [err(run(dt, (1 / 3, 1 / 2, 0.9))) for dt in (0.75, 0.375)]
     assert math.log2(bad[0] / bad[1]) < 0.2, bad
+
+
+def test_lamb_wave_frequency_is_the_discrete_one():
+    """A standing Lamb wave over an isothermal reference state (closed form in
+    tests/lamb_wave.py): over 60 steps (1.3 periods) the oracle's frequency is
+    the discrete forward-backward C-grid value to 1e-8 relative, pi' stays
+    uniform in z and w, theta' stay at second order in the amplitude."""
+    import vibe_mi
+    from lamb_wave import cos_amplitude, discrete_frequency, lamb_case, measured_frequency
+    c = lamb_case(vibe_mi)
+    o = OracleDycore(c)
+    a = []
+    for _ in range(c.steps):
+        a.append(cos_amplitude(o.state["pi"], c))
+        o.step(c.dt, c.nsub, 1)
+    om, om_d = measured_frequency(a, c.dt), discrete_frequency(c)
+    print("omega", om, "discrete", om_d)
+    assert abs(om / om_d - 1.0) <= 1e-8, (om, om_d)
+    p = np.asarray(o.state["pi"]).reshape(c.nz, c.ny, c.nx)
+    assert np.ptp(p, axis=0).max() <= 1e-6 * 1e-5  # relative to the amplitude 1e-5
+    assert np.abs(o.state["w"]).max() <= 1e-7 and np.abs(o.state["theta"]).max() <= 1e-7
