@@ -55,8 +55,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def bytes_per_cell_update(nsub: int, mixed: bool) -> int:
-    return 8 * 48 + (4 if mixed else 8) * 30 * nsub
+def bytes_per_cell_update(nsub: int, mixed: bool, compensated: bool = False) -> int:
+    """FP64 stage traffic (48 values) plus the substeps' 30 nsub values per
+    cell, 4 B in FP32, 8 B in FP64, 6 B with a bf16 remainder."""
+    return 8 * 48 + (6 if compensated else 4 if mixed else 8) * 30 * nsub
 
 
 def substep_bytes_per_launch(cells: int, nsub: int) -> float:
@@ -295,7 +297,9 @@ def main():
         rc_ = run_batch(cases, vibe_mi.MIXED_COMP, args.steps, args.warmup, graph=True)
         cups_c = cells * args.steps / (rc_["ms"] * 1e-3)
         comp = {"value": cups_c, "ms_per_step": rc_["ms"] / args.steps, "tiles": rc_["tuning"],
-                "speedup_vs_fp64": cups_c / res64["value"]}
+                "speedup_vs_fp64": cups_c / res64["value"],
+                "bytes_per_cell_update": bytes_per_cell_update(nsub, True, True),
+                "hbm_frac_step": cups_c * bytes_per_cell_update(nsub, True, True) / HBM_PEAK}
         log(f"[bench] compensated: {rc_['ms'] / args.steps:.2f} ms/step, {cups_c:.3e} cell-updates/s, "
             f"{cups_c / res64['value']:.3f}x FP64")
     alone = None
