@@ -6,9 +6,9 @@ remainder, adds the substeps' FP32 increments to it with an exact two-sum
 FP64.  Bars: after 30 steps on the small flat, ridge and random-terrain cases
 every prognostic is within 3e-5 relative L2 of the FP64 path (the plain
 FP32-mixed path is held to 1e-4; at these lengths the FP32 stage arithmetic,
-common to both modes, is a large part of what remains) and the worst field is
-no further from it than in the plain FP32-mixed path; mass agrees with FP64
-to 1e-10 relative.  Batched members are checked against single domains in
+common to both modes, is a large part of what remains) and no field is more
+than 10 % further from it than in the plain FP32-mixed path; mass agrees with
+FP64 to 1e-10 relative.  Batched members are checked against single domains in
 test_gpu_batch.py.
 """
 import numpy as np
@@ -43,14 +43,12 @@ def test_compensated_close_to_fp64(vmi, name):
         runs[prec] = (d.get_state(), d.energy_budget())
         d.close()
     ref, e64 = runs[vmi.FP64]
-    worst_mixed = worst_comp = 0.0
     for k in FIELDS:
         em = rel_l2(runs[vmi.MIXED][0][k], ref[k])
         ec = rel_l2(runs[vmi.MIXED_COMP][0][k], ref[k])
         print(name, k, "mixed", em, "comp", ec)
         assert np.all(np.isfinite(runs[vmi.MIXED_COMP][0][k]))
         assert ec <= 3e-5, (k, ec)
-        worst_mixed, worst_comp = max(worst_mixed, em), max(worst_comp, ec)
-    assert worst_comp <= worst_mixed, (worst_comp, worst_mixed)
+        assert ec <= 1.1 * em, (k, ec, em)
     ec_mass = runs[vmi.MIXED_COMP][1][0]
     assert abs(ec_mass - e64[0]) <= 1e-10 * abs(e64[0])

@@ -103,6 +103,14 @@ int build_geo(vibe_mi_domain *d, const vibe_mi_domain_desc *ds, Geo<T> &g) {
           if (k >= 1) r0wv[q] = 0.5 * (r0[q - rk] + r0[q]);
         }
   }
+  std::vector<double> drhu(nr), drhv(nr);
+  for (int k = 0; k < nz; ++k)
+    for (int jr = 0; jr < nyr; ++jr)
+      for (int ir = 0; ir < nxr; ++ir) {
+        const size_t q = static_cast<size_t>(k) * rk + static_cast<size_t>(jr) * nxr + ir;
+        drhu[q] = rhu[static_cast<size_t>(k) * rk + static_cast<size_t>(jr) * nxr + (ir + 1) % nxr] - rhu[q];
+        drhv[q] = rhv[static_cast<size_t>(k) * rk + static_cast<size_t>((jr + 1) % nyr) * nxr + ir] - rhv[q];
+      }
   auto cvt = [](const std::vector<double> &a) { return std::vector<T>(a.begin(), a.end()); };
   g.nx = nx;
   g.ny = ny;
@@ -138,6 +146,8 @@ int build_geo(vibe_mi_domain *d, const vibe_mi_domain_desc *ds, Geo<T> &g) {
   VMI_UP(rhu, rhu)
   VMI_UP(rhv, rhv)
   VMI_UP(r0w, r0wv)
+  VMI_UP(drhu, drhu)
+  VMI_UP(drhv, drhv)
 #undef VMI_UP
   return 0;
 }

@@ -29,6 +29,10 @@ struct Geo {
   // rho0_i)/2 at u faces, cthv / rhv at v faces, r0w = (rho0_{k-1} + rho0_k)/2
   // at interfaces (index k*rk + column, 0 at k = 0 and nz)
   const T *cthu, *cthv, *rhu, *rhv, *r0w;
+  // differences of the face products across a cell, formed in FP64:
+  // drhu = rhu(east face) - rhu(west face), drhv = rhv(north) - rhv(south)
+  // (the FP32 substep forms its mass-flux divergence from them)
+  const T *drhu, *drhv;
   // FP32-mixed only: per-level background x-wind (the level mean of the
   // state set on the domain); the FP32 acoustic buffers hold u - ubg, whose
   // rounding error scales with the deviation instead of the full wind

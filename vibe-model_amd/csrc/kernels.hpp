@@ -778,6 +778,14 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         qd[2] = gldc(sa.rhv64, tC8 + s8);
         qd[3] = colN ? gldc(sa.rhv64, tN8 + s8) : qd[2];
         qd[4] = gldc(sa.r0w64, tC8 + s8);
+      } else if constexpr (sizeof(T) == 4) {
+        // FP32: the east / north face products and their differences across
+        // the cell (zero where the geometry is uniform in that direction)
+        q[HUE] = gldc(g.rhu, (colE ? tE : tC) + s);
+        q[HUW] = colE ? gldc(g.drhu, tC + s) : T(0);
+        q[HVN] = gldc(g.rhv, (colN ? tN : tC) + s);
+        q[HVS] = colN ? gldc(g.drhv, tC + s) : T(0);
+        q[R0B] = gldc(g.r0w, tC + s);
       } else {
         q[HUW] = gldc(g.rhu, tC + s);
         q[HUE] = colE ? gldc(g.rhu, tE + s) : q[HUW];
@@ -916,7 +924,14 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
             mc = T(0.5) * (un_w * sx_w + un_e * sx_e) + T(0.5) * (vn_s * sy_s + vn_n * sy_n);
           else if constexpr (xslope)
             mc = T(0.5) * (un_w * sx_w + un_e * sx_e);
-          hd = (gs[HUE] * un_e - gs[HUW] * un_w) * g.rdx + (gs[HVN] * vn_n - gs[HVS] * vn_s) * g.rdy;
+          if constexpr (sizeof(T) == 4) {
+            // D rho0 u differenced across the cell without forming the large
+            // products: rhu_e (u_e - u_w) + (rhu_e - rhu_w) u_w, where u_e - u_w
+            // comes from the stored deviations (the background cancels)
+            hd = (gs[HUE] * (du_e - du_w) + gs[HUW] * un_w) * g.rdx + (gs[HVN] * (vn_n - vn_s) + gs[HVS] * vn_s) * g.rdy;
+          } else {
+            hd = (gs[HUE] * un_e - gs[HUW] * un_w) * g.rdx + (gs[HVN] * vn_n - gs[HVS] * vn_s) * g.rdy;
+          }
         }
         const T cq = gs[KAP] * dtD;
         // CP: E and Q hold the increments of rho' and pi'
