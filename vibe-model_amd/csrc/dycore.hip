@@ -103,14 +103,16 @@ int build_geo(vibe_mi_domain *d, const vibe_mi_domain_desc *ds, Geo<T> &g) {
           if (k >= 1) r0wv[q] = 0.5 * (r0[q - rk] + r0[q]);
         }
   }
-  std::vector<double> drhu(nr), drhv(nr);
-  for (int k = 0; k < nz; ++k)
-    for (int jr = 0; jr < nyr; ++jr)
-      for (int ir = 0; ir < nxr; ++ir) {
-        const size_t q = static_cast<size_t>(k) * rk + static_cast<size_t>(jr) * nxr + ir;
-        drhu[q] = rhu[static_cast<size_t>(k) * rk + static_cast<size_t>(jr) * nxr + (ir + 1) % nxr] - rhu[q];
-        drhv[q] = rhv[static_cast<size_t>(k) * rk + static_cast<size_t>((jr + 1) % nyr) * nxr + ir] - rhv[q];
-      }
+  std::vector<double> drhu(sizeof(T) == 4 ? nr : 0), drhv(sizeof(T) == 4 ? nr : 0);
+  if (sizeof(T) == 4) {
+    for (int k = 0; k < nz; ++k)
+      for (int jr = 0; jr < nyr; ++jr)
+        for (int ir = 0; ir < nxr; ++ir) {
+          const size_t q = static_cast<size_t>(k) * rk + static_cast<size_t>(jr) * nxr + ir;
+          drhu[q] = rhu[static_cast<size_t>(k) * rk + static_cast<size_t>(jr) * nxr + (ir + 1) % nxr] - rhu[q];
+          drhv[q] = rhv[static_cast<size_t>(k) * rk + static_cast<size_t>((jr + 1) % nyr) * nxr + ir] - rhv[q];
+        }
+  }
   auto cvt = [](const std::vector<double> &a) { return std::vector<T>(a.begin(), a.end()); };
   g.nx = nx;
   g.ny = ny;
@@ -146,8 +148,12 @@ int build_geo(vibe_mi_domain *d, const vibe_mi_domain_desc *ds, Geo<T> &g) {
   VMI_UP(rhu, rhu)
   VMI_UP(rhv, rhv)
   VMI_UP(r0w, r0wv)
-  VMI_UP(drhu, drhu)
-  VMI_UP(drhv, drhv)
+  if constexpr (sizeof(T) == 4) {  // only the FP32 substep reads the differences
+    VMI_UP(drhu, drhu)
+    VMI_UP(drhv, drhv)
+  } else {
+    g.drhu = g.drhv = nullptr;
+  }
 #undef VMI_UP
   return 0;
 }
