@@ -323,11 +323,15 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"[bench] CPU oracle baseline on {threads} threads")
+        # every core this process may run on, up to the host's CPU share per
+        # GPU (OMP_NUM_THREADS, 16 on a one-GPU share); 1 thread beside it
+        visible = len(os.sched_getaffinity(0))
+        share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or visible
+        threads = max(1, min(visible, share))
+        log(f"[bench] CPU oracle baseline on {threads} threads ({visible} cores visible)")
         cpu_cups, sample, cpu_cups1 = cpu_baseline(args.config, nsub, threads)
         cpu = {"value": cpu_cups, "unit": "cell-updates/s", "cores": threads, "kind": "port", "sample": sample,
-               "value_1_thread": cpu_cups1, "cpu_model": cpu_model()}
+               "value_1_thread": cpu_cups1, "cores_visible": visible, "cpu_model": cpu_model()}
 
     out = {
         "metric": "cell-updates/s per RK3 step + % HBM roofline, FP32-mixed/batched vs FP64",

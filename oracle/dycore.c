@@ -73,6 +73,18 @@ struct orc_dycore {
 static int g_threads = 0;
 void orc_set_threads(int n) { g_threads = n; }
 
+/* planted-mistake factors (oracle.h): exactly 1 (and pl_zeta 0) unless a
+ * test plants a slip */
+static double pl_buoy = 1.0, pl_gam = 1.0, pl_met = 1.0, pl_om = 1.0;
+static int pl_zeta = 0;
+void orc_plant(int which) {
+  pl_buoy = which == 1 ? -1.0 : 1.0;
+  pl_gam = which == 2 ? 0.5 : 1.0;
+  pl_met = which == 3 ? -1.0 : 1.0;
+  pl_om = which == 4 ? -1.0 : 1.0;
+  pl_zeta = which == 5;
+}
+
 static inline int wrap(int i, int n) {
   i %= n;
   return i < 0 ? i + n : i;
@@ -270,7 +282,8 @@ static void compute_omega(orc_dycore *dc, const double *u, const double *v, cons
       for (int i = 0; i < nx; ++i) {
         double o = 0.0;
         if (k > 0 && k < nz)
-          o = w[IX(i, j, k)] - (1.0 - zeta[k]) * (0.5 * (mc[IX(i, j, k - 1)] + mc[IX(i, j, k)]));
+          o = w[IX(i, j, k)] -
+              ((pl_zeta ? zeta[k] : 1.0 - zeta[k]) * pl_om) * (0.5 * (mc[IX(i, j, k - 1)] + mc[IX(i, j, k)]));
         om[IX(i, j, k)] = o;
       }
 }
@@ -289,13 +302,13 @@ static inline double gzc(const orc_dycore *dc, const double *pi, int i, int j, i
 static inline double px_op(const orc_dycore *dc, const double *pi, int i, int j, int k) {
   const int nx = dc->nx, ny = dc->ny;
   const double dpx = (pi[IX(i, j, k)] - pi[IX(i - 1, j, k)]) * dc->rdx;
-  const double met = (1.0 - dc->zc[k]) * dc->metx[HX(i, j)];
+  const double met = ((1.0 - dc->zc[k]) * pl_met) * dc->metx[HX(i, j)];
   return dpx - met * (0.5 * (gzc(dc, pi, i - 1, j, k) + gzc(dc, pi, i, j, k)));
 }
 static inline double py_op(const orc_dycore *dc, const double *pi, int i, int j, int k) {
   const int nx = dc->nx, ny = dc->ny;
   const double dpy = (pi[IX(i, j, k)] - pi[IX(i, j - 1, k)]) * dc->rdy;
-  const double met = (1.0 - dc->zc[k]) * dc->mety[HX(i, j)];
+  const double met = ((1.0 - dc->zc[k]) * pl_met) * dc->mety[HX(i, j)];
   return dpy - met * (0.5 * (gzc(dc, pi, i, j - 1, k) + gzc(dc, pi, i, j, k)));
 }
 
@@ -304,7 +317,7 @@ static inline double gam(const orc_dycore *dc, int i, int j, int k) {
   const int nx = dc->nx, ny = dc->ny, nz = dc->nz;
   int kk = clampi(k, 1, nz - 1);
   const double *t0 = dc->d.theta0;
-  return (t0[IX(i, j, kk)] - t0[IX(i, j, kk - 1)]) * dc->rDc[HX(i, j)] * dc->rdzw[kk];
+  return (t0[IX(i, j, kk)] - t0[IX(i, j, kk - 1)]) * dc->rDc[HX(i, j)] * dc->rdzw[kk] * pl_gam;
 }
 
 /* ---- slow tendencies --------------------------------------------------- */
@@ -435,7 +448,7 @@ void orc_dycore_slow_tendency(orc_dycore *dc, const orc_state *s, orc_state *R) 
         const double thw = 0.5 * (th[IX(i, j, k - 1)] + th[IX(i, j, k)]);
         const double th0w = 0.5 * (dc->d.theta0[IX(i, j, k - 1)] + dc->d.theta0[IX(i, j, k)]);
         const double pz = (pi[IX(i, j, k)] - pi[IX(i, j, k - 1)]) * rdzw[k] * rD;
-        R->w[IX(i, j, k)] = adv * rD - CP_ * thw * pz + G_ * thw / th0w;
+        R->w[IX(i, j, k)] = adv * rD - CP_ * thw * pz + (pl_buoy * G_) * thw / th0w;
 #undef W_
       }
 
@@ -544,7 +557,7 @@ void orc_dycore_substep(orc_dycore *dc, orc_state *s, double dtau) {
         for (int k = 0; k <= nz; ++k) {
           Mw[k] = r0w[k] = gk[k] = Fh[k] = 0.0;
           if (k > 0 && k < nz) {
-            Mw[k] = (1.0 - zeta[k]) * (0.5 * (mcc[k - 1] + mcc[k]));
+            Mw[k] = ((pl_zeta ? zeta[k] : 1.0 - zeta[k]) * pl_om) * (0.5 * (mcc[k - 1] + mcc[k]));
             r0w[k] = 0.5 * (r0[IX(i, j, k - 1)] + r0[IX(i, j, k)]);
             const double th0w = 0.5 * (t0[IX(i, j, k - 1)] + t0[IX(i, j, k)]);
             gk[k] = dtD * CP_ * th0w * dc->rdzw[k];

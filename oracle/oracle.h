@@ -81,6 +81,14 @@ typedef int (*orc_physics_fn)(void *user, int stage, double dt, const orc_state 
 void orc_dycore_set_physics(orc_dycore *dc, const double *theta_rate, const double *u_rate, const double *v_rate);
 void orc_dycore_set_physics_callback(orc_dycore *dc, orc_physics_fn fn, void *user);
 void orc_set_threads(int n);
+/* Planted mistakes, for the oracle's own pins only: orc_plant(n) changes one
+ * term of the dycore so a test can show that its check fails under that
+ * slip.  0 restores the true equations (every factor exactly 1.0, so results
+ * are bit-identical to a build without the hook).
+ *   1 buoyancy sign flipped          2 dtheta0/dz term halved
+ *   3 Gal-Chen metric sign flipped   4 Omega slope-flux sign flipped
+ *   5 Omega metric factor zeta in place of (1 - zeta) */
+void orc_plant(int which);
 /* Face primitives of the dycore (dycore.c), exported for tests:
  * orc_weno_face   WENO5 value at the face right of c, one-division form
  * orc_upwind_weno U times the upwind WENO5 value at the face between q-1, q0

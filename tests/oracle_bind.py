@@ -84,6 +84,8 @@ def oracle_lib():
         L.orc_dycore_energy.argtypes = [_VP, ctypes.POINTER(OrcState), _PD]
         L.orc_set_threads.restype = None
         L.orc_set_threads.argtypes = [_I]
+        L.orc_plant.restype = None
+        L.orc_plant.argtypes = [_I]
         L.orc_weno_face.restype = _D
         L.orc_weno_face.argtypes = [_D] * 5
         L.orc_upwind_weno.restype = _D
@@ -120,6 +122,24 @@ def ref_lib():
 
 
 FIELDS = ("u", "v", "w", "rho", "theta", "pi")
+
+
+class planted:
+    """Context manager: the oracle runs with one planted mistake (orc_plant
+    codes in oracle/oracle.h), restored to the true equations on exit."""
+
+    BUOYANCY_SIGN, GAMMA_HALF, METRIC_SIGN, OMEGA_SIGN, OMEGA_ZETA = 1, 2, 3, 4, 5
+
+    def __init__(self, which: int):
+        self.which = which
+
+    def __enter__(self):
+        oracle_lib().orc_plant(self.which)
+        return self
+
+    def __exit__(self, *exc):
+        oracle_lib().orc_plant(0)
+        return False
 
 
 class OracleDycore:
