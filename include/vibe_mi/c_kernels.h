@@ -31,6 +31,7 @@ extern "C" {
 #define VIBE_MI_ERR_NO_DEVICE 101 /* no usable GPU is visible             */
 #define VIBE_MI_ERR_NONFINITE 102 /* the model state became non-finite    */
 #define VIBE_MI_ERR_PHYSICS 103   /* a physics callback returned non-zero */
+#define VIBE_MI_ERR_TOO_LARGE 104 /* the input exceeds a fixed scratch size */
 
 /* ---- reference entry points (same names / signatures) ------------------- */
 
@@ -92,6 +93,25 @@ int vibe_mi_tridiagonal_solve_batched_f32(const float *a, const float *b,
                                           const float *c, const float *d,
                                           int n, int nsys, float *x, float *ws,
                                           void *stream);
+
+/* No counterpart in the reference: the same solve for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  It takes no workspace and calls no hipMalloc/hipFree:
+ * the singular flag and the forward-sweep workspace live in a statically
+ * sized device array of the library (32 MiB per GPU, held until the process
+ * ends), so n * nsys may be at most vibe_mi_noalloc_scratch_capacity()
+ * (4194272); larger inputs return VIBE_MI_ERR_TOO_LARGE without touching the
+ * GPU.  Every call on one GPU shares that GPU's array: calls that overlap on
+ * one GPU (other threads, other streams) race and must be serialised by the
+ * caller.  The flag is cleared by a kernel at the start of every call.
+ * stream NULL: the null stream.  Results equal
+ * vibe_mi_tridiagonal_solve_batched_f64 bit for bit. */
+int vibe_mi_tridiagonal_solve_batched_noalloc_f64(const double *a, const double *b,
+                                                  const double *c, const double *d,
+                                                  int n, int nsys, double *x,
+                                                  void *stream);
+/* n * nsys limit of vibe_mi_tridiagonal_solve_batched_noalloc_f64. */
+size_t vibe_mi_noalloc_scratch_capacity(void);
 
 /* q: n reconstruction stencils, q[5*s + 0..4] = (qm2, qm1, q0, q1, q2). */
 int vibe_mi_weno5_reconstruct_batched_f64(const double *q, size_t n,

@@ -12,6 +12,7 @@
 // tree and agrees to a few ulp.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -99,6 +100,44 @@ __global__ void __launch_bounds__(256) k_thomas(const T *__restrict__ a, const T
     xn = x[o] - ws[o] * xn;
     x[o] = xn;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the no-allocation batched Thomas entry point: one statically
+// sized device array per GPU.  Bytes [0, 256) hold the singular flag, the rest
+// the forward-sweep workspace c' (n * nsys values).  In host code the array's
+// name is only a stand-in for the symbol: its address on each device is taken
+// with hipGetSymbolAddress, once per device with that device current, and that
+// pointer is what host code passes to kernels and copies.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = size_t(32) << 20;  // 32 MiB per GPU
+constexpr size_t kScratchHead = 256;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device.
+int scratch_address(void **out) {
+  int dev = 0;
+  VMI_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return VIBE_C_ERR_BAD_ARG;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void *p = nullptr;
+    VMI_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (p == nullptr) return VIBE_MI_ERR_HIP;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+// Clears the flag with a kernel (not a memset node) so the enqueue sequence
+// stays capturable.
+__global__ void k_clear_flag(int *flag) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *flag = 0;
 }
 
 template <typename T>
@@ -407,6 +446,30 @@ int vibe_mi_tridiagonal_solve_batched_f64(const double *a, const double *b, cons
   VMI_HIP(hipStreamSynchronize(st));
   return flag ? VIBE_C_ERR_SINGULAR : VIBE_C_OK;
 }
+
+int vibe_mi_tridiagonal_solve_batched_noalloc_f64(const double *a, const double *b, const double *c,
+                                                  const double *d, int n, int nsys, double *x, void *stream) {
+  if (n <= 0 || nsys <= 0 || b == nullptr || d == nullptr || x == nullptr) return VIBE_C_ERR_BAD_ARG;
+  const size_t need = static_cast<size_t>(n) * static_cast<size_t>(nsys) * sizeof(double);
+  if (need > kScratchBytes - kScratchHead) {
+    set_last_error(VIBE_MI_ERR_TOO_LARGE);
+    return VIBE_MI_ERR_TOO_LARGE;
+  }
+  void *base = nullptr;
+  if (int rc = scratch_address(&base)) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  int *dflag = static_cast<int *>(base);
+  double *ws = reinterpret_cast<double *>(static_cast<unsigned char *>(base) + kScratchHead);
+  hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(64), 0, st, dflag);
+  VMI_HIP(hipGetLastError());
+  if (int rc = thomas_device<double>(a, b, c, d, n, nsys, x, ws, st, dflag)) return rc;
+  int flag = 0;
+  VMI_HIP(hipMemcpyAsync(&flag, dflag, sizeof(int), hipMemcpyDeviceToHost, st));
+  VMI_HIP(hipStreamSynchronize(st));
+  return flag ? VIBE_C_ERR_SINGULAR : VIBE_C_OK;
+}
+
+size_t vibe_mi_noalloc_scratch_capacity(void) { return (kScratchBytes - kScratchHead) / sizeof(double); }
 
 int vibe_mi_tridiagonal_solve_batched_f32(const float *a, const float *b, const float *c,
                                           const float *d, int n, int nsys, float *x, float *ws,

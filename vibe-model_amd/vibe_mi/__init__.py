@@ -43,6 +43,7 @@ VIBE_MI_ERR_HIP = 100
 VIBE_MI_ERR_NO_DEVICE = 101
 VIBE_MI_ERR_NONFINITE = 102
 VIBE_MI_ERR_PHYSICS = 103
+VIBE_MI_ERR_TOO_LARGE = 104
 VIBE_MI_ERR_IO = 110
 VIBE_MI_ERR_FORMAT = 111
 VIBE_MI_ERR_NOT_FOUND = 112
@@ -121,6 +122,8 @@ def lib() -> ctypes.CDLL:
         "vibe_neumaier_sum_f32": (_D, [_PF, _SZ]),
         "vibe_advect_scalar_flux_f64": (None, [_PD, _PD, _PD, _PD, _PD, _PD, _PD, _I, _I, _I, _D, _D, _D, _PD]),
         "vibe_mi_tridiagonal_solve_batched_f64": (_I, [_VP, _VP, _VP, _VP, _I, _I, _VP, _VP, _VP]),
+        "vibe_mi_tridiagonal_solve_batched_noalloc_f64": (_I, [_VP, _VP, _VP, _VP, _I, _I, _VP, _VP]),
+        "vibe_mi_noalloc_scratch_capacity": (_SZ, []),
         "vibe_mi_weno5_reconstruct_batched_f64": (_I, [_VP, _SZ, _VP, _VP, _VP]),
         "vibe_mi_neumaier_sum_device_f64": (_I, [_VP, _SZ, _PD, _VP]),
         "vibe_mi_last_error": (_I, []),
