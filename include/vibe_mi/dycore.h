@@ -171,6 +171,15 @@ int vibe_mi_terrain_gaussian(int nx, int ny, double dx, double dy, double sigma,
 int vibe_mi_reference_state(int kind, double theta_sfc, double bv, double p_sfc, int nx,
                             int ny, int nz, double ztop, const double *zeta, const double *zs,
                             double *rho0, double *theta0, double *pi0);
+/* Reference state from a (z, theta) sounding: replaces
+ * ReferenceState::from_profile (reference_state.cpp:61-110) with the same
+ * discrete integration, bit for bit: theta linear in the profile at the
+ * interfaces, pi integrated upward with the interface-mean theta, theta0 =
+ * that mean, pi0 = pi at the cell's upper interface, rho0 from the EOS.
+ * np >= 2 profile points, zp increasing.  Returns 0, or 2 on bad arguments. */
+int vibe_mi_reference_state_from_profile(int nx, int ny, int nz, double ztop, const double *zeta,
+                                         const double *zs, int np, const double *zp, const double *thp,
+                                         double p_sfc, double *rho0, double *theta0, double *pi0);
 /* Rayleigh sponge rates per level (damping.cpp:24-41, column (0,0)). */
 int vibe_mi_sponge(int nz, double ztop, const double *zeta, double zs00, double start_fraction,
                    double alpha_max, double *alpha);

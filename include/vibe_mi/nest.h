@@ -1,7 +1,8 @@
-/* vibe_mi/nest.h -- C ABI of the nest lateral boundary (one-way nesting).
+/* vibe_mi/nest.h -- C ABI of the nest boundaries (one-way and two-way).
  *
  * Replaces, on the device, the reference's nest operators
  *   Nest::interpolate_from_parent    src/grid/nest.cpp:52-83
+ *   Nest::restrict_to_parent         src/grid/nest.cpp:85-108
  *   Nest::apply_lateral_boundary     src/grid/nest.cpp:110-128
  *   davies_profile                   src/grid/nest.cpp:34-43
  *   vibe_bilinear_f64 (arithmetic)   src/common/c_kernels.c:108-144
@@ -61,6 +62,22 @@ int vibe_mi_nest_relax(vibe_mi_domain *child, vibe_mi_domain *parent, const vibe
  * ordered after it.  The relaxation between the sub-steps of a nest
  * hierarchy (vibe-model_amd/vibe_mi/nesting.py). */
 int vibe_mi_batch_nest_relax(vibe_mi_batch *b, int child, int parent, const vibe_mi_nest_desc *nd, double frac);
+
+/* Two-way feedback (Nest::restrict_to_parent, nest.cpp:85-108): every parent
+ * cell covered by whole child cells, (i0/r + i, j0/r + j) for i < nx_child/r
+ * and j < ny_child/r, takes the mean of the child state over it: rho', theta',
+ * pi' and w the mean of the r x r child values (summed row by row, then times
+ * 1/r^2, as the reference), u on the parent's west face the mean of the r
+ * child faces on it, v likewise.  The reference averages u and v as cell
+ * fields and reads its i0, j0 as parent cells here (as child cells in its
+ * interpolation); this entry keeps one placement for all three operators, so
+ * i0 and j0 must be multiples of r (VIBE_C_ERR_BAD_ARG otherwise).  Writes
+ * the parent's current state; its state before its last step is untouched. */
+int vibe_mi_nest_restrict(vibe_mi_domain *child, vibe_mi_domain *parent, const vibe_mi_nest_desc *nd);
+
+/* The same feedback between members of one batch, enqueued on the batch's
+ * stream with no host synchronisation. */
+int vibe_mi_batch_nest_restrict(vibe_mi_batch *b, int child, int parent, const vibe_mi_nest_desc *nd);
 
 #ifdef __cplusplus
 }

@@ -8,6 +8,9 @@
  *                             c_kernels.c:108-144)
  *   apply_lateral_boundary    nest.cpp:110-128 (d = min distance to the
  *                             edge, c = (1 - alpha(d)) c + alpha(d) p)
+ *   restrict_to_parent        nest.cpp:85-108 (r x r mean of the child cells
+ *                             over each covered parent cell, summed jj-outer
+ *                             and ii-inner, times 1/r^2)
  * with two choices of this build: parent indices wrap (periodic parent)
  * instead of clamping, and u / v are sampled at their own staggered
  * positions (u: xp = (i0 + i)/r, v: yp = (j0 + j)/r).  The parent value is
@@ -79,4 +82,42 @@ void orc_nest_apply(const orc_state *pnew, const orc_state *pold, int pnx, int p
         }
       }
     }
+}
+
+/* Two-way feedback (nest.cpp:85-108): parent cell (i0/r + i, j0/r + j),
+ * i < cnx/r, j < cny/r, takes the r x r mean of the child cells over it for
+ * rho', theta', pi' and w (w sits at cell centres horizontally); u (west
+ * face) takes the mean of the r child faces on the parent face,
+ * u(i r, j r + jj), and v the mean of v(i r + ii, j r).  The reference
+ * averages every field as a cell field and writes to parent index
+ * i0 + i with i0 read as parent cells; here i0, j0 are child cells, as in
+ * the interpolation, and must be multiples of r.  Parent indices wrap. */
+void orc_nest_restrict(const orc_state *child, int cnx, int cny, orc_state *parent, int pnx, int pny, int nz,
+                       int r, int i0, int j0) {
+  const double *c[6] = {child->u, child->v, child->w, child->rho, child->theta, child->pi};
+  double *p[6] = {parent->u, parent->v, parent->w, parent->rho, parent->theta, parent->pi};
+  const size_t pxy = (size_t)pnx * pny, cxy = (size_t)cnx * cny;
+  const int iext = cnx / r, jext = cny / r, I0 = i0 / r, J0 = j0 / r;
+  const double w2 = 1.0 / (double)(r * r), w1 = 1.0 / (double)r;
+  for (int f = 0; f < 6; ++f) {
+    const int nk = f == 2 ? nz + 1 : nz;
+    for (int j = 0; j < jext; ++j)
+      for (int i = 0; i < iext; ++i)
+        for (int k = 0; k < nk; ++k) {
+          const double *src = c[f] + (size_t)k * cxy;
+          double acc = 0.0, wt = w2;
+          if (f == 0) {
+            for (int jj = 0; jj < r; ++jj) acc += src[(size_t)(j * r + jj) * cnx + i * r];
+            wt = w1;
+          } else if (f == 1) {
+            for (int ii = 0; ii < r; ++ii) acc += src[(size_t)(j * r) * cnx + i * r + ii];
+            wt = w1;
+          } else {
+            for (int jj = 0; jj < r; ++jj)
+              for (int ii = 0; ii < r; ++ii) acc += src[(size_t)(j * r + jj) * cnx + i * r + ii];
+          }
+          const size_t pc = (size_t)k * pxy + (size_t)wrap_mod(J0 + j, pny) * pnx + wrap_mod(I0 + i, pnx);
+          p[f][pc] = acc * wt;
+        }
+  }
 }

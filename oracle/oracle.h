@@ -103,6 +103,9 @@ void orc_davies_profile(int zone, double alpha_max, double *alpha);
  * relaxation of the zone toward pold + frac (pnew - pold). */
 void orc_nest_apply(const orc_state *pnew, const orc_state *pold, int pnx, int pny, orc_state *child, int cnx,
                     int cny, int nz, int r, int i0, int j0, int zone, double alpha_max, double frac, int full);
+/* Two-way feedback of a child onto its parent (restrict_to_parent role). */
+void orc_nest_restrict(const orc_state *child, int cnx, int cny, orc_state *parent, int pnx, int pny, int nz,
+                       int r, int i0, int j0);
 
 #ifdef __cplusplus
 }

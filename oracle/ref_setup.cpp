@@ -13,6 +13,9 @@
 //     sponge from 14 km, alpha_max 0.2                        damping.cpp:24-41
 //   ReferenceState::isothermal(theta 300 K, p_s 1000 hPa) on the
 //     8 x 6 x 16 grid with the Agnesi terrain                 reference_state.cpp:35-59
+//   ReferenceState::from_profile of a constant-N sounding (p_s 1000 hPa)
+//     and of an uneven sounding with an inversion (985 hPa) on that grid
+//                                                          reference_state.cpp:61-110
 //   energy_budget of a dry state on that grid and reference state
 //     (diagnostics.cpp:255-277): u = 5 + i/4 + j/2 - k/8 (face nx = face 0),
 //     v = -2 + i/10 - j/5 + k/20 (face ny = face 0), w = (i - j)/100 + k/500,
@@ -26,9 +29,12 @@
 //     16 x 14 x 3 at 3 km, child 12 x 9 x 3 at 1 km, ratio 3, offsets
 //     (12, 15) child cells (inside the parent: no clamping), Davies zone 4;
 //     parent p = i/2 - j/4 + k/8 + i j / 100, child c = 10 + i/10 - j/5 + k;
-//     interpolate_from_parent(p) and apply_lateral_boundary(c, that), and
+//     interpolate_from_parent(p) and apply_lateral_boundary(c, that),
+//     restrict_to_parent(the relaxed child) onto a copy of p, and
 //     davies_profile(4, 1) and (5, 0.8)
+#include <cmath>
 #include <cstdio>
+#include <string>
 #include <vector>
 
 #include "vibe/config/config.hpp"
@@ -96,6 +102,39 @@ int main() {
   put("isothermal_theta0", th);
   put("isothermal_pi0", pi);
   put("isothermal_rho0", rho);
+
+  // ReferenceState::from_profile on the same grid: a constant-N sounding
+  // (N = 0.01 1/s, 300 K at the ground, 1 km spacing to 20 km) and the
+  // reference's own two-layer standard atmosphere (built on from_profile)
+  std::vector<Real> zp, thp;
+  for (Int q = 0; q <= 20; ++q) {
+    zp.push_back(1000.0 * q);
+    thp.push_back(300.0 * std::exp(1e-4 / 9.80665 * 1000.0 * q));
+  }
+  put("profile_n001_z", zp);
+  put("profile_n001_theta", thp);
+  auto dump3 = [&](const vibe::dyn::ReferenceState &r, const char *tag) {
+    std::vector<Real> a, b, c;
+    for (Int k = 0; k < nz; ++k)
+      for (Int j = 0; j < ny; ++j)
+        for (Int i = 0; i < nx; ++i) {
+          a.push_back(r.theta0()(i, j, k));
+          b.push_back(r.pi0()(i, j, k));
+          c.push_back(r.rho0()(i, j, k));
+        }
+    std::string n(tag);
+    put((n + "_theta0").c_str(), a);
+    put((n + "_pi0").c_str(), b);
+    put((n + "_rho0").c_str(), c);
+  };
+  dump3(vibe::dyn::ReferenceState::from_profile(ghill, zp, thp, 100000.0, 0.0), "from_profile_n001");
+  // an uneven sounding with an inversion, starting above part of the terrain
+  // and ending below the lid (both held-value branches of the interpolation)
+  const std::vector<Real> zq{200.0, 500.0, 1500.0, 1600.0, 4000.0, 9000.0, 12000.0};
+  const std::vector<Real> thq{295.0, 296.0, 299.0, 304.0, 310.0, 325.0, 360.0};
+  put("profile_uneven_z", zq);
+  put("profile_uneven_theta", thq);
+  dump3(vibe::dyn::ReferenceState::from_profile(ghill, zq, thq, 98500.0, 0.0), "from_profile_uneven");
 
   vibe::dyn::State st(ghill);
   for (Int k = 0; k <= nz; ++k)
@@ -165,6 +204,25 @@ int main() {
       }
   put("nest_interpolated_cell", vi);
   put("nest_relaxed_cell_zone4", vc);
+  // restrict_to_parent of the relaxed child onto a copy of p; this call reads
+  // NestPlacement::i0, j0 as parent cells (nest.cpp:103), so the placement
+  // is (4, 5): the parent cells the child's 12 x 9 cells at offset (12, 15)
+  // child cells cover
+  grid::NestPlacement pr = pl;
+  pr.i0 = 4;
+  pr.j0 = 5;
+  const grid::Nest nest_r(gchi, gpar, pr);
+  grid::Field<Real> Pr(gpar, grid::Stagger::Cell, "pr");
+  for (Int k = 0; k < 3; ++k)
+    for (Int j = 0; j < 14; ++j)
+      for (Int i = 0; i < 16; ++i) Pr(i, j, k) = P(i, j, k);
+  std::vector<grid::Field<Real> *> prs{&Pr};
+  nest_r.restrict_to_parent(cs, prs);
+  std::vector<Real> vr;
+  for (Int k = 0; k < 3; ++k)
+    for (Int j = 0; j < 14; ++j)
+      for (Int i = 0; i < 16; ++i) vr.push_back(Pr(i, j, k));
+  put("nest_restricted_parent_cell", vr);
   put("davies_4_1", grid::davies_profile(4, 1.0));
   put("davies_5_0.8", grid::davies_profile(5, 0.8), true);
   std::printf("}\n");

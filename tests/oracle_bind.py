@@ -94,6 +94,9 @@ def oracle_lib():
         L.orc_up5.argtypes = [_D] * 7
         L.orc_davies_profile.restype = None
         L.orc_davies_profile.argtypes = [_I, _D, _PD]
+        L.orc_nest_restrict.restype = None
+        L.orc_nest_restrict.argtypes = [ctypes.POINTER(OrcState), _I, _I, ctypes.POINTER(OrcState), _I, _I, _I, _I,
+                                        _I, _I]
         L.orc_nest_apply.restype = None
         L.orc_nest_apply.argtypes = [ctypes.POINTER(OrcState), ctypes.POINTER(OrcState), _I, _I,
                                      ctypes.POINTER(OrcState), _I, _I, _I, _I, _I, _I, _I, _D, _D, _I]
@@ -246,4 +249,13 @@ def oracle_nest_apply(pnew, pold, pnx, pny, child, cnx, cny, nz, ratio, i0, j0, 
     sc = OrcState(*[_pd(out[k]) for k in FIELDS])
     oracle_lib().orc_nest_apply(ctypes.byref(sn), ctypes.byref(so), pnx, pny, ctypes.byref(sc), cnx, cny, nz,
                                 ratio, i0, j0, zone, alpha_max, frac, full)
+    return out
+
+
+def oracle_nest_restrict(child, cnx, cny, parent, pnx, pny, nz, ratio, i0, j0):
+    """Oracle two-way feedback on dict states; returns the new parent state."""
+    out = {k: np.array(parent[k], dtype=np.float64, copy=True) for k in FIELDS}
+    sc, _kc = _ostate(child)
+    sp = OrcState(*[_pd(out[k]) for k in FIELDS])
+    oracle_lib().orc_nest_restrict(ctypes.byref(sc), cnx, cny, ctypes.byref(sp), pnx, pny, nz, ratio, i0, j0)
     return out

@@ -65,6 +65,46 @@ def test_isothermal_reference_state(L, gold):
     assert np.array_equal(r0, gold["isothermal_rho0"])
 
 
+@pytest.mark.parametrize("tag,p_sfc", [("n001", 100000.0), ("uneven", 98500.0)])
+def test_from_profile_reference_state(L, gold, tag, p_sfc):
+    """vibe_mi_reference_state_from_profile against the reference's own
+    ReferenceState::from_profile (reference_state.cpp:61-110) on the 8x6x16
+    Agnesi grid, bit for bit: a constant-N sounding and an uneven one with
+    an inversion that starts above part of the terrain and ends below the
+    lid.  Beside the golden vectors: each cell's pi0 drop is the hydrostatic
+    increment -g dz / (cp theta0) across the cell (pi0 is stored at the
+    cell's upper interface), and rho0 obeys the reference's equation of
+    state."""
+    lib, pd = L
+    nx, ny, nz = 8, 6, 16
+    z = gold["zeta_16_1_1.08"].copy()
+    zs = gold["agnesi_8x6"].copy()
+    zp = gold[f"profile_{tag}_z"].astype(np.float64)
+    thp = gold[f"profile_{tag}_theta"].astype(np.float64)
+    r0, t0, p0 = (np.empty(nx * ny * nz) for _ in range(3))
+    assert lib.vibe_mi_reference_state_from_profile(nx, ny, nz, 20000.0, pd(z), pd(zs), len(zp), pd(zp), pd(thp),
+                                                    p_sfc, pd(r0), pd(t0), pd(p0)) == 0
+    assert np.array_equal(t0, gold[f"from_profile_{tag}_theta0"])
+    assert np.array_equal(p0, gold[f"from_profile_{tag}_pi0"])
+    assert np.array_equal(r0, gold[f"from_profile_{tag}_rho0"])
+    RD, CP, CV, G, P0 = 287.05, 1004.64, 717.63, 9.80665, 1e5
+    # EOS in the reference's form rho0 = p00 pi0^(cv/Rd) / (Rd theta0)
+    # (reference_state.cpp:26-28; its cp - cv differs from Rd by 0.04)
+    assert np.max(np.abs(r0 * RD * t0 / (P0 * p0 ** (CV / RD)) - 1.0)) < 1e-13
+    # hydrostatic integration as stored: pi0 of cell k is pi at its upper
+    # interface, so pi0(k) - pi0(k-1) = -g dz_k / (cp theta0(k)) exactly (to
+    # rounding), with pi_s = (p_s/p00)^(Rd/cp) below the first cell
+    pi3, th3 = p0.reshape(nz, ny, nx), t0.reshape(nz, ny, nx)
+    H = 20000.0 - zs.reshape(ny, nx)
+    dz = np.diff(z)[:, None, None] * H[None]
+    below = np.concatenate([np.full((1, ny, nx), (p_sfc / P0) ** (RD / CP)), pi3[:-1]])
+    step = -G * dz / (CP * th3)
+    assert np.max(np.abs((pi3 - below) - step) / np.abs(step)) < 1e-9
+    # bad arguments: fewer than two profile points
+    assert lib.vibe_mi_reference_state_from_profile(nx, ny, nz, 20000.0, pd(z), pd(zs), 1, pd(zp), pd(thp),
+                                                    p_sfc, pd(r0), pd(t0), pd(p0)) == 2
+
+
 def test_energy_budget_definitions_match_reference(gold):
     """The oracle's compensated totals (the GPU's checker) on the state of
     oracle/ref_setup.cpp against the reference's energy_budget

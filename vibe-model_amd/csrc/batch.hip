@@ -591,6 +591,17 @@ int vibe_mi_batch_nest_relax(vibe_mi_batch *b, int child, int parent, const vibe
   return 0;
 }
 
+int vibe_mi_batch_nest_restrict(vibe_mi_batch *b, int child, int parent, const vibe_mi_nest_desc *nd) {
+  if (b == nullptr) return VIBE_C_ERR_BAD_ARG;
+  const int n = static_cast<int>(b->doms.size());
+  if (child < 0 || child >= n || parent < 0 || parent >= n) return VIBE_C_ERR_BAD_ARG;
+  vibe_mi_domain *p = b->doms[parent];
+  if (int rc = nest_restrict_enqueue(b->doms[child], p, nd, b->stream)) return rc;
+  VMI_HIP(hipEventRecord(b->ev[5], b->stream));
+  VMI_HIP(hipStreamWaitEvent(p->stream, b->ev[5], 0));
+  return 0;
+}
+
 int vibe_mi_batch_create(const vibe_mi_domain_desc *descs, int ndom, vibe_mi_batch **out) {
   if (descs == nullptr || out == nullptr || ndom <= 0) return VIBE_C_ERR_BAD_ARG;
   *out = nullptr;

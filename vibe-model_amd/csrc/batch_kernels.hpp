@@ -188,7 +188,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kS
       sa.rhu64 = dd.rhu64, sa.rhv64 = dd.rhv64, sa.r0w64 = dd.r0w64, sa.sx64 = dd.sx64, sa.sy64 = dd.sy64;
       sa.zeta64 = dd.zeta64, sa.rdx64 = dd.rdx64, sa.rdy64 = dd.rdy64;
     }
-    substep_row<T, T, O, NW, MAXCH, GM, CP>(g, sa, tl.y, tl.z);
+    substep_row<T, T, O, NW, MAXCH, GM, CP, kSubstepIncrement<T, T, CP>>(g, sa, tl.y, tl.z);
   });
 }
 

@@ -87,6 +87,9 @@ int domain_create(const vibe_mi_domain_desc *ds, vibe_mi_domain **out);
 // `parent` enqueued on `stream`, with no host synchronisation (nest.hip).
 int nest_enqueue(vibe_mi_domain *child, const vibe_mi_domain *parent, const vibe_mi_nest_desc *nd, double frac,
                  int full, hipStream_t stream);
+// Enqueues the child -> parent feedback (nest.h) on `stream`.
+int nest_restrict_enqueue(const vibe_mi_domain *child, vibe_mi_domain *parent, const vibe_mi_nest_desc *nd,
+                          hipStream_t stream);
 // Factorise the HEVI matrix of table slot `slot` for dtau (cached per slot).
 int domain_ensure_table(vibe_mi_domain *d, int slot, double dtau);
 // Base of table slot `slot` (layout of HeviTab, entries of type T).

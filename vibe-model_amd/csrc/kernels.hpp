@@ -642,6 +642,14 @@ struct SubArgs {
 template <typename T, int NW, int MAXCH, bool CP = false>
 inline constexpr int kSubstepWavesPerEu = (sizeof(T) == 4 && MAXCH <= 12 && !CP) ? 4 : 1;
 
+// Increment form of the plain FP32 substep (compile-time switch
+// VMI_FP32_INCREMENT): rho', pi' and w updated as old value + increment.
+#ifndef VMI_FP32_INCREMENT
+#define VMI_FP32_INCREMENT 0
+#endif
+template <typename T, typename I, bool CP>
+inline constexpr bool kSubstepIncrement = VMI_FP32_INCREMENT != 0 && sizeof(T) == 4 && sizeof(I) == 4 && !CP;
+
 // Row of 64 columns starting at x-block bx, row j; all 64*NW threads of the
 // workgroup call it together (it synchronises the workgroup).
 //
@@ -667,9 +675,15 @@ inline constexpr int kSubstepWavesPerEu = (sizeof(T) == 4 && MAXCH <= 12 && !CP)
 // A w_new = w - gk (ao dpi'_old + an dP^ao) of the plain form (P^ao with the
 // ao part of the old-w flux), and the closing flux rho0_w (w - M) + an rho0_w dw
 // equals rho0_w (an w_new + ao w - M).
-template <typename T, typename I, typename O, int NW, int MAXCH, int GM, bool CP = false>
+template <typename T, typename I, typename O, int NW, int MAXCH, int GM, bool CP = false, bool INC = false>
 __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I, O> &sa, int bx, int j) {
   static_assert(!CP || (sizeof(T) == 4 && sizeof(I) == 4), "compensated storage is FP32");
+  static_assert(!INC || (sizeof(T) == 4 && sizeof(I) == 4), "the increment form is an FP32 form");
+  // IN: updates formed as increments of the old state (rho', pi' and w each
+  // rounded once at full magnitude); RM: bf16 remainders and FP64 mass-flux
+  // sums on top of it (the compensated mode)
+  constexpr bool IN = CP || INC;
+  constexpr bool RM = CP;
   const I *__restrict__ u = sa.u;
   const I *__restrict__ v = sa.v;
   const I *__restrict__ w = sa.w;
@@ -688,8 +702,8 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   T *s_td = reinterpret_cast<T *>(smem_raw);  // [nz-1][64] old-w RHS part -> RHS -> d' -> w
   T *s_be = s_td + nlds;                      // [nz-1][64] beta
   T *s_cp = s_be + nlds;                      // [nz-1][64] c'
-  // CP: the slope and vertical mass fluxes are formed in FP64
-  using TF = std::conditional_t<CP, double, T>;
+  // RM: the slope and vertical mass fluxes are formed in FP64
+  using TF = std::conditional_t<RM, double, T>;
   __shared__ TF s_mc[NW][64], s_fh[NW][64];
   __shared__ T s_ph[NW][64];
   const int lane = threadIdx.x & 63;
@@ -750,7 +764,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
       x[3] = T(gld(v, vN + s));
       x[4] = T(gld(w, vC + s));
       x[5] = T(gld(rho, vC + s));
-      if constexpr (CP) {
+      if constexpr (RM) {
         const uint32_t sl = static_cast<uint32_t>(k) * lsL;
         x[6] = bf16_val(gld(sa.ul, lC + sl));
         x[7] = bf16_val(gld(sa.vl, lC + sl));
@@ -763,7 +777,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
     // west/east u faces and south/north v faces, kappa, gk and r0w of the
     // interface below the level
     enum { GUW, GUE, GVS, GVN, HUW, HUE, HVS, HVN, KAP, GKB, R0B, NG };
-    // CP: the mass-flux products D rho0 at the faces and r0w come in FP64
+    // RM: the mass-flux products D rho0 at the faces and r0w come in FP64
     // (qd: HUW, HUE, HVS, HVN, R0B) in place of their FP32 values
     auto ld_geo = [&](int k, T *q, double *qd) {
       const uint32_t s = static_cast<uint32_t>(k) * lsT;
@@ -771,7 +785,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
       q[GUE] = colE ? gldc(g.cthu, tE + s) : q[GUW];
       q[GVS] = gldc(g.cthv, tC + s);
       q[GVN] = colN ? gldc(g.cthv, tN + s) : q[GVS];
-      if constexpr (CP) {
+      if constexpr (RM) {
         const uint32_t s8 = static_cast<uint32_t>(k) * lsT8;
         qd[0] = gldc(sa.rhu64, tC8 + s8);
         qd[1] = colE ? gldc(sa.rhu64, tE8 + s8) : qd[0];
@@ -814,8 +828,8 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
       for (int q = 0; q < 5; ++q) gl[q] = T(0);
     }
     ld5(min(k0 + 1, nz - 1), P[1]);
-    T X[2][CP ? 11 : 6], Gq[2][NG];
-    double Gd[2][CP ? 5 : 1];
+    T X[2][RM ? 11 : 6], Gq[2][NG];
+    double Gd[2][RM ? 5 : 1];
     ld_state(k0, X[0]);
     ld_geo(k0, Gq[0], Gd[0]);
     TF mc_prev = TF(0);
@@ -866,11 +880,11 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         }
         // FP32 buffers hold u - ubg (see Geo::ubg); un_* are the full winds
         F2 cu{}, cv{};
-        if constexpr (CP) {
+        if constexpr (RM) {
           cu = comp_add(xs[0], xs[6], -(cpdt * (gs[GUW] * px_w)));
           cv = comp_add(xs[2], xs[7], -(cpdt * (gs[GVS] * py_s)));
         }
-        const T du_w = CP ? cu.hi : xs[0] - cpdt * (gs[GUW] * px_w);
+        const T du_w = RM ? cu.hi : xs[0] - cpdt * (gs[GUW] * px_w);
         const T du_e = xs[1] - cpdt * (gs[GUE] * px_e);
         T un_w = du_w, un_e = du_e;
         if constexpr (sizeof(T) == 4) {
@@ -878,12 +892,12 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
           un_w = ubk + du_w;
           un_e = ubk + du_e;
         }
-        const T vn_s = CP ? cv.hi : xs[2] - cpdt * (gs[GVS] * py_s);
+        const T vn_s = RM ? cv.hi : xs[2] - cpdt * (gs[GVS] * py_s);
         const T vn_n = xs[3] - cpdt * (gs[GVN] * py_n);
         if (active) {
           if constexpr (sizeof(O) == 8) {
             double uf = static_cast<double>(du_w), vf = static_cast<double>(vn_s);
-            if constexpr (CP) {
+            if constexpr (RM) {
               uf += static_cast<double>(cu.lo);
               vf += static_cast<double>(cv.lo);
             }
@@ -893,7 +907,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
           } else {
             gst(uo, oC + so, static_cast<O>(du_w));
             gst(vo, oC + so, static_cast<O>(vn_s));
-            if constexpr (CP) {
+            if constexpr (RM) {
               const uint32_t sl = static_cast<uint32_t>(k) * lsL;
               gst(sa.ulo, lC + sl, bf16_bits(cu.lo));
               gst(sa.vlo, lC + sl, bf16_bits(cv.lo));
@@ -902,7 +916,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         }
         TF mc = TF(0);
         T hd;
-        if constexpr (CP) {
+        if constexpr (RM) {
           // full new winds hi + lo (+ background) in FP64; the face mass
           // fluxes D rho0 u (about 1e5) are differenced across the cell in
           // FP64 from FP64 geometry, as is the slope flux
@@ -934,23 +948,26 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
           }
         }
         const T cq = gs[KAP] * dtD;
-        // CP: E and Q hold the increments of rho' and pi'
-        E[c] = CP ? -(dtD * hd) : xs[5] - dtD * hd;
-        Q[c] = CP ? -(cq * hd) : pc[0] - cq * hd;
+        // IN: E and Q hold the increments of rho' and pi'
+        E[c] = IN ? -(dtD * hd) : xs[5] - dtD * hd;
+        Q[c] = IN ? -(cq * hd) : pc[0] - cq * hd;
         const T wk = xs[4];
         Fh[c] = TF(0);
         if (k >= 1) {
-          s_td[(k - 1) * 64 + lane] = CP ? -(gs[GKB] * ao * (pc[0] - pmC)) : wk - gs[GKB] * ao * (pc[0] - pmC);
+          s_td[(k - 1) * 64 + lane] = IN ? -(gs[GKB] * ao * (pc[0] - pmC)) : wk - gs[GKB] * ao * (pc[0] - pmC);
           TF awk, r0k;
-          if constexpr (CP) {  // the full old-w flux
+          if constexpr (RM) {  // the full old-w flux
             awk = static_cast<double>(wk) + static_cast<double>(xs[10]);
             r0k = gd[4];
+          } else if constexpr (IN) {
+            awk = wk;
+            r0k = gs[R0B];
           } else {
             awk = ao * wk;
             r0k = gs[R0B];
           }
           if (c > 0) {
-            const TF zk = CP ? TF(C(sa.zeta64)[k]) : TF(C(g.zeta)[k]);
+            const TF zk = RM ? TF(C(sa.zeta64)[k]) : TF(C(g.zeta)[k]);
             const TF Mw = (TF(1) - zk) * (TF(0.5) * (mc_prev + mc));
             Fh[c] = r0k * (awk - Mw);
           } else {
@@ -974,7 +991,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   // scaled by 1/pivot for the factored sweep.
   if (nlev > 0) {
     if (k0 >= 1) {
-      const TF zk = CP ? TF(C(sa.zeta64)[k0]) : TF(C(g.zeta)[k0]);
+      const TF zk = RM ? TF(C(sa.zeta64)[k0]) : TF(C(g.zeta)[k0]);
       const TF Mw = (TF(1) - zk) * (TF(0.5) * (s_mc[wv - 1][lane] + mc_first));
       Fh[0] = r0w_first * (aw_first - Mw);
     }
@@ -989,7 +1006,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         const int k = k0 + c;
         const T ck = gldc(sa.tab.CK, tC + static_cast<uint32_t>(k) * lsT);
         T qf = Q[c];
-        if constexpr (CP) qf = T(gld(pi, vC + static_cast<uint32_t>(k) * lsI)) + qf;
+        if constexpr (IN) qf = T(gld(pi, vC + static_cast<uint32_t>(k) * lsI)) + qf;
         s_ph[wv][lane] = qf - ck * static_cast<T>(fh_next_chunk - Fh[c]);
       }
     }
@@ -1005,12 +1022,12 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
       t[2] = gldc(sa.tab.INV, sm);
       t[3] = gldc(sa.tab.A, sm);
       t[4] = gldc(sa.tab.CP, sm);
-      if constexpr (CP) t[5] = T(gld(pi, vC + static_cast<uint32_t>(k) * lsI));
+      if constexpr (IN) t[5] = T(gld(pi, vC + static_cast<uint32_t>(k) * lsI));
     };
     const TF fh_next_chunk = (k0 + nlev < nz) ? s_fh[wv + 1][lane] : TF(0);
     T php = (k0 >= 1) ? s_ph[wv - 1][lane] : T(0);
     // operands two levels ahead (the level's own work is short), unconditional
-    T TB[3][CP ? 6 : 5];
+    T TB[3][IN ? 6 : 5];
     ld_tab(k0, TB[0]);
     ld_tab(k0 + min(1, nlev - 1), TB[1]);
 #pragma unroll
@@ -1020,7 +1037,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         const int k = k0 + c;
         const T *tb = TB[c % 3];
         const TF fhn = (c + 1 < nlev) ? Fh[(c + 1) < MAXCH ? c + 1 : c] : fh_next_chunk;
-        const T ph = (CP ? tb[5] + Q[c] : Q[c]) - tb[0] * static_cast<T>(fhn - Fh[c]);
+        const T ph = (IN ? tb[5] + Q[c] : Q[c]) - tb[0] * static_cast<T>(fhn - Fh[c]);
         if (k >= 1) {
           const int m = k - 1;
           const T td = s_td[m * 64 + lane] - tb[1] * an * (ph - php);
@@ -1124,26 +1141,31 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
       const uint32_t s = static_cast<uint32_t>(k) * lsT;
       t[0] = gldc(g.r0w, tC + (s + lsT));  // r0w has nz+1 levels
       t[1] = gldc(g.kap, tC + s);
-      if constexpr (CP) {  // FP64 r0w above, and the old state of the level: rho', pi', w (hi, lo)
-        t8[0] = gldc(sa.r0w64, tC8 + static_cast<uint32_t>(k + 1) * lsT8);
-        const uint32_t si = static_cast<uint32_t>(k) * lsI, sl = static_cast<uint32_t>(k) * lsL;
+      if constexpr (IN) {  // the old state of the level: rho', pi', w (and RM: their remainders, FP64 r0w above)
+        const uint32_t si = static_cast<uint32_t>(k) * lsI;
         t[2] = T(gld(rho, vC + si));
-        t[3] = bf16_val(gld(sa.rl, lC + sl));
         t[4] = T(gld(pi, vC + si));
-        t[5] = bf16_val(gld(sa.pl, lC + sl));
         t[6] = T(gld(w, vC + si));
-        t[7] = bf16_val(gld(sa.wl, lC + sl));
+        if constexpr (RM) {
+          t8[0] = gldc(sa.r0w64, tC8 + static_cast<uint32_t>(k + 1) * lsT8);
+          const uint32_t sl = static_cast<uint32_t>(k) * lsL;
+          t[3] = bf16_val(gld(sa.rl, lC + sl));
+          t[5] = bf16_val(gld(sa.pl, lC + sl));
+          t[7] = bf16_val(gld(sa.wl, lC + sl));
+        } else {
+          t[3] = t[5] = t[7] = T(0);
+        }
       }
     };
     const TF fh_next_chunk = (k0 + nlev < nz) ? s_fh[wv + 1][lane] : TF(0);
-    T DD[3][CP ? 8 : 2];
+    T DD[3][IN ? 8 : 2];
     double DD8[3][1];
     ld_d(k0, DD[0], DD8[0]);
     ld_d(k0 + min(1, nlev - 1), DD[1], DD8[1]);
     // flux through the chunk's lower interface
     TF flo = TF(0);
     if (k0 >= 1) {
-      if constexpr (CP)
+      if constexpr (RM)
         flo = Fh[0] + (static_cast<double>(an) * gldc(sa.r0w64, tC8 + static_cast<uint32_t>(k0) * lsT8)) *
                           static_cast<double>(s_td[(k0 - 1) * 64 + lane]);
       else
@@ -1159,14 +1181,16 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         const TF fhn = (c + 1 < nlev) ? Fh[(c + 1) < MAXCH ? c + 1 : c] : fh_next_chunk;
         TF fhi = TF(0);
         if (k + 1 < nz) {
-          if constexpr (CP)
+          if constexpr (RM)
             fhi = fhn + (static_cast<double>(an) * DD8[c % 3][0]) * static_cast<double>(s_td[k * 64 + lane]);
           else
             fhi = fhn + (an * dd[0]) * s_td[k * 64 + lane];
         }
         const T vd = -dtD * (static_cast<T>(fhi - flo) * C(g.rdzeta)[k]);
         F2 cr{}, cpn{}, cw{};
-        if constexpr (CP) {
+        // IN: increments of rho', pi' and w, each added to the old value once
+        const T dr = E[c] + vd, dp = Q[c] + dd[1] * vd;
+        if constexpr (RM) {
           cr = comp_add(dd[2], dd[3], E[c] + vd);
           cpn = comp_add(dd[4], dd[5], Q[c] + dd[1] * vd);
           if (k >= 1) {
@@ -1176,16 +1200,29 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
             cw.lo = static_cast<float>(mc0 - static_cast<double>(cw.hi));
           }
         }
-        const T rn = CP ? cr.hi : E[c] + vd;
-        const T pn = CP ? cpn.hi : Q[c] + dd[1] * vd;
-        const T wn = CP ? cw.hi : (k >= 1) ? s_td[(k - 1) * 64 + lane] : static_cast<T>(mc0);
+        T rn, pn, wn;
+        if constexpr (RM) {
+          rn = cr.hi, pn = cpn.hi, wn = cw.hi;
+        } else if constexpr (IN) {
+          rn = dd[2] + dr;
+          pn = dd[4] + dp;
+          wn = (k >= 1) ? dd[6] + s_td[(k - 1) * 64 + lane] : static_cast<T>(mc0);
+        } else {
+          rn = E[c] + vd;
+          pn = Q[c] + dd[1] * vd;
+          wn = (k >= 1) ? s_td[(k - 1) * 64 + lane] : static_cast<T>(mc0);
+        }
         if (active) {
           if constexpr (sizeof(O) == 8) {
             double rf = static_cast<double>(rn), pf = static_cast<double>(pn), wf = static_cast<double>(wn);
-            if constexpr (CP) {
+            if constexpr (RM) {
               rf += static_cast<double>(cr.lo);
               pf += static_cast<double>(cpn.lo);
               wf += static_cast<double>(cw.lo);
+            } else if constexpr (IN) {  // the FP64 output takes the increment unrounded
+              rf = static_cast<double>(dd[2]) + static_cast<double>(dr);
+              pf = static_cast<double>(dd[4]) + static_cast<double>(dp);
+              if (k >= 1) wf = static_cast<double>(dd[6]) + static_cast<double>(s_td[(k - 1) * 64 + lane]);
             }
             gst(rho_o, oC + so, sponge_scale(sa.sp, k, nz, rf));
             gst(pi_o, oC + so, sponge_scale(sa.sp, k, nz, pf));
@@ -1194,7 +1231,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
             gst(rho_o, oC + so, static_cast<O>(rn));
             gst(pi_o, oC + so, static_cast<O>(pn));
             gst(wo, oC + so, static_cast<O>(wn));
-            if constexpr (CP) {
+            if constexpr (RM) {
               const uint32_t sl = static_cast<uint32_t>(k) * lsL;
               gst(sa.rlo, lC + sl, bf16_bits(cr.lo));
               gst(sa.plo, lC + sl, bf16_bits(cpn.lo));
@@ -1210,7 +1247,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
       const uint32_t s = static_cast<uint32_t>(nz);
       double wt = static_cast<double>(T(gld(w, vC + s * lsI)));
       uint16_t wtl = 0;
-      if constexpr (CP) {
+      if constexpr (RM) {
         wtl = gld(sa.wl, lC + s * lsL);
         wt += static_cast<double>(bf16_val(wtl));
       }
@@ -1218,7 +1255,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         gst(wo, oC + s * lsO, sponge_scale(sa.sp, nz, nz, wt));
       } else {
         gst(wo, oC + s * lsO, static_cast<O>(T(gld(w, vC + s * lsI))));
-        if constexpr (CP) gst(sa.wlo, lC + s * lsL, wtl);
+        if constexpr (RM) gst(sa.wlo, lC + s * lsL, wtl);
       }
     }
     if (check && bad && active) atomicOr(flag, 1);
@@ -1227,7 +1264,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
 
 template <typename T, typename I, typename O, int NW, int MAXCH, bool CP = false>
 __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kSubstepWavesPerEu<T, NW, MAXCH, CP>))) k_substep(Geo<T> g, SubArgs<T, I, O> sa) {
-  substep_row<T, I, O, NW, MAXCH, kGeoAny, CP>(g, sa, blockIdx.x, blockIdx.y);
+  substep_row<T, I, O, NW, MAXCH, kGeoAny, CP, kSubstepIncrement<T, I, CP>>(g, sa, blockIdx.x, blockIdx.y);
 }
 
 // ---- sponge (damping.cpp:76-138 with perturbation targets 0) -------------

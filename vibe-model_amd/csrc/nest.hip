@@ -1,6 +1,7 @@
-// nest.hip -- one-way nest lateral boundary on the device: bilinear
-// parent -> child interpolation and Davies relaxation of the child's lateral
-// zone (src/grid/nest.cpp:34-128 roles), for domains on the same GPU.
+// nest.hip -- nest boundaries on the device: bilinear parent -> child
+// interpolation, Davies relaxation of the child's lateral zone and the
+// child -> parent feedback (src/grid/nest.cpp:34-128 roles), for domains on
+// the same GPU.
 //
 // One thread per child point (i, j, k) loops over the six prognostics (w has
 // one level more).  The relaxation touches only the zone columns (a thin
@@ -78,7 +79,74 @@ __global__ void __launch_bounds__(256) k_nest(NestArgs a) {
   }
 }
 
+// Two-way feedback: one thread per covered parent cell (I0 + i, J0 + j) and
+// level.  Cell fields and w take the mean of the r x r child cells over the
+// parent cell (accumulated jj-outer, ii-inner, times 1/r^2:
+// nest.cpp:85-108); u (west face) the mean of the r child faces on the
+// parent face, v likewise.
+struct RestrictArgs {
+  const double *c[6];  // child, current state
+  double *p[6];        // parent, current state
+  int pnx, pny, cnx, cny, nz, r, I0, J0, iext, jext;
+};
+
+__global__ void __launch_bounds__(64) k_nest_restrict(RestrictArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+  if (i >= a.iext) return;
+  const size_t pxy = static_cast<size_t>(a.pnx) * a.pny, cxy = static_cast<size_t>(a.cnx) * a.cny;
+  const int r = a.r;
+  const double w2 = 1.0 / static_cast<double>(r * r), w1 = 1.0 / static_cast<double>(r);
+  const size_t pc = static_cast<size_t>(wrap_mod(a.J0 + j, a.pny)) * a.pnx + wrap_mod(a.I0 + i, a.pnx);
+  for (int f = 0; f < 6; ++f) {
+    if (k >= (f == 2 ? a.nz + 1 : a.nz)) continue;
+    const double *__restrict__ src = a.c[f] + static_cast<size_t>(k) * cxy;
+    double acc = 0.0, wt = w2;
+    if (f == 0) {
+      for (int jj = 0; jj < r; ++jj) acc += src[static_cast<size_t>(j * r + jj) * a.cnx + i * r];
+      wt = w1;
+    } else if (f == 1) {
+      for (int ii = 0; ii < r; ++ii) acc += src[static_cast<size_t>(j * r) * a.cnx + i * r + ii];
+      wt = w1;
+    } else {
+      for (int jj = 0; jj < r; ++jj)
+        for (int ii = 0; ii < r; ++ii) acc += src[static_cast<size_t>(j * r + jj) * a.cnx + i * r + ii];
+    }
+    a.p[f][static_cast<size_t>(k) * pxy + pc] = acc * wt;
+  }
+}
+
 }  // namespace
+
+int nest_restrict_enqueue(const vibe_mi_domain *child, vibe_mi_domain *parent, const vibe_mi_nest_desc *nd,
+                          hipStream_t stream) {
+  if (child == nullptr || parent == nullptr || nd == nullptr || child == parent) return VIBE_C_ERR_BAD_ARG;
+  if (nd->ratio < 2 || child->nz != parent->nz || nd->i0 % nd->ratio != 0 || nd->j0 % nd->ratio != 0 ||
+      child->nx < nd->ratio || child->ny < nd->ratio)
+    return VIBE_C_ERR_BAD_ARG;
+  VMI_HIP(hipSetDevice(child->device));
+  const FieldSet &cs = child->S[child->cur0];
+  const FieldSet &ps = parent->S[parent->cur0];
+  RestrictArgs a{};
+  const double *cf[6] = {cs.u, cs.v, cs.w, cs.rho, cs.th, cs.pi};
+  double *pf[6] = {ps.u, ps.v, ps.w, ps.rho, ps.th, ps.pi};
+  for (int f = 0; f < 6; ++f) {
+    a.c[f] = cf[f];
+    a.p[f] = pf[f];
+  }
+  a.pnx = parent->nx;
+  a.pny = parent->ny;
+  a.cnx = child->nx;
+  a.cny = child->ny;
+  a.nz = child->nz;
+  a.r = nd->ratio;
+  a.I0 = nd->i0 / nd->ratio;
+  a.J0 = nd->j0 / nd->ratio;
+  a.iext = child->nx / nd->ratio;
+  a.jext = child->ny / nd->ratio;
+  hipLaunchKernelGGL(k_nest_restrict, dim3(ceil_div(a.iext, 64), a.jext, a.nz + 1), dim3(64), 0, stream, a);
+  VMI_HIP(hipGetLastError());
+  return 0;
+}
 
 int nest_enqueue(vibe_mi_domain *child, const vibe_mi_domain *parent, const vibe_mi_nest_desc *nd, double frac,
                  int full, hipStream_t stream) {
@@ -150,6 +218,18 @@ int vibe_mi_nest_interpolate(vibe_mi_domain *child, vibe_mi_domain *parent, cons
 
 int vibe_mi_nest_relax(vibe_mi_domain *child, vibe_mi_domain *parent, const vibe_mi_nest_desc *nd, double frac) {
   const int rc = nest_launch(child, parent, nd, frac, 0);
+  if (rc) set_last_error(rc);
+  return rc;
+}
+
+int vibe_mi_nest_restrict(vibe_mi_domain *child, vibe_mi_domain *parent, const vibe_mi_nest_desc *nd) {
+  if (child == nullptr || parent == nullptr) return VIBE_C_ERR_BAD_ARG;
+  int rc = 0;
+  if (hipSetDevice(parent->device) != hipSuccess || hipStreamSynchronize(child->stream) != hipSuccess) {
+    rc = VIBE_MI_ERR_HIP;
+  } else if ((rc = nest_restrict_enqueue(child, parent, nd, parent->stream)) == 0) {
+    if (hipStreamSynchronize(parent->stream) != hipSuccess) rc = VIBE_MI_ERR_HIP;
+  }
   if (rc) set_last_error(rc);
   return rc;
 }

@@ -225,6 +225,52 @@ int vibe_mi_reference_state(int kind, double theta_sfc, double bv, double p_sfc,
   return 0;
 }
 
+// ReferenceState::from_profile (reference_state.cpp:61-110), column by column:
+// interface heights z_k = zs + zeta_k (ztop - zs) (Grid::z_interface); theta at
+// an interface is the linear interpolant of the (z, theta) profile, held at
+// its end values outside it (the profile segment is found by the same forward
+// search); pi is integrated upward from pi_s = (p_s / p00)^(Rd/cp) by
+// pi -= g dz / (cp theta_mid), theta_mid the mean of the two interface values.
+// Cell k takes theta0 = theta_mid and pi0 = the integrated value at its UPPER
+// interface (the reference stores pi after the decrement, so its pi0 sits half
+// a cell above the cell centre; kept for parity), rho0 from the equation of
+// state.  The arithmetic is the reference's, operation for operation.
+int vibe_mi_reference_state_from_profile(int nx, int ny, int nz, double ztop, const double *zeta,
+                                         const double *zs, int np, const double *zp, const double *thp,
+                                         double p_sfc, double *rho0, double *theta0, double *pi0) {
+  if (nx <= 0 || ny <= 0 || nz <= 0 || np < 2 || zeta == nullptr || zs == nullptr || zp == nullptr ||
+      thp == nullptr || rho0 == nullptr || theta0 == nullptr || pi0 == nullptr)
+    return 2;
+  const size_t n = static_cast<size_t>(np);
+  auto theta_at = [&](double z) {
+    if (z <= zp[0]) return thp[0];
+    if (z >= zp[n - 1]) return thp[n - 1];
+    size_t k = 0;
+    while (k + 2 < n && zp[k + 1] < z) ++k;
+    const double w = (zp[k + 1] - zp[k] > 0.0) ? (z - zp[k]) / (zp[k + 1] - zp[k]) : 0.0;
+    return thp[k] + w * (thp[k + 1] - thp[k]);
+  };
+  const double pis = std::pow(p_sfc / kP0, kRd / kCp);
+  const size_t nxy = static_cast<size_t>(nx) * ny;
+  std::vector<double> zi(static_cast<size_t>(nz) + 1);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      const size_t col = static_cast<size_t>(j) * nx + i;
+      for (int k = 0; k <= nz; ++k) zi[k] = zs[col] + zeta[k] * (ztop - zs[col]);
+      double pi = pis;
+      for (int k = 0; k < nz; ++k) {
+        const double th_lo = theta_at(zi[k]), th_hi = theta_at(zi[k + 1]);
+        const double th_mid = 0.5 * (th_lo + th_hi);
+        pi -= kG * (zi[k + 1] - zi[k]) / (kCp * th_mid);
+        const size_t c = static_cast<size_t>(k) * nxy + col;
+        theta0[c] = th_mid;
+        pi0[c] = pi;
+        rho0[c] = kP0 * std::pow(pi, kCv / kRd) / (kRd * th_mid);
+      }
+    }
+  return 0;
+}
+
 int vibe_mi_sponge(int nz, double ztop, const double *zeta, double zs00, double start_fraction,
                    double alpha_max, double *alpha) {
   if (nz <= 0 || zeta == nullptr || alpha == nullptr) return 2;

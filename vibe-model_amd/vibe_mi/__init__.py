@@ -157,6 +157,8 @@ def lib() -> ctypes.CDLL:
         "vibe_mi_nest_interpolate": (_I, [_VP, _VP, ctypes.POINTER(NestDesc)]),
         "vibe_mi_nest_relax": (_I, [_VP, _VP, ctypes.POINTER(NestDesc), _D]),
         "vibe_mi_batch_nest_relax": (_I, [_VP, _I, _I, ctypes.POINTER(NestDesc), _D]),
+        "vibe_mi_nest_restrict": (_I, [_VP, _VP, ctypes.POINTER(NestDesc)]),
+        "vibe_mi_batch_nest_restrict": (_I, [_VP, _I, _I, ctypes.POINTER(NestDesc)]),
         "vibe_mi_helmholtz_create": (_I, [ctypes.POINTER(HelmholtzDesc), ctypes.POINTER(_VP)]),
         "vibe_mi_helmholtz_destroy": (_I, [_VP]),
         "vibe_mi_helmholtz_set_coefficients": (_I, [_VP, _PD, _PD]),
@@ -171,6 +173,7 @@ def lib() -> ctypes.CDLL:
         "vibe_mi_terrain_smooth": (_I, [_PD, _I, _I, _I, _D]),
         "vibe_mi_terrain_gaussian": (_I, [_I, _I, _D, _D, _D, _D, ctypes.c_uint64, _PD]),
         "vibe_mi_reference_state": (_I, [_I, _D, _D, _D, _I, _I, _I, _D, _PD, _PD, _PD, _PD, _PD]),
+        "vibe_mi_reference_state_from_profile": (_I, [_I, _I, _I, _D, _PD, _PD, _I, _PD, _PD, _D, _PD, _PD, _PD]),
         "vibe_mi_sponge": (_I, [_I, _D, _PD, _D, _D, _D, _PD]),
         "vibe_mi_ic_bubble": (_I, [_I, _I, _I, _D, _D, _D, _PD, _PD, _D, _D, _D, _D, _D, _D, _PD]),
         "vibe_mi_ic_noise": (_I, [_I, _I, _I, _D, _PD, _PD, _D, _D, ctypes.c_uint64, _PD]),
@@ -726,6 +729,18 @@ def batch_nest_relax(batch: "Batch", child: int, parent: int, desc: NestDesc, fr
     behind its last step without a host synchronisation."""
     _check(lib().vibe_mi_batch_nest_relax(batch.h, child, parent, ctypes.byref(desc), frac),
            "vibe_mi_batch_nest_relax")
+
+
+def nest_restrict(child: "Domain", parent: "Domain", desc: NestDesc):
+    """Two-way feedback: the parent cells covered by the child take the mean
+    of the child state over them (vibe_mi/nest.h)."""
+    _check(lib().vibe_mi_nest_restrict(child.h, parent.h, ctypes.byref(desc)), "vibe_mi_nest_restrict")
+
+
+def batch_nest_restrict(batch: "Batch", child: int, parent: int, desc: NestDesc):
+    """nest_restrict between members of one batch, on the batch stream."""
+    _check(lib().vibe_mi_batch_nest_restrict(batch.h, child, parent, ctypes.byref(desc)),
+           "vibe_mi_batch_nest_restrict")
 
 
 HELM_BICGSTAB = 0

@@ -9,18 +9,21 @@ active domains run in the same batched launches (vibe_mi.Batch).  After each
 sub-step every nest that just stepped is relaxed toward its parent,
 interpolated bilinearly in space and linearly in time across the parent's
 current step (frac = (m + 1) / r for the nest's m-th step inside it), coarse
-levels first, enqueued on the batch stream with no host round trip.  This is
-the one-way form of the reference's nest operators
+levels first, enqueued on the batch stream with no host round trip.  With
+two_way, after the parent step's last sub-step (all levels at the same time)
+every nest feeds its mean state back onto the parent cells it covers, finest
+level first (batch_nest_restrict).  These are the reference's nest operators
 (src/grid/nest.cpp:52-128; its driver never calls them, driver.cpp:212-219).
 """
 from __future__ import annotations
 
-from . import MIXED, Batch, NestDesc, batch_nest_relax, nest_interpolate
+from . import MIXED, Batch, NestDesc, batch_nest_relax, batch_nest_restrict, nest_interpolate
 
 
 class NestedRun:
     def __init__(self, cases, links, precision: int = MIXED, use_graph: bool = True, zone: int = 5,
-                 alpha_max: float = 1.0):
+                 alpha_max: float = 1.0, two_way: bool = False):
+        self.two_way = two_way
         self.cases = list(cases)
         self.links = [(c, p, NestDesc(r, i0, j0, zone, alpha_max)) for (c, p, r, i0, j0) in links]
         ratios = {r for (_, _, r, _, _) in links}
@@ -72,6 +75,9 @@ class NestedRun:
                     self.batch.step(dts, self.nsub, 1, active)
                 for c, p, d, frac in relax:
                     batch_nest_relax(self.batch, c, p, d, frac)
+            if self.two_way:
+                for c, p, d in reversed(self.order):
+                    batch_nest_restrict(self.batch, c, p, d)
         return ms
 
     def close(self):
