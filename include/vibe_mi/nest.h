@@ -79,6 +79,27 @@ int vibe_mi_nest_restrict(vibe_mi_domain *child, vibe_mi_domain *parent, const v
  * stream with no host synchronisation. */
 int vibe_mi_batch_nest_restrict(vibe_mi_batch *b, int child, int parent, const vibe_mi_nest_desc *nd);
 
+/* A run of parent steps of a nest hierarchy held in one batch: links
+ * (child[q], parent[q], descs[q]), q < nlinks, one ratio r for all of them,
+ * each child with one parent; level l = depth in the tree; dt0 the parent
+ * step.  One parent step is r^L sub-steps: at sub-step s the members of
+ * level l step by dt0 / r^l when s % r^(L-l) == 0, all in the same batched
+ * launches, then every nest that stepped is relaxed toward its parent
+ * (vibe_mi_batch_nest_relax, frac = (m + 1) / r for its m-th step in the
+ * parent's step), coarse levels first; with two_way, after the last sub-step
+ * every nest feeds back onto its parent (vibe_mi_batch_nest_restrict),
+ * finest first.  With graphs on (vibe_mi_batch_set_graph) the whole parent
+ * step is captured once per buffer-rotation state and replayed: one graph
+ * launch per parent step, one host synchronisation per call.  Bit-identical
+ * to the same schedule issued call by call.  Members with a physics callback
+ * are rejected (held rates are fine). */
+int vibe_mi_batch_nested_step(vibe_mi_batch *b, int nlinks, const int *child, const int *parent,
+                              const vibe_mi_nest_desc *descs, int two_way, double dt0, int nsub, int nsteps);
+/* The same, timed with HIP events on the batch stream (ms over the call). */
+int vibe_mi_batch_nested_step_timed(vibe_mi_batch *b, int nlinks, const int *child, const int *parent,
+                                    const vibe_mi_nest_desc *descs, int two_way, double dt0, int nsub, int nsteps,
+                                    double *ms_total);
+
 #ifdef __cplusplus
 }
 #endif

@@ -3,6 +3,10 @@
 * A two-level hierarchy (parent, one ratio-3 nest, one nest inside it)
   advanced by the batched driver matches the same schedule written out by
   hand with single-domain steps and the nest entry points, bit for bit.
+* The library's graph-captured parent step (vibe_mi_batch_nested_step)
+  equals the same schedule issued sub-step by sub-step from Python, bit for
+  bit, over four parent steps (every buffer-rotation state of the parent),
+  one-way and two-way, FP64 and FP32-mixed.
 * A uniform flow over flat terrain is an exact steady state of every
   domain, so parent and nests stay uniform through the relaxation (to the
   rounding of the relaxation's weighted sum).
@@ -81,3 +85,21 @@ def test_batch_nest_relax_rejects_bad_members(vmi):
             vmi.batch_nest_relax(b, c, p, d, 0.5)
         assert e.value.code == vmi.VIBE_C_ERR_BAD_ARG
     b.close()
+
+
+@pytest.mark.parametrize("prec_name", ["FP64", "MIXED"])
+@pytest.mark.parametrize("two_way", [False, True])
+def test_graph_parent_step_equals_substep_schedule(vmi, prec_name, two_way):
+    from vibe_mi.nesting import NestedRun
+    prec = getattr(vmi, prec_name)
+    cases, links = _hier()
+    runs = [NestedRun(cases, links, precision=prec, two_way=two_way, native=nat) for nat in (True, False)]
+    for r in runs:
+        r.step(1)
+        r.step(3)
+    for m in range(len(cases)):
+        a, b = runs[0].members[m].get_state(), runs[1].members[m].get_state()
+        for f in FIELDS:
+            assert np.array_equal(a[f], b[f]), (prec_name, two_way, m, f)
+    for r in runs:
+        r.close()

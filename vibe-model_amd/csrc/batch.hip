@@ -32,8 +32,10 @@
 
 #include "batch_kernels.hpp"
 #include "vibe_mi/batch.h"
+#include "vibe_mi/nest.h"
 
-// FP32 kernels live in batch_f32.hip (compiled with multiply-add contraction)
+// FP32 kernels live in batch_f32_{flat,x,any}.hip (compiled with
+// multiply-add contraction)
 VMI_BATCH_F32_ALL(extern)
 
 using namespace vibe_mi;
@@ -575,9 +577,190 @@ int step_batch(vibe_mi_batch *b, const double *dt, int nsub, int nsteps, const u
   return f ? VIBE_MI_ERR_NONFINITE : 0;
 }
 
+// ---------------------------------------------------------------------------
+// Nest hierarchy: whole parent steps replayed from one graph
+// ---------------------------------------------------------------------------
+// One parent step of a hierarchy of ratio r (levels 0..L) is r^L sub-steps.
+// At sub-step s the members of level l step (dt0 / r^l) when
+// s % r^(L-l) == 0, all in the same batched launches; after it every nest
+// that stepped is relaxed toward its parent (Davies, the parent interpolated
+// in time across its current step, frac = (m + 1) / r for the nest's m-th
+// step in it), coarse levels first; with two_way, after the last sub-step
+// every nest feeds back onto its parent, finest level first.  The whole
+// sequence is enqueued on the batch stream and captured once per buffer
+// rotation state (the nest operators take the state buffers as arguments,
+// and a parent step advances level l's rotation by r^l mod 3), so a run of
+// parent steps costs one graph launch each and a single synchronisation.
+struct NestLink {
+  int child, parent;
+  vibe_mi_nest_desc d;
+};
+
+template <typename T>
+int nested_steps(vibe_mi_batch *b, const std::vector<NestLink> &links, bool two_way, double dt0, int nsub, int nsteps,
+                 bool timed, double *ms_total) {
+  const int n = static_cast<int>(b->doms.size());
+  if (links.empty() || !(dt0 > 0.0)) return VIBE_C_ERR_BAD_ARG;
+  const int r = links[0].d.ratio;
+  std::vector<int> par(n, -1);
+  for (const auto &l : links) {
+    if (l.child < 0 || l.child >= n || l.parent < 0 || l.parent >= n || l.child == l.parent || par[l.child] >= 0 ||
+        l.d.ratio != r || r < 2)
+      return VIBE_C_ERR_BAD_ARG;
+    par[l.child] = l.parent;
+  }
+  std::vector<int> level(n, 0);
+  for (int m = 0; m < n; ++m) {
+    int depth = 0;
+    for (int q = m; par[q] >= 0; q = par[q])
+      if (++depth > n) return VIBE_C_ERR_BAD_ARG;  // a cycle
+    level[m] = depth;
+  }
+  const int L = *std::max_element(level.begin(), level.end());
+  auto ipow = [](int base, int e) {
+    long v = 1;
+    for (int q = 0; q < e; ++q) v *= base;
+    return v;
+  };
+  if (ipow(r, L) > 4096) return VIBE_C_ERR_BAD_ARG;
+  const int R = static_cast<int>(ipow(r, L));
+  std::vector<double> dts(n);
+  for (int m = 0; m < n; ++m) {
+    dts[m] = dt0 / static_cast<double>(ipow(r, level[m]));
+    if (b->doms[m]->ph_fn != nullptr) return VIBE_C_ERR_BAD_ARG;
+  }
+  // links coarse level first (relaxation order)
+  std::vector<NestLink> order = links;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](const NestLink &a, const NestLink &c) { return level[a.child] < level[c.child]; });
+  struct Sub {
+    std::vector<unsigned char> act;
+    WorkList *w = nullptr;
+    std::vector<std::pair<int, double>> relax;  // (index into order, frac)
+  };
+  std::vector<Sub> plan(R);
+  for (int s = 0; s < R; ++s) {
+    Sub &sb = plan[s];
+    sb.act.assign(n, 0);
+    for (int m = 0; m < n; ++m) sb.act[m] = (s % ipow(r, L - level[m]) == 0) ? 1 : 0;
+    for (int q = 0; q < static_cast<int>(order.size()); ++q) {
+      const int c = order[q].child;
+      if (!sb.act[c]) continue;
+      const long span = ipow(r, L - level[c]);
+      sb.relax.emplace_back(q, static_cast<double>((s / span) % r + 1) / static_cast<double>(r));
+    }
+  }
+  if (int rc = upload_desc<T>(b, dts.data(), nsub)) return rc;
+  b->phys = false;
+  for (int m = 0; m < n; ++m)
+    if (b->doms[m]->ph_on) b->phys = true;
+  for (Sub &sb : plan) {
+    if (int rc = work_list(b, sb.act, &sb.w)) return rc;
+    if (int rc = autotune<T>(b, *sb.w)) return rc;
+    if constexpr (sizeof(T) == 4)
+      if (int rc = launch_substep<T, double>(b, sb.w->nw, *sb.w, 0, 0, 0, 1, b->ctr, true)) return rc;
+    if (int rc = launch_substep<T, T>(b, sb.w->nw, *sb.w, 0, 0, 0, 0, b->ctr, true)) return rc;
+  }
+  std::string pkey = "nest|" + std::to_string(nsub) + "|" + std::to_string(two_way ? 1 : 0) + (b->phys ? "|p" : "");
+  for (const auto &l : links)
+    pkey += "|" + std::to_string(l.child) + ">" + std::to_string(l.parent) + ":" + std::to_string(l.d.i0) + "," +
+            std::to_string(l.d.j0) + "," + std::to_string(l.d.zone) + "," + std::to_string(l.d.alpha_max);
+  VMI_HIP(hipMemsetAsync(b->flag, 0, sizeof(int), b->stream));
+  // the parent step's launch sequence with the host's buffer rotation
+  // tracked through it (the nest operators read it when enqueued)
+  auto enqueue_parent_step = [&]() -> int {
+    for (const Sub &sb : plan) {
+      if (int rc = enqueue_step<T>(b, *sb.w, nsub)) return rc;
+      for (int m = 0; m < n; ++m)
+        if (sb.act[m]) b->doms[m]->cur0 = (b->doms[m]->cur0 + 1) % 3;
+      for (const auto &rx : sb.relax) {
+        const NestLink &l = order[rx.first];
+        if (int rc = nest_enqueue(b->doms[l.child], b->doms[l.parent], &l.d, rx.second, 0, b->stream)) return rc;
+      }
+    }
+    if (two_way)
+      for (auto it = order.rbegin(); it != order.rend(); ++it)
+        if (int rc = nest_restrict_enqueue(b->doms[it->child], b->doms[it->parent], &it->d, b->stream)) return rc;
+    return 0;
+  };
+  if (timed) VMI_HIP(hipEventRecord(b->ev[0], b->stream));
+  for (int st = 0; st < nsteps; ++st) {
+    std::string gk = pkey + "|rot";
+    std::vector<int> rot0(n);
+    for (int m = 0; m < n; ++m) {
+      rot0[m] = b->doms[m]->cur0;
+      gk += std::to_string(rot0[m]);
+    }
+    if (!b->use_graph) {
+      if (int rc = enqueue_parent_step()) return rc;
+      continue;
+    }
+    auto it = b->graphs.find(gk);
+    hipGraphExec_t exec = nullptr;
+    if (it == b->graphs.end()) {
+      VMI_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      const int rc = enqueue_parent_step();
+      hipGraph_t captured = nullptr;
+      const hipError_t ec = hipStreamEndCapture(b->stream, &captured);
+      for (int m = 0; m < n; ++m) b->doms[m]->cur0 = rot0[m];  // captured, not run yet
+      if (rc) return rc;
+      VMI_HIP(ec);
+      b->graph_objs.push_back(captured);
+      VMI_HIP(hipGraphInstantiate(&exec, captured, nullptr, nullptr, 0));
+      b->graphs[gk] = exec;
+    } else {
+      exec = it->second;
+    }
+    VMI_HIP(hipGraphLaunch(exec, b->stream));
+    for (int m = 0; m < n; ++m) b->doms[m]->cur0 = static_cast<int>((rot0[m] + ipow(r, level[m])) % 3);
+  }
+  if (timed) VMI_HIP(hipEventRecord(b->ev[1], b->stream));
+  int f = 0;
+  VMI_HIP(hipMemcpyAsync(&f, b->flag, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  VMI_HIP(hipStreamSynchronize(b->stream));
+  if (timed && ms_total) {
+    float ms = 0.f;
+    VMI_HIP(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+    *ms_total = ms;
+  }
+  // later work on a member's own stream is ordered after the batch stream
+  VMI_HIP(hipEventRecord(b->ev[5], b->stream));
+  for (auto *d : b->doms) VMI_HIP(hipStreamWaitEvent(d->stream, b->ev[5], 0));
+  return f ? VIBE_MI_ERR_NONFINITE : 0;
+}
+
+int nested_common(vibe_mi_batch *b, int nlinks, const int *child, const int *parent, const vibe_mi_nest_desc *descs,
+                  int two_way, double dt0, int nsub, int nsteps, bool timed, double *ms_total) {
+  if (b == nullptr || nlinks < 1 || child == nullptr || parent == nullptr || descs == nullptr || nsteps < 0 ||
+      nsub < 1 || 3 * (nsub + 1) + 1 > kMaxLaunches || nsub > kMaxNsub)
+    return VIBE_C_ERR_BAD_ARG;
+  if (hipSetDevice(b->device) != hipSuccess) return VIBE_MI_ERR_HIP;
+  for (auto *d : b->doms)
+    if (hipStreamSynchronize(d->stream) != hipSuccess) return VIBE_MI_ERR_HIP;
+  std::vector<NestLink> links(nlinks);
+  for (int q = 0; q < nlinks; ++q) links[q] = NestLink{child[q], parent[q], descs[q]};
+  const int rc = b->precision != VIBE_MI_FP64
+                     ? nested_steps<float>(b, links, two_way != 0, dt0, nsub, nsteps, timed, ms_total)
+                     : nested_steps<double>(b, links, two_way != 0, dt0, nsub, nsteps, timed, ms_total);
+  if (rc) set_last_error(rc);
+  return rc;
+}
+
 }  // namespace
 
 extern "C" {
+
+int vibe_mi_batch_nested_step(vibe_mi_batch *b, int nlinks, const int *child, const int *parent,
+                              const vibe_mi_nest_desc *descs, int two_way, double dt0, int nsub, int nsteps) {
+  return nested_common(b, nlinks, child, parent, descs, two_way, dt0, nsub, nsteps, false, nullptr);
+}
+
+int vibe_mi_batch_nested_step_timed(vibe_mi_batch *b, int nlinks, const int *child, const int *parent,
+                                    const vibe_mi_nest_desc *descs, int two_way, double dt0, int nsub, int nsteps,
+                                    double *ms_total) {
+  return nested_common(b, nlinks, child, parent, descs, two_way, dt0, nsub, nsteps, true, ms_total);
+}
+
 
 int vibe_mi_batch_nest_relax(vibe_mi_batch *b, int child, int parent, const vibe_mi_nest_desc *nd, double frac) {
   if (b == nullptr) return VIBE_C_ERR_BAD_ARG;

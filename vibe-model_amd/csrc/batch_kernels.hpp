@@ -1,6 +1,7 @@
 // batch_kernels.hpp -- device side of the batched step: member descriptors,
 // tile work lists, the persistent tile loop and the batched kernels.  The
-// FP64 kernels are instantiated in batch.hip; the FP32 ones in batch_f32.hip,
+// FP64 kernels are instantiated in batch.hip; the FP32 ones in
+// batch_f32_{flat,x,any}.hip,
 // which is compiled with multiply-add contraction (the FP64 path keeps
 // -ffp-contract=off so it rounds exactly like the CPU oracle).
 #pragma once
@@ -195,7 +196,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kS
 }  // namespace vibe_mi
 
 // Explicit instantiation of every FP32 batched kernel shape (X = extern for a
-// declaration in batch.hip, empty for the definitions in batch_f32.hip).
+// declaration in batch.hip, empty for the definitions in batch_f32_*.hip).
 #define VMI_BATCH_F32_SUBSTEP_GM(X, O, GM, CP)                                                                     \
   X template __global__ void vibe_mi::kb_substep<float, O, 16, 4, GM, CP>(VMI_KB_SUBSTEP_ARGS);                    \
   X template __global__ void vibe_mi::kb_substep<float, O, 16, 8, GM, CP>(VMI_KB_SUBSTEP_ARGS);                    \

@@ -158,6 +158,10 @@ def lib() -> ctypes.CDLL:
         "vibe_mi_nest_relax": (_I, [_VP, _VP, ctypes.POINTER(NestDesc), _D]),
         "vibe_mi_batch_nest_relax": (_I, [_VP, _I, _I, ctypes.POINTER(NestDesc), _D]),
         "vibe_mi_nest_restrict": (_I, [_VP, _VP, ctypes.POINTER(NestDesc)]),
+        "vibe_mi_batch_nested_step": (_I, [_VP, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(NestDesc),
+                                          _I, _D, _I, _I]),
+        "vibe_mi_batch_nested_step_timed": (_I, [_VP, _I, ctypes.POINTER(_I), ctypes.POINTER(_I),
+                                                ctypes.POINTER(NestDesc), _I, _D, _I, _I, _PD]),
         "vibe_mi_batch_nest_restrict": (_I, [_VP, _I, _I, ctypes.POINTER(NestDesc)]),
         "vibe_mi_helmholtz_create": (_I, [ctypes.POINTER(HelmholtzDesc), ctypes.POINTER(_VP)]),
         "vibe_mi_helmholtz_destroy": (_I, [_VP]),
@@ -741,6 +745,25 @@ def batch_nest_restrict(batch: "Batch", child: int, parent: int, desc: NestDesc)
     """nest_restrict between members of one batch, on the batch stream."""
     _check(lib().vibe_mi_batch_nest_restrict(batch.h, child, parent, ctypes.byref(desc)),
            "vibe_mi_batch_nest_restrict")
+
+
+def batch_nested_step(batch: "Batch", links, two_way: bool, dt0: float, nsub: int, nsteps: int = 1,
+                      timed: bool = False) -> float:
+    """nsteps parent steps of a nest hierarchy in one call (vibe_mi/nest.h:
+    vibe_mi_batch_nested_step); links = [(child, parent, NestDesc)].  Returns
+    the elapsed ms on the batch stream when timed."""
+    n = len(links)
+    ch = (_I * n)(*[c for c, _, _ in links])
+    pa = (_I * n)(*[p for _, p, _ in links])
+    ds = (NestDesc * n)(*[d for _, _, d in links])
+    if timed:
+        ms = _D(0.0)
+        _check(lib().vibe_mi_batch_nested_step_timed(batch.h, n, ch, pa, ds, int(two_way), dt0, nsub, nsteps,
+                                                     ctypes.byref(ms)), "vibe_mi_batch_nested_step_timed")
+        return ms.value
+    _check(lib().vibe_mi_batch_nested_step(batch.h, n, ch, pa, ds, int(two_way), dt0, nsub, nsteps),
+           "vibe_mi_batch_nested_step")
+    return 0.0
 
 
 HELM_BICGSTAB = 0

@@ -14,16 +14,22 @@ two_way, after the parent step's last sub-step (all levels at the same time)
 every nest feeds its mean state back onto the parent cells it covers, finest
 level first (batch_nest_restrict).  These are the reference's nest operators
 (src/grid/nest.cpp:52-128; its driver never calls them, driver.cpp:212-219).
+
+By default the schedule runs in the library (vibe_mi_batch_nested_step): a
+whole parent step is one hipGraph launch and a call synchronises once.
+native=False issues the same schedule from here, sub-step by sub-step.
 """
 from __future__ import annotations
 
-from . import MIXED, Batch, NestDesc, batch_nest_relax, batch_nest_restrict, nest_interpolate
+from . import (MIXED, Batch, NestDesc, batch_nest_relax, batch_nest_restrict, batch_nested_step,
+               nest_interpolate)
 
 
 class NestedRun:
     def __init__(self, cases, links, precision: int = MIXED, use_graph: bool = True, zone: int = 5,
-                 alpha_max: float = 1.0, two_way: bool = False):
+                 alpha_max: float = 1.0, two_way: bool = False, native: bool = True):
         self.two_way = two_way
+        self.native = native
         self.cases = list(cases)
         self.links = [(c, p, NestDesc(r, i0, j0, zone, alpha_max)) for (c, p, r, i0, j0) in links]
         ratios = {r for (_, _, r, _, _) in links}
@@ -65,6 +71,8 @@ class NestedRun:
     def step(self, nsteps: int = 1, timed: bool = False) -> float:
         """nsteps parent steps; returns the summed batched-launch time (ms)
         when timed (the relaxation launches are outside it)."""
+        if self.native:
+            return batch_nested_step(self.batch, self.links, self.two_way, self.dt0, self.nsub, nsteps, timed)
         dts, plan = self._substep_plan()
         ms = 0.0
         for _ in range(nsteps):
