@@ -111,6 +111,10 @@ int vibe_mi_domain_set_physics_tendency(vibe_mi_domain *d, const double *theta_r
  * first call, which switches the coupling on): a GPU physics package writes
  * them on vibe_mi_domain_stream(d) between steps. */
 int vibe_mi_domain_physics_buffers(vibe_mi_domain *d, double **theta_rate, double **u_rate, double **v_rate);
+/* Switches the physics coupling of the following steps off (on = 0) or back
+ * on (StepContext::physics_on, integrator.hpp:42-49), keeping the held rates,
+ * buffers and callback.  On by default. */
+int vibe_mi_domain_physics_enable(vibe_mi_domain *d, int on);
 /* Per-stage physics, the reference's call pattern (physics evaluated on each
  * stage's state: slow_tendency at runge_kutta.cpp:88).  Before the
  * slow tendency of each stage the step zeroes the rate buffers and calls
@@ -137,10 +141,47 @@ int vibe_mi_domain_cfl(vibe_mi_domain *d, double dt, double *cfl, double *max_wi
  * stable}, stable = 1 when the state is finite, the advective CFL < 1.5 and
  * the vertical CFL < 2, else 0. */
 int vibe_mi_domain_stability(vibe_mi_domain *d, double dt, double out[6]);
+/* Largest stable large step of the current state
+ * (RungeKutta3Integrator::max_stable_dt, runge_kutta.cpp:121-141):
+ * min(0.8 / advective CFL at dt = 1, 2 / max over cells of
+ * cs_max (1/dx + 1/dy + 1/dz)), cs_max = the largest sqrt(cp/cv Rd max(T, 1))
+ * of the state (Equations::max_sound_speed), dz = max(dzeta (ztop - zs), 1). */
+int vibe_mi_domain_max_stable_dt(vibe_mi_domain *d, double *dt_out);
 /* hipStream_t of the domain, as void*. */
 void *vibe_mi_domain_stream(vibe_mi_domain *d);
 /* Bytes of device memory the handle holds. */
 size_t vibe_mi_domain_device_bytes(vibe_mi_domain *d);
+
+/* ---- the reference's State layout (host only) -------------------------- */
+/* One reference field (include/vibe/grid/field.hpp:34-96): nx x ny x nz
+ * interior points, `halo` points on every side, one more face along the
+ * staggered axis (u: x, v: y, w: z), i fastest:
+ *   offset(i, j, k) = ((k + halo) nsy + (j + halo)) nsx + (i + halo),
+ *   nsx = nx + 2 halo (+1 for u), nsy = ny + 2 halo (+1 for v),
+ *   nsz = nz + 2 halo (+1 for w).
+ * species[12] are the 12 fields of a dyn::State in the order of dyn::Species
+ * (state.hpp:53-56): u, v, w, rho, theta, pi, qv, qc, qr, qi, qs, qg, as
+ * totals; rho0, theta0, pi0 are the ReferenceState's cell fields in the same
+ * layout.  Elements of one field (0 for bad arguments; species 0..11). */
+size_t vibe_mi_reference_field_size(int nx, int ny, int nz, int halo, int species);
+/* Reference State -> this library's perturbation state (dycore.h layout):
+ * u, v from faces 0..nx-1 / 0..ny-1 (face nx / ny duplicates face 0 in a
+ * periodic domain), w from interfaces 0..nz, rho' = rho - rho0 and likewise
+ * theta', pi'.  Halo points are not read; the moisture species
+ * (species[6..11], may be NULL) are not read either: the step is dry. */
+int vibe_mi_state_from_reference(int nx, int ny, int nz, int halo, const double *const species[12],
+                                 const double *rho0, const double *theta0, const double *pi0, double *u, double *v,
+                                 double *w, double *rho_p, double *theta_p, double *pi_p);
+/* Perturbation state -> reference State totals: u at faces 0..nx (face nx =
+ * face 0), v at 0..ny, w at 0..nz, rho = rho0 + rho' and likewise theta, pi
+ * on the interior; fill_halo = 1 also fills the lateral halo of every
+ * interior level with the periodic values (HaloExchange::exchange_local role,
+ * halo.cpp:58-86; the reference state taken at the periodic interior point),
+ * 0 leaves halos untouched.  Vertical halo levels and the moisture species
+ * are never written. */
+int vibe_mi_state_to_reference(int nx, int ny, int nz, int halo, const double *u, const double *v, const double *w,
+                               const double *rho_p, const double *theta_p, const double *pi_p, const double *rho0,
+                               const double *theta0, const double *pi0, int fill_halo, double *const species[12]);
 
 /* ---- setup helpers (host only; inputs to the step) --------------------- */
 /* zeta = k/nz */

@@ -1,8 +1,14 @@
 // vibe_mi/dycore.hpp -- C++20 host interface over the C ABI (dycore.h).
 //
 // Mirrors the roles of the reference's time-integration interface:
-//   vibe::timeint::StepContext          include/vibe/time/integrator.hpp:42-49
-//   vibe::timeint::Integrator::step     include/vibe/time/integrator.hpp:62-82
+//   vibe::timeint::StepContext          include/vibe/time/integrator.hpp:34-50
+//   vibe::timeint::IntegratorStats      include/vibe/time/integrator.hpp:53-59
+//   vibe::timeint::Integrator           include/vibe/time/integrator.hpp:62-82
+//     (step, name, stats, reset_stats, max_stable_dt, set_boundary_callback)
+//   RungeKutta3Integrator::max_stable_dt  src/time/runge_kutta.cpp:121-141
+//   vibe::dyn::State (12 species, halo'd fields) <-> the dense perturbation
+//     state                             include/vibe/dyn/state.hpp:53-56,
+//                                       include/vibe/grid/field.hpp:34-96
 //   vibe::timeint::RungeKutta3Integrator  include/vibe/time/runge_kutta.hpp:46-81
 //   vibe::dyn::energy_budget            src/dyn/diagnostics.cpp:255-277
 //   vibe::timeint::TimeStepController::inspect  src/time/timestep_control.cpp:23-56
@@ -18,7 +24,10 @@
 // results.
 #pragma once
 
+#include <array>
+#include <chrono>
 #include <cstddef>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -69,6 +78,7 @@ struct StepContext {
   double dt_acoustic = 1.0;
   int acoustic_substeps = 1;
   double time = 0.0;
+  bool physics_on = true;  // false: this step runs without the physics coupling
   static StepContext make(double dt, int substeps, double time) {
     StepContext c;
     c.dt = dt;
@@ -116,12 +126,33 @@ struct EnergyBudget {
   double mass = 0, mass_perturbation = 0, kinetic = 0, internal = 0, potential = 0, total = 0;
 };
 
+/// Same fields as the reference's IntegratorStats (the Helmholtz ones stay 0:
+/// this integrator is split-explicit).
+struct IntegratorStats {
+  long long steps = 0, acoustic_substeps = 0;
+  double wall_time = 0.0;  // seconds of the last step() call
+  int helmholtz_iterations = 0;
+  double helmholtz_residual = 0.0;
+  bool diverged = false;
+};
+
 /// Abstract integrator (Integrator role).
 class Integrator {
  public:
   virtual ~Integrator() = default;
   virtual void step(State &s, const StepContext &ctx) = 0;
   virtual const char *name() const noexcept = 0;
+  virtual const IntegratorStats &stats() const noexcept = 0;
+  virtual void reset_stats() = 0;
+  /// Largest stable large step for state s (CFL and acoustic limits).
+  virtual double max_stable_dt(const State &s) = 0;
+  /// Lateral boundary callback, called on the state after every step()
+  /// (the reference calls it after the sponge, runge_kutta.cpp:114).
+  using BoundaryCallback = std::function<void(State &, const StepContext &)>;
+  virtual void set_boundary_callback(BoundaryCallback cb) { boundary_ = std::move(cb); }
+
+ protected:
+  BoundaryCallback boundary_;
 };
 
 /// WS-RK3 + HEVI acoustic substeps on one GPU (RungeKutta3Integrator role).
@@ -165,17 +196,41 @@ class RungeKutta3Integrator final : public Integrator {
                                    s.pi.data()),
           "vibe_mi_domain_get_state");
   }
-  /// Advance the device state by n steps (no host transfer).
+  /// Advance the device state by n steps (no host transfer); the boundary
+  /// callback, which needs the host state, runs in step() only.
   void advance(const StepContext &ctx, int nsteps = 1) {
-    check(vibe_mi_domain_step(h_, ctx.dt, ctx.acoustic_substeps, nsteps), "vibe_mi_domain_step");
+    if (!ctx.physics_on) check(vibe_mi_domain_physics_enable(h_, 0), "vibe_mi_domain_physics_enable");
+    const int rc = vibe_mi_domain_step(h_, ctx.dt, ctx.acoustic_substeps, nsteps);
+    if (!ctx.physics_on) (void)vibe_mi_domain_physics_enable(h_, 1);
+    if (rc == VIBE_MI_ERR_NONFINITE) stats_.diverged = true;
+    check(rc, "vibe_mi_domain_step");
+    stats_.steps += nsteps;
+    stats_.acoustic_substeps += 3LL * (ctx.acoustic_substeps < 1 ? 1 : ctx.acoustic_substeps) * nsteps;
   }
-  /// Integrator::step semantics: upload s, one step, download into s.
+  /// Integrator::step semantics: upload s, one step, download into s, then
+  /// the boundary callback on s.
   void step(State &s, const StepContext &ctx) override {
+    const auto t0 = std::chrono::steady_clock::now();
     upload(s);
     advance(ctx, 1);
     download(s);
     s.time = ctx.time + ctx.dt;
     s.step += 1;
+    if (boundary_) boundary_(s, ctx);
+    stats_.wall_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  const IntegratorStats &stats() const noexcept override { return stats_; }
+  void reset_stats() override { stats_ = IntegratorStats{}; }
+  /// max_stable_dt of the device state (runge_kutta.cpp:121-141).
+  double max_stable_dt() const {
+    double dt = 0.0;
+    check(vibe_mi_domain_max_stable_dt(h_, &dt), "vibe_mi_domain_max_stable_dt");
+    return dt;
+  }
+  /// max_stable_dt of s: s becomes the device state (as in step()).
+  double max_stable_dt(const State &s) override {
+    upload(s);
+    return max_stable_dt();
   }
   EnergyBudget energy_budget() const {
     double e[6];
@@ -194,6 +249,9 @@ class RungeKutta3Integrator final : public Integrator {
     return {c, w};
   }
   State make_state() const { return State(nx_, ny_, nz_); }
+  int nx() const noexcept { return nx_; }
+  int ny() const noexcept { return ny_; }
+  int nz() const noexcept { return nz_; }
 
   /// Physics rates held for every following stage: theta' (K/s), u and v
   /// (m/s^2) in the state layout; empty vectors mean zero, all empty
@@ -214,7 +272,53 @@ class RungeKutta3Integrator final : public Integrator {
  private:
   vibe_mi_domain *h_ = nullptr;
   int nx_, ny_, nz_;
+  IntegratorStats stats_;
 };
+
+/// The reference's prognostic State as plain arrays: the 12 species of
+/// dyn::Species in order (u, v, w, rho, theta, pi, qv, qc, qr, qi, qs, qg),
+/// each a halo'd field of the reference layout (vibe_mi/dycore.h), and the
+/// ReferenceState's rho0, theta0, pi0 in the cell layout.
+struct ReferenceFields {
+  int nx = 0, ny = 0, nz = 0, halo = 4;
+  std::array<std::vector<double>, 12> species;
+  std::vector<double> rho0, theta0, pi0;
+  ReferenceFields() = default;
+  ReferenceFields(int nx_, int ny_, int nz_, int halo_) : nx(nx_), ny(ny_), nz(nz_), halo(halo_) {
+    for (int q = 0; q < 12; ++q) species[q].assign(vibe_mi_reference_field_size(nx, ny, nz, halo, q), 0.0);
+    const std::size_t nc = vibe_mi_reference_field_size(nx, ny, nz, halo, 3);
+    rho0.assign(nc, 0.0);
+    theta0.assign(nc, 0.0);
+    pi0.assign(nc, 0.0);
+  }
+  std::size_t offset(int i, int j, int k, int sp) const {
+    const std::size_t nsx = static_cast<std::size_t>(nx + 2 * halo + (sp == 0 ? 1 : 0));
+    const std::size_t nsy = static_cast<std::size_t>(ny + 2 * halo + (sp == 1 ? 1 : 0));
+    return (static_cast<std::size_t>(k + halo) * nsy + static_cast<std::size_t>(j + halo)) * nsx +
+           static_cast<std::size_t>(i + halo);
+  }
+};
+
+/// dyn::State totals -> perturbation State (the adapter's step-in).
+inline void from_reference(const ReferenceFields &r, State &s) {
+  if (s.u.empty()) s = State(r.nx, r.ny, r.nz);
+  const double *sp[12];
+  for (int q = 0; q < 12; ++q) sp[q] = r.species[q].data();
+  check(vibe_mi_state_from_reference(r.nx, r.ny, r.nz, r.halo, sp, r.rho0.data(), r.theta0.data(), r.pi0.data(),
+                                     s.u.data(), s.v.data(), s.w.data(), s.rho.data(), s.theta.data(), s.pi.data()),
+        "vibe_mi_state_from_reference");
+}
+
+/// Perturbation State -> dyn::State totals (the adapter's step-out); the
+/// moisture species are left as they are.
+inline void to_reference(const State &s, ReferenceFields &r, bool fill_halo = false) {
+  double *sp[12];
+  for (int q = 0; q < 12; ++q) sp[q] = r.species[q].data();
+  check(vibe_mi_state_to_reference(r.nx, r.ny, r.nz, r.halo, s.u.data(), s.v.data(), s.w.data(), s.rho.data(),
+                                   s.theta.data(), s.pi.data(), r.rho0.data(), r.theta0.data(), r.pi0.data(),
+                                   fill_halo ? 1 : 0, sp),
+        "vibe_mi_state_to_reference");
+}
 
 /// time and step of a restart (the fields of the reference's RestartInfo that
 /// its restart file stores).

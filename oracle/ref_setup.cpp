@@ -32,6 +32,8 @@
 //     interpolate_from_parent(p) and apply_lateral_boundary(c, that),
 //     restrict_to_parent(the relaxed child) onto a copy of p, and
 //     davies_profile(4, 1) and (5, 0.8)
+//   the raw storage of a State's 12 species on a 5 x 4 x 3 grid with halo 2
+//     (field.hpp:34-96), filled through its own (i, j, k) indexing
 #include <cmath>
 #include <cstdio>
 #include <string>
@@ -223,6 +225,31 @@ int main() {
     for (Int j = 0; j < 14; ++j)
       for (Int i = 0; i < 16; ++i) vr.push_back(Pr(i, j, k));
   put("nest_restricted_parent_cell", vr);
+  // the State layout (field.hpp:34-96, state.hpp:53-56): a 5 x 4 x 3 grid
+  // with halo 2, species q filled through the reference's own indexing,
+  // value(q, i, j, k) = 1000 q + 100 k + 10 j + i on every interior point and
+  // staggered face (halo points stay 0); raw arrays printed
+  {
+    grid::Geometry lg;
+    lg.nx = 5;
+    lg.ny = 4;
+    lg.nz = 3;
+    lg.dx = lg.dy = 1000.0;
+    lg.z_top = 3000.0;
+    lg.zeta = {0.0, 1.0 / 3.0, 2.0 / 3.0, 1.0};
+    lg.name = "layout";
+    const grid::Grid glay(lg, grid::Decomposition::make(5, 4, 3, 1, 1, 2, true, true));
+    vibe::dyn::State ls(glay);
+    for (int q = 0; q < vibe::dyn::kNumSpecies; ++q) {
+      const auto sp = static_cast<vibe::dyn::Species>(q);
+      grid::Field<Real> &f = ls.field(sp);
+      const Int ni = 5 + (q == 0 ? 1 : 0), nj = 4 + (q == 1 ? 1 : 0), nk = 3 + (q == 2 ? 1 : 0);
+      for (Int k = 0; k < nk; ++k)
+        for (Int j = 0; j < nj; ++j)
+          for (Int i = 0; i < ni; ++i) f(i, j, k) = 1000.0 * q + 100.0 * k + 10.0 * j + i;
+      put((std::string("layout_h2_") + vibe::dyn::species_name(sp)).c_str(), f.storage());
+    }
+  }
   put("davies_4_1", grid::davies_profile(4, 1.0));
   put("davies_5_0.8", grid::davies_profile(5, 0.8), true);
   std::printf("}\n");

@@ -214,7 +214,7 @@ int upload_desc(vibe_mi_batch *b, const double *dt, int nsub) {
     x.ao = static_cast<T>(0.5 * (1.0 - d->eps_s));
     x.dt = dt[m];
     x.sponge = d->sponge;
-    x.ph = d->ph_on ? PhysArgs{d->ph_th, d->ph_u, d->ph_v} : PhysArgs{};
+    x.ph = (d->ph_on && d->ph_enabled) ? PhysArgs{d->ph_th, d->ph_u, d->ph_v} : PhysArgs{};
     x.kmin_sponge = d->kmin_sponge;
     x.cur0 = d->cur0;
     x.flag = b->flag;
@@ -523,7 +523,7 @@ int step_batch(vibe_mi_batch *b, const double *dt, int nsub, int nsteps, const u
   if (int rc = upload_desc<T>(b, dt, nsub)) return rc;
   b->phys = false;
   for (int m = 0; m < n; ++m)
-    if (act[m] && b->doms[m]->ph_on) b->phys = true;
+    if (act[m] && b->doms[m]->ph_on && b->doms[m]->ph_enabled) b->phys = true;
   if (int rc = autotune<T>(b, *w)) return rc;
   if constexpr (sizeof(T) == 4)
     if (int rc = launch_substep<T, double>(b, w->nw, *w, 0, 0, 0, 1, b->ctr, true)) return rc;
@@ -653,7 +653,7 @@ int nested_steps(vibe_mi_batch *b, const std::vector<NestLink> &links, bool two_
   if (int rc = upload_desc<T>(b, dts.data(), nsub)) return rc;
   b->phys = false;
   for (int m = 0; m < n; ++m)
-    if (b->doms[m]->ph_on) b->phys = true;
+    if (b->doms[m]->ph_on && b->doms[m]->ph_enabled) b->phys = true;
   for (Sub &sb : plan) {
     if (int rc = work_list(b, sb.act, &sb.w)) return rc;
     if (int rc = autotune<T>(b, *sb.w)) return rc;

@@ -65,6 +65,7 @@ struct vibe_mi_domain {
   // use), whether the stages add them, and the optional per-stage callback
   double *ph_th = nullptr, *ph_u = nullptr, *ph_v = nullptr;
   bool ph_on = false;
+  bool ph_enabled = true;  // StepContext::physics_on: coupling of the next steps
   vibe_mi_physics_fn ph_fn = nullptr;
   void *ph_user = nullptr;
   int ph_host = 0;

@@ -198,7 +198,7 @@ int run_stage(vibe_mi_domain *d, const Geo<T> &g, const FieldSet &cur, const Fie
     so.ul = l.u, so.vl = l.v, so.wl = l.w, so.rl = l.rho, so.pl = l.pi;
   }
   const dim3 grid(ceil_div(d->nx, 64), ceil_div(d->ny, 4)), block(64, 4);
-  if (d->ph_on)
+  if (d->ph_on && d->ph_enabled)
     hipLaunchKernelGGL((k_stage_col<T, T, 4, true>), grid, block, 0, d->stream, g, ci, si, h, so, d->flag,
                        PhysArgs{d->ph_th, d->ph_u, d->ph_v});
   else
@@ -453,7 +453,7 @@ int step_impl(vibe_mi_domain *d, const Geo<T> &g, AcousticSet<T> *A, double dt, 
   int cur = i0;
   for (int st = 0; st < 3; ++st) {
     FieldSet &nxt = d->S[outs[st]];
-    if (d->ph_fn != nullptr)
+    if (d->ph_fn != nullptr && d->ph_enabled)
       if (int rc = call_physics(d, st, dt, d->S[cur])) return rc;
     if (timed) VMI_HIP(hipEventRecord(d->ev[0], d->stream));
     if (int rc = run_stage<T>(d, g, d->S[cur], s0, hk[st], A[0], nxt)) return rc;
@@ -612,7 +612,7 @@ int create_impl(const vibe_mi_domain_desc *ds, vibe_mi_domain *d) {
   if (int rc = dalloc(d, &d->flag, 4)) return rc;
   if (int rc = dalloc(d, &d->part, static_cast<size_t>(kRedBlocks) * kEnergyComps)) return rc;
   if (int rc = dalloc(d, &d->dres, 8)) return rc;
-  if (int rc = dalloc(d, &d->dmax, 6)) return rc;
+  if (int rc = dalloc(d, &d->dmax, 8)) return rc;
   VMI_HIP(hipMemset(d->flag, 0, 4 * sizeof(int)));
   for (auto &f : d->S) {
     VMI_HIP(hipMemset(f.u, 0, nc * 8));
@@ -860,6 +860,12 @@ int vibe_mi_domain_physics_buffers(vibe_mi_domain *d, double **theta_rate, doubl
   return 0;
 }
 
+int vibe_mi_domain_physics_enable(vibe_mi_domain *d, int on) {
+  if (d == nullptr) return VIBE_C_ERR_BAD_ARG;
+  d->ph_enabled = on != 0;
+  return 0;
+}
+
 int vibe_mi_domain_set_physics_callback(vibe_mi_domain *d, vibe_mi_physics_fn fn, void *user, int host) {
   if (d == nullptr) return VIBE_C_ERR_BAD_ARG;
   if (fn == nullptr) {
@@ -909,24 +915,26 @@ int vibe_mi_domain_energy(vibe_mi_domain *d, double out[6]) {
   return 0;
 }
 
-// One pass of k_cfl_state into r[6] (see the kernel).
-static int stability_pass(vibe_mi_domain *d, double dt, unsigned long long r[6]) {
+// One pass of k_cfl_state into r[8] (see the kernel): advective CFL, max
+// wind, acoustic CFL, vertical CFL, min dz, non-finite flag, max sound speed,
+// max of 1/dx + 1/dy + 1/dz (bit patterns of the doubles).
+static int stability_pass(vibe_mi_domain *d, double dt, unsigned long long r[8]) {
   if (int rc = set_device(d)) return rc;
   const FieldSet &f = d->S[d->cur0];
-  const unsigned long long init[6] = {0ull, 0ull, 0ull, 0ull, 0x7FF0000000000000ull, 0ull};  // min dz from +inf
+  const unsigned long long init[8] = {0ull, 0ull, 0ull, 0ull, 0x7FF0000000000000ull, 0ull, 0ull, 0ull};  // min dz from +inf
   VMI_HIP(hipMemcpyAsync(d->dmax, init, sizeof(init), hipMemcpyHostToDevice, d->stream));
   const unsigned blocks = std::min<size_t>(ceil_div(d->ncell, 256), 2048);
   hipLaunchKernelGGL(k_cfl_state, dim3(blocks), dim3(256), 0, d->stream, d->nx, d->ny, d->nz, d->dx, d->dy, d->ztop,
                      d->zs, d->dzeta64, f.u, f.v, f.w, f.rho, f.th, f.pi, d->theta0, d->pi0, dt, d->dmax);
   VMI_HIP(hipGetLastError());
-  VMI_HIP(hipMemcpyAsync(r, d->dmax, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, d->stream));
+  VMI_HIP(hipMemcpyAsync(r, d->dmax, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, d->stream));
   VMI_HIP(hipStreamSynchronize(d->stream));
   return 0;
 }
 
 int vibe_mi_domain_cfl(vibe_mi_domain *d, double dt, double *cfl, double *max_wind) {
   if (d == nullptr) return VIBE_C_ERR_BAD_ARG;
-  unsigned long long r[6];
+  unsigned long long r[8];
   if (int rc = stability_pass(d, dt, r)) return rc;
   if (cfl) *cfl = __builtin_bit_cast(double, r[0]);
   if (max_wind) *max_wind = __builtin_bit_cast(double, r[1]);
@@ -935,7 +943,7 @@ int vibe_mi_domain_cfl(vibe_mi_domain *d, double dt, double *cfl, double *max_wi
 
 int vibe_mi_domain_stability(vibe_mi_domain *d, double dt, double out[6]) {
   if (d == nullptr || out == nullptr) return VIBE_C_ERR_BAD_ARG;
-  unsigned long long r[6];
+  unsigned long long r[8];
   if (int rc = stability_pass(d, dt, r)) return rc;
   out[0] = __builtin_bit_cast(double, r[0]);  // advective CFL
   out[1] = __builtin_bit_cast(double, r[2]);  // acoustic CFL
@@ -944,6 +952,22 @@ int vibe_mi_domain_stability(vibe_mi_domain *d, double dt, double out[6]) {
   out[4] = __builtin_bit_cast(double, r[4]);  // min dz
   // stable: finite state, advective CFL < 1.5 and vertical CFL < 2 (timestep_control.cpp:48)
   out[5] = (r[5] == 0 && out[0] < 1.5 && out[2] < 2.0) ? 1.0 : 0.0;
+  return 0;
+}
+
+int vibe_mi_domain_max_stable_dt(vibe_mi_domain *d, double *dt_out) {
+  if (d == nullptr || dt_out == nullptr) return VIBE_C_ERR_BAD_ARG;
+  unsigned long long r[8];
+  if (int rc = stability_pass(d, 1.0, r)) return rc;
+  // RungeKutta3Integrator::max_stable_dt (runge_kutta.cpp:121-141): the
+  // advective limit 0.8 / CFL(dt = 1) and the one-substep acoustic limit
+  // 2 / max(cs_max (1/dx + 1/dy + 1/dz)), cs_max the state's largest sound
+  // speed; the product's maximum is cs_max times the geometric maximum
+  const double adv = __builtin_bit_cast(double, r[0]);
+  const double rate = __builtin_bit_cast(double, r[6]) * __builtin_bit_cast(double, r[7]);
+  const double dt_adv = adv > 0.0 ? 0.8 / adv : 1e9;
+  const double dt_ac = rate > 0.0 ? 2.0 / rate : 1e9;
+  *dt_out = std::min(dt_adv, dt_ac);
   return 0;
 }
 

@@ -1379,7 +1379,7 @@ static __global__ void __launch_bounds__(256) k_cfl_state(int nx, int ny, int nz
                                                    const double *__restrict__ pi0, double dt,
                                                    unsigned long long *__restrict__ res) {
   const size_t nxy = static_cast<size_t>(nx) * ny, n = nxy * nz;
-  double worst = 0.0, wind = 0.0, cac = 0.0, cvert = 0.0, mdz = __builtin_huge_val();
+  double worst = 0.0, wind = 0.0, cac = 0.0, cvert = 0.0, mdz = __builtin_huge_val(), csmax = 0.0, geo = 0.0;
   int bad = 0;
   for (size_t c = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n;
        c += static_cast<size_t>(gridDim.x) * blockDim.x) {
@@ -1397,31 +1397,39 @@ static __global__ void __launch_bounds__(256) k_cfl_state(int nx, int ny, int nz
     const double cs = sqrt((kCp / kCv) * kRd * fmax(t, 1.0));
     cac = fmax(cac, dt * cs * (1.0 / dx + 1.0 / dy + 1.0 / dz));
     cvert = fmax(cvert, dt * (fabs(ww) + cs) / dz);
+    csmax = fmax(csmax, cs);                       // Equations::max_sound_speed
+    geo = fmax(geo, 1.0 / dx + 1.0 / dy + 1.0 / dz);  // max_stable_dt's acoustic rate per unit cs
     bad |= !(isfinite(u[c]) && isfinite(v[c]) && isfinite(w[c]) && isfinite(rho[c]) && isfinite(th[c]) &&
              isfinite(pi[c]));
     if (k == static_cast<size_t>(nz) - 1) bad |= !isfinite(w[c + nxy]);
   }
-  __shared__ double s[5][256];
+  __shared__ double s[7][256];
   __shared__ int sb[256];
   s[0][threadIdx.x] = worst;
   s[1][threadIdx.x] = wind;
   s[2][threadIdx.x] = cac;
   s[3][threadIdx.x] = cvert;
   s[4][threadIdx.x] = mdz;
+  s[5][threadIdx.x] = csmax;
+  s[6][threadIdx.x] = geo;
   sb[threadIdx.x] = bad;
   __syncthreads();
   for (int hh = 128; hh > 0; hh >>= 1) {
     if (threadIdx.x < static_cast<unsigned>(hh)) {
-      for (int q = 0; q < 4; ++q) s[q][threadIdx.x] = fmax(s[q][threadIdx.x], s[q][threadIdx.x + hh]);
+      for (int q = 0; q < 7; ++q)
+        if (q != 4) s[q][threadIdx.x] = fmax(s[q][threadIdx.x], s[q][threadIdx.x + hh]);
       s[4][threadIdx.x] = fmin(s[4][threadIdx.x], s[4][threadIdx.x + hh]);
       sb[threadIdx.x] |= sb[threadIdx.x + hh];
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
+    // non-negative doubles order like their bit patterns
     for (int q = 0; q < 4; ++q) atomicMax(&res[q], static_cast<unsigned long long>(__double_as_longlong(s[q][0])));
     atomicMin(&res[4], static_cast<unsigned long long>(__double_as_longlong(s[4][0])));
     if (sb[0]) atomicMax(&res[5], 1ull);
+    atomicMax(&res[6], static_cast<unsigned long long>(__double_as_longlong(s[5][0])));
+    atomicMax(&res[7], static_cast<unsigned long long>(__double_as_longlong(s[6][0])));
   }
 }
 

@@ -177,6 +177,11 @@ def lib() -> ctypes.CDLL:
         "vibe_mi_terrain_smooth": (_I, [_PD, _I, _I, _I, _D]),
         "vibe_mi_terrain_gaussian": (_I, [_I, _I, _D, _D, _D, _D, ctypes.c_uint64, _PD]),
         "vibe_mi_reference_state": (_I, [_I, _D, _D, _D, _I, _I, _I, _D, _PD, _PD, _PD, _PD, _PD]),
+        "vibe_mi_reference_field_size": (_SZ, [_I, _I, _I, _I, _I]),
+        "vibe_mi_state_from_reference": (_I, [_I, _I, _I, _I, ctypes.POINTER(_PD), _PD, _PD, _PD] + [_PD] * 6),
+        "vibe_mi_state_to_reference": (_I, [_I, _I, _I, _I] + [_PD] * 9 + [_I, ctypes.POINTER(_PD)]),
+        "vibe_mi_domain_physics_enable": (_I, [_VP, _I]),
+        "vibe_mi_domain_max_stable_dt": (_I, [_VP, _PD]),
         "vibe_mi_reference_state_from_profile": (_I, [_I, _I, _I, _D, _PD, _PD, _I, _PD, _PD, _D, _PD, _PD, _PD]),
         "vibe_mi_sponge": (_I, [_I, _D, _PD, _D, _D, _D, _PD]),
         "vibe_mi_ic_bubble": (_I, [_I, _I, _I, _D, _D, _D, _PD, _PD, _D, _D, _D, _D, _D, _D, _PD]),
@@ -604,6 +609,19 @@ class Domain:
         _check(lib().vibe_mi_domain_stability(self.h, dt, _pd(out)), "vibe_mi_domain_stability")
         return {"cfl_advection": out[0], "cfl_acoustic": out[1], "cfl_vertical": out[2], "max_wind": out[3],
                 "min_dz": out[4], "stable": bool(out[5])}
+
+    def max_stable_dt(self) -> float:
+        """RungeKutta3Integrator::max_stable_dt (runge_kutta.cpp:121-141) of
+        the device state: min(0.8 / CFL(dt = 1), 2 / (cs_max max(1/dx + 1/dy
+        + 1/dz)))."""
+        out = ctypes.c_double()
+        _check(lib().vibe_mi_domain_max_stable_dt(self.h, ctypes.byref(out)), "vibe_mi_domain_max_stable_dt")
+        return out.value
+
+    def physics_enable(self, on: bool) -> None:
+        """StepContext::physics_on: with False the registered physics (rates
+        or callback) is skipped until enabled again."""
+        _check(lib().vibe_mi_domain_physics_enable(self.h, 1 if on else 0), "vibe_mi_domain_physics_enable")
 
     def cfl(self, dt: float):
         c, w = ctypes.c_double(), ctypes.c_double()
