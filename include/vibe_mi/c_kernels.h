@@ -3,11 +3,26 @@
  * Drop-in for the reference's include/vibe/common/c_kernels.h:24-119.  The six
  * vibe_*_f64 entry points keep the reference's names, byte-identical
  * signatures, argument meaning and return codes; every one of them runs its
- * arithmetic in a HIP kernel on the current GPU (host buffers in, host buffers
- * out, staged through a per-thread device arena owned by the library).  The
- * _f32 twins follow the header convention of c_kernels.h:15-16 (none exist in
- * the reference).  The vibe_mi_*_batched entry points take DEVICE pointers and
- * a hipStream_t (passed as void*) and never allocate.
+ * arithmetic in a HIP kernel on the current GPU.  The reference's contract
+ * (c_kernels.h:13-19) holds: no entry point allocates (no hipMalloc, hipFree,
+ * stream creation or host heap) and none keeps state that changes its
+ * results; all are re-entrant and may be called from concurrent threads.
+ *   - Arrays the GPU can dereference (device memory of the current GPU,
+ *     hipHostMalloc'd host memory) are read and written in place.
+ *   - Pageable host arrays are copied in chunks through a statically sized
+ *     device array of the library (256 MiB per GPU, loaded with the library
+ *     and held until the process ends), taken under a per-GPU lock for the
+ *     length of a call: concurrent calls on one GPU serialise.  Limits of
+ *     the staged path: tridiagonal n <= 5.5e6 (f64); flux advection
+ *     nx*ny <= 1.9e6; other sizes are unlimited.  Beyond them the call
+ *     returns (or, for void functions, reports through vibe_mi_last_error)
+ *     VIBE_MI_ERR_TOO_LARGE.
+ *   - Work runs on hipStreamPerThread; every call returns with its results
+ *     in place.
+ * The _f32 twins follow the header convention of c_kernels.h:15-16 (none
+ * exist in the reference).  The vibe_mi_*_batched entry points take DEVICE
+ * pointers, the caller's workspace and a hipStream_t (passed as void*; NULL
+ * is the null stream), and never allocate.
  *
  * Errors: VIBE_C_OK / VIBE_C_ERR_SINGULAR / VIBE_C_ERR_BAD_ARG as in the
  * reference (c_kernels.h:34-36); failures of the HIP runtime itself are
@@ -83,8 +98,9 @@ double vibe_neumaier_sum_f32(const float *x, size_t n);
 /* nsys independent systems of size n, interleaved k-major: element k of
  * system s lives at [k*nsys + s] (a, c: k = 0..n-2; b, d, x: k = 0..n-1).
  * ws (device, n*nsys) is the caller's workspace, as in the reference's
- * contract.  One system per GPU lane.  Returns VIBE_C_ERR_SINGULAR if any
- * pivot of any system is exactly zero (the other systems are still solved). */
+ * contract: rows 0..n-2 hold the forward sweep's c', row n-1 the singular
+ * flag.  One system per GPU lane.  Returns VIBE_C_ERR_SINGULAR if any pivot
+ * of any system is exactly zero (the other systems are still solved). */
 int vibe_mi_tridiagonal_solve_batched_f64(const double *a, const double *b,
                                           const double *c, const double *d,
                                           int n, int nsys, double *x, double *ws,
@@ -118,7 +134,10 @@ int vibe_mi_weno5_reconstruct_batched_f64(const double *q, size_t n,
                                           double *q_left, double *q_right,
                                           void *stream);
 
-/* Device-pointer Neumaier sum; result written to *out (host). */
+/* Device-pointer Neumaier sum; result written to *out (host).  Uses the
+ * staging array's head for partial sums (under the per-GPU lock); the same
+ * fixed-order reduction as vibe_neumaier_sum_f64, so both give the same bits
+ * for the same values.  n <= 2.5e10. */
 int vibe_mi_neumaier_sum_device_f64(const double *x, size_t n, double *out,
                                     void *stream);
 

@@ -29,18 +29,6 @@ int last_error();
     }                                                   \
   } while (0)
 
-// Device buffers for the host-pointer entry points: one growable arena per
-// host thread, plus one stream per host thread.  Grows by hipMalloc only when
-// a call needs more than it holds; never shrinks.
-struct Arena {
-  void *ptr = nullptr;
-  size_t bytes = 0;
-  hipStream_t stream = nullptr;
-  int device = -1;
-};
-// Returns 0 and a pointer to >= bytes of device memory, or an error code.
-int arena_get(size_t bytes, void **out, hipStream_t *stream);
-
 inline unsigned ceil_div(size_t a, size_t b) { return static_cast<unsigned>((a + b - 1) / b); }
 
 // Error-free transformation s + e = a + b (Knuth TwoSum).
