@@ -1,0 +1,370 @@
+/* oracle/literal.c -- the reference's LITERAL discrete equations, restated
+ * in plain C: Equations::tendencies, acoustic_tendencies, max_sound_speed and
+ * diagnose (src/dyn/equations.cpp:60-237), with the advection
+ * (src/dyn/advection.cpp:102-319), Coriolis (src/dyn/coriolis.cpp:30-74) and
+ * divergence-damping (src/dyn/damping.cpp:43-74) blocks they call.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the GPU library.  Nothing in the
+ * product (vibe-model_amd/) includes, links or calls this code.
+ *
+ * "Literal" keeps every convention of the reference: the prognostics are
+ * FULL values (rho, theta, pi), x and y are bounded (indices clamped to the
+ * interior, no periodic wrap), halos read as zero (the reference's
+ * single-rank path never fills them), and the operation order of each
+ * expression is the reference's, so with -ffp-contract=off the results are
+ * the reference's bits.  Dense layout, i fastest (orc_literal_desc in
+ * oracle.h).  Pinned against the reference's own compiled equations.cpp
+ * (tests/golden/equations_reference.npz).
+ */
+#include <math.h>
+#include <stddef.h>
+#include <string.h>
+
+#include "oracle.h"
+
+static const double kG = 9.80665, kRd = 287.05, kCp = 1004.64, kCv = 717.63;
+static const double kEps = 0.62197, kP00 = 100000.0;
+static const double kOmega = 7.2921159e-5, kEarth = 6.371229e6;
+static const double kPi = 3.14159265358979323846;
+static const double kMoistB = 0.608;
+
+typedef struct {
+  const orc_literal_desc *d;
+  int nx, ny, nz;
+} L;
+
+static int imin(int a, int b) { return a < b ? a : b; }
+static int imax(int a, int b) { return a > b ? a : b; }
+static int iclamp(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* field accessors: cell [nz][ny][nx], u [nz][ny][nx+1], v [nz][ny+1][nx],
+ * w [nz+1][ny][nx] */
+static size_t oc(const L *g, int i, int j, int k) { return ((size_t)k * g->ny + j) * g->nx + i; }
+static size_t ou(const L *g, int i, int j, int k) { return ((size_t)k * g->ny + j) * (g->nx + 1) + i; }
+static size_t ov(const L *g, int i, int j, int k) { return ((size_t)k * (g->ny + 1) + j) * g->nx + i; }
+static size_t ow(const L *g, int i, int j, int k) { return ((size_t)k * g->ny + j) * g->nx + i; }
+
+/* Field::clamp_at of a cell field (field.hpp:91-96) */
+static double cl(const L *g, const double *f, int i, int j, int k) {
+  return f[oc(g, iclamp(i, 0, g->nx - 1), iclamp(j, 0, g->ny - 1), iclamp(k, 0, g->nz - 1))];
+}
+/* a plain read of a cell field that may fall on the (zero) halo */
+static double ch(const L *g, const double *f, int i, int j, int k) {
+  if (i < 0 || i >= g->nx || j < 0 || j >= g->ny || k < 0 || k >= g->nz) return 0.0;
+  return f[oc(g, i, j, k)];
+}
+
+/* Grid::dzeta and Grid::terrain (geometry.cpp:242-256): clamped indices */
+static double dzeta(const L *g, int k) {
+  const int kk = iclamp(k, 0, g->nz - 1);
+  return g->d->zeta[kk + 1] - g->d->zeta[kk];
+}
+static double terrain(const L *g, int i, int j) {
+  if (g->d->zs == NULL) return 0.0;
+  return g->d->zs[(size_t)iclamp(j, 0, g->ny - 1) * g->nx + iclamp(i, 0, g->nx - 1)];
+}
+
+/* advection.cpp:102-137 */
+static double grad_x(const L *g, const double *q, int i, int j, int k, double inv_dx, int order) {
+  if (order == 4)
+    return (cl(g, q, i - 2, j, k) - 8.0 * cl(g, q, i - 1, j, k) + 8.0 * cl(g, q, i + 1, j, k) -
+            cl(g, q, i + 2, j, k)) *
+           (inv_dx / 12.0);
+  if (order == 6)
+    return (-cl(g, q, i - 3, j, k) + 9.0 * cl(g, q, i - 2, j, k) - 45.0 * cl(g, q, i - 1, j, k) +
+            45.0 * cl(g, q, i + 1, j, k) - 9.0 * cl(g, q, i + 2, j, k) + cl(g, q, i + 3, j, k)) *
+           (inv_dx / 60.0);
+  return (cl(g, q, i + 1, j, k) - cl(g, q, i - 1, j, k)) * (0.5 * inv_dx);
+}
+static double grad_y(const L *g, const double *q, int i, int j, int k, double inv_dy, int order) {
+  if (order == 4)
+    return (cl(g, q, i, j - 2, k) - 8.0 * cl(g, q, i, j - 1, k) + 8.0 * cl(g, q, i, j + 1, k) -
+            cl(g, q, i, j + 2, k)) *
+           (inv_dy / 12.0);
+  if (order == 6)
+    return (-cl(g, q, i, j - 3, k) + 9.0 * cl(g, q, i, j - 2, k) - 45.0 * cl(g, q, i, j - 1, k) +
+            45.0 * cl(g, q, i, j + 1, k) - 9.0 * cl(g, q, i, j + 2, k) + cl(g, q, i, j + 3, k)) *
+           (inv_dy / 60.0);
+  return (cl(g, q, i, j + 1, k) - cl(g, q, i, j - 1, k)) * (0.5 * inv_dy);
+}
+
+/* Advection::scalar_nonconservative (advection.cpp:224-244): assigns dq */
+static void scalar_nc(const L *g, const double *q, const double *u, const double *v, const double *w,
+                      double *dq) {
+  const orc_literal_desc *d = g->d;
+  for (int k = 0; k < g->nz; ++k)
+    for (int j = 0; j < g->ny; ++j)
+      for (int i = 0; i < g->nx; ++i) {
+        const double scale = d->z_top - terrain(g, i, j);
+        const double dz = dzeta(g, k) * scale;
+        const double ua = 0.5 * (u[ou(g, i, j, k)] + u[ou(g, i + 1, j, k)]);
+        const double va = 0.5 * (v[ov(g, i, j, k)] + v[ov(g, i, j + 1, k)]);
+        const double wa = 0.5 * (w[ow(g, i, j, k)] + w[ow(g, i, j, k + 1)]);
+        dq[oc(g, i, j, k)] = -(ua * grad_x(g, q, i, j, k, 1.0 / d->dx, d->order) +
+                               va * grad_y(g, q, i, j, k, 1.0 / d->dy, d->order) +
+                               wa * (cl(g, q, i, j, k + 1) - cl(g, q, i, j, k - 1)) * (0.5 / dz));
+      }
+}
+
+/* Coriolis::f_at (coriolis.cpp:30-38), single rank: js = 0 */
+static double f_at(const L *g, int j) {
+  const orc_literal_desc *d = g->d;
+  const double f0 = 2.0 * kOmega * sin(d->latitude * (kPi / 180.0));
+  if (d->coriolis != 2) return f0;
+  const double beta = 2.0 * kOmega * cos(d->latitude * (kPi / 180.0)) / kEarth;
+  const double y = (0.0 + (double)j + 0.5) * d->dy;
+  const double y0 = 0.5 * (double)g->ny * d->dy;
+  return f0 + beta * (y - y0);
+}
+
+int orc_literal_tendencies(const orc_literal_desc *d, const double *const s[12], double *const t[12]) {
+  if (d == NULL || s == NULL || t == NULL || d->nx <= 0 || d->ny <= 0 || d->nz <= 0) return 2;
+  L G = {d, d->nx, d->ny, d->nz};
+  const L *g = &G;
+  const int nx = g->nx, ny = g->ny, nz = g->nz;
+  const size_t ncell = (size_t)nx * ny * nz;
+  const size_t nsz[12] = {(size_t)(nx + 1) * ny * nz, (size_t)nx * (ny + 1) * nz, (size_t)nx * ny * (nz + 1),
+                          ncell, ncell, ncell, ncell, ncell, ncell, ncell, ncell, ncell};
+  for (int q = 0; q < 12; ++q) memset(t[q], 0, nsz[q] * sizeof(double)); /* d.zero() */
+  const double *u = s[0], *v = s[1], *w = s[2], *rho = s[3], *th = s[4], *pi = s[5];
+  double *du = t[0], *dv = t[1], *dw = t[2];
+
+  /* Advection::momentum (advection.cpp:250-319) */
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i <= nx; ++i) {
+        const int ip = imin(i, nx - 1), im = imax(i - 1, 0);
+        const double dx = d->dx, dy = d->dy;
+        const double scale = d->z_top - terrain(g, ip, j);
+        const double dz = dzeta(g, imin(k, nz - 1)) * scale;
+        const double uc = 0.5 * (u[ou(g, im, j, k)] + u[ou(g, ip, j, k)]);
+        const double ux = (u[ou(g, ip, j, k)] - u[ou(g, im, j, k)]) / dx;
+        const double uy = (u[ou(g, ip, imin(j + 1, ny - 1), k)] - u[ou(g, ip, imax(j - 1, 0), k)]) / (2.0 * dy);
+        const double uz = (u[ou(g, ip, j, imin(k + 1, nz - 1))] - u[ou(g, ip, j, imax(k - 1, 0))]) / (2.0 * dz);
+        const double vc = 0.25 * (v[ov(g, im, imax(j - 1, 0), k)] + v[ov(g, ip, imax(j - 1, 0), k)] +
+                                  v[ov(g, im, j, k)] + v[ov(g, ip, j, k)]);
+        const double wc = 0.5 * (w[ow(g, ip, j, k)] + w[ow(g, ip, j, imin(k + 1, nz - 1))]);
+        du[ou(g, i, j, k)] -= uc * ux + vc * uy + wc * uz;
+      }
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j <= ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const int jp = imin(j, ny - 1), jm = imax(j - 1, 0);
+        const double dx = d->dx, dy = d->dy;
+        const double scale = d->z_top - terrain(g, i, jp);
+        const double dz = dzeta(g, imin(k, nz - 1)) * scale;
+        const double vc = 0.5 * (v[ov(g, i, jm, k)] + v[ov(g, i, jp, k)]);
+        const double vy = (v[ov(g, i, jp, k)] - v[ov(g, i, jm, k)]) / dy;
+        const double vx = (v[ov(g, imin(i + 1, nx - 1), jp, k)] - v[ov(g, imax(i - 1, 0), jp, k)]) / (2.0 * dx);
+        const double vz = (v[ov(g, i, jp, imin(k + 1, nz - 1))] - v[ov(g, i, jp, imax(k - 1, 0))]) / (2.0 * dz);
+        const double uc = 0.25 * (u[ou(g, imax(i - 1, 0), jm, k)] + u[ou(g, i, jm, k)] +
+                                  u[ou(g, imax(i - 1, 0), jp, k)] + u[ou(g, i, jp, k)]);
+        const double wc = 0.5 * (w[ow(g, i, jp, k)] + w[ow(g, i, jp, imin(k + 1, nz - 1))]);
+        dv[ov(g, i, j, k)] -= uc * vx + vc * vy + wc * vz;
+      }
+  for (int k = 0; k <= nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const int kc = imin(imax(k, 0), nz - 1), kt = imin(k, nz), kb = imax(k - 1, 0);
+        const double dx = d->dx, dy = d->dy;
+        const double scale = d->z_top - terrain(g, i, j);
+        const double dz = dzeta(g, kc) * scale;
+        const double wc = 0.5 * (w[ow(g, i, j, kc)] + w[ow(g, i, j, kt)]);
+        const double wz = (w[ow(g, i, j, kt)] - w[ow(g, i, j, kb)]) / dz;
+        const double wx = (w[ow(g, imin(i + 1, nx - 1), j, kt)] - w[ow(g, imax(i - 1, 0), j, kt)]) / (2.0 * dx);
+        const double wy = (w[ow(g, i, imin(j + 1, ny - 1), kt)] - w[ow(g, i, imax(j - 1, 0), kt)]) / (2.0 * dy);
+        const double uc = 0.5 * (u[ou(g, i, j, kc)] + u[ou(g, i + 1, j, kc)]);
+        const double vc = 0.5 * (v[ov(g, i, j, kc)] + v[ov(g, i, j + 1, kc)]);
+        dw[ow(g, i, j, k)] -= uc * wx + vc * wy + wc * wz;
+      }
+
+  /* Coriolis::apply (coriolis.cpp:40-74) */
+  if (d->coriolis != 0) {
+    for (int k = 0; k < nz; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i <= nx; ++i) {
+          const double vu = 0.25 * (v[ov(g, imax(i - 1, 0), imax(j - 1, 0), k)] +
+                                    v[ov(g, imin(i, nx - 1), imax(j - 1, 0), k)] + v[ov(g, imax(i - 1, 0), j, k)] +
+                                    v[ov(g, imin(i, nx - 1), j, k)]);
+          du[ou(g, i, j, k)] += f_at(g, j) * vu;
+        }
+    for (int k = 0; k < nz; ++k)
+      for (int j = 0; j <= ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+          const double uv = 0.25 * (u[ou(g, imax(i - 1, 0), imax(j - 1, 0), k)] + u[ou(g, i, imax(j - 1, 0), k)] +
+                                    u[ou(g, imax(i - 1, 0), imin(j, ny - 1), k)] + u[ou(g, i, imin(j, ny - 1), k)]);
+          dv[ov(g, i, j, k)] -= f_at(g, j) * uv;
+        }
+  }
+
+  /* Damping::tendencies (damping.cpp:43-74): on the full pi */
+  if (d->div_coeff > 0.0) {
+    double *dpi = t[5];
+    const double dx = d->dx, dy = d->dy;
+#define LAP2(f, i, j, k)                                                                            \
+  ((cl(g, f, (i) + 1, j, k) - 2.0 * (f)[oc(g, i, j, k)] + cl(g, f, (i) - 1, j, k)) / (dx * dx) + \
+   (cl(g, f, i, (j) + 1, k) - 2.0 * (f)[oc(g, i, j, k)] + cl(g, f, i, (j) - 1, k)) / (dy * dy))
+    if (d->div_order <= 2) {
+      for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+          for (int i = 0; i < nx; ++i) dpi[oc(g, i, j, k)] -= d->div_coeff * LAP2(pi, i, j, k);
+    } else {
+      /* Del^2 of Del^2: the inner Del^2 is held in the theta tendency,
+       * which is assigned (overwritten) further down */
+      double *tmp = t[4];
+      for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+          for (int i = 0; i < nx; ++i) tmp[oc(g, i, j, k)] = LAP2(pi, i, j, k);
+      for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+          for (int i = 0; i < nx; ++i) dpi[oc(g, i, j, k)] -= d->div_coeff * LAP2(tmp, i, j, k);
+    }
+#undef LAP2
+  }
+
+  /* horizontal pressure gradient (equations.cpp:69-88) */
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i <= nx; ++i) {
+        const int ip = imin(i, nx - 1), im = imax(i - 1, 0);
+        const double thf = 0.5 * (th[oc(g, im, j, k)] + th[oc(g, ip, j, k)]);
+        du[ou(g, i, j, k)] -= kCp * thf * (pi[oc(g, ip, j, k)] - pi[oc(g, im, j, k)]) / d->dx;
+      }
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j <= ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const int jp = imin(j, ny - 1), jm = imax(j - 1, 0);
+        const double thf = 0.5 * (th[oc(g, i, jm, k)] + th[oc(g, i, jp, k)]);
+        dv[ov(g, i, j, k)] -= kCp * thf * (pi[oc(g, i, jp, k)] - pi[oc(g, i, jm, k)]) / d->dy;
+      }
+
+  /* vertical pressure gradient and buoyancy (equations.cpp:90-110) */
+  for (int k = 0; k <= nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const int kc = imin(imax(k, 0), nz - 1), km = imax(k - 1, 0);
+        const double scale = d->z_top - terrain(g, i, j);
+        const double dzv = dzeta(g, kc) * scale;
+        const double dz = dzv > 1.0 ? dzv : 1.0;
+        const double thf = 0.5 * (th[oc(g, i, j, km)] + th[oc(g, i, j, kc)]);
+        dw[ow(g, i, j, k)] -= kCp * thf * (pi[oc(g, i, j, kc)] - pi[oc(g, i, j, km)]) / dz;
+        const double th0 = d->theta0[oc(g, i, j, kc)];
+        const double thp = th[oc(g, i, j, kc)] - th0;
+        const double qcond = s[7][oc(g, i, j, kc)] + s[8][oc(g, i, j, kc)] + s[9][oc(g, i, j, kc)] +
+                             s[10][oc(g, i, j, kc)] + s[11][oc(g, i, j, kc)];
+        dw[ow(g, i, j, k)] += kG * (thp / (th0 > 1.0 ? th0 : 1.0) + kMoistB * s[6][oc(g, i, j, kc)] - qcond);
+      }
+
+  /* mass continuity, full density (equations.cpp:112-127): rho(nx, j) and
+   * rho(i, ny) are halo reads (zero) */
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const double scale = d->z_top - terrain(g, i, j);
+        const double dzv = dzeta(g, k) * scale;
+        const double dz = dzv > 1.0 ? dzv : 1.0;
+        const double fx = u[ou(g, i + 1, j, k)] * 0.5 * (rho[oc(g, i, j, k)] + ch(g, rho, i + 1, j, k)) -
+                          u[ou(g, i, j, k)] * 0.5 * (rho[oc(g, imax(i - 1, 0), j, k)] + rho[oc(g, i, j, k)]);
+        const double fy = v[ov(g, i, j + 1, k)] * 0.5 * (rho[oc(g, i, j, k)] + ch(g, rho, i, j + 1, k)) -
+                          v[ov(g, i, j, k)] * 0.5 * (rho[oc(g, i, imax(j - 1, 0), k)] + rho[oc(g, i, j, k)]);
+        const double fz = w[ow(g, i, j, k + 1)] * 0.5 * (rho[oc(g, i, j, k)] + rho[oc(g, i, j, imin(k + 1, nz - 1))]) -
+                          w[ow(g, i, j, k)] * 0.5 * (rho[oc(g, i, j, imax(k - 1, 0))] + rho[oc(g, i, j, k)]);
+        t[3][oc(g, i, j, k)] = -(fx / d->dx + fy / d->dy + fz / dz);
+      }
+
+  /* theta and water advection (equations.cpp:129-137) */
+  scalar_nc(g, th, u, v, w, t[4]);
+  scalar_nc(g, s[6], u, v, w, t[6]);
+  if (d->moist)
+    for (int q = 7; q < 12; ++q) scalar_nc(g, s[q], u, v, w, t[q]);
+
+  /* positivity of the water species (equations.cpp:139-149) */
+  for (int q = 6; q < 12; ++q)
+    for (size_t c = 0; c < ncell; ++c)
+      if (s[q][c] <= 0.0 && t[q][c] < 0.0) t[q][c] = 0.0;
+  return 0;
+}
+
+int orc_literal_acoustic(const orc_literal_desc *d, const double *const s[12], double *const t[12]) {
+  if (d == NULL || s == NULL || t == NULL || d->nx <= 0 || d->ny <= 0 || d->nz <= 0) return 2;
+  L G = {d, d->nx, d->ny, d->nz};
+  const L *g = &G;
+  const int nx = g->nx, ny = g->ny, nz = g->nz;
+  const size_t ncell = (size_t)nx * ny * nz;
+  const size_t nsz[12] = {(size_t)(nx + 1) * ny * nz, (size_t)nx * (ny + 1) * nz, (size_t)nx * ny * (nz + 1),
+                          ncell, ncell, ncell, ncell, ncell, ncell, ncell, ncell, ncell};
+  for (int q = 0; q < 12; ++q) memset(t[q], 0, nsz[q] * sizeof(double));
+  const double *u = s[0], *v = s[1], *w = s[2], *pi = s[5];
+  const double *r0 = d->rho0, *t0 = d->theta0, *p0 = d->pi0;
+  /* equations.cpp:156-182 */
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i <= nx; ++i) {
+        const int ip = imin(i, nx - 1), im = imax(i - 1, 0);
+        const double th0 = 0.5 * (t0[oc(g, im, j, k)] + t0[oc(g, ip, j, k)]);
+        t[0][ou(g, i, j, k)] = -kCp * th0 * (pi[oc(g, ip, j, k)] - pi[oc(g, im, j, k)]) / d->dx;
+      }
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j <= ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const int jp = imin(j, ny - 1), jm = imax(j - 1, 0);
+        const double th0 = 0.5 * (t0[oc(g, i, jm, k)] + t0[oc(g, i, jp, k)]);
+        t[1][ov(g, i, j, k)] = -kCp * th0 * (pi[oc(g, i, jp, k)] - pi[oc(g, i, jm, k)]) / d->dy;
+      }
+  for (int k = 0; k <= nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const int kc = imin(imax(k, 0), nz - 1), km = imax(k - 1, 0);
+        const double scale = d->z_top - terrain(g, i, j);
+        const double dzv = dzeta(g, kc) * scale;
+        const double dz = dzv > 1.0 ? dzv : 1.0;
+        const double th0 = 0.5 * (t0[oc(g, i, j, km)] + t0[oc(g, i, j, kc)]);
+        t[2][ow(g, i, j, k)] = -kCp * th0 * (pi[oc(g, i, j, kc)] - pi[oc(g, i, j, km)]) / dz;
+      }
+  /* equations.cpp:184-198 */
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const double scale = d->z_top - terrain(g, i, j);
+        const double dzv = dzeta(g, k) * scale;
+        const double dz = dzv > 1.0 ? dzv : 1.0;
+        const double fx = u[ou(g, i + 1, j, k)] * 0.5 * (r0[oc(g, i, j, k)] + r0[oc(g, imin(i + 1, nx - 1), j, k)]) -
+                          u[ou(g, i, j, k)] * 0.5 * (r0[oc(g, imax(i - 1, 0), j, k)] + r0[oc(g, i, j, k)]);
+        const double fy = v[ov(g, i, j + 1, k)] * 0.5 * (r0[oc(g, i, j, k)] + r0[oc(g, i, imin(j + 1, ny - 1), k)]) -
+                          v[ov(g, i, j, k)] * 0.5 * (r0[oc(g, i, imax(j - 1, 0), k)] + r0[oc(g, i, j, k)]);
+        const double fz = w[ow(g, i, j, k + 1)] * 0.5 * (r0[oc(g, i, j, k)] + r0[oc(g, i, j, imin(k + 1, nz - 1))]) -
+                          w[ow(g, i, j, k)] * 0.5 * (r0[oc(g, i, j, imax(k - 1, 0))] + r0[oc(g, i, j, k)]);
+        t[3][oc(g, i, j, k)] = -(fx / d->dx + fy / d->dy + fz / dz);
+      }
+  /* equations.cpp:200-208 */
+  for (size_t c = 0; c < ncell; ++c) {
+    const double rr = r0[c] > 1e-8 ? r0[c] : 1e-8;
+    t[5][c] = (kRd / kCv) * (p0[c] / rr) * t[3][c];
+  }
+  return 0;
+}
+
+/* Equations::max_sound_speed (equations.cpp:238-249) */
+double orc_literal_max_sound_speed(const orc_literal_desc *d, const double *const s[12]) {
+  const size_t ncell = (size_t)d->nx * d->ny * d->nz;
+  double best = 0.0;
+  for (size_t c = 0; c < ncell; ++c) {
+    const double gamma = kCp / kCv;
+    const double tt = s[4][c] * s[5][c];
+    const double cs = sqrt(gamma * kRd * (tt > 1.0 ? tt : 1.0));
+    if (cs > best) best = cs;
+  }
+  return best;
+}
+
+/* Equations::diagnose (equations.cpp:211-223) into pi_out (a copy of pi) */
+void orc_literal_diagnose(const orc_literal_desc *d, const double *const s[12], double *pi_out) {
+  const size_t ncell = (size_t)d->nx * d->ny * d->nz;
+  for (size_t c = 0; c < ncell; ++c) {
+    const double qv = s[6][c] > 0.0 ? s[6][c] : 0.0;
+    const double mf = (1.0 + qv / kEps) / (1.0 + qv);
+    const double num = s[3][c] * kRd * s[4][c] * mf;
+    const double val = num / kP00;
+    pi_out[c] = (val > 0.0) ? pow(val, kRd / kCv) : s[5][c];
+  }
+}

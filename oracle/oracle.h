@@ -38,6 +38,30 @@ void orc_advect_scalar_flux(const double *q, const double *rho, const double *u,
                             const double *zs, int nx, int ny, int nz, double dx,
                             double dy, double z_top, double *dqdt);
 
+/* ---- the reference's literal equations (literal.c) --------------------
+ * State and tendency arrays s[12], t[12] in the species order of the
+ * reference's State (u, v, w, rho, theta, pi, qv, qc, qr, qi, qs, qg), FULL
+ * values, dense, i fastest: u [nz][ny][nx+1] (faces 0..nx), v [nz][ny+1][nx]
+ * (faces 0..ny), w [nz+1][ny][nx] (faces 0..nz), the rest [nz][ny][nx]. */
+typedef struct {
+  int nx, ny, nz;
+  double dx, dy, z_top;
+  const double *zeta;   /* nz+1 interfaces */
+  const double *zs;     /* ny*nx terrain, NULL: flat */
+  const double *rho0, *theta0, *pi0; /* reference state, cells */
+  int order;            /* advection: 2, 4 or 6 (central2/4/6) */
+  int coriolis;         /* 0 none, 1 f-plane, 2 beta plane */
+  double latitude;      /* degrees */
+  double div_coeff;     /* divergence damping coefficient (0: off) */
+  int div_order;        /* 2: Del^2, 4: Del^4 */
+  int moist;            /* advect qc..qg (microphysics != "none") */
+} orc_literal_desc;
+
+int orc_literal_tendencies(const orc_literal_desc *d, const double *const s[12], double *const t[12]);
+int orc_literal_acoustic(const orc_literal_desc *d, const double *const s[12], double *const t[12]);
+double orc_literal_max_sound_speed(const orc_literal_desc *d, const double *const s[12]);
+void orc_literal_diagnose(const orc_literal_desc *d, const double *const s[12], double *pi_out);
+
 /* ---- dynamical core (dycore.c) ---------------------------------------- */
 typedef struct {
   int nx, ny, nz;

@@ -48,3 +48,22 @@ $(SETUP): ref_setup.cpp $(REF)/src/grid/geometry.cpp $(REF)/src/dyn/damping.cpp 
 	$(CXX) $(CXXFLAGS) -o $@ ref_setup.cpp $(REF)/src/grid/geometry.cpp $(REF)/src/dyn/damping.cpp \
 	  $(REF)/src/dyn/reference_state.cpp $(REF)/src/dyn/diagnostics.cpp $(REF)/src/dyn/state.cpp \
 	  $(REF)/src/dyn/advection.cpp $(REF)/src/time/timestep_control.cpp $(REF)/src/grid/nest.cpp
+
+# The reference's literal discrete equations (src/dyn/equations.cpp with
+# src/dyn/advection.cpp in the same translation unit -- advection.hpp
+# declares advection_from_string inline and advection.cpp defines it -- and
+# the unmodified coriolis, damping, tendency, state, reference_state and
+# geometry sources) with this directory's ref_equations.cpp driver; prints
+# the golden arrays of tests/golden/equations_reference.npz.
+EQNS = _ref/ref_equations
+
+all: $(EQNS)
+
+$(EQNS): ref_equations.cpp $(REF)/src/dyn/equations.cpp $(REF)/src/dyn/advection.cpp $(REF)/src/dyn/coriolis.cpp \
+	  $(REF)/src/dyn/damping.cpp $(REF)/src/dyn/tendency.cpp $(REF)/src/dyn/state.cpp \
+	  $(REF)/src/dyn/reference_state.cpp $(REF)/src/grid/geometry.cpp
+	mkdir -p _ref
+	$(CXX) $(CXXFLAGS) -ffp-contract=off -c -o _ref/eqadv.o -include $(REF)/src/dyn/advection.cpp $(REF)/src/dyn/equations.cpp
+	$(CXX) $(CXXFLAGS) -ffp-contract=off -o $@ ref_equations.cpp _ref/eqadv.o $(REF)/src/dyn/coriolis.cpp \
+	  $(REF)/src/dyn/damping.cpp $(REF)/src/dyn/tendency.cpp $(REF)/src/dyn/state.cpp \
+	  $(REF)/src/dyn/reference_state.cpp $(REF)/src/grid/geometry.cpp

@@ -180,6 +180,11 @@ def lib() -> ctypes.CDLL:
         "vibe_mi_reference_field_size": (_SZ, [_I, _I, _I, _I, _I]),
         "vibe_mi_state_from_reference": (_I, [_I, _I, _I, _I, ctypes.POINTER(_PD), _PD, _PD, _PD] + [_PD] * 6),
         "vibe_mi_state_to_reference": (_I, [_I, _I, _I, _I] + [_PD] * 9 + [_I, ctypes.POINTER(_PD)]),
+        "vibe_mi_literal_create": (_I, [_VP, ctypes.POINTER(_VP)]),
+        "vibe_mi_literal_destroy": (_I, [_VP]),
+        "vibe_mi_literal_tendencies": (_I, [_VP, ctypes.POINTER(_PD), ctypes.POINTER(_PD)]),
+        "vibe_mi_literal_acoustic": (_I, [_VP, ctypes.POINTER(_PD), ctypes.POINTER(_PD)]),
+        "vibe_mi_literal_max_sound_speed": (_I, [_VP, ctypes.POINTER(_PD), _PD]),
         "vibe_mi_domain_physics_enable": (_I, [_VP, _I]),
         "vibe_mi_domain_max_stable_dt": (_I, [_VP, _PD]),
         "vibe_mi_reference_state_from_profile": (_I, [_I, _I, _I, _D, _PD, _PD, _I, _PD, _PD, _D, _PD, _PD, _PD]),
