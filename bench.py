@@ -175,6 +175,48 @@ def cpu_baseline(name, nsub, threads):
     return cups, f"{what}, {n} RK3 steps, FP64 oracle ({n1} steps on 1 thread)", cups1
 
 
+def cpu_literal(name, nsub, budget_s=10.0, max_steps=3):
+    """Times the oracle's LITERAL restatement of the reference's step
+    (runge_kutta.cpp:77-118 on equations.cpp's operators: full prognostic
+    values, explicit forward-backward substeps, the reference's loop nest and
+    operation order; single thread, as the reference runs) on the same
+    column subset as cpu_baseline.  The literal equations are unbalanced and
+    diverge after a few steps, which does not matter for a rate."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    from literal_cases import LiteralDesc, oracle_step
+    from oracle_bind import _PD
+    from vibe_mi import configs
+
+    nxs = 96
+    case = (configs.ensemble_member(nxs, nxs, 60, seed=20240612) if name == "ensemble"
+            else configs.CONFIGS[name](nx=nxs, ny=nxs))
+    nx, ny, nz = case.nx, case.ny, case.nz
+    keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (case.zeta, case.zs, case.rho0, case.theta0,
+                                                                  case.pi0)]
+    d = LiteralDesc(nx, ny, nz, case.dx, case.dy, case.ztop, *[a.ctypes.data_as(_PD) for a in keep],
+                    2, 1, 45.0, 0.0, 2, 1)  # the reference's defaults: central2, f-plane 45, kessler
+    st = case.copy_state()
+    sh = (nz, ny, nx)
+    def faces(a, axis):
+        a = a.reshape(sh)
+        return np.ascontiguousarray(np.concatenate([a, np.take(a, [0], axis=axis)], axis=axis).ravel())
+    s = [faces(st["u"], 2), faces(st["v"], 1), np.ascontiguousarray(st["w"]),
+         np.ascontiguousarray(keep[2] + st["rho"]), np.ascontiguousarray(keep[3] + st["theta"]),
+         np.ascontiguousarray(keep[4] + st["pi"])] + [np.zeros(nx * ny * nz) for _ in range(6)]
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        oracle_step(d, s, case.dt, nsub, np.ascontiguousarray(case.sponge))
+        n += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or n >= max_steps:
+            break
+    return {"value": case.ncell * n / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{nx}x{ny}x{nz} columns of the same configuration, {n} literal RK3 steps "
+                      f"(runge_kutta.cpp:77-118 restated on the pinned equations.cpp operators), 1 thread"}
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -332,6 +374,8 @@ def main():
         cpu_cups, sample, cpu_cups1 = cpu_baseline(args.config, nsub, threads)
         cpu = {"value": cpu_cups, "unit": "cell-updates/s", "cores": threads, "kind": "port", "sample": sample,
                "value_1_thread": cpu_cups1, "cores_visible": visible, "cpu_model": cpu_model()}
+        log("[bench] literal CPU restatement of the reference's step, 1 thread")
+        cpu["literal"] = cpu_literal(args.config, nsub)
 
     out = {
         "metric": "cell-updates/s per RK3 step + % HBM roofline, FP32-mixed/batched vs FP64",

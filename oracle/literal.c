@@ -18,6 +18,7 @@
  */
 #include <math.h>
 #include <stddef.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "oracle.h"
@@ -367,4 +368,95 @@ void orc_literal_diagnose(const orc_literal_desc *d, const double *const s[12], 
     const double val = num / kP00;
     pi_out[c] = (val > 0.0) ? pow(val, kRd / kCv) : s[5][c];
   }
+}
+
+/* One step of the reference's RK3 integrator with its explicit
+ * forward-backward acoustic substeps, literally (RungeKutta3Integrator::step,
+ * src/time/runge_kutta.cpp:77-118, acoustic_substep :65-75) on the literal
+ * operators above, no physics, followed by the sponge
+ * (Damping::apply_sponge, src/dyn/damping.cpp:76-138) with the given
+ * per-level rates: for each stage h in {dt/3, dt/2, dt}, s = s0 + h R(s_prev)
+ * over all 12 species (runge_kutta.cpp:91-92 -- the reference passes its
+ * Tendency to State::add_scaled, which takes a State, the one compile error
+ * of that file; the intended sum is what is restated), then n_sub substeps
+ * of dtau = h / n_sub, each rho += dtau A_rho(s), pi += dtau A_pi(s), then
+ * u, v, w += dtau A(s) with A re-evaluated on the updated state.  Returns 0,
+ * 3 if the state is non-finite after a stage (runge_kutta.cpp:103-106), or
+ * 2 on bad arguments.  s is updated in place. */
+int orc_literal_step(const orc_literal_desc *d, double *const s[12], double dt, int n_sub, const double *sponge) {
+  if (d == NULL || s == NULL || d->nx <= 0 || d->ny <= 0 || d->nz <= 0) return 2;
+  const int nx = d->nx, ny = d->ny, nz = d->nz;
+  const size_t ncell = (size_t)nx * ny * nz;
+  const size_t nsz[12] = {(size_t)(nx + 1) * ny * nz, (size_t)nx * (ny + 1) * nz, (size_t)nx * ny * (nz + 1),
+                          ncell, ncell, ncell, ncell, ncell, ncell, ncell, ncell, ncell};
+  double *s0[12], *R[12];
+  for (int q = 0; q < 12; ++q) {
+    s0[q] = (double *)malloc(nsz[q] * sizeof(double));
+    R[q] = (double *)malloc(nsz[q] * sizeof(double));
+    if (s0[q] == NULL || R[q] == NULL) return 2;
+    memcpy(s0[q], s[q], nsz[q] * sizeof(double));
+  }
+  const int ns = n_sub > 1 ? n_sub : 1;
+  const double hk[3] = {dt / 3.0, dt / 2.0, dt};
+  int rc = 0;
+  for (int st = 0; st < 3 && rc == 0; ++st) {
+    orc_literal_tendencies(d, (const double *const *)s, R);
+    for (int q = 0; q < 12; ++q)
+      for (size_t c = 0; c < nsz[q]; ++c) s[q][c] = s0[q][c] + hk[st] * R[q][c];
+    const double dtau = hk[st] / (double)ns;
+    for (int n = 0; n < ns; ++n) {
+      orc_literal_acoustic(d, (const double *const *)s, R);
+      for (size_t c = 0; c < ncell; ++c) {
+        s[3][c] += dtau * R[3][c];
+        s[5][c] += dtau * R[5][c];
+      }
+      orc_literal_acoustic(d, (const double *const *)s, R);
+      for (int q = 0; q < 3; ++q)
+        for (size_t c = 0; c < nsz[q]; ++c) s[q][c] += dtau * R[q][c];
+    }
+    for (int q = 0; q < 12 && rc == 0; ++q)
+      for (size_t c = 0; c < nsz[q]; ++c)
+        if (!isfinite(s[q][c])) {
+          rc = 3;
+          break;
+        }
+  }
+  if (rc == 0 && sponge != NULL) {
+    int kmin = nz;
+    for (int k = 0; k < nz; ++k)
+      if (sponge[k] > 0.0) {
+        kmin = k;
+        break;
+      }
+    for (int k = kmin; k <= nz; ++k) {
+      const double a = sponge[k < nz - 1 ? k : nz - 1];
+      if (a <= 0.0) continue;
+      const double w = 1.0 / (1.0 + dt * a);
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+          const size_t cw = ((size_t)k * ny + j) * nx + i;
+          s[2][cw] = w * s[2][cw];
+          if (k == nz) continue;
+          s[3][cw] = w * (s[3][cw] + dt * a * d->rho0[cw]);
+          s[4][cw] = w * (s[4][cw] + dt * a * d->theta0[cw]);
+          s[5][cw] = w * (s[5][cw] + dt * a * d->pi0[cw]);
+        }
+      if (k == nz) continue;
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i <= nx; ++i) {
+          const size_t cu = ((size_t)k * ny + j) * (nx + 1) + i;
+          s[0][cu] = w * s[0][cu];
+        }
+      for (int j = 0; j <= ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+          const size_t cv = ((size_t)k * (ny + 1) + j) * nx + i;
+          s[1][cv] = w * s[1][cv];
+        }
+    }
+  }
+  for (int q = 0; q < 12; ++q) {
+    free(s0[q]);
+    free(R[q]);
+  }
+  return rc;
 }

@@ -61,6 +61,10 @@ int orc_literal_tendencies(const orc_literal_desc *d, const double *const s[12],
 int orc_literal_acoustic(const orc_literal_desc *d, const double *const s[12], double *const t[12]);
 double orc_literal_max_sound_speed(const orc_literal_desc *d, const double *const s[12]);
 void orc_literal_diagnose(const orc_literal_desc *d, const double *const s[12], double *pi_out);
+/* one literal RK3 step (runge_kutta.cpp:77-118) with explicit
+ * forward-backward substeps and the sponge rates sponge[nz] (NULL: none);
+ * 0, 3 if non-finite, 2 bad arguments */
+int orc_literal_step(const orc_literal_desc *d, double *const s[12], double dt, int n_sub, const double *sponge);
 
 /* ---- dynamical core (dycore.c) ---------------------------------------- */
 typedef struct {

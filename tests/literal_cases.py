@@ -101,6 +101,8 @@ def _bind():
         L.orc_literal_acoustic.argtypes = [ctypes.POINTER(LiteralDesc), ctypes.POINTER(_PD), ctypes.POINTER(_PD)]
         L.orc_literal_max_sound_speed.restype = _D
         L.orc_literal_max_sound_speed.argtypes = [ctypes.POINTER(LiteralDesc), ctypes.POINTER(_PD)]
+        L.orc_literal_step.restype = _I
+        L.orc_literal_step.argtypes = [ctypes.POINTER(LiteralDesc), ctypes.POINTER(_PD), _D, _I, _PD]
         L.orc_literal_diagnose.restype = None
         L.orc_literal_diagnose.argtypes = [ctypes.POINTER(LiteralDesc), ctypes.POINTER(_PD), _PD]
         L._literal_bound = True
@@ -123,3 +125,15 @@ def oracle_diagnose(d, s):
     out = np.zeros(s[5].size)
     _bind().orc_literal_diagnose(ctypes.byref(d), ptrs(s), out.ctypes.data_as(_PD))
     return out
+
+
+def oracle_step(d, s, dt, nsub, sponge=None, nsteps=1):
+    """nsteps literal RK3 steps of s in place; returns the last return code."""
+    L = _bind()
+    sp = None if sponge is None else np.ascontiguousarray(sponge, dtype=np.float64)
+    rc = 0
+    for _ in range(nsteps):
+        rc = L.orc_literal_step(ctypes.byref(d), ptrs(s), dt, nsub, None if sp is None else sp.ctypes.data_as(_PD))
+        if rc:
+            break
+    return rc

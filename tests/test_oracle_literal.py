@@ -6,11 +6,20 @@ golden vectors in tests/golden/equations_reference.npz): tendencies of all
 three numerics configurations (central 2/4/6, f-plane / none / beta plane,
 no / Del^2 / Del^4 divergence damping, dry / moist, terrain / flat).  Bit for
 bit: same operation order, -ffp-contract=off on both sides.
+
+The literal RK3 step (orc_literal_step, runge_kutta.cpp:77-118 with the
+explicit forward-backward substeps :65-75 and the sponge, damping.cpp:76-138)
+cannot be compared with the reference's own step: runge_kutta.cpp does not
+compile (line 92 passes a Tendency where State::add_scaled takes a State).
+Its operators are the pinned ones above; the test below checks the stage
+and substep composition against a numpy composition of those operators, and
+that swapping the substep's update order (u, v, w before rho, pi) changes
+the result, so the check sees the order.
 """
 import numpy as np
 import pytest
 
-from literal_cases import (CASES, SPECIES, desc, golden, oracle_diagnose, oracle_max_sound_speed,
+from literal_cases import (CASES, SPECIES, desc, golden, oracle_diagnose, oracle_max_sound_speed, oracle_step,
                            oracle_tendencies, state)
 
 
@@ -55,3 +64,51 @@ def test_acoustic_sound_speed_and_diagnose_bit_exact(gold, tag):
         assert np.array_equal(t[q], want), (tag, SPECIES[q])
     assert oracle_max_sound_speed(d, s) == gold[f"{tag}_max_sound_speed"][0]
     assert np.array_equal(oracle_diagnose(d, s), gold[f"{tag}_diagnose_pi"])
+
+
+def _numpy_step(d, s, dt, nsub, sponge, order_ok=True):
+    s = [a.copy() for a in s]
+    s0 = [a.copy() for a in s]
+    for h in (dt / 3.0, dt / 2.0, dt):
+        R = oracle_tendencies(d, s)
+        s = [a0 + h * r for a0, r in zip(s0, R)]
+        dtau = h / nsub
+        for _ in range(nsub):
+            first, second = ((3, 5), (0, 1, 2)) if order_ok else ((0, 1, 2), (3, 5))
+            A = oracle_tendencies(d, s, acoustic=True)
+            for q in first:
+                s[q] = s[q] + dtau * A[q]
+            A = oracle_tendencies(d, s, acoustic=True)
+            for q in second:
+                s[q] = s[q] + dtau * A[q]
+    nz = 5
+    kmin = next((k for k in range(nz) if sponge[k] > 0), nz)
+    r0, t0, p0 = (np.asarray(x, dtype=np.float64) for x in (d.rho0[:240], d.theta0[:240], d.pi0[:240]))
+    for k in range(kmin, nz + 1):
+        a = sponge[min(k, nz - 1)]
+        if a <= 0:
+            continue
+        w = 1.0 / (1.0 + dt * a)
+        sl = slice(k * 48, (k + 1) * 48)
+        s[2][sl] = w * s[2][sl]
+        if k == nz:
+            continue
+        for q, tgt in ((3, r0), (4, t0), (5, p0)):
+            s[q][sl] = w * (s[q][sl] + dt * a * tgt[sl])
+        s[0][k * 54:(k + 1) * 54] = w * s[0][k * 54:(k + 1) * 54]
+        s[1][k * 56:(k + 1) * 56] = w * s[1][k * 56:(k + 1) * 56]
+    return s
+
+
+def test_literal_step_is_the_rk3_composition(gold):
+    keep = []
+    d = desc(gold, "c2", keep)
+    s = state(gold, "hill")
+    sponge = np.array([0.0, 0.0, 0.0, 0.01, 0.05])
+    want = _numpy_step(d, s, 2.0, 3, sponge)
+    wrong = _numpy_step(d, s, 2.0, 3, sponge, order_ok=False)
+    got = [a.copy() for a in s]
+    assert oracle_step(d, got, 2.0, 3, sponge) == 0
+    for q, name in enumerate(SPECIES):
+        assert np.array_equal(got[q], want[q]), name
+    assert any(not np.array_equal(got[q], wrong[q]) for q in range(6))
