@@ -147,21 +147,18 @@ int work_list(vibe_mi_batch *b, const std::vector<unsigned char> &act, WorkList 
   for (int m = 0; m < static_cast<int>(b->doms.size()); ++m)
     if (act[m]) order.push_back(m);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->doms[x]->nz > b->doms[y]->nz; });
-  // Tiles are listed strip by strip (x-block outer, rows inner), so the
-  // contiguous XCD segments split a member into whole 64-column strips:
-  // each XCD's L2 holds the geometry of its own strips only (on grids whose
-  // geometry depends on x alone that is all of it), and a tile's north and
-  // south neighbour rows are worked on under the same L2.
   std::vector<int4> rows, st2, st4, st8;
   for (int m : order) {
     const vibe_mi_domain *d = b->doms[m];
     const int nbx = static_cast<int>(ceil_div(d->nx, 64));
-    for (int bx = 0; bx < nbx; ++bx) {
-      for (int j = 0; j < d->ny; ++j) rows.push_back(make_int4(m, bx, j, 0));
-      for (int by = 0; by < static_cast<int>(ceil_div(d->ny, 2)); ++by) st2.push_back(make_int4(m, bx, by, 0));
-      for (int by = 0; by < static_cast<int>(ceil_div(d->ny, 4)); ++by) st4.push_back(make_int4(m, bx, by, 0));
-      for (int by = 0; by < static_cast<int>(ceil_div(d->ny, 8)); ++by) st8.push_back(make_int4(m, bx, by, 0));
-    }
+    for (int j = 0; j < d->ny; ++j)
+      for (int bx = 0; bx < nbx; ++bx) rows.push_back(make_int4(m, bx, j, 0));
+    for (int by = 0; by < static_cast<int>(ceil_div(d->ny, 2)); ++by)
+      for (int bx = 0; bx < nbx; ++bx) st2.push_back(make_int4(m, bx, by, 0));
+    for (int by = 0; by < static_cast<int>(ceil_div(d->ny, 4)); ++by)
+      for (int bx = 0; bx < nbx; ++bx) st4.push_back(make_int4(m, bx, by, 0));
+    for (int by = 0; by < static_cast<int>(ceil_div(d->ny, 8)); ++by)
+      for (int bx = 0; bx < nbx; ++bx) st8.push_back(make_int4(m, bx, by, 0));
   }
   WorkList w;
   if (int rc = upload_list(b, rows, &w.rows)) return rc;
