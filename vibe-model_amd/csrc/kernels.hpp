@@ -647,6 +647,9 @@ inline constexpr int kSubstepWavesPerEu = (sizeof(T) == 4 && MAXCH <= 12 && !CP)
 #ifndef VMI_FP32_INCREMENT
 #define VMI_FP32_INCREMENT 0
 #endif
+#ifndef VMI_COMP_FP32_SUMS
+#define VMI_COMP_FP32_SUMS 0
+#endif
 template <typename T, typename I, bool CP>
 inline constexpr bool kSubstepIncrement = VMI_FP32_INCREMENT != 0 && sizeof(T) == 4 && sizeof(I) == 4 && !CP;
 
@@ -684,6 +687,9 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   // sums on top of it (the compensated mode)
   constexpr bool IN = CP || INC;
   constexpr bool RM = CP;
+  // F64S: the compensated mode's mass-flux sums in FP64 from FP64 geometry
+  // (compile-time switch VMI_COMP_FP32_SUMS: the FP32 forms of the plain mode)
+  constexpr bool F64S = CP && VMI_COMP_FP32_SUMS == 0;
   const I *__restrict__ u = sa.u;
   const I *__restrict__ v = sa.v;
   const I *__restrict__ w = sa.w;
@@ -703,7 +709,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   T *s_be = s_td + nlds;                      // [nz-1][64] beta
   T *s_cp = s_be + nlds;                      // [nz-1][64] c'
   // RM: the slope and vertical mass fluxes are formed in FP64
-  using TF = std::conditional_t<RM, double, T>;
+  using TF = std::conditional_t<F64S, double, T>;
   __shared__ TF s_mc[NW][64], s_fh[NW][64];
   __shared__ T s_ph[NW][64];
   const int lane = threadIdx.x & 63;
@@ -785,7 +791,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
       q[GUE] = colE ? gldc(g.cthu, tE + s) : q[GUW];
       q[GVS] = gldc(g.cthv, tC + s);
       q[GVN] = colN ? gldc(g.cthv, tN + s) : q[GVS];
-      if constexpr (RM) {
+      if constexpr (F64S) {
         const uint32_t s8 = static_cast<uint32_t>(k) * lsT8;
         qd[0] = gldc(sa.rhu64, tC8 + s8);
         qd[1] = colE ? gldc(sa.rhu64, tE8 + s8) : qd[0];
@@ -916,7 +922,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         }
         TF mc = TF(0);
         T hd;
-        if constexpr (RM) {
+        if constexpr (F64S) {
           // full new winds hi + lo (+ background) in FP64; the face mass
           // fluxes D rho0 u (about 1e5) are differenced across the cell in
           // FP64 from FP64 geometry, as is the slope flux
@@ -956,7 +962,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         if (k >= 1) {
           s_td[(k - 1) * 64 + lane] = IN ? -(gs[GKB] * ao * (pc[0] - pmC)) : wk - gs[GKB] * ao * (pc[0] - pmC);
           TF awk, r0k;
-          if constexpr (RM) {  // the full old-w flux
+          if constexpr (F64S) {  // the full old-w flux
             awk = static_cast<double>(wk) + static_cast<double>(xs[10]);
             r0k = gd[4];
           } else if constexpr (IN) {
@@ -967,7 +973,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
             r0k = gs[R0B];
           }
           if (c > 0) {
-            const TF zk = RM ? TF(C(sa.zeta64)[k]) : TF(C(g.zeta)[k]);
+            const TF zk = F64S ? TF(C(sa.zeta64)[k]) : TF(C(g.zeta)[k]);
             const TF Mw = (TF(1) - zk) * (TF(0.5) * (mc_prev + mc));
             Fh[c] = r0k * (awk - Mw);
           } else {
@@ -991,7 +997,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   // scaled by 1/pivot for the factored sweep.
   if (nlev > 0) {
     if (k0 >= 1) {
-      const TF zk = RM ? TF(C(sa.zeta64)[k0]) : TF(C(g.zeta)[k0]);
+      const TF zk = F64S ? TF(C(sa.zeta64)[k0]) : TF(C(g.zeta)[k0]);
       const TF Mw = (TF(1) - zk) * (TF(0.5) * (s_mc[wv - 1][lane] + mc_first));
       Fh[0] = r0w_first * (aw_first - Mw);
     }
@@ -1146,8 +1152,8 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         t[2] = T(gld(rho, vC + si));
         t[4] = T(gld(pi, vC + si));
         t[6] = T(gld(w, vC + si));
+        if constexpr (F64S) t8[0] = gldc(sa.r0w64, tC8 + static_cast<uint32_t>(k + 1) * lsT8);
         if constexpr (RM) {
-          t8[0] = gldc(sa.r0w64, tC8 + static_cast<uint32_t>(k + 1) * lsT8);
           const uint32_t sl = static_cast<uint32_t>(k) * lsL;
           t[3] = bf16_val(gld(sa.rl, lC + sl));
           t[5] = bf16_val(gld(sa.pl, lC + sl));
@@ -1165,7 +1171,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
     // flux through the chunk's lower interface
     TF flo = TF(0);
     if (k0 >= 1) {
-      if constexpr (RM)
+      if constexpr (F64S)
         flo = Fh[0] + (static_cast<double>(an) * gldc(sa.r0w64, tC8 + static_cast<uint32_t>(k0) * lsT8)) *
                           static_cast<double>(s_td[(k0 - 1) * 64 + lane]);
       else
@@ -1181,7 +1187,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         const TF fhn = (c + 1 < nlev) ? Fh[(c + 1) < MAXCH ? c + 1 : c] : fh_next_chunk;
         TF fhi = TF(0);
         if (k + 1 < nz) {
-          if constexpr (RM)
+          if constexpr (F64S)
             fhi = fhn + (static_cast<double>(an) * DD8[c % 3][0]) * static_cast<double>(s_td[k * 64 + lane]);
           else
             fhi = fhn + (an * dd[0]) * s_td[k * 64 + lane];
