@@ -241,9 +241,9 @@ int launch_substep_nw(vibe_mi_batch *b, const WorkList &w, int st, int q, int ab
   // s_mc, s_fh (FP64 in the compensated mode), s_ph and the tile loop's slots
   const size_t lds_static = ((CP ? 2 * sizeof(double) : 2 * sizeof(T)) + sizeof(T)) * NW * 64 + 64;
   if (lds + lds_static > 160 * 1024) return VIBE_C_ERR_BAD_ARG;
-#define VMI_BSUB(M)                                                                                           \
+#define VMI_BSUB1(M, RO)                                                                                      \
   do {                                                                                                        \
-    auto *kern = &kb_substep<T, O, NW, M, GM, CP>;                                                                \
+    auto *kern = &kb_substep<T, O, NW, M, GM, CP, RO>;                                                            \
     static bool attr = false;                                                                                 \
     if (!attr) {                                                                                              \
       VMI_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),                                       \
@@ -255,6 +255,17 @@ int launch_substep_nw(vibe_mi_batch *b, const WorkList &w, int st, int q, int ab
     unsigned grid = 0;                                                                                        \
     if (int rc = resident(b, kern, 64 * NW, lds, w.rows.ntiles, &grid)) return rc;                            \
     hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * NW), lds, b->stream, D, w.rows, st, q, abase, last, ctr);  \
+  } while (0)
+  // FP32 outputs: the stage's last substep (last = 1) is the variant that
+  // writes rho' (dry: set both variants' attribute)
+#define VMI_BSUB(M)                                 \
+  do {                                              \
+    if constexpr (sizeof(T) == 4 && sizeof(O) == 4) { \
+      if (dry || !last) VMI_BSUB1(M, false);        \
+      if (dry || last) VMI_BSUB1(M, true);          \
+    } else {                                        \
+      VMI_BSUB1(M, (sizeof(O) == 8));               \
+    }                                               \
   } while (0)
   if constexpr (NW == 16) {
     if (ch <= 4) VMI_BSUB(4);
@@ -275,6 +286,7 @@ int launch_substep_nw(vibe_mi_batch *b, const WorkList &w, int st, int q, int ab
     else return VIBE_C_ERR_BAD_ARG;
   }
 #undef VMI_BSUB
+#undef VMI_BSUB1
   VMI_HIP(hipGetLastError());
   return 0;
 }

@@ -132,8 +132,10 @@ __global__ void __launch_bounds__(64 * TY) kb_stage(const BDom<T> *__restrict__ 
 // Acoustic substep q of stage st over row tiles, from the acoustic set
 // A[(abase+q)&1] to A[(abase+q+1)&1].  The last substep of a stage
 // (last = 1) checks finiteness; with O = double it writes the FP64 stage
-// output S[out] instead.  CP: the compensated FP32 mode (substep_row).
-template <typename T, typename O, int NW, int MAXCH, int GM, bool CP = false>
+// output S[out] instead.  CP: the compensated FP32 mode (substep_row).  FP32
+// substeps carry chi = rho' - pi'/kappa in the stage's output set
+// A[abase&1] in place of rho'; RO (a stage's last substep) writes rho'.
+template <typename T, typename O, int NW, int MAXCH, int GM, bool CP = false, bool RO = (sizeof(O) == 8)>
 __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kSubstepWavesPerEu<T, NW, MAXCH, CP>))) kb_substep(const BDom<T> *__restrict__ D, TileList L, int st, int q,
                                                       int abase, int last, int *__restrict__ ctr) {
   tile_loop(ctr, L, [&](int t) {
@@ -147,6 +149,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kS
     sa.w = in.w;
     sa.rho = in.rho;
     sa.pi = in.pi;
+    sa.chi = dd.A[abase & 1].rho;
     sa.tab = dd.tab[st];
     sa.dtau = dd.dtau[st];
     sa.an = dd.an;
@@ -180,6 +183,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kS
       sa.check = last;
     }
     if constexpr (CP) {
+      sa.chil = dd.L[abase & 1].rho;
       const AcousticLo li = dd.L[(abase + q) & 1];
       sa.ul = li.u, sa.vl = li.v, sa.wl = li.w, sa.rl = li.rho, sa.pl = li.pi;
       if constexpr (sizeof(O) == 4) {
@@ -189,7 +193,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kS
       sa.rhu64 = dd.rhu64, sa.rhv64 = dd.rhv64, sa.r0w64 = dd.r0w64, sa.sx64 = dd.sx64, sa.sy64 = dd.sy64;
       sa.zeta64 = dd.zeta64, sa.rdx64 = dd.rdx64, sa.rdy64 = dd.rdy64;
     }
-    substep_row<T, T, O, NW, MAXCH, GM, CP, kSubstepIncrement<T, T, CP>>(g, sa, tl.y, tl.z);
+    substep_row<T, T, O, NW, MAXCH, GM, CP, kSubstepIncrement<T, T, CP>, RO>(g, sa, tl.y, tl.z);
   });
 }
 
@@ -197,18 +201,23 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kS
 
 // Explicit instantiation of every FP32 batched kernel shape (X = extern for a
 // declaration in batch.hip, empty for the definitions in batch_f32_*.hip).
-#define VMI_BATCH_F32_SUBSTEP_GM(X, O, GM, CP)                                                                     \
-  X template __global__ void vibe_mi::kb_substep<float, O, 16, 4, GM, CP>(VMI_KB_SUBSTEP_ARGS);                    \
-  X template __global__ void vibe_mi::kb_substep<float, O, 16, 8, GM, CP>(VMI_KB_SUBSTEP_ARGS);                    \
-  X template __global__ void vibe_mi::kb_substep<float, O, 8, 4, GM, CP>(VMI_KB_SUBSTEP_ARGS);                     \
-  X template __global__ void vibe_mi::kb_substep<float, O, 8, 8, GM, CP>(VMI_KB_SUBSTEP_ARGS);                     \
-  X template __global__ void vibe_mi::kb_substep<float, O, 8, 12, GM, CP>(VMI_KB_SUBSTEP_ARGS);                    \
-  X template __global__ void vibe_mi::kb_substep<float, O, 8, 16, GM, CP>(VMI_KB_SUBSTEP_ARGS);                    \
-  X template __global__ void vibe_mi::kb_substep<float, O, 4, 8, GM, CP>(VMI_KB_SUBSTEP_ARGS);                     \
-  X template __global__ void vibe_mi::kb_substep<float, O, 4, 12, GM, CP>(VMI_KB_SUBSTEP_ARGS);                    \
-  X template __global__ void vibe_mi::kb_substep<float, O, 4, 16, GM, CP>(VMI_KB_SUBSTEP_ARGS);                    \
-  X template __global__ void vibe_mi::kb_substep<float, O, 4, 24, GM, CP>(VMI_KB_SUBSTEP_ARGS);                    \
-  X template __global__ void vibe_mi::kb_substep<float, O, 4, 32, GM, CP>(VMI_KB_SUBSTEP_ARGS);
+#define VMI_BATCH_F32_SUBSTEP_GM_RO(X, O, GM, CP, RO)                                                              \
+  X template __global__ void vibe_mi::kb_substep<float, O, 16, 4, GM, CP, RO>(VMI_KB_SUBSTEP_ARGS);                \
+  X template __global__ void vibe_mi::kb_substep<float, O, 16, 8, GM, CP, RO>(VMI_KB_SUBSTEP_ARGS);                \
+  X template __global__ void vibe_mi::kb_substep<float, O, 8, 4, GM, CP, RO>(VMI_KB_SUBSTEP_ARGS);                 \
+  X template __global__ void vibe_mi::kb_substep<float, O, 8, 8, GM, CP, RO>(VMI_KB_SUBSTEP_ARGS);                 \
+  X template __global__ void vibe_mi::kb_substep<float, O, 8, 12, GM, CP, RO>(VMI_KB_SUBSTEP_ARGS);                \
+  X template __global__ void vibe_mi::kb_substep<float, O, 8, 16, GM, CP, RO>(VMI_KB_SUBSTEP_ARGS);                \
+  X template __global__ void vibe_mi::kb_substep<float, O, 4, 8, GM, CP, RO>(VMI_KB_SUBSTEP_ARGS);                 \
+  X template __global__ void vibe_mi::kb_substep<float, O, 4, 12, GM, CP, RO>(VMI_KB_SUBSTEP_ARGS);                \
+  X template __global__ void vibe_mi::kb_substep<float, O, 4, 16, GM, CP, RO>(VMI_KB_SUBSTEP_ARGS);                \
+  X template __global__ void vibe_mi::kb_substep<float, O, 4, 24, GM, CP, RO>(VMI_KB_SUBSTEP_ARGS);                \
+  X template __global__ void vibe_mi::kb_substep<float, O, 4, 32, GM, CP, RO>(VMI_KB_SUBSTEP_ARGS);
+// FP32 outputs: without and with rho' (a stage's last substep); FP64
+// outputs always write rho'
+#define VMI_BATCH_F32_SUBSTEP_F(X, GM, CP) \
+  VMI_BATCH_F32_SUBSTEP_GM_RO(X, float, GM, CP, false) VMI_BATCH_F32_SUBSTEP_GM_RO(X, float, GM, CP, true)
+#define VMI_BATCH_F32_SUBSTEP_D(X, GM, CP) VMI_BATCH_F32_SUBSTEP_GM_RO(X, double, GM, CP, true)
 #define VMI_BATCH_F32_STAGE_GM(X, GM, PH)                                                                     \
   X template __global__ void vibe_mi::kb_stage<float, 2, GM, false, PH>(VMI_KB_STAGE_ARGS);                    \
   X template __global__ void vibe_mi::kb_stage<float, 4, GM, false, PH>(VMI_KB_STAGE_ARGS);                    \
@@ -217,10 +226,10 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kS
   X template __global__ void vibe_mi::kb_stage<float, 4, GM, true, PH>(VMI_KB_STAGE_ARGS);                     \
   X template __global__ void vibe_mi::kb_stage<float, 8, GM, true, PH>(VMI_KB_STAGE_ARGS);
 #define VMI_BATCH_F32_GM(X, GM)                                                                                 \
-  VMI_BATCH_F32_SUBSTEP_GM(X, float, GM, false)                                                                 \
-  VMI_BATCH_F32_SUBSTEP_GM(X, double, GM, false)                                                                \
-  VMI_BATCH_F32_SUBSTEP_GM(X, float, GM, true)                                                                  \
-  VMI_BATCH_F32_SUBSTEP_GM(X, double, GM, true)                                                                 \
+  VMI_BATCH_F32_SUBSTEP_F(X, GM, false)                                                                         \
+  VMI_BATCH_F32_SUBSTEP_D(X, GM, false)                                                                         \
+  VMI_BATCH_F32_SUBSTEP_F(X, GM, true)                                                                          \
+  VMI_BATCH_F32_SUBSTEP_D(X, GM, true)                                                                          \
   VMI_BATCH_F32_STAGE_GM(X, GM, false)                                                                          \
   VMI_BATCH_F32_STAGE_GM(X, GM, true)
 #define VMI_KB_SUBSTEP_ARGS const vibe_mi::BDom<float> *__restrict__, vibe_mi::TileList, int, int, int, int, int *__restrict__

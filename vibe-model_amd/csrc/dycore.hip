@@ -297,6 +297,12 @@ int launch_substep(vibe_mi_domain *d, const Geo<T> &g, const T *tab, const I *u,
                    int check, const AcousticLo *li, const AcousticLo *lo) {
   SubArgs<T, I, O> sa{u, v, w, rho, pi, HeviTab<T>::at(tab, d->nz, d->rk), dtau, an, ao,
                       uo, vo, wo, rho_o, pi_o, check, d->flag};
+  if constexpr (sizeof(I) == 4) {
+    // FP32 substeps carry chi = rho' - pi'/kappa, which the stage writes to
+    // A[0] (and its remainder to L32[0]); the last substep forms rho'
+    sa.chi = d->A32[0].rho;
+    if (li != nullptr) sa.chil = d->L32[0].rho;
+  }
   if constexpr (sizeof(T) == 4 && sizeof(I) == 4) {
     if (li != nullptr) {
       sa.ul = li->u, sa.vl = li->v, sa.wl = li->w, sa.rl = li->rho, sa.pl = li->pi;

@@ -503,7 +503,14 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageCur<CI> &
       const double tn = S0[4] + h * static_cast<double>(rt);
       const double pn = S0[5] + h * static_cast<double>(rp);
       if (act) {
-        put_acoustic(out.rho, out.rl, ko, rn);
+        // FP32 acoustic sets carry chi = rho' - pi'/kappa in place of rho':
+        // the substeps change rho' and pi' only together (d pi' = kappa
+        // d rho' per cell), so chi is constant over a stage's substeps and
+        // the stage's last substep recovers rho' = chi + pi'/kappa
+        if constexpr (sizeof(O) == 4)
+          put_acoustic(out.rho, out.rl, ko, rn - pn * (1.0 / static_cast<double>(kapk)));
+        else
+          put_acoustic(out.rho, out.rl, ko, rn);
         gst(out.th, cCb + static_cast<uint32_t>(k) * lsD, sponge_scale(sp, k, nz, tn));
         put_acoustic(out.pi, out.pl, ko, pn);
         bad |= !isfinite(tn);
@@ -629,6 +636,10 @@ struct SubArgs {
   // out
   const uint16_t *ul = nullptr, *vl = nullptr, *wl = nullptr, *rl = nullptr, *pl = nullptr;
   uint16_t *ulo = nullptr, *vlo = nullptr, *wlo = nullptr, *rlo = nullptr, *plo = nullptr;
+  // FP32 state: chi = rho' - pi'/kappa of the stage (and its remainder),
+  // read by the stage's last substep only (rho_o != nullptr)
+  const I *chi = nullptr;
+  const uint16_t *chil = nullptr;
   // compensated mode: FP64 geometry for the mass-flux sums
   const double *rhu64 = nullptr, *rhv64 = nullptr, *r0w64 = nullptr, *sx64 = nullptr, *sy64 = nullptr,
                *zeta64 = nullptr;
@@ -678,7 +689,10 @@ inline constexpr bool kSubstepIncrement = VMI_FP32_INCREMENT != 0 && sizeof(T) =
 // A w_new = w - gk (ao dpi'_old + an dP^ao) of the plain form (P^ao with the
 // ao part of the old-w flux), and the closing flux rho0_w (w - M) + an rho0_w dw
 // equals rho0_w (an w_new + ao w - M).
-template <typename T, typename I, typename O, int NW, int MAXCH, int GM, bool CP = false, bool INC = false>
+// RO (FP32 state): this substep writes rho' (a stage's last substep; always
+// with FP64 outputs).
+template <typename T, typename I, typename O, int NW, int MAXCH, int GM, bool CP = false, bool INC = false,
+          bool RO = (sizeof(O) == 8)>
 __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I, O> &sa, int bx, int j) {
   static_assert(!CP || (sizeof(T) == 4 && sizeof(I) == 4), "compensated storage is FP32");
   static_assert(!INC || (sizeof(T) == 4 && sizeof(I) == 4), "the increment form is an FP32 form");
@@ -690,6 +704,12 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   // F64S: the compensated mode's mass-flux sums in FP64 from FP64 geometry
   // (compile-time switch VMI_COMP_FP32_SUMS: the FP32 forms of the plain mode)
   constexpr bool F64S = CP && VMI_COMP_FP32_SUMS == 0;
+  // RX (FP32 state): rho' is not carried through the substeps.  Every
+  // substep changes pi' by kappa times its change of rho' (Q - pi' =
+  // kappa (E - rho') and the closing adds vd to rho', kappa vd to pi'), so
+  // the stage writes chi = rho' - pi'/kappa and only the stage's last
+  // substep (rho_o != nullptr) forms rho' = chi + pi'/kappa.
+  constexpr bool RX = sizeof(I) == 4;
   const I *__restrict__ u = sa.u;
   const I *__restrict__ v = sa.v;
   const I *__restrict__ w = sa.w;
@@ -769,7 +789,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
       x[2] = T(gld(v, vC + s));
       x[3] = T(gld(v, vN + s));
       x[4] = T(gld(w, vC + s));
-      x[5] = T(gld(rho, vC + s));
+      if constexpr (!RX) x[5] = T(gld(rho, vC + s));
       if constexpr (RM) {
         const uint32_t sl = static_cast<uint32_t>(k) * lsL;
         x[6] = bf16_val(gld(sa.ul, lC + sl));
@@ -955,7 +975,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         }
         const T cq = gs[KAP] * dtD;
         // IN: E and Q hold the increments of rho' and pi'
-        E[c] = IN ? -(dtD * hd) : xs[5] - dtD * hd;
+        if constexpr (!RX) E[c] = IN ? -(dtD * hd) : xs[5] - dtD * hd;
         Q[c] = IN ? -(cq * hd) : pc[0] - cq * hd;
         const T wk = xs[4];
         Fh[c] = TF(0);
@@ -1149,22 +1169,26 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
       t[1] = gldc(g.kap, tC + s);
       if constexpr (IN) {  // the old state of the level: rho', pi', w (and RM: their remainders, FP64 r0w above)
         const uint32_t si = static_cast<uint32_t>(k) * lsI;
-        t[2] = T(gld(rho, vC + si));
+        if constexpr (!RX) t[2] = T(gld(rho, vC + si));
         t[4] = T(gld(pi, vC + si));
         t[6] = T(gld(w, vC + si));
         if constexpr (F64S) t8[0] = gldc(sa.r0w64, tC8 + static_cast<uint32_t>(k + 1) * lsT8);
         if constexpr (RM) {
           const uint32_t sl = static_cast<uint32_t>(k) * lsL;
-          t[3] = bf16_val(gld(sa.rl, lC + sl));
+          if constexpr (!RX) t[3] = bf16_val(gld(sa.rl, lC + sl));
           t[5] = bf16_val(gld(sa.pl, lC + sl));
           t[7] = bf16_val(gld(sa.wl, lC + sl));
         } else {
           t[3] = t[5] = t[7] = T(0);
         }
       }
+      if constexpr (RX && RO) {  // chi of the level (and its remainder)
+        t[2] = T(gld(sa.chi, vC + static_cast<uint32_t>(k) * lsI));
+        t[3] = RM ? bf16_val(gld(sa.chil, lC + static_cast<uint32_t>(k) * lsL)) : T(0);
+      }
     };
     const TF fh_next_chunk = (k0 + nlev < nz) ? s_fh[wv + 1][lane] : TF(0);
-    T DD[3][IN ? 8 : 2];
+    T DD[3][(IN || (RX && RO)) ? 8 : 2];
     double DD8[3][1];
     ld_d(k0, DD[0], DD8[0]);
     ld_d(k0 + min(1, nlev - 1), DD[1], DD8[1]);
@@ -1195,9 +1219,9 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         const T vd = -dtD * (static_cast<T>(fhi - flo) * C(g.rdzeta)[k]);
         F2 cr{}, cpn{}, cw{};
         // IN: increments of rho', pi' and w, each added to the old value once
-        const T dr = E[c] + vd, dp = Q[c] + dd[1] * vd;
+        const T dr = RX ? T(0) : E[c] + vd, dp = Q[c] + dd[1] * vd;
         if constexpr (RM) {
-          cr = comp_add(dd[2], dd[3], E[c] + vd);
+          if constexpr (!RX) cr = comp_add(dd[2], dd[3], E[c] + vd);
           cpn = comp_add(dd[4], dd[5], Q[c] + dd[1] * vd);
           if (k >= 1) {
             cw = comp_add(dd[6], dd[7], s_td[(k - 1) * 64 + lane]);
@@ -1206,7 +1230,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
             cw.lo = static_cast<float>(mc0 - static_cast<double>(cw.hi));
           }
         }
-        T rn, pn, wn;
+        T rn = T(0), pn, wn;
         if constexpr (RM) {
           rn = cr.hi, pn = cpn.hi, wn = cw.hi;
         } else if constexpr (IN) {
@@ -1214,9 +1238,19 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
           pn = dd[4] + dp;
           wn = (k >= 1) ? dd[6] + s_td[(k - 1) * 64 + lane] : static_cast<T>(mc0);
         } else {
-          rn = E[c] + vd;
+          if constexpr (!RX) rn = E[c] + vd;
           pn = Q[c] + dd[1] * vd;
           wn = (k >= 1) ? s_td[(k - 1) * 64 + lane] : static_cast<T>(mc0);
+        }
+        // RX, RO: rho' = chi + pi'/kappa in FP64 from the full new pi'
+        double rx = 0.0;
+        if constexpr (RX && RO) {
+          double pv = static_cast<double>(pn);
+          if constexpr (RM)
+            pv += static_cast<double>(cpn.lo);
+          else if constexpr (IN)
+            pv = static_cast<double>(dd[4]) + static_cast<double>(dp);
+          rx = (static_cast<double>(dd[2]) + static_cast<double>(dd[3])) + pv * (1.0 / static_cast<double>(dd[1]));
         }
         if (active) {
           if constexpr (sizeof(O) == 8) {
@@ -1230,22 +1264,34 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
               pf = static_cast<double>(dd[4]) + static_cast<double>(dp);
               if (k >= 1) wf = static_cast<double>(dd[6]) + static_cast<double>(s_td[(k - 1) * 64 + lane]);
             }
+            if constexpr (RX) rf = rx;
             gst(rho_o, oC + so, sponge_scale(sa.sp, k, nz, rf));
             gst(pi_o, oC + so, sponge_scale(sa.sp, k, nz, pf));
             gst(wo, oC + so, sponge_scale(sa.sp, k, nz, wf));
           } else {
-            gst(rho_o, oC + so, static_cast<O>(rn));
+            const uint32_t sl = static_cast<uint32_t>(k) * lsL;
+            if constexpr (RX && RO) {
+              const float rh = static_cast<float>(rx);
+              gst(rho_o, oC + so, static_cast<O>(rh));
+              if constexpr (RM) gst(sa.rlo, lC + sl, bf16_bits(static_cast<float>(rx - static_cast<double>(rh))));
+            } else if constexpr (!RX) {
+              gst(rho_o, oC + so, static_cast<O>(rn));
+              if constexpr (RM) gst(sa.rlo, lC + sl, bf16_bits(cr.lo));
+            }
             gst(pi_o, oC + so, static_cast<O>(pn));
             gst(wo, oC + so, static_cast<O>(wn));
             if constexpr (RM) {
-              const uint32_t sl = static_cast<uint32_t>(k) * lsL;
-              gst(sa.rlo, lC + sl, bf16_bits(cr.lo));
               gst(sa.plo, lC + sl, bf16_bits(cpn.lo));
               gst(sa.wlo, lC + sl, bf16_bits(cw.lo));
             }
           }
         }
-        bad |= !(isfinite(rn) && isfinite(pn) && isfinite(wn));
+        if constexpr (RX && RO)
+          bad |= !(isfinite(rx) && isfinite(pn) && isfinite(wn));
+        else if constexpr (RX)
+          bad |= !(isfinite(pn) && isfinite(wn));
+        else
+          bad |= !(isfinite(rn) && isfinite(pn) && isfinite(wn));
         flo = fhi;
       }
     }

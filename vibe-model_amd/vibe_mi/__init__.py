@@ -34,7 +34,9 @@ from dataclasses import dataclass, field
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libvibe_mi.so")
+# VIBE_MI_LIB names another build of the library (e.g. one compiled with other
+# tile shapes); the default is the in-tree build
+LIB_PATH = os.environ.get("VIBE_MI_LIB") or os.path.join(_HERE, "lib", "libvibe_mi.so")
 
 VIBE_C_OK = 0
 VIBE_C_ERR_SINGULAR = 1
