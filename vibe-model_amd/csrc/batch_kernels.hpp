@@ -114,7 +114,8 @@ __global__ void __launch_bounds__(64 * TY) kb_stage(const BDom<T> *__restrict__ 
     StageOut<T> so{a.u, a.v, a.w, a.rho, a.pi, nxt.th};
     if constexpr (sizeof(T) == 4) {  // remainders, compensated mode only (else null)
       const AcousticLo l = dd.L[aout];
-      so.ul = l.u, so.vl = l.v, so.wl = l.w, so.rl = l.rho, so.pl = l.pi;
+      so.ul = l.u, so.rl = l.rho;
+      so.vl = cp_keeps(2) ? l.v : nullptr, so.wl = cp_keeps(4) ? l.w : nullptr, so.pl = cp_keeps(8) ? l.pi : nullptr;
     }
     SpongeArgs sp;  // the last stage's theta' is the step's final one
     if (st == 2) sp = SpongeArgs{dd.sponge, dd.dt, dd.kmin_sponge};

@@ -195,7 +195,8 @@ int run_stage(vibe_mi_domain *d, const Geo<T> &g, const FieldSet &cur, const Fie
   StageOut<T> so{a.u, a.v, a.w, a.rho, a.pi, nxt.th};
   if (sizeof(T) == 4 && d->precision == VIBE_MI_MIXED_COMP) {
     const AcousticLo &l = d->L32[0];
-    so.ul = l.u, so.vl = l.v, so.wl = l.w, so.rl = l.rho, so.pl = l.pi;
+    so.ul = l.u, so.rl = l.rho;
+    so.vl = cp_keeps(2) ? l.v : nullptr, so.wl = cp_keeps(4) ? l.w : nullptr, so.pl = cp_keeps(8) ? l.pi : nullptr;
   }
   const dim3 grid(ceil_div(d->nx, 64), ceil_div(d->ny, 4)), block(64, 4);
   if (d->ph_on && d->ph_enabled)

@@ -646,20 +646,41 @@ struct SubArgs {
   double rdx64 = 0.0, rdy64 = 0.0;
 };
 
-// Waves per SIMD the substep kernels are compiled for: FP32 shapes with up
-// to 12 levels per wave are held to 128 VGPRs (four waves per SIMD); the
-// other shapes, and the compensated kernels (about 175 VGPRs, two waves per
-// SIMD; three would spill), are register-limited by their own size.
+// Compensated mode: the substep fields that carry a remainder (bits: 1 u,
+// 2 v, 4 w, 8 pi'); a field without one is rounded to FP32 at every
+// substep as in the plain mode.  Compile-time switch, u alone by default:
+// at 256x256x96 over 1 h the worst field is 5.86e-5 of FP64 with u alone
+// and 5.83e-5 with all four, against 1.01e-4 with v, w and pi' but not u.
+#ifndef VMI_CP_MASK
+#define VMI_CP_MASK 1
+#endif
+// Remainder arrays a compensated stage writes: u and chi (rho' - pi'/kappa,
+// read by the stage's last substep), plus the fields VMI_CP_MASK names.
+__host__ __device__ constexpr bool cp_keeps(int bit) { return (VMI_CP_MASK & bit) != 0; }
+// Waves per SIMD the substep kernels are compiled for: plain FP32 shapes
+// with up to 12 levels per wave (about 110 VGPRs) and compensated ones with
+// up to 8 (about 127) are held to 128 VGPRs (four waves per SIMD); the other
+// shapes are register-limited by their own size (the compensated kernel at
+// 12 levels per wave takes 139 VGPRs, three waves per SIMD; held to four it
+// spills).  With remainders on pi' or w (increment forms) the compensated
+// kernels take about 175 VGPRs.
 template <typename T, int NW, int MAXCH, bool CP = false>
-inline constexpr int kSubstepWavesPerEu = (sizeof(T) == 4 && MAXCH <= 12 && !CP) ? 4 : 1;
+inline constexpr int kSubstepWavesPerEu =
+    (sizeof(T) == 4 && ((!CP && MAXCH <= 12) || (CP && MAXCH <= 8 && (VMI_CP_MASK & 12) == 0))) ? 4 : 1;
 
 // Increment form of the plain FP32 substep (compile-time switch
 // VMI_FP32_INCREMENT): rho', pi' and w updated as old value + increment.
 #ifndef VMI_FP32_INCREMENT
 #define VMI_FP32_INCREMENT 0
 #endif
-#ifndef VMI_COMP_FP32_SUMS
-#define VMI_COMP_FP32_SUMS 0
+// Compensated mode: which mass-flux sums are formed in FP64 from FP64
+// geometry (compile-time switch; bit 1: the horizontal divergence in FP32,
+// bit 2: the slope flux and the vertical mass flux in FP32; 0: both FP64).
+// Default 2: the horizontal divergence in FP64 (in FP32 the worst field is
+// 1.0e-4 at 256x256x96 after 1 h, with it 6.1e-5), the slope and vertical
+// fluxes in FP32 (5.85e-5 in FP64, no gain).
+#ifndef VMI_COMP_SUMS
+#define VMI_COMP_SUMS 2
 #endif
 template <typename T, typename I, bool CP>
 inline constexpr bool kSubstepIncrement = VMI_FP32_INCREMENT != 0 && sizeof(T) == 4 && sizeof(I) == 4 && !CP;
@@ -696,20 +717,26 @@ template <typename T, typename I, typename O, int NW, int MAXCH, int GM, bool CP
 __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I, O> &sa, int bx, int j) {
   static_assert(!CP || (sizeof(T) == 4 && sizeof(I) == 4), "compensated storage is FP32");
   static_assert(!INC || (sizeof(T) == 4 && sizeof(I) == 4), "the increment form is an FP32 form");
-  // IN: updates formed as increments of the old state (rho', pi' and w each
-  // rounded once at full magnitude); RM: bf16 remainders and FP64 mass-flux
-  // sums on top of it (the compensated mode)
-  constexpr bool IN = CP || INC;
+  // RM: the compensated mode (bf16 remainders of the fields VMI_CP_MASK
+  // names and FP64 mass-flux sums); IN: pi' and w updated as increments of
+  // the old state (each rounded once at full magnitude), which the
+  // compensated mode needs when pi' or w carries a remainder
   constexpr bool RM = CP;
-  // F64S: the compensated mode's mass-flux sums in FP64 from FP64 geometry
-  // (compile-time switch VMI_COMP_FP32_SUMS: the FP32 forms of the plain mode)
-  constexpr bool F64S = CP && VMI_COMP_FP32_SUMS == 0;
+  constexpr bool IN = INC || (CP && (VMI_CP_MASK & 12) != 0);
+  // F64H / F64M: the compensated mode's horizontal mass-flux divergence /
+  // slope and vertical mass fluxes in FP64 from FP64 geometry (compile-time
+  // switch VMI_COMP_SUMS; otherwise the FP32 forms of the plain mode)
+  constexpr bool F64H = CP && (VMI_COMP_SUMS & 1) == 0;
+  constexpr bool F64M = CP && (VMI_COMP_SUMS & 2) == 0;
   // RX (FP32 state): rho' is not carried through the substeps.  Every
   // substep changes pi' by kappa times its change of rho' (Q - pi' =
   // kappa (E - rho') and the closing adds vd to rho', kappa vd to pi'), so
   // the stage writes chi = rho' - pi'/kappa and only the stage's last
   // substep (rho_o != nullptr) forms rho' = chi + pi'/kappa.
   constexpr bool RX = sizeof(I) == 4;
+  // RM: which fields keep their remainder through the substep
+  constexpr bool LU = RM && (VMI_CP_MASK & 1), LV = RM && (VMI_CP_MASK & 2), LW = RM && (VMI_CP_MASK & 4),
+                 LP = RM && (VMI_CP_MASK & 8);
   const I *__restrict__ u = sa.u;
   const I *__restrict__ v = sa.v;
   const I *__restrict__ w = sa.w;
@@ -729,7 +756,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   T *s_be = s_td + nlds;                      // [nz-1][64] beta
   T *s_cp = s_be + nlds;                      // [nz-1][64] c'
   // RM: the slope and vertical mass fluxes are formed in FP64
-  using TF = std::conditional_t<F64S, double, T>;
+  using TF = std::conditional_t<F64M, double, T>;
   __shared__ TF s_mc[NW][64], s_fh[NW][64];
   __shared__ T s_ph[NW][64];
   const int lane = threadIdx.x & 63;
@@ -792,11 +819,11 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
       if constexpr (!RX) x[5] = T(gld(rho, vC + s));
       if constexpr (RM) {
         const uint32_t sl = static_cast<uint32_t>(k) * lsL;
-        x[6] = bf16_val(gld(sa.ul, lC + sl));
-        x[7] = bf16_val(gld(sa.vl, lC + sl));
-        x[8] = bf16_val(gld(sa.ul, lE + sl));
-        x[9] = bf16_val(gld(sa.vl, lN + sl));
-        x[10] = bf16_val(gld(sa.wl, lC + sl));
+        x[6] = LU ? bf16_val(gld(sa.ul, lC + sl)) : T(0);
+        x[7] = LV ? bf16_val(gld(sa.vl, lC + sl)) : T(0);
+        x[8] = LU ? bf16_val(gld(sa.ul, lE + sl)) : T(0);
+        x[9] = LV ? bf16_val(gld(sa.vl, lN + sl)) : T(0);
+        x[10] = LW ? bf16_val(gld(sa.wl, lC + sl)) : T(0);
       }
     };
     // face and interface products of a level: cp theta0 and D rho0 at the
@@ -811,21 +838,25 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
       q[GUE] = colE ? gldc(g.cthu, tE + s) : q[GUW];
       q[GVS] = gldc(g.cthv, tC + s);
       q[GVN] = colN ? gldc(g.cthv, tN + s) : q[GVS];
-      if constexpr (F64S) {
+      if constexpr (sizeof(T) == 4) {
         const uint32_t s8 = static_cast<uint32_t>(k) * lsT8;
-        qd[0] = gldc(sa.rhu64, tC8 + s8);
-        qd[1] = colE ? gldc(sa.rhu64, tE8 + s8) : qd[0];
-        qd[2] = gldc(sa.rhv64, tC8 + s8);
-        qd[3] = colN ? gldc(sa.rhv64, tN8 + s8) : qd[2];
-        qd[4] = gldc(sa.r0w64, tC8 + s8);
-      } else if constexpr (sizeof(T) == 4) {
-        // FP32: the east / north face products and their differences across
-        // the cell (zero where the geometry is uniform in that direction)
-        q[HUE] = gldc(g.rhu, (colE ? tE : tC) + s);
-        q[HUW] = colE ? gldc(g.drhu, tC + s) : T(0);
-        q[HVN] = gldc(g.rhv, (colN ? tN : tC) + s);
-        q[HVS] = colN ? gldc(g.drhv, tC + s) : T(0);
-        q[R0B] = gldc(g.r0w, tC + s);
+        if constexpr (F64H) {
+          qd[0] = gldc(sa.rhu64, tC8 + s8);
+          qd[1] = colE ? gldc(sa.rhu64, tE8 + s8) : qd[0];
+          qd[2] = gldc(sa.rhv64, tC8 + s8);
+          qd[3] = colN ? gldc(sa.rhv64, tN8 + s8) : qd[2];
+        } else {
+          // FP32: the east / north face products and their differences across
+          // the cell (zero where the geometry is uniform in that direction)
+          q[HUE] = gldc(g.rhu, (colE ? tE : tC) + s);
+          q[HUW] = colE ? gldc(g.drhu, tC + s) : T(0);
+          q[HVN] = gldc(g.rhv, (colN ? tN : tC) + s);
+          q[HVS] = colN ? gldc(g.drhv, tC + s) : T(0);
+        }
+        if constexpr (F64M)
+          qd[4] = gldc(sa.r0w64, tC8 + s8);
+        else
+          q[R0B] = gldc(g.r0w, tC + s);
       } else {
         q[HUW] = gldc(g.rhu, tC + s);
         q[HUE] = colE ? gldc(g.rhu, tE + s) : q[HUW];
@@ -935,40 +966,57 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
             gst(vo, oC + so, static_cast<O>(vn_s));
             if constexpr (RM) {
               const uint32_t sl = static_cast<uint32_t>(k) * lsL;
-              gst(sa.ulo, lC + sl, bf16_bits(cu.lo));
-              gst(sa.vlo, lC + sl, bf16_bits(cv.lo));
+              if constexpr (LU) gst(sa.ulo, lC + sl, bf16_bits(cu.lo));
+              if constexpr (LV) gst(sa.vlo, lC + sl, bf16_bits(cv.lo));
             }
           }
         }
         TF mc = TF(0);
         T hd;
-        if constexpr (F64S) {
-          // full new winds hi + lo (+ background) in FP64; the face mass
-          // fluxes D rho0 u (about 1e5) are differenced across the cell in
-          // FP64 from FP64 geometry, as is the slope flux
+        // F64H, F64M: full new winds hi + lo (+ background) in FP64; the face
+        // mass fluxes D rho0 u (about 1e5) are differenced across the cell in
+        // FP64 from FP64 geometry (F64H), the slope flux formed in FP64 (F64M)
+        double uwd = 0.0, ued = 0.0, vsd = 0.0, vnd = 0.0;
+        if constexpr (F64H || F64M) {
           const double ub = C(g.ubg64)[k];
-          const double uwd = ub + static_cast<double>(cu.hi) + static_cast<double>(cu.lo);
-          const double ued = ub + static_cast<double>(xs[1]) + static_cast<double>(xs[8]) +
-                             static_cast<double>(-(cpdt * (gs[GUE] * px_e)));
-          const double vsd = static_cast<double>(cv.hi) + static_cast<double>(cv.lo);
-          const double vnd = static_cast<double>(xs[3]) + static_cast<double>(xs[9]) +
-                             static_cast<double>(-(cpdt * (gs[GVN] * py_n)));
+          uwd = ub + static_cast<double>(cu.hi) + static_cast<double>(cu.lo);
+          ued = ub + static_cast<double>(xs[1]) + static_cast<double>(xs[8]) +
+                static_cast<double>(-(cpdt * (gs[GUE] * px_e)));
+          vsd = static_cast<double>(cv.hi) + static_cast<double>(cv.lo);
+          vnd = static_cast<double>(xs[3]) + static_cast<double>(xs[9]) +
+                static_cast<double>(-(cpdt * (gs[GVN] * py_n)));
+        }
+        if constexpr (F64M) {
           if constexpr (yslope)
             mc = 0.5 * (uwd * C(sa.sx64)[gC] + ued * C(sa.sx64)[gE]) + 0.5 * (vsd * C(sa.sy64)[gC] + vnd * C(sa.sy64)[gN]);
           else if constexpr (xslope)
             mc = 0.5 * (uwd * C(sa.sx64)[gC] + ued * C(sa.sx64)[gE]);
-          const double hdd = (gd[1] * ued - gd[0] * uwd) * sa.rdx64 + (gd[3] * vnd - gd[2] * vsd) * sa.rdy64;
-          hd = static_cast<T>(hdd);
         } else {
           if constexpr (yslope)
             mc = T(0.5) * (un_w * sx_w + un_e * sx_e) + T(0.5) * (vn_s * sy_s + vn_n * sy_n);
           else if constexpr (xslope)
             mc = T(0.5) * (un_w * sx_w + un_e * sx_e);
+        }
+        if constexpr (F64H) {
+          const double hdd = (gd[1] * ued - gd[0] * uwd) * sa.rdx64 + (gd[3] * vnd - gd[2] * vsd) * sa.rdy64;
+          hd = static_cast<T>(hdd);
+        } else {
           if constexpr (sizeof(T) == 4) {
             // D rho0 u differenced across the cell without forming the large
             // products: rhu_e (u_e - u_w) + (rhu_e - rhu_w) u_w, where u_e - u_w
-            // comes from the stored deviations (the background cancels)
-            hd = (gs[HUE] * (du_e - du_w) + gs[HUW] * un_w) * g.rdx + (gs[HVN] * (vn_n - vn_s) + gs[HVS] * vn_s) * g.rdy;
+            // comes from the stored deviations (the background cancels) and,
+            // with u remainders, from their difference as well
+            T dux = du_e - du_w, dvy = vn_n - vn_s;
+            if constexpr (RM) {
+              // RM: the new winds of both faces exactly as hi + lo (the west
+              // and south ones are cu, cv), so the differences carry no
+              // rounding of the face values
+              const F2 ce = comp_add(xs[1], xs[8], -(cpdt * (gs[GUE] * px_e)));
+              const F2 cn = comp_add(xs[3], xs[9], -(cpdt * (gs[GVN] * py_n)));
+              dux = (ce.hi - cu.hi) + (ce.lo - cu.lo);
+              dvy = (cn.hi - cv.hi) + (cn.lo - cv.lo);
+            }
+            hd = (gs[HUE] * dux + gs[HUW] * un_w) * g.rdx + (gs[HVN] * dvy + gs[HVS] * vn_s) * g.rdy;
           } else {
             hd = (gs[HUE] * un_e - gs[HUW] * un_w) * g.rdx + (gs[HVN] * vn_n - gs[HVS] * vn_s) * g.rdy;
           }
@@ -982,18 +1030,17 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         if (k >= 1) {
           s_td[(k - 1) * 64 + lane] = IN ? -(gs[GKB] * ao * (pc[0] - pmC)) : wk - gs[GKB] * ao * (pc[0] - pmC);
           TF awk, r0k;
-          if constexpr (F64S) {  // the full old-w flux
-            awk = static_cast<double>(wk) + static_cast<double>(xs[10]);
+          // IN: the full old-w flux; otherwise its ao part
+          if constexpr (F64M) {
+            const double wd = static_cast<double>(wk) + static_cast<double>(xs[10]);
+            awk = IN ? wd : static_cast<double>(ao) * wd;
             r0k = gd[4];
-          } else if constexpr (IN) {
-            awk = wk;
-            r0k = gs[R0B];
           } else {
-            awk = ao * wk;
+            awk = IN ? wk : ao * wk;
             r0k = gs[R0B];
           }
           if (c > 0) {
-            const TF zk = F64S ? TF(C(sa.zeta64)[k]) : TF(C(g.zeta)[k]);
+            const TF zk = F64M ? TF(C(sa.zeta64)[k]) : TF(C(g.zeta)[k]);
             const TF Mw = (TF(1) - zk) * (TF(0.5) * (mc_prev + mc));
             Fh[c] = r0k * (awk - Mw);
           } else {
@@ -1017,7 +1064,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   // scaled by 1/pivot for the factored sweep.
   if (nlev > 0) {
     if (k0 >= 1) {
-      const TF zk = F64S ? TF(C(sa.zeta64)[k0]) : TF(C(g.zeta)[k0]);
+      const TF zk = F64M ? TF(C(sa.zeta64)[k0]) : TF(C(g.zeta)[k0]);
       const TF Mw = (TF(1) - zk) * (TF(0.5) * (s_mc[wv - 1][lane] + mc_first));
       Fh[0] = r0w_first * (aw_first - Mw);
     }
@@ -1172,16 +1219,16 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         if constexpr (!RX) t[2] = T(gld(rho, vC + si));
         t[4] = T(gld(pi, vC + si));
         t[6] = T(gld(w, vC + si));
-        if constexpr (F64S) t8[0] = gldc(sa.r0w64, tC8 + static_cast<uint32_t>(k + 1) * lsT8);
         if constexpr (RM) {
           const uint32_t sl = static_cast<uint32_t>(k) * lsL;
           if constexpr (!RX) t[3] = bf16_val(gld(sa.rl, lC + sl));
-          t[5] = bf16_val(gld(sa.pl, lC + sl));
-          t[7] = bf16_val(gld(sa.wl, lC + sl));
+          t[5] = LP ? bf16_val(gld(sa.pl, lC + sl)) : T(0);
+          t[7] = LW ? bf16_val(gld(sa.wl, lC + sl)) : T(0);
         } else {
           t[3] = t[5] = t[7] = T(0);
         }
       }
+      if constexpr (F64M) t8[0] = gldc(sa.r0w64, tC8 + static_cast<uint32_t>(k + 1) * lsT8);
       if constexpr (RX && RO) {  // chi of the level (and its remainder)
         t[2] = T(gld(sa.chi, vC + static_cast<uint32_t>(k) * lsI));
         t[3] = RM ? bf16_val(gld(sa.chil, lC + static_cast<uint32_t>(k) * lsL)) : T(0);
@@ -1195,7 +1242,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
     // flux through the chunk's lower interface
     TF flo = TF(0);
     if (k0 >= 1) {
-      if constexpr (F64S)
+      if constexpr (F64M)
         flo = Fh[0] + (static_cast<double>(an) * gldc(sa.r0w64, tC8 + static_cast<uint32_t>(k0) * lsT8)) *
                           static_cast<double>(s_td[(k0 - 1) * 64 + lane]);
       else
@@ -1211,7 +1258,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         const TF fhn = (c + 1 < nlev) ? Fh[(c + 1) < MAXCH ? c + 1 : c] : fh_next_chunk;
         TF fhi = TF(0);
         if (k + 1 < nz) {
-          if constexpr (F64S)
+          if constexpr (F64M)
             fhi = fhn + (static_cast<double>(an) * DD8[c % 3][0]) * static_cast<double>(s_td[k * 64 + lane]);
           else
             fhi = fhn + (an * dd[0]) * s_td[k * 64 + lane];
@@ -1220,7 +1267,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         F2 cr{}, cpn{}, cw{};
         // IN: increments of rho', pi' and w, each added to the old value once
         const T dr = RX ? T(0) : E[c] + vd, dp = Q[c] + dd[1] * vd;
-        if constexpr (RM) {
+        if constexpr (RM && IN) {
           if constexpr (!RX) cr = comp_add(dd[2], dd[3], E[c] + vd);
           cpn = comp_add(dd[4], dd[5], Q[c] + dd[1] * vd);
           if (k >= 1) {
@@ -1231,7 +1278,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
           }
         }
         T rn = T(0), pn, wn;
-        if constexpr (RM) {
+        if constexpr (RM && IN) {
           rn = cr.hi, pn = cpn.hi, wn = cw.hi;
         } else if constexpr (IN) {
           rn = dd[2] + dr;
@@ -1246,7 +1293,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         double rx = 0.0;
         if constexpr (RX && RO) {
           double pv = static_cast<double>(pn);
-          if constexpr (RM)
+          if constexpr (RM && IN)
             pv += static_cast<double>(cpn.lo);
           else if constexpr (IN)
             pv = static_cast<double>(dd[4]) + static_cast<double>(dp);
@@ -1255,7 +1302,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         if (active) {
           if constexpr (sizeof(O) == 8) {
             double rf = static_cast<double>(rn), pf = static_cast<double>(pn), wf = static_cast<double>(wn);
-            if constexpr (RM) {
+            if constexpr (RM && IN) {
               rf += static_cast<double>(cr.lo);
               pf += static_cast<double>(cpn.lo);
               wf += static_cast<double>(cw.lo);
@@ -1281,8 +1328,8 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
             gst(pi_o, oC + so, static_cast<O>(pn));
             gst(wo, oC + so, static_cast<O>(wn));
             if constexpr (RM) {
-              gst(sa.plo, lC + sl, bf16_bits(cpn.lo));
-              gst(sa.wlo, lC + sl, bf16_bits(cw.lo));
+              if constexpr (LP) gst(sa.plo, lC + sl, bf16_bits(cpn.lo));
+              if constexpr (LW) gst(sa.wlo, lC + sl, bf16_bits(cw.lo));
             }
           }
         }
@@ -1300,14 +1347,14 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
       double wt = static_cast<double>(T(gld(w, vC + s * lsI)));
       uint16_t wtl = 0;
       if constexpr (RM) {
-        wtl = gld(sa.wl, lC + s * lsL);
+        if constexpr (LW) wtl = gld(sa.wl, lC + s * lsL);
         wt += static_cast<double>(bf16_val(wtl));
       }
       if constexpr (sizeof(O) == 8) {
         gst(wo, oC + s * lsO, sponge_scale(sa.sp, nz, nz, wt));
       } else {
         gst(wo, oC + s * lsO, static_cast<O>(T(gld(w, vC + s * lsI))));
-        if constexpr (RM) gst(sa.wlo, lC + s * lsL, wtl);
+        if constexpr (LW) gst(sa.wlo, lC + s * lsL, wtl);
       }
     }
     if (check && bad && active) atomicOr(flag, 1);
