@@ -27,7 +27,12 @@ bytes per launch: read u, v, w, rho', pi' and write them once, 4 B each in
 the FP32-mixed path (40 B per cell), except the step's last substep, which
 writes the FP64 state (20 + 40 = 60 B per cell; the intermediate stage
 states stay FP32); averaged over the 3 n_s launches of a step:
-(40 (3 n_s - 1) + 60) / (3 n_s) B per cell.  The launch duration is measured
+(40 (3 n_s - 1) + 60) / (3 n_s) B per cell.  This implementation moves less
+than that model: its FP32 substeps carry chi = rho' - pi'/kappa, which no
+substep but a stage's last reads, in place of rho' (32 B per cell on the
+first n_s - 1 substeps of a stage); "bytes_moved_per_cell_update" and the
+roofline's "moved_bytes_per_launch" give what it moves, while "achieved"
+and "hbm_frac_step" keep the model's bytes.  The launch duration is measured
 with HIP events on the batch stream around each stage's substep group (only
 kb_substep launches are in the group), summed over the step.
 "traffic" (HBM bytes per launch) comes from the rocprofv3 FETCH_SIZE /
@@ -56,9 +61,32 @@ def log(*a):
 
 
 def bytes_per_cell_update(nsub: int, mixed: bool, compensated: bool = False) -> int:
-    """FP64 stage traffic (48 values) plus the substeps' 30 nsub values per
-    cell, 4 B in FP32, 8 B in FP64, 6 B with a bf16 remainder."""
-    return 8 * 48 + (6 if compensated else 4 if mixed else 8) * 30 * nsub
+    """Compulsory bytes per cell-update of the reference's step structure:
+    FP64 stage traffic (48 values) plus the substeps' 30 nsub values per cell
+    (u, v, w, rho', pi' read and written), 4 B in FP32, 8 B in FP64; the
+    compensated mode adds u's 1-byte remainder to its 6 nsub u accesses."""
+    return 8 * 48 + (4 if mixed else 8) * 30 * nsub + (6 * nsub if compensated else 0)
+
+
+def bytes_moved_per_cell_update(nsub: int, compensated: bool = False) -> int:
+    """What this implementation's FP32 step moves per cell-update under the
+    same model: the substeps carry chi = rho' - pi'/kappa in place of rho' and
+    leave it untouched, so the first nsub-1 substeps of a stage read and write
+    u, v, w, pi' only (8 values); a stage's last substep also reads chi and
+    writes rho' (10 values).  Compensated: plus u's 1-byte remainder read and
+    written by every substep."""
+    per_stage = 4 * (8 * (nsub - 1) + 10) + (2 * nsub if compensated else 0)
+    return 8 * 48 + 3 * per_stage
+
+
+def substep_moved_bytes_per_launch(cells: int, nsub: int) -> float:
+    """Bytes this implementation's kb_substep moves per launch (FP32-mixed
+    batch), averaged over the 3 nsub launches of a step: 32 B per cell for
+    the first nsub-1 substeps of a stage (u, v, w, pi' read and written),
+    40 B for the last of stages 1 and 2 (chi read, rho' written) and 60 B
+    for the step's last (the FP64 state written)."""
+    n = 3 * nsub
+    return cells * (32.0 * 3 * (nsub - 1) + 40.0 * 2 + 60.0) / n
 
 
 def substep_bytes_per_launch(cells: int, nsub: int) -> float:
@@ -341,6 +369,7 @@ def main():
         comp = {"value": cups_c, "ms_per_step": rc_["ms"] / args.steps, "tiles": rc_["tuning"],
                 "speedup_vs_fp64": cups_c / res64["value"],
                 "bytes_per_cell_update": bytes_per_cell_update(nsub, True, True),
+                "bytes_moved_per_cell_update": bytes_moved_per_cell_update(nsub, True),
                 "hbm_frac_step": cups_c * bytes_per_cell_update(nsub, True, True) / HBM_PEAK}
         log(f"[bench] compensated: {rc_['ms'] / args.steps:.2f} ms/step, {cups_c:.3e} cell-updates/s, "
             f"{cups_c / res64['value']:.3f}x FP64")
@@ -361,6 +390,7 @@ def main():
                 "frac": achieved * 1e9 / HBM_PEAK, "traffic": traffic,
                 "kernel": "kb_substep (fused HEVI acoustic substep, FP32, batched persistent)",
                 "algorithmic_bytes_per_launch": sub_bytes, "avg_launch_ms": sub_ms,
+                "moved_bytes_per_launch": substep_moved_bytes_per_launch(cells, nsub),
                 "traffic_source": traffic_src}
 
     cpu = None
@@ -393,12 +423,14 @@ def main():
         "config": {"workload": wl, "cells": cells, "members": len(cases), "parallelism": "single GPU"},
         "hbm_frac_step": cups_m * bytes_per_cell_update(nsub, True) / HBM_PEAK,
         "bytes_per_cell_update": bytes_per_cell_update(nsub, True),
+        "bytes_moved_per_cell_update": bytes_moved_per_cell_update(nsub),
         "tiles": rm["tuning"],
         "fp64": res64,
         "speedup_mixed_vs_fp64": (cups_m / res64["value"]) if res64 else None,
         "unbatched_mixed": alone,
-        # VIBE_MI_MIXED_COMP (batched, hipGraph): FP32 compute with compensated
-        # substep state and FP64 mass-flux sums, the mode that holds 1e-4 of FP64
+        # VIBE_MI_MIXED_COMP (batched, hipGraph): FP32 compute with u carrying a
+        # remainder and the horizontal mass-flux divergence in FP64, the mode
+        # that holds 1e-4 of FP64 after 1 h
         "compensated": comp,
         "kernel_ms": {"stage_group_per_stage": kt_m["ms_stage"] / max(kt_m["n_stage"], 1),
                       "substep_per_launch": sub_ms},

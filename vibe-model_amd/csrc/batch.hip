@@ -237,7 +237,7 @@ template <typename T, typename O, int NW, int GM, bool CP>
 int launch_substep_nw(vibe_mi_batch *b, const WorkList &w, int st, int q, int abase, int last, int *ctr, bool dry) {
   const auto *D = static_cast<const BDom<T> *>(b->desc);
   const int ch = (b->nzmax + NW - 1) / NW;
-  const size_t lds = 3 * static_cast<size_t>(b->nzmax - 1) * 64 * sizeof(T);
+  const size_t lds = substep_lds_arrays(CP) * static_cast<size_t>(b->nzmax - 1) * 64 * sizeof(T);
   // s_mc, s_fh (FP64 in the compensated mode), s_ph and the tile loop's slots
   const size_t lds_static = ((CP ? 2 * sizeof(double) : 2 * sizeof(T)) + sizeof(T)) * NW * 64 + 64;
   if (lds + lds_static > 160 * 1024) return VIBE_C_ERR_BAD_ARG;

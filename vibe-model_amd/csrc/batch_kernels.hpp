@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(64 * TY) kb_stage(const BDom<T> *__restrict__ 
     StageOut<T> so{a.u, a.v, a.w, a.rho, a.pi, nxt.th};
     if constexpr (sizeof(T) == 4) {  // remainders, compensated mode only (else null)
       const AcousticLo l = dd.L[aout];
-      so.ul = l.u, so.rl = l.rho;
+      so.ul = l.u;
       so.vl = cp_keeps(2) ? l.v : nullptr, so.wl = cp_keeps(4) ? l.w : nullptr, so.pl = cp_keeps(8) ? l.pi : nullptr;
     }
     SpongeArgs sp;  // the last stage's theta' is the step's final one
@@ -184,12 +184,11 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kS
       sa.check = last;
     }
     if constexpr (CP) {
-      sa.chil = dd.L[abase & 1].rho;
       const AcousticLo li = dd.L[(abase + q) & 1];
-      sa.ul = li.u, sa.vl = li.v, sa.wl = li.w, sa.rl = li.rho, sa.pl = li.pi;
+      sa.ul = li.u, sa.vl = li.v, sa.wl = li.w, sa.pl = li.pi;
       if constexpr (sizeof(O) == 4) {
         const AcousticLo lo = dd.L[(abase + q + 1) & 1];
-        sa.ulo = lo.u, sa.vlo = lo.v, sa.wlo = lo.w, sa.rlo = lo.rho, sa.plo = lo.pi;
+        sa.ulo = lo.u, sa.vlo = lo.v, sa.wlo = lo.w, sa.plo = lo.pi;
       }
       sa.rhu64 = dd.rhu64, sa.rhv64 = dd.rhv64, sa.r0w64 = dd.r0w64, sa.sx64 = dd.sx64, sa.sy64 = dd.sy64;
       sa.zeta64 = dd.zeta64, sa.rdx64 = dd.rdx64, sa.rdy64 = dd.rdy64;

@@ -22,8 +22,11 @@ struct AcousticSet {
   T *u = nullptr, *v = nullptr, *w = nullptr, *rho = nullptr, *pi = nullptr;
 };
 // bf16 remainders of an FP32 acoustic set (compensated FP32-mixed mode)
+// Remainders of an FP32 acoustic set (compensated mode): 8-bit for u, bf16
+// for v, w and pi'; rho' (carried as chi) has none.
 struct AcousticLo {
-  uint16_t *u = nullptr, *v = nullptr, *w = nullptr, *rho = nullptr, *pi = nullptr;
+  int8_t *u = nullptr;
+  uint16_t *v = nullptr, *w = nullptr, *pi = nullptr;
 };
 
 

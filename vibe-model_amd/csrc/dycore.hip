@@ -195,7 +195,7 @@ int run_stage(vibe_mi_domain *d, const Geo<T> &g, const FieldSet &cur, const Fie
   StageOut<T> so{a.u, a.v, a.w, a.rho, a.pi, nxt.th};
   if (sizeof(T) == 4 && d->precision == VIBE_MI_MIXED_COMP) {
     const AcousticLo &l = d->L32[0];
-    so.ul = l.u, so.rl = l.rho;
+    so.ul = l.u;
     so.vl = cp_keeps(2) ? l.v : nullptr, so.wl = cp_keeps(4) ? l.w : nullptr, so.pl = cp_keeps(8) ? l.pi : nullptr;
   }
   const dim3 grid(ceil_div(d->nx, 64), ceil_div(d->ny, 4)), block(64, 4);
@@ -275,7 +275,7 @@ constexpr size_t kMaxLds = 160 * 1024;  // LDS per CU (and per workgroup) on gfx
 
 template <typename T, typename I, typename O, int NW, int MAXCH, bool CP>
 int launch_substep_k(vibe_mi_domain *d, const Geo<T> &g, const SubArgs<T, I, O> &sa) {
-  const size_t lds = 3 * static_cast<size_t>(d->nz - 1) * 64 * sizeof(T);
+  const size_t lds = substep_lds_arrays(CP) * static_cast<size_t>(d->nz - 1) * 64 * sizeof(T);
   // s_mc, s_fh (FP64 in the compensated mode) and s_ph
   const size_t lds_static = ((CP ? 2 * sizeof(double) : 2 * sizeof(T)) + sizeof(T)) * NW * 64;
   if (lds + lds_static > kMaxLds) return VIBE_C_ERR_BAD_ARG;  // column too tall for one workgroup
@@ -300,14 +300,13 @@ int launch_substep(vibe_mi_domain *d, const Geo<T> &g, const T *tab, const I *u,
                       uo, vo, wo, rho_o, pi_o, check, d->flag};
   if constexpr (sizeof(I) == 4) {
     // FP32 substeps carry chi = rho' - pi'/kappa, which the stage writes to
-    // A[0] (and its remainder to L32[0]); the last substep forms rho'
+    // A[0]; the last substep forms rho'
     sa.chi = d->A32[0].rho;
-    if (li != nullptr) sa.chil = d->L32[0].rho;
   }
   if constexpr (sizeof(T) == 4 && sizeof(I) == 4) {
     if (li != nullptr) {
-      sa.ul = li->u, sa.vl = li->v, sa.wl = li->w, sa.rl = li->rho, sa.pl = li->pi;
-      if (lo != nullptr) sa.ulo = lo->u, sa.vlo = lo->v, sa.wlo = lo->w, sa.rlo = lo->rho, sa.plo = lo->pi;
+      sa.ul = li->u, sa.vl = li->v, sa.wl = li->w, sa.pl = li->pi;
+      if (lo != nullptr) sa.ulo = lo->u, sa.vlo = lo->v, sa.wlo = lo->w, sa.plo = lo->pi;
       const Geo<double> &g8 = d->g64;
       sa.rhu64 = g8.rhu, sa.rhv64 = g8.rhv, sa.r0w64 = g8.r0w, sa.sx64 = g8.sx, sa.sy64 = g8.sy, sa.zeta64 = g8.zeta;
       sa.rdx64 = g8.rdx, sa.rdy64 = g8.rdy;
@@ -604,9 +603,9 @@ int create_impl(const vibe_mi_domain_desc *ds, vibe_mi_domain *d) {
         if (int rc = dalloc(d, &l.u, d->ncell)) return rc;
         if (int rc = dalloc(d, &l.v, d->ncell)) return rc;
         if (int rc = dalloc(d, &l.w, d->nwf)) return rc;
-        if (int rc = dalloc(d, &l.rho, d->ncell)) return rc;
         if (int rc = dalloc(d, &l.pi, d->ncell)) return rc;
-        for (uint16_t *q : {l.u, l.v, l.rho, l.pi}) VMI_HIP(hipMemset(q, 0, d->ncell * 2));
+        VMI_HIP(hipMemset(l.u, 0, d->ncell));
+        for (uint16_t *q : {l.v, l.pi}) VMI_HIP(hipMemset(q, 0, d->ncell * 2));
         VMI_HIP(hipMemset(l.w, 0, d->nwf * 2));
       }
   } else {

@@ -85,9 +85,10 @@ template <typename O>
 struct StageOut {
   O *u, *v, *w, *rho, *pi;
   double *th;
-  // compensated FP32 mode (FP32 sets only): bf16 remainders of u, v, w, rho',
-  // pi' (nullptr: plain FP32 or FP64 output)
-  uint16_t *ul = nullptr, *vl = nullptr, *wl = nullptr, *rl = nullptr, *pl = nullptr;
+  // compensated FP32 mode (FP32 sets only): remainders of u (8-bit) and v, w,
+  // pi' (bf16) (nullptr: plain FP32 or FP64 output); rho' (chi) has none
+  int8_t *ul = nullptr;
+  uint16_t *vl = nullptr, *wl = nullptr, *pl = nullptr;
 };
 
 // ---- compensated FP32 storage -----------------------------------------------
@@ -114,6 +115,18 @@ __device__ __forceinline__ F2 comp_add(float hi, float lo, float d) {
   const float h2 = s + e;
   return F2{h2, e - (h2 - s)};
 }
+// u keeps an 8-bit remainder instead: with hi = m 2^e (0.5 <= |m| < 1), the
+// remainder lo of a round-to-nearest hi satisfies |lo| <= 2^(e-25) and is
+// stored as q = rint(lo 2^(32-e)), |q| <= 128 (clamped to 127), so hi + lo
+// keeps about 31 significant bits in 5 bytes.
+__device__ __forceinline__ int8_t r8_bits(float hi, float lo) {
+  const int e = __builtin_amdgcn_frexp_expf(hi);
+  const float q = __builtin_rintf(__builtin_ldexpf(lo, 32 - e));
+  return static_cast<int8_t>(fminf(fmaxf(q, -127.0f), 127.0f));
+}
+__device__ __forceinline__ float r8_val(float hi, int q) {
+  return __builtin_ldexpf(static_cast<float>(q), __builtin_amdgcn_frexp_expf(hi) - 32);
+}
 // Writes x to an acoustic-set array of type O at byte offset off; an FP32
 // array with a remainder array also gets the bf16 remainder x - hi.
 template <typename O>
@@ -122,6 +135,17 @@ __device__ __forceinline__ void put_acoustic(O *p, uint16_t *lo, uint32_t off, d
     const float hi = static_cast<float>(x);
     gst(p, off, hi);
     if (lo != nullptr) gst(lo, off >> 1, bf16_bits(static_cast<float>(x - static_cast<double>(hi))));
+  } else {
+    gst(p, off, static_cast<O>(x));
+  }
+}
+// the same with an 8-bit remainder (u)
+template <typename O>
+__device__ __forceinline__ void put_acoustic(O *p, int8_t *lo, uint32_t off, double x) {
+  if constexpr (sizeof(O) == 4) {
+    const float hi = static_cast<float>(x);
+    gst(p, off, hi);
+    if (lo != nullptr) gst(lo, off >> 2, r8_bits(hi, static_cast<float>(x - static_cast<double>(hi))));
   } else {
     gst(p, off, static_cast<O>(x));
   }
@@ -508,9 +532,9 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageCur<CI> &
         // d rho' per cell), so chi is constant over a stage's substeps and
         // the stage's last substep recovers rho' = chi + pi'/kappa
         if constexpr (sizeof(O) == 4)
-          put_acoustic(out.rho, out.rl, ko, rn - pn * (1.0 / static_cast<double>(kapk)));
+          put_acoustic(out.rho, static_cast<uint16_t *>(nullptr), ko, rn - pn * (1.0 / static_cast<double>(kapk)));
         else
-          put_acoustic(out.rho, out.rl, ko, rn);
+          put_acoustic(out.rho, static_cast<uint16_t *>(nullptr), ko, rn);
         gst(out.th, cCb + static_cast<uint32_t>(k) * lsD, sponge_scale(sp, k, nz, tn));
         put_acoustic(out.pi, out.pl, ko, pn);
         bad |= !isfinite(tn);
@@ -634,18 +658,28 @@ struct SubArgs {
   SpongeArgs sp = {};              // FP64 outputs only: sponge of the step's final state
   // compensated FP32 mode: bf16 remainders of the state in and (FP32 outputs)
   // out
-  const uint16_t *ul = nullptr, *vl = nullptr, *wl = nullptr, *rl = nullptr, *pl = nullptr;
-  uint16_t *ulo = nullptr, *vlo = nullptr, *wlo = nullptr, *rlo = nullptr, *plo = nullptr;
-  // FP32 state: chi = rho' - pi'/kappa of the stage (and its remainder),
-  // read by the stage's last substep only (rho_o != nullptr)
+  const int8_t *ul = nullptr;
+  const uint16_t *vl = nullptr, *wl = nullptr, *pl = nullptr;
+  int8_t *ulo = nullptr;
+  uint16_t *vlo = nullptr, *wlo = nullptr, *plo = nullptr;
+  // FP32 state: chi = rho' - pi'/kappa of the stage, read by the stage's
+  // last substep only (rho_o != nullptr)
   const I *chi = nullptr;
-  const uint16_t *chil = nullptr;
   // compensated mode: FP64 geometry for the mass-flux sums
   const double *rhu64 = nullptr, *rhv64 = nullptr, *r0w64 = nullptr, *sx64 = nullptr, *sy64 = nullptr,
                *zeta64 = nullptr;
   double rdx64 = 0.0, rdy64 = 0.0;
 };
 
+// Thomas sweep operands: the compensated kernels' sweeping wave reads beta
+// and c' from the HEVI table in global memory (prefetched 32 rows ahead) and
+// only the RHS is staged in LDS, (nz-1)*64 values per workgroup instead of
+// three times that, so three 4-wave or two 8-wave workgroups fit per CU at
+// their register counts; the other kernels stage all three in LDS in phase
+// B (measured faster for them: their occupancy is bound by registers).
+__host__ __device__ constexpr bool substep_thomas_global(bool cp) { return cp; }
+// LDS arrays of (nz-1)*64 values each a substep workgroup stages
+__host__ __device__ constexpr int substep_lds_arrays(bool cp) { return substep_thomas_global(cp) ? 1 : 3; }
 // Compensated mode: the substep fields that carry a remainder (bits: 1 u,
 // 2 v, 4 w, 8 pi'); a field without one is rounded to FP32 at every
 // substep as in the plain mode.  Compile-time switch, u alone by default:
@@ -654,19 +688,19 @@ struct SubArgs {
 #ifndef VMI_CP_MASK
 #define VMI_CP_MASK 1
 #endif
-// Remainder arrays a compensated stage writes: u and chi (rho' - pi'/kappa,
-// read by the stage's last substep), plus the fields VMI_CP_MASK names.
+// Remainder arrays a compensated stage writes: u, plus the fields VMI_CP_MASK
+// names (chi = rho' - pi'/kappa, read once by the stage's last substep, is
+// stored in FP32 alone).
 __host__ __device__ constexpr bool cp_keeps(int bit) { return (VMI_CP_MASK & bit) != 0; }
-// Waves per SIMD the substep kernels are compiled for: plain FP32 shapes
-// with up to 12 levels per wave (about 110 VGPRs) and compensated ones with
-// up to 8 (about 127) are held to 128 VGPRs (four waves per SIMD); the other
-// shapes are register-limited by their own size (the compensated kernel at
-// 12 levels per wave takes 139 VGPRs, three waves per SIMD; held to four it
-// spills).  With remainders on pi' or w (increment forms) the compensated
-// kernels take about 175 VGPRs.
+// Waves per SIMD the substep kernels are compiled for: FP32 shapes with up
+// to 12 levels per wave (plain about 106 VGPRs, compensated about 133) are
+// held to 128 VGPRs (four waves per SIMD; the compensated kernel then spills
+// 3-4 registers); the other shapes are register-limited by their own size.
+// With remainders on pi' or w (increment forms) the compensated kernels take
+// about 175 VGPRs and are not held.
 template <typename T, int NW, int MAXCH, bool CP = false>
 inline constexpr int kSubstepWavesPerEu =
-    (sizeof(T) == 4 && ((!CP && MAXCH <= 12) || (CP && MAXCH <= 8 && (VMI_CP_MASK & 12) == 0))) ? 4 : 1;
+    (sizeof(T) == 4 && MAXCH <= 12 && (!CP || (VMI_CP_MASK & 12) == 0)) ? 4 : 1;
 
 // Increment form of the plain FP32 substep (compile-time switch
 // VMI_FP32_INCREMENT): rho', pi' and w updated as old value + increment.
@@ -753,8 +787,9 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   extern __shared__ unsigned char smem_raw[];
   const int nlds = (g.nz - 1) * 64;
   T *s_td = reinterpret_cast<T *>(smem_raw);  // [nz-1][64] old-w RHS part -> RHS -> d' -> w
-  T *s_be = s_td + nlds;                      // [nz-1][64] beta
-  T *s_cp = s_be + nlds;                      // [nz-1][64] c'
+  constexpr bool TG = substep_thomas_global(CP);
+  T *s_be = s_td + (TG ? 0 : nlds);           // [nz-1][64] beta (not with TG)
+  T *s_cp = s_be + (TG ? 0 : nlds);           // [nz-1][64] c' (not with TG)
   // RM: the slope and vertical mass fluxes are formed in FP64
   using TF = std::conditional_t<F64M, double, T>;
   __shared__ TF s_mc[NW][64], s_fh[NW][64];
@@ -784,6 +819,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   const uint32_t lsI = nxy * SI, lsO = nxy * SO, lsT = static_cast<uint32_t>(rk) * ST;
   const uint32_t lC = cC * 2u, lsL = nxy * 2u;  // bf16 remainders (CP)
   const uint32_t lE = (static_cast<uint32_t>(j) * unx + iE) * 2u, lN = (static_cast<uint32_t>(jN) * unx + i) * 2u;
+  const uint32_t l8C = cC, l8E = lE >> 1, lsL8 = nxy;  // 8-bit u remainders (CP)
   const uint32_t tC8 = gC * 8u, tE8 = gE * 8u, tN8 = gN * 8u, lsT8 = static_cast<uint32_t>(rk) * 8u;
   constexpr bool xslope = GM != kGeoFlat, yslope = GM == kGeoAny;  // nonzero slope / metric terms
   constexpr bool colE = GM != kGeoFlat, colN = GM == kGeoAny;     // east / north geometry column differs
@@ -809,7 +845,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
       p[3] = T(gld(pi, vS + s));
       p[4] = T(gld(pi, vN + s));
     };
-    auto ld_state = [&](int k, T *x) {
+    auto ld_state = [&](int k, T *x, int *qx) {
       const uint32_t s = static_cast<uint32_t>(k) * lsI;
       x[0] = T(gld(u, vC + s));
       x[1] = T(gld(u, vE + s));
@@ -819,9 +855,11 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
       if constexpr (!RX) x[5] = T(gld(rho, vC + s));
       if constexpr (RM) {
         const uint32_t sl = static_cast<uint32_t>(k) * lsL;
-        x[6] = LU ? bf16_val(gld(sa.ul, lC + sl)) : T(0);
+        // u: raw 8-bit remainders, decoded at use against their hi values
+        const uint32_t sl8 = static_cast<uint32_t>(k) * lsL8;
+        qx[0] = LU ? static_cast<int>(gld(sa.ul, l8C + sl8)) : 0;
+        qx[1] = LU ? static_cast<int>(gld(sa.ul, l8E + sl8)) : 0;
         x[7] = LV ? bf16_val(gld(sa.vl, lC + sl)) : T(0);
-        x[8] = LU ? bf16_val(gld(sa.ul, lE + sl)) : T(0);
         x[9] = LV ? bf16_val(gld(sa.vl, lN + sl)) : T(0);
         x[10] = LW ? bf16_val(gld(sa.wl, lC + sl)) : T(0);
       }
@@ -886,8 +924,9 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
     }
     ld5(min(k0 + 1, nz - 1), P[1]);
     T X[2][RM ? 11 : 6], Gq[2][NG];
+    int QX[2][2] = {{0, 0}, {0, 0}};
     double Gd[2][RM ? 5 : 1];
-    ld_state(k0, X[0]);
+    ld_state(k0, X[0], QX[0]);
     ld_geo(k0, Gq[0], Gd[0]);
     TF mc_prev = TF(0);
 #pragma unroll
@@ -899,14 +938,18 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
       // guard's branches join.
       {
         const int kn = k0 + min(c + 1, nlev - 1);
-        ld_state(kn, X[cs ^ 1]);
+        ld_state(kn, X[cs ^ 1], QX[cs ^ 1]);
         ld_geo(kn, Gq[cs ^ 1], Gd[cs ^ 1]);
         ld5(min(kn + 1, nz - 1), P[(c + 2) % 3]);
       }
       if (c < nlev) {
         const int k = k0 + c;
         const int p0 = c % 3, p1 = (c + 1) % 3;
-        const T *xs = X[cs];
+        T *xs = X[cs];
+        if constexpr (LU) {  // the u remainders of the own and east columns
+          xs[6] = r8_val(xs[0], QX[cs][0]);
+          xs[8] = r8_val(xs[1], QX[cs][1]);
+        }
         const T *gs = Gq[cs];
         const double *gd = Gd[cs];
         const T *pc = P[p0];
@@ -966,7 +1009,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
             gst(vo, oC + so, static_cast<O>(vn_s));
             if constexpr (RM) {
               const uint32_t sl = static_cast<uint32_t>(k) * lsL;
-              if constexpr (LU) gst(sa.ulo, lC + sl, bf16_bits(cu.lo));
+              if constexpr (LU) gst(sa.ulo, l8C + static_cast<uint32_t>(k) * lsL8, r8_bits(cu.hi, cu.lo));
               if constexpr (LV) gst(sa.vlo, lC + sl, bf16_bits(cv.lo));
             }
           }
@@ -1093,8 +1136,10 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
       const uint32_t sm = tC + (k >= 1 ? s - lsT : 0u);  // level 0: read, unused
       t[1] = gldc(sa.tab.GK, sm);
       t[2] = gldc(sa.tab.INV, sm);
-      t[3] = gldc(sa.tab.A, sm);
-      t[4] = gldc(sa.tab.CP, sm);
+      if constexpr (!TG) {
+        t[3] = gldc(sa.tab.A, sm);
+        t[4] = gldc(sa.tab.CP, sm);
+      }
       if constexpr (IN) t[5] = T(gld(pi, vC + static_cast<uint32_t>(k) * lsI));
     };
     const TF fh_next_chunk = (k0 + nlev < nz) ? s_fh[wv + 1][lane] : TF(0);
@@ -1115,8 +1160,10 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
           const int m = k - 1;
           const T td = s_td[m * 64 + lane] - tb[1] * an * (ph - php);
           s_td[m * 64 + lane] = td * tb[2];
-          s_be[m * 64 + lane] = tb[3];
-          s_cp[m * 64 + lane] = tb[4];
+          if constexpr (!TG) {
+            s_be[m * 64 + lane] = tb[3];
+            s_cp[m * 64 + lane] = tb[4];
+          }
         }
         php = ph;
       }
@@ -1130,7 +1177,94 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   // previous block runs, into two register sets that alternate, so only the
   // dependent multiply-subtract per row stays on the critical path and the
   // rows of a block carry no guards.
-  if (wv == 0) {
+  if (TG && wv == 0) {
+    // beta and c' from the table in global memory: D blocks of B rows in
+    // flight ahead of the block being swept (ring slots fixed by unrolling)
+    constexpr int B = 8, D = 4;
+    const int n = nz - 1;
+    const int r = (n - 1) % B, nbf = (n - 1) / B;
+    T *__restrict__ td = s_td + lane;
+    auto tabv = [&](const T *f, int m) { return gldc(f, tC + static_cast<uint32_t>(m) * lsT); };
+    T ring[D][B];
+    // forward elimination: d'_0 = RHS_0, d'_m = RHS_m - beta_m d'_{m-1}
+    T dp = td[0];
+    auto fetch_f = [&](int blk, T *o) {
+      const int m0 = r + 1 + min(blk, nbf - 1) * B;
+#pragma unroll
+      for (int q = 0; q < B; ++q) o[q] = tabv(sa.tab.A, m0 + q);
+    };
+    // the remainder rows 1..r first, then the first D-1 blocks (in that
+    // order: the loads retire in issue order)
+    T rem[B];
+#pragma unroll
+    for (int q = 0; q < B; ++q) rem[q] = tabv(sa.tab.A, min(1 + q, n - 1));
+    if (nbf > 0) {
+#pragma unroll
+      for (int sl = 0; sl < D - 1; ++sl) fetch_f(sl, ring[sl]);
+    }
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      if (q < r) {
+        dp = td[(1 + q) * 64] - rem[q] * dp;
+        td[(1 + q) * 64] = dp;
+      }
+    }
+    for (int blk = 0; blk < nbf; blk += D) {
+#pragma unroll
+      for (int sl = 0; sl < D; ++sl) {
+        if (blk + sl < nbf) {
+          fetch_f(blk + sl + D - 1, ring[(sl + D - 1) % D]);
+          const int m0 = r + 1 + (blk + sl) * B;
+          T v[B];
+#pragma unroll
+          for (int q = 0; q < B; ++q) v[q] = td[(m0 + q) * 64];
+#pragma unroll
+          for (int q = 0; q < B; ++q) {
+            dp = v[q] - ring[sl][q] * dp;
+            td[(m0 + q) * 64] = dp;
+          }
+        }
+      }
+    }
+    // back substitution: x_{n-1} = d'_{n-1}, x_m = d'_m - c'_m x_{m+1}
+    auto fetch_b = [&](int blk, T *o) {
+      const int m0 = n - 2 - r - min(blk, nbf - 1) * B;
+#pragma unroll
+      for (int q = 0; q < B; ++q) o[q] = tabv(sa.tab.CP, m0 - q);
+    };
+#pragma unroll
+    for (int q = 0; q < B; ++q) rem[q] = tabv(sa.tab.CP, max(n - 2 - q, 0));
+    if (nbf > 0) {
+#pragma unroll
+      for (int sl = 0; sl < D - 1; ++sl) fetch_b(sl, ring[sl]);
+    }
+    T x = dp;
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      if (q < r) {
+        x = td[(n - 2 - q) * 64] - rem[q] * x;
+        td[(n - 2 - q) * 64] = x;
+      }
+    }
+    for (int blk = 0; blk < nbf; blk += D) {
+#pragma unroll
+      for (int sl = 0; sl < D; ++sl) {
+        if (blk + sl < nbf) {
+          fetch_b(blk + sl + D - 1, ring[(sl + D - 1) % D]);
+          const int m0 = n - 2 - r - (blk + sl) * B;
+          T v[B];
+#pragma unroll
+          for (int q = 0; q < B; ++q) v[q] = td[(m0 - q) * 64];
+#pragma unroll
+          for (int q = 0; q < B; ++q) {
+            x = v[q] - ring[sl][q] * x;
+            td[(m0 - q) * 64] = x;
+          }
+        }
+      }
+    }
+  }
+  if (!TG && wv == 0) {
     constexpr int B = 8;
     const int n = nz - 1;
     const int r = (n - 1) % B, nbf = (n - 1) / B;
@@ -1221,7 +1355,6 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         t[6] = T(gld(w, vC + si));
         if constexpr (RM) {
           const uint32_t sl = static_cast<uint32_t>(k) * lsL;
-          if constexpr (!RX) t[3] = bf16_val(gld(sa.rl, lC + sl));
           t[5] = LP ? bf16_val(gld(sa.pl, lC + sl)) : T(0);
           t[7] = LW ? bf16_val(gld(sa.wl, lC + sl)) : T(0);
         } else {
@@ -1229,9 +1362,9 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         }
       }
       if constexpr (F64M) t8[0] = gldc(sa.r0w64, tC8 + static_cast<uint32_t>(k + 1) * lsT8);
-      if constexpr (RX && RO) {  // chi of the level (and its remainder)
+      if constexpr (RX && RO) {  // chi of the level
         t[2] = T(gld(sa.chi, vC + static_cast<uint32_t>(k) * lsI));
-        t[3] = RM ? bf16_val(gld(sa.chil, lC + static_cast<uint32_t>(k) * lsL)) : T(0);
+        t[3] = T(0);  // chi carries no remainder
       }
     };
     const TF fh_next_chunk = (k0 + nlev < nz) ? s_fh[wv + 1][lane] : TF(0);
@@ -1320,10 +1453,8 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
             if constexpr (RX && RO) {
               const float rh = static_cast<float>(rx);
               gst(rho_o, oC + so, static_cast<O>(rh));
-              if constexpr (RM) gst(sa.rlo, lC + sl, bf16_bits(static_cast<float>(rx - static_cast<double>(rh))));
             } else if constexpr (!RX) {
               gst(rho_o, oC + so, static_cast<O>(rn));
-              if constexpr (RM) gst(sa.rlo, lC + sl, bf16_bits(cr.lo));
             }
             gst(pi_o, oC + so, static_cast<O>(pn));
             gst(wo, oC + so, static_cast<O>(wn));
