@@ -27,7 +27,7 @@ struct BDom {
   int kmin_sponge;
   int cur0;  // rotation index, advanced on the device by kb_advance
   int *flag;
-  // compensated FP32 mode (null otherwise): bf16 remainders of A[0], A[1] and
+  // compensated FP32 mode (null otherwise): remainders of A[0], A[1] and
   // the FP64 geometry of the substep's mass-flux sums
   AcousticLo L[2];
   const double *rhu64, *rhv64, *r0w64, *sx64, *sy64, *zeta64;

@@ -92,9 +92,10 @@ struct StageOut {
 };
 
 // ---- compensated FP32 storage -----------------------------------------------
-// An acoustic-set value is kept as an FP32 value hi and a remainder lo, stored
-// as the upper 16 bits (bfloat16, rounded to nearest even) of its FP32
-// pattern: hi + lo holds about 32 significant bits.  Substeps add FP32
+// An acoustic-set value is kept as an FP32 value hi and a remainder lo: for
+// v, w and pi' (when the remainder mask names them) the upper 16 bits
+// (bfloat16, rounded to nearest even) of its FP32 pattern, hi + lo holding
+// about 32 significant bits; for u 8 bits (r8_bits below).  Substeps add FP32
 // increments with comp_add, which recovers the rounding error of hi + d
 // exactly (Knuth's two-sum) and carries it into the remainder, so no
 // increment is lost against a large value.
@@ -656,7 +657,7 @@ struct SubArgs {
   int check;                       // 1: flag non-finite outputs
   int *flag;
   SpongeArgs sp = {};              // FP64 outputs only: sponge of the step's final state
-  // compensated FP32 mode: bf16 remainders of the state in and (FP32 outputs)
+  // compensated FP32 mode: remainders of the state in and (FP32 outputs)
   // out
   const int8_t *ul = nullptr;
   const uint16_t *vl = nullptr, *wl = nullptr, *pl = nullptr;
@@ -735,11 +736,14 @@ inline constexpr bool kSubstepIncrement = VMI_FP32_INCREMENT != 0 && sizeof(T) =
 // offset); the 32-bit sum is formed next to each access, which keeps the
 // scalar-base + vector-offset addressing form.
 //
-// CP (compensated FP32 storage, T = I = float): the state carries bf16
-// remainders (comp_add).  Every update is formed as an FP32 increment and
-// added to hi + lo exactly: u and v directly, rho' and pi' from increment
-// forms of E and Q, and w from the increment system A dw = -gk (ao dpi'_old
-// + an dP^), where P^ is the pi' estimate with the full old-w vertical flux
+// CP (compensated FP32 mode, T = I = float): u carries an 8-bit remainder
+// (r8_bits) and its FP32 increment is added to hi + lo exactly (comp_add);
+// the horizontal mass-flux divergence is formed in FP64 from the full new
+// winds and FP64 face products (F64H); the Thomas sweep reads beta and c'
+// from global memory (substep_thomas_global).  VMI_CP_MASK can give v, w and
+// pi' bf16 remainders as well; with one on pi' or w, pi' and w are updated in
+// increment form (IN): w from the increment system A dw = -gk (ao dpi'_old +
+// an dP^), where P^ is the pi' estimate with the full old-w vertical flux
 // rho0_w (w - M).  With w_new = w + dw this is the same system as
 // A w_new = w - gk (ao dpi'_old + an dP^ao) of the plain form (P^ao with the
 // ao part of the old-w flux), and the closing flux rho0_w (w - M) + an rho0_w dw

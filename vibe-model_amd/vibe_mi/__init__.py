@@ -51,7 +51,7 @@ VIBE_MI_ERR_FORMAT = 111
 VIBE_MI_ERR_NOT_FOUND = 112
 FP64 = 0
 MIXED = 1
-MIXED_COMP = 2  # FP32-mixed with compensated (FP32 + bf16 remainder) substep state
+MIXED_COMP = 2  # FP32-mixed with u compensated (FP32 + 8-bit remainder) and an FP64 horizontal divergence
 
 
 class VibeError(RuntimeError):
