@@ -42,7 +42,7 @@ extern "C" {
 /* FP32-compensated: VIBE_MI_MIXED whose FP32 acoustic (substep) state keeps
  * an 8-bit remainder of u (hi + lo, about 31 significant bits), to which the
  * substeps add u's FP32 increments with an exact two-sum, and whose substeps
- * form the horizontal mass-flux divergence in FP64: within 1e-4 of FP64
+ * form the x part of the horizontal mass-flux divergence in FP64: within 1e-4 of FP64
  * after 1 h where VIBE_MI_MIXED is not, at 5 instead of 4 bytes per stored
  * u value. */
 #define VIBE_MI_MIXED_COMP 2

@@ -2,7 +2,8 @@
 
 The mode keeps u in the FP32 substep state as an FP32 value plus an 8-bit
 remainder, adds the substeps' FP32 increments of u to it with an exact
-two-sum, and forms the substep's horizontal mass-flux divergence in FP64.
+two-sum, and forms the x part of the substep's horizontal mass-flux
+divergence in FP64.
 Bars: after 30 steps on the small flat, ridge and random-terrain cases
 every prognostic is within 3e-5 relative L2 of the FP64 path (the plain
 FP32-mixed path is held to 1e-4; at these lengths the FP32 stage arithmetic,

@@ -710,12 +710,15 @@ inline constexpr int kSubstepWavesPerEu =
 #endif
 // Compensated mode: which mass-flux sums are formed in FP64 from FP64
 // geometry (compile-time switch; bit 1: the horizontal divergence in FP32,
-// bit 2: the slope flux and the vertical mass flux in FP32; 0: both FP64).
-// Default 2: the horizontal divergence in FP64 (in FP32 the worst field is
-// 1.0e-4 at 256x256x96 after 1 h, with it 6.1e-5), the slope and vertical
-// fluxes in FP32 (5.85e-5 in FP64, no gain).
+// bit 2: the slope flux and the vertical mass flux in FP32, bit 4: the y
+// part of the horizontal divergence in FP32; 0: all FP64).  Default 6: the
+// x part of the horizontal divergence in FP64, the rest in FP32.  Worst
+// field at 256x256x96 after 1 h: 1.0e-4 with the whole horizontal
+// divergence in FP32, 5.85e-5 with everything in FP64, 6.1e-5 with the
+// slope and vertical fluxes in FP32, 6.9e-5 with the y part in FP32 as well
+// (one FP64 load per level less, 1 % faster at 1024x1024x96).
 #ifndef VMI_COMP_SUMS
-#define VMI_COMP_SUMS 2
+#define VMI_COMP_SUMS 6
 #endif
 template <typename T, typename I, bool CP>
 inline constexpr bool kSubstepIncrement = VMI_FP32_INCREMENT != 0 && sizeof(T) == 4 && sizeof(I) == 4 && !CP;
@@ -738,8 +741,9 @@ inline constexpr bool kSubstepIncrement = VMI_FP32_INCREMENT != 0 && sizeof(T) =
 //
 // CP (compensated FP32 mode, T = I = float): u carries an 8-bit remainder
 // (r8_bits) and its FP32 increment is added to hi + lo exactly (comp_add);
-// the horizontal mass-flux divergence is formed in FP64 from the full new
-// winds and FP64 face products (F64H); the Thomas sweep reads beta and c'
+// the x part of the horizontal mass-flux divergence is formed in FP64 from
+// the full new winds and FP64 face products (F64H, F64Y for the y part
+// too); the Thomas sweep reads beta and c'
 // from global memory (substep_thomas_global).  VMI_CP_MASK can give v, w and
 // pi' bf16 remainders as well; with one on pi' or w, pi' and w are updated in
 // increment form (IN): w from the increment system A dw = -gk (ao dpi'_old +
@@ -765,6 +769,9 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
   // slope and vertical mass fluxes in FP64 from FP64 geometry (compile-time
   // switch VMI_COMP_SUMS; otherwise the FP32 forms of the plain mode)
   constexpr bool F64H = CP && (VMI_COMP_SUMS & 1) == 0;
+  // F64Y: the y part of that divergence in FP64 as well (compile-time
+  // switch VMI_COMP_SUMS bit 4 keeps it in the plain FP32 form)
+  constexpr bool F64Y = F64H && (VMI_COMP_SUMS & 4) == 0;
   constexpr bool F64M = CP && (VMI_COMP_SUMS & 2) == 0;
   // RX (FP32 state): rho' is not carried through the substeps.  Every
   // substep changes pi' by kappa times its change of rho' (Q - pi' =
@@ -885,8 +892,13 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
         if constexpr (F64H) {
           qd[0] = gldc(sa.rhu64, tC8 + s8);
           qd[1] = colE ? gldc(sa.rhu64, tE8 + s8) : qd[0];
-          qd[2] = gldc(sa.rhv64, tC8 + s8);
-          qd[3] = colN ? gldc(sa.rhv64, tN8 + s8) : qd[2];
+          if constexpr (F64Y) {
+            qd[2] = gldc(sa.rhv64, tC8 + s8);
+            qd[3] = colN ? gldc(sa.rhv64, tN8 + s8) : qd[2];
+          } else {
+            q[HVN] = gldc(g.rhv, (colN ? tN : tC) + s);
+            q[HVS] = colN ? gldc(g.drhv, tC + s) : T(0);
+          }
         } else {
           // FP32: the east / north face products and their differences across
           // the cell (zero where the geometry is uniform in that direction)
@@ -1045,8 +1057,13 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
             mc = T(0.5) * (un_w * sx_w + un_e * sx_e);
         }
         if constexpr (F64H) {
-          const double hdd = (gd[1] * ued - gd[0] * uwd) * sa.rdx64 + (gd[3] * vnd - gd[2] * vsd) * sa.rdy64;
-          hd = static_cast<T>(hdd);
+          if constexpr (F64Y) {
+            const double hdd = (gd[1] * ued - gd[0] * uwd) * sa.rdx64 + (gd[3] * vnd - gd[2] * vsd) * sa.rdy64;
+            hd = static_cast<T>(hdd);
+          } else {  // the y part in the plain FP32 difference form
+            const T hdy = (gs[HVN] * (vn_n - vn_s) + gs[HVS] * vn_s) * g.rdy;
+            hd = static_cast<T>((gd[1] * ued - gd[0] * uwd) * sa.rdx64 + static_cast<double>(hdy));
+          }
         } else {
           if constexpr (sizeof(T) == 4) {
             // D rho0 u differenced across the cell without forming the large

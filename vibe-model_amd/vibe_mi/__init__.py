@@ -51,7 +51,7 @@ VIBE_MI_ERR_FORMAT = 111
 VIBE_MI_ERR_NOT_FOUND = 112
 FP64 = 0
 MIXED = 1
-MIXED_COMP = 2  # FP32-mixed with u compensated (FP32 + 8-bit remainder) and an FP64 horizontal divergence
+MIXED_COMP = 2  # FP32-mixed with u compensated (FP32 + 8-bit remainder) and the x part of the horizontal divergence in FP64
 
 
 class VibeError(RuntimeError):
