@@ -694,14 +694,18 @@ __host__ __device__ constexpr int substep_lds_arrays(bool cp) { return substep_t
 // stored in FP32 alone).
 __host__ __device__ constexpr bool cp_keeps(int bit) { return (VMI_CP_MASK & bit) != 0; }
 // Waves per SIMD the substep kernels are compiled for: FP32 shapes with up
-// to 12 levels per wave (plain about 106 VGPRs, compensated about 133) are
-// held to 128 VGPRs (four waves per SIMD; the compensated kernel then spills
-// 3-4 registers); the other shapes are register-limited by their own size.
+// to 12 levels per wave (plain about 106 VGPRs, compensated about 127) are
+// held to 128 VGPRs (four waves per SIMD), compensated shapes with up to 24
+// levels per wave to 168 (three; the variant writing the FP64 state would
+// take 171 and two); the other shapes are register-limited by their own
+// size.
 // With remainders on pi' or w (increment forms) the compensated kernels take
 // about 175 VGPRs and are not held.
 template <typename T, int NW, int MAXCH, bool CP = false>
 inline constexpr int kSubstepWavesPerEu =
-    (sizeof(T) == 4 && MAXCH <= 12 && (!CP || (VMI_CP_MASK & 12) == 0)) ? 4 : 1;
+    (sizeof(T) == 4 && MAXCH <= 12 && (!CP || (VMI_CP_MASK & 12) == 0))               ? 4
+    : (sizeof(T) == 4 && CP && MAXCH <= 24 && (VMI_CP_MASK & 12) == 0) ? 3
+                                                                        : 1;
 
 // Increment form of the plain FP32 substep (compile-time switch
 // VMI_FP32_INCREMENT): rho', pi' and w updated as old value + increment.
