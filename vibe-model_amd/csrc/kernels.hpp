@@ -116,6 +116,12 @@ __device__ __forceinline__ F2 comp_add(float hi, float lo, float d) {
   const float h2 = s + e;
   return F2{h2, e - (h2 - s)};
 }
+// The 1/kappa of chi = rho' - pi'/kappa (FP32 acoustic sets): the stage and
+// the stage's last substep must use the same value, not the exact one; the
+// hardware reciprocal of the FP32 kappa (within 1 ulp) leaves rho' = chi +
+// pi' r off by (kappa r - 1) times the stage's change of rho', about 1e-7 of
+// it, far below the FP32 rounding of rho' itself.
+__device__ __forceinline__ double chi_rkap(float kap) { return static_cast<double>(__builtin_amdgcn_rcpf(kap)); }
 // u keeps an 8-bit remainder instead: with hi = m 2^e (0.5 <= |m| < 1), the
 // remainder lo of a round-to-nearest hi satisfies |lo| <= 2^(e-25) and is
 // stored as q = rint(lo 2^(32-e)), |q| <= 128 (clamped to 127), so hi + lo
@@ -533,7 +539,7 @@ __device__ __forceinline__ void stage_tile(const Geo<T> &g, const StageCur<CI> &
         // d rho' per cell), so chi is constant over a stage's substeps and
         // the stage's last substep recovers rho' = chi + pi'/kappa
         if constexpr (sizeof(O) == 4)
-          put_acoustic(out.rho, static_cast<uint16_t *>(nullptr), ko, rn - pn * (1.0 / static_cast<double>(kapk)));
+          put_acoustic(out.rho, static_cast<uint16_t *>(nullptr), ko, rn - pn * chi_rkap(kapk));
         else
           put_acoustic(out.rho, static_cast<uint16_t *>(nullptr), ko, rn);
         gst(out.th, cCb + static_cast<uint32_t>(k) * lsD, sponge_scale(sp, k, nz, tn));
@@ -1455,7 +1461,7 @@ __device__ __forceinline__ void substep_row(const Geo<T> &g, const SubArgs<T, I,
             pv += static_cast<double>(cpn.lo);
           else if constexpr (IN)
             pv = static_cast<double>(dd[4]) + static_cast<double>(dp);
-          rx = (static_cast<double>(dd[2]) + static_cast<double>(dd[3])) + pv * (1.0 / static_cast<double>(dd[1]));
+          rx = (static_cast<double>(dd[2]) + static_cast<double>(dd[3])) + pv * chi_rkap(dd[1]);
         }
         if (active) {
           if constexpr (sizeof(O) == 8) {
