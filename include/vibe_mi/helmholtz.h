@@ -9,6 +9,10 @@
  *   KrylovJacobiSolver (BiCGSTAB)   src/time/helmholtz.cpp:286-379
  *   RefinementSolver (low-precision solve + FP64 residual correction)
  *                                   src/time/helmholtz.cpp:420-476
+ *   KrylovMultigridSolver (coarse-grid start of the Krylov solve)
+ *                                   src/time/helmholtz.cpp:382-417
+ *   multigrid_restrict / multigrid_prolong
+ *                                   src/time/helmholtz.cpp:131-164
  * for the operator of the reference,
  *
  *   y = -div(a grad x) + b x,
@@ -30,7 +34,17 @@
  *   VIBE_MI_HELM_REFINE     FP32 BiCGSTAB inner solves of the FP64 residual
  *                           equation, x += dx, until the FP64 relative
  *                           residual ||f - A x|| / ||f|| < tol (iterative
- *                           refinement).
+ *                           refinement);
+ *   VIBE_MI_HELM_MULTIGRID  two-grid start: f, a, b and the terrain restricted
+ *                           (multigrid_restrict) to the (nx+1)/2 x (ny+1)/2 x nz
+ *                           grid of spacing 2 dx, 2 dy, that problem solved to
+ *                           the looser of tol and 1e-3, its solution prolonged
+ *                           (multigrid_prolong) as the initial guess of the
+ *                           FP64 BiCGSTAB on the fine grid.  (The reference's
+ *                           KrylovMultigridSolver forms the same start on a
+ *                           field of the fine grid's size and its inner solver
+ *                           then restarts from zero, so its start has no
+ *                           effect; here it is a real coarse grid and is used.)
  * Iterations run without host round trips (scalars stay on the device); the
  * host reads the residual every few iterations.  Calls on one handle must not
  * overlap.  Return codes as in dycore.h.
@@ -48,6 +62,7 @@ extern "C" {
 
 #define VIBE_MI_HELM_BICGSTAB 0
 #define VIBE_MI_HELM_REFINE 1
+#define VIBE_MI_HELM_MULTIGRID 2
 
 typedef struct {
   int nx, ny, nz;
@@ -67,14 +82,32 @@ int vibe_mi_helmholtz_set_coefficients(vibe_mi_helmholtz *h, const double *a, co
 int vibe_mi_helmholtz_apply(vibe_mi_helmholtz *h, const double *x, double *y);
 /* ||A x - f||_2 (helmholtz_residual_l2), host arrays. */
 int vibe_mi_helmholtz_residual(vibe_mi_helmholtz *h, const double *f, const double *x, double *res_l2);
-/* Solves A x = f from x = 0 (the reference's solvers start from zero):
- * kind VIBE_MI_HELM_BICGSTAB or VIBE_MI_HELM_REFINE, relative residual tol,
+/* Solves A x = f from x = 0 (the reference's solvers start from zero; the
+ * multigrid kind from its coarse-grid start): kind VIBE_MI_HELM_BICGSTAB,
+ * VIBE_MI_HELM_REFINE or VIBE_MI_HELM_MULTIGRID, relative residual tol,
  * at most max_iter BiCGSTAB iterations in total.  iterations and rel_res
  * (either may be NULL) report the iterations taken and the final FP64
  * relative residual; like the reference, not converging within max_iter is
  * not an error (check rel_res). */
 int vibe_mi_helmholtz_solve(vibe_mi_helmholtz *h, const double *f, double *x, int kind, double tol, int max_iter,
                             int *iterations, double *rel_res);
+
+/* multigrid_restrict (helmholtz.cpp:131-145): 2:1 full weighting in x and
+ * y at every level,
+ *   coarse(i,j,k) = 1/4 f(2i,2j,k) + 1/8 [f(2i-1,2j,k) + f(2i+1,2j,k)
+ *                   + f(2i,2j-1,k) + f(2i,2j+1,k)] + 1/16 [f(2i-1,2j-1,k)
+ *                   + f(2i+1,2j-1,k) + f(2i-1,2j+1,k) + f(2i+1,2j+1,k)],
+ * fine indices (k included) clamped to the fine extents, for every point of
+ * the coarse field.  fine [fnz][fny][fnx], coarse [cnz][cny][cnx]: DEVICE
+ * arrays; runs on `stream` (hipStream_t, NULL: the null stream), allocates
+ * nothing.  Results equal the reference's bit for bit. */
+int vibe_mi_multigrid_restrict_f64(const double *fine, int fnx, int fny, int fnz, double *coarse, int cnx, int cny,
+                                   int cnz, void *stream);
+/* multigrid_prolong (helmholtz.cpp:147-164): bilinear interpolation, fine
+ * point (i, j) at coarse coordinate (i/2, j/2), coarse indices (k included)
+ * clamped, lerp(a, b, w) = a + w (b - a); DEVICE arrays as above. */
+int vibe_mi_multigrid_prolong_f64(const double *coarse, int cnx, int cny, int cnz, double *fine, int fnx, int fny,
+                                  int fnz, void *stream);
 
 #ifdef __cplusplus
 }

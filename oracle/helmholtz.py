@@ -8,13 +8,23 @@ Follows:
                           face a = mean of the two cells, neighbours clamped at
                           every boundary, dz = max(dzeta_k (ztop - zs), 1)
   helmholtz_residual_l2   src/time/helmholtz.cpp:211-225
+  multigrid_restrict      src/time/helmholtz.cpp:131-145: 2:1 full weighting
+  multigrid_prolong       src/time/helmholtz.cpp:147-164: bilinear, i -> i/2
 and, for a reference solution, the operator assembled as a sparse matrix
 and solved directly (scipy.sparse.linalg.spsolve, a library routine).
 
 Pinned (tests/test_oracle_helmholtz.py) by the reference's unit-test
 values (tests/unit/test_time.cpp:116-185: residual sqrt(64) of x = 0,
 f = 1 on 4x4x4; the vertical manufactured solution) and by closed forms
-(constant x gives b x; second differences of quadratics).
+(constant x gives b x; second differences of quadratics).  The transfers
+are pinned by the reference's own test (restrict then prolong does not grow
+the norm by more than 1.5, tests/unit/test_time.cpp:101-114) and by what
+the mathematics fixes: both reproduce constants (the weights sum to one),
+restriction of a field linear in (i, j) is its value at (2i, 2j) away from
+the clamped edges, prolongation reproduces a field bilinear in the coarse
+indices exactly inside the coarse extent.  The reference's helmholtz.cpp
+does not compile from its sources here ('Precision' has not been declared,
+helmholtz.cpp:431), so no compiled golden exists.
 """
 from __future__ import annotations
 
@@ -83,3 +93,34 @@ def solve(a, b, f, dx, dy, dz):
     import scipy.sparse.linalg as sla
     A = matrix(a, b, dx, dy, dz)
     return sla.spsolve(A.tocsc(), np.asarray(f, dtype=np.float64).reshape(-1)).reshape(np.shape(a))
+
+
+def _clamped(f, k, j, i):
+    """Field::clamp_at of a [nz, ny, nx] array at integer index arrays."""
+    nz, ny, nx = f.shape
+    return f[np.clip(k, 0, nz - 1), np.clip(j, 0, ny - 1), np.clip(i, 0, nx - 1)]
+
+
+def restrict(fine, coarse_shape):
+    """multigrid_restrict into a field of coarse_shape = (nz, ny, nx); the
+    reference's expression, summed in its order."""
+    fine = np.asarray(fine, dtype=np.float64)
+    k, j, i = np.meshgrid(*[np.arange(n) for n in coarse_shape], indexing="ij")
+    fi, fj = 2 * i, 2 * j
+    at = lambda di, dj: _clamped(fine, k, fj + dj, fi + di)  # noqa: E731
+    acc = 0.25 * at(0, 0)
+    acc = acc + 0.125 * (((at(-1, 0) + at(1, 0)) + at(0, -1)) + at(0, 1))
+    acc = acc + 0.0625 * (((at(-1, -1) + at(1, -1)) + at(-1, 1)) + at(1, 1))
+    return acc
+
+
+def prolong(coarse, fine_shape):
+    """multigrid_prolong into a field of fine_shape = (nz, ny, nx)."""
+    coarse = np.asarray(coarse, dtype=np.float64)
+    k, j, i = np.meshgrid(*[np.arange(n) for n in fine_shape], indexing="ij")
+    xs, ys = 0.5 * i, 0.5 * j
+    i0, j0 = np.floor(xs).astype(np.int64), np.floor(ys).astype(np.int64)
+    wx, wy = xs - i0, ys - j0
+    lerp = lambda a, b, w: a + w * (b - a)  # noqa: E731
+    c = lambda di, dj: _clamped(coarse, k, j0 + dj, i0 + di)  # noqa: E731
+    return lerp(lerp(c(0, 0), c(1, 0), wx), lerp(c(0, 1), c(1, 1), wx), wy)

@@ -370,6 +370,44 @@ void orc_literal_diagnose(const orc_literal_desc *d, const double *const s[12], 
   }
 }
 
+/* Damping::apply_sponge (src/dyn/damping.cpp:76-138) on the full state with
+ * the per-level rates sponge[nz]: u, v, w relaxed to zero, rho, theta, pi to
+ * the reference state */
+static void literal_sponge(const orc_literal_desc *d, double *const s[12], double dt, const double *sponge) {
+  const int nx = d->nx, ny = d->ny, nz = d->nz;
+  int kmin = nz;
+  for (int k = 0; k < nz; ++k)
+    if (sponge[k] > 0.0) {
+      kmin = k;
+      break;
+    }
+  for (int k = kmin; k <= nz; ++k) {
+    const double a = sponge[k < nz - 1 ? k : nz - 1];
+    if (a <= 0.0) continue;
+    const double w = 1.0 / (1.0 + dt * a);
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const size_t cw = ((size_t)k * ny + j) * nx + i;
+        s[2][cw] = w * s[2][cw];
+        if (k == nz) continue;
+        s[3][cw] = w * (s[3][cw] + dt * a * d->rho0[cw]);
+        s[4][cw] = w * (s[4][cw] + dt * a * d->theta0[cw]);
+        s[5][cw] = w * (s[5][cw] + dt * a * d->pi0[cw]);
+      }
+    if (k == nz) continue;
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i <= nx; ++i) {
+        const size_t cu = ((size_t)k * ny + j) * (nx + 1) + i;
+        s[0][cu] = w * s[0][cu];
+      }
+    for (int j = 0; j <= ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const size_t cv = ((size_t)k * (ny + 1) + j) * nx + i;
+        s[1][cv] = w * s[1][cv];
+      }
+  }
+}
+
 /* One step of the reference's RK3 integrator with its explicit
  * forward-backward acoustic substeps, literally (RungeKutta3Integrator::step,
  * src/time/runge_kutta.cpp:77-118, acoustic_substep :65-75) on the literal
@@ -421,42 +459,101 @@ int orc_literal_step(const orc_literal_desc *d, double *const s[12], double dt, 
           break;
         }
   }
-  if (rc == 0 && sponge != NULL) {
-    int kmin = nz;
-    for (int k = 0; k < nz; ++k)
-      if (sponge[k] > 0.0) {
-        kmin = k;
-        break;
-      }
-    for (int k = kmin; k <= nz; ++k) {
-      const double a = sponge[k < nz - 1 ? k : nz - 1];
-      if (a <= 0.0) continue;
-      const double w = 1.0 / (1.0 + dt * a);
-      for (int j = 0; j < ny; ++j)
-        for (int i = 0; i < nx; ++i) {
-          const size_t cw = ((size_t)k * ny + j) * nx + i;
-          s[2][cw] = w * s[2][cw];
-          if (k == nz) continue;
-          s[3][cw] = w * (s[3][cw] + dt * a * d->rho0[cw]);
-          s[4][cw] = w * (s[4][cw] + dt * a * d->theta0[cw]);
-          s[5][cw] = w * (s[5][cw] + dt * a * d->pi0[cw]);
-        }
-      if (k == nz) continue;
-      for (int j = 0; j < ny; ++j)
-        for (int i = 0; i <= nx; ++i) {
-          const size_t cu = ((size_t)k * ny + j) * (nx + 1) + i;
-          s[0][cu] = w * s[0][cu];
-        }
-      for (int j = 0; j <= ny; ++j)
-        for (int i = 0; i < nx; ++i) {
-          const size_t cv = ((size_t)k * (ny + 1) + j) * nx + i;
-          s[1][cv] = w * s[1][cv];
-        }
-    }
-  }
+  if (rc == 0 && sponge != NULL) literal_sponge(d, s, dt, sponge);
   for (int q = 0; q < 12; ++q) {
     free(s0[q]);
     free(R[q]);
   }
+  return rc;
+}
+
+/* ---- SemiImplicitIntegrator (src/time/semi_implicit.cpp) -------------- */
+
+/* SemiImplicitIntegrator::assemble_rhs (semi_implicit.cpp:58-108) for the
+ * state s, its tendency n (physics included) and step dt: a[ncell] =
+ * (dt/2)^2 (Rd/cv) pi0 cp theta0 and rhs[ncell] = pi - dt c G + div(a grad
+ * pi) (b = 1), every expression in the reference's order. */
+void orc_si_assemble(const orc_literal_desc *d, const double *const s[12], const double *const n[12], double dt,
+                     double *a, double *rhs) {
+  const L G0 = {d, d->nx, d->ny, d->nz};
+  const L *g = &G0;
+  const int nx = d->nx, ny = d->ny, nz = d->nz;
+  const double half_dt = 0.5 * dt;
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i)
+        a[oc(g, i, j, k)] = half_dt * half_dt * (kRd / kCv) * d->pi0[oc(g, i, j, k)] * kCp * d->theta0[oc(g, i, j, k)];
+  const double *r = d->rho0, *pi = s[5];
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const double dx = d->dx, dy = d->dy;
+        const double scale = d->z_top - (d->zs != NULL ? d->zs[(size_t)j * nx + i] : 0.0);
+        const double dzr = (d->zeta[k + 1] - d->zeta[k]) * scale;
+        const double dz = dzr < 1.0 ? 1.0 : dzr;
+        double div[2];
+        for (int f = 0; f < 2; ++f) {
+          const double *u = f == 0 ? s[0] : n[0], *v = f == 0 ? s[1] : n[1], *w = f == 0 ? s[2] : n[2];
+          const double fx = u[ou(g, i + 1, j, k)] * 0.5 * (r[oc(g, i, j, k)] + r[oc(g, imin(i + 1, nx - 1), j, k)]) -
+                            u[ou(g, i, j, k)] * 0.5 * (r[oc(g, imax(i - 1, 0), j, k)] + r[oc(g, i, j, k)]);
+          const double fy = v[ov(g, i, j + 1, k)] * 0.5 * (r[oc(g, i, j, k)] + r[oc(g, i, imin(j + 1, ny - 1), k)]) -
+                            v[ov(g, i, j, k)] * 0.5 * (r[oc(g, i, imax(j - 1, 0), k)] + r[oc(g, i, j, k)]);
+          const double fz = w[ow(g, i, j, k + 1)] * 0.5 * (r[oc(g, i, j, k)] + r[oc(g, i, j, imin(k + 1, nz - 1))]) -
+                            w[ow(g, i, j, k)] * 0.5 * (r[oc(g, i, j, imax(k - 1, 0))] + r[oc(g, i, j, k)]);
+          div[f] = fx / dx + fy / dy + fz / dz;
+        }
+        const double Gv = div[0] + half_dt * div[1];
+        const double r0c = r[oc(g, i, j, k)];
+        const double c = (kRd / kCv) * (d->pi0[oc(g, i, j, k)] / (r0c < 1e-8 ? 1e-8 : r0c));
+        const double a0 = a[oc(g, i, j, k)];
+        const double p = pi[oc(g, i, j, k)];
+        const double dag = (a0 * (pi[oc(g, imin(i + 1, nx - 1), j, k)] - p) - a0 * (p - pi[oc(g, imax(i - 1, 0), j, k)])) /
+                               (dx * dx) +
+                           (a0 * (pi[oc(g, i, imin(j + 1, ny - 1), k)] - p) - a0 * (p - pi[oc(g, i, imax(j - 1, 0), k)])) /
+                               (dy * dy);
+        rhs[oc(g, i, j, k)] = p - dt * c * Gv + dag;
+      }
+}
+
+/* The state update of SemiImplicitIntegrator::step (semi_implicit.cpp:
+ * 167-226): N = tendencies(s) plus the physics rates phys = {theta, qv, u,
+ * v} (NULL array or entry: zero); theta, qv, rho, qc..qg += dt N; with
+ * filter > 0 and first == 0, pi = (1 - filter) pi + filter pi; the sponge
+ * (sponge[nz], NULL: none).  The Helmholtz solution of the step reaches no
+ * field of s (the reference applies it to a copy it drops,
+ * semi_implicit.cpp:185-196), so the state update does not need it.
+ * Returns 0, 3 if s is non-finite afterwards, 2 on bad arguments. */
+int orc_si_step(const orc_literal_desc *d, double *const s[12], double dt, const double *sponge, double filter,
+                int first, const double *const phys[4]) {
+  if (d == NULL || s == NULL || d->nx <= 0 || d->ny <= 0 || d->nz <= 0) return 2;
+  const int nx = d->nx, ny = d->ny, nz = d->nz;
+  const size_t ncell = (size_t)nx * ny * nz;
+  const size_t nsz[12] = {(size_t)(nx + 1) * ny * nz, (size_t)nx * (ny + 1) * nz, (size_t)nx * ny * (nz + 1),
+                          ncell, ncell, ncell, ncell, ncell, ncell, ncell, ncell, ncell};
+  double *N[12];
+  for (int q = 0; q < 12; ++q) {
+    N[q] = (double *)malloc(nsz[q] * sizeof(double));
+    if (N[q] == NULL) return 2;
+  }
+  orc_literal_tendencies(d, (const double *const *)s, N);
+  static const int target[4] = {4, 6, 0, 1};
+  if (phys != NULL)
+    for (int f = 0; f < 4; ++f)
+      if (phys[f] != NULL)
+        for (size_t c = 0; c < nsz[target[f]]; ++c) N[target[f]][c] += 1.0 * phys[f][c];
+  static const int scalars[8] = {4, 6, 3, 7, 8, 9, 10, 11};
+  for (int f = 0; f < 8; ++f)
+    for (size_t c = 0; c < ncell; ++c) s[scalars[f]][c] += dt * N[scalars[f]][c];
+  if (filter > 0.0 && !first)
+    for (size_t c = 0; c < ncell; ++c) s[5][c] = (1.0 - filter) * s[5][c] + filter * s[5][c];
+  if (sponge != NULL) literal_sponge(d, s, dt, sponge);
+  int rc = 0;
+  for (int q = 0; q < 12 && rc == 0; ++q)
+    for (size_t c = 0; c < nsz[q]; ++c)
+      if (!isfinite(s[q][c])) {
+        rc = 3;
+        break;
+      }
+  for (int q = 0; q < 12; ++q) free(N[q]);
   return rc;
 }

@@ -65,6 +65,14 @@ void orc_literal_diagnose(const orc_literal_desc *d, const double *const s[12], 
  * forward-backward substeps and the sponge rates sponge[nz] (NULL: none);
  * 0, 3 if non-finite, 2 bad arguments */
 int orc_literal_step(const orc_literal_desc *d, double *const s[12], double dt, int n_sub, const double *sponge);
+/* SemiImplicitIntegrator (semi_implicit.cpp): assemble_rhs of state s and
+ * tendency n into a[ncell], rhs[ncell]; the state update of one step
+ * (tendencies + physics rates {theta, qv, u, v}, forward Euler of the
+ * scalars, time filter, sponge); 0, 3 if non-finite, 2 bad arguments */
+void orc_si_assemble(const orc_literal_desc *d, const double *const s[12], const double *const n[12], double dt,
+                     double *a, double *rhs);
+int orc_si_step(const orc_literal_desc *d, double *const s[12], double dt, const double *sponge, double filter,
+                int first, const double *const phys[4]);
 
 /* ---- dynamical core (dycore.c) ---------------------------------------- */
 typedef struct {

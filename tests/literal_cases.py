@@ -105,6 +105,12 @@ def _bind():
         L.orc_literal_step.argtypes = [ctypes.POINTER(LiteralDesc), ctypes.POINTER(_PD), _D, _I, _PD]
         L.orc_literal_diagnose.restype = None
         L.orc_literal_diagnose.argtypes = [ctypes.POINTER(LiteralDesc), ctypes.POINTER(_PD), _PD]
+        L.orc_si_assemble.restype = None
+        L.orc_si_assemble.argtypes = [ctypes.POINTER(LiteralDesc), ctypes.POINTER(_PD), ctypes.POINTER(_PD), _D, _PD,
+                                      _PD]
+        L.orc_si_step.restype = _I
+        L.orc_si_step.argtypes = [ctypes.POINTER(LiteralDesc), ctypes.POINTER(_PD), _D, _PD, _D, _I,
+                                  ctypes.POINTER(_PD)]
         L._literal_bound = True
     return L
 
@@ -134,6 +140,34 @@ def oracle_step(d, s, dt, nsub, sponge=None, nsteps=1):
     rc = 0
     for _ in range(nsteps):
         rc = L.orc_literal_step(ctypes.byref(d), ptrs(s), dt, nsub, None if sp is None else sp.ctypes.data_as(_PD))
+        if rc:
+            break
+    return rc
+
+
+def oracle_si_assemble(d, s, n, dt):
+    """(a, rhs) of SemiImplicitIntegrator::assemble_rhs for state s and
+    tendency n (oracle/literal.c)."""
+    nc = s[3].size
+    a, r = np.zeros(nc), np.zeros(nc)
+    _bind().orc_si_assemble(ctypes.byref(d), ptrs(s), ptrs(n), dt, a.ctypes.data_as(_PD), r.ctypes.data_as(_PD))
+    return a, r
+
+
+def oracle_si_step(d, s, dt, sponge=None, time_filter=0.0, first=True, phys=None, nsteps=1):
+    """nsteps state updates of SemiImplicitIntegrator::step on s in place
+    (the first of them with the filter's first-step flag `first`); returns
+    the last return code."""
+    L = _bind()
+    sp = None if sponge is None else np.ascontiguousarray(sponge, dtype=np.float64)
+    ph = None
+    if phys is not None:
+        keep = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in phys]
+        ph = (_PD * 4)(*[None if a is None else a.ctypes.data_as(_PD) for a in keep])
+    rc = 0
+    for n in range(nsteps):
+        rc = L.orc_si_step(ctypes.byref(d), ptrs(s), dt, None if sp is None else sp.ctypes.data_as(_PD), time_filter,
+                           1 if (first and n == 0) else 0, ph)
         if rc:
             break
     return rc

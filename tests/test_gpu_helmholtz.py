@@ -1,11 +1,11 @@
 """3-D Helmholtz solver on the GPU (include/vibe_mi/helmholtz.h) against the
 numpy restatement (oracle/helmholtz.py) and its direct solution.
 
-Bars: the FP64 operator equals the oracle's bit for bit; residual norms agree
-to 1e-13 relative; both solvers (FP64 BiCGSTAB, FP32 + FP64 iterative
-refinement) reach the FP64 relative residual 1e-10 and the direct solution to
-1e-8 relative; the reference's own test cases hold (tests/unit/test_time.cpp:
-116-171)."""
+Bars: the FP64 operator and the multigrid transfers equal the oracle's bit
+for bit; residual norms agree to 1e-13 relative; the three solvers (FP64
+BiCGSTAB, FP32 + FP64 iterative refinement, the two-grid start) reach the
+FP64 relative residual 1e-10 and the direct solution to 1e-8 relative; the
+reference's own test cases hold (tests/unit/test_time.cpp:101-171)."""
 import os
 import sys
 import time
@@ -51,7 +51,7 @@ def test_apply_and_residual_match_oracle(vmi):
     h.close()
 
 
-@pytest.mark.parametrize("kind", ["bicgstab", "refine"])
+@pytest.mark.parametrize("kind", ["bicgstab", "refine", "multigrid"])
 @pytest.mark.parametrize("terrain", [False, True])
 def test_solvers_reach_the_direct_solution(vmi, kind, terrain):
     nx, ny, nz = 20, 16, 24
@@ -60,7 +60,7 @@ def test_solvers_reach_the_direct_solution(vmi, kind, terrain):
     dx = dy = 1000.0
     h = vmi.Helmholtz(nx, ny, nz, dx, dy, 20000.0, zeta, zs)
     h.set_coefficients(a, b)
-    k = vmi.HELM_BICGSTAB if kind == "bicgstab" else vmi.HELM_REFINE
+    k = {"bicgstab": vmi.HELM_BICGSTAB, "refine": vmi.HELM_REFINE, "multigrid": vmi.HELM_MULTIGRID}[kind]
     x = h.solve(f, kind=k, tol=1e-10, max_iter=400)
     print(kind, terrain, "iterations", h.iterations, "rel residual", h.residual_rel)
     assert h.residual_rel < 1e-10
@@ -72,7 +72,7 @@ def test_solvers_reach_the_direct_solution(vmi, kind, terrain):
     h.close()
 
 
-@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("kind", [0, 1, 2])
 def test_reference_test_cases(vmi, kind):
     # vertical manufactured solution (test_time.cpp:116-152), grid 4x4x16
     nz, ztop = 16, 20000.0
@@ -119,12 +119,52 @@ def test_large_solve_iterations_and_time(vmi):
     f = rng.normal(size=(nz, ny, nx))
     h = vmi.Helmholtz(nx, ny, nz, 1000.0, 1000.0, 20000.0, zeta, zs)
     h.set_coefficients(a, b)
-    for kind in (vmi.HELM_BICGSTAB, vmi.HELM_REFINE):
-        h.solve(f, kind=kind, tol=1e-10, max_iter=500)
-        t0 = time.perf_counter()
-        h.solve(f, kind=kind, tol=1e-10, max_iter=500)
-        ms = (time.perf_counter() - t0) * 1e3
-        print(f"helmholtz 256x256x64 kind {kind}: {h.iterations} iterations, rel residual "
-              f"{h.residual_rel:.2e}, {ms:.1f} ms (host copies included)", flush=True)
-        assert h.residual_rel < 1e-10
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    smooth = np.sin(2 * np.pi * i / nx) * np.cos(4 * np.pi * j / ny) * np.sin(np.pi * (k + 0.5) / nz)
+    for name, rhs in (("noise", f), ("smooth", smooth)):
+        for kind in (vmi.HELM_BICGSTAB, vmi.HELM_REFINE, vmi.HELM_MULTIGRID):
+            h.solve(rhs, kind=kind, tol=1e-10, max_iter=500)
+            t0 = time.perf_counter()
+            h.solve(rhs, kind=kind, tol=1e-10, max_iter=500)
+            ms = (time.perf_counter() - t0) * 1e3
+            print(f"helmholtz 256x256x64 {name} right-hand side, kind {kind}: {h.iterations} iterations, "
+                  f"rel residual {h.residual_rel:.2e}, {ms:.1f} ms (host copies included)", flush=True)
+            assert h.residual_rel < 1e-10
     h.close()
+
+
+@pytest.mark.parametrize("shape", [((3, 16, 20), (3, 8, 10)), ((2, 9, 13), (2, 5, 7)), ((4, 8, 8), (4, 8, 8)),
+                                   ((1, 1, 5), (2, 3, 3))])
+def test_multigrid_transfers_match_oracle_bit_for_bit(vmi, shape):
+    """vibe_mi_multigrid_restrict_f64 / _prolong_f64 against the oracle on
+    even, odd, same-size (the reference's own use) and degenerate extents."""
+    import torch
+    fs, cs = shape
+    rng = np.random.default_rng(sum(fs) + sum(cs))
+    fine = rng.normal(size=fs)
+    coarse = rng.normal(size=cs)
+    dfine = torch.tensor(fine, device="cuda")
+    dco = torch.empty(cs, dtype=torch.float64, device="cuda")
+    vmi.multigrid_restrict(dfine, dco)
+    torch.cuda.synchronize()
+    assert np.array_equal(dco.cpu().numpy(), orc.restrict(fine, cs))
+    dco.copy_(torch.tensor(coarse))
+    dback = torch.empty(fs, dtype=torch.float64, device="cuda")
+    vmi.multigrid_prolong(dco, dback)
+    torch.cuda.synchronize()
+    assert np.array_equal(dback.cpu().numpy(), orc.prolong(coarse, fs))
+
+
+def test_multigrid_transfers_reference_norm_test(vmi):
+    # tests/unit/test_time.cpp:101-114 on the device
+    import torch
+    i = np.arange(8.0)
+    fine = np.broadcast_to(np.sin(0.7 * i)[None, None, :] + np.cos(0.5 * i)[None, :, None], (4, 8, 8)).copy()
+    df = torch.tensor(fine, device="cuda")
+    dc = torch.empty((4, 4, 4), dtype=torch.float64, device="cuda")
+    db = torch.empty((4, 8, 8), dtype=torch.float64, device="cuda")
+    vmi.multigrid_restrict(df, dc)
+    vmi.multigrid_prolong(dc, db)
+    torch.cuda.synchronize()
+    assert torch.isfinite(dc).all()
+    assert float(torch.linalg.norm(db)) <= 1.5 * float(np.linalg.norm(fine))

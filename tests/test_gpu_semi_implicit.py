@@ -1,0 +1,279 @@
+"""The reference's semi-implicit step on the GPU (vibe_mi_si_*,
+include/vibe_mi/semi_implicit.h; SemiImplicitIntegrator,
+src/time/semi_implicit.cpp:58-233) against the oracle's restatement
+(orc_si_assemble / orc_si_step, oracle/literal.c).
+
+Bars: the right-hand side, the coefficient a and the state after several
+steps (sponge, time filter and physics rates on) equal the oracle's bit for
+bit in the literal fixture's three numerics configurations and on a larger
+Agnesi grid; the step's Helmholtz solution satisfies the oracle's operator
+(oracle/helmholtz.py) to the solver tolerance and the direct solution to
+1e-8 relative for each solver kind, and the kind does not change the state
+(the reference drops the solution, semi_implicit.cpp:185-196); the
+reference's own test (a resting state over five steps of 10 s stays finite,
+steps == 5, tests/unit/test_time.cpp:297-313) on the reference's isothermal
+state; statistics, max_stable_dt and the non-finite error path.
+"""
+import ctypes
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from literal_cases import CASES, NX, NY, NZ, DX, DY, ZTOP, LiteralDesc, golden, oracle_si_assemble, oracle_si_step, \
+    oracle_tendencies, shape, state
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import helmholtz as orc_h  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+G = golden()
+SETUP = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "setup_reference.json")
+
+
+class Grid:
+    """A grid, reference state and numerics usable by both the library
+    (vibe_mi.SemiImplicit) and the oracle (LiteralDesc)."""
+
+    def __init__(self, nx, ny, nz, dx, dy, ztop, zeta, zs, r0, t0, p0, order=2, cor=0, lat=45.0, damp=0.0, dord=2,
+                 moist=0):
+        self.nx, self.ny, self.nz, self.dx, self.dy, self.ztop = nx, ny, nz, dx, dy, ztop
+        self.arr = {k: None if v is None else np.ascontiguousarray(v, dtype=np.float64).ravel()
+                    for k, v in dict(zeta=zeta, zs=zs, r0=r0, t0=t0, p0=p0).items()}
+        self.num = dict(order=order, coriolis=cor, latitude=lat, div_coeff=damp, div_order=dord, moist=moist)
+
+    @classmethod
+    def case(cls, tag):
+        grid, order, cor, lat, damp, dord, moist = CASES[tag]
+        return cls(NX, NY, NZ, DX, DY, ZTOP, G["zeta"], G["zs_hill"] if grid == "hill" else None,
+                   G[f"{grid}_rho0"], G[f"{grid}_theta0"], G[f"{grid}_pi0"], order, cor, lat, damp, dord, moist)
+
+    def desc(self):
+        a = self.arr
+        p = [None if a[k] is None else a[k].ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+             for k in ("zeta", "zs", "r0", "t0", "p0")]
+        n = self.num
+        return LiteralDesc(self.nx, self.ny, self.nz, self.dx, self.dy, self.ztop, *p, n["order"], n["coriolis"],
+                           n["latitude"], n["div_coeff"], n["div_order"], n["moist"])
+
+    def si(self, vmi, **kw):
+        a = self.arr
+        return vmi.SemiImplicit(self.nx, self.ny, self.nz, self.dx, self.dy, self.ztop, a["zeta"], a["r0"], a["t0"],
+                                a["p0"], zs=a["zs"], **self.num, **kw)
+
+    def shape(self, q):
+        return (self.nz + (q == 2), self.ny + (q == 1), self.nx + (q == 0))
+
+    def dz(self):
+        zs = self.arr["zs"] if self.arr["zs"] is not None else np.zeros(self.nx * self.ny)
+        return orc_h.dz_of(self.arr["zeta"], zs.reshape(self.ny, self.nx), self.ztop)
+
+
+def _phys(g, seed):
+    rng = np.random.default_rng(seed)
+    return [rng.normal(size=int(np.prod(g.shape(q)))) * s for q, s in ((4, 1e-3), (6, 1e-7), (0, 1e-3), (1, 1e-3))]
+
+
+def _sponge(nz, kmin):
+    r = np.zeros(nz)
+    r[kmin:] = np.linspace(0.004, 0.02, nz - kmin)
+    return r
+
+
+def _agnesi_grid(vmi, nx, ny, nz, order=4, moist=1):
+    """Agnesi ridge, stretched levels, the reference's isothermal state (the
+    library's set-up helpers are pinned bit for bit against the reference's
+    own code in test_setup_golden.py)."""
+    L, pd = vmi.lib(), vmi._pd
+    dx, dy, ztop = 1000.0, 1200.0, 15000.0
+    zeta = np.empty(nz + 1)
+    assert L.vibe_mi_zeta_stretched(nz, 1.0, 1.05, pd(zeta)) == 0
+    zs = np.empty(nx * ny)
+    assert L.vibe_mi_terrain_agnesi_bell(nx, ny, dx, dy, 600.0, 8000.0, 8000.0, 12000.0, pd(zs)) == 0
+    r0, t0, p0 = (np.empty(nx * ny * nz) for _ in range(3))
+    assert L.vibe_mi_reference_state(0, 300.0, 0.0, 100000.0, nx, ny, nz, ztop, pd(zeta), pd(zs), pd(r0), pd(t0),
+                                     pd(p0)) == 0
+    return Grid(nx, ny, nz, dx, dy, ztop, zeta, zs, r0, t0, p0, order=order, cor=1, lat=40.0, damp=1e5, dord=2,
+                moist=moist)
+
+
+def _perturbed(g, seed):
+    rng = np.random.default_rng(seed)
+    s = []
+    for q in range(12):
+        n = int(np.prod(g.shape(q)))
+        if q == 0:
+            a = 10.0 + rng.normal(size=n)
+        elif q == 1:
+            a = -2.0 + rng.normal(size=n)
+        elif q == 2:
+            a = 0.3 * rng.normal(size=n)
+        elif q == 3:
+            a = g.arr["r0"] * (1.0 + 1e-3 * rng.normal(size=n))
+        elif q == 4:
+            a = g.arr["t0"] + rng.normal(size=n)
+        elif q == 5:
+            a = g.arr["p0"] * (1.0 + 1e-4 * rng.normal(size=n))
+        else:
+            a = np.abs(1e-3 * rng.normal(size=n))
+        s.append(np.ascontiguousarray(a))
+    return s
+
+
+@pytest.mark.parametrize("tag", ["c2", "c4", "c6"])
+def test_rhs_matches_oracle_bit_for_bit(vmi, tag):
+    g = Grid.case(tag)
+    s = state(G, CASES[tag][0])
+    si = g.si(vmi)
+    si.set_state(s)
+    ph = _phys(g, 3)
+    si.set_physics_tendency(*ph)
+    a, rhs = si.rhs(6.0)
+    d = g.desc()
+    n = oracle_tendencies(d, s)
+    for q, r in zip((4, 6, 0, 1), ph):
+        n[q] = n[q] + 1.0 * r
+    ao, ro = oracle_si_assemble(d, s, n, 6.0)
+    assert np.array_equal(a, ao)
+    assert np.array_equal(rhs, ro)
+    # the right-hand side does not change the device state
+    assert all(np.array_equal(x, y) for x, y in zip(si.get_state(), s))
+    si.close()
+
+
+@pytest.mark.parametrize("tag", ["c2", "c4", "c6"])
+def test_steps_match_oracle_bit_for_bit(vmi, tag):
+    g = Grid.case(tag)
+    s = state(G, CASES[tag][0])
+    sp = _sponge(NZ, NZ - 2)
+    ph = _phys(g, 11)
+    si = g.si(vmi, sponge=sp, time_filter=0.1)
+    si.set_state(s)
+    si.set_physics_tendency(*ph)
+    si.step(4.0, 3)
+    out = si.get_state()
+    so = [x.copy() for x in s]
+    assert oracle_si_step(g.desc(), so, 4.0, sponge=sp, time_filter=0.1, first=True, phys=ph, nsteps=3) == 0
+    for q in range(12):
+        assert np.array_equal(out[q], so[q]), (tag, q)
+    st = si.stats()
+    assert st["steps"] == 3 and st["diverged"] == 0 and st["acoustic_substeps"] == 0
+    assert st["helmholtz_iterations"] >= 3 and st["helmholtz_residual"] < 1e-8
+    si.reset_stats()
+    assert si.stats()["steps"] == 0 and si.stats()["helmholtz_iterations"] == 0
+    si.close()
+
+
+def test_larger_agnesi_grid_matches_oracle(vmi):
+    # 130 x 20 x 12: three lane blocks along x, terrain, fourth-order
+    # advection, Coriolis, divergence damping, moisture, sponge, filter
+    g = _agnesi_grid(vmi, 130, 20, 12)
+    s = _perturbed(g, 5)
+    sp = _sponge(12, 8)
+    si = g.si(vmi, sponge=sp, time_filter=0.05, solver=vmi.HELM_MULTIGRID, tol=1e-9)
+    si.set_state(s)
+    si.step(3.0, 2)
+    out = si.get_state()
+    so = [x.copy() for x in s]
+    assert oracle_si_step(g.desc(), so, 3.0, sponge=sp, time_filter=0.05, first=True, nsteps=2) == 0
+    for q in range(12):
+        assert np.array_equal(out[q], so[q]), q
+    si.close()
+
+
+@pytest.mark.parametrize("kind", ["bicgstab", "refine", "multigrid"])
+def test_pressure_solves_the_helmholtz_equation(vmi, kind):
+    g = _agnesi_grid(vmi, 24, 16, 10)
+    s = _perturbed(g, 8)
+    k = {"bicgstab": vmi.HELM_BICGSTAB, "refine": vmi.HELM_REFINE, "multigrid": vmi.HELM_MULTIGRID}[kind]
+    dt = 5.0
+    si = g.si(vmi, solver=k, tol=1e-11, max_iter=400)
+    si.set_state(s)
+    si.step(dt, 1)
+    x = si.pressure().reshape(g.nz, g.ny, g.nx)
+    d = g.desc()
+    a, rhs = oracle_si_assemble(d, s, oracle_tendencies(d, s), dt)
+    a, rhs = a.reshape(x.shape), rhs.reshape(x.shape)
+    dz = g.dz()
+    res = orc_h.residual_l2(a, np.ones_like(a), rhs, x, g.dx, g.dy, dz)
+    assert res <= 1e-10 * np.linalg.norm(rhs), res
+    xd = orc_h.solve(a, np.ones_like(a), rhs, g.dx, g.dy, dz)
+    assert np.linalg.norm(x - xd) <= 1e-8 * np.linalg.norm(xd)
+    # the solution reaches no field of the state
+    so = [q.copy() for q in s]
+    assert oracle_si_step(d, so, dt) == 0
+    out = si.get_state()
+    for q in range(12):
+        assert np.array_equal(out[q], so[q]), q
+    print(kind, "iterations", si.stats()["helmholtz_iterations"], "residual", si.stats()["helmholtz_residual"])
+    si.close()
+
+
+def test_reference_resting_state_test(vmi):
+    # tests/unit/test_time.cpp:297-313: resting isothermal state (300 K,
+    # 1000 hPa; here the reference's own isothermal state over its stretched
+    # levels and Agnesi terrain, 8 x 6 x 16), five steps of 10 s with the
+    # vertical line solver's role taken by the library's solver: finite,
+    # stats.steps == 5, and equal to the oracle bit for bit
+    with open(SETUP) as fh:
+        gold = {k: np.array(v) for k, v in json.load(fh).items()}
+    g = Grid(8, 6, 16, 2000.0, 2000.0, 20000.0, gold["zeta_16_1_1.08"], gold["agnesi_8x6"], gold["isothermal_rho0"],
+             gold["isothermal_theta0"], gold["isothermal_pi0"])
+    s = [np.zeros(int(np.prod(g.shape(q)))) for q in range(12)]
+    s[3], s[4], s[5] = g.arr["r0"].copy(), g.arr["t0"].copy(), g.arr["p0"].copy()
+    si = g.si(vmi)
+    si.set_state(s)
+    assert si.max_stable_dt() == 600.0  # at rest: no advective limit
+    si.step(10.0, 5)
+    out = si.get_state()
+    assert all(np.all(np.isfinite(q)) for q in out)
+    assert si.stats()["steps"] == 5
+    so = [q.copy() for q in s]
+    assert oracle_si_step(g.desc(), so, 10.0, nsteps=5) == 0
+    for q in range(12):
+        assert np.array_equal(out[q], so[q]), q
+    si.close()
+
+
+def test_max_stable_dt_is_the_advective_limit(vmi):
+    g = Grid.case("c4")
+    s = state(G, "hill")
+    si = g.si(vmi)
+    si.set_state(s)
+    u = s[0].reshape(g.shape(0))
+    v = s[1].reshape(g.shape(1))
+    w = s[2].reshape(g.shape(2))
+    uu = 0.5 * (u[:, :, :-1] + u[:, :, 1:])
+    vv = 0.5 * (v[:, :-1, :] + v[:, 1:, :])
+    ww = 0.5 * (w[:-1] + w[1:])
+    cfl = np.max(1.0 * (np.abs(uu) / DX + np.abs(vv) / DY + np.abs(ww) / g.dz()))
+    assert si.max_stable_dt() == 0.6 / cfl
+    si.close()
+
+
+def test_nonfinite_state_is_an_error(vmi):
+    g = Grid.case("c6")
+    s = state(G, "flat")
+    s[4][17] = np.nan
+    si = g.si(vmi)
+    si.set_state(s)
+    with pytest.raises(vmi.VibeError) as e:
+        si.step(2.0, 1)
+    assert e.value.code == vmi.VIBE_MI_ERR_NONFINITE
+    assert si.stats()["diverged"] == 1
+    si.close()
+
+
+def test_bad_arguments(vmi):
+    g = Grid.case("c2")
+    with pytest.raises(vmi.VibeError):
+        g.si(vmi, solver=7)
+    with pytest.raises(vmi.VibeError):
+        g.si(vmi, tol=0.0)
+    si = g.si(vmi)
+    with pytest.raises(vmi.VibeError):
+        si.step(-1.0, 1)
+    si.close()

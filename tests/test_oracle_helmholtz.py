@@ -68,3 +68,72 @@ def test_direct_solution_has_tiny_residual():
     f = rng.normal(size=sh)
     x = orc.solve(a, b, f, 800.0, 900.0, dz)
     assert orc.residual_l2(a, b, f, x, 800.0, 900.0, dz) <= 1e-12 * np.linalg.norm(f)
+
+
+# ---- multigrid_restrict / multigrid_prolong (helmholtz.cpp:131-164) ------
+
+def test_restrict_prolong_norm_bound_of_the_reference_test():
+    # tests/unit/test_time.cpp:101-114: 8x8x4 -> 4x4x4 -> 8x8x4,
+    # fine = sin(0.7 i) + cos(0.5 j); the round trip must not grow the norm
+    # by more than 1.5 and stays finite
+    i = np.arange(8.0)
+    fine = np.broadcast_to(np.sin(0.7 * i)[None, None, :] + np.cos(0.5 * i)[None, :, None], (4, 8, 8)).copy()
+    coarse = orc.restrict(fine, (4, 4, 4))
+    back = orc.prolong(coarse, (4, 8, 8))
+    assert np.all(np.isfinite(coarse))
+    assert np.linalg.norm(back) <= 1.5 * np.linalg.norm(fine)
+
+
+def test_transfers_reproduce_constants():
+    # the restriction weights sum to one (1/4 + 4/8 + 4/16), the bilinear
+    # weights too: a constant is reproduced exactly by both
+    c = np.full((3, 9, 11), 2.5)
+    assert np.array_equal(orc.restrict(c, (3, 5, 6)), np.full((3, 5, 6), 2.5))
+    assert np.array_equal(orc.prolong(c, (3, 17, 21)), np.full((3, 17, 21), 2.5))
+
+
+def test_restriction_of_a_linear_field_is_its_value_at_the_coarse_point():
+    # full weighting is symmetric about (2i, 2j): a field linear in (i, j)
+    # restricts to its value there wherever no neighbour is clamped; a
+    # weight moved off-centre or a transposed index breaks it
+    nz, ny, nx = 2, 13, 15
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    fine = 3.0 * i - 2.0 * j + 7.0 * k + 1.0
+    cs = (nz, 6, 7)
+    co = orc.restrict(fine, cs)
+    kc, jc, ic = np.meshgrid(*[np.arange(n) for n in cs], indexing="ij")
+    want = 3.0 * (2 * ic) - 2.0 * (2 * jc) + 7.0 * kc + 1.0
+    inner = (ic >= 1) & (2 * ic + 1 <= nx - 1) & (jc >= 1) & (2 * jc + 1 <= ny - 1)
+    assert np.array_equal(co[inner], want[inner])
+    # at the clamped west edge the i-1 neighbour is the edge value itself
+    assert co[0, 3, 0] == 0.25 * fine[0, 6, 0] + 0.125 * (fine[0, 6, 0] + fine[0, 6, 1] + fine[0, 5, 0] +
+                                                          fine[0, 7, 0]) + 0.0625 * (
+        fine[0, 5, 0] + fine[0, 5, 1] + fine[0, 7, 0] + fine[0, 7, 1])
+
+
+def test_prolongation_is_exact_for_bilinear_fields():
+    # bilinear interpolation reproduces a + b x + c y + d x y exactly at
+    # fine points inside the coarse extent; even fine points take the coarse
+    # value itself
+    nz, cy, cx = 2, 5, 6
+    kc, jc, ic = np.meshgrid(np.arange(nz), np.arange(cy), np.arange(cx), indexing="ij")
+    g = lambda x, y, z: 0.5 + 2.0 * x - 3.0 * y + 0.25 * x * y + z  # noqa: E731
+    co = g(ic * 1.0, jc * 1.0, kc * 1.0)
+    fs = (nz, 2 * cy - 1, 2 * cx - 1)
+    fi = orc.prolong(co, fs)
+    k, j, i = np.meshgrid(*[np.arange(n) for n in fs], indexing="ij")
+    assert np.allclose(fi, g(0.5 * i, 0.5 * j, k * 1.0), rtol=0, atol=1e-13)
+    assert np.array_equal(fi[:, ::2, ::2], co)
+    # past the coarse extent the clamped neighbour is the edge value, so the
+    # last fine column repeats the one on the last coarse column
+    f2 = orc.prolong(co, (nz, 2 * cy, 2 * cx))
+    assert np.array_equal(f2[:, :, -1], f2[:, :, -2])
+    assert np.array_equal(f2[:, -1, :], f2[:, -2, :])
+
+
+def test_restrict_of_larger_coarse_field_clamps_into_the_fine_one():
+    # the coarse field's extents set the loop (the reference calls it with a
+    # field of the fine grid's size): points past nx/2 read clamped values
+    fine = np.arange(2 * 4 * 6, dtype=np.float64).reshape(2, 4, 6)
+    full = orc.restrict(fine, (2, 4, 6))
+    assert np.array_equal(full[:, :, 3:], np.broadcast_to(full[:, :, 3:4], full[:, :, 3:].shape))

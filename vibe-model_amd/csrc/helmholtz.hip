@@ -18,12 +18,15 @@
 // kernel combines them in a fixed order (run-to-run reproducible), so an
 // iteration is a chain of kernels on one stream; the host reads the residual
 // every few iterations.  The mixed solver runs the same BiCGSTAB in FP32 on
-// the FP64 residual equation (iterative refinement).
+// the FP64 residual equation (iterative refinement); the two-grid solver
+// starts the FP64 BiCGSTAB from the prolonged solution of the 2:1 coarser
+// problem (restricted right-hand side, coefficients and terrain).
 #include <algorithm>
 #include <cmath>
 #include <vector>
 
 #include "common.hpp"
+#include "helmholtz_dev.hpp"
 #include "vibe_mi/helmholtz.h"
 
 namespace vibe_mi {
@@ -287,6 +290,37 @@ __global__ void __launch_bounds__(256) k_fill(size_t n, T v, T *__restrict__ dst
   for (size_t q = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; q < n; q += static_cast<size_t>(gridDim.x) * 256)
     dst[q] = v;
 }
+// multigrid_restrict (helmholtz.cpp:131-145 of the reference): 2:1 full
+// weighting in x and y at every level, fine indices clamped to the fine
+// field (Field::clamp_at); the coarse field's own extents set the loop.
+__global__ void __launch_bounds__(256) k_mg_restrict(const double *__restrict__ f, int fnx, int fny, int fnz,
+                                                     double *__restrict__ c, int cnx, int cny) {
+  const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+  if (i >= cnx) return;
+  const double *fk = f + static_cast<size_t>(clampi(k, fnz)) * fny * fnx;
+  auto at = [&](int ii, int jj) { return fk[static_cast<size_t>(clampi(jj, fny)) * fnx + clampi(ii, fnx)]; };
+  const int fi = 2 * i, fj = 2 * j;
+  double acc = 0.25 * at(fi, fj);
+  acc += 0.125 * (at(fi - 1, fj) + at(fi + 1, fj) + at(fi, fj - 1) + at(fi, fj + 1));
+  acc += 0.0625 * (at(fi - 1, fj - 1) + at(fi + 1, fj - 1) + at(fi - 1, fj + 1) + at(fi + 1, fj + 1));
+  c[(static_cast<size_t>(k) * cny + j) * cnx + i] = acc;
+}
+// multigrid_prolong (helmholtz.cpp:147-164): bilinear interpolation, fine
+// point i at coarse coordinate i/2, coarse indices clamped.
+__global__ void __launch_bounds__(256) k_mg_prolong(const double *__restrict__ c, int cnx, int cny, int cnz,
+                                                    double *__restrict__ f, int fnx, int fny) {
+  const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+  if (i >= fnx) return;
+  const double *ck = c + static_cast<size_t>(clampi(k, cnz)) * cny * cnx;
+  auto at = [&](int ii, int jj) { return ck[static_cast<size_t>(clampi(jj, cny)) * cnx + clampi(ii, cnx)]; };
+  auto lerp = [](double a, double b, double w) { return a + w * (b - a); };
+  const double xs = 0.5 * static_cast<double>(i), ys = 0.5 * static_cast<double>(j);
+  const int i0 = static_cast<int>(floor(xs)), j0 = static_cast<int>(floor(ys));
+  const double wx = xs - static_cast<double>(i0), wy = ys - static_cast<double>(j0);
+  f[(static_cast<size_t>(k) * fny + j) * fnx + i] =
+      lerp(lerp(at(i0, j0), at(i0 + 1, j0), wx), lerp(at(i0, j0 + 1), at(i0 + 1, j0 + 1), wx), wy);
+}
+
 __global__ void k_init_scalars(double *S) {
   if (threadIdx.x == 0) {
     for (int q = 0; q < S_COUNT; ++q) S[q] = 0.0;
@@ -322,6 +356,11 @@ struct vibe_mi_helmholtz {
   double *part = nullptr;  // dot partials
   bool coeffs = false;
   unsigned vgrid = 1;      // grid of the vector kernels
+  // two-grid start of VIBE_MI_HELM_MULTIGRID: the 2:1 coarser operator
+  // (coefficients and terrain restricted), built at its first use after
+  // set_coefficients
+  vibe_mi_helmholtz *coarse = nullptr;
+  bool coarse_ready = false;
 };
 
 namespace vibe_mi {
@@ -377,25 +416,49 @@ int line(vibe_mi_helmholtz *h, const Work<T> &w, const T *r, T *z) {
   return 0;
 }
 
-// BiCGSTAB on A x = rhs (rhs in w.r on entry; x from zero) until
-// sqrt(<r,r>) <= tol * sqrt(<rhs,rhs>) or max_iter iterations; returns the
+// restrict / prolong between two cell fields on `stream`
+int mg_restrict(const double *f, int fnx, int fny, int fnz, double *c, int cnx, int cny, int cnz, hipStream_t st) {
+  hipLaunchKernelGGL(k_mg_restrict, dim3(ceil_div(cnx, 256), cny, cnz), dim3(256), 0, st, f, fnx, fny, fnz, c, cnx,
+                     cny);
+  VMI_HIP(hipGetLastError());
+  return 0;
+}
+int mg_prolong(const double *c, int cnx, int cny, int cnz, double *f, int fnx, int fny, int fnz, hipStream_t st) {
+  hipLaunchKernelGGL(k_mg_prolong, dim3(ceil_div(fnx, 256), fny, fnz), dim3(256), 0, st, c, cnx, cny, cnz, f, fnx,
+                     fny);
+  VMI_HIP(hipGetLastError());
+  return 0;
+}
+
+// BiCGSTAB on A x = rhs (rhs in w.r on entry) until
+// sqrt(<r,r>) <= tol * sqrt(<rhs,rhs>) or max_iter iterations; x starts
+// from zero, or from w.x when from_x (r = rhs - A x first); returns the
 // iterations taken in *its.
 template <typename T>
-int bicgstab(vibe_mi_helmholtz *h, Work<T> &w, double tol, int max_iter, int *its) {
+int bicgstab(vibe_mi_helmholtz *h, Work<T> &w, double tol, int max_iter, int *its, bool from_x = false) {
   const size_t n = h->n;
   const unsigned g = h->vgrid;
   hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, h->stream, w.S);
-  hipLaunchKernelGGL((k_fill<T>), dim3(g), dim3(256), 0, h->stream, n, T(0), w.x);
+  if (!from_x) hipLaunchKernelGGL((k_fill<T>), dim3(g), dim3(256), 0, h->stream, n, T(0), w.x);
   hipLaunchKernelGGL((k_fill<T>), dim3(g), dim3(256), 0, h->stream, n, T(0), w.p);
   hipLaunchKernelGGL((k_fill<T>), dim3(g), dim3(256), 0, h->stream, n, T(0), w.v);
-  VMI_HIP(hipMemcpyAsync(w.rh, w.r, n * sizeof(T), hipMemcpyDeviceToDevice, h->stream));
   if (int rc = dot(h, w.S, w.r, w.r, static_cast<const T *>(nullptr), static_cast<const T *>(nullptr), 3)) return rc;
   double S0[S_COUNT];
   VMI_HIP(hipMemcpyAsync(S0, w.S, sizeof(S0), hipMemcpyDeviceToHost, h->stream));
   VMI_HIP(hipStreamSynchronize(h->stream));
   const double rr0 = S0[S_RR];
   *its = 0;
-  if (!(rr0 > 0.0)) return 0;  // zero right-hand side: x = 0
+  if (!(rr0 > 0.0)) {  // zero right-hand side: x = 0
+    if (from_x) hipLaunchKernelGGL((k_fill<T>), dim3(g), dim3(256), 0, h->stream, n, T(0), w.x);
+    VMI_HIP(hipGetLastError());
+    return 0;
+  }
+  if (from_x) {
+    if (int rc = apply(h, w, static_cast<const T *>(w.x), static_cast<const T *>(w.r), w.t, 1)) return rc;
+    VMI_HIP(hipMemcpyAsync(w.r, w.t, n * sizeof(T), hipMemcpyDeviceToDevice, h->stream));
+    if (int rc = dot(h, w.S, w.r, w.r, static_cast<const T *>(nullptr), static_cast<const T *>(nullptr), 3)) return rc;
+  }
+  VMI_HIP(hipMemcpyAsync(w.rh, w.r, n * sizeof(T), hipMemcpyDeviceToDevice, h->stream));
   const double stop = tol * tol * rr0;
   int it = 0;
   while (it < max_iter) {
@@ -441,17 +504,110 @@ int true_residual(vibe_mi_helmholtz *h, double *rel, double *rnorm) {
   return 0;
 }
 
-}  // namespace
-}  // namespace vibe_mi
+int create_handle(const vibe_mi_helmholtz_desc *ds, bool fp32, vibe_mi_helmholtz **out);
 
-extern "C" {
+// factors the line preconditioner of w64's a, b and (fp32) copies the
+// coefficients and the factorisation to w32; synchronous
+int factor(vibe_mi_helmholtz *h) {
+  Work<double> &w = h->w64;
+  hipLaunchKernelGGL(k_line_factor, dim3(ceil_div(h->nx, 256), h->ny), dim3(256), 0, h->stream, op_of(h, w), w.lo,
+                     w.cp, w.inv);
+  VMI_HIP(hipGetLastError());
+  if (h->w32.a != nullptr) {
+    const unsigned g = h->vgrid;
+    const size_t n = h->n;
+    hipLaunchKernelGGL((k_axpy<float, double>), dim3(g), dim3(256), 0, h->stream, n, 1.0, w.a, h->w32.a, 0);
+    hipLaunchKernelGGL((k_axpy<float, double>), dim3(g), dim3(256), 0, h->stream, n, 1.0, w.b, h->w32.b, 0);
+    hipLaunchKernelGGL((k_axpy<float, double>), dim3(g), dim3(256), 0, h->stream, n, 1.0, w.lo, h->w32.lo, 0);
+    hipLaunchKernelGGL((k_axpy<float, double>), dim3(g), dim3(256), 0, h->stream, n, 1.0, w.cp, h->w32.cp, 0);
+    hipLaunchKernelGGL((k_axpy<float, double>), dim3(g), dim3(256), 0, h->stream, n, 1.0, w.inv, h->w32.inv, 0);
+    VMI_HIP(hipGetLastError());
+  }
+  VMI_HIP(hipStreamSynchronize(h->stream));
+  h->coeffs = true;
+  h->coarse_ready = false;
+  return 0;
+}
 
-int vibe_mi_helmholtz_create(const vibe_mi_helmholtz_desc *ds, vibe_mi_helmholtz **out) {
-  if (ds == nullptr || out == nullptr || ds->nx <= 0 || ds->ny <= 0 || ds->nz <= 0 || !(ds->dx > 0) ||
-      !(ds->dy > 0) || ds->zeta == nullptr || ds->zs == nullptr)
-    return VIBE_C_ERR_BAD_ARG;
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return VIBE_MI_ERR_NO_DEVICE;
+// the coarse operator of the two-grid start: (nx+1)/2 x (ny+1)/2 x nz cells
+// of spacing 2 dx, 2 dy (coarse cell i at fine cell 2i), the same levels,
+// terrain, a and b restricted from the fine ones
+int build_coarse(vibe_mi_helmholtz *h) {
+  if (h->coarse == nullptr) {
+    vibe_mi_helmholtz_desc cd{};
+    cd.nx = (h->nx + 1) / 2;
+    cd.ny = (h->ny + 1) / 2;
+    cd.nz = h->nz;
+    cd.dx = 2.0 * h->dx;
+    cd.dy = 2.0 * h->dy;
+    cd.ztop = h->ztop;
+    if (int rc = create_handle(&cd, false, &h->coarse)) return rc;
+  }
+  vibe_mi_helmholtz *c = h->coarse;
+  VMI_HIP(hipStreamSynchronize(h->stream));
+  VMI_HIP(hipMemcpyAsync(c->w64.dzeta, h->w64.dzeta, h->nz * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  if (int rc = mg_restrict(h->w64.zs, h->nx, h->ny, 1, c->w64.zs, c->nx, c->ny, 1, c->stream)) return rc;
+  if (int rc = mg_restrict(h->w64.a, h->nx, h->ny, h->nz, c->w64.a, c->nx, c->ny, c->nz, c->stream)) return rc;
+  if (int rc = mg_restrict(h->w64.b, h->nx, h->ny, h->nz, c->w64.b, c->nx, c->ny, c->nz, c->stream)) return rc;
+  if (int rc = factor(c)) return rc;
+  h->coarse_ready = true;
+  return 0;
+}
+
+// Solves A x = f64 into w64.x; iterations and the final FP64 relative
+// residual in *total / *rel.
+int solve_core(vibe_mi_helmholtz *h, int kind, double tol, int max_iter, int *total_out, double *rel_out) {
+  const size_t n = h->n;
+  const unsigned g = h->vgrid;
+  Work<double> &w = h->w64;
+  int total = 0;
+  double rel = 0.0, rn = 0.0;
+  if (kind == VIBE_MI_HELM_BICGSTAB) {
+    VMI_HIP(hipMemcpyAsync(w.r, h->f64, n * 8, hipMemcpyDeviceToDevice, h->stream));
+    if (int rc = bicgstab(h, w, tol, max_iter, &total)) return rc;
+    if (int rc = true_residual(h, &rel, &rn)) return rc;
+  } else if (kind == VIBE_MI_HELM_MULTIGRID) {
+    // two-grid start: restrict f, solve the coarse problem to the looser of
+    // tol and 1e-3, prolong it as the initial guess of the fine solve
+    if (!h->coarse_ready)
+      if (int rc = build_coarse(h)) return rc;
+    vibe_mi_helmholtz *c = h->coarse;
+    VMI_HIP(hipStreamSynchronize(h->stream));
+    if (int rc = mg_restrict(h->f64, h->nx, h->ny, h->nz, c->w64.r, c->nx, c->ny, c->nz, c->stream)) return rc;
+    int its_c = 0;
+    if (int rc = bicgstab(c, c->w64, std::max(tol, 1e-3), max_iter, &its_c)) return rc;
+    if (int rc = mg_prolong(c->w64.x, c->nx, c->ny, c->nz, w.x, h->nx, h->ny, h->nz, c->stream)) return rc;
+    VMI_HIP(hipStreamSynchronize(c->stream));
+    VMI_HIP(hipMemcpyAsync(w.r, h->f64, n * 8, hipMemcpyDeviceToDevice, h->stream));
+    if (int rc = bicgstab(h, w, tol, max_iter, &total, true)) return rc;
+    if (int rc = true_residual(h, &rel, &rn)) return rc;
+  } else {
+    // iterative refinement: FP32 solves of A dx = r (r scaled to unit norm),
+    // FP64 residuals and accumulation
+    Work<float> &v = h->w32;
+    hipLaunchKernelGGL((k_fill<double>), dim3(g), dim3(256), 0, h->stream, n, 0.0, w.x);
+    VMI_HIP(hipGetLastError());
+    const double tol_in = tol > 1e-5 ? tol : 1e-5;
+    for (int outer = 0; outer < 30; ++outer) {
+      if (int rc = true_residual(h, &rel, &rn)) return rc;
+      if (rel < tol || total >= max_iter || !(rn > 0.0) || !std::isfinite(rel)) break;
+      hipLaunchKernelGGL((k_axpy<float, double>), dim3(g), dim3(256), 0, h->stream, n, 1.0 / rn, w.r, v.r, 0);
+      VMI_HIP(hipGetLastError());
+      int its = 0;
+      if (int rc = bicgstab(h, v, tol_in, max_iter - total, &its)) return rc;
+      total += its;
+      hipLaunchKernelGGL((k_axpy<double, float>), dim3(g), dim3(256), 0, h->stream, n, rn, v.x, w.x, 1);
+      VMI_HIP(hipGetLastError());
+      if (its == 0) break;
+    }
+    if (int rc = true_residual(h, &rel, &rn)) return rc;
+  }
+  *total_out = total;
+  *rel_out = rel;
+  return 0;
+}
+
+int create_handle(const vibe_mi_helmholtz_desc *ds, bool fp32, vibe_mi_helmholtz **out) {
   auto *h = new vibe_mi_helmholtz();
   h->nx = ds->nx;
   h->ny = ds->ny;
@@ -467,25 +623,74 @@ int vibe_mi_helmholtz_create(const vibe_mi_helmholtz_desc *ds, vibe_mi_helmholtz
   if (hipGetDevice(&h->device) != hipSuccess) return fail(VIBE_MI_ERR_HIP);
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return fail(VIBE_MI_ERR_HIP);
   if (int rc = alloc_work(h, h->w64)) return fail(rc);
-  if (int rc = alloc_work(h, h->w32)) return fail(rc);
+  if (fp32)
+    if (int rc = alloc_work(h, h->w32)) return fail(rc);
   if (int rc = halloc(h, &h->f64, h->n)) return fail(rc);
   if (int rc = halloc(h, &h->part, static_cast<size_t>(2 * kDotBlocks))) return fail(rc);
   h->vgrid = static_cast<unsigned>(std::min<size_t>(ceil_div(h->n, 256), 8192));
-  const size_t nxy = static_cast<size_t>(h->nx) * h->ny;
-  std::vector<double> dz(h->nz);
-  for (int k = 0; k < h->nz; ++k) dz[k] = ds->zeta[k + 1] - ds->zeta[k];
-  std::vector<float> dz32(dz.begin(), dz.end()), zs32(ds->zs, ds->zs + nxy);
-  if (hipMemcpy(h->w64.dzeta, dz.data(), dz.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(h->w64.zs, ds->zs, nxy * 8, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(h->w32.dzeta, dz32.data(), dz32.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(h->w32.zs, zs32.data(), nxy * 4, hipMemcpyHostToDevice) != hipSuccess)
-    return fail(VIBE_MI_ERR_HIP);
+  if (ds->zeta != nullptr && ds->zs != nullptr) {
+    const size_t nxy = static_cast<size_t>(h->nx) * h->ny;
+    std::vector<double> dz(h->nz);
+    for (int k = 0; k < h->nz; ++k) dz[k] = ds->zeta[k + 1] - ds->zeta[k];
+    std::vector<float> dz32(dz.begin(), dz.end()), zs32(ds->zs, ds->zs + nxy);
+    if (hipMemcpy(h->w64.dzeta, dz.data(), dz.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->w64.zs, ds->zs, nxy * 8, hipMemcpyHostToDevice) != hipSuccess)
+      return fail(VIBE_MI_ERR_HIP);
+    if (fp32 && (hipMemcpy(h->w32.dzeta, dz32.data(), dz32.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+                 hipMemcpy(h->w32.zs, zs32.data(), nxy * 4, hipMemcpyHostToDevice) != hipSuccess))
+      return fail(VIBE_MI_ERR_HIP);
+  }
   *out = h;
   return 0;
 }
 
+bool valid_kind(int kind) {
+  return kind == VIBE_MI_HELM_BICGSTAB || kind == VIBE_MI_HELM_REFINE || kind == VIBE_MI_HELM_MULTIGRID;
+}
+
+}  // namespace
+
+// Device-array forms for in-library callers (helmholtz_dev.hpp).
+int helmholtz_set_coefficients_device(vibe_mi_helmholtz *h, const double *a, const double *b) {
+  if (h == nullptr || a == nullptr || b == nullptr) return VIBE_C_ERR_BAD_ARG;
+  VMI_HIP(hipSetDevice(h->device));
+  VMI_HIP(hipMemcpyAsync(h->w64.a, a, h->n * 8, hipMemcpyDeviceToDevice, h->stream));
+  VMI_HIP(hipMemcpyAsync(h->w64.b, b, h->n * 8, hipMemcpyDeviceToDevice, h->stream));
+  return factor(h);
+}
+
+int helmholtz_solve_device(vibe_mi_helmholtz *h, const double *f, double *x, int kind, double tol, int max_iter,
+                           int *iterations, double *rel_res) {
+  if (h == nullptr || f == nullptr || x == nullptr || !h->coeffs || !(tol > 0.0) || max_iter < 0 || !valid_kind(kind))
+    return VIBE_C_ERR_BAD_ARG;
+  VMI_HIP(hipSetDevice(h->device));
+  VMI_HIP(hipMemcpyAsync(h->f64, f, h->n * 8, hipMemcpyDeviceToDevice, h->stream));
+  int total = 0;
+  double rel = 0.0;
+  if (int rc = solve_core(h, kind, tol, max_iter, &total, &rel)) return rc;
+  VMI_HIP(hipMemcpyAsync(x, h->w64.x, h->n * 8, hipMemcpyDeviceToDevice, h->stream));
+  VMI_HIP(hipStreamSynchronize(h->stream));
+  if (iterations) *iterations = total;
+  if (rel_res) *rel_res = rel;
+  return 0;
+}
+
+}  // namespace vibe_mi
+
+extern "C" {
+
+int vibe_mi_helmholtz_create(const vibe_mi_helmholtz_desc *ds, vibe_mi_helmholtz **out) {
+  if (ds == nullptr || out == nullptr || ds->nx <= 0 || ds->ny <= 0 || ds->nz <= 0 || !(ds->dx > 0) ||
+      !(ds->dy > 0) || ds->zeta == nullptr || ds->zs == nullptr || ds->ny > 65535 || ds->nz > 65535)
+    return VIBE_C_ERR_BAD_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return VIBE_MI_ERR_NO_DEVICE;
+  return create_handle(ds, true, out);
+}
+
 int vibe_mi_helmholtz_destroy(vibe_mi_helmholtz *h) {
   if (h == nullptr) return 0;
+  if (h->coarse != nullptr) vibe_mi_helmholtz_destroy(h->coarse);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   for (void *p : h->allocs) (void)hipFree(p);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -501,21 +706,7 @@ int vibe_mi_helmholtz_set_coefficients(vibe_mi_helmholtz *h, const double *a, co
   const size_t n = h->n;
   VMI_HIP(hipMemcpy(h->w64.a, a, n * 8, hipMemcpyHostToDevice));
   VMI_HIP(hipMemcpy(h->w64.b, b, n * 8, hipMemcpyHostToDevice));
-  Work<double> &w = h->w64;
-  hipLaunchKernelGGL(k_line_factor, dim3(ceil_div(h->nx, 256), h->ny), dim3(256), 0, h->stream, op_of(h, w), w.lo,
-                     w.cp, w.inv);
-  VMI_HIP(hipGetLastError());
-  const unsigned g = h->vgrid;
-  // FP32 copies of the coefficients and of the FP64 factorisation
-  hipLaunchKernelGGL((k_axpy<float, double>), dim3(g), dim3(256), 0, h->stream, n, 1.0, w.a, h->w32.a, 0);
-  hipLaunchKernelGGL((k_axpy<float, double>), dim3(g), dim3(256), 0, h->stream, n, 1.0, w.b, h->w32.b, 0);
-  hipLaunchKernelGGL((k_axpy<float, double>), dim3(g), dim3(256), 0, h->stream, n, 1.0, w.lo, h->w32.lo, 0);
-  hipLaunchKernelGGL((k_axpy<float, double>), dim3(g), dim3(256), 0, h->stream, n, 1.0, w.cp, h->w32.cp, 0);
-  hipLaunchKernelGGL((k_axpy<float, double>), dim3(g), dim3(256), 0, h->stream, n, 1.0, w.inv, h->w32.inv, 0);
-  VMI_HIP(hipGetLastError());
-  VMI_HIP(hipStreamSynchronize(h->stream));
-  h->coeffs = true;
-  return 0;
+  return factor(h);
 }
 
 int vibe_mi_helmholtz_apply(vibe_mi_helmholtz *h, const double *x, double *y) {
@@ -542,46 +733,34 @@ int vibe_mi_helmholtz_residual(vibe_mi_helmholtz *h, const double *f, const doub
 
 int vibe_mi_helmholtz_solve(vibe_mi_helmholtz *h, const double *f, double *x, int kind, double tol, int max_iter,
                             int *iterations, double *rel_res) {
-  if (h == nullptr || f == nullptr || x == nullptr || !h->coeffs || !(tol > 0.0) || max_iter < 0 ||
-      (kind != VIBE_MI_HELM_BICGSTAB && kind != VIBE_MI_HELM_REFINE))
+  if (h == nullptr || f == nullptr || x == nullptr || !h->coeffs || !(tol > 0.0) || max_iter < 0 || !valid_kind(kind))
     return VIBE_C_ERR_BAD_ARG;
   VMI_HIP(hipSetDevice(h->device));
-  const size_t n = h->n;
-  const unsigned g = h->vgrid;
-  Work<double> &w = h->w64;
-  VMI_HIP(hipMemcpyAsync(h->f64, f, n * 8, hipMemcpyHostToDevice, h->stream));
+  VMI_HIP(hipMemcpyAsync(h->f64, f, h->n * 8, hipMemcpyHostToDevice, h->stream));
   int total = 0;
-  double rel = 0.0, rn = 0.0;
-  if (kind == VIBE_MI_HELM_BICGSTAB) {
-    VMI_HIP(hipMemcpyAsync(w.r, h->f64, n * 8, hipMemcpyDeviceToDevice, h->stream));
-    if (int rc = bicgstab(h, w, tol, max_iter, &total)) return rc;
-    if (int rc = true_residual(h, &rel, &rn)) return rc;
-  } else {
-    // iterative refinement: FP32 solves of A dx = r (r scaled to unit norm),
-    // FP64 residuals and accumulation
-    Work<float> &v = h->w32;
-    hipLaunchKernelGGL((k_fill<double>), dim3(g), dim3(256), 0, h->stream, n, 0.0, w.x);
-    VMI_HIP(hipGetLastError());
-    const double tol_in = tol > 1e-5 ? tol : 1e-5;
-    for (int outer = 0; outer < 30; ++outer) {
-      if (int rc = true_residual(h, &rel, &rn)) return rc;
-      if (rel < tol || total >= max_iter || !(rn > 0.0) || !std::isfinite(rel)) break;
-      hipLaunchKernelGGL((k_axpy<float, double>), dim3(g), dim3(256), 0, h->stream, n, 1.0 / rn, w.r, v.r, 0);
-      VMI_HIP(hipGetLastError());
-      int its = 0;
-      if (int rc = bicgstab(h, v, tol_in, max_iter - total, &its)) return rc;
-      total += its;
-      hipLaunchKernelGGL((k_axpy<double, float>), dim3(g), dim3(256), 0, h->stream, n, rn, v.x, w.x, 1);
-      VMI_HIP(hipGetLastError());
-      if (its == 0) break;
-    }
-    if (int rc = true_residual(h, &rel, &rn)) return rc;
-  }
-  VMI_HIP(hipMemcpyAsync(x, w.x, n * 8, hipMemcpyDeviceToHost, h->stream));
+  double rel = 0.0;
+  if (int rc = solve_core(h, kind, tol, max_iter, &total, &rel)) return rc;
+  VMI_HIP(hipMemcpyAsync(x, h->w64.x, h->n * 8, hipMemcpyDeviceToHost, h->stream));
   VMI_HIP(hipStreamSynchronize(h->stream));
   if (iterations) *iterations = total;
   if (rel_res) *rel_res = rel;
   return 0;
+}
+
+int vibe_mi_multigrid_restrict_f64(const double *fine, int fnx, int fny, int fnz, double *coarse, int cnx, int cny,
+                                   int cnz, void *stream) {
+  if (fine == nullptr || coarse == nullptr || fnx <= 0 || fny <= 0 || fnz <= 0 || cnx <= 0 || cny <= 0 || cnz <= 0 ||
+      cny > 65535 || cnz > 65535)
+    return VIBE_C_ERR_BAD_ARG;
+  return mg_restrict(fine, fnx, fny, fnz, coarse, cnx, cny, cnz, static_cast<hipStream_t>(stream));
+}
+
+int vibe_mi_multigrid_prolong_f64(const double *coarse, int cnx, int cny, int cnz, double *fine, int fnx, int fny,
+                                  int fnz, void *stream) {
+  if (fine == nullptr || coarse == nullptr || fnx <= 0 || fny <= 0 || fnz <= 0 || cnx <= 0 || cny <= 0 || cnz <= 0 ||
+      fny > 65535 || fnz > 65535)
+    return VIBE_C_ERR_BAD_ARG;
+  return mg_prolong(coarse, cnx, cny, cnz, fine, fnx, fny, fnz, static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "literal_dev.hpp"
 #include "vibe_mi/literal.h"
 
 namespace vibe_mi {
@@ -337,6 +338,48 @@ Fields fields_of(vibe_mi_literal *h) {
 
 }  // namespace
 
+namespace vibe_mi {
+
+LiteralView literal_view(vibe_mi_literal *h) {
+  LiteralView v{};
+  const Lit &g = h->g;
+  v.nx = g.nx;
+  v.ny = g.ny;
+  v.nz = g.nz;
+  v.dx = g.dx;
+  v.dy = g.dy;
+  v.ztop = g.ztop;
+  v.zeta = g.zeta;
+  v.zs = g.zs;
+  v.rho0 = g.r0;
+  v.theta0 = g.t0;
+  v.pi0 = g.p0;
+  for (int q = 0; q < 12; ++q) {
+    v.s[q] = h->s[q];
+    v.t[q] = h->t[q];
+    v.n[q] = h->n[q];
+  }
+  v.stream = h->stream;
+  return v;
+}
+
+int literal_tendencies_device(vibe_mi_literal *h) {
+  const Lit &g = h->g;
+  const Fields F = fields_of(h);
+  hipStream_t st = h->stream;
+  hipLaunchKernelGGL(k_lit_u, dim3(ceil_div(g.nx + 1, 64), g.ny, g.nz), dim3(64), 0, st, g, F);
+  hipLaunchKernelGGL(k_lit_v, dim3(ceil_div(g.nx, 64), g.ny + 1, g.nz), dim3(64), 0, st, g, F);
+  hipLaunchKernelGGL(k_lit_w, dim3(ceil_div(g.nx, 64), g.ny, g.nz + 1), dim3(64), 0, st, g, F);
+  hipLaunchKernelGGL(k_lit_cells, dim3(ceil_div(g.nx, 64), g.ny, g.nz), dim3(64), 0, st, g, F, h->lap);
+  if (g.dcoef > 0.0 && g.dord > 2)
+    hipLaunchKernelGGL(k_lit_dpi4, dim3(ceil_div(g.nx, 64), g.ny, g.nz), dim3(64), 0, st, g,
+                       static_cast<const double *>(h->lap), h->t[5]);
+  VMI_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace vibe_mi
+
 extern "C" {
 
 int vibe_mi_literal_create(const vibe_mi_literal_desc *d, vibe_mi_literal **out) {
@@ -417,17 +460,7 @@ int vibe_mi_literal_destroy(vibe_mi_literal *h) {
 int vibe_mi_literal_tendencies(vibe_mi_literal *h, const double *const s[12], double *const t[12]) {
   if (h == nullptr || s == nullptr || t == nullptr) return VIBE_C_ERR_BAD_ARG;
   if (int rc = upload_state(h, s)) return rc;
-  const Lit &g = h->g;
-  const Fields F = fields_of(h);
-  hipStream_t st = h->stream;
-  hipLaunchKernelGGL(k_lit_u, dim3(ceil_div(g.nx + 1, 64), g.ny, g.nz), dim3(64), 0, st, g, F);
-  hipLaunchKernelGGL(k_lit_v, dim3(ceil_div(g.nx, 64), g.ny + 1, g.nz), dim3(64), 0, st, g, F);
-  hipLaunchKernelGGL(k_lit_w, dim3(ceil_div(g.nx, 64), g.ny, g.nz + 1), dim3(64), 0, st, g, F);
-  hipLaunchKernelGGL(k_lit_cells, dim3(ceil_div(g.nx, 64), g.ny, g.nz), dim3(64), 0, st, g, F, h->lap);
-  if (g.dcoef > 0.0 && g.dord > 2)
-    hipLaunchKernelGGL(k_lit_dpi4, dim3(ceil_div(g.nx, 64), g.ny, g.nz), dim3(64), 0, st, g,
-                       static_cast<const double *>(h->lap), h->t[5]);
-  VMI_HIP(hipGetLastError());
+  if (int rc = vibe_mi::literal_tendencies_device(h)) return rc;
   return download(h, t);
 }
 

@@ -97,6 +97,25 @@ class HelmholtzDesc(ctypes.Structure):
                 ("zeta", _PD), ("zs", _PD)]
 
 
+class LiteralDesc(ctypes.Structure):
+    """vibe_mi_literal_desc (vibe_mi/literal.h)."""
+    _fields_ = [("nx", _I), ("ny", _I), ("nz", _I), ("dx", _D), ("dy", _D), ("z_top", _D), ("zeta", _PD),
+                ("zs", _PD), ("rho0", _PD), ("theta0", _PD), ("pi0", _PD), ("order", _I), ("coriolis", _I),
+                ("latitude", _D), ("div_coeff", _D), ("div_order", _I), ("moist", _I)]
+
+
+class SiDesc(ctypes.Structure):
+    """vibe_mi_si_desc (vibe_mi/semi_implicit.h)."""
+    _fields_ = [("eq", LiteralDesc), ("sponge", _PD), ("solver", _I), ("tol", _D), ("max_iter", _I),
+                ("time_filter", _D)]
+
+
+class SiStats(ctypes.Structure):
+    """vibe_mi_si_stats (IntegratorStats, integrator.hpp:53-59)."""
+    _fields_ = [("steps", ctypes.c_longlong), ("acoustic_substeps", ctypes.c_longlong), ("wall_time", _D),
+                ("helmholtz_iterations", _I), ("helmholtz_residual", _D), ("diverged", _I)]
+
+
 class NestDesc(ctypes.Structure):
     """vibe_mi_nest_desc (vibe_mi/nest.h)."""
     _fields_ = [("ratio", _I), ("i0", _I), ("j0", _I), ("zone", _I), ("alpha_max", _D)]
@@ -171,6 +190,19 @@ def lib() -> ctypes.CDLL:
         "vibe_mi_helmholtz_apply": (_I, [_VP, _PD, _PD]),
         "vibe_mi_helmholtz_residual": (_I, [_VP, _PD, _PD, _PD]),
         "vibe_mi_helmholtz_solve": (_I, [_VP, _PD, _PD, _I, _D, _I, ctypes.POINTER(_I), _PD]),
+        "vibe_mi_multigrid_restrict_f64": (_I, [_VP, _I, _I, _I, _VP, _I, _I, _I, _VP]),
+        "vibe_mi_si_create": (_I, [ctypes.POINTER(SiDesc), ctypes.POINTER(_VP)]),
+        "vibe_mi_si_destroy": (_I, [_VP]),
+        "vibe_mi_si_set_state": (_I, [_VP, ctypes.POINTER(_PD)]),
+        "vibe_mi_si_get_state": (_I, [_VP, ctypes.POINTER(_PD)]),
+        "vibe_mi_si_set_physics_tendency": (_I, [_VP, _PD, _PD, _PD, _PD]),
+        "vibe_mi_si_step": (_I, [_VP, _D, _I]),
+        "vibe_mi_si_get_stats": (_I, [_VP, ctypes.POINTER(SiStats)]),
+        "vibe_mi_si_reset_stats": (_I, [_VP]),
+        "vibe_mi_si_max_stable_dt": (_I, [_VP, _PD]),
+        "vibe_mi_si_rhs": (_I, [_VP, _D, _PD, _PD]),
+        "vibe_mi_si_pressure": (_I, [_VP, _PD]),
+        "vibe_mi_multigrid_prolong_f64": (_I, [_VP, _I, _I, _I, _VP, _I, _I, _I, _VP]),
         "vibe_mi_zeta_uniform": (_I, [_I, _PD]),
         "vibe_mi_zeta_tanh": (_I, [_I, _D, _D, _D, _PD]),
         "vibe_mi_terrain_agnesi_ridge": (_I, [_I, _I, _D, _D, _D, _PD]),
@@ -793,6 +825,35 @@ def batch_nested_step(batch: "Batch", links, two_way: bool, dt0: float, nsub: in
 
 HELM_BICGSTAB = 0
 HELM_REFINE = 1
+HELM_MULTIGRID = 2
+
+
+def multigrid_restrict(fine, coarse):
+    """multigrid_restrict (helmholtz.cpp:131-145) of the device tensor fine
+    [nz, ny, nx] into the device tensor coarse (float64, contiguous, on the
+    current device; enqueued on the null stream)."""
+    _check_dev3(fine, coarse)
+    fz, fy, fx = fine.shape
+    cz, cy, cx = coarse.shape
+    _check(lib().vibe_mi_multigrid_restrict_f64(fine.data_ptr(), fx, fy, fz, coarse.data_ptr(), cx, cy, cz, None),
+           "vibe_mi_multigrid_restrict_f64")
+
+
+def multigrid_prolong(coarse, fine):
+    """multigrid_prolong (helmholtz.cpp:147-164) of the device tensor coarse
+    into the device tensor fine (as multigrid_restrict)."""
+    _check_dev3(fine, coarse)
+    fz, fy, fx = fine.shape
+    cz, cy, cx = coarse.shape
+    _check(lib().vibe_mi_multigrid_prolong_f64(coarse.data_ptr(), cx, cy, cz, fine.data_ptr(), fx, fy, fz, None),
+           "vibe_mi_multigrid_prolong_f64")
+
+
+def _check_dev3(*ts):
+    import torch
+    for t in ts:
+        if t.dtype != torch.float64 or t.dim() != 3 or not t.is_contiguous() or not t.is_cuda:
+            raise ValueError("expected contiguous float64 device tensors of shape [nz, ny, nx]")
 
 
 class Helmholtz:
@@ -845,6 +906,122 @@ class Helmholtz:
     def close(self):
         if getattr(self, "h", None):
             lib().vibe_mi_helmholtz_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+SPECIES = ("u", "v", "w", "rho", "theta", "pi", "qv", "qc", "qr", "qi", "qs", "qg")
+
+
+class SemiImplicit:
+    """The reference's semi-implicit integrator on the GPU
+    (SemiImplicitIntegrator, src/time/semi_implicit.cpp:42-233; C ABI
+    vibe_mi/semi_implicit.h) over its literal equations: FULL prognostic
+    values of the 12 species in the reference's State order, bounded x and
+    y.  The device copy of the state is authoritative between calls.
+
+    grid: nx, ny, nz, dx, dy, z_top, zeta (nz+1), zs (ny*nx or None);
+    reference state rho0, theta0, pi0 (cells); numerics: advection order
+    (2/4/6), coriolis (0/1/2), latitude, div_coeff, div_order, moist;
+    sponge (nz rates or None); solver (HELM_*), tol, max_iter;
+    time_filter (set_time_filter alpha)."""
+
+    def __init__(self, nx, ny, nz, dx, dy, z_top, zeta, rho0, theta0, pi0, zs=None, order=2, coriolis=0,
+                 latitude=45.0, div_coeff=0.0, div_order=2, moist=0, sponge=None, solver=HELM_BICGSTAB,
+                 tol=1e-8, max_iter=200, time_filter=0.0):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        nc = nx * ny * nz
+        self._keep = [np.ascontiguousarray(zeta, dtype=np.float64)]
+        if self._keep[0].size != nz + 1:
+            raise ValueError("zeta needs nz+1 values")
+        ref = []
+        for a in (rho0, theta0, pi0):
+            a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+            if a.size != nc:
+                raise ValueError("reference state arrays need nx*ny*nz values")
+            ref.append(a)
+        self._keep += ref
+        zsa = None
+        if zs is not None:
+            zsa = np.ascontiguousarray(zs, dtype=np.float64).reshape(-1)
+            if zsa.size != nx * ny:
+                raise ValueError("zs needs nx*ny values")
+            self._keep.append(zsa)
+        sp = None
+        if sponge is not None:
+            sp = np.ascontiguousarray(sponge, dtype=np.float64).reshape(-1)
+            if sp.size != nz:
+                raise ValueError("sponge needs nz rates")
+            self._keep.append(sp)
+        eq = LiteralDesc(nx, ny, nz, dx, dy, z_top, _pd(self._keep[0]), _pd(zsa), _pd(ref[0]), _pd(ref[1]),
+                         _pd(ref[2]), order, coriolis, latitude, div_coeff, div_order, moist)
+        self.desc = SiDesc(eq, _pd(sp), solver, tol, max_iter, time_filter)
+        h = _VP()
+        _check(lib().vibe_mi_si_create(ctypes.byref(self.desc), ctypes.byref(h)), "vibe_mi_si_create")
+        self.h = h
+
+    def sizes(self):
+        nx, ny, nz = self.nx, self.ny, self.nz
+        c = nx * ny * nz
+        return [(nx + 1) * ny * nz, nx * (ny + 1) * nz, nx * ny * (nz + 1)] + [c] * 9
+
+    def _ptrs(self, arrs):
+        return (_PD * 12)(*[_pd(a) for a in arrs])
+
+    def set_state(self, s):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in s]
+        if [a.size for a in arrs] != self.sizes():
+            raise ValueError("state sizes do not match the grid")
+        _check(lib().vibe_mi_si_set_state(self.h, self._ptrs(arrs)), "vibe_mi_si_set_state")
+
+    def get_state(self):
+        out = [np.empty(n) for n in self.sizes()]
+        _check(lib().vibe_mi_si_get_state(self.h, self._ptrs(out)), "vibe_mi_si_get_state")
+        return out
+
+    def set_physics_tendency(self, theta=None, qv=None, u=None, v=None):
+        self._phys = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+                      for a in (theta, qv, u, v)]
+        _check(lib().vibe_mi_si_set_physics_tendency(self.h, *[_pd(a) for a in self._phys]),
+               "vibe_mi_si_set_physics_tendency")
+
+    def step(self, dt, nsteps=1):
+        _check(lib().vibe_mi_si_step(self.h, dt, nsteps), "vibe_mi_si_step")
+
+    def stats(self) -> dict:
+        st = SiStats()
+        _check(lib().vibe_mi_si_get_stats(self.h, ctypes.byref(st)), "vibe_mi_si_get_stats")
+        return {f: getattr(st, f) for f, _ in SiStats._fields_}
+
+    def reset_stats(self):
+        _check(lib().vibe_mi_si_reset_stats(self.h), "vibe_mi_si_reset_stats")
+
+    def max_stable_dt(self) -> float:
+        r = _D()
+        _check(lib().vibe_mi_si_max_stable_dt(self.h, ctypes.byref(r)), "vibe_mi_si_max_stable_dt")
+        return r.value
+
+    def rhs(self, dt):
+        """(a, rhs) of assemble_rhs for the device state."""
+        c = self.nx * self.ny * self.nz
+        a, r = np.empty(c), np.empty(c)
+        _check(lib().vibe_mi_si_rhs(self.h, dt, _pd(a), _pd(r)), "vibe_mi_si_rhs")
+        return a, r
+
+    def pressure(self):
+        """pi_new of the last step's Helmholtz solve."""
+        x = np.empty(self.nx * self.ny * self.nz)
+        _check(lib().vibe_mi_si_pressure(self.h, _pd(x)), "vibe_mi_si_pressure")
+        return x
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vibe_mi_si_destroy(self.h)
             self.h = None
 
     def __del__(self):
