@@ -80,6 +80,8 @@ int vibe_mi_si_get_state(vibe_mi_si *h, double *const s[12]);
  * switches the coupling off. */
 int vibe_mi_si_set_physics_tendency(vibe_mi_si *h, const double *theta, const double *qv, const double *u,
                                     const double *v);
+/* set_time_filter (semi_implicit.hpp:65): alpha >= 0, 0 switches it off. */
+int vibe_mi_si_set_time_filter(vibe_mi_si *h, double alpha);
 /* nsteps steps of dt on the device state. */
 int vibe_mi_si_step(vibe_mi_si *h, double dt, int nsteps);
 int vibe_mi_si_get_stats(vibe_mi_si *h, vibe_mi_si_stats *out);

@@ -397,6 +397,12 @@ int vibe_mi_si_set_physics_tendency(vibe_mi_si *h, const double *theta, const do
   return 0;
 }
 
+int vibe_mi_si_set_time_filter(vibe_mi_si *h, double alpha) {
+  if (h == nullptr || !(alpha >= 0.0)) return VIBE_C_ERR_BAD_ARG;
+  h->filter = alpha;
+  return 0;
+}
+
 int vibe_mi_si_step(vibe_mi_si *h, double dt, int nsteps) {
   if (h == nullptr || !(dt > 0.0) || nsteps < 0) return VIBE_C_ERR_BAD_ARG;
   const auto t0 = std::chrono::steady_clock::now();

@@ -197,6 +197,7 @@ def lib() -> ctypes.CDLL:
         "vibe_mi_si_get_state": (_I, [_VP, ctypes.POINTER(_PD)]),
         "vibe_mi_si_set_physics_tendency": (_I, [_VP, _PD, _PD, _PD, _PD]),
         "vibe_mi_si_step": (_I, [_VP, _D, _I]),
+        "vibe_mi_si_set_time_filter": (_I, [_VP, _D]),
         "vibe_mi_si_get_stats": (_I, [_VP, ctypes.POINTER(SiStats)]),
         "vibe_mi_si_reset_stats": (_I, [_VP]),
         "vibe_mi_si_max_stable_dt": (_I, [_VP, _PD]),
@@ -992,6 +993,9 @@ class SemiImplicit:
 
     def step(self, dt, nsteps=1):
         _check(lib().vibe_mi_si_step(self.h, dt, nsteps), "vibe_mi_si_step")
+
+    def set_time_filter(self, alpha):
+        _check(lib().vibe_mi_si_set_time_filter(self.h, alpha), "vibe_mi_si_set_time_filter")
 
     def stats(self) -> dict:
         st = SiStats()
