@@ -55,12 +55,37 @@ __device__ __forceinline__ int clampi(int i, int n) { return i < 0 ? 0 : (i >= n
 // E/W/N/S neighbours of level k come from the same or adjacent rows of the
 // tile (cache hits for all but the tile's edge rows).  Same operations in the
 // same order as the reference's loop body.
+//
+// With part != nullptr the kernel also forms the block's partial sums of
+// <d, y> and (tt) <y, y> for the BiCGSTAB scalars, part[2 * block + 0/1]
+// (block = blockIdx.y * gridDim.x + blockIdx.x), so no separate pass reads
+// y again.
 constexpr int kApplyRows = 4;
+__device__ __forceinline__ void block_partials(double s0, double s1, double *__restrict__ part, int blk) {
+  __shared__ double sh[2][256];
+  const int t = static_cast<int>(threadIdx.y * blockDim.x + threadIdx.x);
+  sh[0][t] = s0;
+  sh[1][t] = s1;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      sh[0][t] += sh[0][t + w];
+      sh[1][t] += sh[1][t + w];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    part[2 * blk] = sh[0][0];
+    part[2 * blk + 1] = sh[1][0];
+  }
+}
 template <typename T>
 __global__ void __launch_bounds__(64 * kApplyRows) k_apply(Op<T> o, const T *__restrict__ x, const T *__restrict__ f,
-                                                           T *__restrict__ y, int mode) {
+                                                           T *__restrict__ y, int mode, const T *__restrict__ d,
+                                                           int tt, double *__restrict__ part) {
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * kApplyRows + threadIdx.y;
-  if (i >= o.nx || j >= o.ny) return;
+  double s0 = 0.0, s1 = 0.0;
+  if (i < o.nx && j < o.ny) {
   const size_t nxy = static_cast<size_t>(o.nx) * o.ny;
   const size_t col = static_cast<size_t>(j) * o.nx + i;
   const size_t cE = static_cast<size_t>(j) * o.nx + clampi(i + 1, o.nx);
@@ -107,7 +132,12 @@ __global__ void __launch_bounds__(64 * kApplyRows) k_apply(Op<T> o, const T *__r
     const T fy = (an * (xN - xc) - as * (xc - xS)) / o.dy2;
     const T fz = (atp * (xp - xc) - ab * (xc - xm)) / (dz * dz);
     const T ax = -fx - fy - fz + bc * xc;
-    y[lv + col] = mode ? f[lv + col] - ax : ax;
+    const T yv = mode ? f[lv + col] - ax : ax;
+    y[lv + col] = yv;
+    if (part != nullptr) {
+      s0 += static_cast<double>(d[lv + col]) * static_cast<double>(yv);
+      if (tt) s1 += static_cast<double>(yv) * static_cast<double>(yv);
+    }
     xm = xc;
     xc = xp;
     xp = xn;
@@ -117,6 +147,8 @@ __global__ void __launch_bounds__(64 * kApplyRows) k_apply(Op<T> o, const T *__r
 #pragma unroll
     for (int e = 0; e < 9; ++e) q[e] = qn[e];
   }
+  }
+  if (part != nullptr) block_partials(s0, s1, part, static_cast<int>(blockIdx.y * gridDim.x + blockIdx.x));
 }
 
 // Factorisation of the line preconditioner, one thread per column: rows
@@ -212,6 +244,7 @@ __global__ void __launch_bounds__(256) k_dot(size_t n, const T *__restrict__ x0,
 //   1  alpha = rho / <r^, v>
 //   2  omega = <t, s> / <t, t>
 //   3  rr = <r, r>
+//   4  rr = <r, r> and, with the second partials, op 0 on <r^, r>
 __global__ void __launch_bounds__(256) k_scalar(const double *__restrict__ part, int nblk, double *__restrict__ S,
                                                 int op) {
   __shared__ double sh[2][256];
@@ -244,8 +277,13 @@ __global__ void __launch_bounds__(256) k_scalar(const double *__restrict__ part,
     S[S_TS] = d0;
     S[S_TT] = d1;
     S[S_OMEGA] = d1 != 0.0 ? d0 / d1 : 0.0;
+  } else if (op == 3) {
+    S[S_RR] = d0;
   } else {
     S[S_RR] = d0;
+    if (d1 == 0.0 || S[S_OMEGA] == 0.0) S[S_FLAG] = 1.0;
+    S[S_BETA] = S[S_FLAG] != 0.0 ? 0.0 : (d1 / S[S_RHO]) * (S[S_ALPHA] / S[S_OMEGA]);
+    S[S_RHO] = d1;
   }
 }
 
@@ -265,16 +303,23 @@ __global__ void __launch_bounds__(256) k_upd_s(size_t n, const double *__restric
   for (size_t q = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; q < n; q += static_cast<size_t>(gridDim.x) * 256)
     s[q] = r[q] - alpha * v[q];
 }
-// x += alpha y + omega z;  r = s - omega t
+// x += alpha y + omega z;  r = s - omega t; the block's partial sums of
+// <r, r> and <r^, r> (the next iteration's rho) into part
 template <typename T>
 __global__ void __launch_bounds__(256) k_upd_xr(size_t n, const double *__restrict__ S, const T *__restrict__ y,
                                                 const T *__restrict__ z, const T *__restrict__ s,
-                                                const T *__restrict__ t, T *__restrict__ x, T *__restrict__ r) {
+                                                const T *__restrict__ t, T *__restrict__ x, T *__restrict__ r,
+                                                const T *__restrict__ rh, double *__restrict__ part) {
   const T alpha = static_cast<T>(S[S_ALPHA]), omega = static_cast<T>(S[S_OMEGA]);
+  double s0 = 0.0, s1 = 0.0;
   for (size_t q = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; q < n; q += static_cast<size_t>(gridDim.x) * 256) {
     x[q] = x[q] + alpha * y[q] + omega * z[q];
-    r[q] = s[q] - omega * t[q];
+    const T rn = s[q] - omega * t[q];
+    r[q] = rn;
+    s0 += static_cast<double>(rn) * static_cast<double>(rn);
+    s1 += static_cast<double>(rh[q]) * static_cast<double>(rn);
   }
+  block_partials(s0, s1, part, static_cast<int>(blockIdx.x));
 }
 // dst = scale * src (with conversion), or dst += scale * src (acc)
 template <typename D, typename S_>
@@ -392,10 +437,16 @@ Op<T> op_of(const vibe_mi_helmholtz *h, const Work<T> &w) {
                static_cast<T>(h->ztop), w.zs, w.dzeta, w.a, w.b};
 }
 
+// blocks of the operator's grid (the partial sums it forms with part)
+unsigned apply_blocks(const vibe_mi_helmholtz *h) { return ceil_div(h->nx, 64) * ceil_div(h->ny, kApplyRows); }
+
+// y = A x (mode 0) or f - A x (mode 1); with part, also the partial sums of
+// <d, y> (and <y, y> with tt) for k_scalar over apply_blocks(h) blocks
 template <typename T>
-int apply(vibe_mi_helmholtz *h, const Work<T> &w, const T *x, const T *f, T *y, int mode) {
+int apply(vibe_mi_helmholtz *h, const Work<T> &w, const T *x, const T *f, T *y, int mode, const T *d = nullptr,
+          int tt = 0, double *part = nullptr) {
   hipLaunchKernelGGL((k_apply<T>), dim3(ceil_div(h->nx, 64), ceil_div(h->ny, kApplyRows)), dim3(64, kApplyRows), 0,
-                     h->stream, op_of(h, w), x, f, y, mode);
+                     h->stream, op_of(h, w), x, f, y, mode, d, tt, part);
   VMI_HIP(hipGetLastError());
   return 0;
 }
@@ -463,21 +514,29 @@ int bicgstab(vibe_mi_helmholtz *h, Work<T> &w, double tol, int max_iter, int *it
   int it = 0;
   while (it < max_iter) {
     const int burst = std::min(kCheckEvery, max_iter - it);
+    const int nab = static_cast<int>(apply_blocks(h));
     for (int q = 0; q < burst; ++q) {
-      if (int rc = dot(h, w.S, w.rh, w.r, static_cast<const T *>(nullptr), static_cast<const T *>(nullptr), 0))
-        return rc;
+      // rho = <r^, r>: a pass of its own on the first iteration, afterwards
+      // formed by the previous iteration's x/r update (op 4)
+      if (it + q == 0)
+        if (int rc = dot(h, w.S, w.rh, w.r, static_cast<const T *>(nullptr), static_cast<const T *>(nullptr), 0))
+          return rc;
       hipLaunchKernelGGL((k_upd_p<T>), dim3(g), dim3(256), 0, h->stream, n, w.S, w.r, w.v, w.p);
       if (int rc = line(h, w, w.p, w.y)) return rc;
-      if (int rc = apply(h, w, w.y, static_cast<const T *>(nullptr), w.v, 0)) return rc;
-      if (int rc = dot(h, w.S, w.rh, w.v, static_cast<const T *>(nullptr), static_cast<const T *>(nullptr), 1))
+      // v = A y with <r^, v>
+      if (int rc = apply(h, w, w.y, static_cast<const T *>(nullptr), w.v, 0, static_cast<const T *>(w.rh), 0, h->part))
         return rc;
+      hipLaunchKernelGGL(k_scalar, dim3(1), dim3(256), 0, h->stream, h->part, nab, w.S, 1);
       hipLaunchKernelGGL((k_upd_s<T>), dim3(g), dim3(256), 0, h->stream, n, w.S, w.r, w.v, w.s);
       if (int rc = line(h, w, w.s, w.z)) return rc;
-      if (int rc = apply(h, w, w.z, static_cast<const T *>(nullptr), w.t, 0)) return rc;
-      if (int rc = dot(h, w.S, w.t, w.s, static_cast<const T *>(w.t), static_cast<const T *>(w.t), 2)) return rc;
-      hipLaunchKernelGGL((k_upd_xr<T>), dim3(g), dim3(256), 0, h->stream, n, w.S, w.y, w.z, w.s, w.t, w.x, w.r);
-      if (int rc = dot(h, w.S, w.r, w.r, static_cast<const T *>(nullptr), static_cast<const T *>(nullptr), 3))
+      // t = A z with <s, t> and <t, t>
+      if (int rc = apply(h, w, w.z, static_cast<const T *>(nullptr), w.t, 0, static_cast<const T *>(w.s), 1, h->part))
         return rc;
+      hipLaunchKernelGGL(k_scalar, dim3(1), dim3(256), 0, h->stream, h->part, nab, w.S, 2);
+      // x, r with <r, r> and the next rho
+      hipLaunchKernelGGL((k_upd_xr<T>), dim3(g), dim3(256), 0, h->stream, n, w.S, w.y, w.z, w.s, w.t, w.x, w.r,
+                         static_cast<const T *>(w.rh), h->part);
+      hipLaunchKernelGGL(k_scalar, dim3(1), dim3(256), 0, h->stream, h->part, static_cast<int>(g), w.S, 4);
       VMI_HIP(hipGetLastError());
     }
     it += burst;
@@ -626,8 +685,9 @@ int create_handle(const vibe_mi_helmholtz_desc *ds, bool fp32, vibe_mi_helmholtz
   if (fp32)
     if (int rc = alloc_work(h, h->w32)) return fail(rc);
   if (int rc = halloc(h, &h->f64, h->n)) return fail(rc);
-  if (int rc = halloc(h, &h->part, static_cast<size_t>(2 * kDotBlocks))) return fail(rc);
   h->vgrid = static_cast<unsigned>(std::min<size_t>(ceil_div(h->n, 256), 8192));
+  const size_t nparts = std::max<size_t>({static_cast<size_t>(kDotBlocks), apply_blocks(h), h->vgrid});
+  if (int rc = halloc(h, &h->part, 2 * nparts)) return fail(rc);
   if (ds->zeta != nullptr && ds->zs != nullptr) {
     const size_t nxy = static_cast<size_t>(h->nx) * h->ny;
     std::vector<double> dz(h->nz);
