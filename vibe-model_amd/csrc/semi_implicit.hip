@@ -196,7 +196,10 @@ struct vibe_mi_si {
   SiGrid g{};
   int solver = VIBE_MI_HELM_BICGSTAB, max_iter = 0, kmin = 0;
   double tol = 0.0, filter = 0.0;
-  bool first_step = true, physics = false;
+  // dt whose coefficients the Helmholtz handle holds (a depends on dt alone:
+  // pi0 and theta0 are fixed), so the solver is refactored only when dt changes
+  double coef_dt = -1.0;
+  bool first_step = true;
   std::vector<void *> allocs;
   double *acoef = nullptr, *bcoef = nullptr, *rhs = nullptr, *pinew = nullptr, *rate = nullptr;
   double *phys[4] = {};  // theta, qv, u, v rates (NULL: zero)
@@ -250,7 +253,11 @@ int one_step(vibe_mi_si *h, double dt) {
   if (int rc = tendencies(h)) return rc;
   if (int rc = assemble(h, dt)) return rc;
   VMI_HIP(hipStreamSynchronize(st));
-  if (int rc = helmholtz_set_coefficients_device(h->helm, h->acoef, h->bcoef)) return rc;
+  if (dt != h->coef_dt) {
+    h->coef_dt = -1.0;
+    if (int rc = helmholtz_set_coefficients_device(h->helm, h->acoef, h->bcoef)) return rc;
+    h->coef_dt = dt;
+  }
   int its = 0;
   double rel = 0.0;
   if (int rc = helmholtz_solve_device(h->helm, h->rhs, h->pinew, h->solver, h->tol, h->max_iter, &its, &rel))
