@@ -5,11 +5,16 @@ Agnesi ridge, N = 0.01 1/s, U = 10 m/s, dt = 6 s, 6 acoustic substeps per RK3
 stage).  --config ensemble runs the 48-member uneven ensemble (nx, ny in
 96..320, nz in 40/60/80, dx = 2 km, dt = 8 s) as one batch; --config
 nest_hierarchy the ratio-3 nest hierarchy (one step = one parent step).  The headline
-value is the FP32-mixed batched step (FP64 state and RK3 accumulation, FP32
-tendencies/substeps, FP64 compensated reductions; one persistent launch per
-kernel phase over the tile work list of all members, each step replayed from
-a hipGraph) in cell-updates/s.  Beside it: the plain FP64 batched step on the
-same inputs, and every member stepped alone through the single-domain path.
+value is the FP32-compensated batched step (VIBE_MI_MIXED_COMP: FP64 state and
+RK3 accumulation, FP32 tendencies/substeps with u carrying an 8-bit remainder
+and the x part of the horizontal mass-flux divergence in FP64, FP64
+compensated reductions; one persistent launch per kernel phase over the tile
+work list of all members, each step replayed from a hipGraph) in
+cell-updates/s: the FP32 mode that keeps every prognostic within 1e-4 of
+FP64 after 1 h.  --headline mixed reports the plain FP32-mixed mode
+(VIBE_MI_MIXED) there instead.  Beside it: the plain FP64 batched step on the
+same inputs, the other FP32 mode, and every member stepped alone through the
+single-domain path.
 
 One "step" = one WS-RK3 step of every member (3 stages, 3 x 6 substeps,
 sponge).  Inputs are resident in HBM before timing starts.  Timing: W untimed
@@ -19,15 +24,16 @@ second, eager (no graph) run with HIP events around each stage kernel and
 around each stage's substep launches.
 
 Roofline: algorithmic bytes per cell-update (compulsory prognostic traffic)
-  FP64   8 * (48 + 30 * n_s) = 1824 B     (n_s = 6)
-  mixed  8 * 48 + 4 * 30 * n_s = 1104 B
+  FP64          8 * (48 + 30 * n_s) = 1824 B     (n_s = 6)
+  mixed         8 * 48 + 4 * 30 * n_s = 1104 B
+  compensated   mixed + u's 1-byte remainder on 6 n_s substep accesses = 1140 B
 against 8.0 TB/s.  The dominant kernel is the fused acoustic substep
 kb_substep (3 n_s launches per step, about 70 % of the step).  Its algorithmic
 bytes per launch: read u, v, w, rho', pi' and write them once, 4 B each in
-the FP32-mixed path (40 B per cell), except the step's last substep, which
-writes the FP64 state (20 + 40 = 60 B per cell; the intermediate stage
-states stay FP32); averaged over the 3 n_s launches of a step:
-(40 (3 n_s - 1) + 60) / (3 n_s) B per cell.  This implementation moves less
+the FP32 paths (40 B per cell; compensated 42 B with u's remainder), except
+the step's last substep, which writes the FP64 state (20 + 40 = 60 B per
+cell, compensated 62 B; the intermediate stage states stay FP32); averaged
+over the 3 n_s launches of a step.  This implementation moves less
 than that model: its FP32 substeps carry chi = rho' - pi'/kappa, which no
 substep but a stage's last reads, in place of rho' (32 B per cell on the
 first n_s - 1 substeps of a stage); "bytes_moved_per_cell_update" and the
@@ -79,31 +85,37 @@ def bytes_moved_per_cell_update(nsub: int, compensated: bool = False) -> int:
     return 8 * 48 + 3 * per_stage
 
 
-def substep_moved_bytes_per_launch(cells: int, nsub: int) -> float:
-    """Bytes this implementation's kb_substep moves per launch (FP32-mixed
-    batch), averaged over the 3 nsub launches of a step: 32 B per cell for
-    the first nsub-1 substeps of a stage (u, v, w, pi' read and written),
-    40 B for the last of stages 1 and 2 (chi read, rho' written) and 60 B
-    for the step's last (the FP64 state written)."""
+def substep_moved_bytes_per_launch(cells: int, nsub: int, compensated: bool = False) -> float:
+    """Bytes this implementation's kb_substep moves per launch (FP32 batch),
+    averaged over the 3 nsub launches of a step: 32 B per cell for the first
+    nsub-1 substeps of a stage (u, v, w, pi' read and written), 40 B for the
+    last of stages 1 and 2 (chi read, rho' written) and 60 B for the step's
+    last (the FP64 state written); the compensated mode adds u's 1-byte
+    remainder read and written (2 B) to every launch."""
     n = 3 * nsub
-    return cells * (32.0 * 3 * (nsub - 1) + 40.0 * 2 + 60.0) / n
+    extra = 2.0 if compensated else 0.0
+    return cells * ((32.0 + extra) * 3 * (nsub - 1) + (40.0 + extra) * 2 + 60.0 + extra) / n
 
 
-def substep_bytes_per_launch(cells: int, nsub: int) -> float:
-    """Algorithmic bytes of one kb_substep launch (FP32-mixed batch),
-    averaged over the 3 nsub launches of a step."""
+def substep_bytes_per_launch(cells: int, nsub: int, compensated: bool = False) -> float:
+    """Algorithmic bytes of one kb_substep launch (FP32 batch), averaged over
+    the 3 nsub launches of a step: 40 B per cell, 60 B for the step's last
+    (FP64 state written); compensated plus 2 B (u's remainder)."""
     n = 3 * nsub
-    return cells * (40.0 * (n - 1) + 60.0) / n
+    extra = 2.0 if compensated else 0.0
+    return cells * ((40.0 + extra) * (n - 1) + 60.0 + extra) / n
 
 
-def measured_traffic(cases):
+def measured_traffic(cases, compensated: bool = False):
     """HBM bytes per kb_substep launch from the committed rocprofv3 summary of
-    this workload (FETCH_SIZE + WRITE_SIZE per dispatch, copy-calibrated)."""
+    this workload and FP32 mode (FETCH_SIZE + WRITE_SIZE per dispatch,
+    copy-calibrated)."""
+    mode = "compensated_" if compensated else ""
     if len(cases) == 1:
         c = cases[0]
-        name = f"kb_substep_{c.name}_{c.nx}x{c.ny}x{c.nz}_hbm_traffic.txt"
+        name = f"kb_substep_{mode}{c.name}_{c.nx}x{c.ny}x{c.nz}_hbm_traffic.txt"
     else:
-        name = f"kb_substep_ensemble{len(cases)}_hbm_traffic.txt"
+        name = f"kb_substep_{mode}ensemble{len(cases)}_hbm_traffic.txt"
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None, None
@@ -269,8 +281,12 @@ def bench_nests(args, t0):
     cells = sum(c.ncell for c in cases)
     log(f"[bench] built nest hierarchy: {len(cases)} domains, {cells} cells in {time.perf_counter() - t0:.1f}s")
     res = {}
-    for name, prec in (("mixed", vibe_mi.MIXED), ("fp64", vibe_mi.FP64)):
-        if name == "fp64" and args.no_fp64:
+    head = "compensated" if args.headline == "comp" else "mixed"
+    other = "mixed" if head == "compensated" else "compensated"
+    precs = {"compensated": vibe_mi.MIXED_COMP, "mixed": vibe_mi.MIXED, "fp64": vibe_mi.FP64}
+    for name in (head, "fp64", other):
+        prec = precs[name]
+        if (name == "fp64" and args.no_fp64) or (name == other and args.no_other):
             continue
         run = NestedRun(cases, links, precision=prec)
         cu = run.cell_updates_per_parent_step()
@@ -286,20 +302,26 @@ def bench_nests(args, t0):
         log(f"[bench] {name}: {wall * 1e3 / args.steps:.2f} ms per parent step, "
             f"{cu * args.steps / wall:.3e} cell-updates/s")
         run.close()
-    m = res["mixed"]
+    m = res[head]
     out = {
         "metric": "cell-updates/s per RK3 step + % HBM roofline, FP32-mixed/batched vs FP64",
         "value": m["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": m["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 compute, f64 state", "data": "synthetic (seeded random terrain, random-phase theta')",
+        "dtype": "f32 compute, f64 state" if head == "mixed" else
+                 "f32 compute (u with an 8-bit remainder, x mass-flux divergence f64), f64 state",
+        "data": "synthetic (seeded random terrain, random-phase theta')",
         "config": {"workload": "nest hierarchy: 384x384x60 parent at 3 km, nests 271x271, 211x331, 151x151 at "
                                "1 km, 181x181 at 333 m; dt 18/6/2 s, 6 substeps/stage; one step = one parent step",
                    "cells": cells, "parallelism": "single GPU"},
-        "hbm_frac_step": m["value"] * bytes_per_cell_update(6, True) / HBM_PEAK,
+        "hbm_frac_step": m["value"] * bytes_per_cell_update(6, True, head == "compensated") / HBM_PEAK,
         "fp64": res.get("fp64"),
-        "speedup_mixed_vs_fp64": (m["value"] / res["fp64"]["value"]) if "fp64" in res else None,
+        "speedup_vs_fp64": (m["value"] / res["fp64"]["value"]) if "fp64" in res else None,
+        other: res.get(other),
         "roofline": None, "cpu_baseline": None,
     }
+    out["config"]["mode"] = "FP32-compensated" if head == "compensated" else "FP32-mixed"
+    if other in res and "fp64" in res:
+        res[other]["speedup_vs_fp64"] = res[other]["value"] / res["fp64"]["value"]
     out["config"]["cell_updates_per_step"] = sum(c.ncell * 3 ** c.meta["level"] for c in cases)
     print(json.dumps(out))
 
@@ -313,6 +335,10 @@ def main():
                     help="mountain_wave_large (default), mountain_wave, warm_bubble, ensemble, nest_hierarchy")
     ap.add_argument("--nx", type=int, default=0)
     ap.add_argument("--members", type=int, default=48, help="ensemble size")
+    ap.add_argument("--headline", choices=("comp", "mixed"), default="comp",
+                    help="FP32 mode of the headline value: comp (VIBE_MI_MIXED_COMP, the mode within 1e-4 of "
+                         "FP64 after 1 h; default) or mixed (VIBE_MI_MIXED)")
+    ap.add_argument("--no-other", action="store_true", help="skip the other FP32 mode")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-alone", action="store_true", help="skip the unbatched comparison")
@@ -346,12 +372,23 @@ def main():
     cells = sum(c.ncell for c in cases)
     log(f"[bench] built {wl}: {cells} cells in {time.perf_counter() - t0:.1f}s")
 
-    log("[bench] FP32-mixed batched (hipGraph) timed run")
-    rm = run_batch(cases, vibe_mi.MIXED, args.steps, args.warmup, graph=True)
+    head_comp = args.headline == "comp"
+    modes = {True: (vibe_mi.MIXED_COMP, "FP32-compensated"), False: (vibe_mi.MIXED, "FP32-mixed")}
+    head_prec, head_name = modes[head_comp]
+    other_prec, other_name = modes[not head_comp]
+
+    def mode_summary(cups, ms, tuning, comp_mode):
+        return {"value": cups, "ms_per_step": ms, "tiles": tuning,
+                "bytes_per_cell_update": bytes_per_cell_update(nsub, True, comp_mode),
+                "bytes_moved_per_cell_update": bytes_moved_per_cell_update(nsub, comp_mode),
+                "hbm_frac_step": cups * bytes_per_cell_update(nsub, True, comp_mode) / HBM_PEAK}
+
+    log(f"[bench] {head_name} batched (hipGraph) timed run")
+    rm = run_batch(cases, head_prec, args.steps, args.warmup, graph=True)
     cups_m = cells * args.steps / (rm["ms"] * 1e-3)
-    log(f"[bench] mixed: {rm['ms'] / args.steps:.2f} ms/step, {cups_m:.3e} cell-updates/s, tiles {rm['tuning']}")
-    log("[bench] FP32-mixed batched eager run with per-group HIP events")
-    re_ = run_batch(cases, vibe_mi.MIXED, args.steps, 1, graph=False)
+    log(f"[bench] {head_name}: {rm['ms'] / args.steps:.2f} ms/step, {cups_m:.3e} cell-updates/s, tiles {rm['tuning']}")
+    log(f"[bench] {head_name} batched eager run with per-group HIP events")
+    re_ = run_batch(cases, head_prec, args.steps, 1, graph=False)
     kt_m = re_["kt"]
     res64 = None
     if not args.no_fp64:
@@ -361,36 +398,34 @@ def main():
         log(f"[bench] fp64: {rd['ms'] / args.steps:.2f} ms/step, {cups_d:.3e} cell-updates/s")
         res64 = {"value": cups_d, "ms_per_step": rd["ms"] / args.steps,
                  "hbm_frac": cups_d * bytes_per_cell_update(nsub, False) / HBM_PEAK, "tiles": rd["tuning"]}
-    comp = None
-    if not args.no_fp64:
-        log("[bench] FP32-compensated batched (hipGraph) timed run")
-        rc_ = run_batch(cases, vibe_mi.MIXED_COMP, args.steps, args.warmup, graph=True)
-        cups_c = cells * args.steps / (rc_["ms"] * 1e-3)
-        comp = {"value": cups_c, "ms_per_step": rc_["ms"] / args.steps, "tiles": rc_["tuning"],
-                "speedup_vs_fp64": cups_c / res64["value"],
-                "bytes_per_cell_update": bytes_per_cell_update(nsub, True, True),
-                "bytes_moved_per_cell_update": bytes_moved_per_cell_update(nsub, True),
-                "hbm_frac_step": cups_c * bytes_per_cell_update(nsub, True, True) / HBM_PEAK}
-        log(f"[bench] compensated: {rc_['ms'] / args.steps:.2f} ms/step, {cups_c:.3e} cell-updates/s, "
-            f"{cups_c / res64['value']:.3f}x FP64")
+        log(f"[bench] {head_name}: {cups_m / cups_d:.3f}x FP64")
+    other = None
+    if not args.no_other:
+        log(f"[bench] {other_name} batched (hipGraph) timed run")
+        ro_ = run_batch(cases, other_prec, args.steps, args.warmup, graph=True)
+        cups_o = cells * args.steps / (ro_["ms"] * 1e-3)
+        other = mode_summary(cups_o, ro_["ms"] / args.steps, ro_["tuning"], not head_comp)
+        other["speedup_vs_fp64"] = (cups_o / res64["value"]) if res64 else None
+        log(f"[bench] {other_name}: {ro_['ms'] / args.steps:.2f} ms/step, {cups_o:.3e} cell-updates/s"
+            + (f", {cups_o / res64['value']:.3f}x FP64" if res64 else ""))
     alone = None
     if not args.no_alone:
-        log("[bench] each member alone (single-domain path), FP32-mixed")
-        ms_a = run_alone(cases, vibe_mi.MIXED, args.steps, args.warmup)
+        log(f"[bench] each member alone (single-domain path), {head_name}")
+        ms_a = run_alone(cases, head_prec, args.steps, args.warmup)
         alone = {"value": cells * args.steps / (ms_a * 1e-3), "ms_per_step": ms_a / args.steps,
                  "batched_speedup": ms_a / rm["ms"]}
         log(f"[bench] alone: {ms_a / args.steps:.2f} ms/step; batched speedup {ms_a / rm['ms']:.3f}x")
 
-    # dominant kernel: the fused acoustic substep (kb_substep), FP32-mixed
+    # dominant kernel: the fused acoustic substep (kb_substep) of the headline mode
     sub_ms = kt_m["ms_substep"] / max(kt_m["n_substep"], 1)
-    sub_bytes = sum(substep_bytes_per_launch(c.ncell, nsub) for c in cases)
+    sub_bytes = sum(substep_bytes_per_launch(c.ncell, nsub, head_comp) for c in cases)
     achieved = sub_bytes / (sub_ms * 1e-3) / 1e9
-    traffic, traffic_src = measured_traffic(cases)
+    traffic, traffic_src = measured_traffic(cases, head_comp)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved * 1e9 / HBM_PEAK, "traffic": traffic,
-                "kernel": "kb_substep (fused HEVI acoustic substep, FP32, batched persistent)",
+                "kernel": f"kb_substep (fused HEVI acoustic substep, {head_name}, batched persistent)",
                 "algorithmic_bytes_per_launch": sub_bytes, "avg_launch_ms": sub_ms,
-                "moved_bytes_per_launch": substep_moved_bytes_per_launch(cells, nsub),
+                "moved_bytes_per_launch": substep_moved_bytes_per_launch(cells, nsub, head_comp),
                 "traffic_source": traffic_src}
 
     cpu = None
@@ -418,20 +453,24 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 compute, f64 state",
+        "dtype": ("f32 compute (u with an 8-bit remainder, x mass-flux divergence f64), f64 state" if head_comp
+                  else "f32 compute, f64 state"),
         "data": "synthetic (seeded terrain/noise/bubble initial states)",
-        "config": {"workload": wl, "cells": cells, "members": len(cases), "parallelism": "single GPU"},
-        "hbm_frac_step": cups_m * bytes_per_cell_update(nsub, True) / HBM_PEAK,
-        "bytes_per_cell_update": bytes_per_cell_update(nsub, True),
-        "bytes_moved_per_cell_update": bytes_moved_per_cell_update(nsub),
+        "config": {"workload": wl, "cells": cells, "members": len(cases), "parallelism": "single GPU",
+                   "mode": head_name},
+        "hbm_frac_step": cups_m * bytes_per_cell_update(nsub, True, head_comp) / HBM_PEAK,
+        "bytes_per_cell_update": bytes_per_cell_update(nsub, True, head_comp),
+        "bytes_moved_per_cell_update": bytes_moved_per_cell_update(nsub, head_comp),
         "tiles": rm["tuning"],
         "fp64": res64,
-        "speedup_mixed_vs_fp64": (cups_m / res64["value"]) if res64 else None,
-        "unbatched_mixed": alone,
-        # VIBE_MI_MIXED_COMP (batched, hipGraph): FP32 compute with u carrying a
-        # remainder and the horizontal mass-flux divergence in FP64, the mode
-        # that holds 1e-4 of FP64 after 1 h
-        "compensated": comp,
+        "speedup_vs_fp64": (cups_m / res64["value"]) if res64 else None,
+        "unbatched": alone,
+        # the headline mode: VIBE_MI_MIXED_COMP (FP32 compute with u carrying a
+        # remainder and the x part of the horizontal mass-flux divergence in
+        # FP64), within 1e-4 of FP64 after 1 h; the plain FP32-mixed mode
+        # (VIBE_MI_MIXED, 1.24e-4 after 1 h) beside it, or the other way round
+        # with --headline mixed
+        "mixed" if head_comp else "compensated": other,
         "kernel_ms": {"stage_group_per_stage": kt_m["ms_stage"] / max(kt_m["n_stage"], 1),
                       "substep_per_launch": sub_ms},
         "roofline": roofline,

@@ -28,7 +28,7 @@ def test_davies_profile_matches_oracle(vmi):
         assert np.array_equal(vmi.davies_profile(zone, amax), o)
 
 
-@pytest.mark.parametrize("prec_name", ["FP64", "MIXED"])
+@pytest.mark.parametrize("prec_name", ["FP64", "MIXED", "MIXED_COMP"])
 def test_nest_interpolate_and_relax_match_oracle(vmi, prec_name):
     prec = getattr(vmi, prec_name)
     pc, cc = _domains(vmi)

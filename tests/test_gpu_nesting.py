@@ -87,7 +87,7 @@ def test_batch_nest_relax_rejects_bad_members(vmi):
     b.close()
 
 
-@pytest.mark.parametrize("prec_name", ["FP64", "MIXED"])
+@pytest.mark.parametrize("prec_name", ["FP64", "MIXED", "MIXED_COMP"])
 @pytest.mark.parametrize("two_way", [False, True])
 def test_graph_parent_step_equals_substep_schedule(vmi, prec_name, two_way):
     from vibe_mi.nesting import NestedRun

@@ -471,28 +471,42 @@ int autotune(vibe_mi_batch *b, WorkList &w) {
     *ms = total;
     return 0;
   };
+  // two rounds over the candidates, the faster of each candidate's two
+  // timings kept: the first launches of a process run while clocks and
+  // caches settle, which would otherwise penalise whichever shape is tried
+  // first
   if (!pin_nw) {
-    float best = 1e30f;
-    for (int nw : {4, 8, 16}) {
-      float ms = 0.f;
-      const int rc = time_it([&]() { return launch_substep<T, T>(b, nw, w, 0, 0, 0, 0, b->ctr); }, &ms);
-      if (rc == 0 && ms < best) {
-        best = ms;
-        w.nw = nw;
+    const int cand[3] = {4, 8, 16};
+    float best_of[3] = {1e30f, 1e30f, 1e30f};
+    for (int round = 0; round < 2; ++round)
+      for (int c = 0; c < 3; ++c) {
+        float ms = 0.f;
+        const int rc = time_it([&]() { return launch_substep<T, T>(b, cand[c], w, 0, 0, 0, 0, b->ctr); }, &ms);
+        if (rc == 0 && ms < best_of[c]) best_of[c] = ms;
       }
-    }
+    float best = 1e30f;
+    for (int c = 0; c < 3; ++c)
+      if (best_of[c] < best) {
+        best = best_of[c];
+        w.nw = cand[c];
+      }
     (void)hipGetLastError();
   }
   if (!pin_ty) {
-    float best = 1e30f;
-    for (int ty : {2, 4, 8}) {
-      float ms = 0.f;
-      const int rc = time_it([&]() { return launch_stage<T>(b, ty, w, 0, -1, 0, b->ctr); }, &ms);
-      if (rc == 0 && ms < best) {
-        best = ms;
-        w.ty = ty;
+    const int cand[3] = {2, 4, 8};
+    float best_of[3] = {1e30f, 1e30f, 1e30f};
+    for (int round = 0; round < 2; ++round)
+      for (int c = 0; c < 3; ++c) {
+        float ms = 0.f;
+        const int rc = time_it([&]() { return launch_stage<T>(b, cand[c], w, 0, -1, 0, b->ctr); }, &ms);
+        if (rc == 0 && ms < best_of[c]) best_of[c] = ms;
       }
-    }
+    float best = 1e30f;
+    for (int c = 0; c < 3; ++c)
+      if (best_of[c] < best) {
+        best = best_of[c];
+        w.ty = cand[c];
+      }
     (void)hipGetLastError();
   }
   w.tuned = true;
