@@ -349,7 +349,7 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+        print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU runs are not implemented",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 

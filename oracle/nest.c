@@ -11,7 +11,7 @@
  *   restrict_to_parent        nest.cpp:85-108 (r x r mean of the child cells
  *                             over each covered parent cell, summed jj-outer
  *                             and ii-inner, times 1/r^2)
- * with two choices of this build: parent indices wrap (periodic parent)
+ * with two choices of the library: parent indices wrap (periodic parent)
  * instead of clamping, and u / v are sampled at their own staggered
  * positions (u: xp = (i0 + i)/r, v: yp = (j0 + j)/r).  The parent value is
  * interpolated in time between its previous and current state,

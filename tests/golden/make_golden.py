@@ -22,7 +22,7 @@ def main():
     assert L is not None, "build oracle/_ref first: make -C oracle -f ref.mk"
     rng = np.random.default_rng(20241203)
     out = {}
-    # WENO5: the survey's probe stencil plus seeded stencils of mixed scale
+    # WENO5: a fixed probe stencil plus seeded stencils of mixed scale
     cases = [[0.2, -0.4, 1.1, 0.9, 2.3], [3.5] * 5]
     for _ in range(60):
         cases.append(list(rng.normal(size=5) * float(rng.choice([1e-4, 1.0, 50.0]))))

@@ -37,7 +37,7 @@ def thomas(a, b, c, d):
 
 def test_weno5_golden():
     data = json.load(open(os.path.join(GOLD, "weno5.json")))
-    assert data["cases"][0]["q_left"] == 1.1306425090328729  # survey probe fixture
+    assert data["cases"][0]["q_left"] == 1.1306425090328729  # the fixed probe stencil
     for cs in data["cases"]:
         assert weno(cs["q"]) == (cs["q_left"], cs["q_right"])
 
