@@ -471,14 +471,14 @@ int autotune(vibe_mi_batch *b, WorkList &w) {
     *ms = total;
     return 0;
   };
-  // two rounds over the candidates, the faster of each candidate's two
+  // two passes over the candidates, the faster of each candidate's two
   // timings kept: the first launches of a process run while clocks and
   // caches settle, which would otherwise penalise whichever shape is tried
   // first
   if (!pin_nw) {
     const int cand[3] = {4, 8, 16};
     float best_of[3] = {1e30f, 1e30f, 1e30f};
-    for (int round = 0; round < 2; ++round)
+    for (int pass = 0; pass < 2; ++pass)
       for (int c = 0; c < 3; ++c) {
         float ms = 0.f;
         const int rc = time_it([&]() { return launch_substep<T, T>(b, cand[c], w, 0, 0, 0, 0, b->ctr); }, &ms);
@@ -495,7 +495,7 @@ int autotune(vibe_mi_batch *b, WorkList &w) {
   if (!pin_ty) {
     const int cand[3] = {2, 4, 8};
     float best_of[3] = {1e30f, 1e30f, 1e30f};
-    for (int round = 0; round < 2; ++round)
+    for (int pass = 0; pass < 2; ++pass)
       for (int c = 0; c < 3; ++c) {
         float ms = 0.f;
         const int rc = time_it([&]() { return launch_stage<T>(b, cand[c], w, 0, -1, 0, b->ctr); }, &ms);
