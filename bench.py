@@ -138,20 +138,33 @@ def copy_calibration(nbytes: int = 1 << 30, reps: int = 3):
     counters (run under rocprofv3 --pmc): the runtime's device-to-device copy
     (dispatched as a blit kernel) and torch's elementwise kernel (b = a * 1,
     ordinary vector loads and stores like the dycore kernels); the
-    elementwise kernel sets the scale factors."""
+    elementwise kernel sets the scale factors.  Also prints the bandwidth
+    each copy reaches (bytes read + written over the HIP-event time of the
+    last reps - 1 copies), the practical ceiling a streaming kernel meets."""
     import torch
 
     a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").uniform_()
     b = torch.empty_like(a)
-    for _ in range(reps):
-        b.copy_(a)
-    torch.cuda.synchronize()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record()
+        for _ in range(reps - 1):
+            fn()
+        t1.record()
+        torch.cuda.synchronize()
+        ms = t0.elapsed_time(t1) / max(reps - 1, 1)
+        return 2.0 * nbytes / (ms * 1e-3) / 1e9
+
+    gbs_copy = timed(lambda: b.copy_(a))
     ok = bool(torch.equal(a, b))
-    for _ in range(reps):
-        torch.mul(a, 1.0, out=b)
-    torch.cuda.synchronize()
+    b.zero_()
+    gbs_mul = timed(lambda: torch.mul(a, 1.0, out=b))
     ok = ok and bool(torch.equal(a, b))
-    print(json.dumps({"copy_bytes": nbytes, "reps": reps, "ok": ok}))
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps, "ok": ok, "copy_GBps": round(gbs_copy, 1),
+                      "elementwise_GBps": round(gbs_mul, 1)}))
 
 
 def run_batch(cases, precision, steps, warmup, graph=True):
