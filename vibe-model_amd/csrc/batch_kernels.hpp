@@ -225,13 +225,15 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(kS
   X template __global__ void vibe_mi::kb_stage<float, 2, GM, true, PH>(VMI_KB_STAGE_ARGS);                     \
   X template __global__ void vibe_mi::kb_stage<float, 4, GM, true, PH>(VMI_KB_STAGE_ARGS);                     \
   X template __global__ void vibe_mi::kb_stage<float, 8, GM, true, PH>(VMI_KB_STAGE_ARGS);
-#define VMI_BATCH_F32_GM(X, GM)                                                                                 \
+// every substep shape of one geometry compaction (plain and compensated)
+#define VMI_BATCH_F32_SUBSTEPS(X, GM)                                                                           \
   VMI_BATCH_F32_SUBSTEP_F(X, GM, false)                                                                         \
   VMI_BATCH_F32_SUBSTEP_D(X, GM, false)                                                                         \
   VMI_BATCH_F32_SUBSTEP_F(X, GM, true)                                                                          \
-  VMI_BATCH_F32_SUBSTEP_D(X, GM, true)                                                                          \
-  VMI_BATCH_F32_STAGE_GM(X, GM, false)                                                                          \
-  VMI_BATCH_F32_STAGE_GM(X, GM, true)
+  VMI_BATCH_F32_SUBSTEP_D(X, GM, true)
+// every stage shape of one geometry compaction (without and with physics)
+#define VMI_BATCH_F32_STAGES(X, GM) VMI_BATCH_F32_STAGE_GM(X, GM, false) VMI_BATCH_F32_STAGE_GM(X, GM, true)
+#define VMI_BATCH_F32_GM(X, GM) VMI_BATCH_F32_SUBSTEPS(X, GM) VMI_BATCH_F32_STAGES(X, GM)
 #define VMI_KB_SUBSTEP_ARGS const vibe_mi::BDom<float> *__restrict__, vibe_mi::TileList, int, int, int, int, int *__restrict__
 #define VMI_KB_STAGE_ARGS const vibe_mi::BDom<float> *__restrict__, vibe_mi::TileList, int, int, int, int *__restrict__
 #define VMI_BATCH_F32_ALL(X) \
